@@ -1,0 +1,169 @@
+"""Benchmark: the wavelet-adaptive FK forward solve + likelihood on one MI355X.
+
+Workload (default): the 256^3-equivalent wavelet-adaptive single solve --
+synthetic anatomy (seed 0) on a 17 cm cube, Dw = 0.0013 cm^2/day,
+rho = 0.025/day, dt = 0.9 * CFL of the finest level (0.5088 days), blocks of
+16^3 on levels <= 4, Interp4 refine/compress with tolerances 1e-3 / 1e-4
+every 10 steps, FP32 field, then the final refinement and the flip-rate
+likelihood (FP64) against synthetic observations of a ground-truth run at
+perturbed parameters (Dw x1.25, rho x0.8).
+
+A bench "step" is one FK time step of that solve (with its share of the
+adaptation work); --steps K times K steps from t = 0 (590 = T of 300 days).
+The W warmup steps run on a separate copy of the grid.  `value` is leaf
+cell-updates per second of wall time over the timed region (inputs resident
+in HBM).  The roofline object is for the step kernel (k_fk_step): 16
+algorithmic bytes per cell update (read u, Pwm, Pgm; write u) over its
+event-timed device duration.  The CPU baseline is the FP64 CPU oracle
+(block-wise labs, OpenMP) on the first, still uniform, steps of the same
+solve.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+L_CM = 17.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, n_steps=2):
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    n = u0.shape[0]
+    a = [np.ascontiguousarray(v, np.float64) for v in (u0, pwm, pgm)]
+    t0 = time.perf_counter()
+    O.fk_uniform_steps(a[0], a[1], a[2], 16, 1.0 / n, Dw, Dg, rho, dt, n_steps, threads=threads)
+    wall = time.perf_counter() - t0
+    return {"value": n ** 3 * n_steps / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": "FP64 CPU oracle, %d uniform steps of the %d^3 initial grid of this solve (%.1f s)"
+                      % (n_steps, n, wall)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=590)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256, help="finest points per dimension")
+    ap.add_argument("--adapt-every", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (samples would shard across GPUs; "
+              "run --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import gliomauq_amd as gq
+    from gliomauq_amd import anatomy as A
+
+    n = args.n
+    bpd = n // 16
+    max_level = int(round(np.log2(bpd)))
+    log("[bench] anatomy %d^3 seed %d" % (n, args.seed))
+    pwm, pgm, u0, cvox = A.config_inputs(n, L_CM, args.seed)
+    Dw_cm, rho = 0.0013, 0.025
+    tol_r, tol_c = 1e-3, 1e-4
+
+    ctx = gq.Context(0)
+
+    def fresh_grid():
+        g = gq.Grid(ctx, bpd, max_level, 4)
+        g.sample_volume("phi", u0)
+        g.sample_volume("pw", pwm)
+        g.sample_volume("pg", pgm)
+        return g
+
+    def solver(g):
+        return gq.Glioma_HG_UQ(g, L_CM, cfl=0.9, adaptive=True, tol_refine=tol_r, tol_compress=tol_c,
+                               adapt_every=args.adapt_every)
+
+    s0 = solver(fresh_grid())
+    Dw, Dg, _, dt = s0.params(Dw_cm, rho)
+
+    # observations from a ground-truth solve at perturbed parameters
+    log("[bench] ground truth solve")
+    gt = solver(fresh_grid())
+    gt.run(Dw_cm * 1.25, rho * 0.8, args.steps * dt * (0.999999))
+    u_true = gt.grid.dump_uniform()
+    t1, t2, pet = A.observations(u_true, seed=args.seed)
+    del gt
+
+    # warmup on a separate grid
+    log("[bench] warmup %d steps" % args.warmup)
+    gw = fresh_grid()
+    gq.fk_steps(gw, Dw, Dg, rho, dt, max(args.warmup, 1))
+    del gw
+
+    g = fresh_grid()
+    lik = gq.FlipLikelihood(g, t1, t2, pet)
+    s = solver(g)
+    gq.set_step_timing(g, True)
+    stats = {"kernel_ms": 0.0, "cells": 0, "launches": 0, "last": 0}
+
+    def after_chunk(done):
+        ms, k, cells = gq.step_timing(g)
+        stats["kernel_ms"] += ms
+        stats["cells"] += cells
+        stats["launches"] += k
+        log("[bench] step %d/%d blocks %d" % (done, args.steps, g.n_blocks()))
+
+    log("[bench] timed: %d steps" % args.steps)
+    t0 = time.perf_counter()
+    # the same schedule as Glioma_HG_UQ.run, stopped after args.steps steps
+    done = 0
+    while done < args.steps:
+        m = min(args.adapt_every, args.steps - done)
+        gq.fk_steps(g, Dw, Dg, rho, dt, m)
+        done += m
+        after_chunk(done)
+        if m == args.adapt_every:
+            s._adapt()
+    gq.Science.AutomaticRefinement(g, tol_r, max_level, 1)
+    ll = lik.loglik()
+    wall = time.perf_counter() - t0
+
+    cells = stats["cells"]
+    value = cells / wall
+    kernel_s = stats["kernel_ms"] * 1e-3
+    achieved = 16.0 * cells / kernel_s / 1e9
+    res = {
+        "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "wavelet-adaptive single solve, 256^3-equivalent (levels <= 4, 16^3 blocks), "
+                               "Interp4 tol 1e-3/1e-4 every 10 steps, FP32, %d steps of dt=%.4f d" % (args.steps, dt),
+                   "n": n, "seed": args.seed, "final_blocks": g.n_blocks(), "loglik": float(ll[0])},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_fk_step", "kernel_avg_us": stats["kernel_ms"] * 1e3 / max(stats["launches"], 1),
+                     "algorithmic_bytes_per_cell": 16},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline")
+        res["cpu_baseline"] = cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,14 @@
+"""gliomauq_amd: MI355X-native Fisher-Kolmogorov forward solve + likelihood.
+
+The compute path is libgq.so (HIP kernels for gfx950 behind the C-ABI of
+include/gq_api.h); this package is its host-side mirror of the reference's
+block-operator API (see solver.py) plus the synthetic-input generator.
+"""
+from . import _lib  # noqa: F401
+from .solver import (BlockFWT, BlockProcessing, Context, FlipLikelihood, Glioma_HG_UQ,  # noqa: F401
+                     Glioma_ReactionDiffusionOperator, Grid, HGG_Likelihood, Science, Topology, UpdateTumor,
+                     fk_steps, n_steps_until, set_step_timing, step_timing)
+
+__all__ = ["BlockFWT", "BlockProcessing", "Context", "FlipLikelihood", "Glioma_HG_UQ",
+           "Glioma_ReactionDiffusionOperator", "Grid", "HGG_Likelihood", "Science", "Topology", "UpdateTumor",
+           "fk_steps", "n_steps_until", "set_step_timing", "step_timing"]

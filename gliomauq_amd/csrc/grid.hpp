@@ -1,0 +1,128 @@
+// grid.hpp -- host-side block hierarchy and ghost tables.
+//
+// A leaf-block octree with MRAG's semantics (16^3 blocks, unit cube,
+// periodic, vertex-centred interpolating wavelets):
+//   * topology, neighbourhood with periodic images  (MRAGcore/MRAGrid.inl:841-1149)
+//   * refine planning: Refiner_SpaceExtension + Refiner::createPlan chain
+//     reaction (MRAGscience/MRAGRefiner_SpaceExtension.h:25-67,
+//     MRAGcore/MRAGRefiner.h:43-198)
+//   * compress planning: Compressor::createPlan (MRAGcore/MRAGCompressor.h:68-146)
+//   * ghost tables: the value MRAG_BBInfoCreator assigns to every ghost point
+//     (MRAGcore/MRAG_BBInfoCreator.inl:20-1315), flattened to one row of
+//     (source cell, weight) terms per ghost point.  Terms reaching the same
+//     leaf point are merged and rows are sorted canonically (leaf level, then
+//     global z, y, x at that level), so an FP64 evaluation of a row is
+//     independent of block numbering.
+//
+// The device never sees this structure: it receives flat per-slot tables.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+namespace gq {
+
+constexpr int BS = 16;          // cells per block edge
+constexpr int BS3 = BS * BS * BS;
+
+struct Leaf {
+    int level;
+    int idx[3];
+    int block_id;
+    int slot;  // device pool slot
+};
+
+struct Term {
+    int slot;
+    int cell;
+    double w;
+    int64_t key[4];  // level, gz, gy, gx
+};
+
+// Flattened ghost rows.
+struct Rows {
+    std::vector<int> ptr;    // nrows + 1
+    std::vector<int> src;    // slot*BS3 + cell
+    std::vector<double> w;
+    void clear() { ptr.assign(1, 0); src.clear(); w.clear(); }
+    int nrows() const { return (int)ptr.size() - 1; }
+};
+
+class Grid {
+public:
+    Grid(int blocks_per_dim, int max_level, int order);
+
+    int order() const { return order_; }
+    int max_level() const { return max_level_; }
+    int base_level() const { return base_level_; }
+    int blocks_per_dim() const { return bpd_; }
+    // finest level used for location look-ups (>= every leaf level)
+    int lut_level() const { return lut_level_; }
+
+    const std::vector<Leaf> &leaves() const { return leaves_; }
+    std::vector<Leaf> &leaves_mut() { return leaves_; }
+
+    // Wide lab stencil [s0, s1) of this wavelet (grid max stencil,
+    // MRAGrid.inl:762-789): Interp4 [-2,4), Interp2 [-1,2).
+    int lab_s0() const { return order_ == 4 ? -2 : -1; }
+    int lab_s1() const { return order_ == 4 ? 4 : 2; }
+
+    int find(int level, int x, int y, int z) const;  // leaf index or -1
+    // leaf whose region contains the level-`lam` point p (unwrapped coords)
+    int lying_leaf(int lam, const int64_t p[3]) const;
+
+    // neighbours of leaf i: pairs (leaf index, via periodic image)
+    void neighbors(int i, std::vector<std::pair<int, bool>> &out) const;
+
+    // Planning.  Return leaf indices (refine) or parent keys (compress).
+    std::vector<int> plan_refine(const std::unordered_set<int> &flagged_ids) const;
+    std::vector<std::array<int, 4>> plan_compress(const std::unordered_set<int> &flagged_ids) const;
+
+    // Topology edits (slots are assigned by the caller).
+    int new_block_id() { return next_id_++; }
+    void replace_leaves(const std::vector<int> &remove, const std::vector<Leaf> &add);
+    void check_jump() const;  // throws on a level jump > 1
+
+    // Ghost point expansion (merged, canonical order).
+    int expand_point(int leaf, int lx, int ly, int lz, std::vector<Term> &scratch, std::vector<Term> &out) const;
+
+private:
+    void rebuild_index();
+    void expand_rec(int lam, const int64_t p[3], double w, std::vector<Term> &T, int depth) const;
+
+    int bpd_, base_level_, max_level_, order_, lut_level_;
+    int next_id_ = 0;
+    std::vector<Leaf> leaves_;
+    std::unordered_map<uint64_t, int> key2leaf_;
+    std::vector<int> lut_;  // (2^lut_level)^3 -> leaf index
+};
+
+inline uint64_t leaf_key(int l, int x, int y, int z)
+{
+    return ((uint64_t)l << 60) | ((uint64_t)(uint32_t)x << 40) | ((uint64_t)(uint32_t)y << 20) | (uint64_t)(uint32_t)z;
+}
+
+// Per-slot device tables derived from a Grid.
+struct Tables {
+    int nslots = 0;                 // table capacity in slots
+    // step: halo-1 faces, f = 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z
+    std::vector<int> fnbr;          // [nslots*6] same-level neighbour slot or -1
+    std::vector<int> fbase;         // [nslots*6] first row of a cross-level face or -1
+    Rows face_rows;                 // 256 rows per cross-level face, row = u + 16 v
+    // wide lab [s0,s1): 26 regions, c = (dx+1) + 3(dy+1) + 9(dz+1)
+    std::vector<int> nbr26;         // [nslots*26]
+    std::vector<int> rbase26;       // [nslots*26]
+    Rows lab_rows;
+    // finest-block look-up for the dump / likelihood
+    std::vector<int> lut_slot;      // (bpd)^3 at the base level -> slot
+};
+
+void build_tables(const Grid &g, int nslots, Tables &t, int nthreads);
+
+// Morton order of the leaves (for the launch work list).
+std::vector<int> morton_order(const Grid &g);
+
+}  // namespace gq

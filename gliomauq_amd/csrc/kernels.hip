@@ -1,0 +1,658 @@
+// kernels.hip -- gfx950 kernels of the FK forward-solve + likelihood path.
+//
+// Data layout in HBM: a pool of 16^3 blocks, one float array per field
+// (phi ping-pong, pw, pg), block slot s at [s*4096, s*4096+4096), x fastest.
+// Per-slot tables (built on the host from the grid topology) give the
+// same-level neighbour of each face/edge/corner region and, for regions
+// covered by a coarser or finer leaf, the first row of a CSR table of
+// (source cell, weight) terms.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace gq {
+
+namespace {
+
+constexpr int B = 16;
+constexpr int B3 = 4096;
+constexpr int LW = B + 2;  // step lab width (halo 1)
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous chunk of the Morton-ordered work list so neighbouring blocks
+// share an L2.
+__device__ __forceinline__ int xcd_remap(int wg, int nwg)
+{
+    // nwg is a multiple of 8 (the launcher pads the grid)
+    const int per = nwg >> 3;
+    return (wg & 7) * per + (wg >> 3);
+}
+
+// Face plane cell of a neighbour block seen across face f (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z)
+// at face coordinates (u, v).
+__device__ __forceinline__ int nbr_face_cell(int f, int u, int v)
+{
+    switch (f) {
+    case 0: return (B - 1) + B * (u + B * v);  // neighbour's x = 15, (y,z) = (u,v)
+    case 1: return 0 + B * (u + B * v);
+    case 2: return u + B * ((B - 1) + B * v);  // neighbour's y = 15, (x,z) = (u,v)
+    case 3: return u + B * (0 + B * v);
+    case 4: return u + B * (v + B * (B - 1));  // neighbour's z = 15, (x,y) = (u,v)
+    default: return u + B * (v + B * 0);
+    }
+}
+
+__device__ __forceinline__ int lab_face_index(int f, int u, int v)
+{
+    switch (f) {
+    case 0: return 0 + LW * ((u + 1) + LW * (v + 1));
+    case 1: return (B + 1) + LW * ((u + 1) + LW * (v + 1));
+    case 2: return (u + 1) + LW * (0 + LW * (v + 1));
+    case 3: return (u + 1) + LW * ((B + 1) + LW * (v + 1));
+    case 4: return (u + 1) + LW * ((v + 1) + LW * 0);
+    default: return (u + 1) + LW * ((v + 1) + LW * (B + 1));
+    }
+}
+
+// 1/K = (pw+pg) / (pw*Dw + pg*Dg): the reciprocal of the tissue-normalised
+// diffusivity of Case2 (Glioma_ReactionDiffusionOperator.h:174-188).  0/0 ->
+// NaN (CSF / outside), as in the reference.
+__device__ __forceinline__ float inv_k(float w, float g, float Dw, float Dg)
+{
+    const float b = w + g;
+    const float D = w * Dw + g * Dg;
+    return b * frcp(D);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Fused RD operator + UpdateTumor (one workgroup per 16^3 block).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
+{
+    __shared__ float s_phi[LW * LW * LW];
+    __shared__ float s_ik[LW * LW * LW];
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    if (w >= a.nwork) return;  // padding workgroups (whole workgroup exits)
+    const int slot = a.work[w];
+    const int t = threadIdx.x;
+    const int tx = t & 15, ty = t >> 4;
+    const size_t base = (size_t)slot * B3;
+
+    float breg[B];
+#pragma unroll
+    for (int z = 0; z < B; z++) {
+        const size_t i = base + z * 256 + t;
+        const float p = a.phi_in[i];
+        const float wm = a.pw[i];
+        const float gm = a.pg[i];
+        breg[z] = wm + gm;
+        const int l = (tx + 1) + LW * ((ty + 1) + LW * (z + 1));
+        s_phi[l] = p;
+        s_ik[l] = inv_k(wm, gm, a.Dw, a.Dg);
+    }
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        const int nb = a.fnbr[slot * 6 + f];
+        float pv;
+        if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, tx, ty)];
+        else pv = a.gphi[a.fbase[slot * 6 + f] + t];
+        const int l = lab_face_index(f, tx, ty);
+        s_phi[l] = pv;
+        s_ik[l] = a.fik[((size_t)slot * 6 + f) * 256 + t];
+    }
+    __syncthreads();
+
+    const float ih2 = a.ih2_lvl[a.level[slot]];
+    const float dt = a.dt, rho = a.rho;
+#pragma unroll 2
+    for (int z = 0; z < B; z++) {
+        const int c = (tx + 1) + LW * ((ty + 1) + LW * (z + 1));
+        const float pc = s_phi[c];
+        const float pW = s_phi[c - 1], pE = s_phi[c + 1];
+        const float pS = s_phi[c - LW], pN = s_phi[c + LW];
+        const float pB = s_phi[c - LW * LW], pF = s_phi[c + LW * LW];
+        float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
+        if (breg[z] + pc > 0.f) {  // tissue = p_w + p_g + phi > 0 (:155-157)
+            const float kc = s_ik[c];
+            d0 = 2.f * frcp(kc + s_ik[c - 1]);
+            d1 = 2.f * frcp(kc + s_ik[c + 1]);
+            d2 = 2.f * frcp(kc + s_ik[c - LW]);
+            d3 = 2.f * frcp(kc + s_ik[c + LW]);
+            d4 = 2.f * frcp(kc + s_ik[c - LW * LW]);
+            d5 = 2.f * frcp(kc + s_ik[c + LW * LW]);
+        }
+        // Neumann doubling, sequential as the reference (:198-203)
+        if (d0 == 0.f && pW > 0.f) d1 *= 2.f;
+        if (d1 == 0.f && pE > 0.f) d0 *= 2.f;
+        if (d2 == 0.f && pS > 0.f) d3 *= 2.f;
+        if (d3 == 0.f && pN > 0.f) d2 *= 2.f;
+        if (d4 == 0.f && pB > 0.f) d5 *= 2.f;
+        if (d5 == 0.f && pF > 0.f) d4 *= 2.f;
+        const float sdf = d0 + d1 + d2 + d3 + d4 + d5;
+        const float reac = rho * pc * (1.f - pc);
+        float dphi;
+        if (__builtin_isfinite(sdf)) {
+            // flux form sum_k df_k (phi_k - phi_c): the same quantity as the
+            // reference's in + out, without its cancellation in FP32
+            const float diff = d0 * (pW - pc) + d1 * (pE - pc) + d2 * (pS - pc) + d3 * (pN - pc) +
+                               d4 * (pB - pc) + d5 * (pF - pc);
+            dphi = ih2 * diff + reac;
+        } else {  // NaN/inf coefficients: the reference's literal form
+            const float in = ih2 * (d0 * pW + d1 * pE + d2 * pS + d3 * pN + d4 * pB + d5 * pF);
+            const float out = -(sdf * pc * ih2);
+            dphi = out + in + reac;
+        }
+        float pn = pc + dt * dphi;
+        pn = (0.f < pn) ? pn : 0.f;  // std::max((Real)0, phi): NaN -> 0
+        pn = (pn < 1.f) ? pn : 1.f;  // std::min((Real)1, phi)
+        a.phi_out[base + z * 256 + t] = pn;
+    }
+}
+
+// Cross-level face ghosts of phi for the step: one thread per row.
+__global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ phi, const int *__restrict__ ptr,
+                                                     const int *__restrict__ src, const float *__restrict__ w,
+                                                     float *__restrict__ out, int nrows)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    float acc = 0.f;
+    for (int k = ptr[r]; k < ptr[r + 1]; k++) acc += w[k] * phi[src[k]];
+    out[r] = acc;
+}
+
+// 1/K of every face-halo cell (static between topology / parameter changes).
+__global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw, const float *__restrict__ pg,
+                                                   const int *__restrict__ work, int nwork,
+                                                   const int *__restrict__ fnbr, const int *__restrict__ fbase,
+                                                   const int *__restrict__ ptr, const int *__restrict__ src,
+                                                   const float *__restrict__ w, float *__restrict__ fik, float Dw,
+                                                   float Dg)
+{
+    const int wi = blockIdx.x / 6;
+    const int f = blockIdx.x % 6;
+    if (wi >= nwork) return;
+    const int slot = work[wi];
+    const int t = threadIdx.x;
+    const int nb = fnbr[slot * 6 + f];
+    float wm, gm;
+    if (nb >= 0) {
+        const size_t i = (size_t)nb * B3 + nbr_face_cell(f, t & 15, t >> 4);
+        wm = pw[i];
+        gm = pg[i];
+    } else {
+        const int r = fbase[slot * 6 + f] + t;
+        wm = 0.f;
+        gm = 0.f;
+        for (int k = ptr[r]; k < ptr[r + 1]; k++) {
+            wm += w[k] * pw[src[k]];
+            gm += w[k] * pg[src[k]];
+        }
+    }
+    fik[((size_t)slot * 6 + f) * 256 + t] = inv_k(wm, gm, Dw, Dg);
+}
+
+// ---------------------------------------------------------------------------
+// Wide lab fill (FP64, canonical row order) shared by FWT and split.
+// ---------------------------------------------------------------------------
+namespace {
+template <int S0, int S1>
+struct LabGeom {
+    static constexpr int L = B + S1 - S0 - 1;
+    static constexpr int N = L * L * L;
+};
+
+template <int S0, int S1>
+__device__ void fill_lab(double *lab, const float *__restrict__ fld, int slot, const LabArgs &a)
+{
+    constexpr int L = LabGeom<S0, S1>::L;
+    constexpr int N = LabGeom<S0, S1>::N;
+    for (int q = threadIdx.x; q < N; q += blockDim.x) {
+        const int x = q % L + S0, y = (q / L) % L + S0, z = q / (L * L) + S0;
+        double v;
+        const int dx = x < 0 ? -1 : (x < B ? 0 : 1);
+        const int dy = y < 0 ? -1 : (y < B ? 0 : 1);
+        const int dz = z < 0 ? -1 : (z < B ? 0 : 1);
+        if ((dx | dy | dz) == 0) {
+            v = (double)fld[(size_t)slot * B3 + x + B * (y + B * z)];
+        } else {
+            const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+            const int cc = c < 13 ? c : c - 1;
+            const int nb = a.nbr26[slot * 26 + cc];
+            if (nb >= 0) {
+                const int lx = x - dx * B, ly = y - dy * B, lz = z - dz * B;
+                v = (double)fld[(size_t)nb * B3 + lx + B * (ly + B * lz)];
+            } else {
+                // region extents (z,y,x order of the host builder)
+                const int sx = dx < 0 ? S0 : (dx == 0 ? 0 : B), ex = dx < 0 ? 0 : (dx == 0 ? B : B + S1 - 1);
+                const int sy = dy < 0 ? S0 : (dy == 0 ? 0 : B), ey = dy < 0 ? 0 : (dy == 0 ? B : B + S1 - 1);
+                const int sz = dz < 0 ? S0 : (dz == 0 ? 0 : B);
+                const int r = a.rbase26[slot * 26 + cc] + (x - sx) + (ex - sx) * ((y - sy) + (ey - sy) * (z - sz));
+                double acc = 0.0;
+                for (int k = a.ptr[r]; k < a.ptr[r + 1]; k++)
+                    acc = __dadd_rn(acc, __dmul_rn(a.w[k], (double)fld[a.src[k]]));
+                v = acc;
+            }
+        }
+        lab[q] = v;
+    }
+}
+
+__device__ __forceinline__ double ga_w(int order, int i)
+{
+    if (order == 2) {  // GaSupport {-2,1}: {-1/2, 1, -1/2}
+        return i == -1 ? 1.0 : -0.5;
+    }
+    switch (i) {  // GaSupport {-4,3}: {1/16, 0, -9/16, 1, -9/16, 0, 1/16}
+    case -4: case 2: return 1.0 / 16.0;
+    case -2: case 0: return -9.0 / 16.0;
+    case -1: return 1.0;
+    default: return 0.0;
+    }
+}
+}  // namespace
+
+// BlockFWT::_fwt on phi (MRAGBlockFWT.h:220-298): detail of coarse point d,
+// code (bx,by,bz) = sum over the tensor filter of lab(2d - i) * ((wZ*wY)*wX),
+// z, y, x loop order, FP64 without contraction; block detail = max over
+// codes 1..7 of |value|.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const float *__restrict__ phi,
+                                                    const int *__restrict__ cand, double *__restrict__ out)
+{
+    constexpr int S0 = ORDER == 4 ? -2 : -1;
+    constexpr int S1 = ORDER == 4 ? 4 : 2;
+    constexpr int L = LabGeom<S0, S1>::L;
+    constexpr int gs0 = ORDER == 4 ? -4 : -2, gs1 = ORDER == 4 ? 3 : 1;
+    __shared__ double lab[LabGeom<S0, S1>::N];
+    __shared__ double red[256];
+    const int slot = cand[blockIdx.x];
+    fill_lab<S0, S1>(lab, phi, slot, a);
+    __syncthreads();
+    double m = 0.0;
+    for (int q = threadIdx.x; q < 512; q += blockDim.x) {
+        const int dx = q & 7, dy = (q >> 3) & 7, dz = q >> 6;
+        for (int code = 1; code < 8; code++) {
+            const int bx = code & 1, by = (code >> 1) & 1, bz = (code >> 2) & 1;
+            const int sx = bx ? gs0 : 0, ex = bx ? gs1 : 1;
+            const int sy = by ? gs0 : 0, ey = by ? gs1 : 1;
+            const int sz = bz ? gs0 : 0, ez = bz ? gs1 : 1;
+            double r = 0.0;
+            for (int iz = sz; iz < ez; iz++) {
+                const double wZ = bz ? ga_w(ORDER, iz) : 1.0;
+                for (int iy = sy; iy < ey; iy++) {
+                    const double wY = by ? ga_w(ORDER, iy) : 1.0;
+                    const double wZwY = __dmul_rn(wZ, wY);
+                    for (int ix = sx; ix < ex; ix++) {
+                        const double wX = bx ? ga_w(ORDER, ix) : 1.0;
+                        const int x = 2 * dx - ix - S0, y = 2 * dy - iy - S0, z = 2 * dz - iz - S0;
+                        r = __dadd_rn(r, __dmul_rn(lab[x + L * (y + L * z)], __dmul_rn(wZwY, wX)));
+                    }
+                }
+            }
+            const double av = fabs(r);
+            m = m > av ? m : av;
+        }
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + s] ? red[threadIdx.x] : red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// BlockSplitter::_splitBlocks (MRAGBlockSplitter.h:24-99) as a gather: child
+// point d = sum over parent lab points s (z, y, x increasing) of
+// lab(s) * ((wZ*wY)*wX), w = Hs(2s - d), for phi, pw and pg.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_split(LabArgs a, float *phi, float *pw, float *pg,
+                                               const int *__restrict__ parents, const int *__restrict__ children)
+{
+    constexpr int S0 = ORDER == 4 ? -2 : -1;
+    constexpr int S1 = ORDER == 4 ? 4 : 2;
+    constexpr int L = LabGeom<S0, S1>::L;
+    __shared__ double lab[LabGeom<S0, S1>::N];
+    const int pslot = parents[blockIdx.x];
+    float *fields[3] = {phi, pw, pg};
+    for (int f = 0; f < 3; f++) {
+        fill_lab<S0, S1>(lab, fields[f], pslot, a);
+        __syncthreads();
+        for (int q = threadIdx.x; q < 8 * B3; q += blockDim.x) {
+            const int dx = q & 31, dy = (q >> 5) & 31, dz = q >> 10;
+            const int mx = dx >> 1, my = dy >> 1, mz = dz >> 1;
+            const int px = dx & 1, py = dy & 1, pz = dz & 1;
+            // taps per parity: even -> s = m (w 1); odd -> Interp4: m-1..m+2
+            // (-1/16, 9/16, 9/16, -1/16), Interp2: m, m+1 (1/2, 1/2)
+            const int nz = pz ? (ORDER == 4 ? 4 : 2) : 1;
+            const int ny = py ? (ORDER == 4 ? 4 : 2) : 1;
+            const int nx = px ? (ORDER == 4 ? 4 : 2) : 1;
+            double acc = 0.0;
+            for (int kz = 0; kz < nz; kz++) {
+                const int oz = pz ? (ORDER == 4 ? kz - 1 : kz) : 0;
+                const double wz = pz ? (ORDER == 4 ? ((kz == 0 || kz == 3) ? -1.0 / 16.0 : 9.0 / 16.0) : 0.5) : 1.0;
+                for (int ky = 0; ky < ny; ky++) {
+                    const int oy = py ? (ORDER == 4 ? ky - 1 : ky) : 0;
+                    const double wy = py ? (ORDER == 4 ? ((ky == 0 || ky == 3) ? -1.0 / 16.0 : 9.0 / 16.0) : 0.5) : 1.0;
+                    const double wzy = __dmul_rn(wz, wy);
+                    for (int kx = 0; kx < nx; kx++) {
+                        const int ox = px ? (ORDER == 4 ? kx - 1 : kx) : 0;
+                        const double wx = px ? (ORDER == 4 ? ((kx == 0 || kx == 3) ? -1.0 / 16.0 : 9.0 / 16.0) : 0.5) : 1.0;
+                        const int s = (mx + ox - S0) + L * ((my + oy - S0) + L * (mz + oz - S0));
+                        acc = __dadd_rn(acc, __dmul_rn(lab[s], __dmul_rn(wzy, wx)));
+                    }
+                }
+            }
+            const int child = (dx >> 4) + 2 * ((dy >> 4) + 2 * (dz >> 4));
+            const int cs = children[blockIdx.x * 8 + child];
+            fields[f][(size_t)cs * B3 + (dx & 15) + B * ((dy & 15) + B * (dz & 15))] = (float)acc;
+        }
+        __syncthreads();
+    }
+}
+
+// BlockCollapser::_collapseBlocks (MRAGBlockCollapser.h:62-85): Ha = {1},
+// the parent takes every other point of its 8 children.
+__global__ __launch_bounds__(256) void k_collapse(float *phi, float *pw, float *pg, const int *__restrict__ parents,
+                                                  const int *__restrict__ children)
+{
+    const int pslot = parents[blockIdx.x];
+    for (int q = threadIdx.x; q < B3; q += blockDim.x) {
+        const int x = q & 15, y = (q >> 4) & 15, z = q >> 8;
+        const int c = (x >> 3) + 2 * ((y >> 3) + 2 * (z >> 3));
+        const int cs = children[blockIdx.x * 8 + c];
+        const size_t src = (size_t)cs * B3 + 2 * (x & 7) + B * (2 * (y & 7) + B * 2 * (z & 7));
+        const size_t dst = (size_t)pslot * B3 + q;
+        phi[dst] = phi[src];
+        pw[dst] = pw[src];
+        pg[dst] = pg[src];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Uniform view (Glioma_HG_UQ::_dumpUQoutput, Glioma_HG_UQ.cpp:309-372) and
+// likelihood reductions.  A finest voxel v maps to the leaf covering it; a
+// leaf r = gpd/(16*2^level) times coarser writes only the first min(r,4)
+// voxels of each cell per axis (the reference's `else` branch), the rest
+// are holes (value `hole`).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool voxel_value(const UniformArgs &u, int64_t v, double &val)
+{
+    const int gpd = u.gpd;
+    const int vx = (int)(v % gpd), vy = (int)((v / gpd) % gpd), vz = (int)(v / ((int64_t)gpd * gpd));
+    const int bpd = gpd >> 4;
+    const int slot = u.lut_slot[(vx >> 4) + bpd * ((vy >> 4) + bpd * (vz >> 4))];
+    const int lvl = u.level[slot];
+    const int r = bpd >> lvl;  // voxels per leaf cell per axis (>= 1)
+    const int ox = u.index[slot * 3 + 0] * B * r, oy = u.index[slot * 3 + 1] * B * r, oz = u.index[slot * 3 + 2] * B * r;
+    const int lx = vx - ox, ly = vy - oy, lz = vz - oz;
+    const int ix = lx / r, iy = ly / r, iz = lz / r;
+    const int rep = r < 4 ? r : 4;
+    if (lx - ix * r >= rep || ly - iy * r >= rep || lz - iz * r >= rep) return false;
+    val = (double)u.phi[(size_t)slot * B3 + ix + B * (iy + B * iz)];
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_dump_uniform(UniformArgs u, double *out)
+{
+    const int64_t n = (int64_t)u.gpd * u.gpd * u.gpd;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        double val;
+        if (voxel_value(u, v, val)) out[v] = val;
+    }
+}
+
+namespace {
+__device__ __forceinline__ double wave_sum(double x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    return x;
+}
+// block sum of NV doubles into partial[blockIdx.x*NV + k]
+template <int NV>
+__device__ void block_sum_store(double (&v)[NV], double *partial)
+{
+    __shared__ double sh[NV][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) sh[k][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        partial[(size_t)blockIdx.x * NV + k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+    }
+}
+}  // namespace
+
+// Flip-rate Bernoulli + Gaussian PET log-likelihood, per-block partials.
+__global__ __launch_bounds__(256) void k_lik_flip(UniformArgs u, const uint8_t *__restrict__ t1,
+                                                  const uint8_t *__restrict__ t2, const float *__restrict__ pet,
+                                                  double th1, double th2, double lp1, double lp0, double c0,
+                                                  double is2, double *partial)
+{
+    const int64_t n = (int64_t)u.gpd * u.gpd * u.gpd;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        double m = 0.0;
+        if (!voxel_value(u, v, m)) m = u.hole;
+        const int m1 = m > th1, m2 = m > th2;
+        acc[0] += (m1 == (int)t1[v]) ? lp1 : lp0;
+        acc[1] += (m2 == (int)t2[v]) ? lp1 : lp0;
+        const float p = pet[v];
+        if (!__builtin_isnan(p)) {
+            const double r = (double)p - m;
+            acc[2] += c0 - r * r * is2;
+        }
+    }
+    block_sum_store<3>(acc, partial);
+}
+
+// Reference model, PET part (HGG_Likelihood.cpp:40-75): lattice
+// ix,iy,iz % step == 0, PET > 0: sum (u - scale*PET)^2 and count.
+__global__ __launch_bounds__(256) void k_lik_ref_pet(UniformArgs u, const double *__restrict__ pet, int step,
+                                                     double scale, double *partial)
+{
+    const int gpd = u.gpd;
+    const int nl = (gpd + step - 1) / step;
+    const int64_t n = (int64_t)nl * nl * nl;
+    double acc[2] = {0.0, 0.0};
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ix = (q % nl) * step, iy = ((q / nl) % nl) * step, iz = (q / ((int64_t)nl * nl)) * step;
+        const int64_t v = ix + gpd * (iy + (int64_t)gpd * iz);
+        const double pv = pet[v];
+        if (pv > 0.) {
+            double m = 0.0;
+            if (!voxel_value(u, v, m)) m = u.hole;
+            const double d = m - scale * pv;
+            acc[0] += d * d;
+            acc[1] += 1.0;
+        }
+    }
+    block_sum_store<2>(acc, partial);
+}
+
+namespace {
+__device__ __forceinline__ double log_bernoulli(double m, double y, double uc, double slope)
+{
+    // HGG_Likelihood::_computeLogBernoulli (HGG_Likelihood.cpp:136-160)
+    const double is2 = 1. / slope;
+    const double diff = m - uc;
+    const int sg = diff < -0.0 ? -1 : (diff > 0.0 ? 1 : 0);
+    const double alpha = 0.5 + 0.5 * sg * (1. - exp(-diff * diff * is2));
+    return (y == 1) ? log(alpha) : log(1. - alpha);
+}
+}  // namespace
+
+// Reference model, T1/T2 parts (HGG_Likelihood.cpp:77-128): selective points
+// (filtered by % step) or every voxel.
+__global__ __launch_bounds__(256) void k_lik_ref_ti(UniformArgs u, const double *__restrict__ t1,
+                                                    const double *__restrict__ t2, const int *__restrict__ pts,
+                                                    int npts, int selective, int step, double t1uc, double t2uc,
+                                                    double slope, double *partial)
+{
+    const int gpd = u.gpd;
+    const int64_t n = selective ? (int64_t)npts : (int64_t)gpd * gpd * gpd;
+    double acc[2] = {0.0, 0.0};
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t v;
+        if (selective) {
+            const int ix = pts[3 * q], iy = pts[3 * q + 1], iz = pts[3 * q + 2];
+            if ((ix % step) || (iy % step) || (iz % step)) continue;
+            v = ix + gpd * (iy + (int64_t)gpd * iz);
+        } else {
+            v = q;
+        }
+        double m = 0.0;
+        if (!voxel_value(u, v, m)) m = u.hole;
+        acc[0] += log_bernoulli(m, t1[v], t1uc, slope);
+        acc[1] += log_bernoulli(m, t2[v], t2uc, slope);
+    }
+    block_sum_store<2>(acc, partial);
+}
+
+// Sum per-block partials in block order (deterministic).
+__global__ void k_sum_partials(const double *partial, int nblocks, int nv, double *out)
+{
+    const int k = threadIdx.x;
+    if (k >= nv) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * nv + k];
+    out[k] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_fk_step(const StepArgs &a, hipStream_t st)
+{
+    if (a.nwork <= 0) return hipSuccess;
+    const int grid = (a.nwork + 7) & ~7;
+    hipLaunchKernelGGL(k_fk_step, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_ghosts(const float *phi, const int *ptr, const int *src, const float *w, float *out, int nrows,
+                              hipStream_t st)
+{
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_face_ghosts, dim3((nrows + 255) / 256), dim3(256), 0, st, phi, ptr, src, w, out, nrows);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
+                            const int *fbase, const int *ptr, const int *src, const float *w, float *fik, float Dw,
+                            float Dg, hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_face_prep, dim3(nwork * 6), dim3(256), 0, st, pw, pg, work, nwork, fnbr, fbase, ptr, src, w,
+                       fik, Dw, Dg);
+    return hipGetLastError();
+}
+
+hipError_t launch_fwt_detail(int order, const LabArgs &a, const float *phi, const int *cand, int ncand, double *out,
+                             hipStream_t st)
+{
+    if (ncand <= 0) return hipSuccess;
+    if (order == 4) hipLaunchKernelGGL(k_fwt_detail<4>, dim3(ncand), dim3(256), 0, st, a, phi, cand, out);
+    else hipLaunchKernelGGL(k_fwt_detail<2>, dim3(ncand), dim3(256), 0, st, a, phi, cand, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_split(int order, const LabArgs &a, float *phi, float *pw, float *pg, const int *parents,
+                        const int *children, int nparents, hipStream_t st)
+{
+    if (nparents <= 0) return hipSuccess;
+    if (order == 4) hipLaunchKernelGGL(k_split<4>, dim3(nparents), dim3(256), 0, st, a, phi, pw, pg, parents, children);
+    else hipLaunchKernelGGL(k_split<2>, dim3(nparents), dim3(256), 0, st, a, phi, pw, pg, parents, children);
+    return hipGetLastError();
+}
+
+hipError_t launch_collapse(float *phi, float *pw, float *pg, const int *parents, const int *children, int nparents,
+                           hipStream_t st)
+{
+    if (nparents <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_collapse, dim3(nparents), dim3(256), 0, st, phi, pw, pg, parents, children);
+    return hipGetLastError();
+}
+
+hipError_t launch_dump_uniform(const UniformArgs &u, double *out, hipStream_t st)
+{
+    const int64_t n = (int64_t)u.gpd * u.gpd * u.gpd;
+    int nb = (int)((n + 255) / 256);
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_dump_uniform, dim3(nb), dim3(256), 0, st, u, out);
+    return hipGetLastError();
+}
+
+int reduce_blocks(int64_t n)
+{
+    int64_t nb = (n + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    return (int)nb;
+}
+
+hipError_t launch_lik_flip(const UniformArgs &u, const uint8_t *t1, const uint8_t *t2, const float *pet, double th1,
+                           double th2, double lp1, double lp0, double c0, double is2, double *partial, int nblocks,
+                           double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lik_flip, dim3(nblocks), dim3(256), 0, st, u, t1, t2, pet, th1, th2, lp1, lp0, c0, is2,
+                       partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial, nblocks, 3, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_lik_ref(const UniformArgs &u, const double *pet, const double *t1, const double *t2,
+                          const int *pts, int npts, int selective, int step_pet, int step_ti, double scale,
+                          double t1uc, double t2uc, double slope, double *partial, int nblocks, double *out,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lik_ref_pet, dim3(nblocks), dim3(256), 0, st, u, pet, step_pet, scale, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial, nblocks, 2, out);
+    hipLaunchKernelGGL(k_lik_ref_ti, dim3(nblocks), dim3(256), 0, st, u, t1, t2, pts, npts, selective, step_ti, t1uc,
+                       t2uc, slope, partial + (size_t)nblocks * 2);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial + (size_t)nblocks * 2, nblocks, 2, out + 2);
+    return hipGetLastError();
+}
+
+}  // namespace gq
+
+namespace gq {
+// Block transfer between a dense host-order staging array and pool slots.
+__global__ __launch_bounds__(256) void k_scatter_blocks(float *pool, const float *__restrict__ dense,
+                                                        const int *__restrict__ slots)
+{
+    const int k = blockIdx.x;
+    const size_t d = (size_t)slots[k] * 4096, s = (size_t)k * 4096;
+    for (int i = threadIdx.x; i < 4096; i += 256) pool[d + i] = dense[s + i];
+}
+__global__ __launch_bounds__(256) void k_gather_blocks(const float *__restrict__ pool, float *dense,
+                                                       const int *__restrict__ slots)
+{
+    const int k = blockIdx.x;
+    const size_t s = (size_t)slots[k] * 4096, d = (size_t)k * 4096;
+    for (int i = threadIdx.x; i < 4096; i += 256) dense[d + i] = pool[s + i];
+}
+hipError_t launch_scatter_blocks(float *pool, const float *dense, const int *slots, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_blocks, dim3(n), dim3(256), 0, st, pool, dense, slots);
+    return hipGetLastError();
+}
+hipError_t launch_gather_blocks(const float *pool, float *dense, const int *slots, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_blocks, dim3(n), dim3(256), 0, st, pool, dense, slots);
+    return hipGetLastError();
+}
+}  // namespace gq

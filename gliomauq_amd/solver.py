@@ -1,0 +1,408 @@
+"""Host-side mirror of the reference's block-operator API over the C-ABI.
+
+Names follow the reference so that a caller of the MRAG/Glioma path finds the
+same pieces (paths relative to the reference's source/ tree):
+
+  Grid                              MRAGcore/MRAGrid.h:45-186 (+ device block data)
+  Grid.getBlocksInfo                MRAGcore/MRAGrid.inl:191-219
+  Grid.refine / Grid.compress       MRAGcore/MRAGrid.inl:248-355, :384-486
+  Glioma_ReactionDiffusionOperator  Glioma/Glioma_ReactionDiffusionOperator.h:13-222
+  UpdateTumor                       Glioma/Glioma_ReactionDiffusionOperator.h:227-266
+  BlockProcessing.pipeline_process  MRAGmultithreading/MRAGBlockProcessing_TBB.h:476-512
+  BlockProcessing.process           MRAGmultithreading/MRAGBlockProcessing_TBB.h:249-264
+  BlockFWT.details                  MRAGcore/MRAGBlockFWT.h:448-478
+  Science.AutomaticRefinement       MRAGscience/MRAGScienceCore.h:42-108
+  Science.AutomaticCompression      MRAGscience/MRAGScienceCore.h:117-177
+  Glioma_HG_UQ._reactionDiffusionStep / run / _dumpUQoutput
+                                    Glioma/Glioma_HG_UQ.cpp:266-277, 511-588, 309-372
+  HGG_Likelihood                    Likelihood/source/HGG_Likelihood.cpp:11-189
+
+The RD functor and UpdateTumor execute as one fused kernel: pipeline_process
+with the RD functor stages it, and the following process(UpdateTumor) runs
+the step (a pipeline_process that is not followed by an UpdateTumor raises).
+Every computation runs in libgq.so on the GPU; there is no CPU path here.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import BlockInfo, FKParams, GQError, ptr
+
+BS = 16
+
+
+def _check(L, ctx, st):
+    if st != _lib.GQ_OK:
+        msg = L.gq_last_error(ctx).decode() if ctx else ""
+        raise GQError(st, msg)
+
+
+class Context:
+    """A gq_ctx on one GPU (device memory + stream)."""
+
+    def __init__(self, device=0):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        st = self.L.gq_ctx_create(C.byref(h), device)
+        if st != _lib.GQ_OK:
+            raise GQError(st, "gq_ctx_create(device=%d)" % device)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.gq_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        _check(self.L, self.h, getattr(self.L, name)(self.h, *args))
+
+
+class Grid:
+    """Grid<W,B>(bpd,bpd,bpd) with Refiner_SpaceExtension(1, maxLevel) and
+    Compressor(1); block data (phi, p_w, p_g) resident on the GPU."""
+
+    FIELDS = {"phi": _lib.FIELD_PHI, "pw": _lib.FIELD_PWM, "pg": _lib.FIELD_PGM}
+
+    def __init__(self, ctx, blocks_per_dim, max_level, wavelet_order=4):
+        self.ctx = ctx
+        self.bpd = blocks_per_dim
+        self.max_level = max_level
+        self.order = wavelet_order
+        ctx.call("gq_grid_create", blocks_per_dim, max_level, wavelet_order)
+
+    # -- topology -------------------------------------------------------
+    def n_blocks(self):
+        n = C.c_int()
+        self.ctx.call("gq_grid_block_count", C.byref(n))
+        return n.value
+
+    def getBlocksInfo(self):
+        n = self.n_blocks()
+        arr = (BlockInfo * max(n, 1))()
+        m = C.c_int()
+        self.ctx.call("gq_grid_blocks_info", arr, n, C.byref(m))
+        return [arr[i] for i in range(m.value)]
+
+    def levels_indices(self):
+        info = self.getBlocksInfo()
+        lv = np.array([b.level for b in info], np.int32)
+        ix = np.array([[b.index[0], b.index[1], b.index[2]] for b in info], np.int32).reshape(-1, 3)
+        ids = np.array([b.block_id for b in info], np.int32)
+        return lv, ix, ids
+
+    def refine(self, block_ids):
+        ids = np.ascontiguousarray(sorted(block_ids), np.int32)
+        out = C.c_int()
+        self.ctx.call("gq_refine_blocks", ptr(ids, C.c_int32), len(ids), C.byref(out))
+        return out.value
+
+    def compress(self, block_ids):
+        ids = np.ascontiguousarray(sorted(block_ids), np.int32)
+        out = C.c_int()
+        self.ctx.call("gq_compress_blocks", ptr(ids, C.c_int32), len(ids), C.byref(out))
+        return out.value
+
+    # -- data -------------------------------------------------------------
+    def write(self, name, blocks):
+        a = np.ascontiguousarray(blocks, np.float32)
+        n = self.n_blocks()
+        if a.shape != (n, BS, BS, BS):
+            raise ValueError("expected (%d,16,16,16), got %s" % (n, a.shape))
+        self.ctx.call("gq_grid_write_field", self.FIELDS[name], ptr(a, C.c_float), n)
+
+    def read(self, name):
+        n = self.n_blocks()
+        a = np.zeros((n, BS, BS, BS), np.float32)
+        self.ctx.call("gq_grid_read_field", self.FIELDS[name], ptr(a, C.c_float), n)
+        return a
+
+    def sample(self, name, fn):
+        """Fill a field from fn(x, y, z) evaluated at every block point
+        (vertex-centred positions origin + h*i), arrays indexed [z,y,x]."""
+        blocks = []
+        for b in self.getBlocksInfo():
+            r = np.arange(BS) * b.h
+            z, y, x = np.meshgrid(b.origin[2] + r, b.origin[1] + r, b.origin[0] + r, indexing="ij")
+            blocks.append(fn(x, y, z))
+        self.write(name, np.stack(blocks))
+
+    def sample_volume(self, name, vol):
+        """Point-sample a uniform volume vol[z,y,x] of gpd^3 = (bpd*16)^3 points
+        onto every leaf (a leaf at level l takes every 2^(base-l)-th point)."""
+        gpd = vol.shape[0]
+        base = int(round(math.log2(gpd // BS)))
+        blocks = []
+        for b in self.getBlocksInfo():
+            s = 1 << (base - b.level)
+            o = [b.index[a] * BS * s for a in range(3)]
+            blocks.append(vol[o[2]:o[2] + BS * s:s, o[1]:o[1] + BS * s:s, o[0]:o[0] + BS * s:s])
+        self.write(name, np.stack(blocks))
+
+    # -- uniform view -------------------------------------------------------
+    def dump_uniform(self):
+        """Glioma_HG_UQ::_dumpUQoutput: phi on the gpd^3 grid (float64, [z,y,x])."""
+        gpd = self.bpd * BS
+        out = np.zeros(gpd ** 3, np.float64)
+        self.ctx.call("gq_dump_uniform", ptr(out, C.c_double), out.size)
+        return out.reshape(gpd, gpd, gpd)
+
+
+class Glioma_ReactionDiffusionOperator:
+    """RD functor parameters (Dw, Dg, rho) in unit-cube units; stencil [-1,2)."""
+    stencil_start = (-1, -1, -1)
+    stencil_end = (2, 2, 2)
+
+    def __init__(self, Dw, Dg, rho):
+        self.Dw, self.Dg, self.rho = Dw, Dg, rho
+
+
+class UpdateTumor:
+    def __init__(self, dt):
+        self.dt = dt
+
+
+class BlockProcessing:
+    """Drop-in for BlockProcessing_Pipeline_TBB on the GPU."""
+
+    def __init__(self):
+        self._staged = None
+
+    def pipeline_process(self, vInfo, grid, boundaryInfo, op):
+        if not isinstance(op, Glioma_ReactionDiffusionOperator):
+            raise TypeError("only Glioma_ReactionDiffusionOperator runs on this path")
+        self._staged = (grid, op)
+
+    def process(self, vInfo, grid, op, granularity=4):
+        if not isinstance(op, UpdateTumor):
+            raise TypeError("only UpdateTumor runs on this path")
+        if self._staged is None or self._staged[0] is not grid:
+            raise RuntimeError("UpdateTumor without a staged Glioma_ReactionDiffusionOperator")
+        rd = self._staged[1]
+        self._staged = None
+        fk_steps(grid, rd.Dw, rd.Dg, rd.rho, op.dt, 1)
+
+
+def fk_steps(grid, Dw, Dg, rho, dt, n_steps):
+    p = FKParams(Dw, Dg, rho, dt)
+    grid.ctx.call("gq_fk_steps", C.byref(p), int(n_steps))
+
+
+def step_timing(grid):
+    ms = C.c_double()
+    n = C.c_int()
+    cells = C.c_int64()
+    grid.ctx.call("gq_last_step_timing", C.byref(ms), C.byref(n), C.byref(cells))
+    return ms.value, n.value, cells.value
+
+
+def set_step_timing(grid, per_kernel):
+    grid.ctx.call("gq_set_step_timing", 1 if per_kernel else 0)
+
+
+class BlockFWT:
+    @staticmethod
+    def details(grid):
+        n = grid.n_blocks()
+        out = np.zeros(n, np.float64)
+        grid.ctx.call("gq_fwt_details", ptr(out, C.c_double), n)
+        return out
+
+
+class Science:
+    @staticmethod
+    def AutomaticRefinement(grid, tol, max_level=-1, max_loops=-1):
+        out = C.c_int()
+        grid.ctx.call("gq_auto_refine", tol, max_level, max_loops, C.byref(out))
+        return out.value
+
+    @staticmethod
+    def AutomaticCompression(grid, tol, max_loops=-1):
+        out = C.c_int()
+        grid.ctx.call("gq_auto_compress", tol, max_loops, C.byref(out))
+        return out.value
+
+
+def n_steps_until(T, dt):
+    """Number of iterations of `while (t <= tend) { step; t += dt; }`
+    (Glioma_HG_UQ.cpp:544-552) with t accumulated in double."""
+    t = 0.0
+    n = 0
+    while t <= T:
+        n += 1
+        t += dt
+    return n
+
+
+class Glioma_HG_UQ:
+    """The forward-solve driver of Glioma_HG_UQ::run (Glioma_HG_UQ.cpp:511-588).
+
+    Dw_cm2_day / rho_day / T_days as in HGG_InputParameters.txt; L is the
+    physical edge of the unit cube (cm).  Adaptation runs every
+    `adapt_every` steps (integer schedule) or, when `dumpfreq` is given, with
+    the reference's `t >= whenToWrite` schedule in days."""
+
+    def __init__(self, grid, L_cm, cfl=0.99, adaptive=True, tol_refine=1e-4, tol_compress=1e-5,
+                 adapt_every=None, dumpfreq=None):
+        self.grid = grid
+        self.L = L_cm
+        self.cfl = cfl
+        self.adaptive = adaptive
+        self.tol_r = tol_refine
+        self.tol_c = tol_compress
+        self.adapt_every = adapt_every
+        self.dumpfreq = dumpfreq
+        self.bp = BlockProcessing()
+        self.history = []
+
+    def params(self, Dw_cm2_day, rho_day):
+        Dw = Dw_cm2_day / (self.L * self.L)
+        Dg = 0.1 * Dw
+        h = 1.0 / (BS * self.grid.bpd)
+        dt = self.cfl * h * h / (2.0 * 3 * max(Dw, Dg))
+        return Dw, Dg, rho_day, dt
+
+    def _adapt(self):
+        g = self.grid
+        nr = Science.AutomaticRefinement(g, self.tol_r, g.max_level, 1)
+        nc = Science.AutomaticCompression(g, self.tol_c, -1)
+        self.history.append((nr, nc, g.n_blocks()))
+
+    def run(self, Dw_cm2_day, rho_day, T_days, chunk_cb=None):
+        Dw, Dg, rho, dt = self.params(Dw_cm2_day, rho_day)
+        n_total = n_steps_until(T_days, dt)
+        done = 0
+        if not self.adaptive:
+            fk_steps(self.grid, Dw, Dg, rho, dt, n_total)
+            return n_total, dt
+        if self.dumpfreq is not None:
+            t = 0.0
+            when = self.dumpfreq
+            while t <= T_days:
+                fk_steps(self.grid, Dw, Dg, rho, dt, 1)
+                t += dt
+                done += 1
+                if t >= when:
+                    self._adapt()
+                    when += self.dumpfreq
+        else:
+            k = self.adapt_every
+            while done < n_total:
+                m = min(k, n_total - done)
+                fk_steps(self.grid, Dw, Dg, rho, dt, m)
+                done += m
+                if chunk_cb:
+                    chunk_cb(done)
+                if m == k:
+                    self._adapt()
+        # final refinement before the dump (:574-575)
+        Science.AutomaticRefinement(self.grid, self.tol_r, self.grid.max_level, 1)
+        return n_total, dt
+
+
+class HGG_Likelihood:
+    """Reference likelihood model on the device (HGG_Likelihood.cpp:11-189).
+
+    params = (PETsigma2, PETscale, T1uc, T2uc, slope) as LikelihoodInput.txt;
+    flags as -selectivePoints (default 1), -stepPet (3), -stepTi (2)."""
+
+    def __init__(self, grid, pet, t1, t2, params, points_xyz=None, selective=True, step_pet=3, step_ti=2):
+        self.grid = grid
+        gpd = grid.bpd * BS
+        self.params = np.ascontiguousarray(params, np.float64)
+        self.selective = selective
+        self.step_pet = step_pet
+        self.step_ti = step_ti
+        pts = np.zeros((0, 3), np.int32) if points_xyz is None else np.ascontiguousarray(points_xyz, np.int32)
+        self._keep = [np.ascontiguousarray(a, np.float64) for a in (pet, t1, t2)] + [pts]
+        grid.ctx.call("gq_obs_ref_upload", gpd, ptr(self._keep[0], C.c_double), ptr(self._keep[1], C.c_double),
+                      ptr(self._keep[2], C.c_double), ptr(pts, C.c_int32), len(pts))
+
+    def run(self):
+        out = np.zeros(4)
+        self.grid.ctx.call("gq_loglik_ref", ptr(self.params, C.c_double), int(self.selective), self.step_pet,
+                           self.step_ti, ptr(out, C.c_double))
+        return out  # logL, L_pet, L_T1, L_T2
+
+
+class FlipLikelihood:
+    """Flip-rate Bernoulli (T1Gd, FLAIR) + Gaussian PET likelihood on the device."""
+
+    def __init__(self, grid, t1, t2, pet, th1=0.7, th2=0.25, flip=0.05, sigma=0.05):
+        self.grid = grid
+        gpd = grid.bpd * BS
+        self._keep = (np.ascontiguousarray(t1, np.uint8), np.ascontiguousarray(t2, np.uint8),
+                      np.ascontiguousarray(pet, np.float32))
+        self.params = np.array([th1, th2, flip, sigma], np.float64)
+        grid.ctx.call("gq_obs_flip_upload", gpd, ptr(self._keep[0], C.c_uint8), ptr(self._keep[1], C.c_uint8),
+                      ptr(self._keep[2], C.c_float))
+
+    def loglik(self):
+        out = np.zeros(4)
+        self.grid.ctx.call("gq_loglik_flip", ptr(self.params, C.c_double), ptr(out, C.c_double))
+        return out  # logL, L_T1, L_FLAIR, L_PET
+
+
+class Topology:
+    """Host-only Grid topology (gq_topo_*): refine/compress planning and ghost
+    expansion without block data and without a GPU."""
+
+    def __init__(self, blocks_per_dim, max_level, wavelet_order=4):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        st = self.L.gq_topo_create(C.byref(h), blocks_per_dim, max_level, wavelet_order)
+        if st != _lib.GQ_OK:
+            raise GQError(st, "gq_topo_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.gq_topo_destroy(self.h)
+        except Exception:
+            pass
+
+    def _c(self, st):
+        if st != _lib.GQ_OK:
+            raise GQError(st, "topology call failed")
+
+    def leaves(self):
+        n = C.c_int()
+        self._c(self.L.gq_topo_leaves(self.h, None, None, None, 0, C.byref(n)))
+        m = n.value
+        lv = np.zeros(m, np.int32)
+        ix = np.zeros((m, 3), np.int32)
+        ids = np.zeros(m, np.int32)
+        self._c(self.L.gq_topo_leaves(self.h, ptr(lv, C.c_int32), ptr(ix, C.c_int32), ptr(ids, C.c_int32), m,
+                                      C.byref(n)))
+        return lv, ix, ids
+
+    def refine(self, ids):
+        a = np.ascontiguousarray(sorted(ids), np.int32)
+        out = C.c_int()
+        self._c(self.L.gq_topo_refine(self.h, ptr(a, C.c_int32), len(a), C.byref(out)))
+        return out.value
+
+    def compress(self, ids):
+        a = np.ascontiguousarray(sorted(ids), np.int32)
+        out = C.c_int()
+        self._c(self.L.gq_topo_compress(self.h, ptr(a, C.c_int32), len(a), C.byref(out)))
+        return out.value
+
+    def point_terms(self, leaf, loc, cap=4096):
+        lo = np.ascontiguousarray(loc, np.int32)
+        ol = np.zeros(cap, np.int32)
+        oc = np.zeros(cap, np.int32)
+        ow = np.zeros(cap, np.float64)
+        n = C.c_int()
+        self._c(self.L.gq_topo_point_terms(self.h, leaf, ptr(lo, C.c_int32), cap, ptr(ol, C.c_int32),
+                                           ptr(oc, C.c_int32), ptr(ow, C.c_double), C.byref(n)))
+        m = n.value
+        return ol[:m].copy(), oc[:m].copy(), ow[:m].copy()

@@ -1,0 +1,166 @@
+/*
+ * gq_api.h -- C-ABI of the MI355X-native Fisher-Kolmogorov forward solve and
+ * likelihood path (libgq.so).
+ *
+ * Plain C types only: opaque context, plain pointers and sizes, status codes.
+ * One host thread per context; every call is ordered on the context's HIP
+ * stream and returns when its results are on the host (or resident in device
+ * memory for calls that return nothing).  Device memory is owned by the
+ * context; input buffers stay owned by the caller and are copied.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference's source/ tree).
+ */
+#ifndef GQ_API_H
+#define GQ_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GQ_OK = 0,
+    GQ_ERR_ARG = 1,         /* invalid argument (sizes, ids, ranges) */
+    GQ_ERR_HIP = 2,         /* a HIP runtime call failed */
+    GQ_ERR_NOMEM = 3,       /* device or host allocation failed */
+    GQ_ERR_STATE = 4,       /* call out of order (no grid, no observations) */
+    GQ_ERR_TOPOLOGY = 5,    /* grid invariant violated (level jump > 1) */
+    GQ_ERR_UNSUPPORTED = 6, /* configuration outside what the path supports */
+    GQ_ERR_NUMERIC = 7      /* non-finite parameter or time step */
+} gq_status;
+
+typedef struct gq_ctx gq_ctx;
+
+/* MRAG::BlockInfo (MRAGcore/MRAGCommon.h:166-260) for a 16^3 block on the
+ * unit cube: origin = index * 2^-level (vertex-centred), h = 2^-level / 16. */
+typedef struct {
+    int32_t block_id; /* BlockInfo::blockID (stable until refine/compress) */
+    int16_t index[3]; /* BlockInfo::index */
+    int16_t level;    /* BlockInfo::level */
+    float origin[3];  /* BlockInfo::origin */
+    float h;          /* BlockInfo::h[0] (isotropic) */
+} gq_block_info;
+
+/* Parameters of Glioma_ReactionDiffusionOperator(Dw, Dg, rho) and
+ * UpdateTumor(dt) (Glioma/Glioma_ReactionDiffusionOperator.h:20,232), in
+ * unit-cube units: Dw and Dg are already divided by L^2
+ * (Glioma/Glioma_HG_UQ.cpp:532-533). */
+typedef struct {
+    double Dw, Dg, rho, dt;
+} gq_fk_params;
+
+enum { GQ_FIELD_PHI = 0, GQ_FIELD_PWM = 1, GQ_FIELD_PGM = 2 };
+
+/* ---- context ---------------------------------------------------------- */
+gq_status gq_ctx_create(gq_ctx **out, int device);
+gq_status gq_ctx_destroy(gq_ctx *ctx);
+const char *gq_status_string(gq_status s);
+/* Message of the last failing call on this context ("" if none). */
+const char *gq_last_error(const gq_ctx *ctx);
+
+/* ---- grid: replaces Grid<W,B>(bpd,bpd,bpd,maxStencil) with
+ * Refiner_SpaceExtension(resJump=1, maxLevel) and Compressor(1)
+ * (Glioma/Glioma_HG_UQ.cpp:28-34, MRAGcore/MRAGrid.inl:57-111).
+ * blocks_per_dim must be a power of two; wavelet_order is 2
+ * (Wavelets_Interp2ndOrder, the reference default, MRAGHeaders.h:14) or 4
+ * (Wavelets_Interp4thOrder).  Block size is 16^3. */
+gq_status gq_grid_create(gq_ctx *ctx, int blocks_per_dim, int max_level, int wavelet_order);
+/* Number of leaf blocks (Grid::getBlocksInfo().size()). */
+gq_status gq_grid_block_count(gq_ctx *ctx, int *n);
+/* Grid::getBlocksInfo (MRAGcore/MRAGrid.inl:191-219).  The order returned
+ * here is the order of every per-block array of this API. */
+gq_status gq_grid_blocks_info(gq_ctx *ctx, gq_block_info *out, int capacity, int *n);
+/* Block data transfer: n_blocks x 4096 floats, block order of
+ * gq_grid_blocks_info, x fastest inside a block (Block::operator()(ix,iy,iz)). */
+gq_status gq_grid_write_field(gq_ctx *ctx, int field, const float *data, int n_blocks);
+gq_status gq_grid_read_field(gq_ctx *ctx, int field, float *data, int n_blocks);
+
+/* ---- forward solve: replaces Glioma_HG_UQ::_reactionDiffusionStep
+ * (Glioma/Glioma_HG_UQ.cpp:266-277) = BlockProcessing::pipeline_process(vInfo,
+ * collection, boundaryInfo, Glioma_ReactionDiffusionOperator) followed by
+ * BlockProcessing::process(vInfo, collection, UpdateTumor)
+ * (MRAGmultithreading/MRAGBlockProcessing_TBB.h:476-512, :249-264), fused,
+ * n_steps times with a global dt. */
+gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps);
+
+/* ---- adaptation ---------------------------------------------------------
+ * BlockFWT detail per block, FWTReport::getOverAll_DetailMaxMag with the
+ * phi projector (MRAGcore/MRAGBlockFWT.h:49-192,220-298;
+ * Glioma/Glioma_Types.h:405-413).  FP64; out has one double per block in
+ * gq_grid_blocks_info order. */
+gq_status gq_fwt_details(gq_ctx *ctx, double *out, int n_blocks);
+/* Grid::refine(set<int>) / Grid::compress(set<int>, int&) on explicit block
+ * ids (MRAGcore/MRAGrid.inl:248-355, :384-486), with the split/collapse data
+ * transfer of BlockSplitter/BlockCollapser (MRAGBlockSplitter.h:24-99,
+ * MRAGBlockCollapser.h:24-85).  n_out = parents split / blocks collapsed. */
+gq_status gq_refine_blocks(gq_ctx *ctx, const int32_t *block_ids, int n, int *n_out);
+gq_status gq_compress_blocks(gq_ctx *ctx, const int32_t *block_ids, int n, int *n_out);
+/* Science::AutomaticRefinement<0,0>(grid, fwt, tol, max_level, max_loops)
+ * and Science::AutomaticCompression<0,0>(grid, fwt, tol, max_loops)
+ * (MRAGscience/MRAGScienceCore.h:42-108, :117-177). */
+gq_status gq_auto_refine(gq_ctx *ctx, double tol, int max_level, int max_loops, int *n_out);
+gq_status gq_auto_compress(gq_ctx *ctx, double tol, int max_loops, int *n_out);
+
+/* ---- output for the likelihood: Glioma_HG_UQ::_dumpUQoutput
+ * (Glioma/Glioma_HG_UQ.cpp:309-372): phi on the uniform gpd^3 grid,
+ * gpd = blocks_per_dim*16, piecewise-constant replication.  out: gpd^3
+ * doubles, x fastest, to be zero-initialised by the caller (voxels the
+ * reference leaves unwritten keep their value). */
+gq_status gq_dump_uniform(gq_ctx *ctx, double *out, size_t n_out);
+
+/* ---- likelihood -----------------------------------------------------------
+ * Reference model: HGG_Likelihood::run (Likelihood/source/HGG_Likelihood.cpp:
+ * 40-189).  Observations on the gpd^3 grid (x fastest) and, for
+ * selectivePoints, the Sphere.dat integer points (npts x 3, xyz per row).
+ * params = {PETsigma2, PETscale, T1uc, T2uc, slope} (LikelihoodInput.txt).
+ * The model field is the current grid, mapped as gq_dump_uniform maps it.
+ * out = {logL, L_pet, L_T1, L_T2}; FP64 accumulation on the device. */
+gq_status gq_obs_ref_upload(gq_ctx *ctx, int gpd, const double *pet, const double *t1, const double *t2,
+                            const int32_t *points_xyz, int npts);
+gq_status gq_loglik_ref(gq_ctx *ctx, const double params[5], int selective, int step_pet, int step_ti,
+                        double out[4]);
+
+/* Flip-rate Bernoulli + Gaussian PET model (no reference counterpart: the
+ * likelihood of the synthetic-observation workloads).  t1/t2: 0/1 masks,
+ * pet: value or NaN where not observed, gpd^3 voxels x fastest.
+ * params = {th_T1, th_FLAIR, flip_rate, pet_sigma}; out = {logL, L_T1,
+ * L_FLAIR, L_PET}. FP64 accumulation on the device. */
+gq_status gq_obs_flip_upload(gq_ctx *ctx, int gpd, const uint8_t *t1, const uint8_t *t2, const float *pet);
+gq_status gq_loglik_flip(gq_ctx *ctx, const double params[4], double out[4]);
+
+/* ---- measurement ----------------------------------------------------------
+ * Device time of the last gq_fk_steps call, summed over its step-kernel
+ * launches (ms), with the number of launches and leaf cells updated. */
+gq_status gq_last_step_timing(gq_ctx *ctx, double *step_kernel_ms, int *launches, int64_t *cell_updates);
+/* per_kernel != 0: bracket every step-kernel launch of gq_fk_steps with a
+ * pair of HIP events on the context stream (default 0: one pair around the
+ * whole call, which then includes the cross-level ghost kernels). */
+gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel);
+/* Device-memory footprint of the context (bytes) and pool capacity (blocks). */
+gq_status gq_memory_info(gq_ctx *ctx, int64_t *bytes, int *pool_blocks);
+
+/* ---- host-only topology (no device): the Grid's refine/compress topology
+ * and ghost-point expansion without block data, for callers that plan on
+ * the host and for checking the planner without a GPU.  Leaves are reported
+ * in the same (Morton) order as gq_grid_blocks_info. */
+typedef struct gq_topo gq_topo;
+gq_status gq_topo_create(gq_topo **out, int blocks_per_dim, int max_level, int wavelet_order);
+gq_status gq_topo_destroy(gq_topo *t);
+/* level[n], index[3n], block_id[n] in leaf order; n = number of leaves. */
+gq_status gq_topo_leaves(gq_topo *t, int32_t *level, int32_t *index, int32_t *block_id, int capacity, int *n);
+/* Grid::refine / Grid::compress topology update for explicit block ids. */
+gq_status gq_topo_refine(gq_topo *t, const int32_t *block_ids, int n, int *n_out);
+gq_status gq_topo_compress(gq_topo *t, const int32_t *block_ids, int n, int *n_out);
+/* Ghost value of leaf `leaf` (leaf order) at local coordinates loc (may lie
+ * outside [0,16)): merged (leaf, cell, weight) terms in canonical order. */
+gq_status gq_topo_point_terms(gq_topo *t, int leaf, const int32_t loc[3], int capacity, int32_t *out_leaf,
+                              int32_t *out_cell, double *out_w, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GQ_API_H */

@@ -1,0 +1,186 @@
+"""ctypes bindings of the CPU oracle (fk_oracle.c) -- TEST INFRASTRUCTURE ONLY.
+
+Every function restates a reference function (file:line cited in fk_oracle.c).
+Loaded from oracle/liboracle.so, built by `make -C oracle`.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(path)
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        u8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        L.fk_block_f64.argtypes = [dp, dp, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.fk_block_f32ref.argtypes = [fp, fp, fp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp]
+        L.fk_update_f64.argtypes = [dp, dp, C.c_long, C.c_double]
+        L.fk_update_f32ref.argtypes = [fp, fp, C.c_long, C.c_double]
+        L.fk_uniform_steps_f64.argtypes = [dp, dp, dp, C.c_long, C.c_int, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, C.c_int, C.c_int]
+        L.fk_uniform_steps_f32ref.argtypes = [fp, fp, fp, C.c_long, C.c_int, C.c_float, C.c_float, C.c_float,
+                                              C.c_float, C.c_double, C.c_int, C.c_int]
+        L.og_point_terms.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, ip, C.c_int, C.c_int, ip, C.c_int,
+                                     ip, ip, dp]
+        L.og_fill_lab_f64.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, ip, C.c_int, dp, C.c_int, C.c_int,
+                                      C.c_int, dp]
+        L.fk_adaptive_steps_f64.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, ip, C.c_int, dp, dp, dp,
+                                            C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.fwt_block_detail_f64.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.fwt_block_detail_f64.restype = C.c_double
+        L.dump_uniform_f64.argtypes = [C.c_int, C.c_int, ip, ip, dp, C.c_int, dp]
+        L.lik_ref_f64.argtypes = [dp, dp, dp, dp, C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int,
+                                  dp, C.c_int, C.POINTER(C.c_longdouble)]
+        L.lik_ref_f64.restype = C.c_longdouble
+        L.lik_flip_f64.argtypes = [dp, u8, u8, fp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.lik_flip_f64.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def fk_block(lab_phi, lab_pw, lab_pg, h, Dw, Dg, rho, mode="f64"):
+    """RD functor on one ghosted lab (halo 1). Returns dphidt (B^3)."""
+    L = lab_phi.shape[0]
+    B = L - 2
+    if mode == "f64":
+        a = [np.ascontiguousarray(x, np.float64) for x in (lab_phi, lab_pw, lab_pg)]
+        out = np.zeros((B, B, B), np.float64)
+        lib().fk_block_f64(a[0], a[1], a[2], B, h, Dw, Dg, rho, out)
+    else:
+        a = [np.ascontiguousarray(x, np.float32) for x in (lab_phi, lab_pw, lab_pg)]
+        out = np.zeros((B, B, B), np.float32)
+        lib().fk_block_f32ref(a[0], a[1], a[2], B, h, Dw, Dg, rho, out)
+    return out
+
+
+def fk_update(phi, dphidt, dt, mode="f64"):
+    if mode == "f64":
+        p = np.ascontiguousarray(phi, np.float64).copy()
+        lib().fk_update_f64(p, np.ascontiguousarray(dphidt, np.float64), p.size, dt)
+    else:
+        p = np.ascontiguousarray(phi, np.float32).copy()
+        lib().fk_update_f32ref(p, np.ascontiguousarray(dphidt, np.float32), p.size, dt)
+    return p
+
+
+def fk_uniform_steps(phi, pw, pg, B, h, Dw, Dg, rho, dt, n_steps, mode="f64", threads=0):
+    """n_steps of the uniform periodic block grid; volumes (N,N,N) z,y,x order."""
+    N = phi.shape[0]
+    if mode == "f64":
+        p = np.ascontiguousarray(phi, np.float64).copy()
+        rc = lib().fk_uniform_steps_f64(p, np.ascontiguousarray(pw, np.float64), np.ascontiguousarray(pg, np.float64),
+                                        N, B, h, Dw, Dg, rho, dt, n_steps, threads)
+    else:
+        p = np.ascontiguousarray(phi, np.float32).copy()
+        rc = lib().fk_uniform_steps_f32ref(p, np.ascontiguousarray(pw, np.float32),
+                                           np.ascontiguousarray(pg, np.float32), N, B, h, Dw, Dg, rho, dt,
+                                           n_steps, threads)
+    if rc != 0:
+        raise RuntimeError("oracle uniform steps failed")
+    return p
+
+
+class LeafGrid:
+    """Oracle-side description of a leaf-block hierarchy (levels, indices)."""
+
+    def __init__(self, levels, indices, B, max_level, order):
+        self.level = np.ascontiguousarray(levels, np.int32)
+        self.index = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+        self.B = B
+        self.max_level = max_level
+        self.order = order
+        nF = 1 << max_level
+        lut = -np.ones((nF, nF, nF), np.int32)  # [z][y][x]
+        for i, (l, ix) in enumerate(zip(self.level, self.index)):
+            s = 1 << (max_level - l)
+            x, y, z = ix * s
+            if (lut[z:z + s, y:y + s, x:x + s] >= 0).any():
+                raise ValueError("overlapping leaves")
+            lut[z:z + s, y:y + s, x:x + s] = i
+        if (lut < 0).any():
+            raise ValueError("leaves do not cover the domain")
+        self.lut = np.ascontiguousarray(lut.reshape(-1))
+
+    @property
+    def n(self):
+        return len(self.level)
+
+    def point_terms(self, leaf, loc, cap=4096):
+        ol = np.zeros(cap, np.int32)
+        oc = np.zeros(cap, np.int32)
+        ow = np.zeros(cap, np.float64)
+        m = lib().og_point_terms(self.n, self.max_level, self.B, self.level, self.index.reshape(-1), self.lut,
+                                 self.order, leaf, np.asarray(loc, np.int32), cap, ol, oc, ow)
+        if m < 0:
+            raise RuntimeError("ghost expansion overflow")
+        return ol[:m].copy(), oc[:m].copy(), ow[:m].copy()
+
+    def fill_lab(self, data, leaf, s0, s1):
+        L = self.B + s1 - s0 - 1
+        lab = np.zeros((L, L, L), np.float64)
+        rc = lib().og_fill_lab_f64(self.n, self.max_level, self.B, self.level, self.index.reshape(-1), self.lut,
+                                   self.order, np.ascontiguousarray(data, np.float64), leaf, s0, s1, lab)
+        if rc != 0:
+            raise RuntimeError("ghost expansion overflow")
+        return lab
+
+    def fk_steps(self, phi, pw, pg, Dw, Dg, rho, dt, n_steps, threads=0):
+        p = np.ascontiguousarray(phi, np.float64).copy()
+        rc = lib().fk_adaptive_steps_f64(self.n, self.max_level, self.B, self.level, self.index.reshape(-1),
+                                         self.lut, self.order, p, np.ascontiguousarray(pw, np.float64),
+                                         np.ascontiguousarray(pg, np.float64), Dw, Dg, rho, dt, n_steps, threads)
+        if rc != 0:
+            raise RuntimeError("ghost expansion overflow")
+        return p
+
+    def fwt_details(self, phi):
+        """Per-leaf detail magnitude (FWTReport::getOverAll_DetailMaxMag)."""
+        s0, s1 = (-2, 4) if self.order == 4 else (-1, 2)
+        out = np.zeros(self.n)
+        L = self.B + s1 - s0 - 1
+        for b in range(self.n):
+            lab = self.fill_lab(phi, b, s0, s1)
+            out[b] = lib().fwt_block_detail_f64(lab, self.B, s0, L, self.order)
+        return out
+
+    def dump_uniform(self, phi, gpd):
+        out = np.zeros(gpd ** 3, np.float64)
+        lib().dump_uniform_f64(self.n, self.B, self.level, self.index.reshape(-1),
+                               np.ascontiguousarray(phi, np.float64), gpd, out)
+        return out.reshape(gpd, gpd, gpd)
+
+
+def lik_ref(model, pet, t1, t2, params, selective=True, step_pet=3, step_ti=2, points=None):
+    """HGG_Likelihood::run (returns log L and its three parts)."""
+    nz, ny, nx = model.shape
+    if points is None:
+        points = np.zeros((3, 1))
+        npts = 0
+    else:
+        npts = points.shape[1] if points.ndim == 2 else 0
+    pts = np.ascontiguousarray(points, np.float64)
+    parts = (C.c_longdouble * 3)()
+    v = lib().lik_ref_f64(*(np.ascontiguousarray(a, np.float64) for a in (model, pet, t1, t2)), nx, ny, nz,
+                          np.ascontiguousarray(params, np.float64), int(selective), step_pet, step_ti, pts, npts,
+                          parts)
+    return float(v), [float(parts[i]) for i in range(3)]
+
+
+def lik_flip(u, t1, t2, pet, th1=0.7, th2=0.25, eps=0.05, sigma=0.05):
+    parts = np.zeros(3)
+    v = lib().lik_flip_f64(np.ascontiguousarray(u, np.float64).reshape(-1),
+                           np.ascontiguousarray(t1, np.uint8).reshape(-1),
+                           np.ascontiguousarray(t2, np.uint8).reshape(-1),
+                           np.ascontiguousarray(pet, np.float32).reshape(-1), u.size, th1, th2, eps, sigma, parts)
+    return float(v), parts

@@ -1,0 +1,130 @@
+"""GPU parity of the FK forward solve against the FP64 CPU oracle.
+
+Bar (FP32 field vs the reference's FP64 arithmetic restated by the oracle):
+relative L2 <= 1e-5 and max-abs <= 1e-5 on the field; the cells zeroed by
+the reference's CSF quirk (0/0 -> NaN -> clamp to 0) are zero in both.
+"""
+import numpy as np
+import pytest
+
+import gliomauq_amd as gq
+from gliomauq_amd import anatomy as A
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L_CM = 17.0
+
+
+def rel_l2(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-300))
+
+
+def zero_mismatch(a, b):
+    return int(np.count_nonzero(((a == 0) != (b == 0)) & (np.maximum(a, b) > 1e-30)))
+
+
+def uniform_case(n, seed=0):
+    pwm, pgm, u0, c = A.config_inputs(n, L_CM, seed)
+    return pwm, pgm, u0
+
+
+@pytest.mark.parametrize("n,steps,order", [(64, 60, 2), (128, 40, 4)])
+def test_uniform_grid_parity(ctx, n, steps, order):
+    pwm, pgm, u0 = uniform_case(n, seed=1)
+    # a larger initial tumour so the front crosses many cells
+    u0 = np.maximum(u0, A.initial_condition(n, (n // 2, n // 2, n // 2 + 4), 0.6, 3.0))
+    bpd = n // 16
+    lvl = int(np.log2(bpd))
+    g = gq.Grid(ctx, bpd, lvl, order)
+    for name, v in (("phi", u0), ("pw", pwm), ("pg", pgm)):
+        g.sample_volume(name, v)
+    Dw = 0.0013 / L_CM ** 2 * 40  # faster diffusion so 40-60 steps move the front
+    Dg, rho = 0.1 * Dw, 0.025 * 20
+    h = 1.0 / n
+    dt = 0.9 * h * h / (6 * Dw)
+    gq.fk_steps(g, Dw, Dg, rho, dt, steps)
+    u_gpu = g.dump_uniform()
+    u_ref = O.fk_uniform_steps(u0.astype(np.float64), pwm.astype(np.float64), pgm.astype(np.float64), 16, h, Dw,
+                               Dg, rho, dt, steps)
+    e2 = rel_l2(u_gpu, u_ref)
+    einf = float(np.max(np.abs(u_gpu - u_ref)))
+    print("uniform n=%d steps=%d: rel L2 %.3e, Linf %.3e" % (n, steps, e2, einf))
+    assert e2 <= 1e-5 and einf <= 1e-5
+    assert zero_mismatch(u_gpu, u_ref) == 0
+    assert (u_ref > 0).sum() > 1000
+
+
+def test_pure_reaction_matches_recurrence(ctx):
+    n = 32
+    rng = np.random.default_rng(2)
+    phi = (0.5 * rng.random((n, n, n))).astype(np.float32)
+    pw = rng.random((n, n, n)).astype(np.float32)
+    pg = (1 - pw).astype(np.float32)
+    g = gq.Grid(ctx, 2, 1, 2)
+    for name, v in (("phi", phi), ("pw", pw), ("pg", pg)):
+        g.sample_volume(name, v)
+    dt, rho = 0.8, 0.05
+    gq.fk_steps(g, 0.0, 0.0, rho, dt, 10)
+    u = g.dump_uniform()
+    e = phi.astype(np.float64)
+    for _ in range(10):
+        e = np.clip(e + dt * rho * e * (1 - e), 0, 1)
+    assert np.max(np.abs(u - e)) < 2e-6
+
+
+def test_csf_quirk_cells_are_zero(ctx):
+    n = 32
+    phi = np.full((n, n, n), 0.3, np.float32)
+    pw = np.full((n, n, n), 0.7, np.float32)
+    pg = np.full((n, n, n), 0.3, np.float32)
+    pw[5, 6, 7] = pg[5, 6, 7] = 0
+    pw[16, 15, 0] = pg[16, 15, 0] = 0  # on a block face (x = 0 wraps to x = 31)
+    g = gq.Grid(ctx, 2, 1, 2)
+    for name, v in (("phi", phi), ("pw", pw), ("pg", pg)):
+        g.sample_volume(name, v)
+    gq.fk_steps(g, 1e-6, 1e-7, 0.02, 1.0, 1)
+    u = g.dump_uniform()
+    zeros = set(map(tuple, np.argwhere(u == 0)))
+    exp = set()
+    for c in [(5, 6, 7), (16, 15, 0)]:
+        exp.add(c)
+        for d in [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]:
+            exp.add(tuple((np.array(c) + d) % n))
+    assert zeros == exp
+
+
+def test_fk_steps_rejects_bad_parameters(ctx):
+    g = gq.Grid(ctx, 2, 1, 2)
+    with pytest.raises(gq._lib.GQError) as e:
+        gq.fk_steps(g, float("nan"), 0.0, 0.1, 0.5, 1)
+    assert e.value.status == 7
+    with pytest.raises(gq._lib.GQError) as e:
+        gq.fk_steps(g, 1e-6, 1e-7, 0.1, -1.0, 1)
+    assert e.value.status == 7
+    with pytest.raises(gq._lib.GQError) as e:
+        g.write("phi", np.zeros((3, 16, 16, 16), np.float32))
+    assert "expected" in str(e.value) or e.value.status == 1
+
+
+def test_block_processing_mirror(ctx):
+    """pipeline_process(RD) + process(UpdateTumor) == one fused step."""
+    n = 32
+    rng = np.random.default_rng(3)
+    phi = (0.4 * rng.random((n, n, n))).astype(np.float32)
+    pw = rng.random((n, n, n)).astype(np.float32)
+    pg = (1 - pw).astype(np.float32)
+    outs = []
+    for mode in range(2):
+        g = gq.Grid(ctx, 2, 1, 2)
+        for name, v in (("phi", phi), ("pw", pw), ("pg", pg)):
+            g.sample_volume(name, v)
+        if mode == 0:
+            bp = gq.BlockProcessing()
+            vinfo = g.getBlocksInfo()
+            bp.pipeline_process(vinfo, g, None, gq.Glioma_ReactionDiffusionOperator(1e-5, 1e-6, 0.03))
+            bp.process(vinfo, g, gq.UpdateTumor(0.5))
+        else:
+            gq.fk_steps(g, 1e-5, 1e-6, 0.03, 0.5, 1)
+        outs.append(g.dump_uniform())
+    assert np.array_equal(outs[0], outs[1])
