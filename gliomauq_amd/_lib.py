@@ -20,7 +20,7 @@ EXPORTS = [
     "gq_ctx_create", "gq_ctx_destroy", "gq_status_string", "gq_last_error", "gq_grid_create",
     "gq_grid_block_count", "gq_grid_blocks_info", "gq_grid_write_field", "gq_grid_read_field", "gq_fk_steps",
     "gq_fwt_details", "gq_refine_blocks", "gq_compress_blocks", "gq_auto_refine", "gq_auto_compress",
-    "gq_dump_uniform", "gq_obs_ref_upload", "gq_loglik_ref", "gq_obs_flip_upload", "gq_loglik_flip",
+    "gq_dump_uniform", "gq_obs_ref_upload", "gq_loglik_ref", "gq_loglik_ref_volume", "gq_obs_flip_upload", "gq_loglik_flip",
     "gq_last_step_timing", "gq_set_step_timing", "gq_memory_info", "gq_topo_create", "gq_topo_destroy",
     "gq_topo_leaves", "gq_topo_refine", "gq_topo_compress", "gq_topo_point_terms",
 ]
@@ -75,6 +75,7 @@ def load():
         "gq_dump_uniform": [vp, f64p, C.c_size_t],
         "gq_obs_ref_upload": [vp, C.c_int, f64p, f64p, f64p, i32p, C.c_int],
         "gq_loglik_ref": [vp, f64p, C.c_int, C.c_int, C.c_int, f64p],
+        "gq_loglik_ref_volume": [vp, f64p, C.c_int, f64p, C.c_int, C.c_int, C.c_int, f64p],
         "gq_obs_flip_upload": [vp, C.c_int, u8p, u8p, f32p],
         "gq_loglik_flip": [vp, f64p, f64p],
         "gq_last_step_timing": [vp, f64p, ip, C.POINTER(C.c_int64)],

@@ -346,6 +346,7 @@ struct gq_ctx {
             throw gq_error(GQ_ERR_UNSUPPORTED,
                            "uniform view needs max_level == log2(blocks_per_dim) (no leaf finer than the dump grid)");
         gq::UniformArgs u;
+        u.vol = nullptr;
         u.phi = phi[cur];
         u.lut_slot = d_lut.p;
         u.level = d_level.p;
@@ -785,6 +786,44 @@ gq_status gq_loglik_ref(gq_ctx *ctx, const double params[5], int selective, int 
         HIPCHK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, ctx->st));
         HIPCHK(hipStreamSynchronize(ctx->st));
         // HGG_Likelihood.cpp:70-71
+        const int N = (int)h[1];
+        const double p1 = -0.5 * N * std::log(2. * M_PI * sigma2);
+        const double p2 = -0.5 * (1. / sigma2) * h[0];
+        out[1] = p1 + p2;
+        out[2] = h[2];
+        out[3] = h[3];
+        out[0] = out[1] + out[2] + out[3];
+    });
+}
+
+gq_status gq_loglik_ref_volume(gq_ctx *ctx, const double *model, int gpd, const double params[5], int selective,
+                               int step_pet, int step_ti, double out[4])
+{
+    return guard(ctx, [&] {
+        if (!model || gpd <= 0 || !params || !out || step_pet <= 0 || step_ti <= 0)
+            throw gq_error(GQ_ERR_ARG, "bad arguments");
+        if (ctx->obs_ref_gpd != gpd) throw gq_error(GQ_ERR_STATE, "observations missing or of another size");
+        const size_t n = (size_t)gpd * gpd * gpd;
+        ctx->d_dbl.reserve(n);
+        HIPCHK(hipMemcpyAsync(ctx->d_dbl.p, model, n * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+        gq::UniformArgs u;
+        u.vol = ctx->d_dbl.p;
+        u.phi = nullptr;
+        u.lut_slot = nullptr;
+        u.level = nullptr;
+        u.index = nullptr;
+        u.gpd = gpd;
+        u.hole = 0.0;
+        const double sigma2 = params[0], scale = params[1], t1uc = params[2], t2uc = params[3], slope = params[4];
+        const int nb = gq::reduce_blocks((int64_t)n);
+        ctx->d_partial.reserve((size_t)nb * 4 + 8);
+        double *res = ctx->d_partial.p + (size_t)nb * 4;
+        HIPCHK(gq::launch_lik_ref(u, ctx->d_pet_ref.p, ctx->d_t1_ref.p, ctx->d_t2_ref.p, ctx->d_pts.p,
+                                  ctx->obs_ref_npts, selective ? 1 : 0, step_pet, step_ti, scale, t1uc, t2uc, slope,
+                                  ctx->d_partial.p, nb, res, ctx->st));
+        double h[4];
+        HIPCHK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
         const int N = (int)h[1];
         const double p1 = -0.5 * N * std::log(2. * M_PI * sigma2);
         const double p2 = -0.5 * (1. / sigma2) * h[0];

@@ -385,6 +385,10 @@ __global__ __launch_bounds__(256) void k_collapse(float *phi, float *pw, float *
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool voxel_value(const UniformArgs &u, int64_t v, double &val)
 {
+    if (u.vol) {
+        val = u.vol[v];
+        return true;
+    }
     const int gpd = u.gpd;
     const int vx = (int)(v % gpd), vy = (int)((v / gpd) % gpd), vz = (int)(v / ((int64_t)gpd * gpd));
     const int bpd = gpd >> 4;

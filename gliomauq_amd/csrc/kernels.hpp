@@ -32,6 +32,7 @@ struct LabArgs {
 };
 
 struct UniformArgs {
+    const double *vol;  // if set: a uniform gpd^3 model volume (x fastest) replaces the grid view
     const float *phi;
     const int *lut_slot;
     const int *level;

@@ -123,6 +123,12 @@ gq_status gq_obs_ref_upload(gq_ctx *ctx, int gpd, const double *pet, const doubl
 gq_status gq_loglik_ref(gq_ctx *ctx, const double params[5], int selective, int step_pet, int step_ti,
                         double out[4]);
 
+/* The same reference model on an explicit model volume (the HGG_data.dat
+ * contract of the reference's likelihood program, Likelihood_main.cpp:18-36):
+ * model is gpd^3 doubles, x fastest; observations from gq_obs_ref_upload. */
+gq_status gq_loglik_ref_volume(gq_ctx *ctx, const double *model, int gpd, const double params[5], int selective,
+                               int step_pet, int step_ti, double out[4]);
+
 /* Flip-rate Bernoulli + Gaussian PET model (no reference counterpart: the
  * likelihood of the synthetic-observation workloads).  t1/t2: 0/1 masks,
  * pet: value or NaN where not observed, gpd^3 voxels x fastest.

@@ -715,14 +715,16 @@ long double lik_ref_f64(const double *model, const double *pet, const double *t1
  *   PET:   where an observation exists (pet is not NaN; the generator writes
  *          values only where the ground truth had u > th2):
  *          -0.5*log(2*pi*sigma^2) - (pet - u)^2 / (2*sigma^2)
- * Accumulation in double, voxel order (x fastest).  Returns log L. */
+ * Accumulation in long double, voxel order (x fastest), so that the
+ * oracle's own rounding stays far below the device's FP64 tree sums.
+ * Returns log L. */
 double lik_flip_f64(const double *u, const unsigned char *t1, const unsigned char *t2, const float *pet,
                     long n, double th1, double th2, double eps, double sigma, double *parts)
 {
     const double lp1 = log(1. - eps), lp0 = log(eps);
     const double c0 = -0.5 * log(2. * M_PI * sigma * sigma);
     const double is2 = 1. / (2. * sigma * sigma);
-    double s1 = 0, s2 = 0, sp = 0;
+    long double s1 = 0, s2 = 0, sp = 0;
     for (long i = 0; i < n; i++) {
         const int m1 = u[i] > th1, m2 = u[i] > th2;
         s1 += (m1 == (int)t1[i]) ? lp1 : lp0;
@@ -732,6 +734,6 @@ double lik_flip_f64(const double *u, const unsigned char *t1, const unsigned cha
             sp += c0 - r * r * is2;
         }
     }
-    if (parts) { parts[0] = s1; parts[1] = s2; parts[2] = sp; }
-    return s1 + s2 + sp;
+    if (parts) { parts[0] = (double)s1; parts[1] = (double)s2; parts[2] = (double)sp; }
+    return (double)(s1 + s2 + sp);
 }
