@@ -37,7 +37,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, n_steps=2):
+def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, n_steps=200):
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)
     n = u0.shape[0]
@@ -120,20 +120,30 @@ def main():
         log("[bench] step %d/%d blocks %d" % (done, args.steps, g.n_blocks()))
 
     log("[bench] timed: %d steps" % args.steps)
+    hs0 = gq.host_stats(ctx)
+    t_steps = 0.0
+    t_adapt = 0.0
     t0 = time.perf_counter()
     # the same schedule as Glioma_HG_UQ.run, stopped after args.steps steps
     done = 0
     while done < args.steps:
         m = min(args.adapt_every, args.steps - done)
+        ta = time.perf_counter()
         gq.fk_steps(g, Dw, Dg, rho, dt, m)
+        tb = time.perf_counter()
+        t_steps += tb - ta
         done += m
         after_chunk(done)
         if m == args.adapt_every:
+            tc = time.perf_counter()
             s._adapt()
+            t_adapt += time.perf_counter() - tc
     gq.Science.AutomaticRefinement(g, tol_r, max_level, 1)
     ll = lik.loglik()
     wall = time.perf_counter() - t0
 
+    hs1 = gq.host_stats(ctx)
+    zt, tune = gq.autotune_info(ctx)
     cells = stats["cells"]
     value = cells / wall
     kernel_s = stats["kernel_ms"] * 1e-3
@@ -159,6 +169,8 @@ def main():
                      "kernel": "k_fk_step", "kernel_avg_us": stats["kernel_ms"] * 1e3 / max(stats["launches"], 1),
                      "algorithmic_bytes_per_cell": 16},
     }
+    res["breakdown"] = {"fk_steps_s": t_steps, "adapt_s": t_adapt, "step_kernel_s": kernel_s,
+                        "host": {k: hs1[k] - hs0[k] for k in hs1}, "slab_depth": zt, "autotune_us": tune}
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         res["cpu_baseline"] = cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt)

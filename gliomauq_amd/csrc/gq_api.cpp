@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -59,6 +61,18 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct ScopeTimer {
+    double &acc;
+    double t0;
+    explicit ScopeTimer(double &a) : acc(a), t0(now_s()) {}
+    ~ScopeTimer() { acc += now_s() - t0; }
+};
+
 int hw_threads()
 {
     unsigned n = std::thread::hardware_concurrency();
@@ -104,6 +118,16 @@ struct gq_ctx {
     int obs_ref_gpd = 0, obs_ref_npts = 0;
     DevBuf<double> d_pet_ref, d_t1_ref, d_t2_ref;
     DevBuf<int> d_pts;
+
+    // step tile (slab depth) chosen per grid-size bucket
+    bool autotune = true;
+    std::map<int, int> zt_cache;
+    int last_zt = 16;
+    double tune_us[4] = {0, 0, 0, 0};
+
+    // host-side time accounting (seconds) and counts
+    double t_tables = 0, t_plan = 0, t_fwt = 0, t_split = 0, t_collapse = 0, t_tune = 0;
+    int n_tables = 0, n_fwt = 0;
 
     // timing
     bool per_kernel_timing = false;
@@ -188,6 +212,8 @@ struct gq_ctx {
     {
         need_grid();
         if (tables_valid) return;
+        ScopeTimer tm(t_tables);
+        n_tables++;
         gq::Tables &t = tabs;
         gq::build_tables(*grid, cap, t, hw_threads());
         api_order = gq::morton_order(*grid);
@@ -259,6 +285,8 @@ struct gq_ctx {
         for (size_t k = 0; k < leaf_idx.size(); k++) slots[k] = L[leaf_idx[k]].slot;
         std::vector<double> out(leaf_idx.size());
         if (slots.empty()) return out;
+        ScopeTimer tm(t_fwt);
+        n_fwt++;
         d_list.upload(slots, st);
         d_dbl.reserve(slots.size());
         HIPCHK(gq::launch_fwt_detail(grid->order(), lab_args(), phi[cur], d_list.p, (int)slots.size(), d_dbl.p, st));
@@ -270,9 +298,14 @@ struct gq_ctx {
     int refine_ids(const std::unordered_set<int> &ids)
     {
         need_grid();
-        std::vector<int> parents = grid->plan_refine(ids);
+        std::vector<int> parents;
+        {
+            ScopeTimer tm(t_plan);
+            parents = grid->plan_refine(ids);
+        }
         if (parents.empty()) return 0;
         ensure_tables();  // parent labs use the pre-refine topology
+        ScopeTimer tm(t_split);
         grow((int)(grid->leaves().size() + parents.size() * 8));
         const auto &L = grid->leaves();
         std::vector<int> pslots, cslots;
@@ -306,8 +339,13 @@ struct gq_ctx {
     int compress_ids(const std::unordered_set<int> &ids)
     {
         need_grid();
-        std::vector<std::array<int, 4>> plan = grid->plan_compress(ids);
+        std::vector<std::array<int, 4>> plan;
+        {
+            ScopeTimer tm(t_plan);
+            plan = grid->plan_compress(ids);
+        }
         if (plan.empty()) return 0;
+        ScopeTimer tm(t_collapse);
         grow((int)(grid->leaves().size() + plan.size()));
         std::vector<int> pslots, cslots, remove;
         std::vector<gq::Leaf> add;
@@ -337,6 +375,58 @@ struct gq_ctx {
         grid->replace_leaves(remove, add);
         tables_valid = false;
         return (int)plan.size();
+    }
+
+    static int zt_heuristic(int nwork)
+    {
+        // enough workgroups to fill 256 CUs several times over
+        if (nwork >= 2048) return 16;
+        if (nwork >= 1024) return 8;
+        if (nwork >= 512) return 4;
+        return 2;
+    }
+
+    // Step tile autotuner: time every slab depth on the current grid (the
+    // outputs go to the scratch ping-pong buffer, the state is unchanged) and
+    // keep the fastest per power-of-two bucket of the block count.
+    int choose_zt(gq::StepArgs a, int nrows)
+    {
+        const int nwork = a.nwork;
+        int bucket = 0;
+        while ((1 << bucket) < nwork) bucket++;
+        auto it = zt_cache.find(bucket);
+        if (it != zt_cache.end()) return it->second;
+        if (!autotune) return zt_heuristic(nwork);
+        ScopeTimer tm(t_tune);
+        static const int cand[4] = {16, 8, 4, 2};
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        a.phi_in = phi[cur];
+        a.phi_out = phi[cur ^ 1];
+        if (nrows > 0)
+            HIPCHK(gq::launch_face_ghosts(phi[cur], d_face_ptr.p, d_face_src.p, d_face_w.p, d_gphi.p, nrows, st));
+        int best = 16;
+        float best_ms = 1e30f;
+        for (int k = 0; k < 4; k++) {
+            for (int r = 0; r < 2; r++) HIPCHK(gq::launch_fk_step(a, cand[k], st));
+            HIPCHK(hipEventRecord(e0, st));
+            const int reps = 5;
+            for (int r = 0; r < reps; r++) HIPCHK(gq::launch_fk_step(a, cand[k], st));
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            tune_us[k] = 1e3 * ms / reps;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = cand[k];
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        zt_cache[bucket] = best;
+        return best;
     }
 
     gq::UniformArgs uniform_args()
@@ -555,6 +645,8 @@ gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
         a.Dg = Dg;
         a.rho = rho;
         a.dt = (float)p->dt;
+        const int zt = ctx->choose_zt(a, nrows);
+        ctx->last_zt = zt;
         const int chunk = 1024;
         const int need_ev = ctx->per_kernel_timing ? 2 * std::min(n_steps, chunk) : 2;
         while ((int)ctx->events.size() < need_ev) {
@@ -574,7 +666,7 @@ gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
                 a.phi_in = ctx->phi[ctx->cur];
                 a.phi_out = ctx->phi[ctx->cur ^ 1];
                 if (ctx->per_kernel_timing) HIPCHK(hipEventRecord(ctx->events[2 * s], ctx->st));
-                HIPCHK(gq::launch_fk_step(a, ctx->st));
+                HIPCHK(gq::launch_fk_step(a, zt, ctx->st));
                 if (ctx->per_kernel_timing) HIPCHK(hipEventRecord(ctx->events[2 * s + 1], ctx->st));
                 ctx->cur ^= 1;
             }
@@ -840,6 +932,42 @@ gq_status gq_last_step_timing(gq_ctx *ctx, double *ms, int *launches, int64_t *c
         if (ms) *ms = ctx->last_ms;
         if (launches) *launches = ctx->last_launches;
         if (cells) *cells = ctx->last_cells;
+    });
+}
+
+gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt)
+{
+    return guard(ctx, [&] {
+        if (mode == 2 && fixed_zt != 16 && fixed_zt != 8 && fixed_zt != 4 && fixed_zt != 2)
+            throw gq_error(GQ_ERR_ARG, "slab depth must be 16, 8, 4 or 2");
+        ctx->zt_cache.clear();
+        ctx->autotune = mode == 1;
+        if (mode == 2)
+            for (int b = 0; b < 32; b++) ctx->zt_cache[b] = fixed_zt;
+    });
+}
+
+gq_status gq_autotune_info(gq_ctx *ctx, int *zt, double us[4])
+{
+    return guard(ctx, [&] {
+        if (zt) *zt = ctx->last_zt;
+        if (us)
+            for (int k = 0; k < 4; k++) us[k] = ctx->tune_us[k];
+    });
+}
+
+gq_status gq_host_stats(gq_ctx *ctx, double out[8])
+{
+    return guard(ctx, [&] {
+        if (!out) throw gq_error(GQ_ERR_ARG, "null output");
+        out[0] = ctx->t_tables;
+        out[1] = ctx->t_plan;
+        out[2] = ctx->t_fwt;
+        out[3] = ctx->t_split;
+        out[4] = ctx->t_collapse;
+        out[5] = ctx->t_tune;
+        out[6] = ctx->n_tables;
+        out[7] = ctx->n_fwt;
     });
 }
 

@@ -46,18 +46,6 @@ __device__ __forceinline__ int nbr_face_cell(int f, int u, int v)
     }
 }
 
-__device__ __forceinline__ int lab_face_index(int f, int u, int v)
-{
-    switch (f) {
-    case 0: return 0 + LW * ((u + 1) + LW * (v + 1));
-    case 1: return (B + 1) + LW * ((u + 1) + LW * (v + 1));
-    case 2: return (u + 1) + LW * (0 + LW * (v + 1));
-    case 3: return (u + 1) + LW * ((B + 1) + LW * (v + 1));
-    case 4: return (u + 1) + LW * ((v + 1) + LW * 0);
-    default: return (u + 1) + LW * ((v + 1) + LW * (B + 1));
-    }
-}
-
 // 1/K = (pw+pg) / (pw*Dw + pg*Dg): the reciprocal of the tissue-normalised
 // diffusivity of Case2 (Glioma_ReactionDiffusionOperator.h:174-188).  0/0 ->
 // NaN (CSF / outside), as in the reference.
@@ -71,61 +59,103 @@ __device__ __forceinline__ float inv_k(float w, float g, float Dw, float Dg)
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Fused RD operator + UpdateTumor (one workgroup per 16^3 block).
+// Fused RD operator + UpdateTumor.  A workgroup of 256 threads updates a slab
+// of 16 x 16 x ZT cells of one block (ZT = 16: the whole block); each thread
+// owns one (x,y) column of ZT cells.  phi and 1/K of the slab and its 1-cell
+// halo are staged in LDS ((ZT+2) x 18 x 18 x 2 floats).  Halo planes in z come
+// from the same block when inside it, else from the z-neighbour block (same
+// level) or the cross-level ghost rows; x/y halos from the face neighbours.
 // ---------------------------------------------------------------------------
+template <int ZT>
 __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
 {
-    __shared__ float s_phi[LW * LW * LW];
-    __shared__ float s_ik[LW * LW * LW];
-    const int w = xcd_remap(blockIdx.x, gridDim.x);
-    if (w >= a.nwork) return;  // padding workgroups (whole workgroup exits)
+    constexpr int NS = B / ZT;       // slabs per block
+    constexpr int LZ = ZT + 2;       // LDS planes
+    constexpr int PL = LW * LW;      // LDS plane size
+    __shared__ float s_phi[LZ * PL];
+    __shared__ float s_ik[LZ * PL];
+    const int item = xcd_remap(blockIdx.x, gridDim.x);
+    if (item >= a.nwork * NS) return;  // padding workgroups (whole workgroup exits)
+    const int w = item / NS;
+    const int z0 = (item - w * NS) * ZT;
     const int slot = a.work[w];
     const int t = threadIdx.x;
     const int tx = t & 15, ty = t >> 4;
     const size_t base = (size_t)slot * B3;
+    const float Dw = a.Dw, Dg = a.Dg;
 
-    float breg[B];
+    float breg[ZT];
 #pragma unroll
-    for (int z = 0; z < B; z++) {
-        const size_t i = base + z * 256 + t;
+    for (int k = 0; k < ZT; k++) {
+        const size_t i = base + (z0 + k) * 256 + t;
         const float p = a.phi_in[i];
         const float wm = a.pw[i];
         const float gm = a.pg[i];
-        breg[z] = wm + gm;
-        const int l = (tx + 1) + LW * ((ty + 1) + LW * (z + 1));
+        breg[k] = wm + gm;
+        const int l = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
         s_phi[l] = p;
-        s_ik[l] = inv_k(wm, gm, a.Dw, a.Dg);
+        s_ik[l] = inv_k(wm, gm, Dw, Dg);
     }
+    // z halo planes
 #pragma unroll
-    for (int f = 0; f < 6; f++) {
+    for (int side = 0; side < 2; side++) {
+        const int z = side == 0 ? z0 - 1 : z0 + ZT;
+        const int l = (tx + 1) + LW * (ty + 1) + PL * (side == 0 ? 0 : ZT + 1);
+        float pv, kv;
+        if (z >= 0 && z < B) {
+            const size_t i = base + z * 256 + t;
+            pv = a.phi_in[i];
+            kv = inv_k(a.pw[i], a.pg[i], Dw, Dg);
+        } else {
+            const int f = side == 0 ? 4 : 5;
+            const int nb = a.fnbr[slot * 6 + f];
+            if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, tx, ty)];
+            else pv = a.gphi[a.fbase[slot * 6 + f] + t];
+            kv = a.fik[((size_t)slot * 6 + f) * 256 + t];
+        }
+        s_phi[l] = pv;
+        s_ik[l] = kv;
+    }
+    // x / y faces of the slab: 4 faces x 16 x ZT cells
+    for (int q = t; q < 4 * 16 * ZT; q += 256) {
+        const int f = q / (16 * ZT);
+        const int r = q - f * 16 * ZT;
+        const int u = r & 15, v = z0 + (r >> 4);
         const int nb = a.fnbr[slot * 6 + f];
         float pv;
-        if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, tx, ty)];
-        else pv = a.gphi[a.fbase[slot * 6 + f] + t];
-        const int l = lab_face_index(f, tx, ty);
+        if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, u, v)];
+        else pv = a.gphi[a.fbase[slot * 6 + f] + u + 16 * v];
+        const float kv = a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
+        int l;
+        switch (f) {
+        case 0: l = 0 + LW * (u + 1) + PL * (v - z0 + 1); break;
+        case 1: l = (B + 1) + LW * (u + 1) + PL * (v - z0 + 1); break;
+        case 2: l = (u + 1) + 0 + PL * (v - z0 + 1); break;
+        default: l = (u + 1) + LW * (B + 1) + PL * (v - z0 + 1); break;
+        }
         s_phi[l] = pv;
-        s_ik[l] = a.fik[((size_t)slot * 6 + f) * 256 + t];
+        s_ik[l] = kv;
     }
     __syncthreads();
 
     const float ih2 = a.ih2_lvl[a.level[slot]];
     const float dt = a.dt, rho = a.rho;
 #pragma unroll 2
-    for (int z = 0; z < B; z++) {
-        const int c = (tx + 1) + LW * ((ty + 1) + LW * (z + 1));
+    for (int k = 0; k < ZT; k++) {
+        const int c = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
         const float pc = s_phi[c];
         const float pW = s_phi[c - 1], pE = s_phi[c + 1];
         const float pS = s_phi[c - LW], pN = s_phi[c + LW];
-        const float pB = s_phi[c - LW * LW], pF = s_phi[c + LW * LW];
+        const float pB = s_phi[c - PL], pF = s_phi[c + PL];
         float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
-        if (breg[z] + pc > 0.f) {  // tissue = p_w + p_g + phi > 0 (:155-157)
+        if (breg[k] + pc > 0.f) {  // tissue = p_w + p_g + phi > 0 (:155-157)
             const float kc = s_ik[c];
             d0 = 2.f * frcp(kc + s_ik[c - 1]);
             d1 = 2.f * frcp(kc + s_ik[c + 1]);
             d2 = 2.f * frcp(kc + s_ik[c - LW]);
             d3 = 2.f * frcp(kc + s_ik[c + LW]);
-            d4 = 2.f * frcp(kc + s_ik[c - LW * LW]);
-            d5 = 2.f * frcp(kc + s_ik[c + LW * LW]);
+            d4 = 2.f * frcp(kc + s_ik[c - PL]);
+            d5 = 2.f * frcp(kc + s_ik[c + PL]);
         }
         // Neumann doubling, sequential as the reference (:198-203)
         if (d0 == 0.f && pW > 0.f) d1 *= 2.f;
@@ -151,20 +181,24 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
         float pn = pc + dt * dphi;
         pn = (0.f < pn) ? pn : 0.f;  // std::max((Real)0, phi): NaN -> 0
         pn = (pn < 1.f) ? pn : 1.f;  // std::min((Real)1, phi)
-        a.phi_out[base + z * 256 + t] = pn;
+        a.phi_out[base + (z0 + k) * 256 + t] = pn;
     }
 }
 
-// Cross-level face ghosts of phi for the step: one thread per row.
+// Cross-level face ghosts of phi for the step: four lanes per row, partial
+// sums combined with lane shuffles.
 __global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ phi, const int *__restrict__ ptr,
                                                      const int *__restrict__ src, const float *__restrict__ w,
                                                      float *__restrict__ out, int nrows)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = g >> 2, lane = g & 3;
     float acc = 0.f;
-    for (int k = ptr[r]; k < ptr[r + 1]; k++) acc += w[k] * phi[src[k]];
-    out[r] = acc;
+    if (r < nrows)
+        for (int k = ptr[r] + lane; k < ptr[r + 1]; k += 4) acc += w[k] * phi[src[k]];
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    if (r < nrows && lane == 0) out[r] = acc;
 }
 
 // 1/K of every face-halo cell (static between topology / parameter changes).
@@ -244,17 +278,56 @@ __device__ void fill_lab(double *lab, const float *__restrict__ fld, int slot, c
     }
 }
 
-__device__ __forceinline__ double ga_w(int order, int i)
+// Non-zero taps of the FWT filters in increasing i (the reference loops over
+// the full supports; a zero-weight tap adds an exact zero to a finite sum, so
+// skipping it leaves every bit of the result unchanged).
+template <int ORDER, bool G>
+struct Taps;
+template <int ORDER>
+struct Taps<ORDER, false> {  // Ha = {1} on [0,1)
+    static constexpr int n = 1;
+    __device__ static constexpr int i(int) { return 0; }
+    __device__ static constexpr double w(int) { return 1.0; }
+};
+template <>
+struct Taps<4, true> {  // Ga on [-4,3): {1/16, 0, -9/16, 1, -9/16, 0, 1/16}
+    static constexpr int n = 5;
+    __device__ static constexpr int i(int k) { return k == 0 ? -4 : (k == 1 ? -2 : (k == 2 ? -1 : (k == 3 ? 0 : 2))); }
+    __device__ static constexpr double w(int k)
+    {
+        return k == 0 ? 1.0 / 16.0 : (k == 1 ? -9.0 / 16.0 : (k == 2 ? 1.0 : (k == 3 ? -9.0 / 16.0 : 1.0 / 16.0)));
+    }
+};
+template <>
+struct Taps<2, true> {  // Ga on [-2,1): {-1/2, 1, -1/2}
+    static constexpr int n = 3;
+    __device__ static constexpr int i(int k) { return k - 2; }
+    __device__ static constexpr double w(int k) { return k == 1 ? 1.0 : -0.5; }
+};
+
+template <int ORDER, int L, int S0, bool GX, bool GY, bool GZ>
+__device__ __forceinline__ double fwt_filter(const double *lab, int dx, int dy, int dz)
 {
-    if (order == 2) {  // GaSupport {-2,1}: {-1/2, 1, -1/2}
-        return i == -1 ? 1.0 : -0.5;
+    typedef Taps<ORDER, GX> TX;
+    typedef Taps<ORDER, GY> TY;
+    typedef Taps<ORDER, GZ> TZ;
+    double r = 0.0;
+#pragma unroll
+    for (int kz = 0; kz < TZ::n; kz++) {
+        const double wZ = TZ::w(kz);
+        const int z = 2 * dz - TZ::i(kz) - S0;
+#pragma unroll
+        for (int ky = 0; ky < TY::n; ky++) {
+            const double wZwY = __dmul_rn(wZ, TY::w(ky));
+            const int y = 2 * dy - TY::i(ky) - S0;
+#pragma unroll
+            for (int kx = 0; kx < TX::n; kx++) {
+                const int x = 2 * dx - TX::i(kx) - S0;
+                r = __dadd_rn(r, __dmul_rn(lab[x + L * (y + L * z)], __dmul_rn(wZwY, TX::w(kx))));
+            }
+        }
     }
-    switch (i) {  // GaSupport {-4,3}: {1/16, 0, -9/16, 1, -9/16, 0, 1/16}
-    case -4: case 2: return 1.0 / 16.0;
-    case -2: case 0: return -9.0 / 16.0;
-    case -1: return 1.0;
-    default: return 0.0;
-    }
+    return r;
 }
 }  // namespace
 
@@ -269,7 +342,6 @@ __global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const float *__re
     constexpr int S0 = ORDER == 4 ? -2 : -1;
     constexpr int S1 = ORDER == 4 ? 4 : 2;
     constexpr int L = LabGeom<S0, S1>::L;
-    constexpr int gs0 = ORDER == 4 ? -4 : -2, gs1 = ORDER == 4 ? 3 : 1;
     __shared__ double lab[LabGeom<S0, S1>::N];
     __shared__ double red[256];
     const int slot = cand[blockIdx.x];
@@ -278,25 +350,17 @@ __global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const float *__re
     double m = 0.0;
     for (int q = threadIdx.x; q < 512; q += blockDim.x) {
         const int dx = q & 7, dy = (q >> 3) & 7, dz = q >> 6;
-        for (int code = 1; code < 8; code++) {
-            const int bx = code & 1, by = (code >> 1) & 1, bz = (code >> 2) & 1;
-            const int sx = bx ? gs0 : 0, ex = bx ? gs1 : 1;
-            const int sy = by ? gs0 : 0, ey = by ? gs1 : 1;
-            const int sz = bz ? gs0 : 0, ez = bz ? gs1 : 1;
-            double r = 0.0;
-            for (int iz = sz; iz < ez; iz++) {
-                const double wZ = bz ? ga_w(ORDER, iz) : 1.0;
-                for (int iy = sy; iy < ey; iy++) {
-                    const double wY = by ? ga_w(ORDER, iy) : 1.0;
-                    const double wZwY = __dmul_rn(wZ, wY);
-                    for (int ix = sx; ix < ex; ix++) {
-                        const double wX = bx ? ga_w(ORDER, ix) : 1.0;
-                        const int x = 2 * dx - ix - S0, y = 2 * dy - iy - S0, z = 2 * dz - iz - S0;
-                        r = __dadd_rn(r, __dmul_rn(lab[x + L * (y + L * z)], __dmul_rn(wZwY, wX)));
-                    }
-                }
-            }
-            const double av = fabs(r);
+        double v[7];
+        v[0] = fwt_filter<ORDER, L, S0, true, false, false>(lab, dx, dy, dz);
+        v[1] = fwt_filter<ORDER, L, S0, false, true, false>(lab, dx, dy, dz);
+        v[2] = fwt_filter<ORDER, L, S0, true, true, false>(lab, dx, dy, dz);
+        v[3] = fwt_filter<ORDER, L, S0, false, false, true>(lab, dx, dy, dz);
+        v[4] = fwt_filter<ORDER, L, S0, true, false, true>(lab, dx, dy, dz);
+        v[5] = fwt_filter<ORDER, L, S0, false, true, true>(lab, dx, dy, dz);
+        v[6] = fwt_filter<ORDER, L, S0, true, true, true>(lab, dx, dy, dz);
+#pragma unroll
+        for (int c = 0; c < 7; c++) {
+            const double av = fabs(v[c]);
             m = m > av ? m : av;
         }
     }
@@ -537,11 +601,18 @@ __global__ void k_sum_partials(const double *partial, int nblocks, int nv, doubl
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_fk_step(const StepArgs &a, hipStream_t st)
+hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st)
 {
     if (a.nwork <= 0) return hipSuccess;
-    const int grid = (a.nwork + 7) & ~7;
-    hipLaunchKernelGGL(k_fk_step, dim3(grid), dim3(256), 0, st, a);
+    const int items = a.nwork * (16 / zt);
+    const int grid = (items + 7) & ~7;
+    switch (zt) {
+    case 16: hipLaunchKernelGGL(k_fk_step<16>, dim3(grid), dim3(256), 0, st, a); break;
+    case 8: hipLaunchKernelGGL(k_fk_step<8>, dim3(grid), dim3(256), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_fk_step<4>, dim3(grid), dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_fk_step<2>, dim3(grid), dim3(256), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -549,7 +620,7 @@ hipError_t launch_face_ghosts(const float *phi, const int *ptr, const int *src, 
                               hipStream_t st)
 {
     if (nrows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_face_ghosts, dim3((nrows + 255) / 256), dim3(256), 0, st, phi, ptr, src, w, out, nrows);
+    hipLaunchKernelGGL(k_face_ghosts, dim3((4 * nrows + 255) / 256), dim3(256), 0, st, phi, ptr, src, w, out, nrows);
     return hipGetLastError();
 }
 

@@ -41,7 +41,8 @@ struct UniformArgs {
     double hole;
 };
 
-hipError_t launch_fk_step(const StepArgs &a, hipStream_t st);
+// zt: slab depth (16, 8, 4 or 2 planes per workgroup)
+hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st);
 hipError_t launch_face_ghosts(const float *phi, const int *ptr, const int *src, const float *w, float *out, int nrows,
                               hipStream_t st);
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,

@@ -207,6 +207,25 @@ def set_step_timing(grid, per_kernel):
     grid.ctx.call("gq_set_step_timing", 1 if per_kernel else 0)
 
 
+def set_autotune(ctx, mode=1, fixed_zt=16):
+    """mode 1: autotune the step slab depth; 0: size rule; 2: fixed_zt."""
+    ctx.call("gq_set_autotune", int(mode), int(fixed_zt))
+
+
+def autotune_info(ctx):
+    zt = C.c_int()
+    us = np.zeros(4)
+    ctx.call("gq_autotune_info", C.byref(zt), ptr(us, C.c_double))
+    return zt.value, dict(zip((16, 8, 4, 2), us.tolist()))
+
+
+def host_stats(ctx):
+    out = np.zeros(8)
+    ctx.call("gq_host_stats", ptr(out, C.c_double))
+    keys = ("tables_s", "plan_s", "fwt_s", "split_s", "collapse_s", "autotune_s", "n_tables", "n_fwt")
+    return dict(zip(keys, out.tolist()))
+
+
 class BlockFWT:
     @staticmethod
     def details(grid):

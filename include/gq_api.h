@@ -145,6 +145,18 @@ gq_status gq_last_step_timing(gq_ctx *ctx, double *step_kernel_ms, int *launches
  * pair of HIP events on the context stream (default 0: one pair around the
  * whole call, which then includes the cross-level ghost kernels). */
 gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel);
+/* Step tile autotuner.  mode 1 (default): time the slab depths 16/8/4/2
+ * (planes of a 16^3 block per workgroup) on the grid the first time a block
+ * count of a new power-of-two size is stepped, and keep the fastest; mode 0:
+ * a fixed size rule; mode 2: always use fixed_zt. */
+gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt);
+/* Slab depth used by the last gq_fk_steps and the last autotune timings
+ * (us per launch for depths 16, 8, 4, 2). */
+gq_status gq_autotune_info(gq_ctx *ctx, int *zt, double us[4]);
+/* Host-side seconds spent in {ghost-table builds, refine/compress planning,
+ * FWT calls, splits, collapses, autotuning} and the counts of table builds
+ * and FWT calls, accumulated over the context's life. */
+gq_status gq_host_stats(gq_ctx *ctx, double out[8]);
 /* Device-memory footprint of the context (bytes) and pool capacity (blocks). */
 gq_status gq_memory_info(gq_ctx *ctx, int64_t *bytes, int *pool_blocks);
 
