@@ -100,7 +100,8 @@ struct gq_ctx {
     gq::Tables tabs;
     std::vector<int> api_order;  // leaf indices in API (Morton) order
     DevBuf<int> d_work, d_level, d_index, d_fnbr, d_fbase, d_face_ptr, d_face_src, d_nbr26, d_rbase26, d_lab_ptr,
-        d_lab_src, d_lut;
+        d_lab_src, d_lut, d_row_begin, d_row_end;
+    DevBuf<double> d_ghost;  // evaluated lab rows (3 fields for a split)
     DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2;
     DevBuf<double> d_lab_w;
     bool fik_valid = false;
@@ -240,6 +241,9 @@ struct gq_ctx {
         d_lab_src.upload(t.lab_rows.src, st);
         d_lab_w.upload(t.lab_rows.w, st);
         d_lut.upload(t.lut_slot, st);
+        d_row_begin.upload(t.lab_row_begin, st);
+        d_row_end.upload(t.lab_row_end, st);
+        d_ghost.reserve((size_t)3 * std::max(1, t.lab_rows.nrows()));
         d_gphi.reserve((size_t)std::max(1, t.face_rows.nrows()));
         d_fik.reserve((size_t)cap * 6 * 256);
         std::vector<float> ih2(16);
@@ -262,6 +266,8 @@ struct gq_ctx {
         a.ptr = d_lab_ptr.p;
         a.src = d_lab_src.p;
         a.w = d_lab_w.p;
+        a.row_begin = d_row_begin.p;
+        a.row_end = d_row_end.p;
         return a;
     }
 
@@ -289,7 +295,10 @@ struct gq_ctx {
         n_fwt++;
         d_list.upload(slots, st);
         d_dbl.reserve(slots.size());
-        HIPCHK(gq::launch_fwt_detail(grid->order(), lab_args(), phi[cur], d_list.p, (int)slots.size(), d_dbl.p, st));
+        const gq::LabArgs la = lab_args();
+        HIPCHK(gq::launch_lab_ghosts(la, phi[cur], d_list.p, (int)slots.size(), d_ghost.p, st));
+        HIPCHK(gq::launch_fwt_detail(grid->order(), la, d_ghost.p, phi[cur], d_list.p, (int)slots.size(), d_dbl.p,
+                                     st));
         HIPCHK(hipMemcpyAsync(out.data(), d_dbl.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         return out;
@@ -327,8 +336,13 @@ struct gq_ctx {
         }
         d_list.upload(pslots, st);
         d_list2.upload(cslots, st);
-        HIPCHK(gq::launch_split(grid->order(), lab_args(), phi[cur], pw, pg, d_list.p, d_list2.p, (int)pslots.size(),
-                                st));
+        const gq::LabArgs la = lab_args();
+        const int nrows = std::max(1, tabs.lab_rows.nrows());
+        const float *fl[3] = {phi[cur], pw, pg};
+        for (int f = 0; f < 3; f++)
+            HIPCHK(gq::launch_lab_ghosts(la, fl[f], d_list.p, (int)pslots.size(), d_ghost.p + (size_t)f * nrows, st));
+        HIPCHK(gq::launch_split(grid->order(), la, d_ghost.p, nrows, phi[cur], pw, pg, d_list.p, d_list2.p,
+                                (int)pslots.size(), st));
         HIPCHK(hipStreamSynchronize(st));
         for (int s : pslots) free_slots.push_back(s);
         grid->replace_leaves(parents, add);

@@ -7,6 +7,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <unordered_set>
 
 namespace gq {
 
@@ -328,25 +329,6 @@ inline int face_code(int f)
     return codes[f];
 }
 
-struct LeafRows {
-    std::vector<int> counts;  // terms per row
-    std::vector<int> src;
-    std::vector<double> w;
-};
-
-void expand_rows(const Grid &g, int leaf, const std::vector<std::array<int, 3>> &pts, LeafRows &out,
-                 std::vector<Term> &scratch, std::vector<Term> &terms)
-{
-    const auto &L = g.leaves();
-    for (const auto &p : pts) {
-        g.expand_point(leaf, p[0], p[1], p[2], scratch, terms);
-        out.counts.push_back((int)terms.size());
-        for (const Term &t : terms) {
-            out.src.push_back(L[t.slot].slot * BS3 + t.cell);
-            out.w.push_back(t.w);
-        }
-    }
-}
 }  // namespace
 
 void build_tables(const Grid &g, int nslots, Tables &t, int nthreads)
@@ -359,9 +341,11 @@ void build_tables(const Grid &g, int nslots, Tables &t, int nthreads)
     t.fbase.assign((size_t)nslots * 6, -1);
     t.nbr26.assign((size_t)nslots * 26, -1);
     t.rbase26.assign((size_t)nslots * 26, -1);
+    t.lab_row_begin.assign((size_t)nslots, 0);
+    t.lab_row_end.assign((size_t)nslots, 0);
 
     // same-level neighbours per region (periodic wrap on the leaf's level)
-    std::vector<char> face_cross((size_t)n * 6, 0), reg_cross((size_t)n * 27, 0);
+    std::vector<char> reg_cross((size_t)n * 27, 0);
     for (int i = 0; i < n; i++) {
         const Leaf &lf = L[i];
         const int m = 1 << lf.level;
@@ -375,53 +359,53 @@ void build_tables(const Grid &g, int nslots, Tables &t, int nthreads)
             else reg_cross[(size_t)i * 27 + c] = 1;
         }
         for (int f = 0; f < 6; f++) {
-            const int c = face_code(f);
-            const int cc = c < 13 ? c : c - 1;
-            const int nbs = t.nbr26[(size_t)lf.slot * 26 + cc];
-            if (nbs >= 0) t.fnbr[(size_t)lf.slot * 6 + f] = nbs;
-            else face_cross[(size_t)i * 6 + f] = 1;
+            const int cc = face_code(f) < 13 ? face_code(f) : face_code(f) - 1;
+            t.fnbr[(size_t)lf.slot * 6 + f] = t.nbr26[(size_t)lf.slot * 26 + cc];
         }
     }
 
-    // expansion work, parallel over leaves
-    std::vector<LeafRows> face_out(n), lab_out(n);
+    // neighbour signatures (block ids) decide which cached rows stay valid
+    std::vector<std::vector<int>> sig(n);
+    std::vector<char> need(n, 0);
+    {
+        std::vector<std::pair<int, bool>> nb;
+        for (int i = 0; i < n; i++) {
+            g.neighbors(i, nb);
+            std::vector<int> &v = sig[i];
+            for (auto &p : nb) v.push_back(L[p.first].block_id);
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            auto it = t.cache.find(L[i].block_id);
+            need[i] = (it == t.cache.end() || it->second.nbr_ids != v) ? 1 : 0;
+        }
+    }
+    std::vector<int> todo;
+    for (int i = 0; i < n; i++)
+        if (need[i]) todo.push_back(i);
+    std::vector<LeafRowCache> fresh(todo.size());
     auto work = [&](int tid, int nt) {
         std::vector<Term> scratch, terms;
-        std::vector<std::array<int, 3>> pts;
-        for (int i = tid; i < n; i += nt) {
-            // faces (halo 1)
-            pts.clear();
-            for (int f = 0; f < 6; f++) {
-                if (!face_cross[(size_t)i * 6 + f]) continue;
-                for (int v = 0; v < BS; v++)
-                    for (int u = 0; u < BS; u++) {
-                        std::array<int, 3> p;
-                        switch (f) {
-                        case 0: p = {-1, u, v}; break;
-                        case 1: p = {BS, u, v}; break;
-                        case 2: p = {u, -1, v}; break;
-                        case 3: p = {u, BS, v}; break;
-                        case 4: p = {u, v, -1}; break;
-                        default: p = {u, v, BS}; break;
-                        }
-                        pts.push_back(p);
-                    }
-            }
-            expand_rows(g, i, pts, face_out[i], scratch, terms);
-            // wide-lab regions
-            pts.clear();
+        for (size_t q = tid; q < todo.size(); q += nt) {
+            const int i = todo[q];
+            LeafRowCache &out = fresh[q];
             for (int c = 0; c < 27; c++) {
                 if (c == 13 || !reg_cross[(size_t)i * 27 + c]) continue;
                 const Region r = region_of(c, s0, s1);
                 for (int z = r.s[2]; z < r.e[2]; z++)
                     for (int y = r.s[1]; y < r.e[1]; y++)
-                        for (int x = r.s[0]; x < r.e[0]; x++) pts.push_back({x, y, z});
+                        for (int x = r.s[0]; x < r.e[0]; x++) {
+                            g.expand_point(i, x, y, z, scratch, terms);
+                            out.counts.push_back((int)terms.size());
+                            for (const Term &tm : terms) {
+                                out.src.push_back(L[tm.slot].slot * BS3 + tm.cell);
+                                out.w.push_back(tm.w);
+                            }
+                        }
             }
-            expand_rows(g, i, pts, lab_out[i], scratch, terms);
         }
     };
-    int nt = std::max(1, std::min(nthreads, n));
-    if (nt == 1) work(0, 1);
+    int nt = std::max(1, std::min(nthreads, (int)todo.size()));
+    if (nt <= 1) work(0, 1);
     else {
         std::vector<std::thread> th;
         std::vector<std::exception_ptr> errs(nt);
@@ -433,45 +417,76 @@ void build_tables(const Grid &g, int nslots, Tables &t, int nthreads)
         for (auto &e : errs)
             if (e) std::rethrow_exception(e);
     }
+    for (size_t q = 0; q < todo.size(); q++) {
+        const int i = todo[q];
+        fresh[q].nbr_ids = std::move(sig[i]);
+        t.cache[L[i].block_id] = std::move(fresh[q]);
+    }
+    t.last_expanded = (int)todo.size();
+    t.last_reused = n - (int)todo.size();
+    // drop cache entries of leaves that no longer exist
+    {
+        std::unordered_set<int> live;
+        for (const Leaf &lf : L) live.insert(lf.block_id);
+        for (auto it = t.cache.begin(); it != t.cache.end();)
+            it = live.count(it->first) ? std::next(it) : t.cache.erase(it);
+    }
 
-    // concatenate in leaf order
+    // concatenate in leaf order; face rows are the lab rows of the 1-cell halo
     t.face_rows.clear();
     t.lab_rows.clear();
+    size_t tot = 0;
+    for (int i = 0; i < n; i++) tot += t.cache[L[i].block_id].src.size();
+    t.lab_rows.src.reserve(tot);
+    t.lab_rows.w.reserve(tot);
     for (int i = 0; i < n; i++) {
         const Leaf &lf = L[i];
-        int k = 0;  // row cursor within this leaf's face rows
+        const LeafRowCache &lc = t.cache[lf.block_id];
+        t.lab_row_begin[lf.slot] = t.lab_rows.nrows();
+        int k = 0;
         size_t off = 0;
-        for (int f = 0; f < 6; f++) {
-            if (!face_cross[(size_t)i * 6 + f]) continue;
-            t.fbase[(size_t)lf.slot * 6 + f] = t.face_rows.nrows();
-            for (int r = 0; r < BS * BS; r++, k++) {
-                const int cnt = face_out[i].counts[k];
-                for (int q = 0; q < cnt; q++) {
-                    t.face_rows.src.push_back(face_out[i].src[off + q]);
-                    t.face_rows.w.push_back(face_out[i].w[off + q]);
-                }
-                off += cnt;
-                t.face_rows.ptr.push_back((int)t.face_rows.src.size());
-            }
-        }
-        k = 0;
-        off = 0;
         for (int c = 0; c < 27; c++) {
             if (c == 13 || !reg_cross[(size_t)i * 27 + c]) continue;
             const int cc = c < 13 ? c : c - 1;
-            t.rbase26[(size_t)lf.slot * 26 + cc] = t.lab_rows.nrows();
+            const int row0 = t.lab_rows.nrows();
+            t.rbase26[(size_t)lf.slot * 26 + cc] = row0;
             const Region r = region_of(c, s0, s1);
             const int npts = (r.e[0] - r.s[0]) * (r.e[1] - r.s[1]) * (r.e[2] - r.s[2]);
             for (int q0 = 0; q0 < npts; q0++, k++) {
-                const int cnt = lab_out[i].counts[k];
-                for (int q = 0; q < cnt; q++) {
-                    t.lab_rows.src.push_back(lab_out[i].src[off + q]);
-                    t.lab_rows.w.push_back(lab_out[i].w[off + q]);
-                }
+                const int cnt = lc.counts[k];
+                t.lab_rows.src.insert(t.lab_rows.src.end(), lc.src.begin() + off, lc.src.begin() + off + cnt);
+                t.lab_rows.w.insert(t.lab_rows.w.end(), lc.w.begin() + off, lc.w.begin() + off + cnt);
                 off += cnt;
                 t.lab_rows.ptr.push_back((int)t.lab_rows.src.size());
             }
+            // a cross-level face region: its 1-cell-halo plane becomes the step rows
+            int f = -1;
+            for (int ff = 0; ff < 6; ff++)
+                if (face_code(ff) == c) f = ff;
+            if (f >= 0) {
+                t.fbase[(size_t)lf.slot * 6 + f] = t.face_rows.nrows();
+                const int ex = r.e[0] - r.s[0], ey = r.e[1] - r.s[1];
+                for (int v = 0; v < BS; v++)
+                    for (int u = 0; u < BS; u++) {
+                        int x, y, z;
+                        switch (f) {
+                        case 0: x = -1; y = u; z = v; break;
+                        case 1: x = BS; y = u; z = v; break;
+                        case 2: x = u; y = -1; z = v; break;
+                        case 3: x = u; y = BS; z = v; break;
+                        case 4: x = u; y = v; z = -1; break;
+                        default: x = u; y = v; z = BS; break;
+                        }
+                        const int lr = row0 + (x - r.s[0]) + ex * ((y - r.s[1]) + ey * (z - r.s[2]));
+                        for (int q = t.lab_rows.ptr[lr]; q < t.lab_rows.ptr[lr + 1]; q++) {
+                            t.face_rows.src.push_back(t.lab_rows.src[q]);
+                            t.face_rows.w.push_back(t.lab_rows.w[q]);
+                        }
+                        t.face_rows.ptr.push_back((int)t.face_rows.src.size());
+                    }
+            }
         }
+        t.lab_row_end[lf.slot] = t.lab_rows.nrows();
     }
     if (t.face_rows.src.size() > (size_t)INT32_MAX || t.lab_rows.src.size() > (size_t)INT32_MAX)
         throw std::runtime_error("ghost tables exceed 2^31 terms");

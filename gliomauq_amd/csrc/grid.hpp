@@ -106,6 +106,15 @@ inline uint64_t leaf_key(int l, int x, int y, int z)
 }
 
 // Per-slot device tables derived from a Grid.
+// Lab rows of one leaf, reusable while the leaf and its neighbours (by block
+// id) are unchanged.
+struct LeafRowCache {
+    std::vector<int> nbr_ids;       // sorted block ids of the neighbours
+    std::vector<int> counts;        // terms per row, regions in code order, points z,y,x
+    std::vector<int> src;
+    std::vector<double> w;
+};
+
 struct Tables {
     int nslots = 0;                 // table capacity in slots
     // step: halo-1 faces, f = 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z
@@ -116,6 +125,11 @@ struct Tables {
     std::vector<int> nbr26;         // [nslots*26]
     std::vector<int> rbase26;       // [nslots*26]
     Rows lab_rows;
+    std::vector<int> lab_row_begin; // [nslots] rows of a slot: [begin, end)
+    std::vector<int> lab_row_end;
+    // per-leaf row cache (key: block id) and the counts of the last build
+    std::unordered_map<int, LeafRowCache> cache;
+    int last_expanded = 0, last_reused = 0;
     // finest-block look-up for the dump / likelihood
     std::vector<int> lut_slot;      // (bpd)^3 at the base level -> slot
 };

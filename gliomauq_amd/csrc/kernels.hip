@@ -243,7 +243,8 @@ struct LabGeom {
 };
 
 template <int S0, int S1>
-__device__ void fill_lab(double *lab, const float *__restrict__ fld, int slot, const LabArgs &a)
+__device__ void fill_lab(double *lab, const float *__restrict__ fld, const double *__restrict__ ghost, int slot,
+                         const LabArgs &a)
 {
     constexpr int L = LabGeom<S0, S1>::L;
     constexpr int N = LabGeom<S0, S1>::N;
@@ -267,17 +268,29 @@ __device__ void fill_lab(double *lab, const float *__restrict__ fld, int slot, c
                 const int sx = dx < 0 ? S0 : (dx == 0 ? 0 : B), ex = dx < 0 ? 0 : (dx == 0 ? B : B + S1 - 1);
                 const int sy = dy < 0 ? S0 : (dy == 0 ? 0 : B), ey = dy < 0 ? 0 : (dy == 0 ? B : B + S1 - 1);
                 const int sz = dz < 0 ? S0 : (dz == 0 ? 0 : B);
-                const int r = a.rbase26[slot * 26 + cc] + (x - sx) + (ex - sx) * ((y - sy) + (ey - sy) * (z - sz));
-                double acc = 0.0;
-                for (int k = a.ptr[r]; k < a.ptr[r + 1]; k++)
-                    acc = __dadd_rn(acc, __dmul_rn(a.w[k], (double)fld[a.src[k]]));
-                v = acc;
+                v = ghost[a.rbase26[slot * 26 + cc] + (x - sx) + (ex - sx) * ((y - sy) + (ey - sy) * (z - sz))];
             }
         }
         lab[q] = v;
     }
 }
+}  // namespace
 
+// One workgroup per listed slot; threads evaluate its cross-level lab rows,
+// each row a sequential FP64 sum in the canonical term order.
+__global__ __launch_bounds__(256) void k_lab_ghosts(LabArgs a, const float *__restrict__ fld,
+                                                    const int *__restrict__ slots, double *__restrict__ ghost)
+{
+    const int slot = slots[blockIdx.x];
+    const int r0 = a.row_begin[slot], r1 = a.row_end[slot];
+    for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        double acc = 0.0;
+        for (int k = a.ptr[r]; k < a.ptr[r + 1]; k++) acc = __dadd_rn(acc, __dmul_rn(a.w[k], (double)fld[a.src[k]]));
+        ghost[r] = acc;
+    }
+}
+
+namespace {
 // Non-zero taps of the FWT filters in increasing i (the reference loops over
 // the full supports; a zero-weight tap adds an exact zero to a finite sum, so
 // skipping it leaves every bit of the result unchanged).
@@ -336,8 +349,9 @@ __device__ __forceinline__ double fwt_filter(const double *lab, int dx, int dy, 
 // z, y, x loop order, FP64 without contraction; block detail = max over
 // codes 1..7 of |value|.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const float *__restrict__ phi,
-                                                    const int *__restrict__ cand, double *__restrict__ out)
+__global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const double *__restrict__ ghost,
+                                                    const float *__restrict__ phi, const int *__restrict__ cand,
+                                                    double *__restrict__ out)
 {
     constexpr int S0 = ORDER == 4 ? -2 : -1;
     constexpr int S1 = ORDER == 4 ? 4 : 2;
@@ -345,7 +359,7 @@ __global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const float *__re
     __shared__ double lab[LabGeom<S0, S1>::N];
     __shared__ double red[256];
     const int slot = cand[blockIdx.x];
-    fill_lab<S0, S1>(lab, phi, slot, a);
+    fill_lab<S0, S1>(lab, phi, ghost, slot, a);
     __syncthreads();
     double m = 0.0;
     for (int q = threadIdx.x; q < 512; q += blockDim.x) {
@@ -377,8 +391,9 @@ __global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const float *__re
 // point d = sum over parent lab points s (z, y, x increasing) of
 // lab(s) * ((wZ*wY)*wX), w = Hs(2s - d), for phi, pw and pg.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_split(LabArgs a, float *phi, float *pw, float *pg,
-                                               const int *__restrict__ parents, const int *__restrict__ children)
+__global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restrict__ ghost3, int nrows, float *phi,
+                                               float *pw, float *pg, const int *__restrict__ parents,
+                                               const int *__restrict__ children)
 {
     constexpr int S0 = ORDER == 4 ? -2 : -1;
     constexpr int S1 = ORDER == 4 ? 4 : 2;
@@ -387,7 +402,7 @@ __global__ __launch_bounds__(256) void k_split(LabArgs a, float *phi, float *pw,
     const int pslot = parents[blockIdx.x];
     float *fields[3] = {phi, pw, pg};
     for (int f = 0; f < 3; f++) {
-        fill_lab<S0, S1>(lab, fields[f], pslot, a);
+        fill_lab<S0, S1>(lab, fields[f], ghost3 + (size_t)f * nrows, pslot, a);
         __syncthreads();
         for (int q = threadIdx.x; q < 8 * B3; q += blockDim.x) {
             const int dx = q & 31, dy = (q >> 5) & 31, dz = q >> 10;
@@ -634,21 +649,33 @@ hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, i
     return hipGetLastError();
 }
 
-hipError_t launch_fwt_detail(int order, const LabArgs &a, const float *phi, const int *cand, int ncand, double *out,
+hipError_t launch_lab_ghosts(const LabArgs &a, const float *fld, const int *slots, int nslots, double *ghost,
                              hipStream_t st)
 {
-    if (ncand <= 0) return hipSuccess;
-    if (order == 4) hipLaunchKernelGGL(k_fwt_detail<4>, dim3(ncand), dim3(256), 0, st, a, phi, cand, out);
-    else hipLaunchKernelGGL(k_fwt_detail<2>, dim3(ncand), dim3(256), 0, st, a, phi, cand, out);
+    if (nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lab_ghosts, dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
     return hipGetLastError();
 }
 
-hipError_t launch_split(int order, const LabArgs &a, float *phi, float *pw, float *pg, const int *parents,
-                        const int *children, int nparents, hipStream_t st)
+hipError_t launch_fwt_detail(int order, const LabArgs &a, const double *ghost, const float *phi, const int *cand,
+                             int ncand, double *out, hipStream_t st)
+{
+    if (ncand <= 0) return hipSuccess;
+    if (order == 4) hipLaunchKernelGGL(k_fwt_detail<4>, dim3(ncand), dim3(256), 0, st, a, ghost, phi, cand, out);
+    else hipLaunchKernelGGL(k_fwt_detail<2>, dim3(ncand), dim3(256), 0, st, a, ghost, phi, cand, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_split(int order, const LabArgs &a, const double *ghost3, int nrows, float *phi, float *pw,
+                        float *pg, const int *parents, const int *children, int nparents, hipStream_t st)
 {
     if (nparents <= 0) return hipSuccess;
-    if (order == 4) hipLaunchKernelGGL(k_split<4>, dim3(nparents), dim3(256), 0, st, a, phi, pw, pg, parents, children);
-    else hipLaunchKernelGGL(k_split<2>, dim3(nparents), dim3(256), 0, st, a, phi, pw, pg, parents, children);
+    if (order == 4)
+        hipLaunchKernelGGL(k_split<4>, dim3(nparents), dim3(256), 0, st, a, ghost3, nrows, phi, pw, pg, parents,
+                           children);
+    else
+        hipLaunchKernelGGL(k_split<2>, dim3(nparents), dim3(256), 0, st, a, ghost3, nrows, phi, pw, pg, parents,
+                           children);
     return hipGetLastError();
 }
 

@@ -29,6 +29,8 @@ struct LabArgs {
     const int *ptr;
     const int *src;
     const double *w;
+    const int *row_begin;  // [slot] lab rows of a slot: [row_begin, row_end)
+    const int *row_end;
 };
 
 struct UniformArgs {
@@ -48,10 +50,15 @@ hipError_t launch_face_ghosts(const float *phi, const int *ptr, const int *src, 
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
                             const int *fbase, const int *ptr, const int *src, const float *w, float *fik, float Dw,
                             float Dg, hipStream_t st);
-hipError_t launch_fwt_detail(int order, const LabArgs &a, const float *phi, const int *cand, int ncand, double *out,
+// Evaluate the cross-level lab rows of the listed slots for one field (FP64,
+// canonical row order) into ghost[row].
+hipError_t launch_lab_ghosts(const LabArgs &a, const float *fld, const int *slots, int nslots, double *ghost,
                              hipStream_t st);
-hipError_t launch_split(int order, const LabArgs &a, float *phi, float *pw, float *pg, const int *parents,
-                        const int *children, int nparents, hipStream_t st);
+hipError_t launch_fwt_detail(int order, const LabArgs &a, const double *ghost, const float *phi, const int *cand,
+                             int ncand, double *out, hipStream_t st);
+// ghost3: three ghost arrays (phi, pw, pg) of nrows each
+hipError_t launch_split(int order, const LabArgs &a, const double *ghost3, int nrows, float *phi, float *pw,
+                        float *pg, const int *parents, const int *children, int nparents, hipStream_t st);
 hipError_t launch_collapse(float *phi, float *pw, float *pg, const int *parents, const int *children, int nparents,
                            hipStream_t st);
 hipError_t launch_dump_uniform(const UniformArgs &u, double *out, hipStream_t st);
