@@ -99,11 +99,11 @@ struct gq_ctx {
     bool tables_valid = false;
     gq::Tables tabs;
     std::vector<int> api_order;  // leaf indices in API (Morton) order
-    DevBuf<int> d_work, d_level, d_index, d_fnbr, d_fbase, d_face_ptr, d_face_src, d_nbr26, d_rbase26, d_lab_ptr,
-        d_lab_src, d_lut, d_row_begin, d_row_end;
-    DevBuf<double> d_ghost;  // evaluated lab rows (3 fields for a split)
+    DevBuf<int> d_work, d_level, d_index, d_fnbr, d_fbase, d_face_ptr, d_face_src, d_nbr26, d_lut, d_lutf,
+        d_faces, d_counts;
+    int face_nrows = 0;
+    DevBuf<double> d_ghost;  // cross-level lab points (3 fields for a split)
     DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2;
-    DevBuf<double> d_lab_w;
     bool fik_valid = false;
     float fik_Dw = 0, fik_Dg = 0;
 
@@ -216,7 +216,7 @@ struct gq_ctx {
         ScopeTimer tm(t_tables);
         n_tables++;
         gq::Tables &t = tabs;
-        gq::build_tables(*grid, cap, t, hw_threads());
+        gq::build_tables(*grid, cap, t);
         api_order = gq::morton_order(*grid);
         const auto &L = grid->leaves();
         std::vector<int> work(api_order.size());
@@ -226,25 +226,38 @@ struct gq_ctx {
             lvl[lf.slot] = lf.level;
             for (int a = 0; a < 3; a++) idx[(size_t)lf.slot * 3 + a] = lf.idx[a];
         }
+        std::vector<int> fbase((size_t)cap * 6, -1);
+        for (size_t k = 0; k < t.faces.size(); k++) fbase[t.faces[k]] = (int)k * 256;
         d_work.upload(work, st);
         d_level.upload(lvl, st);
         d_index.upload(idx, st);
         d_fnbr.upload(t.fnbr, st);
-        d_fbase.upload(t.fbase, st);
-        d_face_ptr.upload(t.face_rows.ptr, st);
-        d_face_src.upload(t.face_rows.src, st);
-        std::vector<float> fw(t.face_rows.w.begin(), t.face_rows.w.end());
-        d_face_w.upload(fw, st);
+        d_fbase.upload(fbase, st);
         d_nbr26.upload(t.nbr26, st);
-        d_rbase26.upload(t.rbase26, st);
-        d_lab_ptr.upload(t.lab_rows.ptr, st);
-        d_lab_src.upload(t.lab_rows.src, st);
-        d_lab_w.upload(t.lab_rows.w, st);
         d_lut.upload(t.lut_slot, st);
-        d_row_begin.upload(t.lab_row_begin, st);
-        d_row_end.upload(t.lab_row_end, st);
-        d_ghost.reserve((size_t)3 * std::max(1, t.lab_rows.nrows()));
-        d_gphi.reserve((size_t)std::max(1, t.face_rows.nrows()));
+        d_lutf.upload(t.lut, st);
+        d_faces.upload(t.faces, st);
+        // step rows of the cross-level faces, enumerated on the device
+        const int nfaces = (int)t.faces.size();
+        face_nrows = nfaces * 256;
+        if (nfaces > 0) {
+            const gq::LabArgs la = lab_args();
+            d_counts.reserve((size_t)face_nrows);
+            HIPCHK(gq::launch_face_rows_count(la, d_faces.p, nfaces, d_counts.p, st));
+            std::vector<int> cnt((size_t)face_nrows), ptr((size_t)face_nrows + 1);
+            HIPCHK(hipMemcpyAsync(cnt.data(), d_counts.p, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            ptr[0] = 0;
+            for (int r = 0; r < face_nrows; r++) {
+                if ((int64_t)ptr[r] + cnt[r] > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "face rows exceed 2^31 terms");
+                ptr[r + 1] = ptr[r] + cnt[r];
+            }
+            d_face_ptr.upload(ptr, st);
+            d_face_src.reserve((size_t)std::max(1, ptr[face_nrows]));
+            d_face_w.reserve((size_t)std::max(1, ptr[face_nrows]));
+            HIPCHK(gq::launch_face_rows_fill(la, d_faces.p, nfaces, d_face_ptr.p, d_face_src.p, d_face_w.p, st));
+        }
+        d_gphi.reserve((size_t)std::max(1, face_nrows));
         d_fik.reserve((size_t)cap * 6 * 256);
         std::vector<float> ih2(16);
         for (int l = 0; l < 16; l++) {
@@ -253,7 +266,7 @@ struct gq_ctx {
             ih2[l] = (float)(1. / (hd * hd));
         }
         d_ih2.upload(ih2, st);
-        HIPCHK(hipStreamSynchronize(st));  // host vectors of tabs stay alive; sync for safety of reuse
+        HIPCHK(hipStreamSynchronize(st));  // the host vectors above are released on return
         tables_valid = true;
         fik_valid = false;
     }
@@ -262,12 +275,11 @@ struct gq_ctx {
     {
         gq::LabArgs a;
         a.nbr26 = d_nbr26.p;
-        a.rbase26 = d_rbase26.p;
-        a.ptr = d_lab_ptr.p;
-        a.src = d_lab_src.p;
-        a.w = d_lab_w.p;
-        a.row_begin = d_row_begin.p;
-        a.row_end = d_row_end.p;
+        a.lut = d_lutf.p;
+        a.level = d_level.p;
+        a.index = d_index.p;
+        a.lut_level = grid->lut_level();
+        a.order = grid->order();
         return a;
     }
 
@@ -296,6 +308,7 @@ struct gq_ctx {
         d_list.upload(slots, st);
         d_dbl.reserve(slots.size());
         const gq::LabArgs la = lab_args();
+        d_ghost.reserve(slots.size() * (size_t)gq::lab_points(grid->order()));
         HIPCHK(gq::launch_lab_ghosts(la, phi[cur], d_list.p, (int)slots.size(), d_ghost.p, st));
         HIPCHK(gq::launch_fwt_detail(grid->order(), la, d_ghost.p, phi[cur], d_list.p, (int)slots.size(), d_dbl.p,
                                      st));
@@ -337,11 +350,12 @@ struct gq_ctx {
         d_list.upload(pslots, st);
         d_list2.upload(cslots, st);
         const gq::LabArgs la = lab_args();
-        const int nrows = std::max(1, tabs.lab_rows.nrows());
+        const size_t per = pslots.size() * (size_t)gq::lab_points(grid->order());
+        d_ghost.reserve(3 * per);
         const float *fl[3] = {phi[cur], pw, pg};
         for (int f = 0; f < 3; f++)
-            HIPCHK(gq::launch_lab_ghosts(la, fl[f], d_list.p, (int)pslots.size(), d_ghost.p + (size_t)f * nrows, st));
-        HIPCHK(gq::launch_split(grid->order(), la, d_ghost.p, nrows, phi[cur], pw, pg, d_list.p, d_list2.p,
+            HIPCHK(gq::launch_lab_ghosts(la, fl[f], d_list.p, (int)pslots.size(), d_ghost.p + f * per, st));
+        HIPCHK(gq::launch_split(grid->order(), la, d_ghost.p, phi[cur], pw, pg, d_list.p, d_list2.p,
                                 (int)pslots.size(), st));
         HIPCHK(hipStreamSynchronize(st));
         for (int s : pslots) free_slots.push_back(s);
@@ -543,6 +557,7 @@ gq_status gq_grid_create(gq_ctx *ctx, int bpd, int max_level, int order)
     return guard(ctx, [&] {
         HIPCHK(hipSetDevice(ctx->device));
         auto g = std::make_unique<gq::Grid>(bpd, max_level, order);
+        ctx->tabs = gq::Tables();  // block ids restart: drop the row cache of any previous grid
         ctx->tables_valid = false;
         ctx->fik_valid = false;
         ctx->release_pool();
@@ -643,7 +658,7 @@ gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
         const float Dw = (float)p->Dw, Dg = (float)p->Dg, rho = (float)p->rho;
         ctx->ensure_fik(Dw, Dg);
         const int nwork = (int)ctx->api_order.size();
-        const int nrows = ctx->tabs.face_rows.nrows();
+        const int nrows = ctx->face_nrows;
         gq::StepArgs a;
         a.pw = ctx->pw;
         a.pg = ctx->pg;
@@ -994,7 +1009,7 @@ gq_status gq_memory_info(gq_ctx *ctx, int64_t *bytes, int *pool_blocks)
 {
     return guard(ctx, [&] {
         int64_t b = (int64_t)ctx->cap * gq::BS3 * 4 * 4;
-        b += (int64_t)(ctx->d_lab_w.n * 8 + ctx->d_lab_src.n * 4 + ctx->d_lab_ptr.n * 4);
+        b += (int64_t)(ctx->d_ghost.n * 8);
         b += (int64_t)(ctx->d_face_w.n * 4 + ctx->d_face_src.n * 4 + ctx->d_face_ptr.n * 4 + ctx->d_fik.n * 4);
         if (bytes) *bytes = b;
         if (pool_blocks) *pool_blocks = ctx->cap;

@@ -6,7 +6,6 @@
 #include <set>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <unordered_set>
 
 namespace gq {
@@ -128,7 +127,7 @@ void Grid::neighbors(int i, std::vector<std::pair<int, bool>> &out) const
         bs[a] = (int64_t)lf.idx[a] << d;
         be[a] = ((int64_t)lf.idx[a] + 1) << d;
     }
-    std::set<std::pair<int, bool>> seen;
+    out.reserve(64);
     for (int64_t z = bs[2] - 1; z <= be[2]; z++)
         for (int64_t y = bs[1] - 1; y <= be[1]; y++)
             for (int64_t x = bs[0] - 1; x <= be[0]; x++) {
@@ -141,9 +140,10 @@ void Grid::neighbors(int i, std::vector<std::pair<int, bool>> &out) const
                 const bool img = !(x >= 0 && x < nF && y >= 0 && y < nF && z >= 0 && z < nF);
                 const int k = lut_[(size_t)(wrapi(x, nF) + nF * (wrapi(y, nF) + nF * wrapi(z, nF)))];
                 if (k == i && !img) continue;
-                seen.insert(std::make_pair(k, img));
+                out.emplace_back(k, img);
             }
-    out.assign(seen.begin(), seen.end());
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
 }
 
 std::vector<int> Grid::plan_refine(const std::unordered_set<int> &flagged_ids) const
@@ -308,44 +308,15 @@ int Grid::expand_point(int leaf, int lx, int ly, int lz, std::vector<Term> &scra
 // ---------------------------------------------------------------------------
 // Tables
 // ---------------------------------------------------------------------------
-namespace {
-struct Region {
-    int s[3], e[3];
-};
-inline Region region_of(int c, int s0, int s1)
+void build_tables(const Grid &g, int nslots, Tables &t)
 {
-    Region r;
-    const int d[3] = {c % 3 - 1, (c / 3) % 3 - 1, c / 9 - 1};
-    for (int a = 0; a < 3; a++) {
-        r.s[a] = d[a] < 0 ? s0 : (d[a] == 0 ? 0 : BS);
-        r.e[a] = d[a] < 0 ? 0 : (d[a] == 0 ? BS : BS + s1 - 1);
-    }
-    return r;
-}
-// face f -> region code with a 1-cell halo
-inline int face_code(int f)
-{
-    static const int codes[6] = {12, 14, 10, 16, 4, 22};
-    return codes[f];
-}
-
-}  // namespace
-
-void build_tables(const Grid &g, int nslots, Tables &t, int nthreads)
-{
+    static const int face_codes[6] = {12, 14, 10, 16, 4, 22};
     const auto &L = g.leaves();
     const int n = (int)L.size();
-    const int s0 = g.lab_s0(), s1 = g.lab_s1();
     t.nslots = nslots;
     t.fnbr.assign((size_t)nslots * 6, -1);
-    t.fbase.assign((size_t)nslots * 6, -1);
     t.nbr26.assign((size_t)nslots * 26, -1);
-    t.rbase26.assign((size_t)nslots * 26, -1);
-    t.lab_row_begin.assign((size_t)nslots, 0);
-    t.lab_row_end.assign((size_t)nslots, 0);
-
-    // same-level neighbours per region (periodic wrap on the leaf's level)
-    std::vector<char> reg_cross((size_t)n * 27, 0);
+    t.faces.clear();
     for (int i = 0; i < n; i++) {
         const Leaf &lf = L[i];
         const int m = 1 << lf.level;
@@ -354,143 +325,18 @@ void build_tables(const Grid &g, int nslots, Tables &t, int nthreads)
             const int d[3] = {c % 3 - 1, (c / 3) % 3 - 1, c / 9 - 1};
             const int j = g.find(lf.level, (int)wrapi(lf.idx[0] + d[0], m), (int)wrapi(lf.idx[1] + d[1], m),
                                  (int)wrapi(lf.idx[2] + d[2], m));
-            const int cc = c < 13 ? c : c - 1;
-            if (j >= 0) t.nbr26[(size_t)lf.slot * 26 + cc] = L[j].slot;
-            else reg_cross[(size_t)i * 27 + c] = 1;
+            if (j >= 0) t.nbr26[(size_t)lf.slot * 26 + (c < 13 ? c : c - 1)] = L[j].slot;
         }
         for (int f = 0; f < 6; f++) {
-            const int cc = face_code(f) < 13 ? face_code(f) : face_code(f) - 1;
-            t.fnbr[(size_t)lf.slot * 6 + f] = t.nbr26[(size_t)lf.slot * 26 + cc];
+            const int c = face_codes[f];
+            const int nb = t.nbr26[(size_t)lf.slot * 26 + (c < 13 ? c : c - 1)];
+            t.fnbr[(size_t)lf.slot * 6 + f] = nb;
+            if (nb < 0) t.faces.push_back(lf.slot * 6 + f);
         }
     }
-
-    // neighbour signatures (block ids) decide which cached rows stay valid
-    std::vector<std::vector<int>> sig(n);
-    std::vector<char> need(n, 0);
-    {
-        std::vector<std::pair<int, bool>> nb;
-        for (int i = 0; i < n; i++) {
-            g.neighbors(i, nb);
-            std::vector<int> &v = sig[i];
-            for (auto &p : nb) v.push_back(L[p.first].block_id);
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
-            auto it = t.cache.find(L[i].block_id);
-            need[i] = (it == t.cache.end() || it->second.nbr_ids != v) ? 1 : 0;
-        }
-    }
-    std::vector<int> todo;
-    for (int i = 0; i < n; i++)
-        if (need[i]) todo.push_back(i);
-    std::vector<LeafRowCache> fresh(todo.size());
-    auto work = [&](int tid, int nt) {
-        std::vector<Term> scratch, terms;
-        for (size_t q = tid; q < todo.size(); q += nt) {
-            const int i = todo[q];
-            LeafRowCache &out = fresh[q];
-            for (int c = 0; c < 27; c++) {
-                if (c == 13 || !reg_cross[(size_t)i * 27 + c]) continue;
-                const Region r = region_of(c, s0, s1);
-                for (int z = r.s[2]; z < r.e[2]; z++)
-                    for (int y = r.s[1]; y < r.e[1]; y++)
-                        for (int x = r.s[0]; x < r.e[0]; x++) {
-                            g.expand_point(i, x, y, z, scratch, terms);
-                            out.counts.push_back((int)terms.size());
-                            for (const Term &tm : terms) {
-                                out.src.push_back(L[tm.slot].slot * BS3 + tm.cell);
-                                out.w.push_back(tm.w);
-                            }
-                        }
-            }
-        }
-    };
-    int nt = std::max(1, std::min(nthreads, (int)todo.size()));
-    if (nt <= 1) work(0, 1);
-    else {
-        std::vector<std::thread> th;
-        std::vector<std::exception_ptr> errs(nt);
-        for (int k = 0; k < nt; k++)
-            th.emplace_back([&, k]() {
-                try { work(k, nt); } catch (...) { errs[k] = std::current_exception(); }
-            });
-        for (auto &x : th) x.join();
-        for (auto &e : errs)
-            if (e) std::rethrow_exception(e);
-    }
-    for (size_t q = 0; q < todo.size(); q++) {
-        const int i = todo[q];
-        fresh[q].nbr_ids = std::move(sig[i]);
-        t.cache[L[i].block_id] = std::move(fresh[q]);
-    }
-    t.last_expanded = (int)todo.size();
-    t.last_reused = n - (int)todo.size();
-    // drop cache entries of leaves that no longer exist
-    {
-        std::unordered_set<int> live;
-        for (const Leaf &lf : L) live.insert(lf.block_id);
-        for (auto it = t.cache.begin(); it != t.cache.end();)
-            it = live.count(it->first) ? std::next(it) : t.cache.erase(it);
-    }
-
-    // concatenate in leaf order; face rows are the lab rows of the 1-cell halo
-    t.face_rows.clear();
-    t.lab_rows.clear();
-    size_t tot = 0;
-    for (int i = 0; i < n; i++) tot += t.cache[L[i].block_id].src.size();
-    t.lab_rows.src.reserve(tot);
-    t.lab_rows.w.reserve(tot);
-    for (int i = 0; i < n; i++) {
-        const Leaf &lf = L[i];
-        const LeafRowCache &lc = t.cache[lf.block_id];
-        t.lab_row_begin[lf.slot] = t.lab_rows.nrows();
-        int k = 0;
-        size_t off = 0;
-        for (int c = 0; c < 27; c++) {
-            if (c == 13 || !reg_cross[(size_t)i * 27 + c]) continue;
-            const int cc = c < 13 ? c : c - 1;
-            const int row0 = t.lab_rows.nrows();
-            t.rbase26[(size_t)lf.slot * 26 + cc] = row0;
-            const Region r = region_of(c, s0, s1);
-            const int npts = (r.e[0] - r.s[0]) * (r.e[1] - r.s[1]) * (r.e[2] - r.s[2]);
-            for (int q0 = 0; q0 < npts; q0++, k++) {
-                const int cnt = lc.counts[k];
-                t.lab_rows.src.insert(t.lab_rows.src.end(), lc.src.begin() + off, lc.src.begin() + off + cnt);
-                t.lab_rows.w.insert(t.lab_rows.w.end(), lc.w.begin() + off, lc.w.begin() + off + cnt);
-                off += cnt;
-                t.lab_rows.ptr.push_back((int)t.lab_rows.src.size());
-            }
-            // a cross-level face region: its 1-cell-halo plane becomes the step rows
-            int f = -1;
-            for (int ff = 0; ff < 6; ff++)
-                if (face_code(ff) == c) f = ff;
-            if (f >= 0) {
-                t.fbase[(size_t)lf.slot * 6 + f] = t.face_rows.nrows();
-                const int ex = r.e[0] - r.s[0], ey = r.e[1] - r.s[1];
-                for (int v = 0; v < BS; v++)
-                    for (int u = 0; u < BS; u++) {
-                        int x, y, z;
-                        switch (f) {
-                        case 0: x = -1; y = u; z = v; break;
-                        case 1: x = BS; y = u; z = v; break;
-                        case 2: x = u; y = -1; z = v; break;
-                        case 3: x = u; y = BS; z = v; break;
-                        case 4: x = u; y = v; z = -1; break;
-                        default: x = u; y = v; z = BS; break;
-                        }
-                        const int lr = row0 + (x - r.s[0]) + ex * ((y - r.s[1]) + ey * (z - r.s[2]));
-                        for (int q = t.lab_rows.ptr[lr]; q < t.lab_rows.ptr[lr + 1]; q++) {
-                            t.face_rows.src.push_back(t.lab_rows.src[q]);
-                            t.face_rows.w.push_back(t.lab_rows.w[q]);
-                        }
-                        t.face_rows.ptr.push_back((int)t.face_rows.src.size());
-                    }
-            }
-        }
-        t.lab_row_end[lf.slot] = t.lab_rows.nrows();
-    }
-    if (t.face_rows.src.size() > (size_t)INT32_MAX || t.lab_rows.src.size() > (size_t)INT32_MAX)
-        throw std::runtime_error("ghost tables exceed 2^31 terms");
-
+    const std::vector<int> &gl = g.lut();
+    t.lut.resize(gl.size());
+    for (size_t k = 0; k < gl.size(); k++) t.lut[k] = L[gl[k]].slot;
     // base-level look-up for the uniform dump / likelihood
     const int bpd = g.blocks_per_dim();
     const int bl = g.base_level();
@@ -499,8 +345,7 @@ void build_tables(const Grid &g, int nslots, Tables &t, int nthreads)
         for (int y = 0; y < bpd; y++)
             for (int x = 0; x < bpd; x++) {
                 const int64_t p[3] = {(int64_t)x * BS, (int64_t)y * BS, (int64_t)z * BS};
-                const int li = g.lying_leaf(bl, p);
-                t.lut_slot[(size_t)x + bpd * ((size_t)y + bpd * z)] = L[li].slot;
+                t.lut_slot[(size_t)x + bpd * ((size_t)y + bpd * z)] = L[g.lying_leaf(bl, p)].slot;
             }
 }
 

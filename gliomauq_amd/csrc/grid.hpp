@@ -63,6 +63,7 @@ public:
     int lut_level() const { return lut_level_; }
 
     const std::vector<Leaf> &leaves() const { return leaves_; }
+    const std::vector<int> &lut() const { return lut_; }  // finest block cells -> leaf index
     std::vector<Leaf> &leaves_mut() { return leaves_; }
 
     // Wide lab stencil [s0, s1) of this wavelet (grid max stencil,
@@ -105,36 +106,21 @@ inline uint64_t leaf_key(int l, int x, int y, int z)
     return ((uint64_t)l << 60) | ((uint64_t)(uint32_t)x << 40) | ((uint64_t)(uint32_t)y << 20) | (uint64_t)(uint32_t)z;
 }
 
-// Per-slot device tables derived from a Grid.
-// Lab rows of one leaf, reusable while the leaf and its neighbours (by block
-// id) are unchanged.
-struct LeafRowCache {
-    std::vector<int> nbr_ids;       // sorted block ids of the neighbours
-    std::vector<int> counts;        // terms per row, regions in code order, points z,y,x
-    std::vector<int> src;
-    std::vector<double> w;
-};
-
+// Per-slot device tables derived from a Grid.  Ghost values across levels
+// are evaluated on the device (depth-first over the finest-level look-up);
+// the host only provides neighbour slots and the look-ups.
 struct Tables {
     int nslots = 0;                 // table capacity in slots
-    // step: halo-1 faces, f = 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z
-    std::vector<int> fnbr;          // [nslots*6] same-level neighbour slot or -1
-    std::vector<int> fbase;         // [nslots*6] first row of a cross-level face or -1
-    Rows face_rows;                 // 256 rows per cross-level face, row = u + 16 v
-    // wide lab [s0,s1): 26 regions, c = (dx+1) + 3(dy+1) + 9(dz+1)
+    // f = 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z; same-level face neighbour slot or -1
+    std::vector<int> fnbr;          // [nslots*6]
+    // wide lab regions c = (dx+1) + 3(dy+1) + 9(dz+1) (26 without the centre)
     std::vector<int> nbr26;         // [nslots*26]
-    std::vector<int> rbase26;       // [nslots*26]
-    Rows lab_rows;
-    std::vector<int> lab_row_begin; // [nslots] rows of a slot: [begin, end)
-    std::vector<int> lab_row_end;
-    // per-leaf row cache (key: block id) and the counts of the last build
-    std::unordered_map<int, LeafRowCache> cache;
-    int last_expanded = 0, last_reused = 0;
-    // finest-block look-up for the dump / likelihood
-    std::vector<int> lut_slot;      // (bpd)^3 at the base level -> slot
+    std::vector<int> faces;         // slot*6+f of every cross-level face
+    std::vector<int> lut;           // (2^lut_level)^3 finest block cells -> slot
+    std::vector<int> lut_slot;      // (bpd)^3 base-level blocks -> slot (uniform view)
 };
 
-void build_tables(const Grid &g, int nslots, Tables &t, int nthreads);
+void build_tables(const Grid &g, int nslots, Tables &t);
 
 // Morton order of the leaves (for the launch work list).
 std::vector<int> morton_order(const Grid &g);

@@ -233,61 +233,246 @@ __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw,
 }
 
 // ---------------------------------------------------------------------------
-// Wide lab fill (FP64, canonical row order) shared by FWT and split.
+// Ghost points across levels, evaluated on the device.
+//
+// A point p (global index at level lam, vertex-centred) takes the value MRAG's
+// ghost machinery assigns it (MRAG_BBInfoCreator.inl, MRAG_SmartBlockFinder.h):
+// the leaf whose region holds it is looked up; same level -> its value;
+// finer leaf -> the level lam+1 point 2p (Ha = {1}); coarser leaf ->
+// sum_i Hs(2i - p) * value(lam-1, i), tensor product, zero weights skipped.
+// The recursion is walked depth-first with the synthesis children in
+// increasing (z, y, x) order and child weight w * ((hx*hy)*hz); values are
+// accumulated acc + w*v in that order (FP64 round-to-nearest, no
+// contraction), the same order as the CPU oracle.
 // ---------------------------------------------------------------------------
 namespace {
+constexpr int MAXD = 9;
+
+__device__ __forceinline__ double hs_w(int order, int i)
+{
+    if (order == 2) return (i < -1 || i > 1) ? 0.0 : (i == 0 ? 1.0 : 0.5);
+    switch (i) {
+    case -3: case 3: return -1.0 / 16.0;
+    case -1: case 1: return 9.0 / 16.0;
+    case 0: return 1.0;
+    default: return 0.0;
+    }
+}
+
+__device__ __forceinline__ int lying_slot(const LabArgs &g, int lam, int64_t px, int64_t py, int64_t pz)
+{
+    // finest block cell: floor(p * 2^(L-lam) / 16), periodic (power-of-two wrap)
+    const int sh = g.lut_level - lam;
+    const int64_t m = (1LL << g.lut_level) - 1;
+    const int64_t fx = ((px << sh) >> 4) & m, fy = ((py << sh) >> 4) & m, fz = ((pz << sh) >> 4) & m;
+    return g.lut[fx + (m + 1) * (fy + (m + 1) * fz)];
+}
+
+struct DfsFrame {
+    int lam;
+    int64_t p[3];
+    double w;
+    int64_t s[3], e[3], i[3];
+};
+
+// mode 0: accumulate values of fld; 1: count terms; 2: emit terms
+template <int MODE>
+__device__ double ghost_dfs(const LabArgs &g, const float *__restrict__ fld, int lam0, int64_t p0x, int64_t p0y,
+                            int64_t p0z, int *count, int *src, float *wout)
+{
+    const int lo = g.order == 4 ? -3 : -1, hi = g.order == 4 ? 4 : 2;
+    DfsFrame st[MAXD];
+    int sp = 0;
+    double acc = 0.0;
+    int n = 0;
+    int lam = lam0;
+    int64_t p[3] = {p0x, p0y, p0z};
+    double w = 1.0;
+    while (true) {
+        // resolve (lam, p, w) to a value or a synthesis frame
+        while (true) {
+            const int slot = lying_slot(g, lam, p[0], p[1], p[2]);
+            const int l = g.level[slot];
+            if (l == lam) {
+                const int64_t msk = ((int64_t)16 << lam) - 1;
+                const int cx = (int)((p[0] & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
+                const int cy = (int)((p[1] & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
+                const int cz = (int)((p[2] & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
+                const int cell = cx + 16 * (cy + 16 * cz);
+                if (MODE == 0) acc = __dadd_rn(acc, __dmul_rn(w, (double)fld[(size_t)slot * 4096 + cell]));
+                else if (MODE == 2) {
+                    src[n] = slot * 4096 + cell;
+                    wout[n] = (float)w;
+                }
+                n++;
+                break;
+            }
+            if (l > lam) {  // finer leaf: analysis
+                lam++;
+                p[0] *= 2;
+                p[1] *= 2;
+                p[2] *= 2;
+                continue;
+            }
+            // coarser leaf: synthesis frame (ceil((p+lo)/2) .. floor((p+hi-1)/2))
+            if (sp >= MAXD) return acc;  // unreachable for 2:1 balanced grids
+            DfsFrame &f = st[sp++];
+            f.lam = lam;
+            f.w = w;
+            for (int d = 0; d < 3; d++) {
+                f.p[d] = p[d];
+                f.s[d] = (p[d] + lo + 1) >> 1;       // ceil((p+lo)/2) with arithmetic shift
+                f.e[d] = ((p[d] + hi - 1) >> 1) + 1; // 1 + floor((p+hi-1)/2)
+                f.i[d] = f.s[d];
+            }
+            f.i[0] = f.s[0] - 1;  // before the first child
+            break;
+        }
+        // next child with a non-zero weight
+        bool found = false;
+        while (sp > 0 && !found) {
+            DfsFrame &f = st[sp - 1];
+            while (true) {
+                if (++f.i[0] >= f.e[0]) {
+                    f.i[0] = f.s[0];
+                    if (++f.i[1] >= f.e[1]) {
+                        f.i[1] = f.s[1];
+                        if (++f.i[2] >= f.e[2]) break;
+                    }
+                }
+                const double ww = __dmul_rn(__dmul_rn(hs_w(g.order, (int)(2 * f.i[0] - f.p[0])),
+                                                      hs_w(g.order, (int)(2 * f.i[1] - f.p[1]))),
+                                            hs_w(g.order, (int)(2 * f.i[2] - f.p[2])));
+                if (ww == 0.0) continue;
+                lam = f.lam - 1;
+                p[0] = f.i[0];
+                p[1] = f.i[1];
+                p[2] = f.i[2];
+                w = __dmul_rn(f.w, ww);
+                found = true;
+                break;
+            }
+            if (!found) sp--;
+        }
+        if (!found) break;
+    }
+    if (MODE == 1) *count = n;
+    return acc;
+}
+
 template <int S0, int S1>
 struct LabGeom {
     static constexpr int L = B + S1 - S0 - 1;
     static constexpr int N = L * L * L;
 };
 
+// lab point q -> (x,y,z) and its region (cc = -1 inside the block)
+template <int S0, int S1>
+__device__ __forceinline__ int lab_region(int q, int &x, int &y, int &z, int &dx, int &dy, int &dz)
+{
+    constexpr int L = LabGeom<S0, S1>::L;
+    x = q % L + S0;
+    y = (q / L) % L + S0;
+    z = q / (L * L) + S0;
+    dx = x < 0 ? -1 : (x < B ? 0 : 1);
+    dy = y < 0 ? -1 : (y < B ? 0 : 1);
+    dz = z < 0 ? -1 : (z < B ? 0 : 1);
+    if ((dx | dy | dz) == 0) return -1;
+    const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+    return c < 13 ? c : c - 1;
+}
+
 template <int S0, int S1>
 __device__ void fill_lab(double *lab, const float *__restrict__ fld, const double *__restrict__ ghost, int slot,
                          const LabArgs &a)
 {
-    constexpr int L = LabGeom<S0, S1>::L;
     constexpr int N = LabGeom<S0, S1>::N;
     for (int q = threadIdx.x; q < N; q += blockDim.x) {
-        const int x = q % L + S0, y = (q / L) % L + S0, z = q / (L * L) + S0;
+        int x, y, z, dx, dy, dz;
+        const int cc = lab_region<S0, S1>(q, x, y, z, dx, dy, dz);
         double v;
-        const int dx = x < 0 ? -1 : (x < B ? 0 : 1);
-        const int dy = y < 0 ? -1 : (y < B ? 0 : 1);
-        const int dz = z < 0 ? -1 : (z < B ? 0 : 1);
-        if ((dx | dy | dz) == 0) {
+        if (cc < 0) {
             v = (double)fld[(size_t)slot * B3 + x + B * (y + B * z)];
         } else {
-            const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-            const int cc = c < 13 ? c : c - 1;
             const int nb = a.nbr26[slot * 26 + cc];
             if (nb >= 0) {
                 const int lx = x - dx * B, ly = y - dy * B, lz = z - dz * B;
                 v = (double)fld[(size_t)nb * B3 + lx + B * (ly + B * lz)];
             } else {
-                // region extents (z,y,x order of the host builder)
-                const int sx = dx < 0 ? S0 : (dx == 0 ? 0 : B), ex = dx < 0 ? 0 : (dx == 0 ? B : B + S1 - 1);
-                const int sy = dy < 0 ? S0 : (dy == 0 ? 0 : B), ey = dy < 0 ? 0 : (dy == 0 ? B : B + S1 - 1);
-                const int sz = dz < 0 ? S0 : (dz == 0 ? 0 : B);
-                v = ghost[a.rbase26[slot * 26 + cc] + (x - sx) + (ex - sx) * ((y - sy) + (ey - sy) * (z - sz))];
+                v = ghost[q];
             }
         }
         lab[q] = v;
     }
 }
+
+template <int S0, int S1>
+__device__ void lab_ghosts_block(const LabArgs &a, const float *__restrict__ fld, int slot, double *__restrict__ out)
+{
+    constexpr int N = LabGeom<S0, S1>::N;
+    const int lam = a.level[slot];
+    const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
+                  oz = (int64_t)a.index[slot * 3 + 2] * B;
+    for (int q = threadIdx.x; q < N; q += blockDim.x) {
+        int x, y, z, dx, dy, dz;
+        const int cc = lab_region<S0, S1>(q, x, y, z, dx, dy, dz);
+        if (cc < 0 || a.nbr26[slot * 26 + cc] >= 0) continue;
+        out[q] = ghost_dfs<0>(a, fld, lam, ox + x, oy + y, oz + z, nullptr, nullptr, nullptr);
+    }
+}
 }  // namespace
 
-// One workgroup per listed slot; threads evaluate its cross-level lab rows,
-// each row a sequential FP64 sum in the canonical term order.
+// One workgroup per listed slot: the cross-level points of its wide lab.
+template <int ORDER>
 __global__ __launch_bounds__(256) void k_lab_ghosts(LabArgs a, const float *__restrict__ fld,
                                                     const int *__restrict__ slots, double *__restrict__ ghost)
 {
-    const int slot = slots[blockIdx.x];
-    const int r0 = a.row_begin[slot], r1 = a.row_end[slot];
-    for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        double acc = 0.0;
-        for (int k = a.ptr[r]; k < a.ptr[r + 1]; k++) acc = __dadd_rn(acc, __dmul_rn(a.w[k], (double)fld[a.src[k]]));
-        ghost[r] = acc;
+    constexpr int S0 = ORDER == 4 ? -2 : -1;
+    constexpr int S1 = ORDER == 4 ? 4 : 2;
+    lab_ghosts_block<S0, S1>(a, fld, slots[blockIdx.x], ghost + (size_t)blockIdx.x * LabGeom<S0, S1>::N);
+}
+
+// Step rows of cross-level faces (count / fill passes).
+__device__ __forceinline__ void face_point(int f, int u, int v, int &x, int &y, int &z)
+{
+    switch (f) {
+    case 0: x = -1; y = u; z = v; break;
+    case 1: x = B; y = u; z = v; break;
+    case 2: x = u; y = -1; z = v; break;
+    case 3: x = u; y = B; z = v; break;
+    case 4: x = u; y = v; z = -1; break;
+    default: x = u; y = v; z = B; break;
     }
+}
+
+__global__ __launch_bounds__(256) void k_face_rows_count(LabArgs a, const int *__restrict__ faces, int nfaces,
+                                                         int *__restrict__ counts)
+{
+    const int k = blockIdx.x;
+    const int sf = faces[k];
+    const int slot = sf / 6, f = sf % 6;
+    int x, y, z;
+    face_point(f, threadIdx.x & 15, threadIdx.x >> 4, x, y, z);
+    const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
+                  oz = (int64_t)a.index[slot * 3 + 2] * B;
+    int n = 0;
+    ghost_dfs<1>(a, nullptr, a.level[slot], ox + x, oy + y, oz + z, &n, nullptr, nullptr);
+    counts[k * 256 + threadIdx.x] = n;
+}
+
+__global__ __launch_bounds__(256) void k_face_rows_fill(LabArgs a, const int *__restrict__ faces, int nfaces,
+                                                        const int *__restrict__ ptr, int *__restrict__ src,
+                                                        float *__restrict__ w)
+{
+    const int k = blockIdx.x;
+    const int sf = faces[k];
+    const int slot = sf / 6, f = sf % 6;
+    int x, y, z;
+    face_point(f, threadIdx.x & 15, threadIdx.x >> 4, x, y, z);
+    const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
+                  oz = (int64_t)a.index[slot * 3 + 2] * B;
+    const int r = k * 256 + threadIdx.x;
+    ghost_dfs<2>(a, nullptr, a.level[slot], ox + x, oy + y, oz + z, nullptr, src + ptr[r], w + ptr[r]);
 }
 
 namespace {
@@ -359,7 +544,7 @@ __global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const double *__r
     __shared__ double lab[LabGeom<S0, S1>::N];
     __shared__ double red[256];
     const int slot = cand[blockIdx.x];
-    fill_lab<S0, S1>(lab, phi, ghost, slot, a);
+    fill_lab<S0, S1>(lab, phi, ghost + (size_t)blockIdx.x * LabGeom<S0, S1>::N, slot, a);
     __syncthreads();
     double m = 0.0;
     for (int q = threadIdx.x; q < 512; q += blockDim.x) {
@@ -391,8 +576,8 @@ __global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const double *__r
 // point d = sum over parent lab points s (z, y, x increasing) of
 // lab(s) * ((wZ*wY)*wX), w = Hs(2s - d), for phi, pw and pg.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restrict__ ghost3, int nrows, float *phi,
-                                               float *pw, float *pg, const int *__restrict__ parents,
+__global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restrict__ ghost3, float *phi, float *pw,
+                                               float *pg, const int *__restrict__ parents,
                                                const int *__restrict__ children)
 {
     constexpr int S0 = ORDER == 4 ? -2 : -1;
@@ -402,7 +587,8 @@ __global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restri
     const int pslot = parents[blockIdx.x];
     float *fields[3] = {phi, pw, pg};
     for (int f = 0; f < 3; f++) {
-        fill_lab<S0, S1>(lab, fields[f], ghost3 + (size_t)f * nrows, pslot, a);
+        fill_lab<S0, S1>(lab, fields[f], ghost3 + ((size_t)f * gridDim.x + blockIdx.x) * LabGeom<S0, S1>::N, pslot,
+                         a);
         __syncthreads();
         for (int q = threadIdx.x; q < 8 * B3; q += blockDim.x) {
             const int dx = q & 31, dy = (q >> 5) & 31, dz = q >> 10;
@@ -649,11 +835,14 @@ hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, i
     return hipGetLastError();
 }
 
+int lab_points(int order) { return order == 4 ? 21 * 21 * 21 : 18 * 18 * 18; }
+
 hipError_t launch_lab_ghosts(const LabArgs &a, const float *fld, const int *slots, int nslots, double *ghost,
                              hipStream_t st)
 {
     if (nslots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lab_ghosts, dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
+    if (a.order == 4) hipLaunchKernelGGL(k_lab_ghosts<4>, dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
+    else hipLaunchKernelGGL(k_lab_ghosts<2>, dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
     return hipGetLastError();
 }
 
@@ -666,16 +855,29 @@ hipError_t launch_fwt_detail(int order, const LabArgs &a, const double *ghost, c
     return hipGetLastError();
 }
 
-hipError_t launch_split(int order, const LabArgs &a, const double *ghost3, int nrows, float *phi, float *pw,
-                        float *pg, const int *parents, const int *children, int nparents, hipStream_t st)
+hipError_t launch_split(int order, const LabArgs &a, const double *ghost3, float *phi, float *pw, float *pg,
+                        const int *parents, const int *children, int nparents, hipStream_t st)
 {
     if (nparents <= 0) return hipSuccess;
     if (order == 4)
-        hipLaunchKernelGGL(k_split<4>, dim3(nparents), dim3(256), 0, st, a, ghost3, nrows, phi, pw, pg, parents,
-                           children);
+        hipLaunchKernelGGL(k_split<4>, dim3(nparents), dim3(256), 0, st, a, ghost3, phi, pw, pg, parents, children);
     else
-        hipLaunchKernelGGL(k_split<2>, dim3(nparents), dim3(256), 0, st, a, ghost3, nrows, phi, pw, pg, parents,
-                           children);
+        hipLaunchKernelGGL(k_split<2>, dim3(nparents), dim3(256), 0, st, a, ghost3, phi, pw, pg, parents, children);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_rows_count(const LabArgs &a, const int *faces, int nfaces, int *counts, hipStream_t st)
+{
+    if (nfaces <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_face_rows_count, dim3(nfaces), dim3(256), 0, st, a, faces, nfaces, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_rows_fill(const LabArgs &a, const int *faces, int nfaces, const int *ptr, int *src, float *w,
+                                 hipStream_t st)
+{
+    if (nfaces <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_face_rows_fill, dim3(nfaces), dim3(256), 0, st, a, faces, nfaces, ptr, src, w);
     return hipGetLastError();
 }
 

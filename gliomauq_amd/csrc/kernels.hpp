@@ -23,14 +23,15 @@ struct StepArgs {
     float Dw, Dg, rho, dt;
 };
 
+// Location tables for ghost evaluation on the device: the leaf covering a
+// point is found through a look-up over the finest level.
 struct LabArgs {
-    const int *nbr26;
-    const int *rbase26;
-    const int *ptr;
-    const int *src;
-    const double *w;
-    const int *row_begin;  // [slot] lab rows of a slot: [row_begin, row_end)
-    const int *row_end;
+    const int *nbr26;  // [slot*26] same-level neighbour slot per region or -1
+    const int *lut;    // (2^lut_level)^3 finest block cells -> slot
+    const int *level;  // [slot]
+    const int *index;  // [slot*3]
+    int lut_level;
+    int order;         // 2 or 4
 };
 
 struct UniformArgs {
@@ -50,15 +51,21 @@ hipError_t launch_face_ghosts(const float *phi, const int *ptr, const int *src, 
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
                             const int *fbase, const int *ptr, const int *src, const float *w, float *fik, float Dw,
                             float Dg, hipStream_t st);
-// Evaluate the cross-level lab rows of the listed slots for one field (FP64,
-// canonical row order) into ghost[row].
+// Ghost values of the cross-level points of the wide lab of each listed slot
+// (FP64, depth-first order) into ghost[k * lab_points + q].
 hipError_t launch_lab_ghosts(const LabArgs &a, const float *fld, const int *slots, int nslots, double *ghost,
                              hipStream_t st);
+int lab_points(int order);  // points of the wide lab (Interp4: 21^3, Interp2: 18^3)
 hipError_t launch_fwt_detail(int order, const LabArgs &a, const double *ghost, const float *phi, const int *cand,
                              int ncand, double *out, hipStream_t st);
-// ghost3: three ghost arrays (phi, pw, pg) of nrows each
-hipError_t launch_split(int order, const LabArgs &a, const double *ghost3, int nrows, float *phi, float *pw,
-                        float *pg, const int *parents, const int *children, int nparents, hipStream_t st);
+// ghost3: ghost arrays of phi, pw, pg, each nparents * lab_points
+hipError_t launch_split(int order, const LabArgs &a, const double *ghost3, float *phi, float *pw, float *pg,
+                        const int *parents, const int *children, int nparents, hipStream_t st);
+// Step rows of the cross-level faces: faces[k] = slot*6+f; rows k*256 + u + 16v.
+// Pass 1 counts the depth-first terms of every row, pass 2 writes them.
+hipError_t launch_face_rows_count(const LabArgs &a, const int *faces, int nfaces, int *counts, hipStream_t st);
+hipError_t launch_face_rows_fill(const LabArgs &a, const int *faces, int nfaces, const int *ptr, int *src, float *w,
+                                 hipStream_t st);
 hipError_t launch_collapse(float *phi, float *pw, float *pg, const int *parents, const int *children, int nparents,
                            hipStream_t st);
 hipError_t launch_dump_uniform(const UniformArgs &u, double *out, hipStream_t st);

@@ -468,8 +468,11 @@ int og_point_terms(int n_leaves, int max_level, int B, const int *level, const i
 }
 
 /* Fill a ghosted lab (stencil [s0, s1) in every dim: size B+s1-s0-1) of one
- * field for one leaf, FP64 accumulation in canonical order. data is
- * [n_leaves][B^3] (double). Returns 0, or -1 on overflow. */
+ * field for one leaf.  A ghost value is the FP64 sum acc + w*v over the
+ * expansion terms in depth-first order (synthesis children in increasing
+ * z, y, x; duplicates not merged) -- the order the device kernels use, so
+ * both produce the same bits.  data is [n_leaves][B^3] (double).  Returns 0,
+ * or -1 on overflow. */
 int og_fill_lab_f64(int n_leaves, int max_level, int B, const int *level, const int *index, const int *lut,
                     int order, const double *data, int leaf, int s0, int s1, double *lab)
 {
@@ -491,11 +494,10 @@ int og_fill_lab_f64(int n_leaves, int max_level, int B, const int *level, const 
                     const long p[3] = {(long)index[3 * leaf] * B + x, (long)index[3 * leaf + 1] * B + y,
                                        (long)index[3 * leaf + 2] * B + z};
                     og_expand(&g, lam, p, 1.0, &T, 0);
-                    const int m = og_canonical(&T);
-                    if (m < 0) { rc = -1; v = 0; }
+                    if (T.overflow) { rc = -1; v = 0; }
                     else {
                         v = 0.0;
-                        for (int k = 0; k < m; k++) v += buf[k].w * data[buf[k].leaf * B3 + buf[k].cell];
+                        for (int k = 0; k < T.n; k++) v += buf[k].w * data[buf[k].leaf * B3 + buf[k].cell];
                     }
                 }
                 lab[(x - s0) + L * ((y - s0) + L * (z - s0))] = v;
@@ -544,9 +546,8 @@ int fk_adaptive_steps_f64(int n_leaves, int max_level, int B, const int *level, 
                                     const long p[3] = {(long)index[3 * b] * B + x, (long)index[3 * b + 1] * B + y,
                                                        (long)index[3 * b + 2] * B + z};
                                     og_expand(&g, level[b], p, 1.0, &T, 0);
-                                    const int m = og_canonical(&T);
-                                    if (m < 0) rc = -1;
-                                    for (int k = 0; k < m; k++) v += buf[k].w * fields[f][buf[k].leaf * B3 + buf[k].cell];
+                                    if (T.overflow) rc = -1;
+                                    for (int k = 0; k < T.n; k++) v += buf[k].w * fields[f][buf[k].leaf * B3 + buf[k].cell];
                                 }
                                 lf[(x + 1) + L * ((y + 1) + L * (z + 1))] = v;
                             }
