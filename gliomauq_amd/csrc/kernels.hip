@@ -275,19 +275,19 @@ struct DfsFrame {
     int64_t s[3], e[3], i[3];
 };
 
-// mode 0: accumulate values of fld; 1: count terms; 2: emit terms
+// Depth-first walk from (lam0, p0, w0).  mode 0: acc += w*v over the leaf
+// values of fld; 1: count terms (added to *count); 2: emit terms at src/wout
+// (advancing *count).
 template <int MODE>
-__device__ double ghost_dfs(const LabArgs &g, const float *__restrict__ fld, int lam0, int64_t p0x, int64_t p0y,
-                            int64_t p0z, int *count, int *src, float *wout)
+__device__ void ghost_dfs(const LabArgs &g, const float *__restrict__ fld, int lam0, int64_t p0x, int64_t p0y,
+                          int64_t p0z, double w0, double &acc, int *count, int *src, float *wout)
 {
     const int lo = g.order == 4 ? -3 : -1, hi = g.order == 4 ? 4 : 2;
     DfsFrame st[MAXD];
     int sp = 0;
-    double acc = 0.0;
-    int n = 0;
     int lam = lam0;
     int64_t p[3] = {p0x, p0y, p0z};
-    double w = 1.0;
+    double w = w0;
     while (true) {
         // resolve (lam, p, w) to a value or a synthesis frame
         while (true) {
@@ -301,10 +301,10 @@ __device__ double ghost_dfs(const LabArgs &g, const float *__restrict__ fld, int
                 const int cell = cx + 16 * (cy + 16 * cz);
                 if (MODE == 0) acc = __dadd_rn(acc, __dmul_rn(w, (double)fld[(size_t)slot * 4096 + cell]));
                 else if (MODE == 2) {
-                    src[n] = slot * 4096 + cell;
-                    wout[n] = (float)w;
+                    src[*count] = slot * 4096 + cell;
+                    wout[*count] = (float)w;
                 }
-                n++;
+                if (MODE != 0) (*count)++;
                 break;
             }
             if (l > lam) {  // finer leaf: analysis
@@ -315,7 +315,7 @@ __device__ double ghost_dfs(const LabArgs &g, const float *__restrict__ fld, int
                 continue;
             }
             // coarser leaf: synthesis frame (ceil((p+lo)/2) .. floor((p+hi-1)/2))
-            if (sp >= MAXD) return acc;  // unreachable for 2:1 balanced grids
+            if (sp >= MAXD) return;  // unreachable for 2:1 balanced grids
             DfsFrame &f = st[sp++];
             f.lam = lam;
             f.w = w;
@@ -356,7 +356,64 @@ __device__ double ghost_dfs(const LabArgs &g, const float *__restrict__ fld, int
         }
         if (!found) break;
     }
-    if (MODE == 1) *count = n;
+}
+
+// Value of one ghost point: the same walk with a stack-free fast path for
+// the common cases (the point lies in a finer leaf, or in a coarser leaf
+// whose synthesis children all lie in leaves of their own level); anything
+// else continues with the general walk at the child, in the same order.
+__device__ double ghost_value(const LabArgs &g, const float *__restrict__ fld, int lam, int64_t px, int64_t py,
+                              int64_t pz)
+{
+    for (;;) {
+        const int slot = lying_slot(g, lam, px, py, pz);
+        const int l = g.level[slot];
+        if (l == lam) {
+            const int64_t msk = ((int64_t)16 << lam) - 1;
+            const int cx = (int)((px & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
+            const int cy = (int)((py & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
+            const int cz = (int)((pz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
+            return __dmul_rn(1.0, (double)fld[(size_t)slot * 4096 + cx + 16 * (cy + 16 * cz)]);
+        }
+        if (l > lam) {
+            lam++;
+            px *= 2;
+            py *= 2;
+            pz *= 2;
+            continue;
+        }
+        break;
+    }
+    const int lo = g.order == 4 ? -3 : -1, hi = g.order == 4 ? 4 : 2;
+    const int64_t p[3] = {px, py, pz};
+    int64_t s[3], e[3];
+    for (int d = 0; d < 3; d++) {
+        s[d] = (p[d] + lo + 1) >> 1;
+        e[d] = ((p[d] + hi - 1) >> 1) + 1;
+    }
+    const int cl = lam - 1;
+    const int64_t msk = ((int64_t)16 << cl) - 1;
+    double acc = 0.0;
+    for (int64_t iz = s[2]; iz < e[2]; iz++) {
+        const double wz = hs_w(g.order, (int)(2 * iz - p[2]));
+        for (int64_t iy = s[1]; iy < e[1]; iy++) {
+            const double wy = hs_w(g.order, (int)(2 * iy - p[1]));
+            for (int64_t ix = s[0]; ix < e[0]; ix++) {
+                const double ww = __dmul_rn(__dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), wy), wz);
+                if (ww == 0.0) continue;
+                const double cw = __dmul_rn(1.0, ww);
+                const int slot = lying_slot(g, cl, ix, iy, iz);
+                if (g.level[slot] == cl) {
+                    const int cx = (int)((ix & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
+                    const int cy = (int)((iy & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
+                    const int cz = (int)((iz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
+                    acc = __dadd_rn(acc, __dmul_rn(cw, (double)fld[(size_t)slot * 4096 + cx + 16 * (cy + 16 * cz)]));
+                } else {
+                    ghost_dfs<0>(g, fld, cl, ix, iy, iz, cw, acc, nullptr, nullptr, nullptr);
+                }
+            }
+        }
+    }
     return acc;
 }
 
@@ -409,15 +466,24 @@ __device__ void fill_lab(double *lab, const float *__restrict__ fld, const doubl
 template <int S0, int S1>
 __device__ void lab_ghosts_block(const LabArgs &a, const float *__restrict__ fld, int slot, double *__restrict__ out)
 {
-    constexpr int N = LabGeom<S0, S1>::N;
+    constexpr int L = LabGeom<S0, S1>::L;
     const int lam = a.level[slot];
     const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
-    for (int q = threadIdx.x; q < N; q += blockDim.x) {
-        int x, y, z, dx, dy, dz;
-        const int cc = lab_region<S0, S1>(q, x, y, z, dx, dy, dz);
-        if (cc < 0 || a.nbr26[slot * 26 + cc] >= 0) continue;
-        out[q] = ghost_dfs<0>(a, fld, lam, ox + x, oy + y, oz + z, nullptr, nullptr, nullptr);
+    for (int cc = 0; cc < 26; cc++) {
+        if (a.nbr26[slot * 26 + cc] >= 0) continue;  // same-level region: copied by fill_lab
+        const int c = cc < 13 ? cc : cc + 1;
+        const int d[3] = {c % 3 - 1, (c / 3) % 3 - 1, c / 9 - 1};
+        int rs[3], rn[3];
+        for (int k = 0; k < 3; k++) {
+            rs[k] = d[k] < 0 ? S0 : (d[k] == 0 ? 0 : B);
+            rn[k] = d[k] < 0 ? -S0 : (d[k] == 0 ? B : S1 - 1);
+        }
+        const int np = rn[0] * rn[1] * rn[2];
+        for (int q = threadIdx.x; q < np; q += blockDim.x) {
+            const int x = rs[0] + q % rn[0], y = rs[1] + (q / rn[0]) % rn[1], z = rs[2] + q / (rn[0] * rn[1]);
+            out[(x - S0) + L * ((y - S0) + L * (z - S0))] = ghost_value(a, fld, lam, ox + x, oy + y, oz + z);
+        }
     }
 }
 }  // namespace
@@ -456,7 +522,8 @@ __global__ __launch_bounds__(256) void k_face_rows_count(LabArgs a, const int *_
     const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
     int n = 0;
-    ghost_dfs<1>(a, nullptr, a.level[slot], ox + x, oy + y, oz + z, &n, nullptr, nullptr);
+    double unused = 0.0;
+    ghost_dfs<1>(a, nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, unused, &n, nullptr, nullptr);
     counts[k * 256 + threadIdx.x] = n;
 }
 
@@ -472,7 +539,9 @@ __global__ __launch_bounds__(256) void k_face_rows_fill(LabArgs a, const int *__
     const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
     const int r = k * 256 + threadIdx.x;
-    ghost_dfs<2>(a, nullptr, a.level[slot], ox + x, oy + y, oz + z, nullptr, src + ptr[r], w + ptr[r]);
+    int n = 0;
+    double unused = 0.0;
+    ghost_dfs<2>(a, nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, unused, &n, src + ptr[r], w + ptr[r]);
 }
 
 namespace {

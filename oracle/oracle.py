@@ -45,6 +45,8 @@ def lib():
         L.lik_ref_f64.restype = C.c_longdouble
         L.lik_flip_f64.argtypes = [dp, u8, u8, fp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.lik_flip_f64.restype = C.c_double
+        L.adj_fk_f64.argtypes = [dp, dp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_int, dp, dp, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -184,3 +186,23 @@ def lik_flip(u, t1, t2, pet, th1=0.7, th2=0.25, eps=0.05, sigma=0.05):
                            np.ascontiguousarray(t2, np.uint8).reshape(-1),
                            np.ascontiguousarray(pet, np.float32).reshape(-1), u.size, th1, th2, eps, sigma, parts)
     return float(v), parts
+
+
+def adjoint(u0, pw, pg, h, Dw, Dg, rho, dt, n_steps, dLduN_fn):
+    """FP64 forward + discrete adjoint (adjoint_oracle.c) on a uniform periodic
+    grid.  dLduN_fn(uN) -> cotangent at the final state.  Returns (uN, dL/du0,
+    dL/dlogDw, dL/dlogrho); the forward is run twice (once to get uN)."""
+    N = u0.shape[0]
+    a = [np.ascontiguousarray(x, np.float64) for x in (pw, pg, u0)]
+    uN = np.zeros_like(a[2])
+    d0 = np.zeros_like(a[2])
+    g = np.zeros(2)
+    zero = np.zeros_like(a[2])
+    rc = lib().adj_fk_f64(a[0], a[1], N, h, Dw, Dg, rho, dt, n_steps, a[2], zero, uN, d0, g)
+    if rc != 0:
+        raise MemoryError("adjoint oracle allocation failed")
+    ct = np.ascontiguousarray(dLduN_fn(uN), np.float64)
+    rc = lib().adj_fk_f64(a[0], a[1], N, h, Dw, Dg, rho, dt, n_steps, a[2], ct, uN, d0, g)
+    if rc != 0:
+        raise MemoryError("adjoint oracle allocation failed")
+    return uN, d0, g[0], g[1]
