@@ -309,7 +309,7 @@ struct gq_ctx {
         d_dbl.reserve(slots.size());
         const gq::LabArgs la = lab_args();
         d_ghost.reserve(slots.size() * (size_t)gq::lab_points(grid->order()));
-        HIPCHK(gq::launch_lab_ghosts(la, phi[cur], d_list.p, (int)slots.size(), d_ghost.p, st));
+        HIPCHK(gq::launch_lab_ghosts(la, phi[cur], d_list.p, (int)slots.size(), d_ghost.p, false, st));
         HIPCHK(gq::launch_fwt_detail(grid->order(), la, d_ghost.p, phi[cur], d_list.p, (int)slots.size(), d_dbl.p,
                                      st));
         HIPCHK(hipMemcpyAsync(out.data(), d_dbl.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -354,7 +354,7 @@ struct gq_ctx {
         d_ghost.reserve(3 * per);
         const float *fl[3] = {phi[cur], pw, pg};
         for (int f = 0; f < 3; f++)
-            HIPCHK(gq::launch_lab_ghosts(la, fl[f], d_list.p, (int)pslots.size(), d_ghost.p + f * per, st));
+            HIPCHK(gq::launch_lab_ghosts(la, fl[f], d_list.p, (int)pslots.size(), d_ghost.p + f * per, true, st));
         HIPCHK(gq::launch_split(grid->order(), la, d_ghost.p, phi[cur], pw, pg, d_list.p, d_list2.p,
                                 (int)pslots.size(), st));
         HIPCHK(hipStreamSynchronize(st));

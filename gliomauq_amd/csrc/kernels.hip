@@ -463,7 +463,18 @@ __device__ void fill_lab(double *lab, const float *__restrict__ fld, const doubl
     }
 }
 
-template <int S0, int S1>
+// Lab coordinates a consumer reads outside the block: the FWT filters touch
+// only {-2, 16, 18} (Interp4; the taps at odd offsets -3 and +1 are zero) or
+// {16} (Interp2); the split reads [-1, 17] (Interp4) or [0, 16] (Interp2).
+template <int ORDER, bool FOR_SPLIT>
+__device__ __forceinline__ bool lab_coord_used(int v)
+{
+    if (v >= 0 && v < B) return true;
+    if (FOR_SPLIT) return ORDER == 4 ? (v >= -1 && v <= B + 1) : (v >= 0 && v <= B);
+    return ORDER == 4 ? (v == -2 || v == B || v == B + 2) : (v == B);
+}
+
+template <int S0, int S1, int ORDER, bool FOR_SPLIT>
 __device__ void lab_ghosts_block(const LabArgs &a, const float *__restrict__ fld, int slot, double *__restrict__ out)
 {
     constexpr int L = LabGeom<S0, S1>::L;
@@ -482,6 +493,9 @@ __device__ void lab_ghosts_block(const LabArgs &a, const float *__restrict__ fld
         const int np = rn[0] * rn[1] * rn[2];
         for (int q = threadIdx.x; q < np; q += blockDim.x) {
             const int x = rs[0] + q % rn[0], y = rs[1] + (q / rn[0]) % rn[1], z = rs[2] + q / (rn[0] * rn[1]);
+            if (!lab_coord_used<ORDER, FOR_SPLIT>(x) || !lab_coord_used<ORDER, FOR_SPLIT>(y) ||
+                !lab_coord_used<ORDER, FOR_SPLIT>(z))
+                continue;
             out[(x - S0) + L * ((y - S0) + L * (z - S0))] = ghost_value(a, fld, lam, ox + x, oy + y, oz + z);
         }
     }
@@ -489,13 +503,14 @@ __device__ void lab_ghosts_block(const LabArgs &a, const float *__restrict__ fld
 }  // namespace
 
 // One workgroup per listed slot: the cross-level points of its wide lab.
-template <int ORDER>
+template <int ORDER, bool FOR_SPLIT>
 __global__ __launch_bounds__(256) void k_lab_ghosts(LabArgs a, const float *__restrict__ fld,
                                                     const int *__restrict__ slots, double *__restrict__ ghost)
 {
     constexpr int S0 = ORDER == 4 ? -2 : -1;
     constexpr int S1 = ORDER == 4 ? 4 : 2;
-    lab_ghosts_block<S0, S1>(a, fld, slots[blockIdx.x], ghost + (size_t)blockIdx.x * LabGeom<S0, S1>::N);
+    lab_ghosts_block<S0, S1, ORDER, FOR_SPLIT>(a, fld, slots[blockIdx.x],
+                                               ghost + (size_t)blockIdx.x * LabGeom<S0, S1>::N);
 }
 
 // Step rows of cross-level faces (count / fill passes).
@@ -907,11 +922,16 @@ hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, i
 int lab_points(int order) { return order == 4 ? 21 * 21 * 21 : 18 * 18 * 18; }
 
 hipError_t launch_lab_ghosts(const LabArgs &a, const float *fld, const int *slots, int nslots, double *ghost,
-                             hipStream_t st)
+                             bool for_split, hipStream_t st)
 {
     if (nslots <= 0) return hipSuccess;
-    if (a.order == 4) hipLaunchKernelGGL(k_lab_ghosts<4>, dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
-    else hipLaunchKernelGGL(k_lab_ghosts<2>, dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
+    if (a.order == 4) {
+        if (for_split) hipLaunchKernelGGL((k_lab_ghosts<4, true>), dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
+        else hipLaunchKernelGGL((k_lab_ghosts<4, false>), dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
+    } else {
+        if (for_split) hipLaunchKernelGGL((k_lab_ghosts<2, true>), dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
+        else hipLaunchKernelGGL((k_lab_ghosts<2, false>), dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
+    }
     return hipGetLastError();
 }
 

@@ -53,8 +53,9 @@ hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, i
                             float Dg, hipStream_t st);
 // Ghost values of the cross-level points of the wide lab of each listed slot
 // (FP64, depth-first order) into ghost[k * lab_points + q].
+// for_split: the points the split reads; else the points the FWT reads.
 hipError_t launch_lab_ghosts(const LabArgs &a, const float *fld, const int *slots, int nslots, double *ghost,
-                             hipStream_t st);
+                             bool for_split, hipStream_t st);
 int lab_points(int order);  // points of the wide lab (Interp4: 21^3, Interp2: 18^3)
 hipError_t launch_fwt_detail(int order, const LabArgs &a, const double *ghost, const float *phi, const int *cand,
                              int ncand, double *out, hipStream_t st);
