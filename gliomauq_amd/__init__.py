@@ -5,11 +5,11 @@ include/gq_api.h); this package is its host-side mirror of the reference's
 block-operator API (see solver.py) plus the synthetic-input generator.
 """
 from . import _lib  # noqa: F401
-from .solver import (BlockFWT, BlockProcessing, Context, FlipLikelihood, Glioma_HG_UQ,  # noqa: F401
+from .solver import (BlockFWT, BlockProcessing, Context, Ensemble, FlipLikelihood, Glioma_HG_UQ,  # noqa: F401
                      Glioma_ReactionDiffusionOperator, Grid, HGG_Likelihood, Science, Topology, UpdateTumor,
                      fk_steps, n_steps_until, set_step_timing, step_timing, set_autotune, autotune_info, host_stats)
 
-__all__ = ["BlockFWT", "BlockProcessing", "Context", "FlipLikelihood", "Glioma_HG_UQ",
+__all__ = ["BlockFWT", "BlockProcessing", "Context", "Ensemble", "FlipLikelihood", "Glioma_HG_UQ",
            "Glioma_ReactionDiffusionOperator", "Grid", "HGG_Likelihood", "Science", "Topology", "UpdateTumor",
            "fk_steps", "n_steps_until", "set_step_timing", "step_timing", "set_autotune", "autotune_info",
            "host_stats"]

@@ -24,7 +24,11 @@ EXPORTS = [
     "gq_last_step_timing", "gq_set_step_timing", "gq_memory_info", "gq_set_autotune", "gq_autotune_info",
     "gq_host_stats", "gq_topo_create", "gq_topo_destroy",
     "gq_topo_leaves", "gq_topo_refine", "gq_topo_compress", "gq_topo_point_terms",
+    "gq_grid_write_volume", "gq_ensemble_create", "gq_ensemble_size", "gq_sample_select",
+    "gq_ensemble_block_counts", "gq_ensemble_run", "gq_ensemble_stats", "gq_ensemble_loglik_flip",
 ]
+
+RUN_GRAPH = 1
 
 
 class BlockInfo(C.Structure):
@@ -35,6 +39,12 @@ class BlockInfo(C.Structure):
 
 class FKParams(C.Structure):
     _fields_ = [("Dw", C.c_double), ("Dg", C.c_double), ("rho", C.c_double), ("dt", C.c_double)]
+
+
+class RunOpts(C.Structure):
+    """gq_run_opts"""
+    _fields_ = [("adapt_every", C.c_int32), ("max_level", C.c_int32), ("tol_refine", C.c_double),
+                ("tol_compress", C.c_double), ("flags", C.c_int32)]
 
 
 class GQError(RuntimeError):
@@ -91,6 +101,14 @@ def load():
         "gq_topo_refine": [vp, i32p, C.c_int, ip],
         "gq_topo_compress": [vp, i32p, C.c_int, ip],
         "gq_topo_point_terms": [vp, C.c_int, i32p, C.c_int, i32p, i32p, f64p, ip],
+        "gq_grid_write_volume": [vp, C.c_int, f32p, C.c_int, C.c_int],
+        "gq_ensemble_create": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
+        "gq_ensemble_size": [vp, ip],
+        "gq_sample_select": [vp, C.c_int],
+        "gq_ensemble_block_counts": [vp, i32p],
+        "gq_ensemble_run": [vp, C.POINTER(FKParams), i32p, C.POINTER(RunOpts)],
+        "gq_ensemble_stats": [vp, f64p],
+        "gq_ensemble_loglik_flip": [vp, f64p, f64p],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -1,0 +1,306 @@
+// adjoint.hip -- discrete adjoint of the fused FK step on uniform grids.
+//
+// Forward (one step, per cell c; the map k_fk_step computes):
+//   v_c  = u_c + dt * (ih2 * sum_k df_ck (u_k - u_c) + rho u_c (1 - u_c))
+//   u'_c = min(1, max(0, v_c))             (NaN -> 0)
+// with df_ck = 2 / (1/K_c + 1/K_k) when tissue_c = pw+pg+u_c > 0, else 0.
+// The reference has no adjoint; this is the exact reverse-mode derivative of
+// that discrete map (df is piecewise constant in u: the tissue switch and the
+// clamp have zero derivative almost everywhere).  With
+//   mask_c = [0 < v_c < 1] = [0 < u'_c < 1]   and   mu_c = mask_c * lambda'_c:
+//   lambda_c = mu_c (1 + dt (rho (1 - 2 u_c) - ih2 sum_k df_ck)) + dt ih2 sum_k df_kc mu_k
+//   dL/dlog Dw += mu_c dt ih2 sum_k df_ck (u_k - u_c)     (df ~ Dw when Dg = Dw/10)
+//   dL/dlog rho += mu_c dt rho u_c (1 - u_c)
+// Terms whose adjoint factor is zero are skipped (their coefficients can be
+// NaN next to CSF, where the forward clamps NaN to 0).
+//
+// One backward step is one kernel: it stages u_n and 1/K with a one-cell halo
+// in LDS, builds mu from lambda_{n+1} and u_{n+1} (mask from the clamped
+// state, so the mask equals the forward's branch bit for bit), writes
+// lambda_n and adds the two parameter-gradient contributions of its slab to
+// per-slab FP64 accumulators (each owned by one workgroup: deterministic).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace gq {
+
+namespace {
+
+constexpr int B = 16;
+constexpr int B3 = 4096;
+constexpr int LW = B + 2;
+constexpr int PL = LW * LW;
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ float inv_k(float w, float g, float Dw, float Dg)
+{
+    const float b = w + g;
+    const float D = w * Dw + g * Dg;
+    return b * frcp(D);
+}
+
+__device__ __forceinline__ int nbr_face_cell(int f, int u, int v)
+{
+    switch (f) {
+    case 0: return (B - 1) + B * (u + B * v);
+    case 1: return 0 + B * (u + B * v);
+    case 2: return u + B * ((B - 1) + B * v);
+    case 3: return u + B * (0 + B * v);
+    case 4: return u + B * (v + B * (B - 1));
+    default: return u + B * (v + B * 0);
+    }
+}
+
+__device__ __forceinline__ float mu_of(float lam, float unext) { return (unext > 0.f && unext < 1.f) ? lam : 0.f; }
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    return x;
+}
+
+}  // namespace
+
+// ZT planes of one block per workgroup of 256 threads (thread = one (x,y) column).
+template <int ZT>
+__global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
+{
+    constexpr int NS = B / ZT;
+    constexpr int LZ = ZT + 2;
+    __shared__ float s_u[LZ * PL];
+    __shared__ float s_ik[LZ * PL];
+    __shared__ float s_mu[LZ * PL];
+    __shared__ double s_red[2][4];
+    const int item = blockIdx.x;
+    if (item >= a.nwork * NS) return;
+    const int w = item / NS;
+    const int slab = item - w * NS;
+    const int z0 = slab * ZT;
+    const int slot = a.work[w];
+    const int t = threadIdx.x;
+    const int tx = t & 15, ty = t >> 4;
+    const size_t base = (size_t)slot * B3;
+    const float *q = a.sprm + 4 * a.slot_sample[slot];
+    const float Dw = q[0], Dg = q[1], rho = q[2], dt = q[3];
+
+    float breg[ZT];
+#pragma unroll
+    for (int k = 0; k < ZT; k++) {
+        const size_t i = base + (z0 + k) * 256 + t;
+        const float wm = a.pw[i], gm = a.pg[i];
+        breg[k] = wm + gm;
+        const int l = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
+        s_u[l] = a.u[i];
+        s_ik[l] = inv_k(wm, gm, Dw, Dg);
+        s_mu[l] = mu_of(a.lam_in[i], a.u_next[i]);
+    }
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        const int z = side == 0 ? z0 - 1 : z0 + ZT;
+        const int l = (tx + 1) + LW * (ty + 1) + PL * (side == 0 ? 0 : ZT + 1);
+        size_t i;
+        if (z >= 0 && z < B) i = base + z * 256 + t;
+        else {
+            const int f = side == 0 ? 4 : 5;
+            i = (size_t)a.fnbr[slot * 6 + f] * B3 + nbr_face_cell(f, tx, ty);
+        }
+        s_u[l] = a.u[i];
+        s_ik[l] = inv_k(a.pw[i], a.pg[i], Dw, Dg);
+        s_mu[l] = mu_of(a.lam_in[i], a.u_next[i]);
+    }
+    for (int r4 = t; r4 < 4 * 16 * ZT; r4 += 256) {
+        const int f = r4 / (16 * ZT);
+        const int r = r4 - f * 16 * ZT;
+        const int u = r & 15, v = z0 + (r >> 4);
+        const size_t i = (size_t)a.fnbr[slot * 6 + f] * B3 + nbr_face_cell(f, u, v);
+        int l;
+        switch (f) {
+        case 0: l = 0 + LW * (u + 1) + PL * (v - z0 + 1); break;
+        case 1: l = (B + 1) + LW * (u + 1) + PL * (v - z0 + 1); break;
+        case 2: l = (u + 1) + 0 + PL * (v - z0 + 1); break;
+        default: l = (u + 1) + LW * (B + 1) + PL * (v - z0 + 1); break;
+        }
+        s_u[l] = a.u[i];
+        s_ik[l] = inv_k(a.pw[i], a.pg[i], Dw, Dg);
+        s_mu[l] = mu_of(a.lam_in[i], a.u_next[i]);
+    }
+    __syncthreads();
+
+    const float ih2 = a.ih2_lvl[a.level[slot]];
+    const int offs[6] = {-1, 1, -LW, LW, -PL, PL};
+    double gD = 0.0, gR = 0.0;
+    for (int k = 0; k < ZT; k++) {
+        const int c = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
+        const float uc = s_u[c];
+        const float kc = s_ik[c];
+        const float muc = s_mu[c];
+        float lam = 0.f;
+        if (muc != 0.f) {  // the cell's own update was inside (0,1): its coefficients are finite
+            float d[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (breg[k] + uc > 0.f) {
+#pragma unroll
+                for (int f = 0; f < 6; f++) d[f] = 2.f * frcp(kc + s_ik[c + offs[f]]);
+            }
+            const float sdf = d[0] + d[1] + d[2] + d[3] + d[4] + d[5];
+            const float diff = d[0] * (s_u[c - 1] - uc) + d[1] * (s_u[c + 1] - uc) + d[2] * (s_u[c - LW] - uc) +
+                               d[3] * (s_u[c + LW] - uc) + d[4] * (s_u[c - PL] - uc) + d[5] * (s_u[c + PL] - uc);
+            lam = muc * (1.f + dt * (rho * (1.f - 2.f * uc) - ih2 * sdf));
+            gD += (double)muc * (double)dt * (double)ih2 * (double)diff;
+            gR += (double)muc * (double)dt * (double)rho * (double)uc * (1.0 - (double)uc);
+        }
+        // transpose: neighbour k's update reads u_c with k's coefficient toward c
+        float tr = 0.f;
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            const int n = c + offs[f];
+            const float mun = s_mu[n];
+            if (mun != 0.f) {
+                // the neighbour's tissue switch pw+pg+u > 0: with Dw, Dg > 0, 1/K is
+                // finite and positive exactly where pw+pg > 0 (else 0/0 = NaN), and
+                // u >= 0, so tissue_n > 0 <=> 1/K_n finite || u_n > 0
+                const float kn = s_ik[n];
+                const bool tissue = (kn > 0.f && kn < INFINITY) || s_u[n] > 0.f;
+                if (tissue) tr += mun * (2.f * frcp(kn + kc));
+            }
+        }
+        lam += dt * ih2 * tr;
+        a.lam_out[base + (z0 + k) * 256 + t] = lam;
+    }
+    // block sums of the two parameter gradients -> this slab's accumulators
+    gD = wave_sum(gD);
+    gR = wave_sum(gR);
+    const int lane = t & 63, wv = t >> 6;
+    if (lane == 0) {
+        s_red[0][wv] = gD;
+        s_red[1][wv] = gR;
+    }
+    __syncthreads();
+    if (t < 2) {
+        const double s = ((s_red[t][0] + s_red[t][1]) + s_red[t][2]) + s_red[t][3];
+        a.gacc[((size_t)slot * NS + slab) * 2 + t] += s;
+    }
+}
+
+// dL/du of the flip-rate model at the final state (uniform grids): only the
+// Gaussian PET term depends smoothly on u; the Bernoulli terms are piecewise
+// constant in u (derivative zero almost everywhere).
+__global__ __launch_bounds__(256) void k_lik_flip_grad(const float *__restrict__ u, const float *__restrict__ pet,
+                                                       const int *__restrict__ work, const int *__restrict__ index,
+                                                       int gpd, double inv_s2, float *__restrict__ lam)
+{
+    const int slot = work[blockIdx.x];
+    const size_t base = (size_t)slot * B3;
+    const int64_t ox = (int64_t)index[slot * 3] * B, oy = (int64_t)index[slot * 3 + 1] * B,
+                  oz = (int64_t)index[slot * 3 + 2] * B;
+    for (int i = threadIdx.x; i < B3; i += 256) {
+        const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
+        const float p = pet[(ox + x) + gpd * ((oy + y) + (int64_t)gpd * (oz + z))];
+        lam[base + i] = __builtin_isnan(p) ? 0.f : (float)(((double)p - (double)u[base + i]) * inv_s2);
+    }
+}
+
+namespace {
+// signed periodic displacement x - x0 in [-n/2, n/2)
+__device__ __forceinline__ double pdisp(double x, double x0, double n)
+{
+    double d = x - x0;
+    d -= n * floor(d / n + 0.5);
+    return d;
+}
+}  // namespace
+
+// Gaussian initial condition u0 = A exp(-|x - x0|^2 / (2 s^2)) (voxel units,
+// periodic distance); icp[sample*5] = x0, y0, z0, A, s.
+__global__ __launch_bounds__(256) void k_ic_gaussian(float *__restrict__ phi, const int *__restrict__ work,
+                                                     const int *__restrict__ index,
+                                                     const int *__restrict__ slot_sample,
+                                                     const double *__restrict__ icp, int gpd)
+{
+    const int slot = work[blockIdx.x];
+    const double *p = icp + 5 * slot_sample[slot];
+    const double n = gpd;
+    const double is2 = 1.0 / (2.0 * p[4] * p[4]);
+    for (int i = threadIdx.x; i < B3; i += 256) {
+        const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
+        const double dx = pdisp(index[slot * 3] * B + x, p[0], n), dy = pdisp(index[slot * 3 + 1] * B + y, p[1], n),
+                     dz = pdisp(index[slot * 3 + 2] * B + z, p[2], n);
+        phi[(size_t)slot * B3 + i] = (float)(p[3] * exp(-(dx * dx + dy * dy + dz * dz) * is2));
+    }
+}
+
+// dL/d(x0, y0, z0, A) of the Gaussian initial condition from lambda_0, per slot.
+__global__ __launch_bounds__(256) void k_ic_grad(const float *__restrict__ lam0, const int *__restrict__ work,
+                                                 const int *__restrict__ index, const int *__restrict__ slot_sample,
+                                                 const double *__restrict__ icp, int gpd, double *__restrict__ out)
+{
+    __shared__ double s_red[4][4];
+    const int slot = work[blockIdx.x];
+    const double *p = icp + 5 * slot_sample[slot];
+    const double n = gpd;
+    const double s2 = p[4] * p[4];
+    const double is2 = 1.0 / (2.0 * s2);
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < B3; i += 256) {
+        const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
+        const double dx = pdisp(index[slot * 3] * B + x, p[0], n), dy = pdisp(index[slot * 3 + 1] * B + y, p[1], n),
+                     dz = pdisp(index[slot * 3 + 2] * B + z, p[2], n);
+        const double e = exp(-(dx * dx + dy * dy + dz * dz) * is2);
+        const double l = lam0[(size_t)slot * B3 + i];
+        const double lu = l * p[3] * e;
+        g[0] += lu * dx / s2;
+        g[1] += lu * dy / s2;
+        g[2] += lu * dz / s2;
+        g[3] += l * e;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double s = wave_sum(g[k]);
+        if (lane == 0) s_red[k][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        out[(size_t)blockIdx.x * 4 + k] = ((s_red[k][0] + s_red[k][1]) + s_red[k][2]) + s_red[k][3];
+    }
+}
+
+int adj_slabs() { return 2; }
+
+hipError_t launch_adj_step(const AdjArgs &a, hipStream_t st)
+{
+    if (a.nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_step<8>, dim3(a.nwork * 2), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_lik_flip_grad(const float *u, const float *pet, const int *work, int nwork, const int *index,
+                                int gpd, double inv_s2, float *lam, hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lik_flip_grad, dim3(nwork), dim3(256), 0, st, u, pet, work, index, gpd, inv_s2, lam);
+    return hipGetLastError();
+}
+
+hipError_t launch_ic_gaussian(float *phi, const int *work, int nwork, const int *index, const int *slot_sample,
+                              const double *icp, int gpd, hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ic_gaussian, dim3(nwork), dim3(256), 0, st, phi, work, index, slot_sample, icp, gpd);
+    return hipGetLastError();
+}
+
+hipError_t launch_ic_grad(const float *lam0, const int *work, int nwork, const int *index, const int *slot_sample,
+                          const double *icp, int gpd, double *out, hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ic_grad, dim3(nwork), dim3(256), 0, st, lam0, work, index, slot_sample, icp, gpd, out);
+    return hipGetLastError();
+}
+
+}  // namespace gq

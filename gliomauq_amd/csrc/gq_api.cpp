@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <exception>
 #include <chrono>
 #include <map>
 #include <cmath>
@@ -82,12 +84,23 @@ int hw_threads()
 
 }  // namespace
 
+// One grid of an ensemble with its host-side tables.
+struct Sample {
+    std::unique_ptr<gq::Grid> grid;
+    bool dirty = true;  // topology changed since the tables were built
+    gq::SampleTables t;
+};
+
 struct gq_ctx {
     int device = 0;
     hipStream_t st = nullptr;
     std::string err;
 
-    std::unique_ptr<gq::Grid> grid;
+    // samples: independent grids of one configuration sharing the block pool
+    std::vector<Sample> samples;
+    int cur_sample = 0;              // the sample single-grid calls address
+    std::vector<int> sample_order;   // work-list order of the samples
+
     // block pool
     int cap = 0;
     float *phi[2] = {nullptr, nullptr};
@@ -95,17 +108,21 @@ struct gq_ctx {
     float *pw = nullptr, *pg = nullptr;
     std::vector<int> free_slots;
 
-    // tables
+    // merged tables (slot-indexed arrays cover the whole pool)
     bool tables_valid = false;
-    gq::Tables tabs;
-    std::vector<int> api_order;  // leaf indices in API (Morton) order
-    DevBuf<int> d_work, d_level, d_index, d_fnbr, d_fbase, d_face_ptr, d_face_src, d_nbr26, d_lut, d_lutf,
-        d_faces, d_counts;
+    std::vector<int> h_fnbr, h_nbr26;  // [cap*6], [cap*26]
+    std::vector<int> work;             // live slots: samples in sample_order, leaves in Morton order
+    std::vector<int> work_begin;       // [S+1] per position in sample_order
+    std::vector<int> face_begin;       // [S+1] cross-level faces per position
+    DevBuf<int> d_work, d_level, d_index, d_fnbr, d_fbase, d_face_src, d_nbr26, d_lut, d_lutf, d_lutoff,
+        d_slot_sample, d_faces, d_counts;
+    DevBuf<int64_t> d_face_ptr;
+    DevBuf<char> d_scan_tmp;
     int face_nrows = 0;
     DevBuf<double> d_ghost;  // cross-level lab points (3 fields for a split)
-    DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2;
+    DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2, d_sprm;
     bool fik_valid = false;
-    float fik_Dw = 0, fik_Dg = 0;
+    std::vector<float> fik_key;  // Dw, Dg (per sample when per-sample parameters are set)
 
     // scratch
     DevBuf<int> d_list, d_list2;
@@ -137,8 +154,17 @@ struct gq_ctx {
     int last_launches = 0;
     int64_t last_cells = 0;
 
+    // ensemble run statistics (gq_ensemble_run)
+    double ens_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    // captured step chunks
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
     ~gq_ctx()
     {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
         for (auto e : events) (void)hipEventDestroy(e);
         release_pool();
         if (st) (void)hipStreamDestroy(st);
@@ -156,16 +182,26 @@ struct gq_ctx {
         cap = 0;
     }
 
+    int nsamples() const { return (int)samples.size(); }
+
     void need_grid() const
     {
-        if (!grid) throw gq_error(GQ_ERR_STATE, "no grid: call gq_grid_create first");
+        if (samples.empty()) throw gq_error(GQ_ERR_STATE, "no grid: call gq_grid_create first");
     }
+
+    gq::Grid &G(int s) { return *samples[s].grid; }
+    gq::Grid &G() { return *samples[cur_sample].grid; }
+    const std::vector<int> &api_order() { return samples[cur_sample].t.order; }
 
     // Grow the pool to at least `want` slots, keeping slot contents.
     void grow(int want)
     {
         if (want <= cap) return;
         int ncap = std::max(want, cap + cap / 2);
+        if ((int64_t)ncap * gq::BS3 > INT32_MAX) {
+            ncap = INT32_MAX / gq::BS3;
+            if (want > ncap) throw gq_error(GQ_ERR_NOMEM, "block pool beyond 2^31 cells");
+        }
         const size_t nb = (size_t)ncap * gq::BS3 * sizeof(float);
         const size_t ob = (size_t)cap * gq::BS3 * sizeof(float);
         float *np[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -188,7 +224,12 @@ struct gq_ctx {
         pg = np[3];
         for (int s = ncap - 1; s >= cap; s--) free_slots.push_back(s);
         cap = ncap;
-        tables_valid = false;
+        tables_valid = false;  // slot-indexed device arrays are sized by the pool
+    }
+
+    void reserve_free(size_t k)
+    {
+        if (free_slots.size() < k) grow(cap + (int)(k - free_slots.size()));
     }
 
     int alloc_slot()
@@ -209,63 +250,123 @@ struct gq_ctx {
         }
     }
 
+    // Run f(i) for i in [0, n) on up to 16 host threads.
+    template <typename F>
+    static void parallel_for(int n, F &&f)
+    {
+        const int nt = std::min(hw_threads(), n);
+        if (nt <= 1) {
+            for (int i = 0; i < n; i++) f(i);
+            return;
+        }
+        std::atomic<int> next{0};
+        std::vector<std::exception_ptr> errs(nt);
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; t++)
+            th.emplace_back([&, t] {
+                try {
+                    for (int i; (i = next.fetch_add(1)) < n;) f(i);
+                } catch (...) {
+                    errs[t] = std::current_exception();
+                }
+            });
+        for (auto &x : th) x.join();
+        for (auto &e : errs)
+            if (e) std::rethrow_exception(e);
+    }
+
     void ensure_tables()
     {
         need_grid();
         if (tables_valid) return;
         ScopeTimer tm(t_tables);
         n_tables++;
-        gq::Tables &t = tabs;
-        gq::build_tables(*grid, cap, t);
-        api_order = gq::morton_order(*grid);
-        const auto &L = grid->leaves();
-        std::vector<int> work(api_order.size());
-        for (size_t k = 0; k < api_order.size(); k++) work[k] = L[api_order[k]].slot;
-        std::vector<int> lvl((size_t)cap, 0), idx((size_t)cap * 3, 0);
-        for (const auto &lf : L) {
-            lvl[lf.slot] = lf.level;
-            for (int a = 0; a < 3; a++) idx[(size_t)lf.slot * 3 + a] = lf.idx[a];
+        const int S = nsamples();
+        if ((int)sample_order.size() != S) {
+            sample_order.resize(S);
+            for (int s = 0; s < S; s++) sample_order[s] = s;
         }
+        h_fnbr.resize((size_t)cap * 6, -1);
+        h_nbr26.resize((size_t)cap * 26, -1);
+        std::vector<int> dirty;
+        for (int s = 0; s < S; s++)
+            if (samples[s].dirty) dirty.push_back(s);
+        parallel_for((int)dirty.size(), [&](int k) {
+            Sample &sm = samples[dirty[k]];
+            gq::build_sample_tables(*sm.grid, h_fnbr.data(), h_nbr26.data(), sm.t);
+            sm.dirty = false;
+        });
+        // merged arrays
+        std::vector<int> lvl((size_t)cap, 0), idx((size_t)cap * 3, 0), ssample((size_t)cap, 0),
+            lutoff((size_t)cap, 0), lutf, lut_slot, faces;
+        work.clear();
+        work_begin.assign(1, 0);
+        face_begin.assign(1, 0);
+        for (int pos = 0; pos < S; pos++) {
+            const Sample &sm = samples[sample_order[pos]];
+            const auto &L = sm.grid->leaves();
+            for (int li : sm.t.order) work.push_back(L[li].slot);
+            faces.insert(faces.end(), sm.t.faces.begin(), sm.t.faces.end());
+            work_begin.push_back((int)work.size());
+            face_begin.push_back((int)faces.size());
+        }
+        for (int s = 0; s < S; s++) {
+            const Sample &sm = samples[s];
+            const int off = (int)lutf.size();
+            lutf.insert(lutf.end(), sm.t.lut.begin(), sm.t.lut.end());
+            lut_slot.insert(lut_slot.end(), sm.t.lut_slot.begin(), sm.t.lut_slot.end());
+            for (const auto &lf : sm.grid->leaves()) {
+                lvl[lf.slot] = lf.level;
+                for (int a = 0; a < 3; a++) idx[(size_t)lf.slot * 3 + a] = lf.idx[a];
+                ssample[lf.slot] = s;
+                lutoff[lf.slot] = off;
+            }
+        }
+        if ((int64_t)faces.size() * 256 > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "too many cross-level faces");
         std::vector<int> fbase((size_t)cap * 6, -1);
-        for (size_t k = 0; k < t.faces.size(); k++) fbase[t.faces[k]] = (int)k * 256;
+        for (size_t k = 0; k < faces.size(); k++) fbase[faces[k]] = (int)k * 256;
         d_work.upload(work, st);
         d_level.upload(lvl, st);
         d_index.upload(idx, st);
-        d_fnbr.upload(t.fnbr, st);
+        d_slot_sample.upload(ssample, st);
+        d_lutoff.upload(lutoff, st);
+        d_fnbr.upload(h_fnbr, st);
         d_fbase.upload(fbase, st);
-        d_nbr26.upload(t.nbr26, st);
-        d_lut.upload(t.lut_slot, st);
-        d_lutf.upload(t.lut, st);
-        d_faces.upload(t.faces, st);
+        d_nbr26.upload(h_nbr26, st);
+        d_lut.upload(lut_slot, st);
+        d_lutf.upload(lutf, st);
+        d_faces.upload(faces, st);
         // step rows of the cross-level faces, enumerated on the device
-        const int nfaces = (int)t.faces.size();
+        const int nfaces = (int)faces.size();
         face_nrows = nfaces * 256;
+        d_face_ptr.reserve((size_t)face_nrows + 1);
         if (nfaces > 0) {
             const gq::LabArgs la = lab_args();
             d_counts.reserve((size_t)face_nrows);
             HIPCHK(gq::launch_face_rows_count(la, d_faces.p, nfaces, d_counts.p, st));
-            std::vector<int> cnt((size_t)face_nrows), ptr((size_t)face_nrows + 1);
-            HIPCHK(hipMemcpyAsync(cnt.data(), d_counts.p, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+            size_t tb = 0;
+            HIPCHK(gq::scan_counts(d_counts.p, d_face_ptr.p, face_nrows, nullptr, tb, st));
+            d_scan_tmp.reserve(tb + 1);
+            HIPCHK(gq::scan_counts(d_counts.p, d_face_ptr.p, face_nrows, d_scan_tmp.p, tb, st));
+            int64_t total = 0;
+            HIPCHK(hipMemcpyAsync(&total, d_face_ptr.p + face_nrows, sizeof(total), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
-            ptr[0] = 0;
-            for (int r = 0; r < face_nrows; r++) {
-                if ((int64_t)ptr[r] + cnt[r] > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "face rows exceed 2^31 terms");
-                ptr[r + 1] = ptr[r] + cnt[r];
-            }
-            d_face_ptr.upload(ptr, st);
-            d_face_src.reserve((size_t)std::max(1, ptr[face_nrows]));
-            d_face_w.reserve((size_t)std::max(1, ptr[face_nrows]));
+            if (total > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "face rows exceed 2^31 terms");
+            d_face_src.reserve((size_t)std::max<int64_t>(1, total));
+            d_face_w.reserve((size_t)std::max<int64_t>(1, total));
             HIPCHK(gq::launch_face_rows_fill(la, d_faces.p, nfaces, d_face_ptr.p, d_face_src.p, d_face_w.p, st));
         }
         d_gphi.reserve((size_t)std::max(1, face_nrows));
         d_fik.reserve((size_t)cap * 6 * 256);
-        std::vector<float> ih2(16);
-        for (int l = 0; l < 16; l++) {
-            const float h = (float)(std::ldexp(1.0, -l) / gq::BS);  // BlockInfo::h (Real)
-            const double hd = h;
-            ih2[l] = (float)(1. / (hd * hd));
+        if (!d_ih2.p) {
+            std::vector<float> ih2(16);
+            for (int l = 0; l < 16; l++) {
+                const float h = (float)(std::ldexp(1.0, -l) / gq::BS);  // BlockInfo::h (Real)
+                const double hd = h;
+                ih2[l] = (float)(1. / (hd * hd));
+            }
+            d_ih2.upload(ih2, st);
         }
-        d_ih2.upload(ih2, st);
         HIPCHK(hipStreamSynchronize(st));  // the host vectors above are released on return
         tables_valid = true;
         fik_valid = false;
@@ -276,123 +377,149 @@ struct gq_ctx {
         gq::LabArgs a;
         a.nbr26 = d_nbr26.p;
         a.lut = d_lutf.p;
+        a.lutoff = d_lutoff.p;
         a.level = d_level.p;
         a.index = d_index.p;
-        a.lut_level = grid->lut_level();
-        a.order = grid->order();
+        a.lut_level = samples[0].grid->lut_level();
+        a.order = samples[0].grid->order();
         return a;
     }
 
-    void ensure_fik(float Dw, float Dg)
+    // 1/K of the face halos for (Dw, Dg), or for the per-sample parameters in d_sprm when key has 2*S entries.
+    void ensure_fik(const std::vector<float> &key, bool per_sample)
     {
         ensure_tables();
-        if (fik_valid && Dw == fik_Dw && Dg == fik_Dg) return;
-        HIPCHK(gq::launch_face_prep(pw, pg, d_work.p, (int)api_order.size(), d_fnbr.p, d_fbase.p, d_face_ptr.p,
-                                    d_face_src.p, d_face_w.p, d_fik.p, Dw, Dg, st));
+        if (fik_valid && key == fik_key) return;
+        HIPCHK(gq::launch_face_prep(pw, pg, d_work.p, (int)work.size(), d_fnbr.p, d_fbase.p, d_face_ptr.p,
+                                    d_face_src.p, d_face_w.p, d_fik.p, key[0], key[1],
+                                    per_sample ? d_slot_sample.p : nullptr, per_sample ? d_sprm.p : nullptr, st));
         fik_valid = true;
-        fik_Dw = Dw;
-        fik_Dg = Dg;
+        fik_key = key;
     }
 
-    // detail per API-order leaf subset (leaf indices)
-    std::vector<double> fwt(const std::vector<int> &leaf_idx)
+    // FWT detail of each listed slot (any samples)
+    std::vector<double> fwt_slots(const std::vector<int> &slots)
     {
         ensure_tables();
-        const auto &L = grid->leaves();
-        std::vector<int> slots(leaf_idx.size());
-        for (size_t k = 0; k < leaf_idx.size(); k++) slots[k] = L[leaf_idx[k]].slot;
-        std::vector<double> out(leaf_idx.size());
+        std::vector<double> out(slots.size());
         if (slots.empty()) return out;
         ScopeTimer tm(t_fwt);
         n_fwt++;
         d_list.upload(slots, st);
         d_dbl.reserve(slots.size());
         const gq::LabArgs la = lab_args();
-        d_ghost.reserve(slots.size() * (size_t)gq::lab_points(grid->order()));
+        const int order = samples[0].grid->order();
+        d_ghost.reserve(slots.size() * (size_t)gq::lab_points(order));
         HIPCHK(gq::launch_lab_ghosts(la, phi[cur], d_list.p, (int)slots.size(), d_ghost.p, false, st));
-        HIPCHK(gq::launch_fwt_detail(grid->order(), la, d_ghost.p, phi[cur], d_list.p, (int)slots.size(), d_dbl.p,
-                                     st));
+        HIPCHK(gq::launch_fwt_detail(order, la, d_ghost.p, phi[cur], d_list.p, (int)slots.size(), d_dbl.p, st));
         HIPCHK(hipMemcpyAsync(out.data(), d_dbl.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         return out;
     }
 
-    int refine_ids(const std::unordered_set<int> &ids)
+    // Refine the flagged block ids of each listed sample (Refiner_SpaceExtension plan + split).
+    std::vector<int> refine_batch(const std::vector<int> &which, const std::vector<std::unordered_set<int>> &ids)
     {
         need_grid();
-        std::vector<int> parents;
+        const int n = (int)which.size();
+        std::vector<std::vector<int>> parents(n);
         {
             ScopeTimer tm(t_plan);
-            parents = grid->plan_refine(ids);
+            parallel_for(n, [&](int k) {
+                if (!ids[k].empty()) parents[k] = G(which[k]).plan_refine(ids[k]);
+            });
         }
-        if (parents.empty()) return 0;
+        size_t np = 0;
+        for (auto &p : parents) np += p.size();
+        std::vector<int> counts(n, 0);
+        if (np == 0) return counts;
         ensure_tables();  // parent labs use the pre-refine topology
         ScopeTimer tm(t_split);
-        grow((int)(grid->leaves().size() + parents.size() * 8));
-        const auto &L = grid->leaves();
+        reserve_free(np * 8);
         std::vector<int> pslots, cslots;
-        std::vector<gq::Leaf> add;
-        for (int p : parents) {
-            const gq::Leaf &pl = L[p];
-            pslots.push_back(pl.slot);
-            for (int c = 0; c < 8; c++) {
-                gq::Leaf ch;
-                ch.level = pl.level + 1;
-                ch.idx[0] = 2 * pl.idx[0] + (c & 1);
-                ch.idx[1] = 2 * pl.idx[1] + ((c >> 1) & 1);
-                ch.idx[2] = 2 * pl.idx[2] + ((c >> 2) & 1);
-                ch.block_id = grid->new_block_id();
-                ch.slot = alloc_slot();
-                cslots.push_back(ch.slot);
-                add.push_back(ch);
+        std::vector<std::vector<gq::Leaf>> add(n);
+        for (int k = 0; k < n; k++) {
+            gq::Grid &g = G(which[k]);
+            const auto &L = g.leaves();
+            for (int p : parents[k]) {
+                const gq::Leaf &pl = L[p];
+                pslots.push_back(pl.slot);
+                for (int c = 0; c < 8; c++) {
+                    gq::Leaf ch;
+                    ch.level = pl.level + 1;
+                    ch.idx[0] = 2 * pl.idx[0] + (c & 1);
+                    ch.idx[1] = 2 * pl.idx[1] + ((c >> 1) & 1);
+                    ch.idx[2] = 2 * pl.idx[2] + ((c >> 2) & 1);
+                    ch.block_id = g.new_block_id();
+                    ch.slot = alloc_slot();
+                    cslots.push_back(ch.slot);
+                    add[k].push_back(ch);
+                }
             }
         }
         d_list.upload(pslots, st);
         d_list2.upload(cslots, st);
         const gq::LabArgs la = lab_args();
-        const size_t per = pslots.size() * (size_t)gq::lab_points(grid->order());
+        const int order = samples[0].grid->order();
+        const size_t per = pslots.size() * (size_t)gq::lab_points(order);
         d_ghost.reserve(3 * per);
         const float *fl[3] = {phi[cur], pw, pg};
         for (int f = 0; f < 3; f++)
             HIPCHK(gq::launch_lab_ghosts(la, fl[f], d_list.p, (int)pslots.size(), d_ghost.p + f * per, true, st));
-        HIPCHK(gq::launch_split(grid->order(), la, d_ghost.p, phi[cur], pw, pg, d_list.p, d_list2.p,
-                                (int)pslots.size(), st));
+        HIPCHK(gq::launch_split(order, la, d_ghost.p, phi[cur], pw, pg, d_list.p, d_list2.p, (int)pslots.size(),
+                                st));
         HIPCHK(hipStreamSynchronize(st));
         for (int s : pslots) free_slots.push_back(s);
-        grid->replace_leaves(parents, add);
+        parallel_for(n, [&](int k) {
+            if (!parents[k].empty()) G(which[k]).replace_leaves(parents[k], add[k]);
+        });
+        for (int k = 0; k < n; k++) {
+            counts[k] = (int)parents[k].size();
+            if (counts[k]) samples[which[k]].dirty = true;
+        }
         tables_valid = false;
-        return (int)parents.size();
+        return counts;
     }
 
-    int compress_ids(const std::unordered_set<int> &ids)
+    std::vector<int> compress_batch(const std::vector<int> &which, const std::vector<std::unordered_set<int>> &ids)
     {
         need_grid();
-        std::vector<std::array<int, 4>> plan;
+        const int n = (int)which.size();
+        std::vector<std::vector<std::array<int, 4>>> plan(n);
         {
             ScopeTimer tm(t_plan);
-            plan = grid->plan_compress(ids);
+            parallel_for(n, [&](int k) {
+                if (!ids[k].empty()) plan[k] = G(which[k]).plan_compress(ids[k]);
+            });
         }
-        if (plan.empty()) return 0;
+        size_t np = 0;
+        for (auto &p : plan) np += p.size();
+        std::vector<int> counts(n, 0);
+        if (np == 0) return counts;
         ScopeTimer tm(t_collapse);
-        grow((int)(grid->leaves().size() + plan.size()));
-        std::vector<int> pslots, cslots, remove;
-        std::vector<gq::Leaf> add;
-        for (const auto &p : plan) {
-            gq::Leaf pl;
-            pl.level = p[0];
-            pl.idx[0] = p[1];
-            pl.idx[1] = p[2];
-            pl.idx[2] = p[3];
-            pl.block_id = grid->new_block_id();
-            pl.slot = alloc_slot();
-            pslots.push_back(pl.slot);
-            add.push_back(pl);
-            for (int c = 0; c < 8; c++) {
-                const int li = grid->find(p[0] + 1, 2 * p[1] + (c & 1), 2 * p[2] + ((c >> 1) & 1),
-                                          2 * p[3] + ((c >> 2) & 1));
-                if (li < 0) throw gq_error(GQ_ERR_TOPOLOGY, "compress plan references a missing child");
-                remove.push_back(li);
-                cslots.push_back(grid->leaves()[li].slot);
+        reserve_free(np);
+        std::vector<int> pslots, cslots;
+        std::vector<std::vector<int>> remove(n);
+        std::vector<std::vector<gq::Leaf>> add(n);
+        for (int k = 0; k < n; k++) {
+            gq::Grid &g = G(which[k]);
+            for (const auto &p : plan[k]) {
+                gq::Leaf pl;
+                pl.level = p[0];
+                pl.idx[0] = p[1];
+                pl.idx[1] = p[2];
+                pl.idx[2] = p[3];
+                pl.block_id = g.new_block_id();
+                pl.slot = alloc_slot();
+                pslots.push_back(pl.slot);
+                add[k].push_back(pl);
+                for (int c = 0; c < 8; c++) {
+                    const int li =
+                        g.find(p[0] + 1, 2 * p[1] + (c & 1), 2 * p[2] + ((c >> 1) & 1), 2 * p[3] + ((c >> 2) & 1));
+                    if (li < 0) throw gq_error(GQ_ERR_TOPOLOGY, "compress plan references a missing child");
+                    remove[k].push_back(li);
+                    cslots.push_back(g.leaves()[li].slot);
+                }
             }
         }
         d_list.upload(pslots, st);
@@ -400,9 +527,106 @@ struct gq_ctx {
         HIPCHK(gq::launch_collapse(phi[cur], pw, pg, d_list.p, d_list2.p, (int)pslots.size(), st));
         HIPCHK(hipStreamSynchronize(st));
         for (int s : cslots) free_slots.push_back(s);
-        grid->replace_leaves(remove, add);
+        parallel_for(n, [&](int k) {
+            if (!plan[k].empty()) G(which[k]).replace_leaves(remove[k], add[k]);
+        });
+        for (int k = 0; k < n; k++) {
+            counts[k] = (int)plan[k].size();
+            if (counts[k]) samples[which[k]].dirty = true;
+        }
         tables_valid = false;
-        return (int)plan.size();
+        return counts;
+    }
+
+    // Science::AutomaticRefinement<0,0> (MRAGscience/MRAGScienceCore.h:42-108) over several samples at once.
+    std::vector<int> auto_refine(const std::vector<int> &which, double tol, int max_level, int max_loops)
+    {
+        const int n = (int)which.size();
+        std::vector<std::unordered_set<int>> nice(n);
+        std::vector<int> total(n, 0);
+        std::vector<char> live(n, 1);
+        int loops = 0;
+        while (true) {
+            ensure_tables();
+            std::vector<int> cand_slots, cand_k, cand_li;
+            for (int k = 0; k < n; k++) {
+                if (!live[k]) continue;
+                const Sample &sm = samples[which[k]];
+                const auto &L = sm.grid->leaves();
+                for (int li : sm.t.order) {
+                    if (max_level >= 0 && max_level <= L[li].level) continue;
+                    if (nice[k].count(L[li].block_id)) continue;
+                    cand_slots.push_back(L[li].slot);
+                    cand_k.push_back(k);
+                    cand_li.push_back(li);
+                }
+            }
+            if (cand_slots.empty()) break;
+            std::vector<double> det = fwt_slots(cand_slots);
+            std::vector<std::unordered_set<int>> flagged(n);
+            for (size_t c = 0; c < cand_slots.size(); c++) {
+                const int k = cand_k[c];
+                const int id = G(which[k]).leaves()[cand_li[c]].block_id;
+                if (det[c] >= tol) flagged[k].insert(id);
+                else nice[k].insert(id);
+            }
+            bool any = false;
+            for (int k = 0; k < n; k++) {
+                if (flagged[k].empty()) live[k] = 0;
+                else any = true;
+            }
+            if (!any) break;
+            std::vector<int> r = refine_batch(which, flagged);
+            for (int k = 0; k < n; k++) total[k] += r[k];
+            loops++;
+            if (max_loops > 0 && loops >= max_loops) break;
+        }
+        return total;
+    }
+
+    // Science::AutomaticCompression<0,0> (MRAGscience/MRAGScienceCore.h:117-177) over several samples at once.
+    std::vector<int> auto_compress(const std::vector<int> &which, double tol, int max_loops)
+    {
+        const int n = (int)which.size();
+        std::vector<std::unordered_set<int>> nice(n);
+        std::vector<int> total(n, 0);
+        std::vector<char> live(n, 1);
+        int loops = 0;
+        while (true) {
+            ensure_tables();
+            std::vector<int> cand_slots, cand_k, cand_li;
+            for (int k = 0; k < n; k++) {
+                if (!live[k]) continue;
+                const Sample &sm = samples[which[k]];
+                const auto &L = sm.grid->leaves();
+                for (int li : sm.t.order) {
+                    if (nice[k].count(L[li].block_id)) continue;
+                    cand_slots.push_back(L[li].slot);
+                    cand_k.push_back(k);
+                    cand_li.push_back(li);
+                }
+            }
+            if (cand_slots.empty()) break;
+            std::vector<double> det = fwt_slots(cand_slots);
+            std::vector<std::unordered_set<int>> flagged(n);
+            for (size_t c = 0; c < cand_slots.size(); c++) {
+                const int k = cand_k[c];
+                const int id = G(which[k]).leaves()[cand_li[c]].block_id;
+                if (det[c] < tol) flagged[k].insert(id);
+                else nice[k].insert(id);
+            }
+            std::vector<int> r = compress_batch(which, flagged);
+            bool any = false;
+            for (int k = 0; k < n; k++) {
+                total[k] += r[k];
+                if (flagged[k].empty() || r[k] == 0) live[k] = 0;
+                else any = true;
+            }
+            if (!any) break;
+            loops++;
+            if (max_loops > 0 && loops >= max_loops) break;
+        }
+        return total;
     }
 
     static int zt_heuristic(int nwork)
@@ -457,22 +681,142 @@ struct gq_ctx {
         return best;
     }
 
-    gq::UniformArgs uniform_args()
+    gq::StepArgs step_args(const gq_fk_params *p)
+    {
+        gq::StepArgs a;
+        a.pw = pw;
+        a.pg = pg;
+        a.work = d_work.p;
+        a.nwork = (int)work.size();
+        a.fnbr = d_fnbr.p;
+        a.fbase = d_fbase.p;
+        a.fik = d_fik.p;
+        a.gphi = d_gphi.p;
+        a.level = d_level.p;
+        a.ih2_lvl = d_ih2.p;
+        a.slot_sample = d_slot_sample.p;
+        if (p) {  // the operator stores Dw, Dg, rho as Real (float)
+            a.Dw = (float)p->Dw;
+            a.Dg = (float)p->Dg;
+            a.rho = (float)p->rho;
+            a.dt = (float)p->dt;
+            a.sprm = nullptr;
+        } else {
+            a.Dw = a.Dg = a.rho = a.dt = 0.f;
+            a.sprm = d_sprm.p;
+        }
+        return a;
+    }
+
+    void need_events(int n)
+    {
+        while ((int)events.size() < n) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            events.push_back(e);
+        }
+    }
+
+    // Advance m steps; step j updates the first nwork[j] work slots and the
+    // first nrows[j] face rows (a prefix of the samples in work order).
+    // Returns the device time (ms): step kernels only with per-kernel timing,
+    // else the whole chunk.
+    double advance(gq::StepArgs a, int zt, const std::vector<int> &nwork, const std::vector<int> &nrows,
+                   bool use_graph)
+    {
+        const int m = (int)nwork.size();
+        if (m == 0) return 0.0;
+        const bool pk = per_kernel_timing && !use_graph;
+        need_events(pk ? 2 * m : 2);
+        auto enqueue = [&](int c0) {
+            int c = c0;
+            for (int j = 0; j < m; j++) {
+                if (nrows[j] > 0)
+                    HIPCHK(gq::launch_face_ghosts(phi[c], d_face_ptr.p, d_face_src.p, d_face_w.p, d_gphi.p, nrows[j],
+                                                  st));
+                a.phi_in = phi[c];
+                a.phi_out = phi[c ^ 1];
+                a.nwork = nwork[j];
+                if (pk) HIPCHK(hipEventRecord(events[2 * j], st));
+                if (nwork[j] > 0) HIPCHK(gq::launch_fk_step(a, zt, st));
+                if (pk) HIPCHK(hipEventRecord(events[2 * j + 1], st));
+                c ^= 1;
+            }
+        };
+        double total = 0.0;
+        if (use_graph) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue(cur);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(st, &g));
+            bool updated = false;
+            if (graph_exec) {
+                hipGraphExecUpdateResult res;
+                hipGraphNode_t bad = nullptr;
+                updated = hipGraphExecUpdate(graph_exec, g, &bad, &res) == hipSuccess &&
+                          res == hipGraphExecUpdateSuccess;
+                if (!updated) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(graph_exec);
+                    graph_exec = nullptr;
+                }
+            }
+            if (!updated) HIPCHK(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
+            if (graph) (void)hipGraphDestroy(graph);
+            graph = g;
+            HIPCHK(hipEventRecord(events[0], st));
+            HIPCHK(hipGraphLaunch(graph_exec, st));
+            HIPCHK(hipEventRecord(events[1], st));
+            HIPCHK(hipStreamSynchronize(st));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, events[0], events[1]));
+            total = ms;
+        } else {
+            if (!pk) HIPCHK(hipEventRecord(events[0], st));
+            enqueue(cur);
+            if (!pk) HIPCHK(hipEventRecord(events[1], st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (pk) {
+                for (int j = 0; j < m; j++) {
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, events[2 * j], events[2 * j + 1]));
+                    total += ms;
+                }
+            } else {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, events[0], events[1]));
+                total = ms;
+            }
+        }
+        if (m & 1) cur ^= 1;
+        return total;
+    }
+
+    gq::UniformArgs uniform_args(int sample)
     {
         ensure_tables();
-        if (grid->max_level() != grid->base_level())
+        const gq::Grid &g = G(sample);
+        if (g.max_level() != g.base_level())
             throw gq_error(GQ_ERR_UNSUPPORTED,
                            "uniform view needs max_level == log2(blocks_per_dim) (no leaf finer than the dump grid)");
+        const int bpd = g.blocks_per_dim();
         gq::UniformArgs u;
         u.vol = nullptr;
         u.phi = phi[cur];
-        u.lut_slot = d_lut.p;
+        u.lut_slot = d_lut.p + (size_t)sample * bpd * bpd * bpd;
         u.level = d_level.p;
         u.index = d_index.p;
-        u.gpd = grid->blocks_per_dim() * gq::BS;
+        u.gpd = bpd * gq::BS;
         u.hole = 0.0;
         return u;
     }
+    gq::UniformArgs uniform_args() { return uniform_args(cur_sample); }
 };
 
 namespace {
@@ -552,32 +896,79 @@ const char *gq_status_string(gq_status s)
 
 const char *gq_last_error(const gq_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
 
+static void create_samples(gq_ctx *ctx, int nsamples, int bpd, int max_level, int order)
+{
+    if (nsamples <= 0) throw gq_error(GQ_ERR_ARG, "nsamples must be positive");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<Sample> smp(nsamples);
+    for (auto &sm : smp) sm.grid = std::make_unique<gq::Grid>(bpd, max_level, order);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->tables_valid = false;
+    ctx->fik_valid = false;
+    ctx->h_fnbr.clear();
+    ctx->h_nbr26.clear();
+    ctx->release_pool();
+    ctx->free_slots.clear();
+    ctx->samples = std::move(smp);
+    ctx->cur_sample = 0;
+    ctx->sample_order.clear();
+    const int n = (int)ctx->samples[0].grid->leaves().size();
+    const int64_t want = (int64_t)n * nsamples;
+    if (want * gq::BS3 > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "block pool beyond 2^31 cells");
+    ctx->grow((int)std::max<int64_t>(64, want + want / 2));
+    // slots in Morton order of the initial leaves, sample after sample
+    std::vector<int> mo = gq::morton_order(*ctx->samples[0].grid);
+    std::vector<char> used(ctx->cap, 0);
+    for (int s = 0; s < nsamples; s++) {
+        auto &L = ctx->samples[s].grid->leaves_mut();
+        for (int k = 0; k < n; k++) {
+            L[mo[k]].slot = s * n + k;
+            used[s * n + k] = 1;
+        }
+    }
+    ctx->free_slots.clear();
+    for (int s = ctx->cap - 1; s >= 0; s--)
+        if (!used[s]) ctx->free_slots.push_back(s);
+    ctx->cur = 0;
+    ctx->ensure_tables();
+}
+
 gq_status gq_grid_create(gq_ctx *ctx, int bpd, int max_level, int order)
 {
+    return guard(ctx, [&] { create_samples(ctx, 1, bpd, max_level, order); });
+}
+
+gq_status gq_ensemble_create(gq_ctx *ctx, int nsamples, int bpd, int max_level, int order)
+{
+    return guard(ctx, [&] { create_samples(ctx, nsamples, bpd, max_level, order); });
+}
+
+gq_status gq_ensemble_size(gq_ctx *ctx, int *nsamples)
+{
     return guard(ctx, [&] {
-        HIPCHK(hipSetDevice(ctx->device));
-        auto g = std::make_unique<gq::Grid>(bpd, max_level, order);
-        ctx->tabs = gq::Tables();  // block ids restart: drop the row cache of any previous grid
-        ctx->tables_valid = false;
-        ctx->fik_valid = false;
-        ctx->release_pool();
-        ctx->free_slots.clear();
-        ctx->grid = std::move(g);
-        const int n = (int)ctx->grid->leaves().size();
-        ctx->grow(std::max(64, n + n / 2));
-        // slots in Morton order of the initial leaves
-        std::vector<int> mo = gq::morton_order(*ctx->grid);
-        std::vector<char> used(ctx->cap, 0);
-        auto &L = ctx->grid->leaves_mut();
-        for (int k = 0; k < n; k++) {
-            L[mo[k]].slot = k;
-            used[k] = 1;
-        }
-        ctx->free_slots.clear();
-        for (int s = ctx->cap - 1; s >= 0; s--)
-            if (!used[s]) ctx->free_slots.push_back(s);
-        ctx->cur = 0;
-        ctx->ensure_tables();
+        if (!nsamples) throw gq_error(GQ_ERR_ARG, "null output");
+        *nsamples = ctx->nsamples();
+    });
+}
+
+gq_status gq_sample_select(gq_ctx *ctx, int sample)
+{
+    return guard(ctx, [&] {
+        ctx->need_grid();
+        if (sample < 0 || sample >= ctx->nsamples()) throw gq_error(GQ_ERR_ARG, "sample out of range");
+        ctx->cur_sample = sample;
+    });
+}
+
+gq_status gq_ensemble_block_counts(gq_ctx *ctx, int32_t *out)
+{
+    return guard(ctx, [&] {
+        ctx->need_grid();
+        if (!out) throw gq_error(GQ_ERR_ARG, "null output");
+        for (int s = 0; s < ctx->nsamples(); s++) out[s] = (int32_t)ctx->G(s).leaves().size();
     });
 }
 
@@ -586,7 +977,7 @@ gq_status gq_grid_block_count(gq_ctx *ctx, int *n)
     return guard(ctx, [&] {
         ctx->need_grid();
         if (!n) throw gq_error(GQ_ERR_ARG, "null output");
-        *n = (int)ctx->grid->leaves().size();
+        *n = (int)ctx->G().leaves().size();
     });
 }
 
@@ -594,13 +985,14 @@ gq_status gq_grid_blocks_info(gq_ctx *ctx, gq_block_info *out, int capacity, int
 {
     return guard(ctx, [&] {
         ctx->ensure_tables();
-        const auto &L = ctx->grid->leaves();
+        const auto &L = ctx->G().leaves();
+        const auto &order = ctx->api_order();
         const int m = (int)L.size();
         if (n) *n = m;
         if (!out) return;
         if (capacity < m) throw gq_error(GQ_ERR_ARG, "capacity smaller than the block count");
         for (int k = 0; k < m; k++) {
-            const gq::Leaf &lf = L[ctx->api_order[k]];
+            const gq::Leaf &lf = L[order[k]];
             gq_block_info &b = out[k];
             b.block_id = lf.block_id;
             b.level = (int16_t)lf.level;
@@ -616,13 +1008,14 @@ gq_status gq_grid_blocks_info(gq_ctx *ctx, gq_block_info *out, int capacity, int
 static void transfer(gq_ctx *ctx, int field, float *host, int n_blocks, bool write)
 {
     ctx->ensure_tables();
-    const auto &L = ctx->grid->leaves();
+    const auto &L = ctx->G().leaves();
+    const auto &order = ctx->api_order();
     const int m = (int)L.size();
     if (n_blocks != m) throw gq_error(GQ_ERR_ARG, "n_blocks does not match the grid");
     if (!host) throw gq_error(GQ_ERR_ARG, "null buffer");
     float *dst = ctx->field_ptr(field);
     std::vector<int> slots(m);
-    for (int k = 0; k < m; k++) slots[k] = L[ctx->api_order[k]].slot;
+    for (int k = 0; k < m; k++) slots[k] = L[order[k]].slot;
     ctx->d_list.upload(slots, ctx->st);
     ctx->d_dense.reserve((size_t)m * gq::BS3);
     const size_t bytes = (size_t)m * gq::BS3 * sizeof(float);
@@ -647,71 +1040,57 @@ gq_status gq_grid_read_field(gq_ctx *ctx, int field, float *data, int n_blocks)
     return guard(ctx, [&] { transfer(ctx, field, data, n_blocks, false); });
 }
 
+gq_status gq_grid_write_volume(gq_ctx *ctx, int field, const float *vol, int gpd, int sample)
+{
+    return guard(ctx, [&] {
+        ctx->ensure_tables();
+        if (!vol) throw gq_error(GQ_ERR_ARG, "null volume");
+        const gq::Grid &g0 = ctx->G(0);
+        if (gpd != g0.blocks_per_dim() * gq::BS) throw gq_error(GQ_ERR_ARG, "gpd must be 16 * blocks_per_dim");
+        if (sample < -1 || sample >= ctx->nsamples()) throw gq_error(GQ_ERR_ARG, "sample out of range");
+        for (int s = 0; s < ctx->nsamples(); s++)
+            for (const auto &lf : ctx->G(s).leaves())
+                if (lf.level > g0.base_level())
+                    throw gq_error(GQ_ERR_UNSUPPORTED, "a leaf is finer than the volume");
+        float *dst = ctx->field_ptr(field);
+        std::vector<int> slots;
+        for (int s = 0; s < ctx->nsamples(); s++)
+            if (sample < 0 || s == sample)
+                for (const auto &lf : ctx->G(s).leaves()) slots.push_back(lf.slot);
+        const size_t n = (size_t)gpd * gpd * gpd;
+        ctx->d_dense.reserve(n);
+        ctx->d_list.upload(slots, ctx->st);
+        HIPCHK(hipMemcpyAsync(ctx->d_dense.p, vol, n * sizeof(float), hipMemcpyHostToDevice, ctx->st));
+        HIPCHK(gq::launch_sample_volume(dst, ctx->d_dense.p, gpd, g0.base_level(), ctx->d_list.p, (int)slots.size(),
+                                        ctx->d_level.p, ctx->d_index.p, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        if (field != GQ_FIELD_PHI) ctx->fik_valid = false;
+    });
+}
+
+static void check_params(const gq_fk_params &p)
+{
+    if (!std::isfinite(p.Dw) || !std::isfinite(p.Dg) || !std::isfinite(p.rho) || !std::isfinite(p.dt) || p.dt <= 0)
+        throw gq_error(GQ_ERR_NUMERIC, "non-finite parameter or dt <= 0");
+}
+
 gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
 {
     return guard(ctx, [&] {
         if (!p || n_steps < 0) throw gq_error(GQ_ERR_ARG, "bad arguments");
-        if (!std::isfinite(p->Dw) || !std::isfinite(p->Dg) || !std::isfinite(p->rho) || !std::isfinite(p->dt) ||
-            p->dt <= 0)
-            throw gq_error(GQ_ERR_NUMERIC, "non-finite parameter or dt <= 0");
-        // the operator stores Dw, Dg, rho as Real (float)
-        const float Dw = (float)p->Dw, Dg = (float)p->Dg, rho = (float)p->rho;
-        ctx->ensure_fik(Dw, Dg);
-        const int nwork = (int)ctx->api_order.size();
+        check_params(*p);
+        ctx->ensure_fik({(float)p->Dw, (float)p->Dg}, false);
+        gq::StepArgs a = ctx->step_args(p);
+        const int nwork = (int)ctx->work.size();
         const int nrows = ctx->face_nrows;
-        gq::StepArgs a;
-        a.pw = ctx->pw;
-        a.pg = ctx->pg;
-        a.work = ctx->d_work.p;
-        a.nwork = nwork;
-        a.fnbr = ctx->d_fnbr.p;
-        a.fbase = ctx->d_fbase.p;
-        a.fik = ctx->d_fik.p;
-        a.gphi = ctx->d_gphi.p;
-        a.level = ctx->d_level.p;
-        a.ih2_lvl = ctx->d_ih2.p;
-        a.Dw = Dw;
-        a.Dg = Dg;
-        a.rho = rho;
-        a.dt = (float)p->dt;
         const int zt = ctx->choose_zt(a, nrows);
         ctx->last_zt = zt;
         const int chunk = 1024;
-        const int need_ev = ctx->per_kernel_timing ? 2 * std::min(n_steps, chunk) : 2;
-        while ((int)ctx->events.size() < need_ev) {
-            hipEvent_t e;
-            HIPCHK(hipEventCreate(&e));
-            ctx->events.push_back(e);
-        }
         double total = 0.0;
         int done = 0;
         while (done < n_steps) {
             const int m = std::min(chunk, n_steps - done);
-            if (!ctx->per_kernel_timing) HIPCHK(hipEventRecord(ctx->events[0], ctx->st));
-            for (int s = 0; s < m; s++) {
-                if (nrows > 0)
-                    HIPCHK(gq::launch_face_ghosts(ctx->phi[ctx->cur], ctx->d_face_ptr.p, ctx->d_face_src.p,
-                                                  ctx->d_face_w.p, ctx->d_gphi.p, nrows, ctx->st));
-                a.phi_in = ctx->phi[ctx->cur];
-                a.phi_out = ctx->phi[ctx->cur ^ 1];
-                if (ctx->per_kernel_timing) HIPCHK(hipEventRecord(ctx->events[2 * s], ctx->st));
-                HIPCHK(gq::launch_fk_step(a, zt, ctx->st));
-                if (ctx->per_kernel_timing) HIPCHK(hipEventRecord(ctx->events[2 * s + 1], ctx->st));
-                ctx->cur ^= 1;
-            }
-            if (!ctx->per_kernel_timing) HIPCHK(hipEventRecord(ctx->events[1], ctx->st));
-            HIPCHK(hipStreamSynchronize(ctx->st));
-            if (ctx->per_kernel_timing) {
-                for (int s = 0; s < m; s++) {
-                    float ms = 0;
-                    HIPCHK(hipEventElapsedTime(&ms, ctx->events[2 * s], ctx->events[2 * s + 1]));
-                    total += ms;
-                }
-            } else {
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
-                total += ms;
-            }
+            total += ctx->advance(a, zt, std::vector<int>(m, nwork), std::vector<int>(m, nrows), false);
             done += m;
         }
         ctx->last_ms = total;
@@ -720,13 +1099,123 @@ gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
     });
 }
 
+gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_steps, const gq_run_opts *o)
+{
+    return guard(ctx, [&] {
+        ctx->need_grid();
+        const int S = ctx->nsamples();
+        if (!p || !n_steps || !o) throw gq_error(GQ_ERR_ARG, "null argument");
+        if (o->adapt_every < 0) throw gq_error(GQ_ERR_ARG, "adapt_every must be >= 0");
+        for (int s = 0; s < S; s++) {
+            check_params(p[s]);
+            if (n_steps[s] < 0) throw gq_error(GQ_ERR_ARG, "negative step count");
+        }
+        const bool use_graph = (o->flags & GQ_RUN_GRAPH) != 0;
+        const bool adaptive = o->adapt_every > 0;
+        // samples in decreasing step count: the active samples of any step are a prefix of the work list
+        std::vector<int> order(S);
+        for (int s = 0; s < S; s++) order[s] = s;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n_steps[x] > n_steps[y]; });
+        if (order != ctx->sample_order) {
+            ctx->sample_order = order;
+            ctx->tables_valid = false;
+        }
+        std::vector<float> prm((size_t)S * 4), key((size_t)S * 2);
+        for (int s = 0; s < S; s++) {
+            prm[4 * s + 0] = (float)p[s].Dw;  // Real, as the operator stores them
+            prm[4 * s + 1] = (float)p[s].Dg;
+            prm[4 * s + 2] = (float)p[s].rho;
+            prm[4 * s + 3] = (float)p[s].dt;
+            key[2 * s] = prm[4 * s];
+            key[2 * s + 1] = prm[4 * s + 1];
+        }
+        ctx->d_sprm.upload(prm, ctx->st);
+        std::vector<int> rem(n_steps, n_steps + S);
+        const int K = adaptive ? o->adapt_every : std::max(1, std::min(1024, *std::max_element(rem.begin(), rem.end())));
+        double ms = 0.0, t_adapt = 0.0, cells = 0.0;
+        int chunks = 0, adapts = 0;
+        while (true) {
+            const int mx = *std::max_element(rem.begin(), rem.end());
+            if (mx <= 0) break;
+            const int m = std::min(K, mx);
+            ctx->ensure_fik(key, true);
+            // per-step prefixes
+            std::vector<int> nw(m), nr(m);
+            for (int j = 0; j < m; j++) {
+                int npos = 0;
+                while (npos < S && rem[ctx->sample_order[npos]] > j) npos++;
+                nw[j] = ctx->work_begin[npos];
+                nr[j] = ctx->face_begin[npos] * 256;
+                cells += (double)nw[j] * gq::BS3;
+            }
+            gq::StepArgs a = ctx->step_args(nullptr);
+            a.nwork = nw[0];
+            const int zt = ctx->choose_zt(a, nr[0]);
+            ctx->last_zt = zt;
+            const int c0 = ctx->cur;
+            ms += ctx->advance(a, zt, nw, nr, use_graph);
+            chunks++;
+            // a sample that took a number of steps of the other parity holds its state in the other buffer
+            for (int pos = 0; pos < S; pos++) {
+                const int s = ctx->sample_order[pos];
+                const int ms_s = std::min(rem[s], m);
+                if ((ms_s & 1) != (m & 1)) {
+                    const int b0 = ctx->work_begin[pos], b1 = ctx->work_begin[pos + 1];
+                    HIPCHK(gq::launch_copy_blocks(ctx->phi[ctx->cur], ctx->phi[c0 ^ (ms_s & 1)], ctx->d_work.p + b0,
+                                                  b1 - b0, ctx->st));
+                }
+            }
+            std::vector<int> adapt;
+            for (int s = 0; s < S; s++) {
+                const int ms_s = std::min(rem[s], m);
+                rem[s] -= ms_s;
+                if (adaptive && ms_s == K) adapt.push_back(s);
+            }
+            if (!adapt.empty()) {
+                const double t0 = now_s();
+                ctx->auto_refine(adapt, o->tol_refine, o->max_level, 1);
+                ctx->auto_compress(adapt, o->tol_compress, -1);
+                t_adapt += now_s() - t0;
+                adapts++;
+            }
+        }
+        if (adaptive) {  // final refinement before the dump (Glioma/Glioma_HG_UQ.cpp:574-575)
+            std::vector<int> all(S);
+            for (int s = 0; s < S; s++) all[s] = s;
+            const double t0 = now_s();
+            ctx->auto_refine(all, o->tol_refine, o->max_level, 1);
+            t_adapt += now_s() - t0;
+        }
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        ctx->ens_stats[0] = cells;
+        ctx->ens_stats[1] = ms;
+        ctx->ens_stats[2] = chunks;
+        ctx->ens_stats[3] = adapts;
+        ctx->ens_stats[4] = t_adapt;
+        ctx->ens_stats[5] = (double)ctx->work.size();
+        ctx->ens_stats[6] = ctx->last_zt;
+        ctx->ens_stats[7] = use_graph ? 1 : 0;
+    });
+}
+
+gq_status gq_ensemble_stats(gq_ctx *ctx, double out[8])
+{
+    return guard(ctx, [&] {
+        if (!out) throw gq_error(GQ_ERR_ARG, "null output");
+        for (int k = 0; k < 8; k++) out[k] = ctx->ens_stats[k];
+    });
+}
+
 gq_status gq_fwt_details(gq_ctx *ctx, double *out, int n_blocks)
 {
     return guard(ctx, [&] {
         ctx->ensure_tables();
-        const int m = (int)ctx->grid->leaves().size();
+        const auto &L = ctx->G().leaves();
+        const int m = (int)L.size();
         if (n_blocks != m || !out) throw gq_error(GQ_ERR_ARG, "n_blocks does not match the grid");
-        std::vector<double> d = ctx->fwt(ctx->api_order);
+        std::vector<int> slots;
+        for (int li : ctx->api_order()) slots.push_back(L[li].slot);
+        std::vector<double> d = ctx->fwt_slots(slots);
         std::copy(d.begin(), d.end(), out);
     });
 }
@@ -735,9 +1224,9 @@ gq_status gq_refine_blocks(gq_ctx *ctx, const int32_t *ids, int n, int *n_out)
 {
     return guard(ctx, [&] {
         if (n < 0 || (n > 0 && !ids)) throw gq_error(GQ_ERR_ARG, "bad id list");
-        std::unordered_set<int> s(ids, ids + n);
-        const int r = ctx->refine_ids(s);
-        if (n_out) *n_out = r;
+        std::vector<std::unordered_set<int>> s(1, std::unordered_set<int>(ids, ids + n));
+        const std::vector<int> r = ctx->refine_batch({ctx->cur_sample}, s);
+        if (n_out) *n_out = r[0];
     });
 }
 
@@ -745,67 +1234,27 @@ gq_status gq_compress_blocks(gq_ctx *ctx, const int32_t *ids, int n, int *n_out)
 {
     return guard(ctx, [&] {
         if (n < 0 || (n > 0 && !ids)) throw gq_error(GQ_ERR_ARG, "bad id list");
-        std::unordered_set<int> s(ids, ids + n);
-        const int r = ctx->compress_ids(s);
-        if (n_out) *n_out = r;
+        std::vector<std::unordered_set<int>> s(1, std::unordered_set<int>(ids, ids + n));
+        const std::vector<int> r = ctx->compress_batch({ctx->cur_sample}, s);
+        if (n_out) *n_out = r[0];
     });
 }
 
 gq_status gq_auto_refine(gq_ctx *ctx, double tol, int max_level, int max_loops, int *n_out)
 {
     return guard(ctx, [&] {
-        ctx->ensure_tables();
-        std::unordered_set<int> nice;
-        int loops = 0, total = 0;
-        while (true) {
-            ctx->ensure_tables();
-            const auto &L = ctx->grid->leaves();
-            std::vector<int> cand;
-            for (int li : ctx->api_order) {
-                if (max_level >= 0 && max_level <= L[li].level) continue;
-                if (!nice.count(L[li].block_id)) cand.push_back(li);
-            }
-            if (cand.empty()) break;
-            std::vector<double> det = ctx->fwt(cand);
-            std::unordered_set<int> flagged;
-            for (size_t k = 0; k < cand.size(); k++) {
-                if (det[k] >= tol) flagged.insert(L[cand[k]].block_id);
-                else nice.insert(L[cand[k]].block_id);
-            }
-            if (flagged.empty()) break;
-            total += ctx->refine_ids(flagged);
-            loops++;
-            if (max_loops > 0 && loops >= max_loops) break;
-        }
-        if (n_out) *n_out = total;
+        ctx->need_grid();
+        const std::vector<int> r = ctx->auto_refine({ctx->cur_sample}, tol, max_level, max_loops);
+        if (n_out) *n_out = r[0];
     });
 }
 
 gq_status gq_auto_compress(gq_ctx *ctx, double tol, int max_loops, int *n_out)
 {
     return guard(ctx, [&] {
-        std::unordered_set<int> nice;
-        int loops = 0, total = 0;
-        while (true) {
-            ctx->ensure_tables();
-            const auto &L = ctx->grid->leaves();
-            std::vector<int> cand;
-            for (int li : ctx->api_order)
-                if (!nice.count(L[li].block_id)) cand.push_back(li);
-            std::vector<double> det = ctx->fwt(cand);
-            std::unordered_set<int> flagged;
-            for (size_t k = 0; k < cand.size(); k++) {
-                if (det[k] < tol) flagged.insert(L[cand[k]].block_id);
-                else nice.insert(L[cand[k]].block_id);
-            }
-            if (flagged.empty()) break;
-            const int n = ctx->compress_ids(flagged);
-            if (n == 0) break;
-            total += n;
-            loops++;
-            if (max_loops > 0 && loops >= max_loops) break;
-        }
-        if (n_out) *n_out = total;
+        ctx->need_grid();
+        const std::vector<int> r = ctx->auto_compress({ctx->cur_sample}, tol, max_loops);
+        if (n_out) *n_out = r[0];
     });
 }
 
@@ -854,7 +1303,7 @@ gq_status gq_loglik_flip(gq_ctx *ctx, const double params[4], double out[4])
         const int nb = gq::reduce_blocks(n);
         ctx->d_partial.reserve((size_t)nb * 3 + 8);
         double *res = ctx->d_partial.p + (size_t)nb * 3;
-        HIPCHK(gq::launch_lik_flip(u, ctx->d_t1u8.p, ctx->d_t2u8.p, ctx->d_petf.p, th1, th2, lp1, lp0, c0, is2,
+        HIPCHK(gq::launch_lik_flip(u, 1, ctx->d_t1u8.p, ctx->d_t2u8.p, ctx->d_petf.p, th1, th2, lp1, lp0, c0, is2,
                                    ctx->d_partial.p, nb, res, ctx->st));
         double h[3];
         HIPCHK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, ctx->st));
@@ -863,6 +1312,37 @@ gq_status gq_loglik_flip(gq_ctx *ctx, const double params[4], double out[4])
         out[2] = h[1];
         out[3] = h[2];
         out[0] = h[0] + h[1] + h[2];
+    });
+}
+
+gq_status gq_ensemble_loglik_flip(gq_ctx *ctx, const double params[4], double *out)
+{
+    return guard(ctx, [&] {
+        if (!params || !out) throw gq_error(GQ_ERR_ARG, "null argument");
+        const int S = ctx->nsamples();
+        for (int s = 1; s < S; s++) (void)ctx->uniform_args(s);  // every sample must map to the uniform view
+        gq::UniformArgs u = ctx->uniform_args(0);
+        if (ctx->obs_flip_gpd != u.gpd) throw gq_error(GQ_ERR_STATE, "observations missing or of another size");
+        const double th1 = params[0], th2 = params[1], eps = params[2], sigma = params[3];
+        if (!(eps > 0 && eps < 1) || !(sigma > 0)) throw gq_error(GQ_ERR_NUMERIC, "flip rate / sigma out of range");
+        const double lp1 = std::log(1. - eps), lp0 = std::log(eps);
+        const double c0 = -0.5 * std::log(2. * M_PI * sigma * sigma);
+        const double is2 = 1. / (2. * sigma * sigma);
+        const int64_t n = (int64_t)u.gpd * u.gpd * u.gpd;
+        const int nb = gq::reduce_blocks(n);
+        ctx->d_partial.reserve((size_t)S * nb * 3 + (size_t)S * 3);
+        double *res = ctx->d_partial.p + (size_t)S * nb * 3;
+        HIPCHK(gq::launch_lik_flip(u, S, ctx->d_t1u8.p, ctx->d_t2u8.p, ctx->d_petf.p, th1, th2, lp1, lp0, c0, is2,
+                                   ctx->d_partial.p, nb, res, ctx->st));
+        std::vector<double> h((size_t)S * 3);
+        HIPCHK(hipMemcpyAsync(h.data(), res, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        for (int s = 0; s < S; s++) {
+            out[4 * s + 1] = h[3 * s];
+            out[4 * s + 2] = h[3 * s + 1];
+            out[4 * s + 3] = h[3 * s + 2];
+            out[4 * s] = h[3 * s] + h[3 * s + 1] + h[3 * s + 2];
+        }
     });
 }
 
@@ -1010,7 +1490,7 @@ gq_status gq_memory_info(gq_ctx *ctx, int64_t *bytes, int *pool_blocks)
     return guard(ctx, [&] {
         int64_t b = (int64_t)ctx->cap * gq::BS3 * 4 * 4;
         b += (int64_t)(ctx->d_ghost.n * 8);
-        b += (int64_t)(ctx->d_face_w.n * 4 + ctx->d_face_src.n * 4 + ctx->d_face_ptr.n * 4 + ctx->d_fik.n * 4);
+        b += (int64_t)(ctx->d_face_w.n * 4 + ctx->d_face_src.n * 4 + ctx->d_face_ptr.n * 8 + ctx->d_fik.n * 4);
         if (bytes) *bytes = b;
         if (pool_blocks) *pool_blocks = ctx->cap;
     });

@@ -308,29 +308,27 @@ int Grid::expand_point(int leaf, int lx, int ly, int lz, std::vector<Term> &scra
 // ---------------------------------------------------------------------------
 // Tables
 // ---------------------------------------------------------------------------
-void build_tables(const Grid &g, int nslots, Tables &t)
+void build_sample_tables(const Grid &g, int *fnbr, int *nbr26, SampleTables &t)
 {
     static const int face_codes[6] = {12, 14, 10, 16, 4, 22};
     const auto &L = g.leaves();
-    const int n = (int)L.size();
-    t.nslots = nslots;
-    t.fnbr.assign((size_t)nslots * 6, -1);
-    t.nbr26.assign((size_t)nslots * 26, -1);
+    t.order = morton_order(g);
     t.faces.clear();
-    for (int i = 0; i < n; i++) {
+    for (int i : t.order) {
         const Leaf &lf = L[i];
         const int m = 1 << lf.level;
+        int *nb26 = nbr26 + (size_t)lf.slot * 26;
         for (int c = 0; c < 27; c++) {
             if (c == 13) continue;
             const int d[3] = {c % 3 - 1, (c / 3) % 3 - 1, c / 9 - 1};
             const int j = g.find(lf.level, (int)wrapi(lf.idx[0] + d[0], m), (int)wrapi(lf.idx[1] + d[1], m),
                                  (int)wrapi(lf.idx[2] + d[2], m));
-            if (j >= 0) t.nbr26[(size_t)lf.slot * 26 + (c < 13 ? c : c - 1)] = L[j].slot;
+            nb26[c < 13 ? c : c - 1] = j >= 0 ? L[j].slot : -1;
         }
         for (int f = 0; f < 6; f++) {
             const int c = face_codes[f];
-            const int nb = t.nbr26[(size_t)lf.slot * 26 + (c < 13 ? c : c - 1)];
-            t.fnbr[(size_t)lf.slot * 6 + f] = nb;
+            const int nb = nb26[c < 13 ? c : c - 1];
+            fnbr[(size_t)lf.slot * 6 + f] = nb;
             if (nb < 0) t.faces.push_back(lf.slot * 6 + f);
         }
     }

@@ -106,21 +106,23 @@ inline uint64_t leaf_key(int l, int x, int y, int z)
     return ((uint64_t)l << 60) | ((uint64_t)(uint32_t)x << 40) | ((uint64_t)(uint32_t)y << 20) | (uint64_t)(uint32_t)z;
 }
 
-// Per-slot device tables derived from a Grid.  Ghost values across levels
+// Per-sample device tables derived from a Grid.  Ghost values across levels
 // are evaluated on the device (depth-first over the finest-level look-up);
 // the host only provides neighbour slots and the look-ups.
-struct Tables {
-    int nslots = 0;                 // table capacity in slots
-    // f = 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z; same-level face neighbour slot or -1
-    std::vector<int> fnbr;          // [nslots*6]
-    // wide lab regions c = (dx+1) + 3(dy+1) + 9(dz+1) (26 without the centre)
-    std::vector<int> nbr26;         // [nslots*26]
-    std::vector<int> faces;         // slot*6+f of every cross-level face
-    std::vector<int> lut;           // (2^lut_level)^3 finest block cells -> slot
-    std::vector<int> lut_slot;      // (bpd)^3 base-level blocks -> slot (uniform view)
+struct SampleTables {
+    std::vector<int> order;     // leaf indices in Morton order
+    std::vector<int> faces;     // slot*6+f of every cross-level face, leaves in Morton order
+    std::vector<int> lut;       // (2^lut_level)^3 finest block cells -> slot
+    std::vector<int> lut_slot;  // bpd^3 base-level blocks -> slot (uniform view)
 };
 
-void build_tables(const Grid &g, int nslots, Tables &t);
+// Writes, for every leaf slot of g (other slots are left untouched):
+//   fnbr[slot*6+f]   same-level face neighbour slot or -1
+//                    (f = 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z)
+//   nbr26[slot*26+c] same-level neighbour per wide lab region or -1
+//                    (c = (dx+1) + 3(dy+1) + 9(dz+1), the centre left out)
+// and fills t.  Grids whose leaves own disjoint slots can be built in parallel.
+void build_sample_tables(const Grid &g, int *fnbr, int *nbr26, SampleTables &t);
 
 // Morton order of the leaves (for the launch work list).
 std::vector<int> morton_order(const Grid &g);

@@ -12,6 +12,8 @@
 
 #include "kernels.hpp"
 
+#include <hipcub/device/device_scan.hpp>
+
 namespace gq {
 
 namespace {
@@ -82,7 +84,14 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
     const int t = threadIdx.x;
     const int tx = t & 15, ty = t >> 4;
     const size_t base = (size_t)slot * B3;
-    const float Dw = a.Dw, Dg = a.Dg;
+    float Dw = a.Dw, Dg = a.Dg, rho = a.rho, dt = a.dt;
+    if (a.sprm) {
+        const float *q = a.sprm + 4 * a.slot_sample[slot];
+        Dw = q[0];
+        Dg = q[1];
+        rho = q[2];
+        dt = q[3];
+    }
 
     float breg[ZT];
 #pragma unroll
@@ -139,7 +148,6 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
     __syncthreads();
 
     const float ih2 = a.ih2_lvl[a.level[slot]];
-    const float dt = a.dt, rho = a.rho;
 #pragma unroll 2
     for (int k = 0; k < ZT; k++) {
         const int c = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
@@ -187,7 +195,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
 
 // Cross-level face ghosts of phi for the step: four lanes per row, partial
 // sums combined with lane shuffles.
-__global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ phi, const int *__restrict__ ptr,
+__global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ phi, const int64_t *__restrict__ ptr,
                                                      const int *__restrict__ src, const float *__restrict__ w,
                                                      float *__restrict__ out, int nrows)
 {
@@ -195,7 +203,7 @@ __global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ p
     const int r = g >> 2, lane = g & 3;
     float acc = 0.f;
     if (r < nrows)
-        for (int k = ptr[r] + lane; k < ptr[r + 1]; k += 4) acc += w[k] * phi[src[k]];
+        for (int64_t k = ptr[r] + lane; k < ptr[r + 1]; k += 4) acc += w[k] * phi[src[k]];
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     if (r < nrows && lane == 0) out[r] = acc;
@@ -205,14 +213,19 @@ __global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ p
 __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw, const float *__restrict__ pg,
                                                    const int *__restrict__ work, int nwork,
                                                    const int *__restrict__ fnbr, const int *__restrict__ fbase,
-                                                   const int *__restrict__ ptr, const int *__restrict__ src,
+                                                   const int64_t *__restrict__ ptr, const int *__restrict__ src,
                                                    const float *__restrict__ w, float *__restrict__ fik, float Dw,
-                                                   float Dg)
+                                                   float Dg, const int *__restrict__ slot_sample,
+                                                   const float *__restrict__ sprm)
 {
     const int wi = blockIdx.x / 6;
     const int f = blockIdx.x % 6;
     if (wi >= nwork) return;
     const int slot = work[wi];
+    if (sprm) {
+        Dw = sprm[4 * slot_sample[slot]];
+        Dg = sprm[4 * slot_sample[slot] + 1];
+    }
     const int t = threadIdx.x;
     const int nb = fnbr[slot * 6 + f];
     float wm, gm;
@@ -224,7 +237,7 @@ __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw,
         const int r = fbase[slot * 6 + f] + t;
         wm = 0.f;
         gm = 0.f;
-        for (int k = ptr[r]; k < ptr[r + 1]; k++) {
+        for (int64_t k = ptr[r]; k < ptr[r + 1]; k++) {
             wm += w[k] * pw[src[k]];
             gm += w[k] * pg[src[k]];
         }
@@ -259,13 +272,14 @@ __device__ __forceinline__ double hs_w(int order, int i)
     }
 }
 
-__device__ __forceinline__ int lying_slot(const LabArgs &g, int lam, int64_t px, int64_t py, int64_t pz)
+__device__ __forceinline__ int lying_slot(const LabArgs &g, const int *__restrict__ lut, int lam, int64_t px,
+                                          int64_t py, int64_t pz)
 {
     // finest block cell: floor(p * 2^(L-lam) / 16), periodic (power-of-two wrap)
     const int sh = g.lut_level - lam;
     const int64_t m = (1LL << g.lut_level) - 1;
     const int64_t fx = ((px << sh) >> 4) & m, fy = ((py << sh) >> 4) & m, fz = ((pz << sh) >> 4) & m;
-    return g.lut[fx + (m + 1) * (fy + (m + 1) * fz)];
+    return lut[fx + (m + 1) * (fy + (m + 1) * fz)];
 }
 
 struct DfsFrame {
@@ -279,8 +293,9 @@ struct DfsFrame {
 // values of fld; 1: count terms (added to *count); 2: emit terms at src/wout
 // (advancing *count).
 template <int MODE>
-__device__ void ghost_dfs(const LabArgs &g, const float *__restrict__ fld, int lam0, int64_t p0x, int64_t p0y,
-                          int64_t p0z, double w0, double &acc, int *count, int *src, float *wout)
+__device__ void ghost_dfs(const LabArgs &g, const int *__restrict__ lut, const float *__restrict__ fld, int lam0,
+                          int64_t p0x, int64_t p0y, int64_t p0z, double w0, double &acc, int *count, int *src,
+                          float *wout)
 {
     const int lo = g.order == 4 ? -3 : -1, hi = g.order == 4 ? 4 : 2;
     DfsFrame st[MAXD];
@@ -291,7 +306,7 @@ __device__ void ghost_dfs(const LabArgs &g, const float *__restrict__ fld, int l
     while (true) {
         // resolve (lam, p, w) to a value or a synthesis frame
         while (true) {
-            const int slot = lying_slot(g, lam, p[0], p[1], p[2]);
+            const int slot = lying_slot(g, lut, lam, p[0], p[1], p[2]);
             const int l = g.level[slot];
             if (l == lam) {
                 const int64_t msk = ((int64_t)16 << lam) - 1;
@@ -362,11 +377,11 @@ __device__ void ghost_dfs(const LabArgs &g, const float *__restrict__ fld, int l
 // the common cases (the point lies in a finer leaf, or in a coarser leaf
 // whose synthesis children all lie in leaves of their own level); anything
 // else continues with the general walk at the child, in the same order.
-__device__ double ghost_value(const LabArgs &g, const float *__restrict__ fld, int lam, int64_t px, int64_t py,
-                              int64_t pz)
+__device__ double ghost_value(const LabArgs &g, const int *__restrict__ lut, const float *__restrict__ fld, int lam,
+                              int64_t px, int64_t py, int64_t pz)
 {
     for (;;) {
-        const int slot = lying_slot(g, lam, px, py, pz);
+        const int slot = lying_slot(g, lut, lam, px, py, pz);
         const int l = g.level[slot];
         if (l == lam) {
             const int64_t msk = ((int64_t)16 << lam) - 1;
@@ -402,14 +417,14 @@ __device__ double ghost_value(const LabArgs &g, const float *__restrict__ fld, i
                 const double ww = __dmul_rn(__dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), wy), wz);
                 if (ww == 0.0) continue;
                 const double cw = __dmul_rn(1.0, ww);
-                const int slot = lying_slot(g, cl, ix, iy, iz);
+                const int slot = lying_slot(g, lut, cl, ix, iy, iz);
                 if (g.level[slot] == cl) {
                     const int cx = (int)((ix & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
                     const int cy = (int)((iy & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
                     const int cz = (int)((iz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
                     acc = __dadd_rn(acc, __dmul_rn(cw, (double)fld[(size_t)slot * 4096 + cx + 16 * (cy + 16 * cz)]));
                 } else {
-                    ghost_dfs<0>(g, fld, cl, ix, iy, iz, cw, acc, nullptr, nullptr, nullptr);
+                    ghost_dfs<0>(g, lut, fld, cl, ix, iy, iz, cw, acc, nullptr, nullptr, nullptr);
                 }
             }
         }
@@ -479,6 +494,7 @@ __device__ void lab_ghosts_block(const LabArgs &a, const float *__restrict__ fld
 {
     constexpr int L = LabGeom<S0, S1>::L;
     const int lam = a.level[slot];
+    const int *lut = a.lut + a.lutoff[slot];
     const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
     for (int cc = 0; cc < 26; cc++) {
@@ -496,7 +512,7 @@ __device__ void lab_ghosts_block(const LabArgs &a, const float *__restrict__ fld
             if (!lab_coord_used<ORDER, FOR_SPLIT>(x) || !lab_coord_used<ORDER, FOR_SPLIT>(y) ||
                 !lab_coord_used<ORDER, FOR_SPLIT>(z))
                 continue;
-            out[(x - S0) + L * ((y - S0) + L * (z - S0))] = ghost_value(a, fld, lam, ox + x, oy + y, oz + z);
+            out[(x - S0) + L * ((y - S0) + L * (z - S0))] = ghost_value(a, lut, fld, lam, ox + x, oy + y, oz + z);
         }
     }
 }
@@ -538,12 +554,13 @@ __global__ __launch_bounds__(256) void k_face_rows_count(LabArgs a, const int *_
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
     int n = 0;
     double unused = 0.0;
-    ghost_dfs<1>(a, nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, unused, &n, nullptr, nullptr);
-    counts[k * 256 + threadIdx.x] = n;
+    ghost_dfs<1>(a, a.lut + a.lutoff[slot], nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, unused, &n, nullptr,
+                 nullptr);
+    counts[(int64_t)k * 256 + threadIdx.x] = n;
 }
 
 __global__ __launch_bounds__(256) void k_face_rows_fill(LabArgs a, const int *__restrict__ faces, int nfaces,
-                                                        const int *__restrict__ ptr, int *__restrict__ src,
+                                                        const int64_t *__restrict__ ptr, int *__restrict__ src,
                                                         float *__restrict__ w)
 {
     const int k = blockIdx.x;
@@ -553,10 +570,11 @@ __global__ __launch_bounds__(256) void k_face_rows_fill(LabArgs a, const int *__
     face_point(f, threadIdx.x & 15, threadIdx.x >> 4, x, y, z);
     const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
-    const int r = k * 256 + threadIdx.x;
+    const int64_t r = (int64_t)k * 256 + threadIdx.x;
     int n = 0;
     double unused = 0.0;
-    ghost_dfs<2>(a, nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, unused, &n, src + ptr[r], w + ptr[r]);
+    ghost_dfs<2>(a, a.lut + a.lutoff[slot], nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, unused, &n,
+                 src + ptr[r], w + ptr[r]);
 }
 
 namespace {
@@ -741,7 +759,7 @@ __device__ __forceinline__ bool voxel_value(const UniformArgs &u, int64_t v, dou
     const int gpd = u.gpd;
     const int vx = (int)(v % gpd), vy = (int)((v / gpd) % gpd), vz = (int)(v / ((int64_t)gpd * gpd));
     const int bpd = gpd >> 4;
-    const int slot = u.lut_slot[(vx >> 4) + bpd * ((vy >> 4) + bpd * (vz >> 4))];
+    const int slot = u.lut_slot[(size_t)blockIdx.y * bpd * bpd * bpd + (vx >> 4) + bpd * ((vy >> 4) + bpd * (vz >> 4))];
     const int lvl = u.level[slot];
     const int r = bpd >> lvl;  // voxels per leaf cell per axis (>= 1)
     const int ox = u.index[slot * 3 + 0] * B * r, oy = u.index[slot * 3 + 1] * B * r, oz = u.index[slot * 3 + 2] * B * r;
@@ -807,7 +825,7 @@ __global__ __launch_bounds__(256) void k_lik_flip(UniformArgs u, const uint8_t *
             acc[2] += c0 - r * r * is2;
         }
     }
-    block_sum_store<3>(acc, partial);
+    block_sum_store<3>(acc, partial + (size_t)blockIdx.y * gridDim.x * 3);
 }
 
 // Reference model, PET part (HGG_Likelihood.cpp:40-75): lattice
@@ -901,8 +919,8 @@ hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_face_ghosts(const float *phi, const int *ptr, const int *src, const float *w, float *out, int nrows,
-                              hipStream_t st)
+hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *src, const float *w, float *out,
+                              int nrows, hipStream_t st)
 {
     if (nrows <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_face_ghosts, dim3((4 * nrows + 255) / 256), dim3(256), 0, st, phi, ptr, src, w, out, nrows);
@@ -910,12 +928,12 @@ hipError_t launch_face_ghosts(const float *phi, const int *ptr, const int *src, 
 }
 
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
-                            const int *fbase, const int *ptr, const int *src, const float *w, float *fik, float Dw,
-                            float Dg, hipStream_t st)
+                            const int *fbase, const int64_t *ptr, const int *src, const float *w, float *fik, float Dw,
+                            float Dg, const int *slot_sample, const float *sprm, hipStream_t st)
 {
     if (nwork <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_face_prep, dim3(nwork * 6), dim3(256), 0, st, pw, pg, work, nwork, fnbr, fbase, ptr, src, w,
-                       fik, Dw, Dg);
+                       fik, Dw, Dg, slot_sample, sprm);
     return hipGetLastError();
 }
 
@@ -962,8 +980,8 @@ hipError_t launch_face_rows_count(const LabArgs &a, const int *faces, int nfaces
     return hipGetLastError();
 }
 
-hipError_t launch_face_rows_fill(const LabArgs &a, const int *faces, int nfaces, const int *ptr, int *src, float *w,
-                                 hipStream_t st)
+hipError_t launch_face_rows_fill(const LabArgs &a, const int *faces, int nfaces, const int64_t *ptr, int *src,
+                                 float *w, hipStream_t st)
 {
     if (nfaces <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_face_rows_fill, dim3(nfaces), dim3(256), 0, st, a, faces, nfaces, ptr, src, w);
@@ -995,13 +1013,15 @@ int reduce_blocks(int64_t n)
     return (int)nb;
 }
 
-hipError_t launch_lik_flip(const UniformArgs &u, const uint8_t *t1, const uint8_t *t2, const float *pet, double th1,
-                           double th2, double lp1, double lp0, double c0, double is2, double *partial, int nblocks,
-                           double *out, hipStream_t st)
+hipError_t launch_lik_flip(const UniformArgs &u, int nsamples, const uint8_t *t1, const uint8_t *t2, const float *pet,
+                           double th1, double th2, double lp1, double lp0, double c0, double is2, double *partial,
+                           int nblocks, double *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_lik_flip, dim3(nblocks), dim3(256), 0, st, u, t1, t2, pet, th1, th2, lp1, lp0, c0, is2,
-                       partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial, nblocks, 3, out);
+    hipLaunchKernelGGL(k_lik_flip, dim3(nblocks, nsamples), dim3(256), 0, st, u, t1, t2, pet, th1, th2, lp1, lp0, c0,
+                       is2, partial);
+    for (int s = 0; s < nsamples; s++)
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial + (size_t)s * nblocks * 3, nblocks, 3,
+                           out + 3 * s);
     return hipGetLastError();
 }
 
@@ -1047,5 +1067,56 @@ hipError_t launch_gather_blocks(const float *pool, float *dense, const int *slot
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather_blocks, dim3(n), dim3(256), 0, st, pool, dense, slots);
     return hipGetLastError();
+}
+}  // namespace gq
+
+namespace gq {
+__global__ __launch_bounds__(256) void k_copy_blocks(float *dst, const float *__restrict__ src,
+                                                     const int *__restrict__ slots)
+{
+    const size_t b = (size_t)slots[blockIdx.x] * 4096;
+    for (int i = threadIdx.x; i < 4096; i += 256) dst[b + i] = src[b + i];
+}
+hipError_t launch_copy_blocks(float *dst, const float *src, const int *slots, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_blocks, dim3(n), dim3(256), 0, st, dst, src, slots);
+    return hipGetLastError();
+}
+}  // namespace gq
+
+namespace gq {
+// Point-sample a uniform gpd^3 volume (x fastest) onto the listed slots: a
+// leaf at level l takes every 2^(base-l)-th point of its region.
+__global__ __launch_bounds__(256) void k_sample_volume(float *__restrict__ pool, const float *__restrict__ vol,
+                                                       int gpd, int base_level, const int *__restrict__ slots,
+                                                       const int *__restrict__ level, const int *__restrict__ index)
+{
+    const int slot = slots[blockIdx.x];
+    const int r = 1 << (base_level - level[slot]);
+    const int64_t ox = (int64_t)index[slot * 3 + 0] * B * r, oy = (int64_t)index[slot * 3 + 1] * B * r,
+                  oz = (int64_t)index[slot * 3 + 2] * B * r;
+    const size_t b = (size_t)slot * B3;
+    for (int i = threadIdx.x; i < B3; i += 256) {
+        const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
+        pool[b + i] = vol[(ox + x * r) + gpd * ((oy + y * r) + (int64_t)gpd * (oz + z * r))];
+    }
+}
+
+hipError_t launch_sample_volume(float *pool, const float *vol, int gpd, int base_level, const int *slots, int n,
+                                const int *level, const int *index, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sample_volume, dim3(n), dim3(256), 0, st, pool, vol, gpd, base_level, slots, level, index);
+    return hipGetLastError();
+}
+
+hipError_t scan_counts(const int *counts, int64_t *ptr, int n, void *temp, size_t &temp_bytes, hipStream_t st)
+{
+    if (!temp) return hipcub::DeviceScan::InclusiveSum(nullptr, temp_bytes, counts, ptr + 1, n, st);
+    hipError_t e = hipMemsetAsync(ptr, 0, sizeof(int64_t), st);
+    if (e != hipSuccess) return e;
+    if (n <= 0) return hipSuccess;
+    return hipcub::DeviceScan::InclusiveSum(temp, temp_bytes, counts, ptr + 1, n, st);
 }
 }  // namespace gq

@@ -20,14 +20,17 @@ struct StepArgs {
     const float *gphi; // cross-level face ghosts of phi_in
     const int *level;  // [slot]
     const float *ih2_lvl;  // 1/h^2 per level
-    float Dw, Dg, rho, dt;
+    float Dw, Dg, rho, dt;     // used when sprm is null
+    const int *slot_sample;    // [slot] sample of the block (ensembles)
+    const float *sprm;         // [sample*4] Dw, Dg, rho, dt (ensembles) or null
 };
 
 // Location tables for ghost evaluation on the device: the leaf covering a
 // point is found through a look-up over the finest level.
 struct LabArgs {
     const int *nbr26;  // [slot*26] same-level neighbour slot per region or -1
-    const int *lut;    // (2^lut_level)^3 finest block cells -> slot
+    const int *lut;    // per sample (2^lut_level)^3 finest block cells -> slot, concatenated
+    const int *lutoff; // [slot] offset of the slot's sample look-up in lut
     const int *level;  // [slot]
     const int *index;  // [slot*3]
     int lut_level;
@@ -37,20 +40,51 @@ struct LabArgs {
 struct UniformArgs {
     const double *vol;  // if set: a uniform gpd^3 model volume (x fastest) replaces the grid view
     const float *phi;
-    const int *lut_slot;
+    const int *lut_slot;  // bpd^3 per sample, concatenated (sample = blockIdx.y)
     const int *level;
     const int *index;  // [slot*3]
     int gpd;
     double hole;
 };
 
+// One backward step of the discrete adjoint on uniform grids (every face
+// neighbour at the same level): lambda_n from lambda_{n+1}, u_n and u_{n+1}.
+struct AdjArgs {
+    const float *u;       // u_n
+    const float *u_next;  // u_{n+1} (its clamp gives the forward's mask)
+    const float *pw;
+    const float *pg;
+    const float *lam_in;  // lambda_{n+1}
+    float *lam_out;       // lambda_n
+    const int *work;
+    int nwork;
+    const int *fnbr;
+    const int *level;
+    const float *ih2_lvl;
+    const int *slot_sample;
+    const float *sprm;    // [sample*4] Dw, Dg, rho, dt
+    double *gacc;         // [(slot*adj_slabs()+slab)*2]: dL/dlogDw, dL/dlogrho
+};
+int adj_slabs();
+hipError_t launch_adj_step(const AdjArgs &a, hipStream_t st);
+// lam = dL/du of the flip-rate model's PET term at u (uniform grids)
+hipError_t launch_lik_flip_grad(const float *u, const float *pet, const int *work, int nwork, const int *index,
+                                int gpd, double inv_s2, float *lam, hipStream_t st);
+// Gaussian initial condition per sample, icp[s*5] = x0, y0, z0 (voxels), A, sigma (voxels)
+hipError_t launch_ic_gaussian(float *phi, const int *work, int nwork, const int *index, const int *slot_sample,
+                              const double *icp, int gpd, hipStream_t st);
+// out[k*4 + {0..3}] = dL/d(x0, y0, z0, A) of work slot k from lambda_0
+hipError_t launch_ic_grad(const float *lam0, const int *work, int nwork, const int *index, const int *slot_sample,
+                          const double *icp, int gpd, double *out, hipStream_t st);
+
 // zt: slab depth (16, 8, 4 or 2 planes per workgroup)
 hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st);
-hipError_t launch_face_ghosts(const float *phi, const int *ptr, const int *src, const float *w, float *out, int nrows,
-                              hipStream_t st);
+hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *src, const float *w, float *out,
+                              int nrows, hipStream_t st);
+// slot_sample/sprm: per-sample Dw, Dg (ensembles) or null for (Dw, Dg)
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
-                            const int *fbase, const int *ptr, const int *src, const float *w, float *fik, float Dw,
-                            float Dg, hipStream_t st);
+                            const int *fbase, const int64_t *ptr, const int *src, const float *w, float *fik, float Dw,
+                            float Dg, const int *slot_sample, const float *sprm, hipStream_t st);
 // Ghost values of the cross-level points of the wide lab of each listed slot
 // (FP64, depth-first order) into ghost[k * lab_points + q].
 // for_split: the points the split reads; else the points the FWT reads.
@@ -65,15 +99,19 @@ hipError_t launch_split(int order, const LabArgs &a, const double *ghost3, float
 // Step rows of the cross-level faces: faces[k] = slot*6+f; rows k*256 + u + 16v.
 // Pass 1 counts the depth-first terms of every row, pass 2 writes them.
 hipError_t launch_face_rows_count(const LabArgs &a, const int *faces, int nfaces, int *counts, hipStream_t st);
-hipError_t launch_face_rows_fill(const LabArgs &a, const int *faces, int nfaces, const int *ptr, int *src, float *w,
-                                 hipStream_t st);
+hipError_t launch_face_rows_fill(const LabArgs &a, const int *faces, int nfaces, const int64_t *ptr, int *src,
+                                 float *w, hipStream_t st);
+// ptr[0] = 0, ptr[1..n] = inclusive sum of counts (64-bit).  With temp null
+// only sets temp_bytes.
+hipError_t scan_counts(const int *counts, int64_t *ptr, int n, void *temp, size_t &temp_bytes, hipStream_t st);
 hipError_t launch_collapse(float *phi, float *pw, float *pg, const int *parents, const int *children, int nparents,
                            hipStream_t st);
 hipError_t launch_dump_uniform(const UniformArgs &u, double *out, hipStream_t st);
 int reduce_blocks(int64_t n);
-hipError_t launch_lik_flip(const UniformArgs &u, const uint8_t *t1, const uint8_t *t2, const float *pet, double th1,
-                           double th2, double lp1, double lp0, double c0, double is2, double *partial, int nblocks,
-                           double *out, hipStream_t st);
+// nsamples views (lut_slot per sample); partial: nsamples*nblocks*3, out: nsamples*3
+hipError_t launch_lik_flip(const UniformArgs &u, int nsamples, const uint8_t *t1, const uint8_t *t2, const float *pet,
+                           double th1, double th2, double lp1, double lp0, double c0, double is2, double *partial,
+                           int nblocks, double *out, hipStream_t st);
 hipError_t launch_lik_ref(const UniformArgs &u, const double *pet, const double *t1, const double *t2,
                           const int *pts, int npts, int selective, int step_pet, int step_ti, double scale,
                           double t1uc, double t2uc, double slope, double *partial, int nblocks, double *out,
@@ -84,4 +122,8 @@ hipError_t launch_lik_ref(const UniformArgs &u, const double *pet, const double 
 namespace gq {
 hipError_t launch_scatter_blocks(float *pool, const float *dense, const int *slots, int n, hipStream_t st);
 hipError_t launch_gather_blocks(const float *pool, float *dense, const int *slots, int n, hipStream_t st);
+hipError_t launch_copy_blocks(float *dst, const float *src, const int *slots, int n, hipStream_t st);
+// Point-sample a uniform gpd^3 volume (gpd = 16 * 2^base_level) onto the listed slots.
+hipError_t launch_sample_volume(float *pool, const float *vol, int gpd, int base_level, const int *slots, int n,
+                                const int *level, const int *index, hipStream_t st);
 }  // namespace gq

@@ -28,7 +28,7 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import BlockInfo, FKParams, GQError, ptr
+from ._lib import BlockInfo, FKParams, GQError, RunOpts, ptr
 
 BS = 16
 
@@ -133,6 +133,13 @@ class Grid:
             z, y, x = np.meshgrid(b.origin[2] + r, b.origin[1] + r, b.origin[0] + r, indexing="ij")
             blocks.append(fn(x, y, z))
         self.write(name, np.stack(blocks))
+
+    def write_volume(self, name, vol):
+        """Point-sample a uniform volume vol[z,y,x] of gpd^3 = (bpd*16)^3
+        points onto every leaf, on the device (gq_grid_write_volume)."""
+        v = np.ascontiguousarray(vol, np.float32)
+        self.ctx.call("gq_grid_write_volume", self.FIELDS[name], ptr(v, C.c_float), v.shape[0],
+                      getattr(self.ctx, "sample", 0))
 
     def sample_volume(self, name, vol):
         """Point-sample a uniform volume vol[z,y,x] of gpd^3 = (bpd*16)^3 points
@@ -349,6 +356,80 @@ class HGG_Likelihood:
         self.grid.ctx.call("gq_loglik_ref", ptr(self.params, C.c_double), int(self.selective), self.step_pet,
                            self.step_ti, ptr(out, C.c_double))
         return out  # logL, L_pet, L_T1, L_T2
+
+
+class _SampleCtx:
+    """Context proxy that addresses one sample of an ensemble."""
+
+    def __init__(self, ctx, sample):
+        self.base = ctx
+        self.sample = sample
+        self.L = ctx.L
+        self.h = ctx.h
+
+    def call(self, name, *args):
+        self.base.call("gq_sample_select", self.sample)
+        self.base.call(name, *args)
+
+
+class Ensemble:
+    """nsamples independent grids of one configuration advancing together
+    (gq_ensemble_*): one step-kernel launch per step covers the blocks of
+    every running sample; each sample's result equals a one-sample run."""
+
+    def __init__(self, ctx, nsamples, blocks_per_dim, max_level, wavelet_order=4):
+        self.ctx = ctx
+        self.n = nsamples
+        self.bpd = blocks_per_dim
+        self.max_level = max_level
+        self.order = wavelet_order
+        ctx.call("gq_ensemble_create", nsamples, blocks_per_dim, max_level, wavelet_order)
+
+    def sample(self, s):
+        """Grid view of sample s (Grid methods address that sample)."""
+        if not 0 <= s < self.n:
+            raise IndexError(s)
+        g = Grid.__new__(Grid)
+        g.ctx = _SampleCtx(self.ctx, s)
+        g.bpd, g.max_level, g.order = self.bpd, self.max_level, self.order
+        return g
+
+    def write_volume(self, name, vol, sample=-1):
+        v = np.ascontiguousarray(vol, np.float32)
+        self.ctx.call("gq_grid_write_volume", Grid.FIELDS[name], ptr(v, C.c_float), v.shape[0], int(sample))
+
+    def block_counts(self):
+        out = np.zeros(self.n, np.int32)
+        self.ctx.call("gq_ensemble_block_counts", ptr(out, C.c_int32))
+        return out
+
+    def run(self, params, n_steps, adapt_every=10, max_level=None, tol_refine=1e-3, tol_compress=1e-4, graph=True):
+        """params: (nsamples, 4) array of (Dw, Dg, rho, dt) in unit-cube units;
+        n_steps: steps per sample."""
+        prm = np.ascontiguousarray(params, np.float64).reshape(self.n, 4)
+        ns = np.ascontiguousarray(n_steps, np.int32).reshape(self.n)
+        arr = (FKParams * self.n)(*[FKParams(*map(float, r)) for r in prm])
+        o = RunOpts(int(adapt_every), int(self.max_level if max_level is None else max_level), float(tol_refine),
+                    float(tol_compress), _lib.RUN_GRAPH if graph else 0)
+        self.ctx.call("gq_ensemble_run", arr, ptr(ns, C.c_int32), C.byref(o))
+        return self.stats()
+
+    def stats(self):
+        out = np.zeros(8)
+        self.ctx.call("gq_ensemble_stats", ptr(out, C.c_double))
+        keys = ("cell_updates", "device_ms", "chunks", "adaptations", "adapt_s", "final_blocks", "zt", "graph")
+        return dict(zip(keys, out.tolist()))
+
+    def loglik_flip(self, t1, t2, pet, th1=0.7, th2=0.25, flip=0.05, sigma=0.05):
+        gpd = self.bpd * BS
+        keep = (np.ascontiguousarray(t1, np.uint8), np.ascontiguousarray(t2, np.uint8),
+                np.ascontiguousarray(pet, np.float32))
+        self.ctx.call("gq_obs_flip_upload", gpd, ptr(keep[0], C.c_uint8), ptr(keep[1], C.c_uint8),
+                      ptr(keep[2], C.c_float))
+        prm = np.array([th1, th2, flip, sigma], np.float64)
+        out = np.zeros((self.n, 4))
+        self.ctx.call("gq_ensemble_loglik_flip", ptr(prm, C.c_double), ptr(out, C.c_double))
+        return out  # per sample: logL, L_T1, L_FLAIR, L_PET
 
 
 class FlipLikelihood:

@@ -78,6 +78,49 @@ gq_status gq_grid_blocks_info(gq_ctx *ctx, gq_block_info *out, int capacity, int
 gq_status gq_grid_write_field(gq_ctx *ctx, int field, const float *data, int n_blocks);
 gq_status gq_grid_read_field(gq_ctx *ctx, int field, float *data, int n_blocks);
 
+/* Point-sample a uniform gpd^3 volume (x fastest, gpd = 16*blocks_per_dim)
+ * onto the leaves: a leaf at level l takes every 2^(log2(blocks_per_dim)-l)-th
+ * point (the reference's _ic_* initialisers evaluate each field at the block
+ * points, Glioma/Glioma_HG_UQ.cpp:67-225).  sample = -1 writes every sample
+ * of an ensemble. */
+gq_status gq_grid_write_volume(gq_ctx *ctx, int field, const float *vol, int gpd, int sample);
+
+/* ---- ensembles (no reference counterpart: the reference runs one solve per
+ * process, one process per sampler proposal, Glioma/main.cpp).  nsamples
+ * independent grids of one configuration share the context's block pool;
+ * gq_grid_create is the one-sample case.  Single-grid calls (blocks_info,
+ * read/write, fwt, refine/compress, dump, likelihoods) address the selected
+ * sample (default 0); gq_fk_steps advances every sample with one parameter
+ * set. */
+gq_status gq_ensemble_create(gq_ctx *ctx, int nsamples, int blocks_per_dim, int max_level, int wavelet_order);
+gq_status gq_ensemble_size(gq_ctx *ctx, int *nsamples);
+gq_status gq_sample_select(gq_ctx *ctx, int sample);
+/* Leaf-block count of every sample (nsamples int32). */
+gq_status gq_ensemble_block_counts(gq_ctx *ctx, int32_t *out);
+
+enum { GQ_RUN_GRAPH = 1 }; /* replay each chunk of steps as one HIP graph */
+typedef struct {
+    int32_t adapt_every;  /* steps between adaptations (0: no adaptation) */
+    int32_t max_level;    /* refinement limit */
+    double tol_refine;    /* Science::AutomaticRefinement tolerance */
+    double tol_compress;  /* Science::AutomaticCompression tolerance */
+    int32_t flags;        /* GQ_RUN_* */
+} gq_run_opts;
+/* Glioma_HG_UQ::run (Glioma/Glioma_HG_UQ.cpp:511-588) for every sample at
+ * once: sample s takes n_steps[s] steps with p[s]; after every full chunk of
+ * adapt_every steps it runs AutomaticRefinement(tol_refine, max_level, 1
+ * loop) + AutomaticCompression(tol_compress), and at the end one more
+ * refinement (:574-575).  Each step is one launch of the step kernel over the
+ * blocks of all samples still running; the result of every sample equals
+ * that of a one-sample grid run alone on the same schedule. */
+gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_steps, const gq_run_opts *opts);
+/* Statistics of the last gq_ensemble_run: {cell updates, device ms of the
+ * step chunks, chunks, adaptation rounds, host seconds in adaptation, final
+ * work-list blocks, last slab depth, graph flag}. */
+gq_status gq_ensemble_stats(gq_ctx *ctx, double out[8]);
+/* gq_loglik_flip for every sample: out = nsamples x {logL, L_T1, L_FLAIR, L_PET}. */
+gq_status gq_ensemble_loglik_flip(gq_ctx *ctx, const double params[4], double *out);
+
 /* ---- forward solve: replaces Glioma_HG_UQ::_reactionDiffusionStep
  * (Glioma/Glioma_HG_UQ.cpp:266-277) = BlockProcessing::pipeline_process(vInfo,
  * collection, boundaryInfo, Glioma_ReactionDiffusionOperator) followed by

@@ -1,0 +1,147 @@
+"""GPU ensembles (gq_ensemble_*): many samples in one step launch.
+
+Bar: every sample of an ensemble run is bit-identical (field, block count,
+levels, flip log-likelihood) to the same sample run alone on a one-sample
+grid with the same schedule (Glioma_HG_UQ::run, Glioma/Glioma_HG_UQ.cpp:
+511-588); the one-sample path is itself held to the FP64 oracle by
+test_gpu_fk / test_gpu_adaptive.  A non-adaptive ensemble is also checked
+directly against the FP64 oracle (relative L2 <= 1e-5, max-abs <= 1e-5).
+"""
+import numpy as np
+import pytest
+
+import gliomauq_amd as gq
+from gliomauq_amd import anatomy as A
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L_CM = 17.0
+N = 128
+
+
+@pytest.fixture(scope="module")
+def inputs():
+    pwm, pgm, u0, c = A.config_inputs(N, L_CM, 3)
+    rng = np.random.default_rng(11)
+    wm = np.argwhere(pwm > 0.5)
+    ics = []
+    for k in range(4):
+        z, y, x = wm[rng.integers(len(wm))]
+        ics.append(A.initial_condition(N, (x, y, z), 0.1, 2.0).astype(np.float32))
+    t1, t2, pet = A.observations(np.maximum(ics[0], ics[1]) * 8.0, seed=2)
+    return pwm, pgm, ics, (t1, t2, pet)
+
+
+def sample_params(k):
+    Dw_cm = [2.4e-3, 9e-3, 6e-4, 1.3e-3][k]
+    rho = [0.05, 0.02, 0.15, 0.025][k]
+    Dw = Dw_cm / L_CM ** 2
+    h = 1.0 / N
+    return Dw, 0.1 * Dw, rho, 0.9 * h * h / (6 * Dw)
+
+
+STEPS = [23, 40, 7, 30]
+K = 10
+TOL_R, TOL_C = 1e-3, 1e-4
+
+
+def run_single(ctx, pwm, pgm, u0, prm, n_steps, adaptive=True):
+    g = gq.Grid(ctx, N // 16, 3, 4)
+    g.write_volume("phi", u0)
+    g.write_volume("pw", pwm)
+    g.write_volume("pg", pgm)
+    done = 0
+    while done < n_steps:
+        m = min(K if adaptive else n_steps, n_steps - done)
+        gq.fk_steps(g, *prm, m)
+        done += m
+        if adaptive and m == K:
+            gq.Science.AutomaticRefinement(g, TOL_R, 3, 1)
+            gq.Science.AutomaticCompression(g, TOL_C, -1)
+    if adaptive:
+        gq.Science.AutomaticRefinement(g, TOL_R, 3, 1)
+    return g
+
+
+def make_ensemble(ctx, pwm, pgm, ics):
+    ens = gq.Ensemble(ctx, len(ics), N // 16, 3, 4)
+    ens.write_volume("pw", pwm)
+    ens.write_volume("pg", pgm)
+    for s, u0 in enumerate(ics):
+        ens.write_volume("phi", u0, sample=s)
+    return ens
+
+
+def test_ensemble_equals_single_runs(inputs):
+    pwm, pgm, ics, obs = inputs
+    ctx = gq.Context(0)
+    singles = []
+    for s, u0 in enumerate(ics):
+        g = run_single(ctx, pwm, pgm, u0, sample_params(s), STEPS[s])
+        lv, ix, _ = g.levels_indices()
+        lik = gq.FlipLikelihood(g, *obs).loglik()
+        singles.append((g.dump_uniform(), g.n_blocks(), sorted(zip(lv.tolist(), map(tuple, ix.tolist()))), lik))
+    for graph in (True, False):
+        ens = make_ensemble(ctx, pwm, pgm, ics)
+        prm = np.array([sample_params(s) for s in range(len(ics))])
+        st = ens.run(prm, STEPS, adapt_every=K, tol_refine=TOL_R, tol_compress=TOL_C, graph=graph)
+        print("graph=%s stats %s" % (graph, st))
+        counts = ens.block_counts()
+        ll = ens.loglik_flip(*obs)
+        for s in range(len(ics)):
+            u_s, nb_s, lvix_s, lik_s = singles[s]
+            gs = ens.sample(s)
+            lv, ix, _ = gs.levels_indices()
+            assert counts[s] == nb_s
+            assert sorted(zip(lv.tolist(), map(tuple, ix.tolist()))) == lvix_s
+            u = gs.dump_uniform()
+            assert np.array_equal(u, u_s), "sample %d differs (max %g)" % (s, np.max(np.abs(u - u_s)))
+            assert np.array_equal(ll[s], lik_s)
+        assert st["cell_updates"] > 0 and st["adaptations"] > 0
+    # the samples did evolve differently
+    assert len({c for c in counts.tolist()}) > 1
+    ctx.close()
+
+
+def test_ensemble_uniform_vs_oracle(inputs):
+    pwm, pgm, ics, _ = inputs
+    ctx = gq.Context(0)
+    ens = make_ensemble(ctx, pwm, pgm, ics[:2])
+    prm = []
+    for s in range(2):
+        Dw, Dg, rho, dt = sample_params(s)
+        prm.append((Dw * 30, Dg * 30, rho * 10, dt / 30))  # faster fronts, same CFL number
+    steps = [12, 7]
+    ens.run(np.array(prm), steps, adapt_every=0)
+    for s in range(2):
+        u = ens.sample(s).dump_uniform()
+        ref = O.fk_uniform_steps(ics[s].astype(np.float64), pwm.astype(np.float64), pgm.astype(np.float64), 16,
+                                 1.0 / N, *prm[s], steps[s])
+        e2 = float(np.sqrt(np.mean((u - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+        einf = float(np.max(np.abs(u - ref)))
+        print("sample %d: rel L2 %.3e Linf %.3e" % (s, e2, einf))
+        assert e2 <= 1e-5 and einf <= 1e-5
+    ctx.close()
+
+
+def test_ensemble_rejects_bad_parameters(inputs):
+    pwm, pgm, ics, _ = inputs
+    ctx = gq.Context(0)
+    ens = make_ensemble(ctx, pwm, pgm, ics[:2])
+    prm = np.array([sample_params(0), sample_params(1)])
+    prm[1, 3] = float("nan")
+    with pytest.raises(gq._lib.GQError) as e:
+        ens.run(prm, [3, 3], adapt_every=0)
+    assert e.value.status == 7
+    with pytest.raises(gq._lib.GQError) as e:
+        ens.run(np.array([sample_params(0), sample_params(1)]), [3, -1], adapt_every=0)
+    assert e.value.status == 1
+    # the state is untouched by the rejected calls: a valid run still matches a single run
+    ens.run(np.array([sample_params(0), sample_params(1)]), [3, 2], adapt_every=0)
+    g = run_single(ctx, pwm, pgm, ics[1], sample_params(1), 2, adaptive=False)
+    u1 = g.dump_uniform()
+    ens2 = make_ensemble(ctx, pwm, pgm, ics[:2])
+    ens2.run(np.array([sample_params(0), sample_params(1)]), [3, 2], adapt_every=0)
+    assert np.array_equal(ens2.sample(1).dump_uniform(), u1)
+    ctx.close()
