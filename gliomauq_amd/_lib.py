@@ -26,6 +26,7 @@ EXPORTS = [
     "gq_topo_leaves", "gq_topo_refine", "gq_topo_compress", "gq_topo_point_terms",
     "gq_grid_write_volume", "gq_ensemble_create", "gq_ensemble_size", "gq_sample_select",
     "gq_ensemble_block_counts", "gq_ensemble_run", "gq_ensemble_stats", "gq_ensemble_loglik_flip",
+    "gq_ensemble_ic_gaussian", "gq_adjoint_flip",
 ]
 
 RUN_GRAPH = 1
@@ -109,6 +110,8 @@ def load():
         "gq_ensemble_run": [vp, C.POINTER(FKParams), i32p, C.POINTER(RunOpts)],
         "gq_ensemble_stats": [vp, f64p],
         "gq_ensemble_loglik_flip": [vp, f64p, f64p],
+        "gq_ensemble_ic_gaussian": [vp, f64p],
+        "gq_adjoint_flip": [vp, C.POINTER(FKParams), C.c_int, C.c_int, f64p, f64p, f64p, f64p],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -157,6 +157,10 @@ struct gq_ctx {
     // ensemble run statistics (gq_ensemble_run)
     double ens_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
+    // discrete adjoint: checkpoints, recomputed states, adjoint fields, accumulators
+    DevBuf<float> d_ckpt, d_stash, d_lam;
+    DevBuf<double> d_gacc, d_icp, d_icg;
+
     // captured step chunks
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -1130,7 +1134,7 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
             key[2 * s + 1] = prm[4 * s + 1];
         }
         ctx->d_sprm.upload(prm, ctx->st);
-        std::vector<int> rem(n_steps, n_steps + S);
+        std::vector<int> rem(n_steps, n_steps + S);  // steps left per sample
         const int K = adaptive ? o->adapt_every : std::max(1, std::min(1024, *std::max_element(rem.begin(), rem.end())));
         double ms = 0.0, t_adapt = 0.0, cells = 0.0;
         int chunks = 0, adapts = 0;
@@ -1195,6 +1199,205 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
         ctx->ens_stats[5] = (double)ctx->work.size();
         ctx->ens_stats[6] = ctx->last_zt;
         ctx->ens_stats[7] = use_graph ? 1 : 0;
+    });
+}
+
+// every sample a uniform grid (all leaves at the base level): the adjoint and the Gaussian IC map cells to voxels 1:1
+static void need_uniform(gq_ctx *ctx)
+{
+    ctx->ensure_tables();
+    for (int s = 0; s < ctx->nsamples(); s++) {
+        const gq::Grid &g = ctx->G(s);
+        for (const auto &lf : g.leaves())
+            if (lf.level != g.base_level()) throw gq_error(GQ_ERR_UNSUPPORTED, "needs uniform grids (no adaptation)");
+    }
+}
+
+gq_status gq_ensemble_ic_gaussian(gq_ctx *ctx, const double *ic)
+{
+    return guard(ctx, [&] {
+        need_uniform(ctx);
+        if (!ic) throw gq_error(GQ_ERR_ARG, "null argument");
+        const int S = ctx->nsamples();
+        for (int k = 0; k < 5 * S; k++)
+            if (!std::isfinite(ic[k])) throw gq_error(GQ_ERR_NUMERIC, "non-finite initial-condition parameter");
+        for (int s = 0; s < S; s++)
+            if (!(ic[5 * s + 4] > 0)) throw gq_error(GQ_ERR_NUMERIC, "sigma must be positive");
+        ctx->d_icp.upload(std::vector<double>(ic, ic + 5 * S), ctx->st);
+        const int gpd = ctx->G(0).blocks_per_dim() * gq::BS;
+        HIPCHK(gq::launch_ic_gaussian(ctx->phi[ctx->cur], ctx->d_work.p, (int)ctx->work.size(), ctx->d_index.p,
+                                      ctx->d_slot_sample.p, ctx->d_icp.p, gpd, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+    });
+}
+
+gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int ckpt_every, const double lik[4],
+                          const double *ic, double *out_ll, double *out_grad)
+{
+    return guard(ctx, [&] {
+        need_uniform(ctx);
+        const int S = ctx->nsamples();
+        if (!p || !lik || !out_ll || !out_grad || n_steps < 1 || ckpt_every < 1)
+            throw gq_error(GQ_ERR_ARG, "bad arguments");
+        for (int s = 0; s < S; s++) {
+            check_params(p[s]);
+            if (!(p[s].Dw > 0) || !(p[s].Dg > 0)) throw gq_error(GQ_ERR_NUMERIC, "the adjoint needs Dw, Dg > 0");
+        }
+        const int gpd = ctx->G(0).blocks_per_dim() * gq::BS;
+        if (ctx->obs_flip_gpd != gpd) throw gq_error(GQ_ERR_STATE, "observations missing or of another size");
+        const double sigma = lik[3], eps = lik[2];
+        if (!(eps > 0 && eps < 1) || !(sigma > 0)) throw gq_error(GQ_ERR_NUMERIC, "flip rate / sigma out of range");
+        if (ic) {
+            const gq_status r = gq_ensemble_ic_gaussian(ctx, ic);
+            if (r != GQ_OK) throw gq_error(r, ctx->err);
+        }
+        // all samples step together: identity order
+        std::vector<int> ident(S);
+        for (int s = 0; s < S; s++) ident[s] = s;
+        if (ctx->sample_order != ident) {
+            ctx->sample_order = ident;
+            ctx->tables_valid = false;
+        }
+        ctx->ensure_tables();
+        if (ctx->face_nrows != 0) throw gq_error(GQ_ERR_UNSUPPORTED, "needs uniform grids (no adaptation)");
+        std::vector<float> prm((size_t)S * 4), key((size_t)S * 2);
+        for (int s = 0; s < S; s++) {
+            prm[4 * s + 0] = (float)p[s].Dw;
+            prm[4 * s + 1] = (float)p[s].Dg;
+            prm[4 * s + 2] = (float)p[s].rho;
+            prm[4 * s + 3] = (float)p[s].dt;
+            key[2 * s] = prm[4 * s];
+            key[2 * s + 1] = prm[4 * s + 1];
+        }
+        ctx->d_sprm.upload(prm, ctx->st);
+        ctx->ensure_fik(key, true);
+        const int nw = (int)ctx->work.size();
+        const size_t pool = (size_t)ctx->cap * gq::BS3;
+        const int E = std::min(ckpt_every, n_steps);
+        const int nseg = (n_steps + E - 1) / E;
+        ctx->d_ckpt.reserve((size_t)nseg * pool);
+        ctx->d_stash.reserve((size_t)E * pool);
+        ctx->d_lam.reserve(2 * pool);
+        gq::StepArgs a = ctx->step_args(nullptr);
+        const int zt = ctx->choose_zt(a, 0);
+        ctx->need_events(4);
+        hipStream_t st = ctx->st;
+        HIPCHK(hipEventRecord(ctx->events[0], st));
+        // forward, keeping the state at the start of every segment
+        const size_t pbytes = pool * sizeof(float);
+        for (int seg = 0; seg < nseg; seg++) {
+            HIPCHK(hipMemcpyAsync(ctx->d_ckpt.p + seg * pool, ctx->phi[ctx->cur], pbytes, hipMemcpyDeviceToDevice, st));
+            const int m = std::min(E, n_steps - seg * E);
+            for (int j = 0; j < m; j++) {
+                a.phi_in = ctx->phi[ctx->cur];
+                a.phi_out = ctx->phi[ctx->cur ^ 1];
+                HIPCHK(gq::launch_fk_step(a, zt, st));
+                ctx->cur ^= 1;
+            }
+        }
+        HIPCHK(hipEventRecord(ctx->events[1], st));
+        // log-likelihood at u_N and its gradient field
+        {
+            gq::UniformArgs u = ctx->uniform_args(0);
+            const double th1 = lik[0], th2 = lik[1];
+            const double lp1 = std::log(1. - eps), lp0 = std::log(eps);
+            const double c0 = -0.5 * std::log(2. * M_PI * sigma * sigma);
+            const double is2 = 1. / (2. * sigma * sigma);
+            const int64_t n = (int64_t)gpd * gpd * gpd;
+            const int nb = gq::reduce_blocks(n);
+            ctx->d_partial.reserve((size_t)S * nb * 3 + (size_t)S * 3);
+            double *res = ctx->d_partial.p + (size_t)S * nb * 3;
+            HIPCHK(gq::launch_lik_flip(u, S, ctx->d_t1u8.p, ctx->d_t2u8.p, ctx->d_petf.p, th1, th2, lp1, lp0, c0, is2,
+                                       ctx->d_partial.p, nb, res, st));
+            std::vector<double> h((size_t)S * 3);
+            HIPCHK(hipMemcpyAsync(h.data(), res, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int s = 0; s < S; s++) {
+                out_ll[4 * s + 1] = h[3 * s];
+                out_ll[4 * s + 2] = h[3 * s + 1];
+                out_ll[4 * s + 3] = h[3 * s + 2];
+                out_ll[4 * s] = h[3 * s] + h[3 * s + 1] + h[3 * s + 2];
+            }
+        }
+        float *lam[2] = {ctx->d_lam.p, ctx->d_lam.p + pool};
+        int lc = 0;
+        HIPCHK(gq::launch_lik_flip_grad(ctx->phi[ctx->cur], ctx->d_petf.p, ctx->d_work.p, nw, ctx->d_index.p, gpd,
+                                        1.0 / (sigma * sigma), lam[lc], st));
+        const int NS = gq::adj_slabs();
+        ctx->d_gacc.reserve((size_t)ctx->cap * NS * 2);
+        HIPCHK(hipMemsetAsync(ctx->d_gacc.p, 0, (size_t)ctx->cap * NS * 2 * sizeof(double), st));
+        HIPCHK(hipEventRecord(ctx->events[2], st));
+        gq::AdjArgs g;
+        g.pw = ctx->pw;
+        g.pg = ctx->pg;
+        g.work = ctx->d_work.p;
+        g.nwork = nw;
+        g.fnbr = ctx->d_fnbr.p;
+        g.level = ctx->d_level.p;
+        g.ih2_lvl = ctx->d_ih2.p;
+        g.slot_sample = ctx->d_slot_sample.p;
+        g.sprm = ctx->d_sprm.p;
+        g.gacc = ctx->d_gacc.p;
+        const float *u_end = ctx->phi[ctx->cur];  // u_N
+        for (int seg = nseg - 1; seg >= 0; seg--) {
+            const int m = std::min(E, n_steps - seg * E);
+            // states u_{s0} .. u_{s0+m}: checkpoint, m-1 recomputed, and the next checkpoint (or u_N)
+            auto state = [&](int j) -> const float * {
+                if (j == 0) return ctx->d_ckpt.p + seg * pool;
+                if (j == m) return seg + 1 < nseg ? ctx->d_ckpt.p + (seg + 1) * pool : u_end;
+                return ctx->d_stash.p + (size_t)(j - 1) * pool;
+            };
+            for (int j = 0; j + 1 < m; j++) {
+                a.phi_in = state(j);
+                a.phi_out = ctx->d_stash.p + (size_t)j * pool;
+                HIPCHK(gq::launch_fk_step(a, zt, st));
+            }
+            for (int j = m - 1; j >= 0; j--) {
+                g.u = state(j);
+                g.u_next = state(j + 1);
+                g.lam_in = lam[lc];
+                g.lam_out = lam[lc ^ 1];
+                HIPCHK(gq::launch_adj_step(g, st));
+                lc ^= 1;
+            }
+        }
+        HIPCHK(hipEventRecord(ctx->events[3], st));
+        // gradients: per-slab accumulators and the initial-condition reduction, summed per sample in work order
+        std::vector<double> gacc((size_t)ctx->cap * NS * 2);
+        HIPCHK(hipMemcpyAsync(gacc.data(), ctx->d_gacc.p, gacc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        std::vector<double> icg;
+        if (ic) {
+            ctx->d_icg.reserve((size_t)nw * 4);
+            HIPCHK(gq::launch_ic_grad(lam[lc], ctx->d_work.p, nw, ctx->d_index.p, ctx->d_slot_sample.p, ctx->d_icp.p,
+                                      gpd, ctx->d_icg.p, st));
+            icg.resize((size_t)nw * 4);
+            HIPCHK(hipMemcpyAsync(icg.data(), ctx->d_icg.p, icg.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        for (int s = 0; s < S; s++) {
+            double gr[6] = {0, 0, 0, 0, 0, 0};
+            for (int k = ctx->work_begin[s]; k < ctx->work_begin[s + 1]; k++) {
+                const int slot = ctx->work[k];
+                for (int sl = 0; sl < NS; sl++) {
+                    gr[0] += gacc[((size_t)slot * NS + sl) * 2];
+                    gr[1] += gacc[((size_t)slot * NS + sl) * 2 + 1];
+                }
+                if (ic)
+                    for (int q = 0; q < 4; q++) gr[2 + q] += icg[(size_t)k * 4 + q];
+            }
+            for (int q = 0; q < 6; q++) out_grad[6 * s + q] = gr[q];
+        }
+        float f_ms = 0, b_ms = 0;
+        HIPCHK(hipEventElapsedTime(&f_ms, ctx->events[0], ctx->events[1]));
+        HIPCHK(hipEventElapsedTime(&b_ms, ctx->events[2], ctx->events[3]));
+        ctx->ens_stats[0] = (double)n_steps * nw * gq::BS3;
+        ctx->ens_stats[1] = f_ms;
+        ctx->ens_stats[2] = b_ms;
+        ctx->ens_stats[3] = nseg;
+        ctx->ens_stats[4] = 0;
+        ctx->ens_stats[5] = nw;
+        ctx->ens_stats[6] = zt;
+        ctx->ens_stats[7] = 0;
     });
 }
 

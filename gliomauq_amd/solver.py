@@ -420,16 +420,42 @@ class Ensemble:
         keys = ("cell_updates", "device_ms", "chunks", "adaptations", "adapt_s", "final_blocks", "zt", "graph")
         return dict(zip(keys, out.tolist()))
 
-    def loglik_flip(self, t1, t2, pet, th1=0.7, th2=0.25, flip=0.05, sigma=0.05):
+    def set_observations(self, t1, t2, pet):
+        """Flip-rate model observations on the gpd^3 grid (gq_obs_flip_upload)."""
         gpd = self.bpd * BS
         keep = (np.ascontiguousarray(t1, np.uint8), np.ascontiguousarray(t2, np.uint8),
                 np.ascontiguousarray(pet, np.float32))
         self.ctx.call("gq_obs_flip_upload", gpd, ptr(keep[0], C.c_uint8), ptr(keep[1], C.c_uint8),
                       ptr(keep[2], C.c_float))
+
+    def loglik_flip(self, t1=None, t2=None, pet=None, th1=0.7, th2=0.25, flip=0.05, sigma=0.05):
+        if t1 is not None:
+            self.set_observations(t1, t2, pet)
         prm = np.array([th1, th2, flip, sigma], np.float64)
         out = np.zeros((self.n, 4))
         self.ctx.call("gq_ensemble_loglik_flip", ptr(prm, C.c_double), ptr(out, C.c_double))
         return out  # per sample: logL, L_T1, L_FLAIR, L_PET
+
+    def ic_gaussian(self, ic):
+        """Gaussian initial condition per sample: ic (nsamples, 5) = x0, y0, z0,
+        amplitude, sigma in voxel units (uniform grids)."""
+        a = np.ascontiguousarray(ic, np.float64).reshape(self.n, 5)
+        self.ctx.call("gq_ensemble_ic_gaussian", ptr(a, C.c_double))
+
+    def adjoint_flip(self, params, n_steps, ckpt_every=50, ic=None, th1=0.7, th2=0.25, flip=0.05, sigma=0.05):
+        """Forward + discrete adjoint (gq_adjoint_flip).  Returns (loglik
+        (nsamples, 4), grad (nsamples, 6) = dL/d(log Dw, log rho, x0, y0, z0, A))."""
+        prm = np.ascontiguousarray(params, np.float64).reshape(self.n, 4)
+        arr = (FKParams * self.n)(*[FKParams(*map(float, r)) for r in prm])
+        lik = np.array([th1, th2, flip, sigma], np.float64)
+        ll = np.zeros((self.n, 4))
+        gr = np.zeros((self.n, 6))
+        icp = None
+        if ic is not None:
+            icp = np.ascontiguousarray(ic, np.float64).reshape(self.n, 5)
+        self.ctx.call("gq_adjoint_flip", arr, int(n_steps), int(ckpt_every), ptr(lik, C.c_double),
+                      None if icp is None else ptr(icp, C.c_double), ptr(ll, C.c_double), ptr(gr, C.c_double))
+        return ll, gr
 
 
 class FlipLikelihood:

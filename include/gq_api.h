@@ -116,10 +116,33 @@ typedef struct {
 gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_steps, const gq_run_opts *opts);
 /* Statistics of the last gq_ensemble_run: {cell updates, device ms of the
  * step chunks, chunks, adaptation rounds, host seconds in adaptation, final
- * work-list blocks, last slab depth, graph flag}. */
+ * work-list blocks, last slab depth, graph flag}; after gq_adjoint_flip:
+ * {forward cell updates, forward ms, backward ms, segments, 0, blocks, slab
+ * depth, 0}. */
 gq_status gq_ensemble_stats(gq_ctx *ctx, double out[8]);
 /* gq_loglik_flip for every sample: out = nsamples x {logL, L_T1, L_FLAIR, L_PET}. */
 gq_status gq_ensemble_loglik_flip(gq_ctx *ctx, const double params[4], double *out);
+
+/* ---- discrete adjoint (no reference counterpart: the reference samples
+ * without gradients).  Uniform grids only (every leaf at the base level).
+ *
+ * Gaussian initial condition per sample, ic[s*5] = {x0, y0, z0, A, sigma}
+ * (voxel units of the gpd^3 grid, periodic distance):
+ * u0 = A exp(-|x - x0|^2 / (2 sigma^2)). */
+gq_status gq_ensemble_ic_gaussian(gq_ctx *ctx, const double *ic);
+/* For every sample: n_steps forward steps with p[s] (Dw, Dg > 0) from the
+ * current state (or from the Gaussian initial condition ic, if not null),
+ * keeping a checkpoint every ckpt_every steps; the flip-rate log-likelihood
+ * at the final state (lik = {th_T1, th_FLAIR, flip_rate, pet_sigma}, the
+ * observations of gq_obs_flip_upload), out_ll[s*4] as gq_loglik_flip; and the
+ * exact derivative of the discrete FP32 forward map, with recomputation from
+ * the checkpoints: out_grad[s*6] = dL/d{log Dw (Dg = Dw/10 scaled with it),
+ * log rho, x0, y0, z0, A} (the IC terms are 0 when ic is null).  The
+ * Bernoulli terms are piecewise constant in u: their derivative is zero
+ * almost everywhere and the gradient is that of the PET term.  The final
+ * state is left in the grids. */
+gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int ckpt_every, const double lik[4],
+                          const double *ic, double *out_ll, double *out_grad);
 
 /* ---- forward solve: replaces Glioma_HG_UQ::_reactionDiffusionStep
  * (Glioma/Glioma_HG_UQ.cpp:266-277) = BlockProcessing::pipeline_process(vInfo,

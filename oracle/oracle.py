@@ -206,3 +206,26 @@ def adjoint(u0, pw, pg, h, Dw, Dg, rho, dt, n_steps, dLduN_fn):
     if rc != 0:
         raise MemoryError("adjoint oracle allocation failed")
     return uN, d0, g[0], g[1]
+
+
+def _pdisp(i, c, n):
+    d = i - c
+    return d - n * np.floor(d / n + 0.5)
+
+
+def ic_gaussian(n, x0, y0, z0, amp, sigma):
+    """Gaussian initial condition A exp(-|x - x0|^2 / (2 sigma^2)) on the n^3
+    voxel grid (periodic signed displacement), FP64 [z, y, x]."""
+    i = np.arange(n, dtype=np.float64)
+    dz, dy, dx = np.meshgrid(_pdisp(i, z0, n), _pdisp(i, y0, n), _pdisp(i, x0, n), indexing="ij")
+    return amp * np.exp(-(dx * dx + dy * dy + dz * dz) / (2.0 * sigma * sigma))
+
+
+def ic_grad(dLdu0, x0, y0, z0, amp, sigma):
+    """Chain rule of ic_gaussian: dL/d(x0, y0, z0, A) from dL/du0."""
+    n = dLdu0.shape[0]
+    i = np.arange(n, dtype=np.float64)
+    dz, dy, dx = np.meshgrid(_pdisp(i, z0, n), _pdisp(i, y0, n), _pdisp(i, x0, n), indexing="ij")
+    e = np.exp(-(dx * dx + dy * dy + dz * dz) / (2.0 * sigma * sigma))
+    lu = dLdu0 * amp * e / (sigma * sigma)
+    return np.array([np.sum(lu * dx), np.sum(lu * dy), np.sum(lu * dz), np.sum(dLdu0 * e)])
