@@ -50,6 +50,150 @@ def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, n_steps=200):
                       % (n_steps, n, wall)}
 
 
+def ensemble_workload(args):
+    """Batched ensemble: 256 parameter samples (seeded), each on its own
+    adaptive grid (levels 1..3, finest 128^3, Interp4, tol 1e-3/1e-4 every 10
+    steps), advanced together; FP64 flip-rate log-likelihood per sample."""
+    import gliomauq_amd as gq
+    from gliomauq_amd import anatomy as A
+
+    n = 128
+    bpd = n // 16
+    S = args.samples
+    rng = np.random.default_rng(args.seed + 77)
+    log("[bench] ensemble: anatomy %d^3, %d samples" % (n, S))
+    pwm, pgm, u0_gt, cvox = A.config_inputs(n, L_CM, args.seed)
+    wm = np.argwhere(pwm > 0.5)
+    Dw_cm = np.exp(rng.uniform(np.log(2e-4), np.log(1.5e-2), S))
+    rho = np.exp(rng.uniform(np.log(2e-3), np.log(0.2), S))
+    T = rng.uniform(30.0, 300.0, S)
+    centers = wm[rng.integers(len(wm), size=S)]  # (z, y, x)
+    h = 1.0 / n
+    prm = np.zeros((S, 4))
+    steps = np.zeros(S, np.int32)
+    for s in range(S):
+        Dw = Dw_cm[s] / L_CM ** 2
+        dt = 0.9 * h * h / (6 * Dw)
+        prm[s] = (Dw, 0.1 * Dw, rho[s], dt)
+        steps[s] = gq.n_steps_until(T[s], dt)
+    ic = np.array([[c[2], c[1], c[0], 0.1, 2.0] for c in centers], np.float64)
+    ctx = gq.Context(0)
+    # observations of a ground-truth solve at perturbed parameters
+    gt = gq.Ensemble(ctx, 1, bpd, 3, 4)
+    gt.write_volume("pw", pwm)
+    gt.write_volume("pg", pgm)
+    gt.write_volume("phi", u0_gt)
+    Dw0 = 0.0013 * 1.25 / L_CM ** 2
+    dt0 = 0.9 * h * h / (6 * Dw0)
+    gt.run(np.array([[Dw0, 0.1 * Dw0, 0.025 * 0.8, dt0]]), [gq.n_steps_until(300.0, dt0)], adapt_every=10)
+    u_true = gt.sample(0).dump_uniform()
+    obs = A.observations(u_true, seed=args.seed)
+    del gt
+
+    def make():
+        e = gq.Ensemble(ctx, S, bpd, 3, 4)
+        e.write_volume("pw", pwm)
+        e.write_volume("pg", pgm)
+        e.ic_gaussian(ic)
+        e.set_observations(*obs)
+        return e
+
+    log("[bench] ensemble warmup")
+    w = make()
+    w.run(prm, np.minimum(steps, 10), adapt_every=10)
+    del w
+    ens = make()
+    log("[bench] ensemble timed: %d sample-steps" % int(steps.sum()))
+    ctx.call("gq_set_step_timing", 1 if args.kernel_timing else 0)
+    t0 = time.perf_counter()
+    st = ens.run(prm, steps, adapt_every=10, tol_refine=1e-3, tol_compress=1e-4, graph=not args.kernel_timing)
+    ll = ens.loglik_flip()
+    wall = time.perf_counter() - t0
+    counts = ens.block_counts()
+    cells = st["cell_updates"]
+    res = {
+        "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
+        "value": cells / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": int(steps.max()),
+        "warmup": 10, "ms_per_step": wall * 1e3 / max(int(steps.max()), 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched ensemble: %d samples, each on its own adaptive grid (levels 1-3, finest "
+                               "128^3, Interp4 tol 1e-3/1e-4 every 10 steps), Dw logU[2e-4,1.5e-2], rho "
+                               "logU[2e-3,0.2], T U[30,300] d, FP32, FP64 flip log-likelihoods" % S,
+                   "samples": S, "seed": args.seed, "sample_steps": int(steps.sum()),
+                   "steps_min_max": [int(steps.min()), int(steps.max())],
+                   "final_blocks_min_max": [int(counts.min()), int(counts.max())],
+                   "loglik_mean": float(np.mean(ll[:, 0]))},
+        "breakdown": {"device_chunk_ms": st["device_ms"], "adapt_s": st["adapt_s"], "chunks": st["chunks"],
+                      "adaptations": st["adaptations"], "graph": bool(st["graph"]), "slab_depth": st["zt"]},
+    }
+    if args.kernel_timing:
+        achieved = 16.0 * cells / (st["device_ms"] * 1e-3) / 1e9
+        res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_fk_step",
+                           "algorithmic_bytes_per_cell": 16}
+    print(json.dumps(res), flush=True)
+
+
+def adjoint_workload(args):
+    """Forward + discrete adjoint of 16 chains on the uniform 128^3 grid,
+    2000 steps, checkpoints every 50 steps: one gradient of the flip-rate
+    log-likelihood w.r.t. (log Dw, log rho, x0, y0, z0, A) per chain."""
+    import gliomauq_amd as gq
+    from gliomauq_amd import anatomy as A
+
+    n = 128
+    S = args.chains
+    steps = args.adj_steps
+    rng = np.random.default_rng(args.seed + 99)
+    pwm, pgm, u0, c = A.config_inputs(n, L_CM, args.seed)
+    h = 1.0 / n
+    dt = 0.9 * h * h / (6 * 1.5e-2 / L_CM ** 2)  # stable for every Dw of the prior
+    ctx = gq.Context(0)
+    ens = gq.Ensemble(ctx, S, n // 16, 3, 4)
+    ens.write_volume("pw", pwm)
+    ens.write_volume("pg", pgm)
+    truth = gq.Ensemble(ctx, 1, n // 16, 3, 4)
+    truth.write_volume("pw", pwm)
+    truth.write_volume("pg", pgm)
+    ic_true = np.array([[c[2], c[1], c[0], 0.1, 2.0]])
+    truth.ic_gaussian(ic_true)
+    Dt = 0.0013 * 1.25 / L_CM ** 2
+    truth.run(np.array([[Dt, 0.1 * Dt, 0.025 * 0.8, dt]]), [steps], adapt_every=0)
+    obs = A.observations(truth.sample(0).dump_uniform(), seed=args.seed)
+    del truth
+    ens = gq.Ensemble(ctx, S, n // 16, 3, 4)
+    ens.write_volume("pw", pwm)
+    ens.write_volume("pg", pgm)
+    ens.set_observations(*obs)
+    Dw = np.exp(rng.uniform(np.log(2e-4), np.log(1.5e-2), S)) / L_CM ** 2
+    rho = np.exp(rng.uniform(np.log(2e-3), np.log(0.2), S))
+    prm = np.stack([Dw, 0.1 * Dw, rho, np.full(S, dt)], 1)
+    ic = np.array([[c[2] + rng.uniform(-2, 2), c[1] + rng.uniform(-2, 2), c[0] + rng.uniform(-2, 2), 0.1, 2.0]
+                   for _ in range(S)])
+    log("[bench] adjoint warmup")
+    ens.adjoint_flip(prm, 100, 50, ic=ic)
+    log("[bench] adjoint timed: %d chains x %d steps" % (S, steps))
+    t0 = time.perf_counter()
+    reps = max(1, args.steps_adj_reps)
+    for _ in range(reps):
+        ll, gr = ens.adjoint_flip(prm, steps, 50, ic=ic)
+    wall = (time.perf_counter() - t0) / reps
+    st = ens.stats()
+    cells = st["cell_updates"]
+    res = {
+        "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
+        "value": S / wall, "unit": "gradients/s (forward + adjoint per chain)", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "discrete adjoint: %d chains, uniform 128^3, %d steps, checkpoints every 50, "
+                               "FP32 fields, FP64 gradients" % (S, steps), "chains": S, "seed": args.seed},
+        "breakdown": {"wall_s": wall, "forward_ms": st["device_ms"], "backward_ms": st["chunks"],
+                      "forward_cell_updates_per_s": cells / (st["device_ms"] * 1e-3),
+                      "grad_norm_mean": float(np.mean(np.abs(gr)))},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,11 +203,21 @@ def main():
     ap.add_argument("--adapt-every", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workload", default="adaptive", choices=["adaptive", "ensemble", "adjoint"])
+    ap.add_argument("--chains", type=int, default=16, help="adjoint chains")
+    ap.add_argument("--adj-steps", type=int, default=2000)
+    ap.add_argument("--steps-adj-reps", type=int, default=2)
+    ap.add_argument("--samples", type=int, default=256, help="ensemble samples")
+    ap.add_argument("--kernel-timing", action="store_true", help="event-time every step kernel (no graphs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (samples would shard across GPUs; "
               "run --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        return ensemble_workload(args)
+    if args.workload == "adjoint":
+        return adjoint_workload(args)
 
     import gliomauq_amd as gq
     from gliomauq_amd import anatomy as A

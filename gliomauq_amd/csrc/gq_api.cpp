@@ -414,7 +414,8 @@ struct gq_ctx {
         const gq::LabArgs la = lab_args();
         const int order = samples[0].grid->order();
         d_ghost.reserve(slots.size() * (size_t)gq::lab_points(order));
-        HIPCHK(gq::launch_lab_ghosts(la, phi[cur], d_list.p, (int)slots.size(), d_ghost.p, false, st));
+        const float *fl[1] = {phi[cur]};
+        HIPCHK(gq::launch_lab_ghosts(la, fl, 1, d_list.p, (int)slots.size(), d_ghost.p, 0, false, st));
         HIPCHK(gq::launch_fwt_detail(order, la, d_ghost.p, phi[cur], d_list.p, (int)slots.size(), d_dbl.p, st));
         HIPCHK(hipMemcpyAsync(out.data(), d_dbl.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -468,8 +469,7 @@ struct gq_ctx {
         const size_t per = pslots.size() * (size_t)gq::lab_points(order);
         d_ghost.reserve(3 * per);
         const float *fl[3] = {phi[cur], pw, pg};
-        for (int f = 0; f < 3; f++)
-            HIPCHK(gq::launch_lab_ghosts(la, fl[f], d_list.p, (int)pslots.size(), d_ghost.p + f * per, true, st));
+        HIPCHK(gq::launch_lab_ghosts(la, fl, 3, d_list.p, (int)pslots.size(), d_ghost.p, per, true, st));
         HIPCHK(gq::launch_split(order, la, d_ghost.p, phi[cur], pw, pg, d_list.p, d_list2.p, (int)pslots.size(),
                                 st));
         HIPCHK(hipStreamSynchronize(st));

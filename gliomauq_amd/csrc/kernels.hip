@@ -292,9 +292,9 @@ struct DfsFrame {
 // Depth-first walk from (lam0, p0, w0).  mode 0: acc += w*v over the leaf
 // values of fld; 1: count terms (added to *count); 2: emit terms at src/wout
 // (advancing *count).
-template <int MODE>
-__device__ void ghost_dfs(const LabArgs &g, const int *__restrict__ lut, const float *__restrict__ fld, int lam0,
-                          int64_t p0x, int64_t p0y, int64_t p0z, double w0, double &acc, int *count, int *src,
+template <int MODE, int NF = 1>
+__device__ void ghost_dfs(const LabArgs &g, const int *__restrict__ lut, const float *const *flds, int lam0,
+                          int64_t p0x, int64_t p0y, int64_t p0z, double w0, double *acc, int *count, int *src,
                           float *wout)
 {
     const int lo = g.order == 4 ? -3 : -1, hi = g.order == 4 ? 4 : 2;
@@ -314,7 +314,11 @@ __device__ void ghost_dfs(const LabArgs &g, const int *__restrict__ lut, const f
                 const int cy = (int)((p[1] & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
                 const int cz = (int)((p[2] & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
                 const int cell = cx + 16 * (cy + 16 * cz);
-                if (MODE == 0) acc = __dadd_rn(acc, __dmul_rn(w, (double)fld[(size_t)slot * 4096 + cell]));
+                if (MODE == 0) {
+#pragma unroll
+                    for (int f = 0; f < NF; f++)
+                        acc[f] = __dadd_rn(acc[f], __dmul_rn(w, (double)flds[f][(size_t)slot * 4096 + cell]));
+                }
                 else if (MODE == 2) {
                     src[*count] = slot * 4096 + cell;
                     wout[*count] = (float)w;
@@ -377,8 +381,9 @@ __device__ void ghost_dfs(const LabArgs &g, const int *__restrict__ lut, const f
 // the common cases (the point lies in a finer leaf, or in a coarser leaf
 // whose synthesis children all lie in leaves of their own level); anything
 // else continues with the general walk at the child, in the same order.
-__device__ double ghost_value(const LabArgs &g, const int *__restrict__ lut, const float *__restrict__ fld, int lam,
-                              int64_t px, int64_t py, int64_t pz)
+template <int NF>
+__device__ void ghost_value(const LabArgs &g, const int *__restrict__ lut, const float *const *flds, int lam,
+                            int64_t px, int64_t py, int64_t pz, double *acc)
 {
     for (;;) {
         const int slot = lying_slot(g, lut, lam, px, py, pz);
@@ -388,7 +393,10 @@ __device__ double ghost_value(const LabArgs &g, const int *__restrict__ lut, con
             const int cx = (int)((px & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
             const int cy = (int)((py & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
             const int cz = (int)((pz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
-            return __dmul_rn(1.0, (double)fld[(size_t)slot * 4096 + cx + 16 * (cy + 16 * cz)]);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+                acc[f] = __dmul_rn(1.0, (double)flds[f][(size_t)slot * 4096 + cx + 16 * (cy + 16 * cz)]);
+            return;
         }
         if (l > lam) {
             lam++;
@@ -408,7 +416,8 @@ __device__ double ghost_value(const LabArgs &g, const int *__restrict__ lut, con
     }
     const int cl = lam - 1;
     const int64_t msk = ((int64_t)16 << cl) - 1;
-    double acc = 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; f++) acc[f] = 0.0;
     for (int64_t iz = s[2]; iz < e[2]; iz++) {
         const double wz = hs_w(g.order, (int)(2 * iz - p[2]));
         for (int64_t iy = s[1]; iy < e[1]; iy++) {
@@ -422,14 +431,15 @@ __device__ double ghost_value(const LabArgs &g, const int *__restrict__ lut, con
                     const int cx = (int)((ix & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
                     const int cy = (int)((iy & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
                     const int cz = (int)((iz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
-                    acc = __dadd_rn(acc, __dmul_rn(cw, (double)fld[(size_t)slot * 4096 + cx + 16 * (cy + 16 * cz)]));
+                    const size_t i = (size_t)slot * 4096 + cx + 16 * (cy + 16 * cz);
+#pragma unroll
+                    for (int f = 0; f < NF; f++) acc[f] = __dadd_rn(acc[f], __dmul_rn(cw, (double)flds[f][i]));
                 } else {
-                    ghost_dfs<0>(g, lut, fld, cl, ix, iy, iz, cw, acc, nullptr, nullptr, nullptr);
+                    ghost_dfs<0, NF>(g, lut, flds, cl, ix, iy, iz, cw, acc, nullptr, nullptr, nullptr);
                 }
             }
         }
     }
-    return acc;
 }
 
 template <int S0, int S1>
@@ -489,44 +499,48 @@ __device__ __forceinline__ bool lab_coord_used(int v)
     return ORDER == 4 ? (v == -2 || v == B || v == B + 2) : (v == B);
 }
 
-template <int S0, int S1, int ORDER, bool FOR_SPLIT>
-__device__ void lab_ghosts_block(const LabArgs &a, const float *__restrict__ fld, int slot, double *__restrict__ out)
-{
-    constexpr int L = LabGeom<S0, S1>::L;
-    const int lam = a.level[slot];
-    const int *lut = a.lut + a.lutoff[slot];
-    const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
-                  oz = (int64_t)a.index[slot * 3 + 2] * B;
-    for (int cc = 0; cc < 26; cc++) {
-        if (a.nbr26[slot * 26 + cc] >= 0) continue;  // same-level region: copied by fill_lab
-        const int c = cc < 13 ? cc : cc + 1;
-        const int d[3] = {c % 3 - 1, (c / 3) % 3 - 1, c / 9 - 1};
-        int rs[3], rn[3];
-        for (int k = 0; k < 3; k++) {
-            rs[k] = d[k] < 0 ? S0 : (d[k] == 0 ? 0 : B);
-            rn[k] = d[k] < 0 ? -S0 : (d[k] == 0 ? B : S1 - 1);
-        }
-        const int np = rn[0] * rn[1] * rn[2];
-        for (int q = threadIdx.x; q < np; q += blockDim.x) {
-            const int x = rs[0] + q % rn[0], y = rs[1] + (q / rn[0]) % rn[1], z = rs[2] + q / (rn[0] * rn[1]);
-            if (!lab_coord_used<ORDER, FOR_SPLIT>(x) || !lab_coord_used<ORDER, FOR_SPLIT>(y) ||
-                !lab_coord_used<ORDER, FOR_SPLIT>(z))
-                continue;
-            out[(x - S0) + L * ((y - S0) + L * (z - S0))] = ghost_value(a, lut, fld, lam, ox + x, oy + y, oz + z);
-        }
+// Lab coordinate of index i of the coordinates a consumer reads (interior
+// and the ghost coordinates of lab_coord_used, increasing).
+template <int ORDER, bool FOR_SPLIT>
+struct UsedCoords {
+    static constexpr int n = ORDER == 4 ? 19 : 17;
+    __device__ static int at(int i)
+    {
+        if (ORDER == 2) return i;
+        if (FOR_SPLIT) return i - 1;
+        return i == 0 ? -2 : (i <= 16 ? i - 1 : (i == 17 ? B : B + 2));
     }
-}
+};
 }  // namespace
 
-// One workgroup per listed slot: the cross-level points of its wide lab.
-template <int ORDER, bool FOR_SPLIT>
-__global__ __launch_bounds__(256) void k_lab_ghosts(LabArgs a, const float *__restrict__ fld,
-                                                    const int *__restrict__ slots, double *__restrict__ ghost)
+// One thread per lab point a consumer reads outside the block, blockIdx.y =
+// listed slot: the cross-level ones are evaluated (NF fields, one walk),
+// the others are copied by fill_lab from the same-level neighbour.
+template <int ORDER, bool FOR_SPLIT, int NF>
+__global__ __launch_bounds__(256) void k_lab_ghosts(LabArgs a, FieldPtrs flds, const int *__restrict__ slots,
+                                                    double *__restrict__ ghost, size_t field_stride)
 {
     constexpr int S0 = ORDER == 4 ? -2 : -1;
     constexpr int S1 = ORDER == 4 ? 4 : 2;
-    lab_ghosts_block<S0, S1, ORDER, FOR_SPLIT>(a, fld, slots[blockIdx.x],
-                                               ghost + (size_t)blockIdx.x * LabGeom<S0, S1>::N);
+    constexpr int L = LabGeom<S0, S1>::L;
+    constexpr int NU = UsedCoords<ORDER, FOR_SPLIT>::n;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= NU * NU * NU) return;
+    const int x = UsedCoords<ORDER, FOR_SPLIT>::at(q % NU), y = UsedCoords<ORDER, FOR_SPLIT>::at((q / NU) % NU),
+              z = UsedCoords<ORDER, FOR_SPLIT>::at(q / (NU * NU));
+    const int dx = x < 0 ? -1 : (x >= B ? 1 : 0), dy = y < 0 ? -1 : (y >= B ? 1 : 0),
+              dz = z < 0 ? -1 : (z >= B ? 1 : 0);
+    if (dx == 0 && dy == 0 && dz == 0) return;  // interior
+    const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+    const int slot = slots[blockIdx.y];
+    if (a.nbr26[slot * 26 + (c < 13 ? c : c - 1)] >= 0) return;  // same-level region
+    const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
+                  oz = (int64_t)a.index[slot * 3 + 2] * B;
+    double v[NF];
+    ghost_value<NF>(a, a.lut + a.lutoff[slot], flds.p, a.level[slot], ox + x, oy + y, oz + z, v);
+    const size_t o = (size_t)blockIdx.y * LabGeom<S0, S1>::N + (x - S0) + L * ((y - S0) + L * (z - S0));
+#pragma unroll
+    for (int f = 0; f < NF; f++) ghost[f * field_stride + o] = v[f];
 }
 
 // Step rows of cross-level faces (count / fill passes).
@@ -554,7 +568,7 @@ __global__ __launch_bounds__(256) void k_face_rows_count(LabArgs a, const int *_
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
     int n = 0;
     double unused = 0.0;
-    ghost_dfs<1>(a, a.lut + a.lutoff[slot], nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, unused, &n, nullptr,
+    ghost_dfs<1>(a, a.lut + a.lutoff[slot], nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, &unused, &n, nullptr,
                  nullptr);
     counts[(int64_t)k * 256 + threadIdx.x] = n;
 }
@@ -573,7 +587,7 @@ __global__ __launch_bounds__(256) void k_face_rows_fill(LabArgs a, const int *__
     const int64_t r = (int64_t)k * 256 + threadIdx.x;
     int n = 0;
     double unused = 0.0;
-    ghost_dfs<2>(a, a.lut + a.lutoff[slot], nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, unused, &n,
+    ghost_dfs<2>(a, a.lut + a.lutoff[slot], nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, &unused, &n,
                  src + ptr[r], w + ptr[r]);
 }
 
@@ -939,17 +953,34 @@ hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, i
 
 int lab_points(int order) { return order == 4 ? 21 * 21 * 21 : 18 * 18 * 18; }
 
-hipError_t launch_lab_ghosts(const LabArgs &a, const float *fld, const int *slots, int nslots, double *ghost,
-                             bool for_split, hipStream_t st)
+hipError_t launch_lab_ghosts(const LabArgs &a, const float *const *flds, int nf, const int *slots, int nslots,
+                             double *ghost, size_t field_stride, bool for_split, hipStream_t st)
 {
     if (nslots <= 0) return hipSuccess;
+    if (nf != 1 && nf != 3) return hipErrorInvalidValue;
+    FieldPtrs fp;
+    for (int f = 0; f < 3; f++) fp.p[f] = flds[f < nf ? f : 0];
+    const int nu = a.order == 4 ? 19 : 17;
+    const dim3 grid((nu * nu * nu + 255) / 256, nslots);
+#define GQ_LAB(O, S, N) hipLaunchKernelGGL((k_lab_ghosts<O, S, N>), grid, dim3(256), 0, st, a, fp, slots, ghost, field_stride)
     if (a.order == 4) {
-        if (for_split) hipLaunchKernelGGL((k_lab_ghosts<4, true>), dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
-        else hipLaunchKernelGGL((k_lab_ghosts<4, false>), dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
+        if (for_split) {
+            if (nf == 3) GQ_LAB(4, true, 3);
+            else GQ_LAB(4, true, 1);
+        } else {
+            if (nf == 3) GQ_LAB(4, false, 3);
+            else GQ_LAB(4, false, 1);
+        }
     } else {
-        if (for_split) hipLaunchKernelGGL((k_lab_ghosts<2, true>), dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
-        else hipLaunchKernelGGL((k_lab_ghosts<2, false>), dim3(nslots), dim3(256), 0, st, a, fld, slots, ghost);
+        if (for_split) {
+            if (nf == 3) GQ_LAB(2, true, 3);
+            else GQ_LAB(2, true, 1);
+        } else {
+            if (nf == 3) GQ_LAB(2, false, 3);
+            else GQ_LAB(2, false, 1);
+        }
     }
+#undef GQ_LAB
     return hipGetLastError();
 }
 

@@ -88,8 +88,12 @@ hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, i
 // Ghost values of the cross-level points of the wide lab of each listed slot
 // (FP64, depth-first order) into ghost[k * lab_points + q].
 // for_split: the points the split reads; else the points the FWT reads.
-hipError_t launch_lab_ghosts(const LabArgs &a, const float *fld, const int *slots, int nslots, double *ghost,
-                             bool for_split, hipStream_t st);
+// nf = 1 or 3 fields evaluated in one walk; field f goes to ghost + f * field_stride.
+struct FieldPtrs {
+    const float *p[3];
+};
+hipError_t launch_lab_ghosts(const LabArgs &a, const float *const *flds, int nf, const int *slots, int nslots,
+                             double *ghost, size_t field_stride, bool for_split, hipStream_t st);
 int lab_points(int order);  // points of the wide lab (Interp4: 21^3, Interp2: 18^3)
 hipError_t launch_fwt_detail(int order, const LabArgs &a, const double *ghost, const float *phi, const int *cand,
                              int ncand, double *out, hipStream_t st);
