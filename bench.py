@@ -105,6 +105,7 @@ def ensemble_workload(args):
     ens = make()
     log("[bench] ensemble timed: %d sample-steps" % int(steps.sum()))
     ctx.call("gq_set_step_timing", 1 if args.kernel_timing else 0)
+    hs0 = gq.host_stats(ctx)
     t0 = time.perf_counter()
     st = ens.run(prm, steps, adapt_every=10, tol_refine=1e-3, tol_compress=1e-4, graph=not args.kernel_timing)
     ll = ens.loglik_flip()
@@ -124,7 +125,8 @@ def ensemble_workload(args):
                    "final_blocks_min_max": [int(counts.min()), int(counts.max())],
                    "loglik_mean": float(np.mean(ll[:, 0]))},
         "breakdown": {"device_chunk_ms": st["device_ms"], "adapt_s": st["adapt_s"], "chunks": st["chunks"],
-                      "adaptations": st["adaptations"], "graph": bool(st["graph"]), "slab_depth": st["zt"]},
+                      "adaptations": st["adaptations"], "graph": bool(st["graph"]), "slab_depth": st["zt"],
+                      "host": {k: v - hs0[k] for k, v in gq.host_stats(ctx).items()}},
     }
     if args.kernel_timing:
         achieved = 16.0 * cells / (st["device_ms"] * 1e-3) / 1e9

@@ -49,6 +49,7 @@ struct DevBuf {
         HIPCHK(hipMalloc((void **)&p, cap * sizeof(T)));
         n = cap;
     }
+    // Stream-ordered copy: v must stay alive until the stream is synchronised.
     void upload(const std::vector<T> &v, hipStream_t st)
     {
         reserve(v.size());
@@ -115,14 +116,15 @@ struct gq_ctx {
     std::vector<int> work_begin;       // [S+1] per position in sample_order
     std::vector<int> face_begin;       // [S+1] cross-level faces per position
     DevBuf<int> d_work, d_level, d_index, d_fnbr, d_fbase, d_face_src, d_nbr26, d_lut, d_lutf, d_lutoff,
-        d_slot_sample, d_faces, d_counts;
+        d_slot_sample, d_faces, d_counts, d_sfbase, d_packed;
     DevBuf<int64_t> d_face_ptr;
     DevBuf<char> d_scan_tmp;
     int face_nrows = 0;
     DevBuf<double> d_ghost;  // cross-level lab points (3 fields for a split)
     DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2, d_sprm;
-    bool fik_valid = false;
-    std::vector<float> fik_key;  // Dw, Dg (per sample when per-sample parameters are set)
+    bool fik_valid = false;       // false: every slot's face 1/K must be recomputed
+    std::vector<char> fik_stale;  // per sample: recompute that sample's slots
+    std::vector<float> fik_key;   // Dw, Dg (per sample when per-sample parameters are set)
 
     // scratch
     DevBuf<int> d_list, d_list2;
@@ -279,6 +281,12 @@ struct gq_ctx {
             if (e) std::rethrow_exception(e);
     }
 
+    // Slot-indexed tables on the host (persistent, sized by the pool) and the
+    // pool size they were last uploaded for.
+    std::vector<int> h_level, h_index, h_ssample, h_lutoff, h_fbase;
+    int uploaded_cap = 0;
+    static constexpr int ROW = 45;  // packed row: slot, level, index[3], sample, lutoff, fnbr[6], fbase[6], nbr26[26]
+
     void ensure_tables()
     {
         need_grid();
@@ -290,53 +298,94 @@ struct gq_ctx {
             sample_order.resize(S);
             for (int s = 0; s < S; s++) sample_order[s] = s;
         }
+        const bool full = uploaded_cap != cap;
         h_fnbr.resize((size_t)cap * 6, -1);
         h_nbr26.resize((size_t)cap * 26, -1);
+        h_level.resize((size_t)cap, 0);
+        h_index.resize((size_t)cap * 3, 0);
+        h_ssample.resize((size_t)cap, 0);
+        h_lutoff.resize((size_t)cap, 0);
+        h_fbase.resize((size_t)cap * 6, -1);
         std::vector<int> dirty;
         for (int s = 0; s < S; s++)
             if (samples[s].dirty) dirty.push_back(s);
+        fik_stale.resize(S, 1);
+        for (int s : dirty) fik_stale[s] = 1;
+        const int lut_n = (int)samples[0].grid->lut().size();
+        // per-sample tables and slot rows of the samples whose topology changed
         parallel_for((int)dirty.size(), [&](int k) {
-            Sample &sm = samples[dirty[k]];
+            const int s = dirty[k];
+            Sample &sm = samples[s];
             gq::build_sample_tables(*sm.grid, h_fnbr.data(), h_nbr26.data(), sm.t);
+            const auto &L = sm.grid->leaves();
+            for (const auto &lf : L) {
+                h_level[lf.slot] = lf.level;
+                for (int a = 0; a < 3; a++) h_index[(size_t)lf.slot * 3 + a] = lf.idx[a];
+                h_ssample[lf.slot] = s;
+                h_lutoff[lf.slot] = s * lut_n;
+                for (int f = 0; f < 6; f++) h_fbase[(size_t)lf.slot * 6 + f] = -1;
+            }
+            // face rows relative to the sample's first row
+            for (size_t j = 0; j < sm.t.faces.size(); j++) h_fbase[sm.t.faces[j]] = (int)j * 256;
             sm.dirty = false;
         });
-        // merged arrays
-        std::vector<int> lvl((size_t)cap, 0), idx((size_t)cap * 3, 0), ssample((size_t)cap, 0),
-            lutoff((size_t)cap, 0), lutf, lut_slot, faces;
+        // merged lists in work order
+        std::vector<int> lutf((size_t)S * lut_n), lut_slot, faces, sfbase(S, 0);
         work.clear();
         work_begin.assign(1, 0);
         face_begin.assign(1, 0);
         for (int pos = 0; pos < S; pos++) {
-            const Sample &sm = samples[sample_order[pos]];
+            const int s = sample_order[pos];
+            const Sample &sm = samples[s];
             const auto &L = sm.grid->leaves();
             for (int li : sm.t.order) work.push_back(L[li].slot);
+            sfbase[s] = (int)faces.size() * 256;
             faces.insert(faces.end(), sm.t.faces.begin(), sm.t.faces.end());
             work_begin.push_back((int)work.size());
             face_begin.push_back((int)faces.size());
         }
         for (int s = 0; s < S; s++) {
             const Sample &sm = samples[s];
-            const int off = (int)lutf.size();
-            lutf.insert(lutf.end(), sm.t.lut.begin(), sm.t.lut.end());
+            std::copy(sm.t.lut.begin(), sm.t.lut.end(), lutf.begin() + (size_t)s * lut_n);
             lut_slot.insert(lut_slot.end(), sm.t.lut_slot.begin(), sm.t.lut_slot.end());
-            for (const auto &lf : sm.grid->leaves()) {
-                lvl[lf.slot] = lf.level;
-                for (int a = 0; a < 3; a++) idx[(size_t)lf.slot * 3 + a] = lf.idx[a];
-                ssample[lf.slot] = s;
-                lutoff[lf.slot] = off;
-            }
         }
         if ((int64_t)faces.size() * 256 > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "too many cross-level faces");
-        std::vector<int> fbase((size_t)cap * 6, -1);
-        for (size_t k = 0; k < faces.size(); k++) fbase[faces[k]] = (int)k * 256;
+        std::vector<int> packed;
+        if (full) {
+            d_level.upload(h_level, st);
+            d_index.upload(h_index, st);
+            d_slot_sample.upload(h_ssample, st);
+            d_lutoff.upload(h_lutoff, st);
+            d_fnbr.upload(h_fnbr, st);
+            d_fbase.upload(h_fbase, st);
+            d_nbr26.upload(h_nbr26, st);
+            uploaded_cap = cap;
+        } else {
+            // only the rows of the rebuilt samples' slots, scattered on the device
+            size_t nrow = 0;
+            for (int s : dirty) nrow += samples[s].grid->leaves().size();
+            packed.resize(nrow * ROW);
+            size_t r = 0;
+            for (int s : dirty)
+                for (const auto &lf : samples[s].grid->leaves()) {
+                    int *q = packed.data() + r * ROW;
+                    const int sl = lf.slot;
+                    q[0] = sl;
+                    q[1] = h_level[sl];
+                    for (int a = 0; a < 3; a++) q[2 + a] = h_index[(size_t)sl * 3 + a];
+                    q[5] = h_ssample[sl];
+                    q[6] = h_lutoff[sl];
+                    for (int f = 0; f < 6; f++) q[7 + f] = h_fnbr[(size_t)sl * 6 + f];
+                    for (int f = 0; f < 6; f++) q[13 + f] = h_fbase[(size_t)sl * 6 + f];
+                    for (int c = 0; c < 26; c++) q[19 + c] = h_nbr26[(size_t)sl * 26 + c];
+                    r++;
+                }
+            d_packed.upload(packed, st);
+            gq::SlotTables t{d_level.p, d_index.p, d_slot_sample.p, d_lutoff.p, d_fnbr.p, d_fbase.p, d_nbr26.p};
+            HIPCHK(gq::launch_scatter_slot_rows(t, d_packed.p, (int)nrow, st));
+        }
         d_work.upload(work, st);
-        d_level.upload(lvl, st);
-        d_index.upload(idx, st);
-        d_slot_sample.upload(ssample, st);
-        d_lutoff.upload(lutoff, st);
-        d_fnbr.upload(h_fnbr, st);
-        d_fbase.upload(fbase, st);
-        d_nbr26.upload(h_nbr26, st);
+        d_sfbase.upload(sfbase, st);
         d_lut.upload(lut_slot, st);
         d_lutf.upload(lutf, st);
         d_faces.upload(faces, st);
@@ -361,7 +410,10 @@ struct gq_ctx {
             HIPCHK(gq::launch_face_rows_fill(la, d_faces.p, nfaces, d_face_ptr.p, d_face_src.p, d_face_w.p, st));
         }
         d_gphi.reserve((size_t)std::max(1, face_nrows));
-        d_fik.reserve((size_t)cap * 6 * 256);
+        if (d_fik.n < (size_t)cap * 6 * 256) {
+            d_fik.reserve((size_t)cap * 6 * 256);  // contents not kept
+            fik_valid = false;
+        }
         if (!d_ih2.p) {
             std::vector<float> ih2(16);
             for (int l = 0; l < 16; l++) {
@@ -373,7 +425,6 @@ struct gq_ctx {
         }
         HIPCHK(hipStreamSynchronize(st));  // the host vectors above are released on return
         tables_valid = true;
-        fik_valid = false;
     }
 
     gq::LabArgs lab_args() const
@@ -389,16 +440,39 @@ struct gq_ctx {
         return a;
     }
 
-    // 1/K of the face halos for (Dw, Dg), or for the per-sample parameters in d_sprm when key has 2*S entries.
+    // 1/K of the face halos for (Dw, Dg), or for the per-sample parameters in d_sprm (per_sample).  Only the
+    // slots of samples whose topology changed are recomputed unless the parameters or the anatomy changed.
     void ensure_fik(const std::vector<float> &key, bool per_sample)
     {
         ensure_tables();
-        if (fik_valid && key == fik_key) return;
-        HIPCHK(gq::launch_face_prep(pw, pg, d_work.p, (int)work.size(), d_fnbr.p, d_fbase.p, d_face_ptr.p,
-                                    d_face_src.p, d_face_w.p, d_fik.p, key[0], key[1],
-                                    per_sample ? d_slot_sample.p : nullptr, per_sample ? d_sprm.p : nullptr, st));
-        fik_valid = true;
-        fik_key = key;
+        const int S = nsamples();
+        if (!fik_valid || key != fik_key) {
+            fik_stale.assign(S, 1);
+            fik_valid = true;
+            fik_key = key;
+        }
+        bool all = true, any = false;
+        for (int s = 0; s < S; s++) {
+            all = all && fik_stale[s];
+            any = any || fik_stale[s];
+        }
+        if (!any) return;
+        const int *list = d_work.p;
+        int n = (int)work.size();
+        if (!all) {
+            std::vector<int> sl;
+            for (int pos = 0; pos < S; pos++)
+                if (fik_stale[sample_order[pos]])
+                    sl.insert(sl.end(), work.begin() + work_begin[pos], work.begin() + work_begin[pos + 1]);
+            d_list.upload(sl, st);
+            list = d_list.p;
+            n = (int)sl.size();
+            HIPCHK(hipStreamSynchronize(st));  // sl is released on return
+        }
+        HIPCHK(gq::launch_face_prep(pw, pg, list, n, d_fnbr.p, d_fbase.p, d_sfbase.p, d_slot_sample.p, d_face_ptr.p,
+                                    d_face_src.p, d_face_w.p, d_fik.p, key[0], key[1], per_sample ? d_sprm.p : nullptr,
+                                    st));
+        fik_stale.assign(S, 0);
     }
 
     // FWT detail of each listed slot (any samples)
@@ -694,6 +768,7 @@ struct gq_ctx {
         a.nwork = (int)work.size();
         a.fnbr = d_fnbr.p;
         a.fbase = d_fbase.p;
+        a.sfbase = d_sfbase.p;
         a.fik = d_fik.p;
         a.gphi = d_gphi.p;
         a.level = d_level.p;
@@ -914,6 +989,12 @@ static void create_samples(gq_ctx *ctx, int nsamples, int bpd, int max_level, in
     ctx->fik_valid = false;
     ctx->h_fnbr.clear();
     ctx->h_nbr26.clear();
+    ctx->h_level.clear();
+    ctx->h_index.clear();
+    ctx->h_ssample.clear();
+    ctx->h_lutoff.clear();
+    ctx->h_fbase.clear();
+    ctx->uploaded_cap = 0;
     ctx->release_pool();
     ctx->free_slots.clear();
     ctx->samples = std::move(smp);
@@ -1223,7 +1304,8 @@ gq_status gq_ensemble_ic_gaussian(gq_ctx *ctx, const double *ic)
             if (!std::isfinite(ic[k])) throw gq_error(GQ_ERR_NUMERIC, "non-finite initial-condition parameter");
         for (int s = 0; s < S; s++)
             if (!(ic[5 * s + 4] > 0)) throw gq_error(GQ_ERR_NUMERIC, "sigma must be positive");
-        ctx->d_icp.upload(std::vector<double>(ic, ic + 5 * S), ctx->st);
+        const std::vector<double> icv(ic, ic + 5 * S);  // kept until the synchronisation below
+        ctx->d_icp.upload(icv, ctx->st);
         const int gpd = ctx->G(0).blocks_per_dim() * gq::BS;
         HIPCHK(gq::launch_ic_gaussian(ctx->phi[ctx->cur], ctx->d_work.p, (int)ctx->work.size(), ctx->d_index.p,
                                       ctx->d_slot_sample.p, ctx->d_icp.p, gpd, ctx->st));

@@ -75,13 +75,10 @@ Grid::Grid(int bpd, int max_level, int order) : bpd_(bpd), max_level_(max_level)
 
 void Grid::rebuild_index()
 {
-    key2leaf_.clear();
-    key2leaf_.reserve(leaves_.size() * 2);
     const int64_t nF = 1LL << lut_level_;
     lut_.assign((size_t)(nF * nF * nF), -1);
     for (int i = 0; i < (int)leaves_.size(); i++) {
         const Leaf &lf = leaves_[i];
-        key2leaf_[leaf_key(lf.level, lf.idx[0], lf.idx[1], lf.idx[2])] = i;
         const int d = lut_level_ - lf.level;
         const int64_t s = 1LL << d;
         for (int64_t z = (int64_t)lf.idx[2] << d; z < ((int64_t)lf.idx[2] << d) + s; z++)
@@ -95,8 +92,15 @@ void Grid::rebuild_index()
 
 int Grid::find(int level, int x, int y, int z) const
 {
-    auto it = key2leaf_.find(leaf_key(level, x, y, z));
-    return it == key2leaf_.end() ? -1 : it->second;
+    // the finest-level look-up cell at the box's corner holds the leaf if it exists
+    if (level < 0 || level > lut_level_) return -1;
+    const int64_t n = 1LL << level;
+    if (x < 0 || y < 0 || z < 0 || x >= n || y >= n || z >= n) return -1;
+    const int d = lut_level_ - level;
+    const int64_t nF = 1LL << lut_level_;
+    const int i = lut_[(size_t)(((int64_t)x << d) + nF * (((int64_t)y << d) + nF * ((int64_t)z << d)))];
+    const Leaf &lf = leaves_[i];
+    return (lf.level == level && lf.idx[0] == x && lf.idx[1] == y && lf.idx[2] == z) ? i : -1;
 }
 
 int Grid::lying_leaf(int lam, const int64_t p[3]) const

@@ -97,7 +97,6 @@ private:
     int bpd_, base_level_, max_level_, order_, lut_level_;
     int next_id_ = 0;
     std::vector<Leaf> leaves_;
-    std::unordered_map<uint64_t, int> key2leaf_;
     std::vector<int> lut_;  // (2^lut_level)^3 -> leaf index
 };
 

@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
             const int f = side == 0 ? 4 : 5;
             const int nb = a.fnbr[slot * 6 + f];
             if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, tx, ty)];
-            else pv = a.gphi[a.fbase[slot * 6 + f] + t];
+            else pv = a.gphi[a.sfbase[a.slot_sample[slot]] + a.fbase[slot * 6 + f] + t];
             kv = a.fik[((size_t)slot * 6 + f) * 256 + t];
         }
         s_phi[l] = pv;
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
         const int nb = a.fnbr[slot * 6 + f];
         float pv;
         if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, u, v)];
-        else pv = a.gphi[a.fbase[slot * 6 + f] + u + 16 * v];
+        else pv = a.gphi[a.sfbase[a.slot_sample[slot]] + a.fbase[slot * 6 + f] + u + 16 * v];
         const float kv = a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
         int l;
         switch (f) {
@@ -213,36 +213,40 @@ __global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ p
 __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw, const float *__restrict__ pg,
                                                    const int *__restrict__ work, int nwork,
                                                    const int *__restrict__ fnbr, const int *__restrict__ fbase,
+                                                   const int *__restrict__ sfbase,
+                                                   const int *__restrict__ slot_sample,
                                                    const int64_t *__restrict__ ptr, const int *__restrict__ src,
                                                    const float *__restrict__ w, float *__restrict__ fik, float Dw,
-                                                   float Dg, const int *__restrict__ slot_sample,
-                                                   const float *__restrict__ sprm)
+                                                   float Dg, const float *__restrict__ sprm)
 {
-    const int wi = blockIdx.x / 6;
-    const int f = blockIdx.x % 6;
+    const int wi = blockIdx.x;
     if (wi >= nwork) return;
     const int slot = work[wi];
+    const int rb = sfbase[slot_sample[slot]];
     if (sprm) {
         Dw = sprm[4 * slot_sample[slot]];
         Dg = sprm[4 * slot_sample[slot] + 1];
     }
     const int t = threadIdx.x;
-    const int nb = fnbr[slot * 6 + f];
-    float wm, gm;
-    if (nb >= 0) {
-        const size_t i = (size_t)nb * B3 + nbr_face_cell(f, t & 15, t >> 4);
-        wm = pw[i];
-        gm = pg[i];
-    } else {
-        const int r = fbase[slot * 6 + f] + t;
-        wm = 0.f;
-        gm = 0.f;
-        for (int64_t k = ptr[r]; k < ptr[r + 1]; k++) {
-            wm += w[k] * pw[src[k]];
-            gm += w[k] * pg[src[k]];
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        const int nb = fnbr[slot * 6 + f];
+        float wm, gm;
+        if (nb >= 0) {
+            const size_t i = (size_t)nb * B3 + nbr_face_cell(f, t & 15, t >> 4);
+            wm = pw[i];
+            gm = pg[i];
+        } else {
+            const int r = rb + fbase[slot * 6 + f] + t;
+            wm = 0.f;
+            gm = 0.f;
+            for (int64_t k = ptr[r]; k < ptr[r + 1]; k++) {
+                wm += w[k] * pw[src[k]];
+                gm += w[k] * pg[src[k]];
+            }
         }
+        fik[((size_t)slot * 6 + f) * 256 + t] = inv_k(wm, gm, Dw, Dg);
     }
-    fik[((size_t)slot * 6 + f) * 256 + t] = inv_k(wm, gm, Dw, Dg);
 }
 
 // ---------------------------------------------------------------------------
@@ -543,6 +547,75 @@ __global__ __launch_bounds__(256) void k_lab_ghosts(LabArgs a, FieldPtrs flds, c
     for (int f = 0; f < NF; f++) ghost[f * field_stride + o] = v[f];
 }
 
+namespace {
+// Terms of one ghost point (mode 1: count, 2: emit (slot*4096+cell, weight)),
+// in the order of the depth-first walk, with the stack-free fast path of
+// ghost_value for children that lie in leaves of their own level.
+template <int MODE>
+__device__ void ghost_terms(const LabArgs &g, const int *__restrict__ lut, int lam, int64_t px, int64_t py,
+                            int64_t pz, int *count, int *src, float *wout)
+{
+    for (;;) {
+        const int slot = lying_slot(g, lut, lam, px, py, pz);
+        const int l = g.level[slot];
+        if (l == lam) {
+            if (MODE == 2) {
+                const int64_t msk = ((int64_t)16 << lam) - 1;
+                const int cx = (int)((px & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
+                const int cy = (int)((py & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
+                const int cz = (int)((pz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
+                src[*count] = slot * 4096 + cx + 16 * (cy + 16 * cz);
+                wout[*count] = 1.0f;
+            }
+            (*count)++;
+            return;
+        }
+        if (l > lam) {
+            lam++;
+            px *= 2;
+            py *= 2;
+            pz *= 2;
+            continue;
+        }
+        break;
+    }
+    const int lo = g.order == 4 ? -3 : -1, hi = g.order == 4 ? 4 : 2;
+    const int64_t p[3] = {px, py, pz};
+    int64_t s[3], e[3];
+    for (int d = 0; d < 3; d++) {
+        s[d] = (p[d] + lo + 1) >> 1;
+        e[d] = ((p[d] + hi - 1) >> 1) + 1;
+    }
+    const int cl = lam - 1;
+    const int64_t msk = ((int64_t)16 << cl) - 1;
+    double unused = 0.0;
+    for (int64_t iz = s[2]; iz < e[2]; iz++) {
+        const double wz = hs_w(g.order, (int)(2 * iz - p[2]));
+        for (int64_t iy = s[1]; iy < e[1]; iy++) {
+            const double wy = hs_w(g.order, (int)(2 * iy - p[1]));
+            for (int64_t ix = s[0]; ix < e[0]; ix++) {
+                const double ww = __dmul_rn(__dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), wy), wz);
+                if (ww == 0.0) continue;
+                const double cw = __dmul_rn(1.0, ww);
+                const int slot = lying_slot(g, lut, cl, ix, iy, iz);
+                if (g.level[slot] == cl) {
+                    if (MODE == 2) {
+                        const int cx = (int)((ix & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
+                        const int cy = (int)((iy & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
+                        const int cz = (int)((iz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
+                        src[*count] = slot * 4096 + cx + 16 * (cy + 16 * cz);
+                        wout[*count] = (float)cw;
+                    }
+                    (*count)++;
+                } else {
+                    ghost_dfs<MODE>(g, lut, nullptr, cl, ix, iy, iz, cw, &unused, count, src, wout);
+                }
+            }
+        }
+    }
+}
+}  // namespace
+
 // Step rows of cross-level faces (count / fill passes).
 __device__ __forceinline__ void face_point(int f, int u, int v, int &x, int &y, int &z)
 {
@@ -567,9 +640,7 @@ __global__ __launch_bounds__(256) void k_face_rows_count(LabArgs a, const int *_
     const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
     int n = 0;
-    double unused = 0.0;
-    ghost_dfs<1>(a, a.lut + a.lutoff[slot], nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, &unused, &n, nullptr,
-                 nullptr);
+    ghost_terms<1>(a, a.lut + a.lutoff[slot], a.level[slot], ox + x, oy + y, oz + z, &n, nullptr, nullptr);
     counts[(int64_t)k * 256 + threadIdx.x] = n;
 }
 
@@ -586,9 +657,7 @@ __global__ __launch_bounds__(256) void k_face_rows_fill(LabArgs a, const int *__
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
     const int64_t r = (int64_t)k * 256 + threadIdx.x;
     int n = 0;
-    double unused = 0.0;
-    ghost_dfs<2>(a, a.lut + a.lutoff[slot], nullptr, a.level[slot], ox + x, oy + y, oz + z, 1.0, &unused, &n,
-                 src + ptr[r], w + ptr[r]);
+    ghost_terms<2>(a, a.lut + a.lutoff[slot], a.level[slot], ox + x, oy + y, oz + z, &n, src + ptr[r], w + ptr[r]);
 }
 
 namespace {
@@ -690,52 +759,70 @@ __global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const double *__r
 
 // BlockSplitter::_splitBlocks (MRAGBlockSplitter.h:24-99) as a gather: child
 // point d = sum over parent lab points s (z, y, x increasing) of
-// lab(s) * ((wZ*wY)*wX), w = Hs(2s - d), for phi, pw and pg.
+// lab(s) * ((wZ*wY)*wX), w = Hs(2s - d), for phi, pw and pg.  One workgroup
+// per (parent, child, field): the child's 16^3 points read parent points
+// m-1..m+2 (Interp4) or m..m+1 (Interp2) around m = d/2, an 11^3 (9^3)
+// window of the parent lab staged in LDS.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restrict__ ghost3, float *phi, float *pw,
                                                float *pg, const int *__restrict__ parents,
-                                               const int *__restrict__ children)
+                                               const int *__restrict__ children, int nparents)
 {
     constexpr int S0 = ORDER == 4 ? -2 : -1;
     constexpr int S1 = ORDER == 4 ? 4 : 2;
     constexpr int L = LabGeom<S0, S1>::L;
-    __shared__ double lab[LabGeom<S0, S1>::N];
-    const int pslot = parents[blockIdx.x];
-    float *fields[3] = {phi, pw, pg};
-    for (int f = 0; f < 3; f++) {
-        fill_lab<S0, S1>(lab, fields[f], ghost3 + ((size_t)f * gridDim.x + blockIdx.x) * LabGeom<S0, S1>::N, pslot,
-                         a);
-        __syncthreads();
-        for (int q = threadIdx.x; q < 8 * B3; q += blockDim.x) {
-            const int dx = q & 31, dy = (q >> 5) & 31, dz = q >> 10;
-            const int mx = dx >> 1, my = dy >> 1, mz = dz >> 1;
-            const int px = dx & 1, py = dy & 1, pz = dz & 1;
-            // taps per parity: even -> s = m (w 1); odd -> Interp4: m-1..m+2
-            // (-1/16, 9/16, 9/16, -1/16), Interp2: m, m+1 (1/2, 1/2)
-            const int nz = pz ? (ORDER == 4 ? 4 : 2) : 1;
-            const int ny = py ? (ORDER == 4 ? 4 : 2) : 1;
-            const int nx = px ? (ORDER == 4 ? 4 : 2) : 1;
-            double acc = 0.0;
-            for (int kz = 0; kz < nz; kz++) {
-                const int oz = pz ? (ORDER == 4 ? kz - 1 : kz) : 0;
-                const double wz = pz ? (ORDER == 4 ? ((kz == 0 || kz == 3) ? -1.0 / 16.0 : 9.0 / 16.0) : 0.5) : 1.0;
-                for (int ky = 0; ky < ny; ky++) {
-                    const int oy = py ? (ORDER == 4 ? ky - 1 : ky) : 0;
-                    const double wy = py ? (ORDER == 4 ? ((ky == 0 || ky == 3) ? -1.0 / 16.0 : 9.0 / 16.0) : 0.5) : 1.0;
-                    const double wzy = __dmul_rn(wz, wy);
-                    for (int kx = 0; kx < nx; kx++) {
-                        const int ox = px ? (ORDER == 4 ? kx - 1 : kx) : 0;
-                        const double wx = px ? (ORDER == 4 ? ((kx == 0 || kx == 3) ? -1.0 / 16.0 : 9.0 / 16.0) : 0.5) : 1.0;
-                        const int s = (mx + ox - S0) + L * ((my + oy - S0) + L * (mz + oz - S0));
-                        acc = __dadd_rn(acc, __dmul_rn(lab[s], __dmul_rn(wzy, wx)));
-                    }
+    constexpr int W0 = ORDER == 4 ? -1 : 0;  // window start relative to 8*child
+    constexpr int WL = ORDER == 4 ? 11 : 9;  // window width
+    __shared__ double win[WL * WL * WL];
+    const int pidx = blockIdx.x;
+    const int child = blockIdx.y;
+    const int f = blockIdx.z;
+    const int pslot = parents[pidx];
+    const int cx = child & 1, cy = (child >> 1) & 1, cz = child >> 2;
+    float *fld = f == 0 ? phi : (f == 1 ? pw : pg);
+    const double *ghost = ghost3 + ((size_t)f * nparents + pidx) * LabGeom<S0, S1>::N;
+    for (int q = threadIdx.x; q < WL * WL * WL; q += 256) {
+        const int x = 8 * cx + W0 + q % WL, y = 8 * cy + W0 + (q / WL) % WL, z = 8 * cz + W0 + q / (WL * WL);
+        const int dx = x < 0 ? -1 : (x < B ? 0 : 1), dy = y < 0 ? -1 : (y < B ? 0 : 1),
+                  dz = z < 0 ? -1 : (z < B ? 0 : 1);
+        double v;
+        if ((dx | dy | dz) == 0) {
+            v = (double)fld[(size_t)pslot * B3 + x + B * (y + B * z)];
+        } else {
+            const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+            const int nb = a.nbr26[pslot * 26 + (c < 13 ? c : c - 1)];
+            if (nb >= 0) v = (double)fld[(size_t)nb * B3 + (x - dx * B) + B * ((y - dy * B) + B * (z - dz * B))];
+            else v = ghost[(x - S0) + L * ((y - S0) + L * (z - S0))];
+        }
+        win[q] = v;
+    }
+    __syncthreads();
+    const int cs = children[pidx * 8 + child];
+    for (int q = threadIdx.x; q < B3; q += 256) {
+        const int lx = q & 15, ly = (q >> 4) & 15, lz = q >> 8;
+        // child point in parent-lab coordinates: m = (16*c + l) / 2, parity of l
+        const int mx = 8 * cx + (lx >> 1), my = 8 * cy + (ly >> 1), mz = 8 * cz + (lz >> 1);
+        const int px = lx & 1, py = ly & 1, pz = lz & 1;
+        const int nz = pz ? (ORDER == 4 ? 4 : 2) : 1;
+        const int ny = py ? (ORDER == 4 ? 4 : 2) : 1;
+        const int nx = px ? (ORDER == 4 ? 4 : 2) : 1;
+        double acc = 0.0;
+        for (int kz = 0; kz < nz; kz++) {
+            const int oz = pz ? (ORDER == 4 ? kz - 1 : kz) : 0;
+            const double wz = pz ? (ORDER == 4 ? ((kz == 0 || kz == 3) ? -1.0 / 16.0 : 9.0 / 16.0) : 0.5) : 1.0;
+            for (int ky = 0; ky < ny; ky++) {
+                const int oy = py ? (ORDER == 4 ? ky - 1 : ky) : 0;
+                const double wy = py ? (ORDER == 4 ? ((ky == 0 || ky == 3) ? -1.0 / 16.0 : 9.0 / 16.0) : 0.5) : 1.0;
+                const double wzy = __dmul_rn(wz, wy);
+                for (int kx = 0; kx < nx; kx++) {
+                    const int ox = px ? (ORDER == 4 ? kx - 1 : kx) : 0;
+                    const double wx = px ? (ORDER == 4 ? ((kx == 0 || kx == 3) ? -1.0 / 16.0 : 9.0 / 16.0) : 0.5) : 1.0;
+                    const int sidx = (mx + ox - 8 * cx - W0) + WL * ((my + oy - 8 * cy - W0) + WL * (mz + oz - 8 * cz - W0));
+                    acc = __dadd_rn(acc, __dmul_rn(win[sidx], __dmul_rn(wzy, wx)));
                 }
             }
-            const int child = (dx >> 4) + 2 * ((dy >> 4) + 2 * (dz >> 4));
-            const int cs = children[blockIdx.x * 8 + child];
-            fields[f][(size_t)cs * B3 + (dx & 15) + B * ((dy & 15) + B * (dz & 15))] = (float)acc;
         }
-        __syncthreads();
+        fld[(size_t)cs * B3 + q] = (float)acc;
     }
 }
 
@@ -905,14 +992,20 @@ __global__ __launch_bounds__(256) void k_lik_ref_ti(UniformArgs u, const double 
     block_sum_store<2>(acc, partial);
 }
 
-// Sum per-block partials in block order (deterministic).
-__global__ void k_sum_partials(const double *partial, int nblocks, int nv, double *out)
+// Sum per-block partials: workgroup (s, k) sums value k of sample s over its
+// nblocks partials (layout [s][block][nv]); each thread a fixed strided
+// subset in increasing order, then a fixed reduction tree (deterministic).
+__global__ __launch_bounds__(256) void k_sum_partials(const double *partial, int nblocks, int nv, double *out)
 {
-    const int k = threadIdx.x;
-    if (k >= nv) return;
+    __shared__ double sh[4];
+    const int k = blockIdx.x, smp = blockIdx.y;
+    const double *p = partial + (size_t)smp * nblocks * nv;
     double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * nv + k];
-    out[k] = s;
+    for (int b = threadIdx.x; b < nblocks; b += 256) s += p[(size_t)b * nv + k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[(size_t)smp * nv + k] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
 // ---------------------------------------------------------------------------
@@ -942,12 +1035,13 @@ hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *s
 }
 
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
-                            const int *fbase, const int64_t *ptr, const int *src, const float *w, float *fik, float Dw,
-                            float Dg, const int *slot_sample, const float *sprm, hipStream_t st)
+                            const int *fbase, const int *sfbase, const int *slot_sample, const int64_t *ptr,
+                            const int *src, const float *w, float *fik, float Dw, float Dg, const float *sprm,
+                            hipStream_t st)
 {
     if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_face_prep, dim3(nwork * 6), dim3(256), 0, st, pw, pg, work, nwork, fnbr, fbase, ptr, src, w,
-                       fik, Dw, Dg, slot_sample, sprm);
+    hipLaunchKernelGGL(k_face_prep, dim3(nwork), dim3(256), 0, st, pw, pg, work, nwork, fnbr, fbase, sfbase,
+                       slot_sample, ptr, src, w, fik, Dw, Dg, sprm);
     return hipGetLastError();
 }
 
@@ -997,10 +1091,11 @@ hipError_t launch_split(int order, const LabArgs &a, const double *ghost3, float
                         const int *parents, const int *children, int nparents, hipStream_t st)
 {
     if (nparents <= 0) return hipSuccess;
+    const dim3 grid(nparents, 8, 3);
     if (order == 4)
-        hipLaunchKernelGGL(k_split<4>, dim3(nparents), dim3(256), 0, st, a, ghost3, phi, pw, pg, parents, children);
+        hipLaunchKernelGGL(k_split<4>, grid, dim3(256), 0, st, a, ghost3, phi, pw, pg, parents, children, nparents);
     else
-        hipLaunchKernelGGL(k_split<2>, dim3(nparents), dim3(256), 0, st, a, ghost3, phi, pw, pg, parents, children);
+        hipLaunchKernelGGL(k_split<2>, grid, dim3(256), 0, st, a, ghost3, phi, pw, pg, parents, children, nparents);
     return hipGetLastError();
 }
 
@@ -1050,9 +1145,7 @@ hipError_t launch_lik_flip(const UniformArgs &u, int nsamples, const uint8_t *t1
 {
     hipLaunchKernelGGL(k_lik_flip, dim3(nblocks, nsamples), dim3(256), 0, st, u, t1, t2, pet, th1, th2, lp1, lp0, c0,
                        is2, partial);
-    for (int s = 0; s < nsamples; s++)
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial + (size_t)s * nblocks * 3, nblocks, 3,
-                           out + 3 * s);
+    hipLaunchKernelGGL(k_sum_partials, dim3(3, nsamples), dim3(256), 0, st, partial, nblocks, 3, out);
     return hipGetLastError();
 }
 
@@ -1062,10 +1155,11 @@ hipError_t launch_lik_ref(const UniformArgs &u, const double *pet, const double 
                           hipStream_t st)
 {
     hipLaunchKernelGGL(k_lik_ref_pet, dim3(nblocks), dim3(256), 0, st, u, pet, step_pet, scale, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial, nblocks, 2, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(2, 1), dim3(256), 0, st, partial, nblocks, 2, out);
     hipLaunchKernelGGL(k_lik_ref_ti, dim3(nblocks), dim3(256), 0, st, u, t1, t2, pts, npts, selective, step_ti, t1uc,
                        t2uc, slope, partial + (size_t)nblocks * 2);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial + (size_t)nblocks * 2, nblocks, 2, out + 2);
+    hipLaunchKernelGGL(k_sum_partials, dim3(2, 1), dim3(256), 0, st, partial + (size_t)nblocks * 2, nblocks, 2,
+                       out + 2);
     return hipGetLastError();
 }
 
@@ -1149,5 +1243,34 @@ hipError_t scan_counts(const int *counts, int64_t *ptr, int n, void *temp, size_
     if (e != hipSuccess) return e;
     if (n <= 0) return hipSuccess;
     return hipcub::DeviceScan::InclusiveSum(temp, temp_bytes, counts, ptr + 1, n, st);
+}
+}  // namespace gq
+
+namespace gq {
+__global__ __launch_bounds__(64) void k_scatter_slot_rows(SlotTables t, const int *__restrict__ packed, int nrows)
+{
+    const int r = blockIdx.x;
+    if (r >= nrows) return;
+    const int *q = packed + (size_t)r * 45;
+    const int sl = q[0];
+    const int k = threadIdx.x;
+    if (k == 0) {
+        t.level[sl] = q[1];
+        t.slot_sample[sl] = q[5];
+        t.lutoff[sl] = q[6];
+    }
+    if (k < 3) t.index[sl * 3 + k] = q[2 + k];
+    if (k < 6) {
+        t.fnbr[sl * 6 + k] = q[7 + k];
+        t.fbase[sl * 6 + k] = q[13 + k];
+    }
+    if (k < 26) t.nbr26[sl * 26 + k] = q[19 + k];
+}
+
+hipError_t launch_scatter_slot_rows(const SlotTables &t, const int *packed, int nrows, hipStream_t st)
+{
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_slot_rows, dim3(nrows), dim3(64), 0, st, t, packed, nrows);
+    return hipGetLastError();
 }
 }  // namespace gq

@@ -15,7 +15,8 @@ struct StepArgs {
     const int *work;   // Morton-ordered slots
     int nwork;
     const int *fnbr;   // [slot*6+f]
-    const int *fbase;  // [slot*6+f]
+    const int *fbase;  // [slot*6+f] first face-ghost row, relative to the sample's (-1: same level)
+    const int *sfbase; // [sample] first face-ghost row of the sample
     const float *fik;  // [slot*6+f][256]
     const float *gphi; // cross-level face ghosts of phi_in
     const int *level;  // [slot]
@@ -81,10 +82,17 @@ hipError_t launch_ic_grad(const float *lam0, const int *work, int nwork, const i
 hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st);
 hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *src, const float *w, float *out,
                               int nrows, hipStream_t st);
-// slot_sample/sprm: per-sample Dw, Dg (ensembles) or null for (Dw, Dg)
+// sprm: per-sample Dw, Dg (ensembles) or null for (Dw, Dg)
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
-                            const int *fbase, const int64_t *ptr, const int *src, const float *w, float *fik, float Dw,
-                            float Dg, const int *slot_sample, const float *sprm, hipStream_t st);
+                            const int *fbase, const int *sfbase, const int *slot_sample, const int64_t *ptr,
+                            const int *src, const float *w, float *fik, float Dw, float Dg, const float *sprm,
+                            hipStream_t st);
+// Slot-indexed tables and the scatter of packed rows {slot, level, index[3],
+// sample, lutoff, fnbr[6], fbase[6], nbr26[26]} into them.
+struct SlotTables {
+    int *level, *index, *slot_sample, *lutoff, *fnbr, *fbase, *nbr26;
+};
+hipError_t launch_scatter_slot_rows(const SlotTables &t, const int *packed, int nrows, hipStream_t st);
 // Ghost values of the cross-level points of the wide lab of each listed slot
 // (FP64, depth-first order) into ghost[k * lab_points + q].
 // for_split: the points the split reads; else the points the FWT reads.

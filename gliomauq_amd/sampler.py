@@ -1,0 +1,215 @@
+"""Samplers over the GPU forward/adjoint path.
+
+The reference samples with an external TMCMC code that calls the `brain` and
+`likelihood` programs once per proposal (makefile/Example/runHGG.sh).  Here
+the posterior over theta = (log Dw [cm^2/day], log rho [1/day], x0, y0, z0 [voxels],
+A) of the Gaussian-IC Fisher-Kolmogorov model with the flip-rate likelihood is
+evaluated for many parameter sets per GPU call:
+
+  Posterior          log-likelihood (+ gradient from the discrete adjoint) of
+                     a batch of parameter sets on a uniform grid, box prior
+  rwm                plain random-walk Metropolis, one forward solve per
+                     proposal (the reference's pattern, on the GPU)
+  mala               Metropolis-adjusted Langevin, many chains per call,
+                     gradient from gq_adjoint_flip, per-chain step sizes
+                     adapted towards an acceptance of 0.57 during burn-in
+  tmcmc              transitional MCMC (tempered stages, resampling and
+                     per-stage Metropolis moves on the whole population)
+  ess_batch_means    effective sample size by batch means
+"""
+import math
+import time
+
+import numpy as np
+
+from .solver import BS, Ensemble
+
+PARAM_NAMES = ("logDw", "logrho", "x0", "y0", "z0", "A")
+
+
+class Posterior:
+    """Log-posterior of the Gaussian-IC FK model on an n^3 uniform grid of a
+    cube of edge L_cm with n_steps steps of dt days; box prior `lo`/`hi`."""
+
+    def __init__(self, ctx, pwm, pgm, obs, n_steps, dt_days, lo, hi, batch, L_cm=17.0, sigma_ic=2.0,
+                 ckpt_every=50, lik=(0.7, 0.25, 0.05, 0.05)):
+        n = pwm.shape[0]
+        self.n = n
+        self.L = L_cm
+        self.n_steps = int(n_steps)
+        self.dt = float(dt_days)
+        self.lo = np.asarray(lo, np.float64)
+        self.hi = np.asarray(hi, np.float64)
+        self.sigma_ic = sigma_ic
+        self.ckpt = ckpt_every
+        self.lik = lik
+        self.batch = batch
+        lvl = int(round(math.log2(n // BS)))
+        self.ens = Ensemble(ctx, batch, n // BS, lvl, 4)
+        self.ens.write_volume("pw", pwm)
+        self.ens.write_volume("pg", pgm)
+        self.ens.set_observations(*obs)
+        self.n_evals = 0
+        self.n_grad_evals = 0
+
+    def _inputs(self, th):
+        th = np.asarray(th, np.float64).reshape(-1, 6)
+        m = th.shape[0]
+        if m > self.batch:
+            raise ValueError("more parameter sets than the batch")
+        inside = np.all((th >= self.lo) & (th <= self.hi), axis=1)
+        thc = np.where(inside[:, None], th, 0.5 * (self.lo + self.hi))  # keep the solver in range
+        full = np.tile(0.5 * (self.lo + self.hi), (self.batch, 1))
+        full[:m] = thc
+        Dw = np.exp(full[:, 0]) / self.L ** 2
+        prm = np.stack([Dw, 0.1 * Dw, np.exp(full[:, 1]), np.full(self.batch, self.dt)], 1)
+        ic = np.stack([full[:, 2], full[:, 3], full[:, 4], full[:, 5], np.full(self.batch, self.sigma_ic)], 1)
+        return m, inside, prm, ic
+
+    def logpost_grad(self, th):
+        m, inside, prm, ic = self._inputs(th)
+        ll, gr = self.ens.adjoint_flip(prm, self.n_steps, self.ckpt, ic=ic, th1=self.lik[0], th2=self.lik[1],
+                                       flip=self.lik[2], sigma=self.lik[3])
+        self.n_evals += m
+        self.n_grad_evals += m
+        lp = np.where(inside, ll[:m, 0], -np.inf)
+        return lp, gr[:m]
+
+    def logpost(self, th):
+        m, inside, prm, ic = self._inputs(th)
+        self.ens.ic_gaussian(ic)
+        self.ens.run(prm, np.full(self.batch, self.n_steps, np.int32), adapt_every=0)
+        ll = self.ens.loglik_flip(th1=self.lik[0], th2=self.lik[1], flip=self.lik[2], sigma=self.lik[3])
+        self.n_evals += m
+        return np.where(inside, ll[:m, 0], -np.inf)
+
+
+def rwm(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, callback=None):
+    """Random-walk Metropolis, one chain, one forward solve per proposal.
+    Gaussian proposals N(0, diag(s^2)); the global factor adapts towards
+    `target` acceptance for the first `adapt_until` iterations."""
+    th = np.asarray(theta0, np.float64).copy()
+    lp = post.logpost(th[None])[0]
+    logf = 0.0
+    out = np.zeros((n_iter, 6))
+    acc = np.zeros(n_iter, bool)
+    for it in range(n_iter):
+        prop = th + np.exp(logf) * scale * rng.standard_normal(6)
+        lpp = post.logpost(prop[None])[0]
+        if np.log(rng.random()) < lpp - lp:
+            th, lp = prop, lpp
+            acc[it] = True
+        if it < adapt_until:
+            logf += (float(acc[it]) - target) / math.sqrt(it + 1.0)
+        out[it] = th
+        if callback:
+            callback(it, th, lp)
+    return out, acc
+
+
+def mala(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.57, callback=None):
+    """Metropolis-adjusted Langevin for a population of chains (theta0: (C, 6)),
+    preconditioner diag(scale^2), per-chain step size eps adapted towards
+    `target` acceptance for the first `adapt_until` iterations."""
+    th = np.asarray(theta0, np.float64).copy()
+    C = th.shape[0]
+    M = np.asarray(scale, np.float64) ** 2
+    lp, g = post.logpost_grad(th)
+    logeps = np.full(C, math.log(0.5))
+    out = np.zeros((n_iter, C, 6))
+    acc = np.zeros((n_iter, C), bool)
+    for it in range(n_iter):
+        eps = np.exp(logeps)[:, None]
+        mean_f = th + 0.5 * eps ** 2 * M * g
+        prop = mean_f + eps * np.sqrt(M) * rng.standard_normal((C, 6))
+        lpp, gp = post.logpost_grad(prop)
+        mean_b = prop + 0.5 * eps ** 2 * M * gp
+        lq_f = -np.sum((prop - mean_f) ** 2 / M, 1) / (2 * eps[:, 0] ** 2)
+        lq_b = -np.sum((th - mean_b) ** 2 / M, 1) / (2 * eps[:, 0] ** 2)
+        la = lpp - lp + lq_b - lq_f
+        a = np.log(rng.random(C)) < np.where(np.isfinite(la), la, -np.inf)
+        th = np.where(a[:, None], prop, th)
+        lp = np.where(a, lpp, lp)
+        g = np.where(a[:, None], gp, g)
+        acc[it] = a
+        if it < adapt_until:
+            logeps += (a.astype(np.float64) - target) / math.sqrt(it + 1.0)
+        out[it] = th
+        if callback:
+            callback(it, th, lp)
+    return out, acc
+
+
+def ess_batch_means(x, n_batches=None):
+    """ESS of a 1-D chain by batch means (batches of ~sqrt(n) samples)."""
+    x = np.asarray(x, np.float64)
+    n = len(x)
+    if n < 4:
+        return float(n)
+    b = n_batches or max(2, int(math.sqrt(n)))
+    m = n // b
+    xb = x[: m * b].reshape(b, m).mean(1)
+    var = x.var(ddof=1)
+    if var == 0:
+        return float(n)
+    bvar = m * xb.var(ddof=1)
+    return float(min(n, n * var / bvar)) if bvar > 0 else float(n)
+
+
+def tmcmc(post, prior_sample, n_pop, rng, cov_scale=0.2, max_stages=50, moves=1, cov_target=1.0):
+    """Transitional MCMC: tempered stages p_j ~ L^q_j prior, q_j chosen so the
+    coefficient of variation of the incremental weights is cov_target;
+    resampling by weight; one Metropolis move per sample per stage with the
+    weighted sample covariance scaled by cov_scale^2.  The population is
+    evaluated in batches of the posterior's batch size."""
+    th = prior_sample(n_pop, rng)
+    ll = _batched(post.logpost, th, post.batch)
+    q = 0.0
+    stages = []
+    for _ in range(max_stages):
+        lo, hi = q, 1.0
+        finite = np.isfinite(ll)
+        llf = np.where(finite, ll, -np.inf)
+        for _ in range(60):  # bisection on the tempering increment
+            mid = 0.5 * (lo + hi)
+            w = np.exp((mid - q) * (llf - llf[finite].max()))
+            cov = w.std() / max(w.mean(), 1e-300)
+            if cov > cov_target:
+                hi = mid
+            else:
+                lo = mid
+        qn = hi if hi < 1.0 else 1.0
+        w = np.exp((qn - q) * (llf - llf[finite].max()))
+        w /= w.sum()
+        mean = (w[:, None] * th).sum(0)
+        d = th - mean
+        S = (w[:, None, None] * d[:, :, None] * d[:, None, :]).sum(0) * cov_scale ** 2
+        idx = rng.choice(n_pop, size=n_pop, p=w)
+        th, ll = th[idx].copy(), ll[idx].copy()
+        chol = np.linalg.cholesky(S + 1e-12 * np.eye(6))
+        acc = 0
+        for _ in range(moves):
+            prop = th + rng.standard_normal((n_pop, 6)) @ chol.T
+            llp = _batched(post.logpost, prop, post.batch)
+            a = np.log(rng.random(n_pop)) < qn * (llp - ll)
+            th[a] = prop[a]
+            ll[a] = llp[a]
+            acc += int(a.sum())
+        stages.append({"q": qn, "accept": acc / (n_pop * moves)})
+        q = qn
+        if q >= 1.0:
+            break
+    return th, ll, stages
+
+
+def _batched(fn, th, batch):
+    out = np.empty(len(th))
+    for i in range(0, len(th), batch):
+        out[i:i + batch] = fn(th[i:i + batch])
+    return out
+
+
+def timed(fn, *a, **k):
+    t0 = time.perf_counter()
+    r = fn(*a, **k)
+    return r, time.perf_counter() - t0
