@@ -196,6 +196,102 @@ def adjoint_workload(args):
     print(json.dumps(res), flush=True)
 
 
+def mcmc_workload(args):
+    """Posterior samples/hour: 16-chain MALA (gradients from the discrete
+    adjoint, all chains in one ensemble call) against plain random-walk
+    Metropolis with one forward solve per proposal, on the same posterior of
+    theta = (log Dw, log rho, x0, y0, z0, A) (uniform grid, flip-rate
+    likelihood of synthetic observations).  Both samplers get the same
+    diagonal preconditioner from a pilot run and the same wall-time budget;
+    ESS by batch means, min over parameters, summed over chains."""
+    import gliomauq_amd as gq
+    from gliomauq_amd import anatomy as A
+    from gliomauq_amd import sampler as SM
+
+    n, steps, C = args.mcmc_n, args.mcmc_steps, args.chains
+    rng = np.random.default_rng(args.seed + 5)
+    pwm, pgm, u0, c = A.config_inputs(n, L_CM, args.seed)
+    h = 1.0 / n
+    dt_unit = 0.9 * h * h / (6 * 1.5e-2 / L_CM ** 2)  # stable for every Dw of the prior
+    theta_true = np.array([np.log(0.0013 * 1.25), np.log(0.025 * 0.8) + 1.0, c[2] + 0.3, c[1] - 0.4, c[0] + 0.2, 0.1])
+    lo = np.array([np.log(2e-4), np.log(2e-3), c[2] - 6, c[1] - 6, c[0] - 6, 0.02])
+    hi = np.array([np.log(1.5e-2), np.log(0.2), c[2] + 6, c[1] + 6, c[0] + 6, 0.3])
+    ctx = gq.Context(0)
+    # observations from the model at theta_true
+    truth = gq.Ensemble(ctx, 1, n // 16, int(np.log2(n // 16)), 4)
+    truth.write_volume("pw", pwm)
+    truth.write_volume("pg", pgm)
+    truth.ic_gaussian(np.array([[*theta_true[2:5], theta_true[5], 2.0]]))
+    Dt = np.exp(theta_true[0]) / L_CM ** 2
+    truth.run(np.array([[Dt, 0.1 * Dt, np.exp(theta_true[1]), dt_unit]]), [steps], adapt_every=0)
+    obs = A.observations(truth.sample(0).dump_uniform(), seed=args.seed)
+    del truth
+    # one context per posterior: a context holds one grid or ensemble at a time
+    post_b = SM.Posterior(gq.Context(0), pwm, pgm, obs, steps, dt_unit, lo, hi, batch=C)
+    post_1 = SM.Posterior(gq.Context(0), pwm, pgm, obs, steps, dt_unit, lo, hi, batch=1)
+    # preconditioner: per parameter, the offset from the truth where the
+    # log-posterior has dropped by 2 (geometric probe, batched)
+    log("[bench] mcmc scale probe")
+    scale = SM.probe_scales(post_b, theta_true, drop=2.0)
+    log("[bench] scales %s" % scale)
+    th0 = theta_true + 0.5 * scale * rng.standard_normal((C, 6))
+    log("[bench] mcmc pilot")
+    pil, _, logeps = SM.mala(post_b, th0, args.mcmc_pilot, scale, rng, adapt_until=args.mcmc_pilot)
+    start = pil[-1]
+
+    log("[bench] mcmc MALA %d chains, %.0f s" % (C, args.mcmc_budget))
+    samples_m, accs_m = [], []
+    th = start
+    t1 = time.perf_counter()
+    while time.perf_counter() - t1 < args.mcmc_budget:
+        o, a, logeps = SM.mala(post_b, th, 10, scale, rng, logeps=logeps)  # adapted step sizes kept
+        samples_m.append(o)
+        accs_m.append(a)
+        th = o[-1]
+    wall_m = time.perf_counter() - t1
+    xm = np.concatenate(samples_m, 0)  # (iters, C, 6)
+
+    log("[bench] mcmc RWM 1 chain, %.0f s" % args.mcmc_budget)
+    out_r, acc_r, logf = SM.rwm(post_1, start[0], 200, scale, rng, adapt_until=200)
+    samples_r, accs_r = [], []
+    th1 = out_r[-1]
+    t3 = time.perf_counter()
+    while time.perf_counter() - t3 < args.mcmc_budget:
+        o, a, logf = SM.rwm(post_1, th1, 50, scale, rng, logf=logf)
+        samples_r.append(o)
+        accs_r.append(a)
+        th1 = o[-1]
+    wall_r = time.perf_counter() - t3
+    xr = np.concatenate(samples_r, 0)
+
+    ess_m = min(sum(SM.ess_batch_means(xm[:, k, p]) for k in range(C)) for p in range(6))
+    ess_r = min(SM.ess_batch_means(xr[:, p]) for p in range(6))
+    mean_m = xm.reshape(-1, 6).mean(0)
+    mean_r = xr.mean(0)
+    sd_r = xr.std(0)
+    rate_m = ess_m / wall_m * 3600
+    rate_r = ess_r / wall_r * 3600
+    res = {
+        "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
+        "value": rate_m, "unit": "effective posterior samples/hour", "n_gpus": 1, "steps": steps, "warmup": args.mcmc_pilot,
+        "ms_per_step": 1e3 * wall_m / max(1, xm.shape[0]) / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "MALA (%d chains, adjoint gradients) vs plain RWM (one forward solve per proposal), "
+                               "uniform %d^3, %d steps, flip-rate likelihood, 6 parameters" % (C, n, steps),
+                   "seed": args.seed},
+        "mcmc": {"mala": {"ess_per_hour": rate_m, "ess": ess_m, "iterations": int(xm.shape[0]), "wall_s": wall_m,
+                          "accept": float(np.concatenate(accs_m).mean())},
+                 "rwm": {"ess_per_hour": rate_r, "ess": ess_r, "iterations": int(xr.shape[0]), "wall_s": wall_r,
+                         "accept": float(np.concatenate(accs_r).mean())},
+                 "speedup": rate_m / max(rate_r, 1e-30),
+                 "posterior_mean_mala": mean_m.tolist(), "posterior_mean_rwm": mean_r.tolist(),
+                 "mean_dev_over_rwm_sd": (np.abs(mean_m - mean_r) / np.maximum(sd_r, 1e-30)).tolist(),
+                 "theta_true": theta_true.tolist(), "preconditioner": scale.tolist()},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,7 +301,11 @@ def main():
     ap.add_argument("--adapt-every", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="adaptive", choices=["adaptive", "ensemble", "adjoint"])
+    ap.add_argument("--workload", default="adaptive", choices=["adaptive", "ensemble", "adjoint", "mcmc"])
+    ap.add_argument("--mcmc-n", type=int, default=64)
+    ap.add_argument("--mcmc-steps", type=int, default=500)
+    ap.add_argument("--mcmc-pilot", type=int, default=40)
+    ap.add_argument("--mcmc-budget", type=float, default=60.0, help="wall seconds per sampler")
     ap.add_argument("--chains", type=int, default=16, help="adjoint chains")
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--steps-adj-reps", type=int, default=2)
@@ -220,6 +320,8 @@ def main():
         return ensemble_workload(args)
     if args.workload == "adjoint":
         return adjoint_workload(args)
+    if args.workload == "mcmc":
+        return mcmc_workload(args)
 
     import gliomauq_amd as gq
     from gliomauq_amd import anatomy as A

@@ -719,7 +719,7 @@ __device__ __forceinline__ double fwt_filter(const double *lab, int dx, int dy, 
 // z, y, x loop order, FP64 without contraction; block detail = max over
 // codes 1..7 of |value|.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const double *__restrict__ ghost,
+__global__ __launch_bounds__(512) void k_fwt_detail(LabArgs a, const double *__restrict__ ghost,
                                                     const float *__restrict__ phi, const int *__restrict__ cand,
                                                     double *__restrict__ out)
 {
@@ -727,12 +727,13 @@ __global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const double *__r
     constexpr int S1 = ORDER == 4 ? 4 : 2;
     constexpr int L = LabGeom<S0, S1>::L;
     __shared__ double lab[LabGeom<S0, S1>::N];
-    __shared__ double red[256];
+    __shared__ double red[8];
     const int slot = cand[blockIdx.x];
     fill_lab<S0, S1>(lab, phi, ghost + (size_t)blockIdx.x * LabGeom<S0, S1>::N, slot, a);
     __syncthreads();
     double m = 0.0;
-    for (int q = threadIdx.x; q < 512; q += blockDim.x) {
+    {
+        const int q = threadIdx.x;  // one coarse point per thread
         const int dx = q & 7, dy = (q >> 3) & 7, dz = q >> 6;
         double v[7];
         v[0] = fwt_filter<ORDER, L, S0, true, false, false>(lab, dx, dy, dz);
@@ -748,13 +749,17 @@ __global__ __launch_bounds__(256) void k_fwt_detail(LabArgs a, const double *__r
             m = m > av ? m : av;
         }
     }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + s] ? red[threadIdx.x] : red[threadIdx.x + s];
-        __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) {
+        const double u = __shfl_down(m, o, 64);
+        m = m > u ? m : u;
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int k = 1; k < 8; k++) r = r > red[k] ? r : red[k];
+        out[blockIdx.x] = r;
+    }
 }
 
 // BlockSplitter::_splitBlocks (MRAGBlockSplitter.h:24-99) as a gather: child
@@ -1082,8 +1087,8 @@ hipError_t launch_fwt_detail(int order, const LabArgs &a, const double *ghost, c
                              int ncand, double *out, hipStream_t st)
 {
     if (ncand <= 0) return hipSuccess;
-    if (order == 4) hipLaunchKernelGGL(k_fwt_detail<4>, dim3(ncand), dim3(256), 0, st, a, ghost, phi, cand, out);
-    else hipLaunchKernelGGL(k_fwt_detail<2>, dim3(ncand), dim3(256), 0, st, a, ghost, phi, cand, out);
+    if (order == 4) hipLaunchKernelGGL(k_fwt_detail<4>, dim3(ncand), dim3(512), 0, st, a, ghost, phi, cand, out);
+    else hipLaunchKernelGGL(k_fwt_detail<2>, dim3(ncand), dim3(512), 0, st, a, ghost, phi, cand, out);
     return hipGetLastError();
 }
 

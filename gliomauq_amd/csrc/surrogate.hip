@@ -1,0 +1,478 @@
+// surrogate.hip -- kernels of the 3D convolutional surrogate (no reference
+// counterpart: the reference has no learned model).
+//
+// Activations are channels-last bf16, [batch][z][y][x][channel]; a tensor
+// view is (pointer, edge D, channel stride ld, channel offset off), so a
+// concatenation of two tensors is one buffer written at two offsets.
+// 3^3 convolutions are implicit GEMMs on the bf16 MFMA of gfx950
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation): M = voxels, N = output
+// channels, K = 27 taps x input channels, k = tap * Cin + ci with
+// tap = (dz+1)*9 + (dy+1)*3 + (dx+1), zero padding outside the volume.
+// Weight gradients accumulate in FP32 on the vector ALUs.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "surrogate.hpp"
+
+namespace gq {
+namespace sur {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
+
+__device__ __forceinline__ bf16x8 load8(const __bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+__device__ __forceinline__ bf16x8 zero8()
+{
+    bf16x8 z;
+#pragma unroll
+    for (int j = 0; j < 8; j++) z[j] = (__bf16)0.0f;
+    return z;
+}
+
+}  // namespace
+
+// One wave: 16 consecutive voxels x NT tiles of 16 output channels.  Lane l
+// supplies A rows (voxel l%16, k = k0 + 8*(l/16) .. +7) and B columns
+// (channel l%16 of tile t, same k); it receives D rows 4*(l/16)+r, column l%16.
+template <int NT, bool OUT_BF16>
+__global__ __launch_bounds__(256) void k_conv3(ConvArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t D = a.in.D, D3 = D * D * D;
+    const int64_t nvox = D3 * a.batch;
+    const int64_t vbase = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    if (vbase >= nvox) return;
+    const int i = lane & 15, kq = lane >> 4;
+    // voxel of this lane's A row
+    const int64_t v = vbase + i;
+    const bool vok = v < nvox;
+    const int64_t vb = vok ? v / D3 : 0, vr = vok ? v - vb * D3 : 0;
+    const int x = (int)(vr % D), y = (int)((vr / D) % D), z = (int)(vr / (D * D));
+    const int Cin = a.cin, K = 27 * Cin;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const int k = k0 + 8 * kq;
+        bf16x8 va = zero8();
+        if (k < K && vok) {
+            const int tap = k / Cin, ci = k - tap * Cin;
+            const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+            const int xx = x + dx, yy = y + dy, zz = z + dz;
+            if (xx >= 0 && xx < D && yy >= 0 && yy < D && zz >= 0 && zz < D) {
+                const int64_t vn = vb * D3 + xx + D * (yy + D * (int64_t)zz);
+                va = load8(a.in.p + vn * a.in.ld + a.in.off + ci);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int co = 16 * t + i;
+            bf16x8 vw = zero8();
+            if (k < K && co < a.cout) vw = load8(a.w + (int64_t)co * K + k);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vw, acc[t], 0, 0, 0);
+        }
+    }
+    // epilogue: D row 4*kq + r = voxel, column i = channel within the tile
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int co = 16 * t + i;
+        if (co >= a.cout) continue;
+        const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t vo = vbase + 4 * kq + r;
+            if (vo >= nvox) continue;
+            float val = acc[t][r] + b;
+            if (a.relu) val = val > 0.f ? val : 0.f;
+            const int64_t o = vo * a.out_ld + a.out_off + co;
+            if (OUT_BF16) a.out_bf[o] = (__bf16)val;
+            else a.out_f[o] = a.accumulate ? a.out_f[o] + val : val;
+        }
+    }
+}
+
+// Weight gradient of a 3^3 convolution: dW[co][tap*Cin + ci] (FP32) summed
+// over the voxels of the batch of dZ[v][co] * X[v + tap][ci].  Workgroup
+// (tap, chunk): the chunk's voxels are staged 64 at a time in LDS; each
+// thread owns 4x4 (co, ci) pairs.  Partial sums per chunk, reduced in chunk
+// order by k_reduce_chunks (deterministic).
+__global__ __launch_bounds__(256) void k_conv3_wgrad(WgradArgs a)
+{
+    constexpr int VT = 64;
+    __shared__ float s_x[VT * 64];   // [voxel][ci] of one 64-channel ci block
+    __shared__ float s_dz[VT * 64];  // [voxel][co] of one 64-channel co block
+    const int tap = blockIdx.x;
+    const int chunk = blockIdx.y;
+    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+    const int64_t D = a.x.D, D3 = D * D * D, nvox = D3 * a.batch;
+    const int64_t v0 = (int64_t)chunk * a.chunk_vox, v1 = v0 + a.chunk_vox < nvox ? v0 + a.chunk_vox : nvox;
+    const int Cin = a.cin, Cout = a.cout;
+    const int t = threadIdx.x;
+    for (int cob = 0; cob < Cout; cob += 64) {
+        const int ncob = Cout - cob < 64 ? Cout - cob : 64;
+        for (int cib = 0; cib < Cin; cib += 64) {
+            const int ncib = Cin - cib < 64 ? Cin - cib : 64;
+            // thread tile: co = cob + 4*(t/16) + 0..3, ci = cib + 4*(t%16) + 0..3
+            const int tco = 4 * (t >> 4), tci = 4 * (t & 15);
+            float acc[4][4];
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[p][q] = 0.f;
+            for (int64_t vs = v0; vs < v1; vs += VT) {
+                __syncthreads();
+                for (int e = t; e < VT * 64; e += 256) {
+                    const int vi = e >> 6, c = e & 63;
+                    const int64_t v = vs + vi;
+                    float xv = 0.f, zv = 0.f;
+                    if (v < v1) {
+                        const int64_t b = v / D3, r = v - b * D3;
+                        const int x = (int)(r % D), y = (int)((r / D) % D), z = (int)(r / (D * D));
+                        const int xx = x + dx, yy = y + dy, zz = z + dz;
+                        if (c < ncib && xx >= 0 && xx < D && yy >= 0 && yy < D && zz >= 0 && zz < D)
+                            xv = bf2f(a.x.p[(b * D3 + xx + D * (yy + D * (int64_t)zz)) * a.x.ld + a.x.off + cib + c]);
+                        if (c < ncob) zv = bf2f(a.dz.p[v * a.dz.ld + a.dz.off + cob + c]);
+                    }
+                    s_x[e] = xv;
+                    s_dz[e] = zv;
+                }
+                __syncthreads();
+                for (int vi = 0; vi < VT; vi++) {
+                    float xr[4], zr[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) xr[q] = s_x[vi * 64 + tci + q];
+#pragma unroll
+                    for (int p = 0; p < 4; p++) zr[p] = s_dz[vi * 64 + tco + p];
+#pragma unroll
+                    for (int p = 0; p < 4; p++)
+#pragma unroll
+                        for (int q = 0; q < 4; q++) acc[p][q] += zr[p] * xr[q];
+                }
+            }
+            const int K = 27 * Cin;
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int co = tco + p, ci = tci + q;
+                    if (co < ncob && ci < ncib)
+                        a.partial[(int64_t)chunk * Cout * K + (int64_t)(cob + co) * K + tap * Cin + cib + ci] =
+                            acc[p][q];
+                }
+        }
+    }
+}
+
+// Bias gradient partials: db[co] over the chunk's voxels.
+__global__ __launch_bounds__(256) void k_bias_grad(BiasGradArgs a)
+{
+    const int chunk = blockIdx.x;
+    const int64_t D3 = (int64_t)a.dz.D * a.dz.D * a.dz.D, nvox = D3 * a.batch;
+    const int64_t v0 = (int64_t)chunk * a.chunk_vox, v1 = v0 + a.chunk_vox < nvox ? v0 + a.chunk_vox : nvox;
+    for (int co = threadIdx.x; co < a.cout; co += 256) {
+        float s = 0.f;
+        for (int64_t v = v0; v < v1; v++) s += bf2f(a.dz.p[v * a.dz.ld + a.dz.off + co]);
+        a.partial[(int64_t)chunk * a.cout + co] = s;
+    }
+}
+
+// out[i] = sum over chunks c (in order) of partial[c*n + i]
+__global__ __launch_bounds__(256) void k_reduce_chunks(const float *partial, int nchunks, int64_t n, float *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunks; c++) s += partial[(int64_t)c * n + i];
+    out[i] = s;
+}
+
+// 2x2x2 average pooling (forward) and its adjoint (accumulated into dx).
+__global__ __launch_bounds__(256) void k_avgpool(View in, View out, int C, int batch)
+{
+    const int64_t Do = out.D, n = Do * Do * Do * batch * C;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int c = (int)(e % C);
+    const int64_t v = e / C, D3o = Do * Do * Do, b = v / D3o, r = v - b * D3o;
+    const int x = (int)(r % Do), y = (int)((r / Do) % Do), z = (int)(r / (Do * Do));
+    const int64_t Di = in.D, D3i = Di * Di * Di;
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) {
+        const int64_t vi = b * D3i + (2 * x + (k & 1)) + Di * ((2 * y + ((k >> 1) & 1)) + Di * (int64_t)(2 * z + (k >> 2)));
+        s += bf2f(in.p[vi * in.ld + in.off + c]);
+    }
+    out.p[v * out.ld + out.off + c] = (__bf16)(0.125f * s);
+}
+
+__global__ __launch_bounds__(256) void k_avgpool_bwd(const float *dout, int64_t dout_ld, FView din, int C, int batch)
+{
+    // din at the fine level; each fine voxel receives dout(parent) / 8
+    const int64_t Di = din.D, n = Di * Di * Di * batch * C;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int c = (int)(e % C);
+    const int64_t v = e / C, D3i = Di * Di * Di, b = v / D3i, r = v - b * D3i;
+    const int x = (int)(r % Di), y = (int)((r / Di) % Di), z = (int)(r / (Di * Di));
+    const int64_t Do = Di / 2;
+    const int64_t vo = b * Do * Do * Do + (x >> 1) + Do * ((y >> 1) + Do * (int64_t)(z >> 1));
+    din.p[v * din.ld + din.off + c] += 0.125f * dout[vo * dout_ld + c];
+}
+
+// Nearest-neighbour 2x upsampling (forward) and its adjoint (sum of 8 children).
+__global__ __launch_bounds__(256) void k_upsample(View in, View out, int C, int batch)
+{
+    const int64_t Do = out.D, n = Do * Do * Do * batch * C;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int c = (int)(e % C);
+    const int64_t v = e / C, D3o = Do * Do * Do, b = v / D3o, r = v - b * D3o;
+    const int x = (int)(r % Do), y = (int)((r / Do) % Do), z = (int)(r / (Do * Do));
+    const int64_t Di = in.D;
+    const int64_t vi = b * Di * Di * Di + (x >> 1) + Di * ((y >> 1) + Di * (int64_t)(z >> 1));
+    out.p[v * out.ld + out.off + c] = in.p[vi * in.ld + in.off + c];
+}
+
+__global__ __launch_bounds__(256) void k_upsample_bwd(FView dout, float *din, int64_t din_ld, int C, int batch)
+{
+    const int64_t Di = dout.D / 2, n = Di * Di * Di * batch * C;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int c = (int)(e % C);
+    const int64_t v = e / C, D3i = Di * Di * Di, b = v / D3i, r = v - b * D3i;
+    const int x = (int)(r % Di), y = (int)((r / Di) % Di), z = (int)(r / (Di * Di));
+    const int64_t Do = dout.D, D3o = Do * Do * Do;
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) {
+        const int64_t vo = b * D3o + (2 * x + (k & 1)) + Do * ((2 * y + ((k >> 1) & 1)) + Do * (int64_t)(2 * z + (k >> 2)));
+        s += dout.p[vo * dout.ld + dout.off + c];
+    }
+    din[v * din_ld + c] = s;
+}
+
+// dZ = dY * [Y > 0] -> bf16 (ReLU backward through the post-activation value)
+__global__ __launch_bounds__(256) void k_relu_bwd(FView dy, View y, __bf16 *dz, int C, int64_t nvox)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nvox * C) return;
+    const int c = (int)(e % C);
+    const int64_t v = e / C;
+    const float g = dy.p[v * dy.ld + dy.off + c];
+    const float yv = bf2f(y.p[v * y.ld + y.off + c]);
+    dz[v * C + c] = (__bf16)(yv > 0.f ? g : 0.f);
+}
+
+// Output head: p = sigmoid(sum_c w[c] h[c] + b); loss = mean (p - t)^2 +
+// lam * mean BCE(sigmoid(kappa (p - th)), [t > th]); writes p and, when dh is
+// set, dL/dh (FP32) and per-block partials of the loss and of dL/dw, dL/db.
+__global__ __launch_bounds__(256) void k_head(HeadArgs a)
+{
+    __shared__ float red[4][20];
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool ok = v < a.nvox;
+    float h[16];
+    float zl = a.b;
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        h[c] = ok ? bf2f(a.h.p[v * a.h.ld + a.h.off + c]) : 0.f;
+        zl += a.w[c] * h[c];
+    }
+    const float p = 1.f / (1.f + __expf(-zl));
+    float part[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) part[k] = 0.f;
+    if (ok) {
+        a.pred[v] = p;
+        if (a.target) {
+            const float tv = a.target[v];
+            const float m = tv > a.th ? 1.f : 0.f;
+            const float s = 1.f / (1.f + __expf(-a.kappa * (p - a.th)));
+            const float sc = fminf(fmaxf(s, 1e-7f), 1.f - 1e-7f);
+            const float bce = -(m * __logf(sc) + (1.f - m) * __logf(1.f - sc));
+            const float invn = 1.f / (float)a.nvox;
+            part[16] = ((p - tv) * (p - tv) + a.lam * bce) * invn;
+            const float dldp = (2.f * (p - tv) + a.lam * (s - m) * a.kappa) * invn;
+            const float dz = dldp * p * (1.f - p);
+            if (a.dh) {
+#pragma unroll
+                for (int c = 0; c < 16; c++) {
+                    a.dh[v * 16 + c] = dz * a.w[c];
+                    part[c] = dz * h[c];
+                }
+            }
+            part[17] = dz;
+        }
+    }
+    if (!a.partial) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 18; k++) {
+        float s = part[k];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (lane == 0) red[wv][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 18)
+        a.partial[(int64_t)blockIdx.x * 18 + threadIdx.x] =
+            ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+
+// Input assembly: channels {params[0..3] broadcast, u0, pw, pg, 1}.
+__global__ __launch_bounds__(256) void k_assemble_input(const float *params, const float *u0, const float *pw,
+                                                        const float *pg, int64_t D3, int batch, __bf16 *x)
+{
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= D3 * batch) return;
+    const int64_t b = v / D3, r = v - b * D3;
+    __bf16 *o = x + v * 8;
+    for (int c = 0; c < 4; c++) o[c] = (__bf16)params[b * 4 + c];
+    o[4] = (__bf16)u0[v];
+    o[5] = (__bf16)pw[r];
+    o[6] = (__bf16)pg[r];
+    o[7] = (__bf16)1.0f;
+}
+
+// Adam step on FP32 master weights; refreshes the bf16 forward copy.
+__global__ __launch_bounds__(256) void k_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 copies of a conv's weights: forward layout [co][tap*Cin+ci] and the
+// flipped, transposed layout of the data gradient [ci][tap'*Cout+co] with
+// tap' = 26 - tap.
+__global__ __launch_bounds__(256) void k_pack_weights(const float *w, int cin, int cout, __bf16 *wf, __bf16 *wt)
+{
+    const int64_t K = 27 * (int64_t)cin, n = (int64_t)cout * K;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int co = (int)(i / K), k = (int)(i % K), tap = k / cin, ci = k % cin;
+    const __bf16 b = (__bf16)w[i];
+    wf[i] = b;
+    wt[(int64_t)ci * 27 * cout + (int64_t)(26 - tap) * cout + co] = b;
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_conv3(const ConvArgs &a, hipStream_t st)
+{
+    const int64_t nvox = (int64_t)a.in.D * a.in.D * a.in.D * a.batch;
+    const dim3 grid((unsigned)((nvox + 63) / 64));
+    const int nt = (a.cout + 15) / 16;
+    if (a.cin % 8 != 0 || a.in.ld % 8 != 0 || a.in.off % 8 != 0 || nt > 12) return hipErrorInvalidValue;
+    const bool bf = a.out_bf != nullptr;
+#define GQ_CONV(N)                                                                  \
+    if (bf) hipLaunchKernelGGL((k_conv3<N, true>), grid, dim3(256), 0, st, a);      \
+    else hipLaunchKernelGGL((k_conv3<N, false>), grid, dim3(256), 0, st, a);
+    switch (nt) {
+    case 1: GQ_CONV(1) break;
+    case 2: GQ_CONV(2) break;
+    case 3: GQ_CONV(3) break;
+    case 4: GQ_CONV(4) break;
+    case 6: GQ_CONV(6) break;
+    case 8: GQ_CONV(8) break;
+    case 12: GQ_CONV(12) break;
+    default: return hipErrorInvalidValue;
+    }
+#undef GQ_CONV
+    return hipGetLastError();
+}
+
+hipError_t launch_conv3_wgrad(const WgradArgs &a, int nchunks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_conv3_wgrad, dim3(27, nchunks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bias_grad(const BiasGradArgs &a, int nchunks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bias_grad, dim3(nchunks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_chunks(const float *partial, int nchunks, int64_t n, float *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial, nchunks, n,
+                       out);
+    return hipGetLastError();
+}
+
+static dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+hipError_t launch_avgpool(const View &in, const View &out, int C, int batch, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_avgpool, g1((int64_t)out.D * out.D * out.D * batch * C), dim3(256), 0, st, in, out, C,
+                       batch);
+    return hipGetLastError();
+}
+
+hipError_t launch_avgpool_bwd(const float *dout, int64_t dout_ld, const FView &din, int C, int batch, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_avgpool_bwd, g1((int64_t)din.D * din.D * din.D * batch * C), dim3(256), 0, st, dout,
+                       dout_ld, din, C, batch);
+    return hipGetLastError();
+}
+
+hipError_t launch_upsample(const View &in, const View &out, int C, int batch, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_upsample, g1((int64_t)out.D * out.D * out.D * batch * C), dim3(256), 0, st, in, out, C,
+                       batch);
+    return hipGetLastError();
+}
+
+hipError_t launch_upsample_bwd(const FView &dout, float *din, int64_t din_ld, int C, int batch, hipStream_t st)
+{
+    const int64_t Di = dout.D / 2;
+    hipLaunchKernelGGL(k_upsample_bwd, g1(Di * Di * Di * batch * C), dim3(256), 0, st, dout, din, din_ld, C, batch);
+    return hipGetLastError();
+}
+
+hipError_t launch_relu_bwd(const FView &dy, const View &y, __bf16 *dz, int C, int64_t nvox, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_relu_bwd, g1(nvox * C), dim3(256), 0, st, dy, y, dz, C, nvox);
+    return hipGetLastError();
+}
+
+hipError_t launch_head(const HeadArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_head, g1(a.nvox), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+int head_blocks(int64_t nvox) { return (int)((nvox + 255) / 256); }
+
+hipError_t launch_assemble_input(const float *params, const float *u0, const float *pw, const float *pg, int64_t D3,
+                                 int batch, __bf16 *x, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_assemble_input, g1(D3 * batch), dim3(256), 0, st, params, u0, pw, pg, D3, batch, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adam, g1(n), dim3(256), 0, st, w, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_weights(const float *w, int cin, int cout, __bf16 *wf, __bf16 *wt, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_weights, g1((int64_t)cout * 27 * cin), dim3(256), 0, st, w, cin, cout, wf, wt);
+    return hipGetLastError();
+}
+
+}  // namespace sur
+}  // namespace gq

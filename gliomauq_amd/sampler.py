@@ -84,13 +84,36 @@ class Posterior:
         return np.where(inside, ll[:m, 0], -np.inf)
 
 
-def rwm(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, callback=None):
+def probe_scales(post, theta, drop=2.0, lo=1e-7, hi=1.0, n=16):
+    """Per parameter k, the relative offset d (geometric grid in [lo, hi] of
+    the parameter's box width) where logpost(theta + d e_k) first falls `drop`
+    below logpost(theta); parameter sets are evaluated in batches."""
+    theta = np.asarray(theta, np.float64)
+    width = post.hi - post.lo
+    grid = np.geomspace(lo, hi, n)
+    rows = [theta]
+    for k in range(6):
+        for g in grid:
+            t = theta.copy()
+            t[k] += g * width[k]
+            rows.append(t)
+    lp = _batched(post.logpost, np.array(rows), post.batch)
+    base = lp[0]
+    out = np.zeros(6)
+    for k in range(6):
+        seg = lp[1 + k * n:1 + (k + 1) * n]
+        j = int(np.argmax(seg < base - drop)) if np.any(seg < base - drop) else n - 1
+        out[k] = grid[j] * width[k]
+    return out
+
+
+def rwm(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, callback=None, logf=0.0):
     """Random-walk Metropolis, one chain, one forward solve per proposal.
-    Gaussian proposals N(0, diag(s^2)); the global factor adapts towards
-    `target` acceptance for the first `adapt_until` iterations."""
+    Gaussian proposals N(0, diag((f s)^2)); log f starts at `logf` and adapts
+    towards `target` acceptance for the first `adapt_until` iterations.
+    Returns (samples, accepted, final log f)."""
     th = np.asarray(theta0, np.float64).copy()
     lp = post.logpost(th[None])[0]
-    logf = 0.0
     out = np.zeros((n_iter, 6))
     acc = np.zeros(n_iter, bool)
     for it in range(n_iter):
@@ -104,18 +127,19 @@ def rwm(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, callback=
         out[it] = th
         if callback:
             callback(it, th, lp)
-    return out, acc
+    return out, acc, logf
 
 
-def mala(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.57, callback=None):
+def mala(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.57, callback=None, logeps=None):
     """Metropolis-adjusted Langevin for a population of chains (theta0: (C, 6)),
-    preconditioner diag(scale^2), per-chain step size eps adapted towards
-    `target` acceptance for the first `adapt_until` iterations."""
+    preconditioner diag(scale^2), per-chain step sizes exp(logeps) (default
+    0.5) adapted towards `target` acceptance for the first `adapt_until`
+    iterations.  Returns (samples, accepted, final logeps)."""
     th = np.asarray(theta0, np.float64).copy()
     C = th.shape[0]
     M = np.asarray(scale, np.float64) ** 2
     lp, g = post.logpost_grad(th)
-    logeps = np.full(C, math.log(0.5))
+    logeps = np.full(C, math.log(0.5)) if logeps is None else np.array(logeps, np.float64)
     out = np.zeros((n_iter, C, 6))
     acc = np.zeros((n_iter, C), bool)
     for it in range(n_iter):
@@ -137,7 +161,7 @@ def mala(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.57, callback=
         out[it] = th
         if callback:
             callback(it, th, lp)
-    return out, acc
+    return out, acc, logeps
 
 
 def ess_batch_means(x, n_batches=None):
@@ -151,7 +175,7 @@ def ess_batch_means(x, n_batches=None):
     xb = x[: m * b].reshape(b, m).mean(1)
     var = x.var(ddof=1)
     if var == 0:
-        return float(n)
+        return 1.0  # a chain that never moved carries one sample's information
     bvar = m * xb.var(ddof=1)
     return float(min(n, n * var / bvar)) if bvar > 0 else float(n)
 

@@ -64,6 +64,14 @@ class Context:
     def call(self, name, *args):
         _check(self.L, self.h, getattr(self.L, name)(self.h, *args))
 
+    # a context holds one grid or ensemble at a time; creating another one
+    # replaces it, and the older Python object must not address the new one
+    owner = None
+
+    def check_owner(self, obj):
+        if self.owner is not obj:
+            raise RuntimeError("this context now holds another grid/ensemble: create one context per grid")
+
 
 class Grid:
     """Grid<W,B>(bpd,bpd,bpd) with Refiner_SpaceExtension(1, maxLevel) and
@@ -77,6 +85,7 @@ class Grid:
         self.max_level = max_level
         self.order = wavelet_order
         ctx.call("gq_grid_create", blocks_per_dim, max_level, wavelet_order)
+        ctx.owner = self
 
     # -- topology -------------------------------------------------------
     def n_blocks(self):
@@ -384,6 +393,7 @@ class Ensemble:
         self.max_level = max_level
         self.order = wavelet_order
         ctx.call("gq_ensemble_create", nsamples, blocks_per_dim, max_level, wavelet_order)
+        ctx.owner = self
 
     def sample(self, s):
         """Grid view of sample s (Grid methods address that sample)."""

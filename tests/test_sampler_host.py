@@ -39,7 +39,7 @@ def test_mala_targets_gaussian():
     rng = np.random.default_rng(0)
     post = Gauss(MU, SD)
     th0 = MU + SD * rng.standard_normal((16, 6))
-    out, acc = SM.mala(post, th0, 3000, SD, rng, adapt_until=500)
+    out, acc, _ = SM.mala(post, th0, 3000, SD, rng, adapt_until=500)
     x = out[500:].reshape(-1, 6)
     a = acc[500:].mean()
     assert 0.45 < a < 0.7
@@ -50,7 +50,7 @@ def test_mala_targets_gaussian():
 def test_rwm_targets_gaussian():
     rng = np.random.default_rng(1)
     post = Gauss(MU, SD)
-    out, acc = SM.rwm(post, MU, 40000, SD, rng, adapt_until=4000)
+    out, acc, _ = SM.rwm(post, MU, 40000, SD, rng, adapt_until=4000)
     x = out[4000:]
     assert 0.15 < acc[4000:].mean() < 0.35
     assert np.all(np.abs(x.mean(0) - MU) < 0.1 * SD)
