@@ -27,6 +27,9 @@ EXPORTS = [
     "gq_grid_write_volume", "gq_ensemble_create", "gq_ensemble_size", "gq_sample_select",
     "gq_ensemble_block_counts", "gq_ensemble_run", "gq_ensemble_stats", "gq_ensemble_loglik_flip",
     "gq_ensemble_ic_gaussian", "gq_adjoint_flip",
+    "gq_sur_create", "gq_sur_destroy", "gq_sur_last_error", "gq_sur_set_anatomy", "gq_sur_param_count",
+    "gq_sur_get_params", "gq_sur_set_params", "gq_sur_predict", "gq_sur_loss_grad", "gq_sur_train_step",
+    "gq_sur_timing",
 ]
 
 RUN_GRAPH = 1

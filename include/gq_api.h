@@ -144,6 +144,34 @@ gq_status gq_ensemble_ic_gaussian(gq_ctx *ctx, const double *ic);
 gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int ckpt_every, const double lik[4],
                           const double *ic, double *out_ll, double *out_grad);
 
+/* ---- learned surrogate (no reference counterpart): a 3D convolutional
+ * encoder-decoder on 64^3 that maps {4 parameters broadcast as channels,
+ * the initial condition u0, pw, pg} to the final tumour field, trained and
+ * evaluated by bf16 MFMA kernels with FP32 accumulation (FP32 master
+ * weights, Adam).  Fields are 64^3 floats, x fastest; params are 4 floats
+ * per item.  Loss: mean (p - t)^2 + 0.1 mean BCE(sigmoid(20 (p - 0.25)),
+ * [t > 0.25]).  One context per surrogate; batch <= the capacity given at
+ * creation (<= 64). */
+typedef struct gq_sur gq_sur;
+gq_status gq_sur_create(gq_sur **out, int device, int batch_capacity, uint64_t seed);
+gq_status gq_sur_destroy(gq_sur *s);
+const char *gq_sur_last_error(const gq_sur *s);
+gq_status gq_sur_set_anatomy(gq_sur *s, const float *pw, const float *pg);
+gq_status gq_sur_param_count(gq_sur *s, int64_t *n);
+/* all parameters: per conv (E1..E4, D3, D2, D1) weights [cout][27*cin] then
+ * bias [cout]; head weights [16] and bias */
+gq_status gq_sur_get_params(gq_sur *s, float *out);
+gq_status gq_sur_set_params(gq_sur *s, const float *in);
+gq_status gq_sur_predict(gq_sur *s, const float *params, const float *u0, int batch, float *out);
+/* loss and its gradient w.r.t. every parameter (gq_sur_get_params layout), no update */
+gq_status gq_sur_loss_grad(gq_sur *s, const float *params, const float *u0, const float *target, int batch,
+                           double *loss, float *grads);
+/* one Adam step (lr) on a batch; loss before the step */
+gq_status gq_sur_train_step(gq_sur *s, const float *params, const float *u0, const float *target, int batch, float lr,
+                            double *loss);
+/* device ms of the last forward and backward pass */
+gq_status gq_sur_timing(gq_sur *s, double out[2]);
+
 /* ---- forward solve: replaces Glioma_HG_UQ::_reactionDiffusionStep
  * (Glioma/Glioma_HG_UQ.cpp:266-277) = BlockProcessing::pipeline_process(vInfo,
  * collection, boundaryInfo, Glioma_ReactionDiffusionOperator) followed by

@@ -1,0 +1,88 @@
+"""GPU surrogate (gq_sur_*): bf16 MFMA forward/backward against the CPU
+oracle (oracle/surrogate_oracle.py, pinned by finite differences in
+test_oracle_surrogate.py) on identical parameters and inputs.
+
+Bars: the oracle runs with the same bf16 rounding points in float64, so the
+remaining differences are FP32 accumulation order and the occasional
+one-ulp bf16 flip it causes: prediction relative L2 <= 1e-2; every layer's
+gradient relative L2 <= 3e-2 and cosine >= 0.999; the loss within 1e-3
+relative.  Training (Adam) on a fixed batch lowers the loss; results are
+bit-reproducible.
+"""
+import numpy as np
+import pytest
+
+from gliomauq_amd import surrogate as S
+from oracle import surrogate_oracle as SO
+
+pytestmark = pytest.mark.gpu
+
+
+def batch(seed=0, B=1):
+    rng = np.random.default_rng(seed)
+    n = 64
+    z, y, x = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    pw = (0.5 + 0.4 * np.sin(x / 7.0) * np.cos(y / 5.0)).astype(np.float32)
+    pg = (1 - pw).astype(np.float32)
+    params = rng.uniform(-1, 1, (B, 4)).astype(np.float32)
+    u0 = np.zeros((B, n, n, n), np.float32)
+    tgt = np.zeros((B, n, n, n), np.float32)
+    for b in range(B):
+        c = rng.uniform(20, 44, 3)
+        r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
+        u0[b] = 0.1 * np.exp(-r2 / 8.0)
+        tgt[b] = 1.0 / (1.0 + np.exp((np.sqrt(r2) - 9.0 - 3 * params[b, 0]) / 1.5))
+    return params, u0, pw, pg, tgt
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_surrogate_matches_oracle():
+    params, u0, pw, pg, tgt = batch()
+    s = S.Surrogate(batch=2, seed=3)
+    s.set_anatomy(pw, pg)
+    theta = s.get_params().astype(np.float64)
+    pred = s.predict(params, u0)
+    loss, g = s.loss_grad(params, u0, tgt)
+    l_ref, p_ref, g_ref = SO.forward_backward(theta, params.astype(np.float64), u0.astype(np.float64),
+                                             pw.astype(np.float64), pg.astype(np.float64), tgt.astype(np.float64))
+    e_p = rel(pred, p_ref)
+    print("prediction rel L2 %.3e, loss gpu %.6e oracle %.6e" % (e_p, loss, l_ref))
+    assert e_p <= 1e-2
+    assert abs(loss - l_ref) <= 1e-3 * abs(l_ref)
+    o = 0
+    for i, (cin, cout) in enumerate(SO.SPEC + [(16, 1)]):
+        n = cout * 27 * cin + cout if i < 7 else 17
+        a, b = g[o:o + n].astype(np.float64), g_ref[o:o + n]
+        cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-300))
+        print("layer %d grad rel L2 %.3e cos %.6f" % (i, rel(a, b), cos))
+        assert rel(a, b) <= 3e-2 and cos >= 0.999
+        o += n
+    s.close()
+
+
+def test_surrogate_training_and_reproducibility():
+    params, u0, pw, pg, tgt = batch(seed=1, B=2)
+    s = S.Surrogate(batch=2, seed=4)
+    s.set_anatomy(pw, pg)
+    p1 = s.predict(params, u0)
+    p2 = s.predict(params, u0)
+    assert np.array_equal(p1, p2)
+    losses = [s.train_step(params, u0, tgt, 2e-3) for _ in range(40)]
+    print("loss %.4e -> %.4e" % (losses[0], losses[-1]))
+    assert losses[-1] < 0.5 * losses[0]
+    t = s.timing()
+    assert t["forward_ms"] > 0 and t["backward_ms"] > 0
+    s.close()
+
+
+def test_surrogate_rejects_bad_batch():
+    s = S.Surrogate(batch=1, seed=0)
+    params, u0, pw, pg, tgt = batch(B=2)
+    s.set_anatomy(pw, pg)
+    with pytest.raises(S.GQError) as e:
+        s.train_step(params, u0, tgt, 1e-3)
+    assert e.value.status == 1
+    s.close()
