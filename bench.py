@@ -292,6 +292,197 @@ def mcmc_workload(args):
     print(json.dumps(res), flush=True)
 
 
+def surrogate_workload(args):
+    """Learned surrogate + delayed acceptance.  Full model: uniform 128^3
+    grid, --sur-steps steps of a fixed dt, Gaussian IC, flip-rate likelihood
+    against observations on the 64^3 grid (every other voxel).  Training
+    fields: the batched solver's final states at parameters drawn around the
+    truth, downsampled to 64^3.  The surrogate is trained by the bf16 MFMA
+    kernels; then 16-chain delayed-acceptance RWM (surrogate first stage,
+    full solver second) is compared with 16-chain batched RWM, 16-chain MALA
+    and plain one-chain RWM at the same wall-time budget."""
+    import gliomauq_amd as gq
+    from gliomauq_amd import anatomy as A
+    from gliomauq_amd import sampler as SM
+    from gliomauq_amd import surrogate as SU
+
+    n, steps, C = 128, args.sur_steps, args.chains
+    rng = np.random.default_rng(args.seed + 11)
+    pwm, pgm, u0, c = A.config_inputs(n, L_CM, args.seed)
+    h = 1.0 / n
+    dt = 0.9 * h * h / (6 * 1.5e-2 / L_CM ** 2)
+    tt = np.array([np.log(0.0013 * 1.25), np.log(0.025 * 0.8) + 1.0, c[2] + 0.3, c[1] - 0.4, c[0] + 0.2, 0.1])
+    lo = np.array([np.log(2e-4), np.log(2e-3), c[2] - 8, c[1] - 8, c[0] - 8, 0.02])
+    hi = np.array([np.log(1.5e-2), np.log(0.2), c[2] + 8, c[1] + 8, c[0] + 8, 0.3])
+    half = (slice(None, None, 2),) * 3
+
+    def solve_fields(ths, ens):
+        """final states at 64^3 for parameter rows ths (batched)"""
+        out = []
+        for i in range(0, len(ths), ens.n):
+            blk = ths[i:i + ens.n]
+            m = len(blk)
+            full = np.tile(0.5 * (lo + hi), (ens.n, 1))
+            full[:m] = blk
+            Dw = np.exp(full[:, 0]) / L_CM ** 2
+            prm = np.stack([Dw, 0.1 * Dw, np.exp(full[:, 1]), np.full(ens.n, dt)], 1)
+            ens.ic_gaussian(np.stack([full[:, 2], full[:, 3], full[:, 4], full[:, 5], np.full(ens.n, 2.0)], 1))
+            st = np.zeros(ens.n, np.int32)
+            st[:m] = steps
+            ens.run(prm, st, adapt_every=0)
+            for s in range(m):
+                out.append(ens.sample(s).dump_uniform()[half].astype(np.float32))
+        return np.array(out)
+
+    ctx = gq.Context(0)
+    gen = gq.Ensemble(ctx, args.sur_gen_batch, n // 16, 3, 4)
+    gen.write_volume("pw", pwm)
+    gen.write_volume("pg", pgm)
+    log("[bench] surrogate: truth + %d training fields (%d steps at 128^3)" % (args.sur_fields, steps))
+    u_true = solve_fields(tt[None], gen)[0]
+    obs64 = A.observations(u_true, seed=args.seed)
+    # training parameters: a box around the truth
+    span = np.array([0.15, 0.15, 1.5, 1.5, 1.5, 0.02])
+    t0 = time.perf_counter()
+    ths = tt + span * rng.uniform(-1, 1, (args.sur_fields + 64, 6))
+    fields = solve_fields(ths, gen)
+    t_gen = time.perf_counter() - t0
+    del gen
+    sur = SU.Surrogate(batch=args.sur_batch, seed=args.seed)
+    sur.set_anatomy(pwm[half], pgm[half])
+    mu = tt[[0, 1, 5]]
+    sd = span[[0, 1, 5]]
+    spost = SM.SurrogatePosterior(sur, obs64, lo, hi, ratio=2, norm_mu=mu, norm_sd=sd)
+    P, U0 = spost.inputs(ths)
+    ntr = args.sur_fields
+    log("[bench] surrogate: training %d epochs" % args.sur_epochs)
+    t1 = time.perf_counter()
+    losses = []
+    fwd_ms = bwd_ms = 0.0
+    nsteps_tr = 0
+    for ep in range(args.sur_epochs):
+        perm = rng.permutation(ntr)
+        lr = args.sur_lr * (0.5 ** (ep // max(1, args.sur_epochs // 3)))
+        for i in range(0, ntr - args.sur_batch + 1, args.sur_batch):
+            j = perm[i:i + args.sur_batch]
+            losses.append(sur.train_step(P[j], U0[j], fields[j], lr))
+            tm = sur.timing()
+            fwd_ms += tm["forward_ms"]
+            bwd_ms += tm["backward_ms"]
+            nsteps_tr += 1
+        log("[bench] epoch %d loss %.4e" % (ep, np.mean(losses[-(ntr // args.sur_batch):])))
+    t_train = time.perf_counter() - t1
+    f1, f2 = SU.conv_flops(args.sur_batch)
+    # held-out field error and likelihood agreement
+    pv = sur.predict(P[ntr:], U0[ntr:])
+    err = np.sqrt(np.mean((pv - fields[ntr:]) ** 2, axis=(1, 2, 3))) / np.sqrt(np.mean(fields[ntr:] ** 2, axis=(1, 2, 3)))
+    # serving throughput
+    t2 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        sur.predict(P[:args.sur_batch], U0[:args.sur_batch])
+    t_pred = (time.perf_counter() - t2) / reps
+    pred_ms = sur.timing()["forward_ms"]
+
+    # samplers on the full posterior (observations at 64^3)
+    post_b = SM.Posterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=C)
+    post_1 = SM.Posterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=1)
+    log("[bench] scale probe")
+    scale = SM.probe_scales(post_b, tt, drop=2.0)
+    start = tt + 0.3 * scale * rng.standard_normal((C, 6))
+    budget = args.mcmc_budget
+    res_s = {}
+
+    def run_loop(fn, state):
+        xs, acs = [], []
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            state = fn(state, xs, acs)
+        return np.concatenate(xs, 0), np.concatenate(acs, 0), time.perf_counter() - t0, state
+
+    # plain RWM
+    log("[bench] plain RWM")
+    o, a, lf = SM.rwm(post_1, start[0], 40, scale, rng, adapt_until=40)
+
+    def step_rwm(st, xs, acs):
+        o, a, lf = SM.rwm(post_1, st[0], 20, scale, rng, logf=st[1])
+        xs.append(o[:, None, :])
+        acs.append(a[:, None])
+        return (o[-1], lf)
+    x_r, a_r, w_r, _ = run_loop(step_rwm, (o[-1], lf))
+    # batched RWM
+    log("[bench] batched RWM")
+    o, a, lf = SM.rwm_batched(post_b, start, 20, scale, rng, adapt_until=20)
+
+    def step_rwmb(st, xs, acs):
+        o, a, lf = SM.rwm_batched(post_b, st[0], 5, scale, rng, logf=st[1])
+        xs.append(o)
+        acs.append(a)
+        return (o[-1], lf)
+    x_b, a_b, w_b, _ = run_loop(step_rwmb, (o[-1], lf))
+    # DA RWM
+    log("[bench] delayed-acceptance RWM")
+    o, a, lf, nf0 = SM.da_rwm(post_b, spost, start, 20, scale, rng, adapt_until=20)
+    nfull = [0]
+
+    def step_da(st, xs, acs):
+        o, a, lf, nf = SM.da_rwm(post_b, spost, st[0], 5, scale, rng, logf=st[1])
+        nfull[0] += nf
+        xs.append(o)
+        acs.append(a)
+        return (o[-1], lf)
+    x_d, a_d, w_d, _ = run_loop(step_da, (o[-1], lf))
+    # MALA
+    log("[bench] MALA")
+    o, a, le = SM.mala(post_b, start, 10, scale, rng, adapt_until=10)
+
+    def step_mala(st, xs, acs):
+        o, a, le = SM.mala(post_b, st[0], 3, scale, rng, logeps=st[1])
+        xs.append(o)
+        acs.append(a)
+        return (o[-1], le)
+    x_m, a_m, w_m, _ = run_loop(step_mala, (o[-1], le))
+
+    def ess(x):
+        return min(sum(SM.ess_batch_means(x[:, k, p]) for k in range(x.shape[1])) for p in range(6))
+
+    rates = {}
+    for name, x, a, w in (("rwm_plain", x_r, a_r, w_r), ("rwm_16", x_b, a_b, w_b), ("da_rwm_16", x_d, a_d, w_d),
+                          ("mala_16", x_m, a_m, w_m)):
+        e = ess(x)
+        rates[name] = {"ess": e, "ess_per_hour": e / w * 3600, "iterations": int(x.shape[0]),
+                       "accept": float(a.mean()), "wall_s": w,
+                       "posterior_mean": x.reshape(-1, 6).mean(0).tolist()}
+    rates["da_rwm_16"]["full_solves"] = nfull[0]
+    rates["da_rwm_16"]["proposals"] = int(x_d.shape[0] * C)
+    ref_mean = np.array(rates["rwm_16"]["posterior_mean"])
+    for k in rates:
+        rates[k]["mean_rel_dev_vs_rwm16"] = (np.abs(np.array(rates[k]["posterior_mean"]) - ref_mean) /
+                                            np.abs(ref_mean)).max()
+    best = max(rates, key=lambda k: rates[k]["ess_per_hour"])
+    res = {
+        "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
+        "value": rates[best]["ess_per_hour"], "unit": "effective posterior samples/hour", "n_gpus": 1,
+        "steps": steps, "warmup": 20, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (surrogate) / f32 (solver)", "data": "synthetic",
+        "config": {"workload": "surrogate + delayed acceptance: 128^3 uniform solver, %d steps, observations 64^3; "
+                               "surrogate 64^3 encoder-decoder (16-128 channels, 3^3), bf16 MFMA, %d fields, "
+                               "%d epochs" % (steps, ntr, args.sur_epochs), "seed": args.seed, "best": best},
+        "surrogate": {"train_fields": ntr, "gen_s": t_gen, "train_s": t_train, "train_steps": nsteps_tr,
+                      "loss_first": losses[0], "loss_last": float(np.mean(losses[-(ntr // args.sur_batch):])),
+                      "heldout_rel_l2_mean": float(err.mean()), "heldout_rel_l2_max": float(err.max()),
+                      "train_tflops": (f2 * nsteps_tr) / ((fwd_ms + bwd_ms) * 1e-3) / 1e12,
+                      "forward_tflops": f1 / (pred_ms * 1e-3) / 1e12,
+                      "mfma_util_forward": f1 / (pred_ms * 1e-3) / 2.5e15,
+                      "mfma_util_train": (f2 * nsteps_tr) / ((fwd_ms + bwd_ms) * 1e-3) / 2.5e15,
+                      "predict_ms_per_batch": pred_ms, "predict_wall_s_per_batch": t_pred},
+        "mcmc": rates,
+        "speedup_vs_plain": {k: rates[k]["ess_per_hour"] / max(rates["rwm_plain"]["ess_per_hour"], 1e-30)
+                             for k in rates},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,7 +492,14 @@ def main():
     ap.add_argument("--adapt-every", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="adaptive", choices=["adaptive", "ensemble", "adjoint", "mcmc"])
+    ap.add_argument("--workload", default="adaptive",
+                    choices=["adaptive", "ensemble", "adjoint", "mcmc", "surrogate"])
+    ap.add_argument("--sur-steps", type=int, default=2000)
+    ap.add_argument("--sur-fields", type=int, default=1024)
+    ap.add_argument("--sur-epochs", type=int, default=6)
+    ap.add_argument("--sur-batch", type=int, default=8)
+    ap.add_argument("--sur-lr", type=float, default=2e-3)
+    ap.add_argument("--sur-gen-batch", type=int, default=64)
     ap.add_argument("--mcmc-n", type=int, default=64)
     ap.add_argument("--mcmc-steps", type=int, default=500)
     ap.add_argument("--mcmc-pilot", type=int, default=40)
@@ -322,6 +520,8 @@ def main():
         return adjoint_workload(args)
     if args.workload == "mcmc":
         return mcmc_workload(args)
+    if args.workload == "surrogate":
+        return surrogate_workload(args)
 
     import gliomauq_amd as gq
     from gliomauq_amd import anatomy as A

@@ -29,7 +29,7 @@ EXPORTS = [
     "gq_ensemble_ic_gaussian", "gq_adjoint_flip",
     "gq_sur_create", "gq_sur_destroy", "gq_sur_last_error", "gq_sur_set_anatomy", "gq_sur_param_count",
     "gq_sur_get_params", "gq_sur_set_params", "gq_sur_predict", "gq_sur_loss_grad", "gq_sur_train_step",
-    "gq_sur_timing",
+    "gq_sur_timing", "gq_sur_obs_flip_upload", "gq_sur_loglik_flip",
 ]
 
 RUN_GRAPH = 1

@@ -893,6 +893,7 @@ struct gq_ctx {
         u.index = d_index.p;
         u.gpd = bpd * gq::BS;
         u.hole = 0.0;
+        u.obs_stride = 1;
         return u;
     }
     gq::UniformArgs uniform_args() { return uniform_args(cur_sample); }
@@ -1573,12 +1574,20 @@ gq_status gq_obs_flip_upload(gq_ctx *ctx, int gpd, const uint8_t *t1, const uint
     });
 }
 
+// observations on the model's grid or on a coarser one whose size divides it
+static int flip_stride(gq_ctx *ctx, int gpd)
+{
+    const int og = ctx->obs_flip_gpd;
+    if (og <= 0 || gpd % og != 0) throw gq_error(GQ_ERR_STATE, "observations missing or of another size");
+    return gpd / og;
+}
+
 gq_status gq_loglik_flip(gq_ctx *ctx, const double params[4], double out[4])
 {
     return guard(ctx, [&] {
         if (!params || !out) throw gq_error(GQ_ERR_ARG, "null argument");
         gq::UniformArgs u = ctx->uniform_args();
-        if (ctx->obs_flip_gpd != u.gpd) throw gq_error(GQ_ERR_STATE, "observations missing or of another size");
+        u.obs_stride = flip_stride(ctx, u.gpd);
         const double th1 = params[0], th2 = params[1], eps = params[2], sigma = params[3];
         if (!(eps > 0 && eps < 1) || !(sigma > 0)) throw gq_error(GQ_ERR_NUMERIC, "flip rate / sigma out of range");
         const double lp1 = std::log(1. - eps), lp0 = std::log(eps);
@@ -1607,7 +1616,7 @@ gq_status gq_ensemble_loglik_flip(gq_ctx *ctx, const double params[4], double *o
         const int S = ctx->nsamples();
         for (int s = 1; s < S; s++) (void)ctx->uniform_args(s);  // every sample must map to the uniform view
         gq::UniformArgs u = ctx->uniform_args(0);
-        if (ctx->obs_flip_gpd != u.gpd) throw gq_error(GQ_ERR_STATE, "observations missing or of another size");
+        u.obs_stride = flip_stride(ctx, u.gpd);
         const double th1 = params[0], th2 = params[1], eps = params[2], sigma = params[3];
         if (!(eps > 0 && eps < 1) || !(sigma > 0)) throw gq_error(GQ_ERR_NUMERIC, "flip rate / sigma out of range");
         const double lp1 = std::log(1. - eps), lp0 = std::log(eps);
@@ -1700,6 +1709,7 @@ gq_status gq_loglik_ref_volume(gq_ctx *ctx, const double *model, int gpd, const 
         u.index = nullptr;
         u.gpd = gpd;
         u.hole = 0.0;
+        u.obs_stride = 1;
         const double sigma2 = params[0], scale = params[1], t1uc = params[2], t2uc = params[3], slope = params[4];
         const int nb = gq::reduce_blocks((int64_t)n);
         ctx->d_partial.reserve((size_t)nb * 4 + 8);

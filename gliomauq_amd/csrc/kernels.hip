@@ -917,11 +917,17 @@ __global__ __launch_bounds__(256) void k_lik_flip(UniformArgs u, const uint8_t *
                                                   double th1, double th2, double lp1, double lp0, double c0,
                                                   double is2, double *partial)
 {
-    const int64_t n = (int64_t)u.gpd * u.gpd * u.gpd;
+    const int s = u.obs_stride;
+    const int go = u.gpd / s;
+    const int64_t n = (int64_t)go * go * go;
     double acc[3] = {0.0, 0.0, 0.0};
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
         double m = 0.0;
-        if (!voxel_value(u, v, m)) m = u.hole;
+        const int64_t vm = s == 1 ? v
+                                  : (int64_t)(v % go) * s +
+                                        (int64_t)u.gpd * ((int64_t)((v / go) % go) * s +
+                                                          (int64_t)u.gpd * ((v / ((int64_t)go * go)) * s));
+        if (!voxel_value(u, vm, m)) m = u.hole;
         const int m1 = m > th1, m2 = m > th2;
         acc[0] += (m1 == (int)t1[v]) ? lp1 : lp0;
         acc[1] += (m2 == (int)t2[v]) ? lp1 : lp0;

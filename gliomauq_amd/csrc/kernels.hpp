@@ -46,6 +46,7 @@ struct UniformArgs {
     const int *index;  // [slot*3]
     int gpd;
     double hole;
+    int obs_stride;  // flip likelihood: observation voxel o reads model voxel o*obs_stride (obs grid gpd/obs_stride)
 };
 
 // One backward step of the discrete adjoint on uniform grids (every face

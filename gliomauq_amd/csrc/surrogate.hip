@@ -366,6 +366,60 @@ __global__ __launch_bounds__(256) void k_pack_weights(const float *w, int cin, i
     wt[(int64_t)ci * 27 * cout + (int64_t)(26 - tap) * cout + co] = b;
 }
 
+// Flip-rate log-likelihood of each batch item's prediction against 64^3
+// observations: per-block FP64 partials [item][block][3], then an ordered
+// per-item reduction (T1, FLAIR, PET terms).
+__global__ __launch_bounds__(256) void k_sur_lik_flip(const float *__restrict__ pred, int64_t n,
+                                                      const uint8_t *__restrict__ t1, const uint8_t *__restrict__ t2,
+                                                      const float *__restrict__ pet, double th1, double th2,
+                                                      double lp1, double lp0, double c0, double is2, double *partial)
+{
+    __shared__ double red[3][4];
+    const int b = blockIdx.y;
+    const float *p = pred + (int64_t)b * n;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256) {
+        const double m = p[v];
+        acc[0] += ((m > th1) == (t1[v] != 0)) ? lp1 : lp0;
+        acc[1] += ((m > th2) == (t2[v] != 0)) ? lp1 : lp0;
+        const float q = pet[v];
+        if (!__builtin_isnan(q)) {
+            const double r = (double)q - m;
+            acc[2] += c0 - r * r * is2;
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double s = acc[k];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (lane == 0) red[k][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3)
+        partial[((int64_t)b * gridDim.x + blockIdx.x) * 3 + threadIdx.x] =
+            ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+}
+
+__global__ void k_sum_item_partials(const double *partial, int nb, double *out)
+{
+    const int k = threadIdx.x, b = blockIdx.x;
+    if (k >= 3) return;
+    double s = 0.0;
+    for (int i = 0; i < nb; i++) s += partial[((int64_t)b * nb + i) * 3 + k];
+    out[b * 3 + k] = s;
+}
+
+hipError_t launch_sur_lik_flip(const float *pred, int64_t n, int batch, const uint8_t *t1, const uint8_t *t2,
+                               const float *pet, double th1, double th2, double lp1, double lp0, double c0,
+                               double is2, double *partial, int nb, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sur_lik_flip, dim3(nb, batch), dim3(256), 0, st, pred, n, t1, t2, pet, th1, th2, lp1, lp0,
+                       c0, is2, partial);
+    hipLaunchKernelGGL(k_sum_item_partials, dim3(batch), dim3(64), 0, st, partial, nb, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 hipError_t launch_conv3(const ConvArgs &a, hipStream_t st)
 {

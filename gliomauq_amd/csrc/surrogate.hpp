@@ -76,6 +76,10 @@ hipError_t launch_assemble_input(const float *params, const float *u0, const flo
                                  int batch, __bf16 *x, hipStream_t st);
 hipError_t launch_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
                        float eps, float c1, float c2, hipStream_t st);
+// out[b*3 + {0,1,2}] = T1, FLAIR, PET log-likelihood terms of item b (partial: batch*nb*3)
+hipError_t launch_sur_lik_flip(const float *pred, int64_t n, int batch, const uint8_t *t1, const uint8_t *t2,
+                               const float *pet, double th1, double th2, double lp1, double lp0, double c0,
+                               double is2, double *partial, int nb, double *out, hipStream_t st);
 hipError_t launch_pack_weights(const float *w, int cin, int cout, __bf16 *wf, __bf16 *wt, hipStream_t st);
 
 }  // namespace sur

@@ -76,6 +76,11 @@ struct gq_sur {
     int64_t partial_n = 0;
     // inputs
     float *pw = nullptr, *pg = nullptr, *prm = nullptr, *u0 = nullptr, *tgt = nullptr, *pred = nullptr;
+    // flip-rate observations on the 64^3 grid
+    uint8_t *o_t1 = nullptr, *o_t2 = nullptr;
+    float *o_pet = nullptr;
+    double *lpart = nullptr, *lres = nullptr;
+    bool have_obs = false;
     double last_fwd_ms = 0, last_bwd_ms = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     std::vector<void *> allocs;
@@ -175,6 +180,11 @@ struct gq_sur {
         u0 = A<float>(V64);
         tgt = A<float>(V64);
         pred = A<float>(V64);
+        o_t1 = A<uint8_t>(64 * 64 * 64);
+        o_t2 = A<uint8_t>(64 * 64 * 64);
+        o_pet = A<float>(64 * 64 * 64);
+        lpart = A<double>((size_t)batch * 256 * 3);
+        lres = A<double>((size_t)batch * 3);
         for (auto &e : ev) SCHK(hipEventCreate(&e));
         SCHK(hipDeviceSynchronize());  // the synchronous copies and fills above precede any work on st
         pack();
@@ -547,6 +557,50 @@ gq_status gq_sur_train_step(gq_sur *s, const float *params, const float *u0, con
         SCHK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
         s->last_fwd_ms = f;
         s->last_bwd_ms = b;
+    });
+}
+
+gq_status gq_sur_obs_flip_upload(gq_sur *s, const uint8_t *t1, const uint8_t *t2, const float *pet)
+{
+    return sur_guard(s, [&] {
+        if (!t1 || !t2 || !pet) throw sur_error(GQ_ERR_ARG, "null observations");
+        const size_t n = (size_t)64 * 64 * 64;
+        SCHK(hipMemcpy(s->o_t1, t1, n, hipMemcpyHostToDevice));
+        SCHK(hipMemcpy(s->o_t2, t2, n, hipMemcpyHostToDevice));
+        SCHK(hipMemcpy(s->o_pet, pet, n * 4, hipMemcpyHostToDevice));
+        s->have_obs = true;
+    });
+}
+
+gq_status gq_sur_loglik_flip(gq_sur *s, const float *params, const float *u0, int batch, const double lik[4],
+                             double *out)
+{
+    return sur_guard(s, [&] {
+        if (!params || !u0 || !lik || !out) throw sur_error(GQ_ERR_ARG, "null argument");
+        if (!s->have_obs) throw sur_error(GQ_ERR_STATE, "observations missing");
+        const double eps = lik[2], sigma = lik[3];
+        if (!(eps > 0 && eps < 1) || !(sigma > 0)) throw sur_error(GQ_ERR_NUMERIC, "flip rate / sigma out of range");
+        s->upload_batch(params, u0, nullptr, batch);
+        SCHK(hipEventRecord(s->ev[0], s->st));
+        s->forward(batch, false, false);
+        SCHK(hipEventRecord(s->ev[1], s->st));
+        const double lp1 = std::log(1. - eps), lp0 = std::log(eps);
+        const double c0 = -0.5 * std::log(2. * M_PI * sigma * sigma);
+        const double is2 = 1. / (2. * sigma * sigma);
+        SCHK(gq::sur::launch_sur_lik_flip(s->pred, (int64_t)64 * 64 * 64, batch, s->o_t1, s->o_t2, s->o_pet, lik[0],
+                                          lik[1], lp1, lp0, c0, is2, s->lpart, 256, s->lres, s->st));
+        std::vector<double> h((size_t)batch * 3);
+        SCHK(hipMemcpyAsync(h.data(), s->lres, h.size() * 8, hipMemcpyDeviceToHost, s->st));
+        SCHK(hipStreamSynchronize(s->st));
+        for (int b = 0; b < batch; b++) {
+            out[4 * b + 1] = h[3 * b];
+            out[4 * b + 2] = h[3 * b + 1];
+            out[4 * b + 3] = h[3 * b + 2];
+            out[4 * b] = h[3 * b] + h[3 * b + 1] + h[3 * b + 2];
+        }
+        float f = 0;
+        SCHK(hipEventElapsedTime(&f, s->ev[0], s->ev[1]));
+        s->last_fwd_ms = f;
     });
 }
 

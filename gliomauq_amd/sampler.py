@@ -78,10 +78,119 @@ class Posterior:
     def logpost(self, th):
         m, inside, prm, ic = self._inputs(th)
         self.ens.ic_gaussian(ic)
-        self.ens.run(prm, np.full(self.batch, self.n_steps, np.int32), adapt_every=0)
+        steps = np.zeros(self.batch, np.int32)
+        steps[:m] = self.n_steps  # padding slots take no steps
+        self.ens.run(prm, steps, adapt_every=0)
         ll = self.ens.loglik_flip(th1=self.lik[0], th2=self.lik[1], flip=self.lik[2], sigma=self.lik[3])
         self.n_evals += m
         return np.where(inside, ll[:m, 0], -np.inf)
+
+
+class SurrogatePosterior:
+    """Log-posterior with the surrogate's prediction in place of the solver:
+    the same parameters, prior box and flip-rate likelihood, observations on
+    the surrogate's 64^3 grid.  The surrogate inputs are the normalised
+    (log Dw, log rho, A) and the Gaussian initial condition at (x0, y0, z0)
+    on the 64^3 grid (coordinates of the model grid divided by `ratio`)."""
+
+    def __init__(self, sur, obs64, lo, hi, ratio, sigma_ic_model=2.0, lik=(0.7, 0.25, 0.05, 0.05),
+                 norm_mu=None, norm_sd=None):
+        self.sur = sur
+        sur.set_observations(*obs64)
+        self.lo = np.asarray(lo, np.float64)
+        self.hi = np.asarray(hi, np.float64)
+        self.ratio = ratio
+        self.sigma = sigma_ic_model / ratio
+        self.lik = lik
+        self.batch = sur.batch
+        self.mu = np.zeros(3) if norm_mu is None else np.asarray(norm_mu, np.float64)
+        self.sd = np.ones(3) if norm_sd is None else np.asarray(norm_sd, np.float64)
+        self.n_evals = 0
+
+    def inputs(self, th):
+        th = np.asarray(th, np.float64).reshape(-1, 6)
+        p = np.zeros((len(th), 4), np.float32)
+        p[:, :3] = (th[:, [0, 1, 5]] - self.mu) / self.sd
+        n = 64
+        i = np.arange(n, dtype=np.float64)
+        u0 = np.empty((len(th), n, n, n), np.float32)
+        for b, t in enumerate(th):
+            d = [i - t[2 + a] / self.ratio for a in range(3)]
+            d = [x - n * np.floor(x / n + 0.5) for x in d]
+            dz, dy, dx = np.meshgrid(d[2], d[1], d[0], indexing="ij")
+            u0[b] = t[5] * np.exp(-(dx * dx + dy * dy + dz * dz) / (2 * self.sigma ** 2))
+        return p, u0
+
+    def logpost(self, th):
+        th = np.asarray(th, np.float64).reshape(-1, 6)
+        inside = np.all((th >= self.lo) & (th <= self.hi), axis=1)
+        p, u0 = self.inputs(th)
+        ll = self.sur.loglik_flip(p, u0, self.lik)
+        self.n_evals += len(th)
+        return np.where(inside, ll[:, 0], -np.inf)
+
+
+def da_rwm(post, spost, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, logf=None, callback=None):
+    """Delayed-acceptance random-walk Metropolis for a population of chains:
+    stage 1 accepts a symmetric proposal with min(1, pi~(t')/pi~(t)) on the
+    surrogate posterior `spost`; only the survivors are solved with the full
+    model `post` (one batched call) and accepted with
+    min(1, pi(t') pi~(t) / (pi(t) pi~(t'))) (Christen & Fox 2005), so the
+    chains target pi exactly.  Returns (samples, accepted, final log f,
+    number of full solves)."""
+    th = np.asarray(theta0, np.float64).copy()
+    Cn = th.shape[0]
+    lp = _batched(post.logpost, th, post.batch)
+    ls = spost.logpost(th)
+    logf = np.zeros(Cn) if logf is None else np.array(logf, np.float64)
+    out = np.zeros((n_iter, Cn, 6))
+    acc = np.zeros((n_iter, Cn), bool)
+    n_full = 0
+    for it in range(n_iter):
+        prop = th + np.exp(logf)[:, None] * scale * rng.standard_normal((Cn, 6))
+        lsp = spost.logpost(prop)
+        a1 = np.log(rng.random(Cn)) < np.where(np.isfinite(lsp), lsp - ls, -np.inf)
+        idx = np.nonzero(a1)[0]
+        a = np.zeros(Cn, bool)
+        if len(idx):
+            lpp = _batched(post.logpost, prop[idx], post.batch)
+            n_full += len(idx)
+            r2 = (lpp - lp[idx]) - (lsp[idx] - ls[idx])
+            ok = np.log(rng.random(len(idx))) < np.where(np.isfinite(r2), r2, -np.inf)
+            j = idx[ok]
+            a[j] = True
+            th[j] = prop[j]
+            lp[j] = lpp[ok]
+            ls[j] = lsp[j]
+        acc[it] = a
+        if it < adapt_until:
+            logf += (a.astype(np.float64) - target) / math.sqrt(it + 1.0)
+        out[it] = th
+        if callback:
+            callback(it, th, lp)
+    return out, acc, logf, n_full
+
+
+def rwm_batched(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, logf=None):
+    """Random-walk Metropolis for a population of chains, all proposals of an
+    iteration in one batched solve.  Returns (samples, accepted, log f)."""
+    th = np.asarray(theta0, np.float64).copy()
+    Cn = th.shape[0]
+    lp = _batched(post.logpost, th, post.batch)
+    logf = np.zeros(Cn) if logf is None else np.array(logf, np.float64)
+    out = np.zeros((n_iter, Cn, 6))
+    acc = np.zeros((n_iter, Cn), bool)
+    for it in range(n_iter):
+        prop = th + np.exp(logf)[:, None] * scale * rng.standard_normal((Cn, 6))
+        lpp = _batched(post.logpost, prop, post.batch)
+        a = np.log(rng.random(Cn)) < np.where(np.isfinite(lpp), lpp - lp, -np.inf)
+        th[a] = prop[a]
+        lp[a] = lpp[a]
+        acc[it] = a
+        if it < adapt_until:
+            logf += (a.astype(np.float64) - target) / math.sqrt(it + 1.0)
+        out[it] = th
+    return out, acc, logf
 
 
 def probe_scales(post, theta, drop=2.0, lo=1e-7, hi=1.0, n=16):

@@ -431,8 +431,9 @@ class Ensemble:
         return dict(zip(keys, out.tolist()))
 
     def set_observations(self, t1, t2, pet):
-        """Flip-rate model observations on the gpd^3 grid (gq_obs_flip_upload)."""
-        gpd = self.bpd * BS
+        """Flip-rate model observations on the gpd^3 grid or on a grid coarser
+        by an integer factor (gq_obs_flip_upload)."""
+        gpd = np.asarray(t1).shape[0]
         keep = (np.ascontiguousarray(t1, np.uint8), np.ascontiguousarray(t2, np.uint8),
                 np.ascontiguousarray(pet, np.float32))
         self.ctx.call("gq_obs_flip_upload", gpd, ptr(keep[0], C.c_uint8), ptr(keep[1], C.c_uint8),

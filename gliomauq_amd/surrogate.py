@@ -4,7 +4,7 @@ A 3D convolutional encoder-decoder on 64^3 that maps four parameters
 (broadcast as channels), the initial condition u0 and the anatomy (pw, pg)
 to the final tumour field, trained and served by the project's bf16 MFMA
 kernels (FP32 accumulation, FP32 master weights, Adam).  It is the first
-stage of delayed-acceptance MCMC (sampler.da_mcmc); the full solver judges
+stage of delayed-acceptance MCMC (sampler.da_rwm); the full solver judges
 every proposal it lets through.
 """
 import ctypes as C
@@ -32,6 +32,8 @@ def _sig(L):
         "gq_sur_loss_grad": [vp, f32p, f32p, f32p, C.c_int, f64p, f32p],
         "gq_sur_train_step": [vp, f32p, f32p, f32p, C.c_int, C.c_float, f64p],
         "gq_sur_timing": [vp, f64p],
+        "gq_sur_obs_flip_upload": [vp, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), f32p],
+        "gq_sur_loglik_flip": [vp, f32p, f32p, C.c_int, f64p, f64p],
     }
     for name, args in table.items():
         fn = getattr(L, name)
@@ -116,6 +118,24 @@ class Surrogate:
         self._c("gq_sur_train_step", ptr(p, C.c_float), ptr(u, C.c_float), ptr(t, C.c_float), len(p),
                 C.c_float(lr), C.byref(loss))
         return loss.value
+
+    def set_observations(self, t1, t2, pet):
+        """Flip-rate observations on the 64^3 grid."""
+        self._obs = (np.ascontiguousarray(t1, np.uint8), np.ascontiguousarray(t2, np.uint8),
+                     np.ascontiguousarray(pet, np.float32))
+        self._c("gq_sur_obs_flip_upload", ptr(self._obs[0], C.c_uint8), ptr(self._obs[1], C.c_uint8),
+                ptr(self._obs[2], C.c_float))
+
+    def loglik_flip(self, params, u0, lik=(0.7, 0.25, 0.05, 0.05)):
+        """(batch, 4): logL, L_T1, L_FLAIR, L_PET of the predictions."""
+        p, u, _ = self._arr(params, u0)
+        lk = np.ascontiguousarray(lik, np.float64)
+        out = np.zeros((len(p), 4))
+        for i in range(0, len(p), self.batch):
+            b = min(self.batch, len(p) - i)
+            self._c("gq_sur_loglik_flip", ptr(p[i:i + b], C.c_float), ptr(u[i:i + b], C.c_float), b,
+                    ptr(lk, C.c_double), ptr(out[i:i + b], C.c_double))
+        return out
 
     def timing(self):
         out = np.zeros(2)

@@ -171,6 +171,12 @@ gq_status gq_sur_train_step(gq_sur *s, const float *params, const float *u0, con
                             double *loss);
 /* device ms of the last forward and backward pass */
 gq_status gq_sur_timing(gq_sur *s, double out[2]);
+/* flip-rate likelihood of the surrogate's predictions (the first stage of
+ * delayed-acceptance sampling): observations on the 64^3 grid as
+ * gq_obs_flip_upload; out = batch x {logL, L_T1, L_FLAIR, L_PET}, FP64 sums */
+gq_status gq_sur_obs_flip_upload(gq_sur *s, const uint8_t *t1, const uint8_t *t2, const float *pet);
+gq_status gq_sur_loglik_flip(gq_sur *s, const float *params, const float *u0, int batch, const double lik[4],
+                             double *out);
 
 /* ---- forward solve: replaces Glioma_HG_UQ::_reactionDiffusionStep
  * (Glioma/Glioma_HG_UQ.cpp:266-277) = BlockProcessing::pipeline_process(vInfo,
@@ -225,7 +231,9 @@ gq_status gq_loglik_ref_volume(gq_ctx *ctx, const double *model, int gpd, const 
 
 /* Flip-rate Bernoulli + Gaussian PET model (no reference counterpart: the
  * likelihood of the synthetic-observation workloads).  t1/t2: 0/1 masks,
- * pet: value or NaN where not observed, gpd^3 voxels x fastest.
+ * pet: value or NaN where not observed, gpd^3 voxels x fastest.  The
+ * observation grid may be coarser than the model's by an integer factor s
+ * (observation voxel o is compared with model voxel s*o).
  * params = {th_T1, th_FLAIR, flip_rate, pet_sigma}; out = {logL, L_T1,
  * L_FLAIR, L_PET}. FP64 accumulation on the device. */
 gq_status gq_obs_flip_upload(gq_ctx *ctx, int gpd, const uint8_t *t1, const uint8_t *t2, const float *pet);

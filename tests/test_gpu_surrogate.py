@@ -4,8 +4,8 @@ test_oracle_surrogate.py) on identical parameters and inputs.
 
 Bars: the oracle runs with the same bf16 rounding points in float64, so the
 remaining differences are FP32 accumulation order and the occasional
-one-ulp bf16 flip it causes: prediction relative L2 <= 1e-2; every layer's
-gradient relative L2 <= 3e-2 and cosine >= 0.999; the loss within 1e-3
+one-ulp bf16 flip it causes: prediction relative L2 <= 2e-3; every layer's
+gradient relative L2 <= 2e-3 and cosine >= 0.99999; the loss within 1e-5
 relative.  Training (Adam) on a fixed batch lowers the loss; results are
 bit-reproducible.
 """
@@ -50,15 +50,15 @@ def test_surrogate_matches_oracle():
                                              pw.astype(np.float64), pg.astype(np.float64), tgt.astype(np.float64))
     e_p = rel(pred, p_ref)
     print("prediction rel L2 %.3e, loss gpu %.6e oracle %.6e" % (e_p, loss, l_ref))
-    assert e_p <= 1e-2
-    assert abs(loss - l_ref) <= 1e-3 * abs(l_ref)
+    assert e_p <= 2e-3
+    assert abs(loss - l_ref) <= 1e-5 * abs(l_ref)
     o = 0
     for i, (cin, cout) in enumerate(SO.SPEC + [(16, 1)]):
         n = cout * 27 * cin + cout if i < 7 else 17
         a, b = g[o:o + n].astype(np.float64), g_ref[o:o + n]
         cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-300))
         print("layer %d grad rel L2 %.3e cos %.6f" % (i, rel(a, b), cos))
-        assert rel(a, b) <= 3e-2 and cos >= 0.999
+        assert rel(a, b) <= 2e-3 and cos >= 0.99999
         o += n
     s.close()
 
@@ -85,4 +85,23 @@ def test_surrogate_rejects_bad_batch():
     with pytest.raises(S.GQError) as e:
         s.train_step(params, u0, tgt, 1e-3)
     assert e.value.status == 1
+    s.close()
+
+
+def test_surrogate_likelihood_matches_oracle():
+    from oracle import oracle as O
+    params, u0, pw, pg, tgt = batch(seed=2, B=2)
+    s = S.Surrogate(batch=2, seed=5)
+    s.set_anatomy(pw, pg)
+    rng = np.random.default_rng(0)
+    t1 = (rng.random((64, 64, 64)) < 0.3).astype(np.uint8)
+    t2 = (rng.random((64, 64, 64)) < 0.5).astype(np.uint8)
+    pet = np.where(rng.random((64, 64, 64)) < 0.2, rng.random((64, 64, 64)), np.nan).astype(np.float32)
+    s.set_observations(t1, t2, pet)
+    ll = s.loglik_flip(params, u0)
+    pred = s.predict(params, u0)
+    for b in range(2):
+        ref, parts = O.lik_flip(pred[b].astype(np.float64), t1, t2, pet)
+        assert abs(ll[b, 0] - ref) <= 1e-10 * abs(ref)
+        assert np.allclose(ll[b, 1:], parts, rtol=1e-10)
     s.close()
