@@ -347,6 +347,7 @@ def surrogate_workload(args):
     ths = tt + span * rng.uniform(-1, 1, (args.sur_fields + 64, 6))
     fields = solve_fields(ths, gen)
     t_gen = time.perf_counter() - t0
+    log("[bench] generated %d fields in %.1f s" % (len(fields), t_gen))
     del gen
     sur = SU.Surrogate(batch=args.sur_batch, seed=args.seed)
     sur.set_anatomy(pwm[half], pgm[half])
@@ -370,7 +371,9 @@ def surrogate_workload(args):
             fwd_ms += tm["forward_ms"]
             bwd_ms += tm["backward_ms"]
             nsteps_tr += 1
-        log("[bench] epoch %d loss %.4e" % (ep, np.mean(losses[-(ntr // args.sur_batch):])))
+        log("[bench] epoch %d loss %.4e (%.1f s, device fwd+bwd %.2f ms/step)"
+            % (ep, np.mean(losses[-(ntr // args.sur_batch):]), time.perf_counter() - t1,
+               (fwd_ms + bwd_ms) / max(nsteps_tr, 1)))
     t_train = time.perf_counter() - t1
     f1, f2 = SU.conv_flops(args.sur_batch)
     # held-out field error and likelihood agreement
@@ -498,7 +501,7 @@ def main():
     ap.add_argument("--sur-fields", type=int, default=1024)
     ap.add_argument("--sur-epochs", type=int, default=6)
     ap.add_argument("--sur-batch", type=int, default=8)
-    ap.add_argument("--sur-lr", type=float, default=2e-3)
+    ap.add_argument("--sur-lr", type=float, default=3e-4)
     ap.add_argument("--sur-gen-batch", type=int, default=64)
     ap.add_argument("--mcmc-n", type=int, default=64)
     ap.add_argument("--mcmc-steps", type=int, default=500)

@@ -191,16 +191,22 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
 // constant in u (derivative zero almost everywhere).
 __global__ __launch_bounds__(256) void k_lik_flip_grad(const float *__restrict__ u, const float *__restrict__ pet,
                                                        const int *__restrict__ work, const int *__restrict__ index,
-                                                       int gpd, double inv_s2, float *__restrict__ lam)
+                                                       int gpd, int stride, double inv_s2, float *__restrict__ lam)
 {
+    // observations on the grid coarser by `stride`: only model voxels at multiples of it are observed
     const int slot = work[blockIdx.x];
     const size_t base = (size_t)slot * B3;
     const int64_t ox = (int64_t)index[slot * 3] * B, oy = (int64_t)index[slot * 3 + 1] * B,
                   oz = (int64_t)index[slot * 3 + 2] * B;
+    const int64_t go = gpd / stride;
     for (int i = threadIdx.x; i < B3; i += 256) {
-        const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
-        const float p = pet[(ox + x) + gpd * ((oy + y) + (int64_t)gpd * (oz + z))];
-        lam[base + i] = __builtin_isnan(p) ? 0.f : (float)(((double)p - (double)u[base + i]) * inv_s2);
+        const int64_t gx = ox + (i & 15), gy = oy + ((i >> 4) & 15), gz = oz + (i >> 8);
+        float g = 0.f;
+        if (gx % stride == 0 && gy % stride == 0 && gz % stride == 0) {
+            const float p = pet[gx / stride + go * (gy / stride + go * (gz / stride))];
+            if (!__builtin_isnan(p)) g = (float)(((double)p - (double)u[base + i]) * inv_s2);
+        }
+        lam[base + i] = g;
     }
 }
 
@@ -280,10 +286,11 @@ hipError_t launch_adj_step(const AdjArgs &a, hipStream_t st)
 }
 
 hipError_t launch_lik_flip_grad(const float *u, const float *pet, const int *work, int nwork, const int *index,
-                                int gpd, double inv_s2, float *lam, hipStream_t st)
+                                int gpd, int stride, double inv_s2, float *lam, hipStream_t st)
 {
     if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lik_flip_grad, dim3(nwork), dim3(256), 0, st, u, pet, work, index, gpd, inv_s2, lam);
+    hipLaunchKernelGGL(k_lik_flip_grad, dim3(nwork), dim3(256), 0, st, u, pet, work, index, gpd, stride, inv_s2,
+                       lam);
     return hipGetLastError();
 }
 

@@ -1284,6 +1284,14 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
     });
 }
 
+// observations on the model's grid or on a coarser one whose size divides it
+static int flip_stride(gq_ctx *ctx, int gpd)
+{
+    const int og = ctx->obs_flip_gpd;
+    if (og <= 0 || gpd % og != 0) throw gq_error(GQ_ERR_STATE, "observations missing or of another size");
+    return gpd / og;
+}
+
 // every sample a uniform grid (all leaves at the base level): the adjoint and the Gaussian IC map cells to voxels 1:1
 static void need_uniform(gq_ctx *ctx)
 {
@@ -1327,7 +1335,7 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
             if (!(p[s].Dw > 0) || !(p[s].Dg > 0)) throw gq_error(GQ_ERR_NUMERIC, "the adjoint needs Dw, Dg > 0");
         }
         const int gpd = ctx->G(0).blocks_per_dim() * gq::BS;
-        if (ctx->obs_flip_gpd != gpd) throw gq_error(GQ_ERR_STATE, "observations missing or of another size");
+        const int ostride = flip_stride(ctx, gpd);
         const double sigma = lik[3], eps = lik[2];
         if (!(eps > 0 && eps < 1) || !(sigma > 0)) throw gq_error(GQ_ERR_NUMERIC, "flip rate / sigma out of range");
         if (ic) {
@@ -1382,6 +1390,7 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
         // log-likelihood at u_N and its gradient field
         {
             gq::UniformArgs u = ctx->uniform_args(0);
+            u.obs_stride = ostride;
             const double th1 = lik[0], th2 = lik[1];
             const double lp1 = std::log(1. - eps), lp0 = std::log(eps);
             const double c0 = -0.5 * std::log(2. * M_PI * sigma * sigma);
@@ -1405,7 +1414,7 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
         float *lam[2] = {ctx->d_lam.p, ctx->d_lam.p + pool};
         int lc = 0;
         HIPCHK(gq::launch_lik_flip_grad(ctx->phi[ctx->cur], ctx->d_petf.p, ctx->d_work.p, nw, ctx->d_index.p, gpd,
-                                        1.0 / (sigma * sigma), lam[lc], st));
+                                        ostride, 1.0 / (sigma * sigma), lam[lc], st));
         const int NS = gq::adj_slabs();
         ctx->d_gacc.reserve((size_t)ctx->cap * NS * 2);
         HIPCHK(hipMemsetAsync(ctx->d_gacc.p, 0, (size_t)ctx->cap * NS * 2 * sizeof(double), st));
@@ -1572,14 +1581,6 @@ gq_status gq_obs_flip_upload(gq_ctx *ctx, int gpd, const uint8_t *t1, const uint
         HIPCHK(hipStreamSynchronize(ctx->st));
         ctx->obs_flip_gpd = gpd;
     });
-}
-
-// observations on the model's grid or on a coarser one whose size divides it
-static int flip_stride(gq_ctx *ctx, int gpd)
-{
-    const int og = ctx->obs_flip_gpd;
-    if (og <= 0 || gpd % og != 0) throw gq_error(GQ_ERR_STATE, "observations missing or of another size");
-    return gpd / og;
 }
 
 gq_status gq_loglik_flip(gq_ctx *ctx, const double params[4], double out[4])

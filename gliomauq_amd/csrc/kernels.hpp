@@ -69,9 +69,9 @@ struct AdjArgs {
 };
 int adj_slabs();
 hipError_t launch_adj_step(const AdjArgs &a, hipStream_t st);
-// lam = dL/du of the flip-rate model's PET term at u (uniform grids)
+// lam = dL/du of the flip-rate model's PET term at u (uniform grids; observations on the grid coarser by stride)
 hipError_t launch_lik_flip_grad(const float *u, const float *pet, const int *work, int nwork, const int *index,
-                                int gpd, double inv_s2, float *lam, hipStream_t st);
+                                int gpd, int stride, double inv_s2, float *lam, hipStream_t st);
 // Gaussian initial condition per sample, icp[s*5] = x0, y0, z0 (voxels), A, sigma (voxels)
 hipError_t launch_ic_gaussian(float *phi, const int *work, int nwork, const int *index, const int *slot_sample,
                               const double *icp, int gpd, hipStream_t st);

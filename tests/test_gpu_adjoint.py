@@ -83,9 +83,12 @@ def test_adjoint_vs_fp64_oracle():
     ctx.close()
 
 
-def test_adjoint_vs_finite_differences():
+@pytest.mark.parametrize("stride", [1, 2])
+def test_adjoint_vs_finite_differences(stride):
     n, steps = 64, 200
     pwm, pgm, c, obs = setup(n, seed=7)
+    if stride > 1:  # observations on the grid coarser by `stride`
+        obs = tuple(a[::stride, ::stride, ::stride].copy() for a in obs)
     ctx = gq.Context(0)
     ic0 = np.array([c[2] + 0.25, c[1] - 0.35, c[0] + 0.15, 0.3, 2.0])
     Dw_cm, rho = 0.05, 0.2
