@@ -386,6 +386,19 @@ def surrogate_workload(args):
         sur.predict(P[:args.sur_batch], U0[:args.sur_batch])
     t_pred = (time.perf_counter() - t2) / reps
     pred_ms = sur.timing()["forward_ms"]
+    sur_res = {"train_fields": ntr, "gen_s": t_gen, "train_s": t_train, "train_steps": nsteps_tr,
+               "loss_first": losses[0], "loss_last": float(np.mean(losses[-(ntr // args.sur_batch):])),
+               "heldout_rel_l2_mean": float(err.mean()), "heldout_rel_l2_max": float(err.max()),
+               "train_tflops": (f2 * nsteps_tr) / ((fwd_ms + bwd_ms) * 1e-3) / 1e12,
+               "forward_tflops": f1 / (pred_ms * 1e-3) / 1e12,
+               "mfma_util_forward": f1 / (pred_ms * 1e-3) / 2.5e15,
+               "mfma_util_train": (f2 * nsteps_tr) / ((fwd_ms + bwd_ms) * 1e-3) / 2.5e15,
+               "predict_ms_per_batch": pred_ms, "predict_wall_s_per_batch": t_pred,
+               "train_fwd_ms_per_step": fwd_ms / max(nsteps_tr, 1), "train_bwd_ms_per_step": bwd_ms / max(nsteps_tr, 1)}
+    log("[bench] surrogate %s" % json.dumps(sur_res))
+    if args.sur_train_only:
+        print(json.dumps({"metric": "surrogate training only", "surrogate": sur_res}), flush=True)
+        return
 
     # samplers on the full posterior (observations at 64^3)
     post_b = SM.Posterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=C)
@@ -471,14 +484,7 @@ def surrogate_workload(args):
         "config": {"workload": "surrogate + delayed acceptance: 128^3 uniform solver, %d steps, observations 64^3; "
                                "surrogate 64^3 encoder-decoder (16-128 channels, 3^3), bf16 MFMA, %d fields, "
                                "%d epochs" % (steps, ntr, args.sur_epochs), "seed": args.seed, "best": best},
-        "surrogate": {"train_fields": ntr, "gen_s": t_gen, "train_s": t_train, "train_steps": nsteps_tr,
-                      "loss_first": losses[0], "loss_last": float(np.mean(losses[-(ntr // args.sur_batch):])),
-                      "heldout_rel_l2_mean": float(err.mean()), "heldout_rel_l2_max": float(err.max()),
-                      "train_tflops": (f2 * nsteps_tr) / ((fwd_ms + bwd_ms) * 1e-3) / 1e12,
-                      "forward_tflops": f1 / (pred_ms * 1e-3) / 1e12,
-                      "mfma_util_forward": f1 / (pred_ms * 1e-3) / 2.5e15,
-                      "mfma_util_train": (f2 * nsteps_tr) / ((fwd_ms + bwd_ms) * 1e-3) / 2.5e15,
-                      "predict_ms_per_batch": pred_ms, "predict_wall_s_per_batch": t_pred},
+        "surrogate": sur_res,
         "mcmc": rates,
         "speedup_vs_plain": {k: rates[k]["ess_per_hour"] / max(rates["rwm_plain"]["ess_per_hour"], 1e-30)
                              for k in rates},
@@ -503,6 +509,7 @@ def main():
     ap.add_argument("--sur-batch", type=int, default=8)
     ap.add_argument("--sur-lr", type=float, default=3e-4)
     ap.add_argument("--sur-gen-batch", type=int, default=64)
+    ap.add_argument("--sur-train-only", action="store_true")
     ap.add_argument("--mcmc-n", type=int, default=64)
     ap.add_argument("--mcmc-steps", type=int, default=500)
     ap.add_argument("--mcmc-pilot", type=int, default=40)

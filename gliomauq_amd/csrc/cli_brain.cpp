@@ -69,8 +69,9 @@ void patient_anatomy(const gq::io::Volume &GM, const gq::io::Volume &WM, const g
         for (int iz = 0; iz < 16; iz++)
             for (int iy = 0; iy < 16; iy++)
                 for (int ix = 0; ix < 16; ix++) {
-                    const double x[3] = {bi.origin[0] + (double)bi.h * ix, bi.origin[1] + (double)bi.h * iy,
-                                         bi.origin[2] + (double)bi.h * iz};
+                    // BlockInfo::pos: Real arithmetic, then widened (MRAGCommon.h:199-204)
+                    const double x[3] = {(double)(bi.origin[0] + bi.h * ix), (double)(bi.origin[1] + bi.h * iy),
+                                         (double)(bi.origin[2] + bi.h * iz)};
                     int m[3] = {(int)std::floor(x[0] / bh), (int)std::floor(x[1] / bh), (int)std::floor(x[2] / bh)};
                     m[0] -= (int)((smax - sx) * 0.5);
                     m[1] -= (int)((smax - sy) * 0.5);
@@ -91,7 +92,7 @@ void patient_anatomy(const gq::io::Volume &GM, const gq::io::Volume &WM, const g
                         PGt = (float)(PGt / all);
                         PWt = (float)(PWt / all);
                         Pcsf = (float)(Pcsf / all);
-                        Pcsf = (Pcsf > 0.1f) ? 1.f : Pcsf;
+                        Pcsf = ((double)Pcsf > 0.1) ? 1.f : Pcsf;
                         if (Pcsf < 1.f) {
                             pw[c] = PWt / (Pcsf + PWt + PGt);
                             pg[c] = PGt / (Pcsf + PWt + PGt);
@@ -101,7 +102,7 @@ void patient_anatomy(const gq::io::Volume &GM, const gq::io::Volume &WM, const g
                                         (float)(x[2] - tumor_ic[2])};
                     const float dist = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
                     const float psi = (dist - tumorRadius) * iw;
-                    if ((psi < -1) && ((PGt > 0.001f) || (PWt > 0.001f))) phi[c] = 1.0f;
+                    if ((psi < -1) && (((double)PGt > 0.001) || ((double)PWt > 0.001))) phi[c] = 1.0f;
                     else if (((-1 <= psi) && (psi <= 1)) && ((PGt > 0) || (PWt > 0)))
                         phi[c] = (float)(1.0 * 0.5 * (1 - psi - std::sin(M_PI * psi) / (M_PI)));
                     else phi[c] = 0.f;

@@ -8,7 +8,8 @@
 // (v_mfma_f32_16x16x32_bf16, FP32 accumulation): M = voxels, N = output
 // channels, K = 27 taps x input channels, k = tap * Cin + ci with
 // tap = (dz+1)*9 + (dy+1)*3 + (dx+1), zero padding outside the volume.
-// Weight gradients accumulate in FP32 on the vector ALUs.
+// Weight gradients accumulate in FP32 on the vector ALUs.  Activation:
+// leaky ReLU (slope 0.05), so no unit can die for good during training.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -22,6 +23,8 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr float kLeak = 0.05f;  // leaky ReLU slope for negative inputs
 
 __device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
 
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a)
             const int64_t vo = vbase + 4 * kq + r;
             if (vo >= nvox) continue;
             float val = acc[t][r] + b;
-            if (a.relu) val = val > 0.f ? val : 0.f;
+            if (a.relu) val = val > 0.f ? val : kLeak * val;
             const int64_t o = vo * a.out_ld + a.out_off + co;
             if (OUT_BF16) a.out_bf[o] = (__bf16)val;
             else a.out_f[o] = a.accumulate ? a.out_f[o] + val : val;
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(256) void k_upsample_bwd(FView dout, float *din, in
     din[v * din_ld + c] = s;
 }
 
-// dZ = dY * [Y > 0] -> bf16 (ReLU backward through the post-activation value)
+// dZ = dY * (Y > 0 ? 1 : leak) -> bf16 (leaky-ReLU backward through the post-activation value)
 __global__ __launch_bounds__(256) void k_relu_bwd(FView dy, View y, __bf16 *dz, int C, int64_t nvox)
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -265,7 +268,7 @@ __global__ __launch_bounds__(256) void k_relu_bwd(FView dy, View y, __bf16 *dz, 
     const int64_t v = e / C;
     const float g = dy.p[v * dy.ld + dy.off + c];
     const float yv = bf2f(y.p[v * y.ld + y.off + c]);
-    dz[v * C + c] = (__bf16)(yv > 0.f ? g : 0.f);
+    dz[v * C + c] = (__bf16)(yv > 0.f ? g : kLeak * g);
 }
 
 // Output head: p = sigmoid(sum_c w[c] h[c] + b); loss = mean (p - t)^2 +

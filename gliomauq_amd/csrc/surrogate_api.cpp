@@ -3,7 +3,7 @@
 // Network (64^3, channels-last bf16 activations, FP32 master weights, Adam):
 //   input 8 ch = {4 parameters broadcast, u0, pw, pg, 1}
 //   E1 conv 8->16 @64   E2 conv 16->32 @32   E3 conv 32->64 @16   E4 conv 64->128 @8
-//   (2x average pooling between levels; ReLU after every conv)
+//   (2x average pooling between levels; leaky ReLU (0.05) after every conv)
 //   D3 conv [up(E4) | E3] 192->64 @16   D2 conv [up(D3) | E2] 96->32 @32
 //   D1 conv [up(D2) | E1] 48->16 @64    head 1x1x1 16->1, sigmoid
 // Loss: mean (p - t)^2 + lam * mean BCE(sigmoid(kappa (p - th)), [t > th]).
@@ -143,7 +143,7 @@ struct gq_sur {
             maxpart = std::max(maxpart, nch * n);
         }
         for (int c = 0; c < 16; c++) hw[c] = 0.25f * nd(rng);
-        hb = 0.f;
+        hb = -3.f;  // start near the background value (most voxels hold no tumour)
         std::fill(hmw, hmw + 17, 0.f);
         std::fill(hvw, hvw + 17, 0.f);
         d_hw = A<float>(16);

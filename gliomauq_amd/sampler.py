@@ -90,8 +90,8 @@ class SurrogatePosterior:
     """Log-posterior with the surrogate's prediction in place of the solver:
     the same parameters, prior box and flip-rate likelihood, observations on
     the surrogate's 64^3 grid.  The surrogate inputs are the normalised
-    (log Dw, log rho, A) and the Gaussian initial condition at (x0, y0, z0)
-    on the 64^3 grid (coordinates of the model grid divided by `ratio`)."""
+    (log Dw, log rho, A) and a unit-peak Gaussian map at (x0, y0, z0) on the
+    64^3 grid (coordinates of the model grid divided by `ratio`)."""
 
     def __init__(self, sur, obs64, lo, hi, ratio, sigma_ic_model=2.0, lik=(0.7, 0.25, 0.05, 0.05),
                  norm_mu=None, norm_sd=None):
@@ -118,7 +118,7 @@ class SurrogatePosterior:
             d = [i - t[2 + a] / self.ratio for a in range(3)]
             d = [x - n * np.floor(x / n + 0.5) for x in d]
             dz, dy, dx = np.meshgrid(d[2], d[1], d[0], indexing="ij")
-            u0[b] = t[5] * np.exp(-(dx * dx + dy * dy + dz * dz) / (2 * self.sigma ** 2))
+            u0[b] = np.exp(-(dx * dx + dy * dy + dz * dz) / (2 * self.sigma ** 2))  # position map, unit peak
         return p, u0
 
     def logpost(self, th):

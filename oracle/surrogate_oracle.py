@@ -16,6 +16,7 @@ import numpy as np
 
 SPEC = [(8, 16), (16, 32), (32, 64), (64, 128), (192, 64), (96, 32), (48, 16)]  # E1 E2 E3 E4 D3 D2 D1
 TH, KAPPA, LAM = 0.25, 20.0, 0.1
+LEAK = 0.05  # leaky ReLU slope
 
 
 def bf16(x):
@@ -130,7 +131,7 @@ def forward_backward(theta, params, u0, pw, pg, target=None, bf=True):
     convs, (hw, hb) = unpack(np.asarray(theta, np.float64))
     W = [R(w) for w, _ in convs]
     Bs = [b for _, b in convs]
-    relu = lambda a: np.maximum(a, 0.0)  # noqa: E731
+    relu = lambda a: np.where(a > 0, a, LEAK * a)  # noqa: E731
     x0 = R(assemble(params, u0, pw, pg))
     e1 = R(relu(conv3(x0, W[0], Bs[0])))
     p1 = R(pool(e1))
@@ -162,7 +163,7 @@ def forward_backward(theta, params, u0, pw, pg, target=None, bf=True):
     grads = [None] * 7
 
     def layer(i, dy, y, x):
-        dz = R(dy * (y > 0))
+        dz = R(dy * np.where(y > 0, 1.0, LEAK))
         grads[i] = (conv3_wgrad(x, dz), dz.sum(axis=(0, 1, 2, 3)))
         return conv3_bwd_data(dz, W[i], x.shape[4])
 
