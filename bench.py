@@ -37,6 +37,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def pmc_traffic_ratio():
+    """HBM traffic / algorithmic bytes of k_fk_step from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs of this bench,
+    calibrated on a copy kernel of known size); None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_fk_step_adaptive256_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("{") and "traffic_over_algorithmic" in line:
+                return float(json.loads(line)["traffic_over_algorithmic"])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, n_steps=200):
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)
@@ -616,6 +630,8 @@ def main():
     value = cells / wall
     kernel_s = stats["kernel_ms"] * 1e-3
     achieved = 16.0 * cells / kernel_s / 1e9
+    bytes_per_launch = 16.0 * cells / max(stats["launches"], 1)
+    ratio = pmc_traffic_ratio()
     res = {
         "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
         "value": value,
@@ -633,8 +649,13 @@ def main():
                                "Interp4 tol 1e-3/1e-4 every 10 steps, FP32, %d steps of dt=%.4f d" % (args.steps, dt),
                    "n": n, "seed": args.seed, "final_blocks": g.n_blocks(), "loglik": float(ll[0])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (None if ratio is None else ratio * bytes_per_launch),
+                     "traffic_over_algorithmic": ratio,
+                     "traffic_source": "profiles/k_fk_step_adaptive256_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
+                                       "WRITE_SIZE, calibrated on a copy kernel)",
                      "kernel": "k_fk_step", "kernel_avg_us": stats["kernel_ms"] * 1e3 / max(stats["launches"], 1),
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
                      "algorithmic_bytes_per_cell": 16},
     }
     res["breakdown"] = {"fk_steps_s": t_steps, "adapt_s": t_adapt, "step_kernel_s": kernel_s,

@@ -143,7 +143,7 @@ struct gq_ctx {
     bool autotune = true;
     std::map<int, int> zt_cache;
     int last_zt = 16;
-    double tune_us[4] = {0, 0, 0, 0};
+    double tune_us[5] = {0, 0, 0, 0, 0};
 
     // host-side time accounting (seconds) and counts
     double t_tables = 0, t_plan = 0, t_fwt = 0, t_split = 0, t_collapse = 0, t_tune = 0;
@@ -728,7 +728,7 @@ struct gq_ctx {
         if (it != zt_cache.end()) return it->second;
         if (!autotune) return zt_heuristic(nwork);
         ScopeTimer tm(t_tune);
-        static const int cand[4] = {16, 8, 4, 2};
+        static const int cand[5] = {16, 8, 4, 2, 1};
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
@@ -738,7 +738,7 @@ struct gq_ctx {
             HIPCHK(gq::launch_face_ghosts(phi[cur], d_face_ptr.p, d_face_src.p, d_face_w.p, d_gphi.p, nrows, st));
         int best = 16;
         float best_ms = 1e30f;
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < 5; k++) {
             for (int r = 0; r < 2; r++) HIPCHK(gq::launch_fk_step(a, cand[k], st));
             HIPCHK(hipEventRecord(e0, st));
             const int reps = 5;
@@ -1743,8 +1743,8 @@ gq_status gq_last_step_timing(gq_ctx *ctx, double *ms, int *launches, int64_t *c
 gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt)
 {
     return guard(ctx, [&] {
-        if (mode == 2 && fixed_zt != 16 && fixed_zt != 8 && fixed_zt != 4 && fixed_zt != 2)
-            throw gq_error(GQ_ERR_ARG, "slab depth must be 16, 8, 4 or 2");
+        if (mode == 2 && fixed_zt != 16 && fixed_zt != 8 && fixed_zt != 4 && fixed_zt != 2 && fixed_zt != 1)
+            throw gq_error(GQ_ERR_ARG, "slab depth must be 16, 8, 4, 2 or 1 (column streaming)");
         ctx->zt_cache.clear();
         ctx->autotune = mode == 1;
         if (mode == 2)
@@ -1752,12 +1752,12 @@ gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt)
     });
 }
 
-gq_status gq_autotune_info(gq_ctx *ctx, int *zt, double us[4])
+gq_status gq_autotune_info(gq_ctx *ctx, int *zt, double us[5])
 {
     return guard(ctx, [&] {
         if (zt) *zt = ctx->last_zt;
         if (us)
-            for (int k = 0; k < 4; k++) us[k] = ctx->tune_us[k];
+            for (int k = 0; k < 5; k++) us[k] = ctx->tune_us[k];
     });
 }
 

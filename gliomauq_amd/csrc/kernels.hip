@@ -21,6 +21,7 @@ namespace {
 constexpr int B = 16;
 constexpr int B3 = 4096;
 constexpr int LW = B + 2;  // step lab width (halo 1)
+constexpr int PLW = B + 2; // plane width of the column-streaming step
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -246,6 +247,146 @@ __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw,
             }
         }
         fik[((size_t)slot * 6 + f) * 256 + t] = inv_k(wm, gm, Dw, Dg);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The same fused step streaming z: one workgroup per block, thread = one (x,y)
+// column of 16 cells.  The column's phi and 1/K of planes z-1, z, z+1 live in
+// registers; only the current plane (with its x/y halo) is staged in LDS,
+// double-buffered (one barrier per plane), so a workgroup needs 5 KB of LDS
+// and many blocks stay in flight.  The arithmetic per cell is k_fk_step's.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
+{
+    __shared__ float s_phi[2][PLW * PLW];
+    __shared__ float s_ik[2][PLW * PLW];
+    const int item = xcd_remap(blockIdx.x, gridDim.x);
+    if (item >= a.nwork) return;
+    const int slot = a.work[item];
+    const int t = threadIdx.x;
+    const int tx = t & 15, ty = t >> 4;
+    const size_t base = (size_t)slot * B3;
+    float Dw = a.Dw, Dg = a.Dg, rho = a.rho, dt = a.dt;
+    if (a.sprm) {
+        const float *q = a.sprm + 4 * a.slot_sample[slot];
+        Dw = q[0];
+        Dg = q[1];
+        rho = q[2];
+        dt = q[3];
+    }
+    const int rbase = a.sfbase[a.slot_sample[slot]];
+    // halo role of this thread for the x/y faces: t < 64 -> face t/16, coordinate t%16
+    const int hf = t >> 4, hu = t & 15;
+    int hnb = -1, hrow = 0;
+    if (t < 64) {
+        hnb = a.fnbr[slot * 6 + hf];
+        if (hnb < 0) hrow = rbase + a.fbase[slot * 6 + hf];
+    }
+    // z-1 plane of this column
+    float pB, kB;
+    {
+        const int nb = a.fnbr[slot * 6 + 4];
+        if (nb >= 0) pB = a.phi_in[(size_t)nb * B3 + nbr_face_cell(4, tx, ty)];
+        else pB = a.gphi[rbase + a.fbase[slot * 6 + 4] + t];
+        kB = a.fik[((size_t)slot * 6 + 4) * 256 + t];
+    }
+    float pC = a.phi_in[base + t];
+    float bC;
+    float kC;
+    {
+        const float wm = a.pw[base + t], gm = a.pg[base + t];
+        bC = wm + gm;
+        kC = inv_k(wm, gm, Dw, Dg);
+    }
+    // z+1 plane (register prefetch)
+    float pF = a.phi_in[base + 256 + t];
+    float wF = a.pw[base + 256 + t], gF = a.pg[base + 256 + t];
+    const float ih2 = a.ih2_lvl[a.level[slot]];
+    for (int k = 0; k < B; k++) {
+        const int buf = k & 1;
+        float *sp = s_phi[buf];
+        float *sk = s_ik[buf];
+        const int c = (tx + 1) + PLW * (ty + 1);
+        sp[c] = pC;
+        sk[c] = kC;
+        if (t < 64) {
+            float pv;
+            if (hnb >= 0) pv = a.phi_in[(size_t)hnb * B3 + nbr_face_cell(hf, hu, k)];
+            else pv = a.gphi[hrow + hu + 16 * k];
+            const float kv = a.fik[((size_t)slot * 6 + hf) * 256 + hu + 16 * k];
+            int l;
+            switch (hf) {
+            case 0: l = 0 + PLW * (hu + 1); break;
+            case 1: l = (B + 1) + PLW * (hu + 1); break;
+            case 2: l = (hu + 1); break;
+            default: l = (hu + 1) + PLW * (B + 1); break;
+            }
+            sp[l] = pv;
+            sk[l] = kv;
+        }
+        // plane k+1's 1/K for this column (pF, wF, gF were loaded one plane ahead)
+        float kF;
+        if (k + 1 < B) kF = inv_k(wF, gF, Dw, Dg);
+        else {
+            const int nb = a.fnbr[slot * 6 + 5];
+            if (nb >= 0) pF = a.phi_in[(size_t)nb * B3 + nbr_face_cell(5, tx, ty)];
+            else pF = a.gphi[rbase + a.fbase[slot * 6 + 5] + t];
+            kF = a.fik[((size_t)slot * 6 + 5) * 256 + t];
+        }
+        // prefetch plane k+2
+        float pN = 0.f, wN = 0.f, gN = 0.f;
+        if (k + 2 < B) {
+            const size_t i = base + (k + 2) * 256 + t;
+            pN = a.phi_in[i];
+            wN = a.pw[i];
+            gN = a.pg[i];
+        }
+        __syncthreads();
+        const float pc = pC;
+        const float pW = sp[c - 1], pE = sp[c + 1];
+        const float pS = sp[c - PLW], pN2 = sp[c + PLW];
+        float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
+        if (bC + pc > 0.f) {  // tissue = p_w + p_g + phi > 0 (:155-157)
+            const float kc = kC;
+            d0 = 2.f * frcp(kc + sk[c - 1]);
+            d1 = 2.f * frcp(kc + sk[c + 1]);
+            d2 = 2.f * frcp(kc + sk[c - PLW]);
+            d3 = 2.f * frcp(kc + sk[c + PLW]);
+            d4 = 2.f * frcp(kc + kB);
+            d5 = 2.f * frcp(kc + kF);
+        }
+        if (d0 == 0.f && pW > 0.f) d1 *= 2.f;
+        if (d1 == 0.f && pE > 0.f) d0 *= 2.f;
+        if (d2 == 0.f && pS > 0.f) d3 *= 2.f;
+        if (d3 == 0.f && pN2 > 0.f) d2 *= 2.f;
+        if (d4 == 0.f && pB > 0.f) d5 *= 2.f;
+        if (d5 == 0.f && pF > 0.f) d4 *= 2.f;
+        const float sdf = d0 + d1 + d2 + d3 + d4 + d5;
+        const float reac = rho * pc * (1.f - pc);
+        float dphi;
+        if (__builtin_isfinite(sdf)) {
+            const float diff = d0 * (pW - pc) + d1 * (pE - pc) + d2 * (pS - pc) + d3 * (pN2 - pc) +
+                               d4 * (pB - pc) + d5 * (pF - pc);
+            dphi = ih2 * diff + reac;
+        } else {
+            const float in = ih2 * (d0 * pW + d1 * pE + d2 * pS + d3 * pN2 + d4 * pB + d5 * pF);
+            const float out = -(sdf * pc * ih2);
+            dphi = out + in + reac;
+        }
+        float pn = pc + dt * dphi;
+        pn = (0.f < pn) ? pn : 0.f;
+        pn = (pn < 1.f) ? pn : 1.f;
+        a.phi_out[base + k * 256 + t] = pn;
+        // shift the column window
+        pB = pC;
+        kB = kC;
+        pC = pF;
+        kC = kF;
+        bC = wF + gF;
+        pF = pN;
+        wF = wN;
+        gF = gN;
     }
 }
 
@@ -1025,6 +1166,11 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *partial, int
 hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st)
 {
     if (a.nwork <= 0) return hipSuccess;
+    if (zt == 1) {
+        const int grid = (a.nwork + 7) & ~7;
+        hipLaunchKernelGGL(k_fk_step_col, dim3(grid), dim3(256), 0, st, a);
+        return hipGetLastError();
+    }
     const int items = a.nwork * (16 / zt);
     const int grid = (items + 7) & ~7;
     switch (zt) {

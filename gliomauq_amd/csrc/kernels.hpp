@@ -79,7 +79,8 @@ hipError_t launch_ic_gaussian(float *phi, const int *work, int nwork, const int 
 hipError_t launch_ic_grad(const float *lam0, const int *work, int nwork, const int *index, const int *slot_sample,
                           const double *icp, int gpd, double *out, hipStream_t st);
 
-// zt: slab depth (16, 8, 4 or 2 planes per workgroup)
+// zt: slab depth (16, 8, 4 or 2 planes per workgroup), or 1: one workgroup per
+// block streaming its planes through registers (k_fk_step_col)
 hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st);
 hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *src, const float *w, float *out,
                               int nrows, hipStream_t st);

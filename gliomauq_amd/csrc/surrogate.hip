@@ -8,7 +8,7 @@
 // (v_mfma_f32_16x16x32_bf16, FP32 accumulation): M = voxels, N = output
 // channels, K = 27 taps x input channels, k = tap * Cin + ci with
 // tap = (dz+1)*9 + (dy+1)*3 + (dx+1), zero padding outside the volume.
-// Weight gradients accumulate in FP32 on the vector ALUs.  Activation:
+// Weight gradients are GEMMs over the voxels on the same MFMA.  Activation:
 // leaky ReLU (slope 0.05), so no unit can die for good during training.
 #include <hip/hip_runtime.h>
 
@@ -101,88 +101,120 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a)
     }
 }
 
-// Weight gradient of a 3^3 convolution: dW[co][tap*Cin + ci] (FP32) summed
-// over the voxels of the batch of dZ[v][co] * X[v + tap][ci].  Workgroup
-// (tap, chunk): the chunk's voxels are staged 64 at a time in LDS; each
-// thread owns 4x4 (co, ci) pairs.  Partial sums per chunk, reduced in chunk
-// order by k_reduce_chunks (deterministic).
-__global__ __launch_bounds__(256) void k_conv3_wgrad(WgradArgs a)
+// Weight gradient on the MFMA: dW[co][col] = sum_v dZ[v][co] X[v + tap(col)][ci(col)]
+// with col = tap*Cin + ci, as a GEMM with M = Cout, N = columns, K = voxels.
+// Workgroup (column block of 8192/Cout columns, 4096-voxel chunk): per 32
+// voxels, dZ^T [Cout][32] and X^T [columns][32] are staged in LDS as bf16
+// (the gather applies the tap offset and the zero padding); each wave owns
+// 8 of the 32 16x16 output tiles and issues one v_mfma_f32_16x16x32_bf16 per
+// tile.  Partial sums per chunk (FP32), reduced in chunk order.  The column
+// block 0 also sums dZ per channel (bias gradient partials).
+__global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(WgradArgs a, float *bias_partial)
 {
-    constexpr int VT = 64;
-    __shared__ float s_x[VT * 64];   // [voxel][ci] of one 64-channel ci block
-    __shared__ float s_dz[VT * 64];  // [voxel][co] of one 64-channel co block
-    const int tap = blockIdx.x;
-    const int chunk = blockIdx.y;
-    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-    const int64_t D = a.x.D, D3 = D * D * D, nvox = D3 * a.batch;
-    const int64_t v0 = (int64_t)chunk * a.chunk_vox, v1 = v0 + a.chunk_vox < nvox ? v0 + a.chunk_vox : nvox;
+    constexpr int KS = 32;
+    __shared__ __attribute__((aligned(16))) __bf16 s_dz[128 * KS];
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[512 * KS];
+    __shared__ float s_b[256];
+    __shared__ int s_ci[512];       // input channel of each column of the block
+    __shared__ int s_off[512];      // packed tap offset of each column: (dx+1) | (dy+1) << 2 | (dz+1) << 4, -1 past the end
+    __shared__ int s_vx[KS], s_vy[KS], s_vz[KS], s_vb[KS];  // coordinates of the K-step's voxels (-1: past the end)
     const int Cin = a.cin, Cout = a.cout;
-    const int t = threadIdx.x;
-    for (int cob = 0; cob < Cout; cob += 64) {
-        const int ncob = Cout - cob < 64 ? Cout - cob : 64;
-        for (int cib = 0; cib < Cin; cib += 64) {
-            const int ncib = Cin - cib < 64 ? Cin - cib : 64;
-            // thread tile: co = cob + 4*(t/16) + 0..3, ci = cib + 4*(t%16) + 0..3
-            const int tco = 4 * (t >> 4), tci = 4 * (t & 15);
-            float acc[4][4];
-#pragma unroll
-            for (int p = 0; p < 4; p++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) acc[p][q] = 0.f;
-            for (int64_t vs = v0; vs < v1; vs += VT) {
-                __syncthreads();
-                for (int e = t; e < VT * 64; e += 256) {
-                    const int vi = e >> 6, c = e & 63;
-                    const int64_t v = vs + vi;
-                    float xv = 0.f, zv = 0.f;
-                    if (v < v1) {
-                        const int64_t b = v / D3, r = v - b * D3;
-                        const int x = (int)(r % D), y = (int)((r / D) % D), z = (int)(r / (D * D));
-                        const int xx = x + dx, yy = y + dy, zz = z + dz;
-                        if (c < ncib && xx >= 0 && xx < D && yy >= 0 && yy < D && zz >= 0 && zz < D)
-                            xv = bf2f(a.x.p[(b * D3 + xx + D * (yy + D * (int64_t)zz)) * a.x.ld + a.x.off + cib + c]);
-                        if (c < ncob) zv = bf2f(a.dz.p[v * a.dz.ld + a.dz.off + cob + c]);
-                    }
-                    s_x[e] = xv;
-                    s_dz[e] = zv;
-                }
-                __syncthreads();
-                for (int vi = 0; vi < VT; vi++) {
-                    float xr[4], zr[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) xr[q] = s_x[vi * 64 + tci + q];
-#pragma unroll
-                    for (int p = 0; p < 4; p++) zr[p] = s_dz[vi * 64 + tco + p];
-#pragma unroll
-                    for (int p = 0; p < 4; p++)
-#pragma unroll
-                        for (int q = 0; q < 4; q++) acc[p][q] += zr[p] * xr[q];
-                }
-            }
-            const int K = 27 * Cin;
-#pragma unroll
-            for (int p = 0; p < 4; p++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int co = tco + p, ci = tci + q;
-                    if (co < ncob && ci < ncib)
-                        a.partial[(int64_t)chunk * Cout * K + (int64_t)(cob + co) * K + tap * Cin + cib + ci] =
-                            acc[p][q];
-                }
+    const int NCB = 8192 / Cout;       // columns per workgroup (32 tiles of 16x16)
+    const int NT = NCB / 16;
+    const int lgN = 31 - __builtin_clz(NCB), lgC = 31 - __builtin_clz(Cout);
+    const int Kw = 27 * Cin;
+    const int c0 = blockIdx.x * NCB;
+    const int chunk = blockIdx.y;
+    const int D = a.x.D, lgD = 31 - __builtin_clz(D);
+    const int64_t nvox = ((int64_t)D * D * D) * a.batch;
+    const int64_t v0 = (int64_t)chunk * a.chunk_vox, v1 = v0 + a.chunk_vox < nvox ? v0 + a.chunk_vox : nvox;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, kq = lane >> 4;
+    for (int cl = t; cl < NCB; cl += 256) {
+        const int col = c0 + cl;
+        if (col < Kw) {
+            const int tap = col / Cin;
+            s_ci[cl] = col - tap * Cin;
+            s_off[cl] = (tap % 3) | (((tap / 3) % 3) << 2) | ((tap / 9) << 4);
+        } else {
+            s_ci[cl] = 0;
+            s_off[cl] = -1;
         }
     }
-}
-
-// Bias gradient partials: db[co] over the chunk's voxels.
-__global__ __launch_bounds__(256) void k_bias_grad(BiasGradArgs a)
-{
-    const int chunk = blockIdx.x;
-    const int64_t D3 = (int64_t)a.dz.D * a.dz.D * a.dz.D, nvox = D3 * a.batch;
-    const int64_t v0 = (int64_t)chunk * a.chunk_vox, v1 = v0 + a.chunk_vox < nvox ? v0 + a.chunk_vox : nvox;
-    for (int co = threadIdx.x; co < a.cout; co += 256) {
-        float s = 0.f;
-        for (int64_t v = v0; v < v1; v++) s += bf2f(a.dz.p[v * a.dz.ld + a.dz.off + co]);
-        a.partial[(int64_t)chunk * a.cout + co] = s;
+    f32x4 acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    for (int64_t vs = v0; vs < v1; vs += KS) {
+        if (t < KS) {
+            const int64_t v = vs + t;
+            if (v < v1) {
+                const int64_t r = v & (((int64_t)1 << (3 * lgD)) - 1);
+                s_vb[t] = (int)(v >> (3 * lgD));
+                s_vx[t] = (int)(r & (D - 1));
+                s_vy[t] = (int)((r >> lgD) & (D - 1));
+                s_vz[t] = (int)(r >> (2 * lgD));
+            } else {
+                s_vx[t] = -1;
+            }
+        }
+        __syncthreads();
+        // dZ^T: element e -> voxel e / Cout, channel e % Cout (coalesced rows)
+        for (int e = t; e < Cout * KS; e += 256) {
+            const int k = e >> lgC, co = e & (Cout - 1);
+            const int64_t v = vs + k;
+            __bf16 val = (__bf16)0.0f;
+            if (v < v1) val = a.dz.p[v * a.dz.ld + a.dz.off + co];
+            s_dz[co * KS + k] = val;
+            if (blockIdx.x == 0) bsum += bf2f(val);
+        }
+        // X^T: element e -> voxel e / NCB, column e % NCB
+        for (int e = t; e < NCB * KS; e += 256) {
+            const int k = e >> lgN, cl = e & (NCB - 1);
+            const int off = s_off[cl];
+            __bf16 val = (__bf16)0.0f;
+            const int vx = s_vx[k];
+            if (off >= 0 && vx >= 0) {
+                const int x = vx + (off & 3) - 1, y = s_vy[k] + ((off >> 2) & 3) - 1, z = s_vz[k] + (off >> 4) - 1;
+                if ((unsigned)x < (unsigned)D && (unsigned)y < (unsigned)D && (unsigned)z < (unsigned)D) {
+                    const int64_t vn = ((int64_t)s_vb[k] << (3 * lgD)) + x + ((int64_t)y << lgD) + ((int64_t)z << (2 * lgD));
+                    val = a.x.p[vn * a.x.ld + a.x.off + s_ci[cl]];
+                }
+            }
+            s_x[cl * KS + k] = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int tile = wave + 4 * j;  // 32 tiles: tile = mt * NT + nt
+            const int mt = tile / NT, nt = tile - mt * NT;
+            const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_dz[(16 * mt + li) * KS + 8 * kq]);
+            const bf16x8 vb = *reinterpret_cast<const bf16x8 *>(&s_x[(16 * nt + li) * KS + 8 * kq]);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc[j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int tile = wave + 4 * j;
+        const int mt = tile / NT, nt = tile - mt * NT;
+        const int col = c0 + 16 * nt + li;
+        if (col >= Kw) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int co = 16 * mt + 4 * kq + r;
+            a.partial[(int64_t)chunk * Cout * Kw + (int64_t)co * Kw + col] = acc[j][r];
+        }
+    }
+    if (blockIdx.x == 0) {
+        // threads t, t + Cout, ... hold partial sums of channel t % Cout
+        s_b[t] = bsum;
+        __syncthreads();
+        if (t < Cout) {
+            float sb = 0.f;
+            for (int q = t; q < 256; q += Cout) sb += s_b[q];
+            bias_partial[(int64_t)chunk * Cout + t] = sb;
+        }
     }
 }
 
@@ -448,15 +480,15 @@ hipError_t launch_conv3(const ConvArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_conv3_wgrad(const WgradArgs &a, int nchunks, hipStream_t st)
+hipError_t launch_conv3_wgrad_mfma(const WgradArgs &a, int nchunks, float *bias_partial, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_conv3_wgrad, dim3(27, nchunks), dim3(256), 0, st, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_bias_grad(const BiasGradArgs &a, int nchunks, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_bias_grad, dim3(nchunks), dim3(256), 0, st, a);
+    const int D = a.x.D;
+    if (a.cout % 16 != 0 || a.cout > 128 || 256 % a.cout != 0 || a.chunk_vox % 32 != 0 || D < 2 || (D & (D - 1)))
+        return hipErrorInvalidValue;
+    const int ncb = 8192 / a.cout;
+    const int64_t kw = 27 * (int64_t)a.cin;
+    const dim3 grid((unsigned)((kw + ncb - 1) / ncb), nchunks);
+    hipLaunchKernelGGL(k_conv3_wgrad_mfma, grid, dim3(256), 0, st, a, bias_partial);
     return hipGetLastError();
 }
 

@@ -42,13 +42,6 @@ struct WgradArgs {
     float *partial;  // [nchunks][cout][27*cin]
 };
 
-struct BiasGradArgs {
-    View dz;
-    int cout, batch;
-    int64_t chunk_vox;
-    float *partial;  // [nchunks][cout]
-};
-
 struct HeadArgs {
     View h;               // 16 channels
     const float *w;       // [16] (host-updated copy on the device)
@@ -62,8 +55,8 @@ struct HeadArgs {
 };
 
 hipError_t launch_conv3(const ConvArgs &a, hipStream_t st);
-hipError_t launch_conv3_wgrad(const WgradArgs &a, int nchunks, hipStream_t st);
-hipError_t launch_bias_grad(const BiasGradArgs &a, int nchunks, hipStream_t st);
+// MFMA version (Cout in {16, 32, 64, 128}); also writes bias partials [nchunks][cout]
+hipError_t launch_conv3_wgrad_mfma(const WgradArgs &a, int nchunks, float *bias_partial, hipStream_t st);
 hipError_t launch_reduce_chunks(const float *partial, int nchunks, int64_t n, float *out, hipStream_t st);
 hipError_t launch_avgpool(const View &in, const View &out, int C, int batch, hipStream_t st);
 hipError_t launch_avgpool_bwd(const float *dout, int64_t dout_ld, const FView &din, int C, int batch, hipStream_t st);

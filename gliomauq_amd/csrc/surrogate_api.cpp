@@ -71,8 +71,8 @@ struct gq_sur {
            *p3 = nullptr, *e4 = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr, *dz = nullptr;
     // gradients (FP32)
     float *dh = nullptr, *dcat1 = nullptr, *dcat2 = nullptr, *dcat3 = nullptr, *dd2 = nullptr, *dd3 = nullptr,
-          *de4 = nullptr, *dp3 = nullptr, *dp2 = nullptr, *dp1 = nullptr, *partial = nullptr, *hpart = nullptr,
-          *hred = nullptr;
+          *de4 = nullptr, *dp3 = nullptr, *dp2 = nullptr, *dp1 = nullptr, *partial = nullptr, *bpart = nullptr,
+          *hpart = nullptr, *hred = nullptr;
     int64_t partial_n = 0;
     // inputs
     float *pw = nullptr, *pg = nullptr, *prm = nullptr, *u0 = nullptr, *tgt = nullptr, *pred = nullptr;
@@ -115,7 +115,7 @@ struct gq_sur {
                                     {192, 64, 16}, {96, 32, 32}, {48, 16, 64}};
         std::mt19937_64 rng(seed);
         std::normal_distribution<float> nd(0.f, 1.f);
-        int64_t maxpart = 0;
+        int64_t maxpart = 0, maxbpart = 0;
         for (int l = 0; l < NCONV; l++) {
             Conv &c = cv[l];
             c.cin = spec[l][0];
@@ -141,6 +141,7 @@ struct gq_sur {
             const int64_t nvox = (int64_t)c.D * c.D * c.D * batch;
             const int64_t nch = (nvox + CHUNK - 1) / CHUNK;
             maxpart = std::max(maxpart, nch * n);
+            maxbpart = std::max(maxbpart, nch * c.cout);
         }
         for (int c = 0; c < 16; c++) hw[c] = 0.25f * nd(rng);
         hb = -3.f;  // start near the background value (most voxels hold no tumour)
@@ -172,6 +173,7 @@ struct gq_sur {
         dp1 = A<float>(V32 * 16);
         partial_n = std::max<int64_t>(maxpart, 1);
         partial = A<float>(partial_n);
+        bpart = A<float>(std::max<int64_t>(maxbpart, 1));
         hpart = A<float>((size_t)gq::sur::head_blocks(V64) * 18);
         hred = A<float>(18);
         pw = A<float>(64 * 64 * 64);
@@ -252,16 +254,9 @@ struct gq_sur {
         a.batch = B;
         a.chunk_vox = CHUNK;
         a.partial = partial;
-        SCHK(gq::sur::launch_conv3_wgrad(a, nch, st));
+        SCHK(gq::sur::launch_conv3_wgrad_mfma(a, nch, bpart, st));
         SCHK(gq::sur::launch_reduce_chunks(partial, nch, c.nw(), c.gw, st));
-        gq::sur::BiasGradArgs bgr{};
-        bgr.dz = a.dz;
-        bgr.cout = c.cout;
-        bgr.batch = B;
-        bgr.chunk_vox = CHUNK;
-        bgr.partial = partial;
-        SCHK(gq::sur::launch_bias_grad(bgr, nch, st));
-        SCHK(gq::sur::launch_reduce_chunks(partial, nch, c.cout, c.gb, st));
+        SCHK(gq::sur::launch_reduce_chunks(bpart, nch, c.cout, c.gb, st));
     }
 
     void upload_batch(const float *params, const float *u0h, const float *th, int B)

@@ -224,15 +224,16 @@ def set_step_timing(grid, per_kernel):
 
 
 def set_autotune(ctx, mode=1, fixed_zt=16):
-    """mode 1: autotune the step slab depth; 0: size rule; 2: fixed_zt."""
+    """mode 1: autotune the step slab depth (16, 8, 4, 2 planes, or 1 = column
+    streaming); 0: size rule; 2: fixed_zt."""
     ctx.call("gq_set_autotune", int(mode), int(fixed_zt))
 
 
 def autotune_info(ctx):
     zt = C.c_int()
-    us = np.zeros(4)
+    us = np.zeros(5)
     ctx.call("gq_autotune_info", C.byref(zt), ptr(us, C.c_double))
-    return zt.value, dict(zip((16, 8, 4, 2), us.tolist()))
+    return zt.value, dict(zip((16, 8, 4, 2, 1), us.tolist()))
 
 
 def host_stats(ctx):

@@ -63,7 +63,7 @@ def run_brain(d, tend, Dw=0.02, rho=0.05, ic=(0.45, 0.5, 0.52)):
     with open(os.path.join(d, "HGG_InputParameters.txt"), "w") as f:
         f.write("%r %r %r\n" % (Dw, rho, tend))
     np.array(list(ic) + [0.0] * 9, np.float32).tofile(os.path.join(d, "HGG_TumorIC.bin"))
-    r = subprocess.run([os.path.join(BIN, "gq_brain"), "-bpd", "2", "-maxlevel", "1", "-wavelets", "4",
+    r = subprocess.run([os.path.join(BIN, "gq_brain"), "-bpd", "8", "-maxlevel", "3", "-wavelets", "4",
                         "-adaptive", "0", "-anatomyDir", d + "/", "-pID", "1"], cwd=d, capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -74,16 +74,17 @@ def test_gq_brain_initial_condition_and_solve():
     d = tempfile.mkdtemp(prefix="gqbrain_")
     gm, wm, csf, t1, t2, _ = anatomy_files(d)
     ic = (0.45, 0.5, 0.52)
-    phi0, pw, pg = PO.patient_ic(gm, wm, csf, t1, t2, ic, 32)
-    assert (phi0 > 0).sum() > 10 and (pw > 0).sum() > 1000
+    n = 128
+    phi0, pw, pg = PO.patient_ic(gm, wm, csf, t1, t2, ic, n)
+    assert (phi0 > 0).sum() > 10 and (pw > 0).sum() > 10000
     u0 = run_brain(d, -1.0)  # while (t <= tend) never runs: the dump is the initial field
     assert np.array_equal(u0, phi0.astype(np.float64))
-    tend = 3.0
+    tend = 0.4
     u = run_brain(d, tend)
     L = 32 * 0.1
     Dw = float(np.float32(np.float32(0.02) / (L * L)))
     Dg = float(np.float32(0.1 * Dw))
-    h = 1.0 / 32
+    h = 1.0 / n
     dt = 0.99 * h * h / (6 * max(Dw, Dg))
     steps = 0
     t = 0.0

@@ -131,3 +131,29 @@ def test_block_processing_mirror(ctx):
             gq.fk_steps(g, 1e-5, 1e-6, 0.03, 0.5, 1)
         outs.append(g.dump_uniform())
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_step_variants_bit_identical():
+    """Every step tile the autotuner may pick (slab depths 16/8/4/2 and the
+    column-streaming kernel) gives the same bits on an adaptive grid with
+    cross-level faces, so the tuning never changes results."""
+    n = 64
+    pwm, pgm, u0 = uniform_case(n, seed=4)
+    u0 = np.maximum(u0, A.initial_condition(n, (n // 2, n // 2, n // 2), 0.6, 3.0))
+    outs = []
+    for zt in (16, 8, 4, 2, 1):
+        ctx = gq.Context(0)
+        gq.set_autotune(ctx, 2, zt)
+        g = gq.Grid(ctx, 4, 3, 4)
+        for name, v in (("phi", u0), ("pw", pwm), ("pg", pgm)):
+            g.write_volume(name, v)
+        gq.Science.AutomaticCompression(g, 1e-3, -1)  # coarse leaves far from the tumour
+        Dw = 0.0013 / L_CM ** 2 * 40
+        h = 1.0 / n
+        gq.fk_steps(g, Dw, 0.1 * Dw, 0.3, 0.9 * h * h / (6 * Dw), 7)
+        assert gq.autotune_info(ctx)[0] == zt
+        outs.append((g.n_blocks(), g.read("phi")))
+        ctx.close()
+    assert outs[0][0] < 64  # the grid is adaptive
+    for nb, a in outs[1:]:
+        assert nb == outs[0][0] and np.array_equal(a, outs[0][1])

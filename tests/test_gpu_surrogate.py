@@ -30,8 +30,8 @@ def batch(seed=0, B=1):
     for b in range(B):
         c = rng.uniform(20, 44, 3)
         r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
-        u0[b] = 0.1 * np.exp(-r2 / 8.0)
-        tgt[b] = 1.0 / (1.0 + np.exp((np.sqrt(r2) - 9.0 - 3 * params[b, 0]) / 1.5))
+        u0[b] = np.exp(-r2 / 8.0)
+        tgt[b] = 1.0 / (1.0 + np.exp((np.sqrt(r2) - 13.0 - 3 * params[b, 0]) / 1.5))
     return params, u0, pw, pg, tgt
 
 
@@ -63,6 +63,33 @@ def test_surrogate_matches_oracle():
     s.close()
 
 
+def test_surrogate_adam_steps_match_oracle():
+    """Three Adam steps on the device against Adam on the oracle's gradients
+    from the same starting parameters (the first Adam step moves every
+    parameter by ~lr * sign(g), so a wrong or missing update shows at once)."""
+    params, u0, pw, pg, tgt = batch(seed=1, B=1)
+    s = S.Surrogate(batch=1, seed=4)
+    s.set_anatomy(pw, pg)
+    th = s.get_params().astype(np.float64)
+    m = np.zeros_like(th)
+    v = np.zeros_like(th)
+    lr = 1e-3
+    for it in range(3):
+        s.train_step(params, u0, tgt, lr)
+        _, _, g = SO.forward_backward(th, params.astype(np.float64), u0.astype(np.float64), pw.astype(np.float64),
+                                      pg.astype(np.float64), tgt.astype(np.float64))
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        th = th - lr * (m / (1 - 0.9 ** (it + 1))) / (np.sqrt(v / (1 - 0.999 ** (it + 1))) + 1e-8)
+    got = s.get_params().astype(np.float64)
+    moved = np.abs(got - (th + 0))
+    step = 3 * lr
+    frac_bad = float(np.mean(moved > 0.5 * step))
+    print("params off by more than half the 3-step budget: %.4f%%" % (100 * frac_bad))
+    assert frac_bad < 1e-3  # only sign flips of near-zero gradients may disagree
+    s.close()
+
+
 def test_surrogate_training_and_reproducibility():
     params, u0, pw, pg, tgt = batch(seed=1, B=2)
     s = S.Surrogate(batch=2, seed=4)
@@ -70,9 +97,11 @@ def test_surrogate_training_and_reproducibility():
     p1 = s.predict(params, u0)
     p2 = s.predict(params, u0)
     assert np.array_equal(p1, p2)
-    losses = [s.train_step(params, u0, tgt, 2e-3) for _ in range(40)]
+    losses = [s.train_step(params, u0, tgt, 1e-3) for _ in range(80)]
     print("loss %.4e -> %.4e" % (losses[0], losses[-1]))
-    assert losses[-1] < 0.5 * losses[0]
+    # well below what a constant prediction can reach (the tumours fill ~5% of the volume)
+    const = min(np.mean((c - tgt) ** 2) for c in np.linspace(0, 0.3, 31))
+    assert losses[-1] < 0.8 * const
     t = s.timing()
     assert t["forward_ms"] > 0 and t["backward_ms"] > 0
     s.close()
