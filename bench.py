@@ -506,6 +506,59 @@ def surrogate_workload(args):
     print(json.dumps(res), flush=True)
 
 
+def tmcmc_workload(args):
+    """Transitional MCMC (the reference's sampler family) over the batched
+    likelihood: a population of parameter sets evaluated --tm-batch at a time
+    per ensemble call, tempered from the box prior to the posterior.  Uniform
+    64^3 grid, 500 steps, flip-rate likelihood against observations on a grid
+    coarser by --tm-stride."""
+    import gliomauq_amd as gq
+    from gliomauq_amd import anatomy as A
+    from gliomauq_amd import sampler as SM
+
+    n, steps = 64, 500
+    rng = np.random.default_rng(args.seed + 21)
+    pwm, pgm, u0, c = A.config_inputs(n, L_CM, args.seed)
+    h = 1.0 / n
+    dt = 0.9 * h * h / (6 * 1.5e-2 / L_CM ** 2)
+    tt = np.array([np.log(0.0013 * 1.25), np.log(0.025 * 0.8) + 1.0, c[2] + 0.3, c[1] - 0.4, c[0] + 0.2, 0.1])
+    lo = np.array([np.log(2e-4), np.log(2e-3), c[2] - 4, c[1] - 4, c[0] - 4, 0.05])
+    hi = np.array([np.log(1.5e-2), np.log(0.2), c[2] + 4, c[1] + 4, c[0] + 4, 0.2])
+    ctx = gq.Context(0)
+    truth = gq.Ensemble(ctx, 1, n // 16, 2, 4)
+    truth.write_volume("pw", pwm)
+    truth.write_volume("pg", pgm)
+    truth.ic_gaussian(np.array([[tt[2], tt[3], tt[4], tt[5], 2.0]]))
+    Dt = np.exp(tt[0]) / L_CM ** 2
+    truth.run(np.array([[Dt, 0.1 * Dt, np.exp(tt[1]), dt]]), [steps], adapt_every=0)
+    s_ = args.tm_stride
+    obs = tuple(o[::s_, ::s_, ::s_].copy() for o in A.observations(truth.sample(0).dump_uniform(), seed=args.seed))
+    del truth
+    post = SM.Posterior(gq.Context(0), pwm, pgm, obs, steps, dt, lo, hi, batch=args.tm_batch)
+
+    def prior(m, r):
+        return lo + (hi - lo) * r.random((m, 6))
+
+    log("[bench] tmcmc population %d, batch %d" % (args.tm_pop, args.tm_batch))
+    t0 = time.perf_counter()
+    th, ll, stages = SM.tmcmc(post, prior, args.tm_pop, rng, cov_scale=0.5, moves=args.tm_moves,
+                              max_stages=args.tm_stages)
+    wall = time.perf_counter() - t0
+    res = {
+        "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
+        "value": post.n_evals / wall, "unit": "likelihood evaluations/s", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "TMCMC, population %d, %d Metropolis moves per stage, uniform 64^3, 500 steps, "
+                               "observations 1/%d per axis" % (args.tm_pop, args.tm_moves, s_), "seed": args.seed},
+        "tmcmc": {"stages": len(stages), "final_q": stages[-1]["q"] if stages else 0.0,
+                  "accept": [round(st["accept"], 3) for st in stages], "evaluations": post.n_evals, "wall_s": wall,
+                  "posterior_mean": th.mean(0).tolist(), "posterior_sd": th.std(0).tolist(),
+                  "theta_true": tt.tolist()},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -516,7 +569,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="adaptive",
-                    choices=["adaptive", "ensemble", "adjoint", "mcmc", "surrogate"])
+                    choices=["adaptive", "ensemble", "adjoint", "mcmc", "surrogate", "tmcmc"])
+    ap.add_argument("--tm-pop", type=int, default=512)
+    ap.add_argument("--tm-batch", type=int, default=64)
+    ap.add_argument("--tm-moves", type=int, default=3)
+    ap.add_argument("--tm-stages", type=int, default=40)
+    ap.add_argument("--tm-stride", type=int, default=4)
     ap.add_argument("--sur-steps", type=int, default=2000)
     ap.add_argument("--sur-fields", type=int, default=1024)
     ap.add_argument("--sur-epochs", type=int, default=6)
@@ -546,6 +604,8 @@ def main():
         return mcmc_workload(args)
     if args.workload == "surrogate":
         return surrogate_workload(args)
+    if args.workload == "tmcmc":
+        return tmcmc_workload(args)
 
     import gliomauq_amd as gq
     from gliomauq_amd import anatomy as A

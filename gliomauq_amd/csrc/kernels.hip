@@ -52,11 +52,70 @@ __device__ __forceinline__ int nbr_face_cell(int f, int u, int v)
 // 1/K = (pw+pg) / (pw*Dw + pg*Dg): the reciprocal of the tissue-normalised
 // diffusivity of Case2 (Glioma_ReactionDiffusionOperator.h:174-188).  0/0 ->
 // NaN (CSF / outside), as in the reference.
+// Contraction is off in the per-cell arithmetic and every FMA is explicit, so
+// each step kernel (whatever its tiling, unrolling or scheduling) rounds
+// exactly alike and the tile choice never changes a bit.
 __device__ __forceinline__ float inv_k(float w, float g, float Dw, float Dg)
 {
+#pragma clang fp contract(off)
     const float b = w + g;
-    const float D = w * Dw + g * Dg;
+    const float D = __builtin_fmaf(g, Dg, w * Dw);
     return b * frcp(D);
+}
+
+// One cell of the fused RD operator + UpdateTumor (Case2,
+// Glioma_ReactionDiffusionOperator.h:150-210, UpdateTumor :224-236): phi of
+// the cell and its six face neighbours (-x,+x,-y,+y,-z,+z), their 1/K, and
+// p_w + p_g of the cell.
+__device__ __forceinline__ float fk_cell(float pc, float pW, float pE, float pS, float pN, float pB, float pF,
+                                         float kc, float kW, float kE, float kS, float kN, float kB, float kF,
+                                         float bsum, float ih2, float rho, float dt)
+{
+#pragma clang fp contract(off)
+    float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
+    if (bsum + pc > 0.f) {  // tissue = p_w + p_g + phi > 0 (:155-157)
+        d0 = 2.f * frcp(kc + kW);
+        d1 = 2.f * frcp(kc + kE);
+        d2 = 2.f * frcp(kc + kS);
+        d3 = 2.f * frcp(kc + kN);
+        d4 = 2.f * frcp(kc + kB);
+        d5 = 2.f * frcp(kc + kF);
+    }
+    // Neumann doubling, sequential as the reference (:198-203)
+    if (d0 == 0.f && pW > 0.f) d1 *= 2.f;
+    if (d1 == 0.f && pE > 0.f) d0 *= 2.f;
+    if (d2 == 0.f && pS > 0.f) d3 *= 2.f;
+    if (d3 == 0.f && pN > 0.f) d2 *= 2.f;
+    if (d4 == 0.f && pB > 0.f) d5 *= 2.f;
+    if (d5 == 0.f && pF > 0.f) d4 *= 2.f;
+    const float sdf = d0 + d1 + d2 + d3 + d4 + d5;
+    const float reac = rho * pc * (1.f - pc);
+    float dphi;
+    if (__builtin_isfinite(sdf)) {
+        // flux form sum_k df_k (phi_k - phi_c): the same quantity as the
+        // reference's in + out, without its cancellation in FP32
+        float diff = d0 * (pW - pc);
+        diff = __builtin_fmaf(d1, pE - pc, diff);
+        diff = __builtin_fmaf(d2, pS - pc, diff);
+        diff = __builtin_fmaf(d3, pN - pc, diff);
+        diff = __builtin_fmaf(d4, pB - pc, diff);
+        diff = __builtin_fmaf(d5, pF - pc, diff);
+        dphi = __builtin_fmaf(ih2, diff, reac);
+    } else {  // NaN/inf coefficients: the reference's literal form
+        float s = d0 * pW;
+        s = __builtin_fmaf(d1, pE, s);
+        s = __builtin_fmaf(d2, pS, s);
+        s = __builtin_fmaf(d3, pN, s);
+        s = __builtin_fmaf(d4, pB, s);
+        s = __builtin_fmaf(d5, pF, s);
+        const float in = ih2 * s;
+        const float out = -(sdf * pc * ih2);
+        dphi = out + in + reac;
+    }
+    float pn = __builtin_fmaf(dt, dphi, pc);
+    pn = (0.f < pn) ? pn : 0.f;  // std::max((Real)0, phi): NaN -> 0
+    pn = (pn < 1.f) ? pn : 1.f;  // std::min((Real)1, phi)
+    return pn;
 }
 
 }  // namespace
@@ -156,40 +215,8 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
         const float pW = s_phi[c - 1], pE = s_phi[c + 1];
         const float pS = s_phi[c - LW], pN = s_phi[c + LW];
         const float pB = s_phi[c - PL], pF = s_phi[c + PL];
-        float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
-        if (breg[k] + pc > 0.f) {  // tissue = p_w + p_g + phi > 0 (:155-157)
-            const float kc = s_ik[c];
-            d0 = 2.f * frcp(kc + s_ik[c - 1]);
-            d1 = 2.f * frcp(kc + s_ik[c + 1]);
-            d2 = 2.f * frcp(kc + s_ik[c - LW]);
-            d3 = 2.f * frcp(kc + s_ik[c + LW]);
-            d4 = 2.f * frcp(kc + s_ik[c - PL]);
-            d5 = 2.f * frcp(kc + s_ik[c + PL]);
-        }
-        // Neumann doubling, sequential as the reference (:198-203)
-        if (d0 == 0.f && pW > 0.f) d1 *= 2.f;
-        if (d1 == 0.f && pE > 0.f) d0 *= 2.f;
-        if (d2 == 0.f && pS > 0.f) d3 *= 2.f;
-        if (d3 == 0.f && pN > 0.f) d2 *= 2.f;
-        if (d4 == 0.f && pB > 0.f) d5 *= 2.f;
-        if (d5 == 0.f && pF > 0.f) d4 *= 2.f;
-        const float sdf = d0 + d1 + d2 + d3 + d4 + d5;
-        const float reac = rho * pc * (1.f - pc);
-        float dphi;
-        if (__builtin_isfinite(sdf)) {
-            // flux form sum_k df_k (phi_k - phi_c): the same quantity as the
-            // reference's in + out, without its cancellation in FP32
-            const float diff = d0 * (pW - pc) + d1 * (pE - pc) + d2 * (pS - pc) + d3 * (pN - pc) +
-                               d4 * (pB - pc) + d5 * (pF - pc);
-            dphi = ih2 * diff + reac;
-        } else {  // NaN/inf coefficients: the reference's literal form
-            const float in = ih2 * (d0 * pW + d1 * pE + d2 * pS + d3 * pN + d4 * pB + d5 * pF);
-            const float out = -(sdf * pc * ih2);
-            dphi = out + in + reac;
-        }
-        float pn = pc + dt * dphi;
-        pn = (0.f < pn) ? pn : 0.f;  // std::max((Real)0, phi): NaN -> 0
-        pn = (pn < 1.f) ? pn : 1.f;  // std::min((Real)1, phi)
+        const float pn = fk_cell(pc, pW, pE, pS, pN, pB, pF, s_ik[c], s_ik[c - 1], s_ik[c + 1], s_ik[c - LW],
+                                 s_ik[c + LW], s_ik[c - PL], s_ik[c + PL], breg[k], ih2, rho, dt);
         a.phi_out[base + (z0 + k) * 256 + t] = pn;
     }
 }
@@ -346,37 +373,8 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         const float pc = pC;
         const float pW = sp[c - 1], pE = sp[c + 1];
         const float pS = sp[c - PLW], pN2 = sp[c + PLW];
-        float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
-        if (bC + pc > 0.f) {  // tissue = p_w + p_g + phi > 0 (:155-157)
-            const float kc = kC;
-            d0 = 2.f * frcp(kc + sk[c - 1]);
-            d1 = 2.f * frcp(kc + sk[c + 1]);
-            d2 = 2.f * frcp(kc + sk[c - PLW]);
-            d3 = 2.f * frcp(kc + sk[c + PLW]);
-            d4 = 2.f * frcp(kc + kB);
-            d5 = 2.f * frcp(kc + kF);
-        }
-        if (d0 == 0.f && pW > 0.f) d1 *= 2.f;
-        if (d1 == 0.f && pE > 0.f) d0 *= 2.f;
-        if (d2 == 0.f && pS > 0.f) d3 *= 2.f;
-        if (d3 == 0.f && pN2 > 0.f) d2 *= 2.f;
-        if (d4 == 0.f && pB > 0.f) d5 *= 2.f;
-        if (d5 == 0.f && pF > 0.f) d4 *= 2.f;
-        const float sdf = d0 + d1 + d2 + d3 + d4 + d5;
-        const float reac = rho * pc * (1.f - pc);
-        float dphi;
-        if (__builtin_isfinite(sdf)) {
-            const float diff = d0 * (pW - pc) + d1 * (pE - pc) + d2 * (pS - pc) + d3 * (pN2 - pc) +
-                               d4 * (pB - pc) + d5 * (pF - pc);
-            dphi = ih2 * diff + reac;
-        } else {
-            const float in = ih2 * (d0 * pW + d1 * pE + d2 * pS + d3 * pN2 + d4 * pB + d5 * pF);
-            const float out = -(sdf * pc * ih2);
-            dphi = out + in + reac;
-        }
-        float pn = pc + dt * dphi;
-        pn = (0.f < pn) ? pn : 0.f;
-        pn = (pn < 1.f) ? pn : 1.f;
+        const float pn = fk_cell(pc, pW, pE, pS, pN2, pB, pF, kC, sk[c - 1], sk[c + 1], sk[c - PLW], sk[c + PLW],
+                                 kB, kF, bC, ih2, rho, dt);
         a.phi_out[base + k * 256 + t] = pn;
         // shift the column window
         pB = pC;

@@ -97,11 +97,13 @@ def test_surrogate_training_and_reproducibility():
     p1 = s.predict(params, u0)
     p2 = s.predict(params, u0)
     assert np.array_equal(p1, p2)
-    losses = [s.train_step(params, u0, tgt, 1e-3) for _ in range(80)]
-    print("loss %.4e -> %.4e" % (losses[0], losses[-1]))
+    losses = [s.train_step(params, u0, tgt, 3e-4) for _ in range(120)]
+    mse = float(np.mean((s.predict(params, u0) - tgt) ** 2))
     # well below what a constant prediction can reach (the tumours fill ~5% of the volume)
     const = min(np.mean((c - tgt) ** 2) for c in np.linspace(0, 0.3, 31))
-    assert losses[-1] < 0.8 * const
+    print("loss %.4e -> %.4e, prediction MSE %.4e (best constant %.4e)" % (losses[0], losses[-1], mse, const))
+    assert losses[-1] < 0.5 * losses[0]
+    assert mse < 0.5 * const
     t = s.timing()
     assert t["forward_ms"] > 0 and t["backward_ms"] > 0
     s.close()
