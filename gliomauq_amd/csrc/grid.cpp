@@ -105,47 +105,62 @@ int Grid::find(int level, int x, int y, int z) const
 
 int Grid::lying_leaf(int lam, const int64_t p[3]) const
 {
-    const int64_t nF = 1LL << lut_level_;
+    // every extent is a power of two: floor division and periodic wrap are
+    // arithmetic shifts and masks
+    const int64_t mask = (1LL << lut_level_) - 1;
     int64_t f[3];
     for (int d = 0; d < 3; d++) {
         int64_t q = p[d];
         if (lut_level_ >= lam) q = q * (1LL << (lut_level_ - lam));
-        else q = floordiv(q, 1LL << (lam - lut_level_));
-        f[d] = wrapi(floordiv(q, BS), nF);
+        else q = q >> (lam - lut_level_);
+        f[d] = (q >> 4) & mask;  // BS = 16
     }
-    return lut_[(size_t)(f[0] + nF * (f[1] + nF * f[2]))];
+    return lut_[(size_t)(f[0] + (mask + 1) * (f[1] + (mask + 1) * f[2]))];
 }
 
-void Grid::neighbors(int i, std::vector<std::pair<int, bool>> &out) const
+template <class F>
+bool Grid::any_ring(int i, F &&f) const
 {
-    // Ring of finest cells around the leaf's box; a leaf is a neighbour if
-    // its box touches this box (face, edge or corner), directly or through a
-    // periodic image (Grid::_computeNeighborhood with FlattenedBlock::
-    // areAdjacent, MRAGridHelpers.h:63-72).
-    out.clear();
+    // Ring of finest look-up cells around the leaf's box; a leaf is a
+    // neighbour if its box touches this box (face, edge or corner), directly
+    // or through a periodic image (Grid::_computeNeighborhood with
+    // FlattenedBlock::areAdjacent, MRAGridHelpers.h:63-72).  f(k, image)
+    // returning true stops the walk; a neighbour may be visited repeatedly.
     const Leaf &lf = leaves_[i];
-    const int64_t nF = 1LL << lut_level_;
+    const int64_t nF = 1LL << lut_level_, mask = nF - 1;
     const int d = lut_level_ - lf.level;
     int64_t bs[3], be[3];
     for (int a = 0; a < 3; a++) {
         bs[a] = (int64_t)lf.idx[a] << d;
         be[a] = ((int64_t)lf.idx[a] + 1) << d;
     }
-    out.reserve(64);
-    for (int64_t z = bs[2] - 1; z <= be[2]; z++)
-        for (int64_t y = bs[1] - 1; y <= be[1]; y++)
-            for (int64_t x = bs[0] - 1; x <= be[0]; x++) {
-                const bool inside = x >= bs[0] && x < be[0] && y >= bs[1] && y < be[1] && z >= bs[2] && z < be[2];
-                if (inside) {
-                    // skip the interior quickly
-                    if (x == bs[0]) x = be[0] - 1;
-                    continue;
-                }
-                const bool img = !(x >= 0 && x < nF && y >= 0 && y < nF && z >= 0 && z < nF);
-                const int k = lut_[(size_t)(wrapi(x, nF) + nF * (wrapi(y, nF) + nF * wrapi(z, nF)))];
+    for (int64_t z = bs[2] - 1; z <= be[2]; z++) {
+        const bool zin = z >= bs[2] && z < be[2];
+        for (int64_t y = bs[1] - 1; y <= be[1]; y++) {
+            const bool yzin = zin && y >= bs[1] && y < be[1];
+            const int64_t row = nF * ((y & mask) + nF * (z & mask));
+            const bool yzimg = y < 0 || y >= nF || z < 0 || z >= nF;
+            // inside the box in y and z: only the two x ends of the row
+            const int64_t step = yzin ? be[0] - bs[0] + 1 : 1;
+            for (int64_t x = bs[0] - 1; x <= be[0]; x += step) {
+                const bool img = yzimg || x < 0 || x >= nF;
+                const int k = lut_[(size_t)((x & mask) + row)];
                 if (k == i && !img) continue;
-                out.emplace_back(k, img);
+                if (f(k, img)) return true;
             }
+        }
+    }
+    return false;
+}
+
+void Grid::neighbors(int i, std::vector<std::pair<int, bool>> &out) const
+{
+    out.clear();
+    out.reserve(64);
+    any_ring(i, [&](int k, bool img) {
+        out.emplace_back(k, img);
+        return false;
+    });
     std::sort(out.begin(), out.end());
     out.erase(std::unique(out.begin(), out.end()), out.end());
 }
@@ -155,14 +170,12 @@ std::vector<int> Grid::plan_refine(const std::unordered_set<int> &flagged_ids) c
     const int n = (int)leaves_.size();
     std::vector<char> flag(n, 0);
     for (int i = 0; i < n; i++) flag[i] = flagged_ids.count(leaves_[i].block_id) ? 1 : 0;
-    std::vector<std::pair<int, bool>> nb;
     // Refiner_SpaceExtension step 3: a leaf with a finer, non-image neighbour
     // is refined as well (MRAGRefiner_SpaceExtension.h:35-51)
     for (int i = 0; i < n; i++) {
         if (flag[i]) continue;
-        neighbors(i, nb);
-        for (auto &p : nb)
-            if (!p.second && leaves_[p.first].level > leaves_[i].level) { flag[i] = 1; break; }
+        const int li = leaves_[i].level;
+        if (any_ring(i, [&](int k, bool img) { return !img && leaves_[k].level > li; })) flag[i] = 1;
     }
     // step 4: never refine on the finest allowed level (:54-63)
     for (int i = 0; i < n; i++)
@@ -177,13 +190,13 @@ std::vector<int> Grid::plan_refine(const std::unordered_set<int> &flagged_ids) c
         new_v.clear();
         for (int v : old_v) {
             const int nl = leaves_[v].level + 1;
-            neighbors(v, nb);
-            for (auto &p : nb) {
-                if (std::abs(leaves_[p.first].level - nl) > 1 && !victim[p.first]) {
-                    victim[p.first] = 1;
-                    new_v.push_back(p.first);
+            any_ring(v, [&](int k, bool) {
+                if (std::abs(leaves_[k].level - nl) > 1 && !victim[k]) {
+                    victim[k] = 1;
+                    new_v.push_back(k);
                 }
-            }
+                return false;
+            });
         }
         std::swap(old_v, new_v);
     }
@@ -202,7 +215,6 @@ std::vector<std::array<int, 4>> Grid::plan_compress(const std::unordered_set<int
     for (const Leaf &lf : leaves_)
         if (lf.level > 0) parents.insert({lf.level - 1, lf.idx[0] >> 1, lf.idx[1] >> 1, lf.idx[2] >> 1});
     std::vector<std::array<int, 4>> out;
-    std::vector<std::pair<int, bool>> nb;
     for (const auto &p : parents) {
         int ch[8];
         bool ok = true;
@@ -211,11 +223,8 @@ std::vector<std::array<int, 4>> Grid::plan_compress(const std::unordered_set<int
             ok = ch[c] >= 0 && flagged_ids.count(leaves_[ch[c]].block_id);
         }
         if (!ok) continue;
-        for (int c = 0; c < 8 && ok; c++) {
-            neighbors(ch[c], nb);
-            for (auto &q : nb)
-                if (std::abs(leaves_[q.first].level - p[0]) > 1) { ok = false; break; }
-        }
+        for (int c = 0; c < 8 && ok; c++)
+            ok = !any_ring(ch[c], [&](int k, bool) { return std::abs(leaves_[k].level - p[0]) > 1; });
         if (ok) out.push_back(p);
     }
     return out;
@@ -236,12 +245,10 @@ void Grid::replace_leaves(const std::vector<int> &remove, const std::vector<Leaf
 
 void Grid::check_jump() const
 {
-    std::vector<std::pair<int, bool>> nb;
     for (int i = 0; i < (int)leaves_.size(); i++) {
-        neighbors(i, nb);
-        for (auto &p : nb)
-            if (std::abs(leaves_[p.first].level - leaves_[i].level) > 1)
-                throw std::runtime_error("level jump > 1 between neighbouring blocks");
+        const int li = leaves_[i].level;
+        if (any_ring(i, [&](int k, bool) { return std::abs(leaves_[k].level - li) > 1; }))
+            throw std::runtime_error("level jump > 1 between neighbouring blocks");
     }
 }
 
@@ -318,16 +325,27 @@ void build_sample_tables(const Grid &g, int *fnbr, int *nbr26, SampleTables &t)
     const auto &L = g.leaves();
     t.order = morton_order(g);
     t.faces.clear();
+    const std::vector<int> &gl = g.lut();
+    const int lutl = g.lut_level();
+    const size_t nF = (size_t)1 << lutl;
     for (int i : t.order) {
         const Leaf &lf = L[i];
-        const int m = 1 << lf.level;
+        const int m = 1 << lf.level, sh = lutl - lf.level;
         int *nb26 = nbr26 + (size_t)lf.slot * 26;
-        for (int c = 0; c < 27; c++) {
-            if (c == 13) continue;
-            const int d[3] = {c % 3 - 1, (c / 3) % 3 - 1, c / 9 - 1};
-            const int j = g.find(lf.level, (int)wrapi(lf.idx[0] + d[0], m), (int)wrapi(lf.idx[1] + d[1], m),
-                                 (int)wrapi(lf.idx[2] + d[2], m));
-            nb26[c < 13 ? c : c - 1] = j >= 0 ? L[j].slot : -1;
+        // a same-level neighbour box is a leaf iff the leaf holding its corner
+        // look-up cell has that level (boxes of one level are aligned)
+        int c = 0;
+        for (int dz = -1; dz <= 1; dz++) {
+            const size_t z = (size_t)((lf.idx[2] + dz) & (m - 1)) << sh;
+            for (int dy = -1; dy <= 1; dy++) {
+                const size_t y = (size_t)((lf.idx[1] + dy) & (m - 1)) << sh;
+                for (int dx = -1; dx <= 1; dx++, c++) {
+                    if (c == 13) continue;
+                    const size_t x = (size_t)((lf.idx[0] + dx) & (m - 1)) << sh;
+                    const Leaf &q = L[gl[x + nF * (y + nF * z)]];
+                    nb26[c < 13 ? c : c - 1] = q.level == lf.level ? q.slot : -1;
+                }
+            }
         }
         for (int f = 0; f < 6; f++) {
             const int c = face_codes[f];
@@ -336,7 +354,6 @@ void build_sample_tables(const Grid &g, int *fnbr, int *nbr26, SampleTables &t)
             if (nb < 0) t.faces.push_back(lf.slot * 6 + f);
         }
     }
-    const std::vector<int> &gl = g.lut();
     t.lut.resize(gl.size());
     for (size_t k = 0; k < gl.size(); k++) t.lut[k] = L[gl[k]].slot;
     // base-level look-up for the uniform dump / likelihood
@@ -351,24 +368,35 @@ void build_sample_tables(const Grid &g, int *fnbr, int *nbr26, SampleTables &t)
             }
 }
 
+namespace {
+// Interleave the low 21 bits of x with two zero bits between each.
+inline uint64_t spread3(uint64_t x)
+{
+    x &= 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffULL;
+    x = (x | x << 16) & 0x1f0000ff0000ffULL;
+    x = (x | x << 8) & 0x100f00f00f00f00fULL;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+    x = (x | x << 2) & 0x1249249249249249ULL;
+    return x;
+}
+}  // namespace
+
 std::vector<int> morton_order(const Grid &g)
 {
+    // codes use 3 * lut_level <= 24 bits: code << 32 | leaf index sorts as one integer
     const auto &L = g.leaves();
     const int lv = g.lut_level();
-    std::vector<std::pair<uint64_t, int>> keys;
-    keys.reserve(L.size());
-    for (int i = 0; i < (int)L.size(); i++) {
+    std::vector<uint64_t> keys(L.size());
+    for (size_t i = 0; i < L.size(); i++) {
         const int d = lv - L[i].level;
-        uint64_t x = (uint64_t)L[i].idx[0] << d, y = (uint64_t)L[i].idx[1] << d, z = (uint64_t)L[i].idx[2] << d;
-        uint64_t m = 0;
-        for (int b = 0; b < 20; b++)
-            m |= ((x >> b) & 1) << (3 * b) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b + 2);
-        keys.push_back({m, i});
+        const uint64_t m = spread3((uint64_t)L[i].idx[0] << d) | spread3((uint64_t)L[i].idx[1] << d) << 1 |
+                           spread3((uint64_t)L[i].idx[2] << d) << 2;
+        keys[i] = m << 32 | (uint64_t)i;
     }
     std::sort(keys.begin(), keys.end());
-    std::vector<int> out;
-    out.reserve(keys.size());
-    for (auto &k : keys) out.push_back(k.second);
+    std::vector<int> out(keys.size());
+    for (size_t k = 0; k < keys.size(); k++) out[k] = (int)(keys[k] & 0xffffffffu);
     return out;
 }
 

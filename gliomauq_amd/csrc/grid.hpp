@@ -91,6 +91,8 @@ public:
     int expand_point(int leaf, int lx, int ly, int lz, std::vector<Term> &scratch, std::vector<Term> &out) const;
 
 private:
+    template <class F>
+    bool any_ring(int i, F &&f) const;
     void rebuild_index();
     void expand_rec(int lam, const int64_t p[3], double w, std::vector<Term> &T, int depth) const;
 
