@@ -685,7 +685,7 @@ def main():
     wall = time.perf_counter() - t0
 
     hs1 = gq.host_stats(ctx)
-    zt, tune = gq.autotune_info(ctx)
+    zt, fused, tune = gq.autotune_info(ctx)
     cells = stats["cells"]
     value = cells / wall
     kernel_s = stats["kernel_ms"] * 1e-3
@@ -719,7 +719,8 @@ def main():
                      "algorithmic_bytes_per_cell": 16},
     }
     res["breakdown"] = {"fk_steps_s": t_steps, "adapt_s": t_adapt, "step_kernel_s": kernel_s,
-                        "host": {k: hs1[k] - hs0[k] for k in hs1}, "slab_depth": zt, "autotune_us": tune}
+                        "host": {k: hs1[k] - hs0[k] for k in hs1}, "slab_depth": zt,
+                        "face_ghosts_in_step": fused, "autotune_us": tune}
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         res["cpu_baseline"] = cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt)

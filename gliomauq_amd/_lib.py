@@ -97,7 +97,7 @@ def load():
         "gq_set_step_timing": [vp, C.c_int],
         "gq_memory_info": [vp, C.POINTER(C.c_int64), ip],
         "gq_set_autotune": [vp, C.c_int, C.c_int],
-        "gq_autotune_info": [vp, ip, f64p],
+        "gq_autotune_info": [vp, ip, ip, f64p],
         "gq_host_stats": [vp, f64p],
         "gq_topo_create": [C.POINTER(vp), C.c_int, C.c_int, C.c_int],
         "gq_topo_destroy": [vp],

@@ -120,6 +120,7 @@ struct gq_ctx {
     DevBuf<int64_t> d_face_ptr;
     DevBuf<char> d_scan_tmp;
     int face_nrows = 0;
+    bool face_rows_valid = false;  // device face-ghost rows (ptr/src/w) match the tables
     DevBuf<double> d_ghost;  // cross-level lab points (3 fields for a split)
     DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2, d_sprm;
     bool fik_valid = false;       // false: every slot's face 1/K must be recomputed
@@ -141,9 +142,15 @@ struct gq_ctx {
 
     // step tile (slab depth) chosen per grid-size bucket
     bool autotune = true;
-    std::map<int, int> zt_cache;
-    int last_zt = 16;
-    double tune_us[5] = {0, 0, 0, 0, 0};
+    // step configuration: slab depth zt (16/8/4/2, 1 = column streaming) and
+    // whether the step evaluates the cross-level face-ghost rows itself
+    // (fused) or reads them from a separate k_face_ghosts launch
+    static int cfg_zt(int cfg) { return cfg & 31; }
+    static bool cfg_fused(int cfg) { return (cfg >> 5) & 1; }
+    static int make_cfg(int zt, bool fused) { return zt | (fused ? 32 : 0); }
+    std::map<int, int> zt_cache;  // work-list size bucket -> cfg
+    int last_cfg = 16;
+    double tune_us[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // host-side time accounting (seconds) and counts
     double t_tables = 0, t_plan = 0, t_fwt = 0, t_split = 0, t_collapse = 0, t_tune = 0;
@@ -389,9 +396,34 @@ struct gq_ctx {
         d_lut.upload(lut_slot, st);
         d_lutf.upload(lutf, st);
         d_faces.upload(faces, st);
-        // step rows of the cross-level faces, enumerated on the device
-        const int nfaces = (int)faces.size();
-        face_nrows = nfaces * 256;
+        face_nrows = (int)faces.size() * 256;
+        face_rows_valid = false;
+        if (d_fik.n < (size_t)cap * 6 * 256) {
+            d_fik.reserve((size_t)cap * 6 * 256);  // contents not kept
+            fik_valid = false;
+        }
+        if (!d_ih2.p) {
+            std::vector<float> ih2(16);
+            for (int l = 0; l < 16; l++) {
+                const float h = (float)(std::ldexp(1.0, -l) / gq::BS);  // BlockInfo::h (Real)
+                const double hd = h;
+                ih2[l] = (float)(1. / (hd * hd));
+            }
+            d_ih2.upload(ih2, st);
+        }
+        HIPCHK(hipStreamSynchronize(st));  // the host vectors above are released on return
+        tables_valid = true;
+    }
+
+    // Face-ghost rows of the step (ptr/src/w of every cross-level face-halo
+    // cell), enumerated on the device.  Only stepping needs them, so a table
+    // rebuild between a refinement and the compression that follows does not.
+    void ensure_face_rows()
+    {
+        ensure_tables();
+        if (face_rows_valid) return;
+        ScopeTimer tm(t_tables);
+        const int nfaces = face_nrows / 256;
         d_face_ptr.reserve((size_t)face_nrows + 1);
         if (nfaces > 0) {
             const gq::LabArgs la = lab_args();
@@ -410,21 +442,7 @@ struct gq_ctx {
             HIPCHK(gq::launch_face_rows_fill(la, d_faces.p, nfaces, d_face_ptr.p, d_face_src.p, d_face_w.p, st));
         }
         d_gphi.reserve((size_t)std::max(1, face_nrows));
-        if (d_fik.n < (size_t)cap * 6 * 256) {
-            d_fik.reserve((size_t)cap * 6 * 256);  // contents not kept
-            fik_valid = false;
-        }
-        if (!d_ih2.p) {
-            std::vector<float> ih2(16);
-            for (int l = 0; l < 16; l++) {
-                const float h = (float)(std::ldexp(1.0, -l) / gq::BS);  // BlockInfo::h (Real)
-                const double hd = h;
-                ih2[l] = (float)(1. / (hd * hd));
-            }
-            d_ih2.upload(ih2, st);
-        }
-        HIPCHK(hipStreamSynchronize(st));  // the host vectors above are released on return
-        tables_valid = true;
+        face_rows_valid = true;
     }
 
     gq::LabArgs lab_args() const
@@ -444,7 +462,7 @@ struct gq_ctx {
     // slots of samples whose topology changed are recomputed unless the parameters or the anatomy changed.
     void ensure_fik(const std::vector<float> &key, bool per_sample)
     {
-        ensure_tables();
+        ensure_face_rows();
         const int S = nsamples();
         if (!fik_valid || key != fik_key) {
             fik_stale.assign(S, 1);
@@ -716,17 +734,19 @@ struct gq_ctx {
         return 2;
     }
 
-    // Step tile autotuner: time every slab depth on the current grid (the
-    // outputs go to the scratch ping-pong buffer, the state is unchanged) and
-    // keep the fastest per power-of-two bucket of the block count.
-    int choose_zt(gq::StepArgs a, int nrows)
+    // Step autotuner: time every slab depth, with the face-ghost rows from a
+    // separate launch (timed together with the step) and evaluated inside the
+    // step, on the current grid (outputs go to the scratch ping-pong buffer,
+    // the state is unchanged); keep the fastest per power-of-two bucket of the
+    // block count.  Every configuration gives the same bits.
+    int choose_step(gq::StepArgs a, int nrows)
     {
         const int nwork = a.nwork;
         int bucket = 0;
         while ((1 << bucket) < nwork) bucket++;
         auto it = zt_cache.find(bucket);
         if (it != zt_cache.end()) return it->second;
-        if (!autotune) return zt_heuristic(nwork);
+        if (!autotune) return make_cfg(zt_heuristic(nwork), false);
         ScopeTimer tm(t_tune);
         static const int cand[5] = {16, 8, 4, 2, 1};
         hipEvent_t e0, e1;
@@ -734,23 +754,32 @@ struct gq_ctx {
         HIPCHK(hipEventCreate(&e1));
         a.phi_in = phi[cur];
         a.phi_out = phi[cur ^ 1];
-        if (nrows > 0)
-            HIPCHK(gq::launch_face_ghosts(phi[cur], d_face_ptr.p, d_face_src.p, d_face_w.p, d_gphi.p, nrows, st));
-        int best = 16;
+        int best = make_cfg(16, false);
         float best_ms = 1e30f;
-        for (int k = 0; k < 5; k++) {
-            for (int r = 0; r < 2; r++) HIPCHK(gq::launch_fk_step(a, cand[k], st));
-            HIPCHK(hipEventRecord(e0, st));
-            const int reps = 5;
-            for (int r = 0; r < reps; r++) HIPCHK(gq::launch_fk_step(a, cand[k], st));
-            HIPCHK(hipEventRecord(e1, st));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            tune_us[k] = 1e3 * ms / reps;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = cand[k];
+        for (int k = 0; k < 10; k++) tune_us[k] = 0.0;
+        for (int fused = 0; fused < (nrows > 0 ? 2 : 1); fused++) {
+            gq::StepArgs b = a;
+            if (fused) b.gphi = nullptr;
+            for (int k = 0; k < 5; k++) {
+                auto rep = [&] {
+                    if (nrows > 0 && !fused)
+                        HIPCHK(gq::launch_face_ghosts(phi[cur], d_face_ptr.p, d_face_src.p, d_face_w.p, d_gphi.p,
+                                                      nrows, st));
+                    HIPCHK(gq::launch_fk_step(b, cand[k], st));
+                };
+                for (int r = 0; r < 2; r++) rep();
+                HIPCHK(hipEventRecord(e0, st));
+                const int reps = 5;
+                for (int r = 0; r < reps; r++) rep();
+                HIPCHK(hipEventRecord(e1, st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                tune_us[5 * fused + k] = 1e3 * ms / reps;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = make_cfg(cand[k], fused);
+                }
             }
         }
         (void)hipEventDestroy(e0);
@@ -771,6 +800,9 @@ struct gq_ctx {
         a.sfbase = d_sfbase.p;
         a.fik = d_fik.p;
         a.gphi = d_gphi.p;
+        a.fptr = d_face_ptr.p;
+        a.fsrc = d_face_src.p;
+        a.fw = d_face_w.p;
         a.level = d_level.p;
         a.ih2_lvl = d_ih2.p;
         a.slot_sample = d_slot_sample.p;
@@ -800,9 +832,12 @@ struct gq_ctx {
     // first nrows[j] face rows (a prefix of the samples in work order).
     // Returns the device time (ms): step kernels only with per-kernel timing,
     // else the whole chunk.
-    double advance(gq::StepArgs a, int zt, const std::vector<int> &nwork, const std::vector<int> &nrows,
+    double advance(gq::StepArgs a, int cfg, const std::vector<int> &nwork, const std::vector<int> &nrows,
                    bool use_graph)
     {
+        const int zt = cfg_zt(cfg);
+        const bool fused = cfg_fused(cfg);
+        if (fused) a.gphi = nullptr;
         const int m = (int)nwork.size();
         if (m == 0) return 0.0;
         const bool pk = per_kernel_timing && !use_graph;
@@ -810,7 +845,7 @@ struct gq_ctx {
         auto enqueue = [&](int c0) {
             int c = c0;
             for (int j = 0; j < m; j++) {
-                if (nrows[j] > 0)
+                if (nrows[j] > 0 && !fused)
                     HIPCHK(gq::launch_face_ghosts(phi[c], d_face_ptr.p, d_face_src.p, d_face_w.p, d_gphi.p, nrows[j],
                                                   st));
                 a.phi_in = phi[c];
@@ -1169,14 +1204,14 @@ gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
         gq::StepArgs a = ctx->step_args(p);
         const int nwork = (int)ctx->work.size();
         const int nrows = ctx->face_nrows;
-        const int zt = ctx->choose_zt(a, nrows);
-        ctx->last_zt = zt;
+        const int cfg = ctx->choose_step(a, nrows);
+        ctx->last_cfg = cfg;
         const int chunk = 1024;
         double total = 0.0;
         int done = 0;
         while (done < n_steps) {
             const int m = std::min(chunk, n_steps - done);
-            total += ctx->advance(a, zt, std::vector<int>(m, nwork), std::vector<int>(m, nrows), false);
+            total += ctx->advance(a, cfg, std::vector<int>(m, nwork), std::vector<int>(m, nrows), false);
             done += m;
         }
         ctx->last_ms = total;
@@ -1236,10 +1271,10 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
             }
             gq::StepArgs a = ctx->step_args(nullptr);
             a.nwork = nw[0];
-            const int zt = ctx->choose_zt(a, nr[0]);
-            ctx->last_zt = zt;
+            const int cfg = ctx->choose_step(a, nr[0]);
+            ctx->last_cfg = cfg;
             const int c0 = ctx->cur;
-            ms += ctx->advance(a, zt, nw, nr, use_graph);
+            ms += ctx->advance(a, cfg, nw, nr, use_graph);
             chunks++;
             // a sample that took a number of steps of the other parity holds its state in the other buffer
             for (int pos = 0; pos < S; pos++) {
@@ -1279,7 +1314,7 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
         ctx->ens_stats[3] = adapts;
         ctx->ens_stats[4] = t_adapt;
         ctx->ens_stats[5] = (double)ctx->work.size();
-        ctx->ens_stats[6] = ctx->last_zt;
+        ctx->ens_stats[6] = gq_ctx::cfg_zt(ctx->last_cfg);
         ctx->ens_stats[7] = use_graph ? 1 : 0;
     });
 }
@@ -1370,7 +1405,7 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
         ctx->d_stash.reserve((size_t)E * pool);
         ctx->d_lam.reserve(2 * pool);
         gq::StepArgs a = ctx->step_args(nullptr);
-        const int zt = ctx->choose_zt(a, 0);
+        const int zt = gq_ctx::cfg_zt(ctx->choose_step(a, 0));  // uniform grid: no face rows
         ctx->need_events(4);
         hipStream_t st = ctx->st;
         HIPCHK(hipEventRecord(ctx->events[0], st));
@@ -1743,21 +1778,23 @@ gq_status gq_last_step_timing(gq_ctx *ctx, double *ms, int *launches, int64_t *c
 gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt)
 {
     return guard(ctx, [&] {
-        if (mode == 2 && fixed_zt != 16 && fixed_zt != 8 && fixed_zt != 4 && fixed_zt != 2 && fixed_zt != 1)
+        if (mode < 0 || mode > 3) throw gq_error(GQ_ERR_ARG, "autotune mode must be 0, 1, 2 or 3");
+        if (mode >= 2 && fixed_zt != 16 && fixed_zt != 8 && fixed_zt != 4 && fixed_zt != 2 && fixed_zt != 1)
             throw gq_error(GQ_ERR_ARG, "slab depth must be 16, 8, 4, 2 or 1 (column streaming)");
         ctx->zt_cache.clear();
         ctx->autotune = mode == 1;
-        if (mode == 2)
-            for (int b = 0; b < 32; b++) ctx->zt_cache[b] = fixed_zt;
+        if (mode >= 2)
+            for (int b = 0; b < 32; b++) ctx->zt_cache[b] = gq_ctx::make_cfg(fixed_zt, mode == 3);
     });
 }
 
-gq_status gq_autotune_info(gq_ctx *ctx, int *zt, double us[5])
+gq_status gq_autotune_info(gq_ctx *ctx, int *zt, int *fused, double us[10])
 {
     return guard(ctx, [&] {
-        if (zt) *zt = ctx->last_zt;
+        if (zt) *zt = gq_ctx::cfg_zt(ctx->last_cfg);
+        if (fused) *fused = gq_ctx::cfg_fused(ctx->last_cfg) ? 1 : 0;
         if (us)
-            for (int k = 0; k < 5; k++) us[k] = ctx->tune_us[k];
+            for (int k = 0; k < 10; k++) us[k] = ctx->tune_us[k];
     });
 }
 

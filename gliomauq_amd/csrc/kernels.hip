@@ -120,6 +120,35 @@ __device__ __forceinline__ float fk_cell(float pc, float pW, float pE, float pS,
 
 }  // namespace
 
+// One cross-level face-ghost row: sum_k w_k phi[src_k] as four interleaved
+// FP32 partial sums (terms j, j+4, ...) combined as (s0 + s1) + (s2 + s3) --
+// exactly what k_face_ghosts' four lanes and two shuffles compute, so the
+// in-kernel evaluation and the separate launch give the same bits.
+__device__ __forceinline__ float face_row_sum(const float *__restrict__ phi, const int64_t *__restrict__ ptr,
+                                              const int *__restrict__ src, const float *__restrict__ w, int row)
+{
+#pragma clang fp contract(off)
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int64_t k = ptr[row];
+    const int64_t e = ptr[row + 1];
+    for (; k + 3 < e; k += 4) {
+        s0 = __builtin_fmaf(w[k], phi[src[k]], s0);
+        s1 = __builtin_fmaf(w[k + 1], phi[src[k + 1]], s1);
+        s2 = __builtin_fmaf(w[k + 2], phi[src[k + 2]], s2);
+        s3 = __builtin_fmaf(w[k + 3], phi[src[k + 3]], s3);
+    }
+    if (k < e) s0 = __builtin_fmaf(w[k], phi[src[k]], s0);
+    if (k + 1 < e) s1 = __builtin_fmaf(w[k + 1], phi[src[k + 1]], s1);
+    if (k + 2 < e) s2 = __builtin_fmaf(w[k + 2], phi[src[k + 2]], s2);
+    return (s0 + s1) + (s2 + s3);
+}
+
+// Cross-level face ghost of phi_in for the step (row index absolute).
+__device__ __forceinline__ float step_face_ghost(const StepArgs &a, int row)
+{
+    return a.gphi ? a.gphi[row] : face_row_sum(a.phi_in, a.fptr, a.fsrc, a.fw, row);
+}
+
 // ---------------------------------------------------------------------------
 // Fused RD operator + UpdateTumor.  A workgroup of 256 threads updates a slab
 // of 16 x 16 x ZT cells of one block (ZT = 16: the whole block); each thread
@@ -179,7 +208,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
             const int f = side == 0 ? 4 : 5;
             const int nb = a.fnbr[slot * 6 + f];
             if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, tx, ty)];
-            else pv = a.gphi[a.sfbase[a.slot_sample[slot]] + a.fbase[slot * 6 + f] + t];
+            else pv = step_face_ghost(a, a.sfbase[a.slot_sample[slot]] + a.fbase[slot * 6 + f] + t);
             kv = a.fik[((size_t)slot * 6 + f) * 256 + t];
         }
         s_phi[l] = pv;
@@ -193,7 +222,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
         const int nb = a.fnbr[slot * 6 + f];
         float pv;
         if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, u, v)];
-        else pv = a.gphi[a.sfbase[a.slot_sample[slot]] + a.fbase[slot * 6 + f] + u + 16 * v];
+        else pv = step_face_ghost(a, a.sfbase[a.slot_sample[slot]] + a.fbase[slot * 6 + f] + u + 16 * v);
         const float kv = a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
         int l;
         switch (f) {
@@ -227,11 +256,12 @@ __global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ p
                                                      const int *__restrict__ src, const float *__restrict__ w,
                                                      float *__restrict__ out, int nrows)
 {
+#pragma clang fp contract(off)
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = g >> 2, lane = g & 3;
     float acc = 0.f;
     if (r < nrows)
-        for (int64_t k = ptr[r] + lane; k < ptr[r + 1]; k += 4) acc += w[k] * phi[src[k]];
+        for (int64_t k = ptr[r] + lane; k < ptr[r + 1]; k += 4) acc = __builtin_fmaf(w[k], phi[src[k]], acc);
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     if (r < nrows && lane == 0) out[r] = acc;
@@ -315,7 +345,7 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
     {
         const int nb = a.fnbr[slot * 6 + 4];
         if (nb >= 0) pB = a.phi_in[(size_t)nb * B3 + nbr_face_cell(4, tx, ty)];
-        else pB = a.gphi[rbase + a.fbase[slot * 6 + 4] + t];
+        else pB = step_face_ghost(a, rbase + a.fbase[slot * 6 + 4] + t);
         kB = a.fik[((size_t)slot * 6 + 4) * 256 + t];
     }
     float pC = a.phi_in[base + t];
@@ -340,7 +370,7 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         if (t < 64) {
             float pv;
             if (hnb >= 0) pv = a.phi_in[(size_t)hnb * B3 + nbr_face_cell(hf, hu, k)];
-            else pv = a.gphi[hrow + hu + 16 * k];
+            else pv = step_face_ghost(a, hrow + hu + 16 * k);
             const float kv = a.fik[((size_t)slot * 6 + hf) * 256 + hu + 16 * k];
             int l;
             switch (hf) {
@@ -358,7 +388,7 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         else {
             const int nb = a.fnbr[slot * 6 + 5];
             if (nb >= 0) pF = a.phi_in[(size_t)nb * B3 + nbr_face_cell(5, tx, ty)];
-            else pF = a.gphi[rbase + a.fbase[slot * 6 + 5] + t];
+            else pF = step_face_ghost(a, rbase + a.fbase[slot * 6 + 5] + t);
             kF = a.fik[((size_t)slot * 6 + 5) * 256 + t];
         }
         // prefetch plane k+2

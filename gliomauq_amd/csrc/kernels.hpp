@@ -18,7 +18,10 @@ struct StepArgs {
     const int *fbase;  // [slot*6+f] first face-ghost row, relative to the sample's (-1: same level)
     const int *sfbase; // [sample] first face-ghost row of the sample
     const float *fik;  // [slot*6+f][256]
-    const float *gphi; // cross-level face ghosts of phi_in
+    const float *gphi; // cross-level face ghosts of phi_in, or null: the step evaluates the rows itself
+    const int64_t *fptr;  // face-ghost rows (ptr/src/w) for the in-kernel evaluation
+    const int *fsrc;
+    const float *fw;
     const int *level;  // [slot]
     const float *ih2_lvl;  // 1/h^2 per level
     float Dw, Dg, rho, dt;     // used when sprm is null

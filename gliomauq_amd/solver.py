@@ -224,16 +224,22 @@ def set_step_timing(grid, per_kernel):
 
 
 def set_autotune(ctx, mode=1, fixed_zt=16):
-    """mode 1: autotune the step slab depth (16, 8, 4, 2 planes, or 1 = column
-    streaming); 0: size rule; 2: fixed_zt."""
+    """mode 1: autotune the step (slab depth 16, 8, 4, 2 planes or 1 = column
+    streaming; cross-level face ghosts from a separate launch or evaluated in
+    the step); 0: size rule; 2: fixed_zt, separate launch; 3: fixed_zt,
+    evaluated in the step."""
     ctx.call("gq_set_autotune", int(mode), int(fixed_zt))
 
 
 def autotune_info(ctx):
+    """(slab depth, face ghosts in the step?, {"<depth>": us, "<depth>f": us})
+    of the last step call and the last autotune timings."""
     zt = C.c_int()
-    us = np.zeros(5)
-    ctx.call("gq_autotune_info", C.byref(zt), ptr(us, C.c_double))
-    return zt.value, dict(zip((16, 8, 4, 2, 1), us.tolist()))
+    fused = C.c_int()
+    us = np.zeros(10)
+    ctx.call("gq_autotune_info", C.byref(zt), C.byref(fused), ptr(us, C.c_double))
+    names = ["%d" % d for d in (16, 8, 4, 2, 1)] + ["%df" % d for d in (16, 8, 4, 2, 1)]
+    return zt.value, bool(fused.value), dict(zip(names, us.tolist()))
 
 
 def host_stats(ctx):

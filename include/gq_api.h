@@ -247,15 +247,20 @@ gq_status gq_last_step_timing(gq_ctx *ctx, double *step_kernel_ms, int *launches
  * pair of HIP events on the context stream (default 0: one pair around the
  * whole call, which then includes the cross-level ghost kernels). */
 gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel);
-/* Step tile autotuner.  mode 1 (default): time the slab depths 16/8/4/2
- * (planes of a 16^3 block per workgroup) and 1 (one workgroup per block
- * streaming its planes through registers) on the grid the first time a block
- * count of a new power-of-two size is stepped, and keep the fastest; mode 0:
- * a fixed size rule; mode 2: always use fixed_zt. */
+/* Step autotuner.  mode 1 (default): time the slab depths 16/8/4/2 (planes
+ * of a 16^3 block per workgroup) and 1 (one workgroup per block streaming its
+ * planes through registers), each with the cross-level face ghosts from a
+ * separate launch and evaluated inside the step, on the grid the first time a
+ * block count of a new power-of-two size is stepped, and keep the fastest;
+ * mode 0: a fixed size rule (separate face-ghost launch); mode 2: always
+ * fixed_zt with the separate launch; mode 3: always fixed_zt with the
+ * in-step evaluation.  Every configuration gives the same bits. */
 gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt);
-/* Slab depth used by the last gq_fk_steps and the last autotune timings
- * (us per launch for depths 16, 8, 4, 2, 1). */
-gq_status gq_autotune_info(gq_ctx *ctx, int *zt, double us[5]);
+/* Slab depth and face-ghost mode (1 = evaluated in the step) used by the last
+ * step call, and the last autotune timings in us per step: [0..4] depths 16,
+ * 8, 4, 2, 1 with the separate face-ghost launch (both kernels timed), [5..9]
+ * the same depths with the in-step evaluation (0 where not timed). */
+gq_status gq_autotune_info(gq_ctx *ctx, int *zt, int *fused, double us[10]);
 /* Host-side seconds spent in {ghost-table builds, refine/compress planning,
  * FWT calls, splits, collapses, autotuning} and the counts of table builds
  * and FWT calls, accumulated over the context's life. */

@@ -134,27 +134,30 @@ def test_block_processing_mirror(ctx):
 
 
 def test_step_variants_bit_identical():
-    """Every step tile the autotuner may pick (slab depths 16/8/4/2 and the
-    column-streaming kernel) gives the same bits on an adaptive grid with
-    cross-level faces, so the tuning never changes results."""
+    """Every step configuration the autotuner may pick (slab depths 16/8/4/2
+    and the column-streaming kernel, each with the cross-level face ghosts
+    from a separate launch or evaluated inside the step) gives the same bits
+    on an adaptive grid with cross-level faces, so the tuning never changes
+    results."""
     n = 64
     pwm, pgm, _ = uniform_case(n, seed=4)
     # a tumour inside one 32^3 octant: the other seven compress to level 1
     u0 = A.initial_condition(n, (n // 4, n // 4, n // 4), 0.6, 3.0)
     outs = []
-    for zt in (16, 8, 4, 2, 1):
-        ctx = gq.Context(0)
-        gq.set_autotune(ctx, 2, zt)
-        g = gq.Grid(ctx, 4, 3, 4)
-        for name, v in (("phi", u0), ("pw", pwm), ("pg", pgm)):
-            g.write_volume(name, v)
-        gq.Science.AutomaticCompression(g, 1e-3, -1)  # coarse leaves far from the tumour
-        Dw = 0.0013 / L_CM ** 2 * 40
-        h = 1.0 / n
-        gq.fk_steps(g, Dw, 0.1 * Dw, 0.3, 0.9 * h * h / (6 * Dw), 7)
-        assert gq.autotune_info(ctx)[0] == zt
-        outs.append((g.n_blocks(), g.read("phi")))
-        ctx.close()
+    for mode in (2, 3):
+        for zt in (16, 8, 4, 2, 1):
+            ctx = gq.Context(0)
+            gq.set_autotune(ctx, mode, zt)
+            g = gq.Grid(ctx, 4, 3, 4)
+            for name, v in (("phi", u0), ("pw", pwm), ("pg", pgm)):
+                g.write_volume(name, v)
+            gq.Science.AutomaticCompression(g, 1e-3, -1)  # coarse leaves far from the tumour
+            Dw = 0.0013 / L_CM ** 2 * 40
+            h = 1.0 / n
+            gq.fk_steps(g, Dw, 0.1 * Dw, 0.3, 0.9 * h * h / (6 * Dw), 7)
+            assert gq.autotune_info(ctx)[:2] == (zt, mode == 3)
+            outs.append((g.n_blocks(), g.read("phi")))
+            ctx.close()
     assert 8 < outs[0][0] < 64  # the grid is adaptive
     for nb, a in outs[1:]:
         assert nb == outs[0][0] and np.array_equal(a, outs[0][1])
