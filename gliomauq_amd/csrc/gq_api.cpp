@@ -142,6 +142,7 @@ struct gq_ctx {
 
     // step tile (slab depth) chosen per grid-size bucket
     bool autotune = true;
+    bool autotune_fused = false;  // also time the in-step face-ghost evaluation
     // step configuration: slab depth zt (16/8/4/2, 1 = column streaming) and
     // whether the step evaluates the cross-level face-ghost rows itself
     // (fused) or reads them from a separate k_face_ghosts launch
@@ -757,7 +758,7 @@ struct gq_ctx {
         int best = make_cfg(16, false);
         float best_ms = 1e30f;
         for (int k = 0; k < 10; k++) tune_us[k] = 0.0;
-        for (int fused = 0; fused < (nrows > 0 ? 2 : 1); fused++) {
+        for (int fused = 0; fused < (nrows > 0 && autotune_fused ? 2 : 1); fused++) {
             gq::StepArgs b = a;
             if (fused) b.gphi = nullptr;
             for (int k = 0; k < 5; k++) {
@@ -1778,11 +1779,12 @@ gq_status gq_last_step_timing(gq_ctx *ctx, double *ms, int *launches, int64_t *c
 gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt)
 {
     return guard(ctx, [&] {
-        if (mode < 0 || mode > 3) throw gq_error(GQ_ERR_ARG, "autotune mode must be 0, 1, 2 or 3");
+        if (mode < 0 || mode > 4) throw gq_error(GQ_ERR_ARG, "autotune mode must be 0, 1, 2, 3 or 4");
         if (mode >= 2 && fixed_zt != 16 && fixed_zt != 8 && fixed_zt != 4 && fixed_zt != 2 && fixed_zt != 1)
             throw gq_error(GQ_ERR_ARG, "slab depth must be 16, 8, 4, 2 or 1 (column streaming)");
         ctx->zt_cache.clear();
-        ctx->autotune = mode == 1;
+        ctx->autotune = mode == 1 || mode == 4;
+        ctx->autotune_fused = mode == 4;
         if (mode >= 2)
             for (int b = 0; b < 32; b++) ctx->zt_cache[b] = gq_ctx::make_cfg(fixed_zt, mode == 3);
     });

@@ -260,8 +260,25 @@ __global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ p
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = g >> 2, lane = g & 3;
     float acc = 0.f;
-    if (r < nrows)
-        for (int64_t k = ptr[r] + lane; k < ptr[r + 1]; k += 4) acc = __builtin_fmaf(w[k], phi[src[k]], acc);
+    if (r < nrows) {
+        // terms lane, lane+4, ... in order; four of them loaded per round trip
+        const int64_t e = ptr[r + 1];
+        for (int64_t k = ptr[r] + lane; k < e; k += 16) {
+            int si[4];
+            float wk[4], v[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int64_t kk = k + 4 * j < e ? k + 4 * j : k;
+                si[j] = src[kk];
+                wk[j] = w[kk];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = phi[si[j]];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (k + 4 * j < e) acc = __builtin_fmaf(wk[j], v[j], acc);
+        }
+    }
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     if (r < nrows && lane == 0) out[r] = acc;
@@ -298,9 +315,28 @@ __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw,
             const int r = rb + fbase[slot * 6 + f] + t;
             wm = 0.f;
             gm = 0.f;
-            for (int64_t k = ptr[r]; k < ptr[r + 1]; k++) {
-                wm += w[k] * pw[src[k]];
-                gm += w[k] * pg[src[k]];
+            // terms in order, eight loaded per round trip
+            const int64_t e = ptr[r + 1];
+            for (int64_t k = ptr[r]; k < e; k += 8) {
+                int si[8];
+                float wk[8], a[8], b[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int64_t kk = k + j < e ? k + j : k;
+                    si[j] = src[kk];
+                    wk[j] = w[kk];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    a[j] = pw[si[j]];
+                    b[j] = pg[si[j]];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (k + j < e) {
+                        wm = __builtin_fmaf(wk[j], a[j], wm);
+                        gm = __builtin_fmaf(wk[j], b[j], gm);
+                    }
             }
         }
         fik[((size_t)slot * 6 + f) * 256 + t] = inv_k(wm, gm, Dw, Dg);
@@ -550,10 +586,117 @@ __device__ void ghost_dfs(const LabArgs &g, const int *__restrict__ lut, const f
     }
 }
 
+// Synthesis children of the level-lam point p (in a coarser leaf): the
+// children with a non-zero weight w * ((hx*hy)*hz), in increasing (z, y, x)
+// order (per axis: p odd -> the 4 (Interp4) or 2 (Interp2) points around
+// p/2, p even -> p/2 alone; the other points of the support have weight 0
+// and are skipped, as in the general walk).  A batch of children at a time:
+// their look-ups, levels and values are loaded together and, when every child
+// of the batch resolves directly, consumed in order from registers; otherwise the
+// remaining children are walked one by one with the general walk where needed
+// (same order, same arithmetic).  A point in a leaf of its own level sits at
+// cell (p mod 16) per axis.
+template <int MODE, int NF>
+__device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, const float *const *flds, int lam,
+                               const int64_t p[3], double w, double *acc, int *count, int *src, float *wout)
+{
+    const int cl = lam - 1;
+    int64_t i0[3];
+    int n[3];
+    for (int d = 0; d < 3; d++) {
+        if (p[d] & 1) {
+            n[d] = g.order == 4 ? 4 : 2;
+            i0[d] = (p[d] - (g.order == 4 ? 3 : 1)) >> 1;
+        } else {
+            n[d] = 1;
+            i0[d] = p[d] >> 1;
+        }
+    }
+    const int nc = n[0] * n[1] * n[2];
+    constexpr int NB = MODE == 0 ? 4 : 8;  // children per batch (register pressure of the value loads)
+    for (int c0 = 0; c0 < nc; c0 += NB) {
+        // a child i at level cl lies in a leaf of level cl (value at i) or of a
+        // finer level (the walk descends once: the point 2i at level cl+1,
+        // normally in a leaf of that level); both look-ups are issued at once
+        double cw[NB];
+        int sa[NB], sb[NB], ca[NB], cb[NB];
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const int c = c0 + k < nc ? c0 + k : c0;
+            const int64_t ix = i0[0] + c % n[0], iy = i0[1] + (c / n[0]) % n[1], iz = i0[2] + c / (n[0] * n[1]);
+            const double ww = __dmul_rn(
+                __dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), hs_w(g.order, (int)(2 * iy - p[1]))),
+                hs_w(g.order, (int)(2 * iz - p[2])));
+            cw[k] = __dmul_rn(w, ww);
+            sa[k] = lying_slot(g, lut, cl, ix, iy, iz);
+            sb[k] = lying_slot(g, lut, cl + 1, 2 * ix, 2 * iy, 2 * iz);
+            ca[k] = (int)((ix & 15) + 16 * ((iy & 15) + 16 * (iz & 15)));
+            cb[k] = (int)(((2 * ix) & 15) + 16 * (((2 * iy) & 15) + 16 * ((2 * iz) & 15)));
+        }
+        int sl[NB], cell[NB];
+        bool plain = true;  // every child resolved by at most one descent
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const int la = g.level[sa[k]], lb = g.level[sb[k]];
+            const bool fine = la > cl;
+            sl[k] = fine ? sb[k] : sa[k];
+            cell[k] = fine ? cb[k] : ca[k];
+            plain = plain && (c0 + k >= nc || la == cl || (fine && lb == cl + 1));
+        }
+        float val[NB][NF];
+#pragma unroll
+        for (int k = 0; k < NB; k++)
+#pragma unroll
+            for (int f = 0; f < NF; f++) val[k][f] = MODE == 0 ? flds[f][(size_t)sl[k] * 4096 + cell[k]] : 0.f;
+        if (!plain) {
+            for (int c = c0; c < nc; c++) {
+                const int64_t ix = i0[0] + c % n[0], iy = i0[1] + (c / n[0]) % n[1], iz = i0[2] + c / (n[0] * n[1]);
+                const double ww = __dmul_rn(
+                    __dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), hs_w(g.order, (int)(2 * iy - p[1]))),
+                    hs_w(g.order, (int)(2 * iz - p[2])));
+                const double cwc = __dmul_rn(w, ww);
+                const int slot = lying_slot(g, lut, cl, ix, iy, iz);
+                if (g.level[slot] == cl) {
+                    const int ce = (int)((ix & 15) + 16 * ((iy & 15) + 16 * (iz & 15)));
+                    if (MODE == 0) {
+#pragma unroll
+                        for (int f = 0; f < NF; f++)
+                            acc[f] = __dadd_rn(acc[f], __dmul_rn(cwc, (double)flds[f][(size_t)slot * 4096 + ce]));
+                    } else {
+                        if (MODE == 2) {
+                            src[*count] = slot * 4096 + ce;
+                            wout[*count] = (float)cwc;
+                        }
+                        (*count)++;
+                    }
+                } else {
+                    ghost_dfs<MODE, NF>(g, lut, flds, cl, ix, iy, iz, cwc, acc, count, src, wout);
+                }
+            }
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            if (c0 + k >= nc) break;
+            if (MODE == 0) {
+#pragma unroll
+                for (int f = 0; f < NF; f++) acc[f] = __dadd_rn(acc[f], __dmul_rn(cw[k], (double)val[k][f]));
+            } else {
+                if (MODE == 2) {
+                    src[*count] = sl[k] * 4096 + cell[k];
+                    wout[*count] = (float)cw[k];
+                }
+                (*count)++;
+            }
+        }
+    }
+}
+
 // Value of one ghost point: the same walk with a stack-free fast path for
 // the common cases (the point lies in a finer leaf, or in a coarser leaf
-// whose synthesis children all lie in leaves of their own level); anything
-// else continues with the general walk at the child, in the same order.
+// whose synthesis children lie in leaves of their own level or one level
+// finer); anything else continues with the general walk at the child, in the
+// same order.
 template <int NF>
 __device__ void ghost_value(const LabArgs &g, const int *__restrict__ lut, const float *const *flds, int lam,
                             int64_t px, int64_t py, int64_t pz, double *acc)
@@ -562,13 +705,9 @@ __device__ void ghost_value(const LabArgs &g, const int *__restrict__ lut, const
         const int slot = lying_slot(g, lut, lam, px, py, pz);
         const int l = g.level[slot];
         if (l == lam) {
-            const int64_t msk = ((int64_t)16 << lam) - 1;
-            const int cx = (int)((px & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
-            const int cy = (int)((py & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
-            const int cz = (int)((pz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
+            const size_t i = (size_t)slot * 4096 + (px & 15) + 16 * ((py & 15) + 16 * (pz & 15));
 #pragma unroll
-            for (int f = 0; f < NF; f++)
-                acc[f] = __dmul_rn(1.0, (double)flds[f][(size_t)slot * 4096 + cx + 16 * (cy + 16 * cz)]);
+            for (int f = 0; f < NF; f++) acc[f] = __dmul_rn(1.0, (double)flds[f][i]);
             return;
         }
         if (l > lam) {
@@ -580,39 +719,10 @@ __device__ void ghost_value(const LabArgs &g, const int *__restrict__ lut, const
         }
         break;
     }
-    const int lo = g.order == 4 ? -3 : -1, hi = g.order == 4 ? 4 : 2;
-    const int64_t p[3] = {px, py, pz};
-    int64_t s[3], e[3];
-    for (int d = 0; d < 3; d++) {
-        s[d] = (p[d] + lo + 1) >> 1;
-        e[d] = ((p[d] + hi - 1) >> 1) + 1;
-    }
-    const int cl = lam - 1;
-    const int64_t msk = ((int64_t)16 << cl) - 1;
 #pragma unroll
     for (int f = 0; f < NF; f++) acc[f] = 0.0;
-    for (int64_t iz = s[2]; iz < e[2]; iz++) {
-        const double wz = hs_w(g.order, (int)(2 * iz - p[2]));
-        for (int64_t iy = s[1]; iy < e[1]; iy++) {
-            const double wy = hs_w(g.order, (int)(2 * iy - p[1]));
-            for (int64_t ix = s[0]; ix < e[0]; ix++) {
-                const double ww = __dmul_rn(__dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), wy), wz);
-                if (ww == 0.0) continue;
-                const double cw = __dmul_rn(1.0, ww);
-                const int slot = lying_slot(g, lut, cl, ix, iy, iz);
-                if (g.level[slot] == cl) {
-                    const int cx = (int)((ix & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
-                    const int cy = (int)((iy & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
-                    const int cz = (int)((iz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
-                    const size_t i = (size_t)slot * 4096 + cx + 16 * (cy + 16 * cz);
-#pragma unroll
-                    for (int f = 0; f < NF; f++) acc[f] = __dadd_rn(acc[f], __dmul_rn(cw, (double)flds[f][i]));
-                } else {
-                    ghost_dfs<0, NF>(g, lut, flds, cl, ix, iy, iz, cw, acc, nullptr, nullptr, nullptr);
-                }
-            }
-        }
-    }
+    const int64_t p[3] = {px, py, pz};
+    synth_children<0, NF>(g, lut, flds, lam, p, 1.0, acc, nullptr, nullptr, nullptr);
 }
 
 template <int S0, int S1>
@@ -620,46 +730,6 @@ struct LabGeom {
     static constexpr int L = B + S1 - S0 - 1;
     static constexpr int N = L * L * L;
 };
-
-// lab point q -> (x,y,z) and its region (cc = -1 inside the block)
-template <int S0, int S1>
-__device__ __forceinline__ int lab_region(int q, int &x, int &y, int &z, int &dx, int &dy, int &dz)
-{
-    constexpr int L = LabGeom<S0, S1>::L;
-    x = q % L + S0;
-    y = (q / L) % L + S0;
-    z = q / (L * L) + S0;
-    dx = x < 0 ? -1 : (x < B ? 0 : 1);
-    dy = y < 0 ? -1 : (y < B ? 0 : 1);
-    dz = z < 0 ? -1 : (z < B ? 0 : 1);
-    if ((dx | dy | dz) == 0) return -1;
-    const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-    return c < 13 ? c : c - 1;
-}
-
-template <int S0, int S1>
-__device__ void fill_lab(double *lab, const float *__restrict__ fld, const double *__restrict__ ghost, int slot,
-                         const LabArgs &a)
-{
-    constexpr int N = LabGeom<S0, S1>::N;
-    for (int q = threadIdx.x; q < N; q += blockDim.x) {
-        int x, y, z, dx, dy, dz;
-        const int cc = lab_region<S0, S1>(q, x, y, z, dx, dy, dz);
-        double v;
-        if (cc < 0) {
-            v = (double)fld[(size_t)slot * B3 + x + B * (y + B * z)];
-        } else {
-            const int nb = a.nbr26[slot * 26 + cc];
-            if (nb >= 0) {
-                const int lx = x - dx * B, ly = y - dy * B, lz = z - dz * B;
-                v = (double)fld[(size_t)nb * B3 + lx + B * (ly + B * lz)];
-            } else {
-                v = ghost[q];
-            }
-        }
-        lab[q] = v;
-    }
-}
 
 // Lab coordinates a consumer reads outside the block: the FWT filters touch
 // only {-2, 16, 18} (Interp4; the taps at odd offsets -3 and +1 are zero) or
@@ -684,11 +754,67 @@ struct UsedCoords {
         return i == 0 ? -2 : (i <= 16 ? i - 1 : (i == 17 ? B : B + 2));
     }
 };
+
+// Lab points an FWT reads (the block and the used ghost coordinates), FP64,
+// from the block, its same-level neighbours or the evaluated cross-level
+// ghosts.  Each thread gathers K points with all loads issued before any is
+// consumed (one memory round trip per K points, not per point); unread lab
+// points are left unset.
+template <int ORDER>
+__device__ void fill_lab_used(double *lab, const float *__restrict__ fld, const double *__restrict__ ghost, int slot,
+                              const LabArgs &a)
+{
+    constexpr int S0 = ORDER == 4 ? -2 : -1;
+    constexpr int S1 = ORDER == 4 ? 4 : 2;
+    constexpr int L = LabGeom<S0, S1>::L;
+    constexpr int NU = UsedCoords<ORDER, false>::n;
+    constexpr int NQ = NU * NU * NU;
+    constexpr int K = 8;
+    __shared__ int s_nb[26];
+    if (threadIdx.x < 26) s_nb[threadIdx.x] = a.nbr26[slot * 26 + threadIdx.x];
+    __syncthreads();
+    const size_t sbase = (size_t)slot * B3;
+    for (int q0 = threadIdx.x; q0 < NQ; q0 += K * (int)blockDim.x) {
+        size_t fi[K];
+        int gi[K], li[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int q = q0 + k * (int)blockDim.x;
+            const int qq = q < NQ ? q : 0;
+            const int x = UsedCoords<ORDER, false>::at(qq % NU), y = UsedCoords<ORDER, false>::at((qq / NU) % NU),
+                      z = UsedCoords<ORDER, false>::at(qq / (NU * NU));
+            const int dx = x < 0 ? -1 : (x < B ? 0 : 1), dy = y < 0 ? -1 : (y < B ? 0 : 1),
+                      dz = z < 0 ? -1 : (z < B ? 0 : 1);
+            const int l = (x - S0) + L * ((y - S0) + L * (z - S0));
+            li[k] = q < NQ ? l : -1;
+            fi[k] = sbase;
+            gi[k] = -1;
+            if ((dx | dy | dz) == 0) {
+                fi[k] = sbase + x + B * (y + B * z);
+            } else {
+                const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                const int nb = s_nb[c < 13 ? c : c - 1];
+                if (nb >= 0) fi[k] = (size_t)nb * B3 + (x - dx * B) + B * ((y - dy * B) + B * (z - dz * B));
+                else gi[k] = l;
+            }
+        }
+        float fv[K];
+        double gv[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            fv[k] = fld[fi[k]];
+            gv[k] = ghost[gi[k] >= 0 ? gi[k] : 0];
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (li[k] >= 0) lab[li[k]] = gi[k] >= 0 ? gv[k] : (double)fv[k];
+    }
+}
 }  // namespace
 
 // One thread per lab point a consumer reads outside the block, blockIdx.y =
 // listed slot: the cross-level ones are evaluated (NF fields, one walk),
-// the others are copied by fill_lab from the same-level neighbour.
+// the others are copied by fill_lab_used from the same-level neighbour.
 template <int ORDER, bool FOR_SPLIT, int NF>
 __global__ __launch_bounds__(256) void k_lab_ghosts(LabArgs a, FieldPtrs flds, const int *__restrict__ slots,
                                                     double *__restrict__ ghost, size_t field_stride)
@@ -729,11 +855,7 @@ __device__ void ghost_terms(const LabArgs &g, const int *__restrict__ lut, int l
         const int l = g.level[slot];
         if (l == lam) {
             if (MODE == 2) {
-                const int64_t msk = ((int64_t)16 << lam) - 1;
-                const int cx = (int)((px & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
-                const int cy = (int)((py & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
-                const int cz = (int)((pz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
-                src[*count] = slot * 4096 + cx + 16 * (cy + 16 * cz);
+                src[*count] = slot * 4096 + (int)((px & 15) + 16 * ((py & 15) + 16 * (pz & 15)));
                 wout[*count] = 1.0f;
             }
             (*count)++;
@@ -748,40 +870,9 @@ __device__ void ghost_terms(const LabArgs &g, const int *__restrict__ lut, int l
         }
         break;
     }
-    const int lo = g.order == 4 ? -3 : -1, hi = g.order == 4 ? 4 : 2;
     const int64_t p[3] = {px, py, pz};
-    int64_t s[3], e[3];
-    for (int d = 0; d < 3; d++) {
-        s[d] = (p[d] + lo + 1) >> 1;
-        e[d] = ((p[d] + hi - 1) >> 1) + 1;
-    }
-    const int cl = lam - 1;
-    const int64_t msk = ((int64_t)16 << cl) - 1;
     double unused = 0.0;
-    for (int64_t iz = s[2]; iz < e[2]; iz++) {
-        const double wz = hs_w(g.order, (int)(2 * iz - p[2]));
-        for (int64_t iy = s[1]; iy < e[1]; iy++) {
-            const double wy = hs_w(g.order, (int)(2 * iy - p[1]));
-            for (int64_t ix = s[0]; ix < e[0]; ix++) {
-                const double ww = __dmul_rn(__dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), wy), wz);
-                if (ww == 0.0) continue;
-                const double cw = __dmul_rn(1.0, ww);
-                const int slot = lying_slot(g, lut, cl, ix, iy, iz);
-                if (g.level[slot] == cl) {
-                    if (MODE == 2) {
-                        const int cx = (int)((ix & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
-                        const int cy = (int)((iy & msk) - (int64_t)g.index[slot * 3 + 1] * 16);
-                        const int cz = (int)((iz & msk) - (int64_t)g.index[slot * 3 + 2] * 16);
-                        src[*count] = slot * 4096 + cx + 16 * (cy + 16 * cz);
-                        wout[*count] = (float)cw;
-                    }
-                    (*count)++;
-                } else {
-                    ghost_dfs<MODE>(g, lut, nullptr, cl, ix, iy, iz, cw, &unused, count, src, wout);
-                }
-            }
-        }
-    }
+    synth_children<MODE, 1>(g, lut, nullptr, lam, p, 1.0, &unused, count, src, wout);
 }
 }  // namespace
 
@@ -898,7 +989,7 @@ __global__ __launch_bounds__(512) void k_fwt_detail(LabArgs a, const double *__r
     __shared__ double lab[LabGeom<S0, S1>::N];
     __shared__ double red[8];
     const int slot = cand[blockIdx.x];
-    fill_lab<S0, S1>(lab, phi, ghost + (size_t)blockIdx.x * LabGeom<S0, S1>::N, slot, a);
+    fill_lab_used<ORDER>(lab, phi, ghost + (size_t)blockIdx.x * LabGeom<S0, S1>::N, slot, a);
     __syncthreads();
     double m = 0.0;
     {
@@ -955,20 +1046,44 @@ __global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restri
     const int cx = child & 1, cy = (child >> 1) & 1, cz = child >> 2;
     float *fld = f == 0 ? phi : (f == 1 ? pw : pg);
     const double *ghost = ghost3 + ((size_t)f * nparents + pidx) * LabGeom<S0, S1>::N;
-    for (int q = threadIdx.x; q < WL * WL * WL; q += 256) {
-        const int x = 8 * cx + W0 + q % WL, y = 8 * cy + W0 + (q / WL) % WL, z = 8 * cz + W0 + q / (WL * WL);
-        const int dx = x < 0 ? -1 : (x < B ? 0 : 1), dy = y < 0 ? -1 : (y < B ? 0 : 1),
-                  dz = z < 0 ? -1 : (z < B ? 0 : 1);
-        double v;
-        if ((dx | dy | dz) == 0) {
-            v = (double)fld[(size_t)pslot * B3 + x + B * (y + B * z)];
-        } else {
-            const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-            const int nb = a.nbr26[pslot * 26 + (c < 13 ? c : c - 1)];
-            if (nb >= 0) v = (double)fld[(size_t)nb * B3 + (x - dx * B) + B * ((y - dy * B) + B * (z - dz * B))];
-            else v = ghost[(x - S0) + L * ((y - S0) + L * (z - S0))];
+    __shared__ int s_nb[26];
+    if (threadIdx.x < 26) s_nb[threadIdx.x] = a.nbr26[pslot * 26 + threadIdx.x];
+    __syncthreads();
+    {
+        // every thread gathers its window points with all loads in flight at once
+        constexpr int NW = WL * WL * WL, K = (NW + 255) / 256;
+        size_t fi[K];
+        int gi[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int q = threadIdx.x + 256 * k;
+            const int qq = q < NW ? q : 0;
+            const int x = 8 * cx + W0 + qq % WL, y = 8 * cy + W0 + (qq / WL) % WL, z = 8 * cz + W0 + qq / (WL * WL);
+            const int dx = x < 0 ? -1 : (x < B ? 0 : 1), dy = y < 0 ? -1 : (y < B ? 0 : 1),
+                      dz = z < 0 ? -1 : (z < B ? 0 : 1);
+            fi[k] = (size_t)pslot * B3;
+            gi[k] = -1;
+            if ((dx | dy | dz) == 0) {
+                fi[k] = (size_t)pslot * B3 + x + B * (y + B * z);
+            } else {
+                const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                const int nb = s_nb[c < 13 ? c : c - 1];
+                if (nb >= 0) fi[k] = (size_t)nb * B3 + (x - dx * B) + B * ((y - dy * B) + B * (z - dz * B));
+                else gi[k] = (x - S0) + L * ((y - S0) + L * (z - S0));
+            }
         }
-        win[q] = v;
+        float fv[K];
+        double gv[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            fv[k] = fld[fi[k]];
+            gv[k] = ghost[gi[k] >= 0 ? gi[k] : 0];
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int q = threadIdx.x + 256 * k;
+            if (q < NW) win[q] = gi[k] >= 0 ? gv[k] : (double)fv[k];
+        }
     }
     __syncthreads();
     const int cs = children[pidx * 8 + child];
