@@ -116,7 +116,7 @@ struct gq_ctx {
     std::vector<int> work_begin;       // [S+1] per position in sample_order
     std::vector<int> face_begin;       // [S+1] cross-level faces per position
     DevBuf<int> d_work, d_level, d_index, d_fnbr, d_fbase, d_face_src, d_nbr26, d_lut, d_lutf, d_lutoff,
-        d_slot_sample, d_faces, d_counts, d_sfbase, d_packed;
+        d_slot_sample, d_faces, d_counts, d_sfbase, d_packed, d_items;
     DevBuf<int64_t> d_face_ptr;
     DevBuf<char> d_scan_tmp;
     int face_nrows = 0;
@@ -142,7 +142,7 @@ struct gq_ctx {
 
     // step tile (slab depth) chosen per grid-size bucket
     bool autotune = true;
-    bool autotune_fused = false;  // also time the in-step face-ghost evaluation
+    bool autotune_all = false;  // also time the column kernel and the in-step face-ghost evaluation
     // step configuration: slab depth zt (16/8/4/2, 1 = column streaming) and
     // whether the step evaluates the cross-level face-ghost rows itself
     // (fused) or reads them from a separate k_face_ghosts launch
@@ -508,7 +508,10 @@ struct gq_ctx {
         const int order = samples[0].grid->order();
         d_ghost.reserve(slots.size() * (size_t)gq::lab_points(order));
         const float *fl[1] = {phi[cur]};
-        HIPCHK(gq::launch_lab_ghosts(la, fl, 1, d_list.p, (int)slots.size(), d_ghost.p, 0, false, st));
+        std::vector<int> items;
+        gq::lab_items(slots, h_nbr26, order, items);
+        d_items.upload(items, st);
+        HIPCHK(gq::launch_lab_ghosts(la, fl, 1, d_list.p, d_items.p, (int)items.size(), d_ghost.p, 0, false, st));
         HIPCHK(gq::launch_fwt_detail(order, la, d_ghost.p, phi[cur], d_list.p, (int)slots.size(), d_dbl.p, st));
         HIPCHK(hipMemcpyAsync(out.data(), d_dbl.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -562,7 +565,10 @@ struct gq_ctx {
         const size_t per = pslots.size() * (size_t)gq::lab_points(order);
         d_ghost.reserve(3 * per);
         const float *fl[3] = {phi[cur], pw, pg};
-        HIPCHK(gq::launch_lab_ghosts(la, fl, 3, d_list.p, (int)pslots.size(), d_ghost.p, per, true, st));
+        std::vector<int> items;
+        gq::lab_items(pslots, h_nbr26, order, items);
+        d_items.upload(items, st);
+        HIPCHK(gq::launch_lab_ghosts(la, fl, 3, d_list.p, d_items.p, (int)items.size(), d_ghost.p, per, true, st));
         HIPCHK(gq::launch_split(order, la, d_ghost.p, phi[cur], pw, pg, d_list.p, d_list2.p, (int)pslots.size(),
                                 st));
         HIPCHK(hipStreamSynchronize(st));
@@ -758,10 +764,11 @@ struct gq_ctx {
         int best = make_cfg(16, false);
         float best_ms = 1e30f;
         for (int k = 0; k < 10; k++) tune_us[k] = 0.0;
-        for (int fused = 0; fused < (nrows > 0 && autotune_fused ? 2 : 1); fused++) {
+        const int ncand = autotune_all ? 5 : 4;
+        for (int fused = 0; fused < (nrows > 0 && autotune_all ? 2 : 1); fused++) {
             gq::StepArgs b = a;
             if (fused) b.gphi = nullptr;
-            for (int k = 0; k < 5; k++) {
+            for (int k = 0; k < ncand; k++) {
                 auto rep = [&] {
                     if (nrows > 0 && !fused)
                         HIPCHK(gq::launch_face_ghosts(phi[cur], d_face_ptr.p, d_face_src.p, d_face_w.p, d_gphi.p,
@@ -1784,7 +1791,7 @@ gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt)
             throw gq_error(GQ_ERR_ARG, "slab depth must be 16, 8, 4, 2 or 1 (column streaming)");
         ctx->zt_cache.clear();
         ctx->autotune = mode == 1 || mode == 4;
-        ctx->autotune_fused = mode == 4;
+        ctx->autotune_all = mode == 4;
         if (mode >= 2)
             for (int b = 0; b < 32; b++) ctx->zt_cache[b] = gq_ctx::make_cfg(fixed_zt, mode == 3);
     });

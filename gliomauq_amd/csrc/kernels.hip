@@ -812,34 +812,58 @@ __device__ void fill_lab_used(double *lab, const float *__restrict__ fld, const 
 }
 }  // namespace
 
-// One thread per lab point a consumer reads outside the block, blockIdx.y =
-// listed slot: the cross-level ones are evaluated (NF fields, one walk),
-// the others are copied by fill_lab_used from the same-level neighbour.
+// Index range of the used lab coordinates of one side of an axis (d = -1:
+// below the block, 0: inside, +1: above) in UsedCoords order.
+template <int ORDER>
+__device__ __forceinline__ void used_range(int d, int &start, int &count)
+{
+    if (ORDER == 4) {
+        start = d < 0 ? 0 : (d == 0 ? 1 : 17);
+        count = d < 0 ? 1 : (d == 0 ? 16 : 2);
+    } else {
+        start = d < 0 ? 0 : (d == 0 ? 0 : 16);
+        count = d < 0 ? 0 : (d == 0 ? 16 : 1);
+    }
+}
+
+// The lab points a consumer reads in the cross-level regions of the listed
+// slots.  One 64-thread workgroup per item = (listed slot j, region r, chunk
+// of 64 points): j << 8 | r << 3 | chunk, the 26 regions around the block
+// with the centre left out; the host lists only regions whose neighbour is
+// not a same-level leaf (fill_lab_used and k_split copy the others directly).
+// Each thread evaluates one point (NF fields in one walk).
 template <int ORDER, bool FOR_SPLIT, int NF>
-__global__ __launch_bounds__(256) void k_lab_ghosts(LabArgs a, FieldPtrs flds, const int *__restrict__ slots,
-                                                    double *__restrict__ ghost, size_t field_stride)
+__global__ __launch_bounds__(64) void k_lab_ghosts(LabArgs a, FieldPtrs flds, const int *__restrict__ slots,
+                                                    const int *__restrict__ items, double *__restrict__ ghost,
+                                                    size_t field_stride)
 {
     constexpr int S0 = ORDER == 4 ? -2 : -1;
     constexpr int S1 = ORDER == 4 ? 4 : 2;
     constexpr int L = LabGeom<S0, S1>::L;
-    constexpr int NU = UsedCoords<ORDER, FOR_SPLIT>::n;
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= NU * NU * NU) return;
-    const int x = UsedCoords<ORDER, FOR_SPLIT>::at(q % NU), y = UsedCoords<ORDER, FOR_SPLIT>::at((q / NU) % NU),
-              z = UsedCoords<ORDER, FOR_SPLIT>::at(q / (NU * NU));
-    const int dx = x < 0 ? -1 : (x >= B ? 1 : 0), dy = y < 0 ? -1 : (y >= B ? 1 : 0),
-              dz = z < 0 ? -1 : (z >= B ? 1 : 0);
-    if (dx == 0 && dy == 0 && dz == 0) return;  // interior
-    const int c = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-    const int slot = slots[blockIdx.y];
-    if (a.nbr26[slot * 26 + (c < 13 ? c : c - 1)] >= 0) return;  // same-level region
+    const int it = items[blockIdx.x];
+    const int j = it >> 8, r = (it >> 3) & 31;
+    const int c = r < 13 ? r : r + 1;  // region code (dx+1) + 3(dy+1) + 9(dz+1)
+    int sx, nx, sy, ny, sz, nz;
+    used_range<ORDER>(c % 3 - 1, sx, nx);
+    used_range<ORDER>((c / 3) % 3 - 1, sy, ny);
+    used_range<ORDER>(c / 9 - 1, sz, nz);
+    const int npts = nx * ny * nz;
+    const int slot = slots[j];
     const int64_t ox = (int64_t)a.index[slot * 3 + 0] * B, oy = (int64_t)a.index[slot * 3 + 1] * B,
                   oz = (int64_t)a.index[slot * 3 + 2] * B;
-    double v[NF];
-    ghost_value<NF>(a, a.lut + a.lutoff[slot], flds.p, a.level[slot], ox + x, oy + y, oz + z, v);
-    const size_t o = (size_t)blockIdx.y * LabGeom<S0, S1>::N + (x - S0) + L * ((y - S0) + L * (z - S0));
+    const int *lut = a.lut + a.lutoff[slot];
+    const int lvl = a.level[slot];
+    const int q = (it & 7) * 64 + (int)threadIdx.x;
+    if (q < npts) {
+        const int x = UsedCoords<ORDER, FOR_SPLIT>::at(sx + q % nx),
+                  y = UsedCoords<ORDER, FOR_SPLIT>::at(sy + (q / nx) % ny),
+                  z = UsedCoords<ORDER, FOR_SPLIT>::at(sz + q / (nx * ny));
+        double v[NF];
+        ghost_value<NF>(a, lut, flds.p, lvl, ox + x, oy + y, oz + z, v);
+        const size_t o = (size_t)j * LabGeom<S0, S1>::N + (x - S0) + L * ((y - S0) + L * (z - S0));
 #pragma unroll
-    for (int f = 0; f < NF; f++) ghost[f * field_stride + o] = v[f];
+        for (int f = 0; f < NF; f++) ghost[f * field_stride + o] = v[f];
+    }
 }
 
 namespace {
@@ -1347,16 +1371,35 @@ hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, i
 
 int lab_points(int order) { return order == 4 ? 21 * 21 * 21 : 18 * 18 * 18; }
 
-hipError_t launch_lab_ghosts(const LabArgs &a, const float *const *flds, int nf, const int *slots, int nslots,
-                             double *ghost, size_t field_stride, bool for_split, hipStream_t st)
+void lab_items(const std::vector<int> &slots, const std::vector<int> &nbr26, int order, std::vector<int> &items)
 {
-    if (nslots <= 0) return hipSuccess;
+    items.clear();
+    for (size_t j = 0; j < slots.size(); j++) {
+        const int *nb = nbr26.data() + (size_t)slots[j] * 26;
+        for (int r = 0; r < 26; r++) {
+            if (nb[r] >= 0) continue;  // same-level neighbour: copied, not evaluated
+            const int c = r < 13 ? r : r + 1;
+            // Interp2 reads no lab coordinate below the block
+            if (order == 2 && (c % 3 == 0 || (c / 3) % 3 == 0 || c / 9 == 0)) continue;
+            // used points per axis side: below 1 (Interp4), inside 16, above 2 (Interp4) or 1 (Interp2)
+            int npts = 1;
+            for (int d : {c % 3, (c / 3) % 3, c / 9}) npts *= d == 0 ? 1 : (d == 1 ? 16 : (order == 4 ? 2 : 1));
+            for (int k = 0; k * 64 < npts; k++) items.push_back((int)j << 8 | r << 3 | k);
+        }
+    }
+}
+
+hipError_t launch_lab_ghosts(const LabArgs &a, const float *const *flds, int nf, const int *slots,
+                             const int *items, int nitems, double *ghost, size_t field_stride, bool for_split,
+                             hipStream_t st)
+{
+    if (nitems <= 0) return hipSuccess;
     if (nf != 1 && nf != 3) return hipErrorInvalidValue;
     FieldPtrs fp;
     for (int f = 0; f < 3; f++) fp.p[f] = flds[f < nf ? f : 0];
-    const int nu = a.order == 4 ? 19 : 17;
-    const dim3 grid((nu * nu * nu + 255) / 256, nslots);
-#define GQ_LAB(O, S, N) hipLaunchKernelGGL((k_lab_ghosts<O, S, N>), grid, dim3(256), 0, st, a, fp, slots, ghost, field_stride)
+    const dim3 grid(nitems);
+#define GQ_LAB(O, S, N) \
+    hipLaunchKernelGGL((k_lab_ghosts<O, S, N>), grid, dim3(64), 0, st, a, fp, slots, items, ghost, field_stride)
     if (a.order == 4) {
         if (for_split) {
             if (nf == 3) GQ_LAB(4, true, 3);

@@ -1,5 +1,6 @@
 // kernels.hpp -- argument blocks and launchers of the HIP kernels.
 #pragma once
+#include <vector>
 
 #include <hip/hip_runtime.h>
 
@@ -105,8 +106,16 @@ hipError_t launch_scatter_slot_rows(const SlotTables &t, const int *packed, int 
 struct FieldPtrs {
     const float *p[3];
 };
-hipError_t launch_lab_ghosts(const LabArgs &a, const float *const *flds, int nf, const int *slots, int nslots,
-                             double *ghost, size_t field_stride, bool for_split, hipStream_t st);
+// Cross-level lab points of the listed slots: items = j << 8 | region << 3 |
+// chunk (region 0..25: the regions around the block in (dx, dy, dz) order,
+// centre left out; chunks of 64 points) for the regions whose neighbour is
+// not a same-level leaf (lab_items builds them).
+hipError_t launch_lab_ghosts(const LabArgs &a, const float *const *flds, int nf, const int *slots,
+                             const int *items, int nitems, double *ghost, size_t field_stride, bool for_split,
+                             hipStream_t st);
+// Host: the items of launch_lab_ghosts for `slots` from the slot-indexed
+// same-level neighbour table nbr26 (regions without used points skipped).
+void lab_items(const std::vector<int> &slots, const std::vector<int> &nbr26, int order, std::vector<int> &items);
 int lab_points(int order);  // points of the wide lab (Interp4: 21^3, Interp2: 18^3)
 hipError_t launch_fwt_detail(int order, const LabArgs &a, const double *ghost, const float *phi, const int *cand,
                              int ncand, double *out, hipStream_t st);

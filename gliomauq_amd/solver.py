@@ -224,10 +224,11 @@ def set_step_timing(grid, per_kernel):
 
 
 def set_autotune(ctx, mode=1, fixed_zt=16):
-    """mode 1: autotune the step slab depth (16, 8, 4, 2 planes or 1 = column
-    streaming) with the cross-level face ghosts from a separate launch; 4: the
-    same plus every depth with the face ghosts evaluated in the step; 0: size
-    rule; 2: fixed_zt, separate launch; 3: fixed_zt, evaluated in the step."""
+    """mode 1: autotune the step slab depth (16, 8, 4, 2 planes) with the
+    cross-level face ghosts from a separate launch; 4: also the column kernel
+    (depth 1) and every depth with the face ghosts evaluated in the step; 0:
+    size rule; 2: fixed_zt (16/8/4/2/1), separate launch; 3: fixed_zt,
+    evaluated in the step."""
     ctx.call("gq_set_autotune", int(mode), int(fixed_zt))
 
 

@@ -248,13 +248,14 @@ gq_status gq_last_step_timing(gq_ctx *ctx, double *step_kernel_ms, int *launches
  * whole call, which then includes the cross-level ghost kernels). */
 gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel);
 /* Step autotuner.  mode 1 (default): time the slab depths 16/8/4/2 (planes
- * of a 16^3 block per workgroup) and 1 (one workgroup per block streaming its
- * planes through registers), with the cross-level face ghosts from a separate
- * launch, on the grid the first time a block count of a new power-of-two
- * size is stepped, and keep the fastest; mode 4: the same, each depth also
- * with the face ghosts evaluated inside the step; mode 0: a fixed size rule;
- * mode 2: always fixed_zt with the separate launch; mode 3: always fixed_zt
- * with the in-step evaluation.  Every configuration gives the same bits. */
+ * of a 16^3 block per workgroup) with the cross-level face ghosts from a
+ * separate launch, on the grid the first time a block count of a new
+ * power-of-two size is stepped, and keep the fastest; mode 4: also time
+ * depth 1 (one workgroup per block streaming its planes through registers)
+ * and every depth with the face ghosts evaluated inside the step; mode 0: a
+ * fixed size rule; mode 2: always fixed_zt (16/8/4/2/1) with the separate
+ * launch; mode 3: always fixed_zt with the in-step evaluation.  Every
+ * configuration gives the same bits. */
 gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt);
 /* Slab depth and face-ghost mode (1 = evaluated in the step) used by the last
  * step call, and the last autotune timings in us per step: [0..4] depths 16,
