@@ -355,8 +355,17 @@ def surrogate_workload(args):
     log("[bench] surrogate: truth + %d training fields (%d steps at 128^3)" % (args.sur_fields, steps))
     u_true = solve_fields(tt[None], gen)[0]
     obs64 = A.observations(u_true, seed=args.seed)
-    # training parameters: a box around the truth
-    span = np.array([0.15, 0.15, 1.5, 1.5, 1.5, 0.02])
+    # samplers on the full posterior (observations at 64^3)
+    post_b = SM.Posterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=C)
+    post_1 = SM.Posterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=1)
+    log("[bench] scale probe")
+    scale = SM.probe_scales(post_b, tt, drop=2.0)
+    # training parameters: a fixed box around the mode, or (--sur-box probe) the
+    # mode +- K probe scales, the region the chains explore
+    if args.sur_box == "probe":
+        span = args.sur_box_k * scale
+    else:
+        span = np.array([0.15, 0.15, 1.5, 1.5, 1.5, 0.02])
     t0 = time.perf_counter()
     ths = tt + span * rng.uniform(-1, 1, (args.sur_fields + 64, 6))
     fields = solve_fields(ths, gen)
@@ -414,11 +423,6 @@ def surrogate_workload(args):
         print(json.dumps({"metric": "surrogate training only", "surrogate": sur_res}), flush=True)
         return
 
-    # samplers on the full posterior (observations at 64^3)
-    post_b = SM.Posterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=C)
-    post_1 = SM.Posterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=1)
-    log("[bench] scale probe")
-    scale = SM.probe_scales(post_b, tt, drop=2.0)
     start = tt + 0.3 * scale * rng.standard_normal((C, 6))
     budget = args.mcmc_budget
     res_s = {}
@@ -454,9 +458,10 @@ def surrogate_workload(args):
     log("[bench] delayed-acceptance RWM")
     o, a, lf, nf0 = SM.da_rwm(post_b, spost, start, 20, scale, rng, adapt_until=20)
     nfull = [0]
+    diag = []
 
     def step_da(st, xs, acs):
-        o, a, lf, nf = SM.da_rwm(post_b, spost, st[0], 5, scale, rng, logf=st[1])
+        o, a, lf, nf = SM.da_rwm(post_b, spost, st[0], 5, scale, rng, logf=st[1], diag=diag)
         nfull[0] += nf
         xs.append(o)
         acs.append(a)
@@ -485,6 +490,16 @@ def surrogate_workload(args):
                        "posterior_mean": x.reshape(-1, 6).mean(0).tolist()}
     rates["da_rwm_16"]["full_solves"] = nfull[0]
     rates["da_rwm_16"]["proposals"] = int(x_d.shape[0] * C)
+    if diag:
+        ds = np.concatenate([d[0] for d in diag])
+        df = np.concatenate([d[1] for d in diag])
+        ok = np.isfinite(ds) & np.isfinite(df)
+        ds, df = ds[ok], df[ok]
+        if len(ds) > 2:
+            # does the surrogate order the proposals it passes on the way the full posterior does?
+            rates["da_rwm_16"]["stage2_corr_sur_full"] = float(np.corrcoef(ds, df)[0, 1])
+            rates["da_rwm_16"]["stage2_slope_full_on_sur"] = float(np.polyfit(ds, df, 1)[0])
+            rates["da_rwm_16"]["stage2_mean_dlogpost_sur_full"] = [float(ds.mean()), float(df.mean())]
     ref_mean = np.array(rates["rwm_16"]["posterior_mean"])
     for k in rates:
         rates[k]["mean_rel_dev_vs_rwm16"] = (np.abs(np.array(rates[k]["posterior_mean"]) - ref_mean) /
@@ -497,7 +512,8 @@ def surrogate_workload(args):
         "vs_baseline": None, "dtype": "bf16 (surrogate) / f32 (solver)", "data": "synthetic",
         "config": {"workload": "surrogate + delayed acceptance: 128^3 uniform solver, %d steps, observations 64^3; "
                                "surrogate 64^3 encoder-decoder (16-128 channels, 3^3), bf16 MFMA, %d fields, "
-                               "%d epochs" % (steps, ntr, args.sur_epochs), "seed": args.seed, "best": best},
+                               "%d epochs" % (steps, ntr, args.sur_epochs), "seed": args.seed, "best": best,
+                   "training_box": args.sur_box, "training_span": span.tolist(), "probe_scale": scale.tolist()},
         "surrogate": sur_res,
         "mcmc": rates,
         "speedup_vs_plain": {k: rates[k]["ess_per_hour"] / max(rates["rwm_plain"]["ess_per_hour"], 1e-30)
@@ -582,6 +598,9 @@ def main():
     ap.add_argument("--sur-lr", type=float, default=3e-4)
     ap.add_argument("--sur-gen-batch", type=int, default=64)
     ap.add_argument("--sur-train-only", action="store_true")
+    ap.add_argument("--sur-box", default="fixed", choices=["fixed", "probe"],
+                    help="training parameters: fixed box around the mode, or mode +- K probe scales")
+    ap.add_argument("--sur-box-k", type=float, default=4.0)
     ap.add_argument("--mcmc-n", type=int, default=64)
     ap.add_argument("--mcmc-steps", type=int, default=500)
     ap.add_argument("--mcmc-pilot", type=int, default=40)

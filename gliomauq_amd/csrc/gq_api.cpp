@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <exception>
+#include <initializer_list>
 #include <chrono>
 #include <map>
 #include <cmath>
@@ -64,6 +65,31 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Host int arrays copied to the device in one transfer: the parts lie back to
+// back (each at a 64-int boundary) in one device buffer.  The host copy stays
+// alive in the object; callers synchronise the stream before the next upload
+// through the same object.
+struct PackedInts {
+    DevBuf<int> dev;
+    std::vector<int> host;
+    std::vector<size_t> off;
+    void upload(std::initializer_list<const std::vector<int> *> parts, hipStream_t st)
+    {
+        size_t total = 0;
+        off.clear();
+        for (const std::vector<int> *v : parts) {
+            off.push_back(total);
+            total += (v->size() + 63) & ~(size_t)63;
+        }
+        host.resize(std::max<size_t>(total, 1));
+        size_t k = 0;
+        for (const std::vector<int> *v : parts) std::copy(v->begin(), v->end(), host.begin() + off[k++]);
+        dev.reserve(host.size());
+        if (total) HIPCHK(hipMemcpyAsync(dev.p, host.data(), total * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    int *ptr(int i) const { return dev.p + off[i]; }
+};
+
 double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -115,8 +141,12 @@ struct gq_ctx {
     std::vector<int> work;             // live slots: samples in sample_order, leaves in Morton order
     std::vector<int> work_begin;       // [S+1] per position in sample_order
     std::vector<int> face_begin;       // [S+1] cross-level faces per position
-    DevBuf<int> d_work, d_level, d_index, d_fnbr, d_fbase, d_face_src, d_nbr26, d_lut, d_lutf, d_lutoff,
-        d_slot_sample, d_faces, d_counts, d_sfbase, d_packed, d_items;
+    DevBuf<int> d_level, d_index, d_fnbr, d_fbase, d_face_src, d_nbr26, d_lutoff, d_slot_sample, d_counts;
+    // per-topology lists in one transfer: work list, sample face-row bases, base-level and finest look-ups,
+    // cross-level faces, changed slot rows
+    PackedInts tabs;
+    int *p_work = nullptr, *p_sfbase = nullptr, *p_lut = nullptr, *p_lutf = nullptr, *p_faces = nullptr;
+    PackedInts lists;  // slot lists of one adaptation kernel sequence
     DevBuf<int64_t> d_face_ptr;
     DevBuf<char> d_scan_tmp;
     int face_nrows = 0;
@@ -128,7 +158,7 @@ struct gq_ctx {
     std::vector<float> fik_key;   // Dw, Dg (per sample when per-sample parameters are set)
 
     // scratch
-    DevBuf<int> d_list, d_list2;
+    DevBuf<int> d_list;
     DevBuf<float> d_dense;
     DevBuf<double> d_dbl, d_partial;
 
@@ -388,15 +418,17 @@ struct gq_ctx {
                     for (int c = 0; c < 26; c++) q[19 + c] = h_nbr26[(size_t)sl * 26 + c];
                     r++;
                 }
-            d_packed.upload(packed, st);
-            gq::SlotTables t{d_level.p, d_index.p, d_slot_sample.p, d_lutoff.p, d_fnbr.p, d_fbase.p, d_nbr26.p};
-            HIPCHK(gq::launch_scatter_slot_rows(t, d_packed.p, (int)nrow, st));
         }
-        d_work.upload(work, st);
-        d_sfbase.upload(sfbase, st);
-        d_lut.upload(lut_slot, st);
-        d_lutf.upload(lutf, st);
-        d_faces.upload(faces, st);
+        tabs.upload({&work, &sfbase, &lut_slot, &lutf, &faces, &packed}, st);
+        p_work = tabs.ptr(0);
+        p_sfbase = tabs.ptr(1);
+        p_lut = tabs.ptr(2);
+        p_lutf = tabs.ptr(3);
+        p_faces = tabs.ptr(4);
+        if (!packed.empty()) {
+            gq::SlotTables t{d_level.p, d_index.p, d_slot_sample.p, d_lutoff.p, d_fnbr.p, d_fbase.p, d_nbr26.p};
+            HIPCHK(gq::launch_scatter_slot_rows(t, tabs.ptr(5), (int)(packed.size() / ROW), st));
+        }
         face_nrows = (int)faces.size() * 256;
         face_rows_valid = false;
         if (d_fik.n < (size_t)cap * 6 * 256) {
@@ -429,7 +461,7 @@ struct gq_ctx {
         if (nfaces > 0) {
             const gq::LabArgs la = lab_args();
             d_counts.reserve((size_t)face_nrows);
-            HIPCHK(gq::launch_face_rows_count(la, d_faces.p, nfaces, d_counts.p, st));
+            HIPCHK(gq::launch_face_rows_count(la, p_faces, nfaces, d_counts.p, st));
             size_t tb = 0;
             HIPCHK(gq::scan_counts(d_counts.p, d_face_ptr.p, face_nrows, nullptr, tb, st));
             d_scan_tmp.reserve(tb + 1);
@@ -440,7 +472,7 @@ struct gq_ctx {
             if (total > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "face rows exceed 2^31 terms");
             d_face_src.reserve((size_t)std::max<int64_t>(1, total));
             d_face_w.reserve((size_t)std::max<int64_t>(1, total));
-            HIPCHK(gq::launch_face_rows_fill(la, d_faces.p, nfaces, d_face_ptr.p, d_face_src.p, d_face_w.p, st));
+            HIPCHK(gq::launch_face_rows_fill(la, p_faces, nfaces, d_face_ptr.p, d_face_src.p, d_face_w.p, st));
         }
         d_gphi.reserve((size_t)std::max(1, face_nrows));
         face_rows_valid = true;
@@ -450,7 +482,7 @@ struct gq_ctx {
     {
         gq::LabArgs a;
         a.nbr26 = d_nbr26.p;
-        a.lut = d_lutf.p;
+        a.lut = p_lutf;
         a.lutoff = d_lutoff.p;
         a.level = d_level.p;
         a.index = d_index.p;
@@ -476,7 +508,7 @@ struct gq_ctx {
             any = any || fik_stale[s];
         }
         if (!any) return;
-        const int *list = d_work.p;
+        const int *list = p_work;
         int n = (int)work.size();
         if (!all) {
             std::vector<int> sl;
@@ -488,7 +520,7 @@ struct gq_ctx {
             n = (int)sl.size();
             HIPCHK(hipStreamSynchronize(st));  // sl is released on return
         }
-        HIPCHK(gq::launch_face_prep(pw, pg, list, n, d_fnbr.p, d_fbase.p, d_sfbase.p, d_slot_sample.p, d_face_ptr.p,
+        HIPCHK(gq::launch_face_prep(pw, pg, list, n, d_fnbr.p, d_fbase.p, p_sfbase, d_slot_sample.p, d_face_ptr.p,
                                     d_face_src.p, d_face_w.p, d_fik.p, key[0], key[1], per_sample ? d_sprm.p : nullptr,
                                     st));
         fik_stale.assign(S, 0);
@@ -502,7 +534,6 @@ struct gq_ctx {
         if (slots.empty()) return out;
         ScopeTimer tm(t_fwt);
         n_fwt++;
-        d_list.upload(slots, st);
         d_dbl.reserve(slots.size());
         const gq::LabArgs la = lab_args();
         const int order = samples[0].grid->order();
@@ -510,9 +541,10 @@ struct gq_ctx {
         const float *fl[1] = {phi[cur]};
         std::vector<int> items;
         gq::lab_items(slots, h_nbr26, order, items);
-        d_items.upload(items, st);
-        HIPCHK(gq::launch_lab_ghosts(la, fl, 1, d_list.p, d_items.p, (int)items.size(), d_ghost.p, 0, false, st));
-        HIPCHK(gq::launch_fwt_detail(order, la, d_ghost.p, phi[cur], d_list.p, (int)slots.size(), d_dbl.p, st));
+        lists.upload({&slots, &items}, st);
+        HIPCHK(gq::launch_lab_ghosts(la, fl, 1, lists.ptr(0), lists.ptr(1), (int)items.size(), d_ghost.p, 0, false,
+                                     st));
+        HIPCHK(gq::launch_fwt_detail(order, la, d_ghost.p, phi[cur], lists.ptr(0), (int)slots.size(), d_dbl.p, st));
         HIPCHK(hipMemcpyAsync(out.data(), d_dbl.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         return out;
@@ -558,8 +590,6 @@ struct gq_ctx {
                 }
             }
         }
-        d_list.upload(pslots, st);
-        d_list2.upload(cslots, st);
         const gq::LabArgs la = lab_args();
         const int order = samples[0].grid->order();
         const size_t per = pslots.size() * (size_t)gq::lab_points(order);
@@ -567,9 +597,10 @@ struct gq_ctx {
         const float *fl[3] = {phi[cur], pw, pg};
         std::vector<int> items;
         gq::lab_items(pslots, h_nbr26, order, items);
-        d_items.upload(items, st);
-        HIPCHK(gq::launch_lab_ghosts(la, fl, 3, d_list.p, d_items.p, (int)items.size(), d_ghost.p, per, true, st));
-        HIPCHK(gq::launch_split(order, la, d_ghost.p, phi[cur], pw, pg, d_list.p, d_list2.p, (int)pslots.size(),
+        lists.upload({&pslots, &cslots, &items}, st);
+        HIPCHK(gq::launch_lab_ghosts(la, fl, 3, lists.ptr(0), lists.ptr(2), (int)items.size(), d_ghost.p, per, true,
+                                     st));
+        HIPCHK(gq::launch_split(order, la, d_ghost.p, phi[cur], pw, pg, lists.ptr(0), lists.ptr(1), (int)pslots.size(),
                                 st));
         HIPCHK(hipStreamSynchronize(st));
         for (int s : pslots) free_slots.push_back(s);
@@ -625,9 +656,8 @@ struct gq_ctx {
                 }
             }
         }
-        d_list.upload(pslots, st);
-        d_list2.upload(cslots, st);
-        HIPCHK(gq::launch_collapse(phi[cur], pw, pg, d_list.p, d_list2.p, (int)pslots.size(), st));
+        lists.upload({&pslots, &cslots}, st);
+        HIPCHK(gq::launch_collapse(phi[cur], pw, pg, lists.ptr(0), lists.ptr(1), (int)pslots.size(), st));
         HIPCHK(hipStreamSynchronize(st));
         for (int s : cslots) free_slots.push_back(s);
         parallel_for(n, [&](int k) {
@@ -801,11 +831,11 @@ struct gq_ctx {
         gq::StepArgs a;
         a.pw = pw;
         a.pg = pg;
-        a.work = d_work.p;
+        a.work = p_work;
         a.nwork = (int)work.size();
         a.fnbr = d_fnbr.p;
         a.fbase = d_fbase.p;
-        a.sfbase = d_sfbase.p;
+        a.sfbase = p_sfbase;
         a.fik = d_fik.p;
         a.gphi = d_gphi.p;
         a.fptr = d_face_ptr.p;
@@ -931,7 +961,7 @@ struct gq_ctx {
         gq::UniformArgs u;
         u.vol = nullptr;
         u.phi = phi[cur];
-        u.lut_slot = d_lut.p + (size_t)sample * bpd * bpd * bpd;
+        u.lut_slot = p_lut + (size_t)sample * bpd * bpd * bpd;
         u.level = d_level.p;
         u.index = d_index.p;
         u.gpd = bpd * gq::BS;
@@ -1290,7 +1320,7 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
                 const int ms_s = std::min(rem[s], m);
                 if ((ms_s & 1) != (m & 1)) {
                     const int b0 = ctx->work_begin[pos], b1 = ctx->work_begin[pos + 1];
-                    HIPCHK(gq::launch_copy_blocks(ctx->phi[ctx->cur], ctx->phi[c0 ^ (ms_s & 1)], ctx->d_work.p + b0,
+                    HIPCHK(gq::launch_copy_blocks(ctx->phi[ctx->cur], ctx->phi[c0 ^ (ms_s & 1)], ctx->p_work + b0,
                                                   b1 - b0, ctx->st));
                 }
             }
@@ -1359,7 +1389,7 @@ gq_status gq_ensemble_ic_gaussian(gq_ctx *ctx, const double *ic)
         const std::vector<double> icv(ic, ic + 5 * S);  // kept until the synchronisation below
         ctx->d_icp.upload(icv, ctx->st);
         const int gpd = ctx->G(0).blocks_per_dim() * gq::BS;
-        HIPCHK(gq::launch_ic_gaussian(ctx->phi[ctx->cur], ctx->d_work.p, (int)ctx->work.size(), ctx->d_index.p,
+        HIPCHK(gq::launch_ic_gaussian(ctx->phi[ctx->cur], ctx->p_work, (int)ctx->work.size(), ctx->d_index.p,
                                       ctx->d_slot_sample.p, ctx->d_icp.p, gpd, ctx->st));
         HIPCHK(hipStreamSynchronize(ctx->st));
     });
@@ -1456,7 +1486,7 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
         }
         float *lam[2] = {ctx->d_lam.p, ctx->d_lam.p + pool};
         int lc = 0;
-        HIPCHK(gq::launch_lik_flip_grad(ctx->phi[ctx->cur], ctx->d_petf.p, ctx->d_work.p, nw, ctx->d_index.p, gpd,
+        HIPCHK(gq::launch_lik_flip_grad(ctx->phi[ctx->cur], ctx->d_petf.p, ctx->p_work, nw, ctx->d_index.p, gpd,
                                         ostride, 1.0 / (sigma * sigma), lam[lc], st));
         const int NS = gq::adj_slabs();
         ctx->d_gacc.reserve((size_t)ctx->cap * NS * 2);
@@ -1465,7 +1495,7 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
         gq::AdjArgs g;
         g.pw = ctx->pw;
         g.pg = ctx->pg;
-        g.work = ctx->d_work.p;
+        g.work = ctx->p_work;
         g.nwork = nw;
         g.fnbr = ctx->d_fnbr.p;
         g.level = ctx->d_level.p;
@@ -1503,7 +1533,7 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
         std::vector<double> icg;
         if (ic) {
             ctx->d_icg.reserve((size_t)nw * 4);
-            HIPCHK(gq::launch_ic_grad(lam[lc], ctx->d_work.p, nw, ctx->d_index.p, ctx->d_slot_sample.p, ctx->d_icp.p,
+            HIPCHK(gq::launch_ic_grad(lam[lc], ctx->p_work, nw, ctx->d_index.p, ctx->d_slot_sample.p, ctx->d_icp.p,
                                       gpd, ctx->d_icg.p, st));
             icg.resize((size_t)nw * 4);
             HIPCHK(hipMemcpyAsync(icg.data(), ctx->d_icg.p, icg.size() * sizeof(double), hipMemcpyDeviceToHost, st));

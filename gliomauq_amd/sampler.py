@@ -130,14 +130,17 @@ class SurrogatePosterior:
         return np.where(inside, ll[:, 0], -np.inf)
 
 
-def da_rwm(post, spost, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, logf=None, callback=None):
+def da_rwm(post, spost, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, logf=None, callback=None,
+           diag=None):
     """Delayed-acceptance random-walk Metropolis for a population of chains:
     stage 1 accepts a symmetric proposal with min(1, pi~(t')/pi~(t)) on the
     surrogate posterior `spost`; only the survivors are solved with the full
     model `post` (one batched call) and accepted with
     min(1, pi(t') pi~(t) / (pi(t) pi~(t'))) (Christen & Fox 2005), so the
     chains target pi exactly.  Returns (samples, accepted, final log f,
-    number of full solves)."""
+    number of full solves).  With a list `diag`, the (surrogate, full)
+    log-posterior differences of every proposal solved in full are appended
+    (arrays of the stage-2 proposals of each iteration)."""
     th = np.asarray(theta0, np.float64).copy()
     Cn = th.shape[0]
     lp = _batched(post.logpost, th, post.batch)
@@ -155,6 +158,8 @@ def da_rwm(post, spost, theta0, n_iter, scale, rng, adapt_until=0, target=0.234,
         if len(idx):
             lpp = _batched(post.logpost, prop[idx], post.batch)
             n_full += len(idx)
+            if diag is not None:
+                diag.append((lsp[idx] - ls[idx], lpp - lp[idx]))
             r2 = (lpp - lp[idx]) - (lsp[idx] - ls[idx])
             ok = np.log(rng.random(len(idx))) < np.where(np.isfinite(r2), r2, -np.inf)
             j = idx[ok]
