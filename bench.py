@@ -59,9 +59,16 @@ def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, n_steps=200):
     t0 = time.perf_counter()
     O.fk_uniform_steps(a[0], a[1], a[2], 16, 1.0 / n, Dw, Dg, rho, dt, n_steps, threads=threads)
     wall = time.perf_counter() - t0
+    # the same oracle on one core, a shorter sample
+    n1 = max(1, n_steps // 20)
+    t1 = time.perf_counter()
+    O.fk_uniform_steps(a[0], a[1], a[2], 16, 1.0 / n, Dw, Dg, rho, dt, n1, threads=1)
+    wall1 = time.perf_counter() - t1
     return {"value": n ** 3 * n_steps / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": "FP64 CPU oracle, %d uniform steps of the %d^3 initial grid of this solve (%.1f s)"
-                      % (n_steps, n, wall)}
+                      % (n_steps, n, wall),
+            "single_core": {"value": n ** 3 * n1 / wall1, "cores": 1,
+                            "sample": "%d steps of the same grid (%.1f s)" % (n1, wall1)}}
 
 
 def ensemble_workload(args):
@@ -490,6 +497,7 @@ def surrogate_workload(args):
                        "posterior_mean": x.reshape(-1, 6).mean(0).tolist()}
     rates["da_rwm_16"]["full_solves"] = nfull[0]
     rates["da_rwm_16"]["proposals"] = int(x_d.shape[0] * C)
+    rates["da_rwm_16"]["stage2_rejection_rate"] = 1.0 - float(a_d.sum()) / max(nfull[0], 1)
     if diag:
         ds = np.concatenate([d[0] for d in diag])
         df = np.concatenate([d[1] for d in diag])
