@@ -1817,12 +1817,12 @@ gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt)
 {
     return guard(ctx, [&] {
         if (mode < 0 || mode > 4) throw gq_error(GQ_ERR_ARG, "autotune mode must be 0, 1, 2, 3 or 4");
-        if (mode >= 2 && fixed_zt != 16 && fixed_zt != 8 && fixed_zt != 4 && fixed_zt != 2 && fixed_zt != 1)
+        if ((mode == 2 || mode == 3) && fixed_zt != 16 && fixed_zt != 8 && fixed_zt != 4 && fixed_zt != 2 && fixed_zt != 1)
             throw gq_error(GQ_ERR_ARG, "slab depth must be 16, 8, 4, 2 or 1 (column streaming)");
         ctx->zt_cache.clear();
         ctx->autotune = mode == 1 || mode == 4;
         ctx->autotune_all = mode == 4;
-        if (mode >= 2)
+        if (mode == 2 || mode == 3)
             for (int b = 0; b < 32; b++) ctx->zt_cache[b] = gq_ctx::make_cfg(fixed_zt, mode == 3);
     });
 }

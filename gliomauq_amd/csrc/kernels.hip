@@ -613,7 +613,7 @@ __device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, co
         }
     }
     const int nc = n[0] * n[1] * n[2];
-    constexpr int NB = MODE == 0 ? 4 : 8;  // children per batch (register pressure of the value loads)
+    constexpr int NB = MODE == 1 ? 8 : 4;  // children per batch (register pressure of the loads and stores)
     for (int c0 = 0; c0 < nc; c0 += NB) {
         // a child i at level cl lies in a leaf of level cl (value at i) or of a
         // finer level (the walk descends once: the point 2i at level cl+1,

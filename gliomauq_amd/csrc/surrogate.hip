@@ -101,119 +101,122 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a)
     }
 }
 
-// Weight gradient on the MFMA: dW[co][col] = sum_v dZ[v][co] X[v + tap(col)][ci(col)]
-// with col = tap*Cin + ci, as a GEMM with M = Cout, N = columns, K = voxels.
-// Workgroup (column block of 8192/Cout columns, 4096-voxel chunk): per 32
-// voxels, dZ^T [Cout][32] and X^T [columns][32] are staged in LDS as bf16
-// (the gather applies the tap offset and the zero padding); each wave owns
-// 8 of the 32 16x16 output tiles and issues one v_mfma_f32_16x16x32_bf16 per
-// tile.  Partial sums per chunk (FP32), reduced in chunk order.  The column
-// block 0 also sums dZ per channel (bias gradient partials).
-__global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(WgradArgs a, float *bias_partial)
+// Weight gradient by voxel bricks: dW[co][tap*Cin + ci] = sum_v dZ[v][co] X[v + tap][ci].
+// Workgroup (16-channel input tile, output group of <= 32 channels, brick
+// group): for each 8x8x4-voxel brick of its group it stages dZ^T [co][voxel]
+// and the 10x10x6 halo of X^T for its 16 input channels in LDS -- three
+// copies shifted by 0/1/2 in x so that every tap's 8-voxel row is one aligned
+// 16-byte read -- and runs all 27 taps on the MFMA (A = dZ^T rows, B = the
+// tap-shifted X^T, K = voxels in 32-voxel steps of 4 brick rows).  Partials
+// per brick group, reduced in group order afterwards (deterministic).
+namespace {
+constexpr int WB_ROWS = 61;   // 10 x 6 halo rows + 1 pad row per input channel (LDS banks)
+constexpr int WB_DZLD = 264;  // 256 voxels + 8 pad per output channel
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_conv3_wgrad_brick(WgradArgs a, float *bias_partial, int bricks_per_group)
 {
-    constexpr int KS = 32;
-    __shared__ __attribute__((aligned(16))) __bf16 s_dz[128 * KS];
-    __shared__ __attribute__((aligned(16))) __bf16 s_x[512 * KS];
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[3 * 16 * WB_ROWS * 8];
+    __shared__ __attribute__((aligned(16))) __bf16 s_dz[32 * WB_DZLD];
     __shared__ float s_b[256];
-    __shared__ int s_ci[512];       // input channel of each column of the block
-    __shared__ int s_off[512];      // packed tap offset of each column: (dx+1) | (dy+1) << 2 | (dz+1) << 4, -1 past the end
-    __shared__ int s_vx[KS], s_vy[KS], s_vz[KS], s_vb[KS];  // coordinates of the K-step's voxels (-1: past the end)
+    const int cit = blockIdx.x, cog = blockIdx.y, bg = blockIdx.z;
     const int Cin = a.cin, Cout = a.cout;
-    const int NCB = 8192 / Cout;       // columns per workgroup (32 tiles of 16x16)
-    const int NT = NCB / 16;
-    const int lgN = 31 - __builtin_clz(NCB), lgC = 31 - __builtin_clz(Cout);
-    const int Kw = 27 * Cin;
-    const int c0 = blockIdx.x * NCB;
-    const int chunk = blockIdx.y;
-    const int D = a.x.D, lgD = 31 - __builtin_clz(D);
-    const int64_t nvox = ((int64_t)D * D * D) * a.batch;
-    const int64_t v0 = (int64_t)chunk * a.chunk_vox, v1 = v0 + a.chunk_vox < nvox ? v0 + a.chunk_vox : nvox;
+    const int nco = Cout - 32 * cog < 32 ? Cout - 32 * cog : 32;  // 16 or 32
+    const int ncot = nco >> 4;
+    const int ntiles = 27 * ncot;
+    const int D = a.x.D;
+    const int nbx = D >> 3, nbz = D >> 2;
+    const int64_t D3 = (int64_t)D * D * D;
+    const int nbricks = a.batch * nbx * nbx * nbz;
+    const int b0 = bg * bricks_per_group;
+    const int b1 = b0 + bricks_per_group < nbricks ? b0 + bricks_per_group : nbricks;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 15, kq = lane >> 4;
-    for (int cl = t; cl < NCB; cl += 256) {
-        const int col = c0 + cl;
-        if (col < Kw) {
-            const int tap = col / Cin;
-            s_ci[cl] = col - tap * Cin;
-            s_off[cl] = (tap % 3) | (((tap / 3) % 3) << 2) | ((tap / 9) << 4);
-        } else {
-            s_ci[cl] = 0;
-            s_off[cl] = -1;
-        }
-    }
-    f32x4 acc[8];
+    const int ci0 = 16 * cit;
+    constexpr int MT = 14;  // tiles per wave: ceil(54 / 4)
+    f32x4 acc[MT];
 #pragma unroll
-    for (int j = 0; j < 8; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < MT; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
     float bsum = 0.f;
-    for (int64_t vs = v0; vs < v1; vs += KS) {
-        if (t < KS) {
-            const int64_t v = vs + t;
-            if (v < v1) {
-                const int64_t r = v & (((int64_t)1 << (3 * lgD)) - 1);
-                s_vb[t] = (int)(v >> (3 * lgD));
-                s_vx[t] = (int)(r & (D - 1));
-                s_vy[t] = (int)((r >> lgD) & (D - 1));
-                s_vz[t] = (int)(r >> (2 * lgD));
-            } else {
-                s_vx[t] = -1;
+    for (int b = b0; b < b1; b++) {
+        const int bx = b % nbx, by = (b / nbx) % nbx, bz = (b / (nbx * nbx)) % nbz, bb = b / (nbx * nbx * nbz);
+        const int x0 = 8 * bx, y0 = 8 * by, z0 = 4 * bz;
+        __syncthreads();  // the previous brick's LDS reads are done
+        // X^T halo, three x-shifted copies: copy s, channel c, row (hz*10 + hy), xp <- halo x = xp + s
+        for (int e = t; e < 600 * 2; e += 256) {
+            const int h = e >> 1, half = e & 1;
+            const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            const int c0 = ci0 + 8 * half;
+            bf16x8 v = zero8();
+            if (c0 < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
+                const int64_t vn = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
+                v = load8(a.x.p + vn * a.x.ld + a.x.off + c0);
+            }
+            const int row = hz * 10 + hy;
+#pragma unroll
+            for (int sh = 0; sh < 3; sh++) {
+                const int xp = hx - sh;
+                if (xp < 0 || xp >= 8) continue;
+#pragma unroll
+                for (int j = 0; j < 8; j++) s_x[((sh * 16 + 8 * half + j) * WB_ROWS + row) * 8 + xp] = v[j];
             }
         }
-        __syncthreads();
-        // dZ^T: element e -> voxel e / Cout, channel e % Cout (coalesced rows)
-        for (int e = t; e < Cout * KS; e += 256) {
-            const int k = e >> lgC, co = e & (Cout - 1);
-            const int64_t v = vs + k;
-            __bf16 val = (__bf16)0.0f;
-            if (v < v1) val = a.dz.p[v * a.dz.ld + a.dz.off + co];
-            s_dz[co * KS + k] = val;
-            if (blockIdx.x == 0) bsum += bf2f(val);
+        // dZ^T [co][voxel], voxel = x + 8 (y + 8 z) inside the brick
+        const int ng = nco >> 3;
+        for (int e = t; e < 256 * ng; e += 256) {
+            const int vox = e / ng, g = e - vox * ng;
+            const int lx = vox & 7, ly = (vox >> 3) & 7, lz = vox >> 6;
+            const int64_t vn = bb * D3 + (x0 + lx) + (int64_t)D * ((y0 + ly) + (int64_t)D * (z0 + lz));
+            const bf16x8 v = load8(a.dz.p + vn * a.dz.ld + a.dz.off + 32 * cog + 8 * g);
+#pragma unroll
+            for (int j = 0; j < 8; j++) s_dz[(8 * g + j) * WB_DZLD + vox] = v[j];
         }
-        // X^T: element e -> voxel e / NCB, column e % NCB
-        for (int e = t; e < NCB * KS; e += 256) {
-            const int k = e >> lgN, cl = e & (NCB - 1);
-            const int off = s_off[cl];
-            __bf16 val = (__bf16)0.0f;
-            const int vx = s_vx[k];
-            if (off >= 0 && vx >= 0) {
-                const int x = vx + (off & 3) - 1, y = s_vy[k] + ((off >> 2) & 3) - 1, z = s_vz[k] + (off >> 4) - 1;
-                if ((unsigned)x < (unsigned)D && (unsigned)y < (unsigned)D && (unsigned)z < (unsigned)D) {
-                    const int64_t vn = ((int64_t)s_vb[k] << (3 * lgD)) + x + ((int64_t)y << lgD) + ((int64_t)z << (2 * lgD));
-                    val = a.x.p[vn * a.x.ld + a.x.off + s_ci[cl]];
+        __syncthreads();
+        if (cit == 0 && (t & 31) < nco) {
+            const int co = t & 31, q = t >> 5;  // voxels 32q .. 32q+31
+            for (int v = 0; v < 32; v++) bsum += bf2f(s_dz[co * WB_DZLD + 32 * q + v]);
+        }
+#pragma unroll 1
+        for (int ks = 0; ks < 8; ks++) {
+            const int r = 4 * ks + kq;  // brick row: y = r % 8, z = r / 8
+            const int ry = r & 7, rz = r >> 3;
+#pragma unroll
+            for (int j = 0; j < MT; j++) {
+                const int tile = wave + 4 * j;
+                if (tile < ntiles) {
+                    const int tap = tile / ncot, cot = tile - tap * ncot;
+                    const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
+                    const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_dz[(16 * cot + li) * WB_DZLD + 8 * r]);
+                    const bf16x8 vb = *reinterpret_cast<const bf16x8 *>(
+                        &s_x[((dx * 16 + li) * WB_ROWS + (rz + dz) * 10 + (ry + dy)) * 8]);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc[j], 0, 0, 0);
                 }
             }
-            s_x[cl * KS + k] = val;
         }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int tile = wave + 4 * j;  // 32 tiles: tile = mt * NT + nt
-            const int mt = tile / NT, nt = tile - mt * NT;
-            const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_dz[(16 * mt + li) * KS + 8 * kq]);
-            const bf16x8 vb = *reinterpret_cast<const bf16x8 *>(&s_x[(16 * nt + li) * KS + 8 * kq]);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc[j], 0, 0, 0);
-        }
-        __syncthreads();
     }
+    // D row 4*kq + rr = output channel within the tile, column li = input channel within the tile
+    const int Kw = 27 * Cin;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < MT; j++) {
         const int tile = wave + 4 * j;
-        const int mt = tile / NT, nt = tile - mt * NT;
-        const int col = c0 + 16 * nt + li;
-        if (col >= Kw) continue;
+        if (tile >= ntiles) continue;
+        const int tap = tile / ncot, cot = tile - tap * ncot;
+        const int ci = ci0 + li;
+        if (ci >= Cin) continue;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int co = 16 * mt + 4 * kq + r;
-            a.partial[(int64_t)chunk * Cout * Kw + (int64_t)co * Kw + col] = acc[j][r];
+        for (int rr = 0; rr < 4; rr++) {
+            const int co = 32 * cog + 16 * cot + 4 * kq + rr;
+            a.partial[(int64_t)bg * Cout * Kw + (int64_t)co * Kw + tap * Cin + ci] = acc[j][rr];
         }
     }
-    if (blockIdx.x == 0) {
-        // threads t, t + Cout, ... hold partial sums of channel t % Cout
+    if (cit == 0) {
         s_b[t] = bsum;
         __syncthreads();
-        if (t < Cout) {
+        if (t < nco) {
             float sb = 0.f;
-            for (int q = t; q < 256; q += Cout) sb += s_b[q];
-            bias_partial[(int64_t)chunk * Cout + t] = sb;
+            for (int q = 0; q < 8; q++) sb += s_b[32 * q + t];
+            bias_partial[(int64_t)bg * Cout + 32 * cog + t] = sb;
         }
     }
 }
@@ -223,8 +226,17 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const float *partial, int
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    // chunks added in order; eight loads in flight at a time
     float s = 0.f;
-    for (int c = 0; c < nchunks; c++) s += partial[(int64_t)c * n + i];
+    int c = 0;
+    for (; c + 8 <= nchunks; c += 8) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = partial[(int64_t)(c + j) * n + i];
+#pragma unroll
+        for (int j = 0; j < 8; j++) s += v[j];
+    }
+    for (; c < nchunks; c++) s += partial[(int64_t)c * n + i];
     out[i] = s;
 }
 
@@ -480,15 +492,23 @@ hipError_t launch_conv3(const ConvArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_conv3_wgrad_mfma(const WgradArgs &a, int nchunks, float *bias_partial, hipStream_t st)
+int wgrad_groups(int cin, int cout, int D, int batch)
+{
+    const int nbricks = batch * (D / 8) * (D / 8) * (D / 4);
+    const int per_group = ((cin + 15) / 16) * ((cout + 31) / 32);  // workgroups per brick group
+    int g = (1024 + per_group - 1) / per_group;                     // about 1024 workgroups in all
+    return g < nbricks ? (g > 0 ? g : 1) : nbricks;
+}
+
+hipError_t launch_conv3_wgrad_brick(const WgradArgs &a, int ngroups, float *bias_partial, hipStream_t st)
 {
     const int D = a.x.D;
-    if (a.cout % 16 != 0 || a.cout > 128 || 256 % a.cout != 0 || a.chunk_vox % 32 != 0 || D < 2 || (D & (D - 1)))
+    if (a.cout % 16 != 0 || a.cin % 8 != 0 || D < 8 || D % 8 != 0 || a.dz.D != D || ngroups < 1)
         return hipErrorInvalidValue;
-    const int ncb = 8192 / a.cout;
-    const int64_t kw = 27 * (int64_t)a.cin;
-    const dim3 grid((unsigned)((kw + ncb - 1) / ncb), nchunks);
-    hipLaunchKernelGGL(k_conv3_wgrad_mfma, grid, dim3(256), 0, st, a, bias_partial);
+    const int nbricks = a.batch * (D / 8) * (D / 8) * (D / 4);
+    const int per = (nbricks + ngroups - 1) / ngroups;
+    const dim3 grid((unsigned)((a.cin + 15) / 16), (unsigned)((a.cout + 31) / 32), (unsigned)ngroups);
+    hipLaunchKernelGGL(k_conv3_wgrad_brick, grid, dim3(256), 0, st, a, bias_partial, per);
     return hipGetLastError();
 }
 

@@ -38,8 +38,7 @@ struct WgradArgs {
     View x;   // layer input, cin channels
     View dz;  // gradient at the pre-activation, cout channels
     int cin, cout, batch;
-    int64_t chunk_vox;
-    float *partial;  // [nchunks][cout][27*cin]
+    float *partial;  // [groups][cout][27*cin]
 };
 
 struct HeadArgs {
@@ -55,8 +54,10 @@ struct HeadArgs {
 };
 
 hipError_t launch_conv3(const ConvArgs &a, hipStream_t st);
-// MFMA version (Cout in {16, 32, 64, 128}); also writes bias partials [nchunks][cout]
-hipError_t launch_conv3_wgrad_mfma(const WgradArgs &a, int nchunks, float *bias_partial, hipStream_t st);
+// Brick version (Cout multiple of 16, Cin multiple of 8, D multiple of 8): partials
+// [ngroups][cout][27*cin] and bias partials [ngroups][cout]; ngroups from wgrad_groups.
+hipError_t launch_conv3_wgrad_brick(const WgradArgs &a, int ngroups, float *bias_partial, hipStream_t st);
+int wgrad_groups(int cin, int cout, int D, int batch);
 hipError_t launch_reduce_chunks(const float *partial, int nchunks, int64_t n, float *out, hipStream_t st);
 hipError_t launch_avgpool(const View &in, const View &out, int C, int batch, hipStream_t st);
 hipError_t launch_avgpool_bwd(const float *dout, int64_t dout_ld, const FView &din, int C, int batch, hipStream_t st);

@@ -53,7 +53,6 @@ struct Conv {
 };
 
 constexpr int NCONV = 7;
-constexpr int64_t CHUNK = 4096;
 
 }  // namespace
 
@@ -138,10 +137,9 @@ struct gq_sur {
             SCHK(hipMemcpy(c.w, w.data(), n * 4, hipMemcpyHostToDevice));
             for (float *p : {c.mw, c.vw}) SCHK(hipMemset(p, 0, n * 4));
             for (float *p : {c.b, c.mb, c.vb}) SCHK(hipMemset(p, 0, c.cout * 4));
-            const int64_t nvox = (int64_t)c.D * c.D * c.D * batch;
-            const int64_t nch = (nvox + CHUNK - 1) / CHUNK;
-            maxpart = std::max(maxpart, nch * n);
-            maxbpart = std::max(maxbpart, nch * c.cout);
+            const int64_t ng = gq::sur::wgrad_groups(c.cin, c.cout, c.D, batch);
+            maxpart = std::max(maxpart, ng * n);
+            maxbpart = std::max(maxbpart, ng * c.cout);
         }
         for (int c = 0; c < 16; c++) hw[c] = 0.25f * nd(rng);
         hb = -3.f;  // start near the background value (most voxels hold no tumour)
@@ -244,19 +242,17 @@ struct gq_sur {
     void conv_wgrad(int l, gq::sur::View x, int B)
     {
         const Conv &c = cv[l];
-        const int64_t nvox = (int64_t)c.D * c.D * c.D * B;
-        const int nch = (int)((nvox + CHUNK - 1) / CHUNK);
+        const int ng = gq::sur::wgrad_groups(c.cin, c.cout, c.D, B);
         gq::sur::WgradArgs a{};
         a.x = x;
         a.dz = V(dz, c.D, c.cout, 0);
         a.cin = c.cin;
         a.cout = c.cout;
         a.batch = B;
-        a.chunk_vox = CHUNK;
         a.partial = partial;
-        SCHK(gq::sur::launch_conv3_wgrad_mfma(a, nch, bpart, st));
-        SCHK(gq::sur::launch_reduce_chunks(partial, nch, c.nw(), c.gw, st));
-        SCHK(gq::sur::launch_reduce_chunks(bpart, nch, c.cout, c.gb, st));
+        SCHK(gq::sur::launch_conv3_wgrad_brick(a, ng, bpart, st));
+        SCHK(gq::sur::launch_reduce_chunks(partial, ng, c.nw(), c.gw, st));
+        SCHK(gq::sur::launch_reduce_chunks(bpart, ng, c.cout, c.gb, st));
     }
 
     void upload_batch(const float *params, const float *u0h, const float *th, int B)
