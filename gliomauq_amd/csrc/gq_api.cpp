@@ -141,7 +141,7 @@ struct gq_ctx {
     std::vector<int> work;             // live slots: samples in sample_order, leaves in Morton order
     std::vector<int> work_begin;       // [S+1] per position in sample_order
     std::vector<int> face_begin;       // [S+1] cross-level faces per position
-    DevBuf<int> d_level, d_index, d_fnbr, d_fbase, d_face_src, d_nbr26, d_lutoff, d_slot_sample, d_counts;
+    DevBuf<int> d_level, d_index, d_fnbr, d_fbase, d_frow, d_face_src, d_nbr26, d_lutoff, d_slot_sample, d_counts;
     // per-topology lists in one transfer: work list, sample face-row bases, base-level and finest look-ups,
     // cross-level faces, changed slot rows
     PackedInts tabs;
@@ -429,6 +429,9 @@ struct gq_ctx {
             gq::SlotTables t{d_level.p, d_index.p, d_slot_sample.p, d_lutoff.p, d_fnbr.p, d_fbase.p, d_nbr26.p};
             HIPCHK(gq::launch_scatter_slot_rows(t, tabs.ptr(5), (int)(packed.size() / ROW), st));
         }
+        // absolute face-row bases for the step (after the slot rows and the sample bases are in place)
+        d_frow.reserve((size_t)cap * 6);
+        HIPCHK(gq::launch_face_rows_abs(p_work, (int)work.size(), d_fbase.p, p_sfbase, d_slot_sample.p, d_frow.p, st));
         face_nrows = (int)faces.size() * 256;
         face_rows_valid = false;
         if (d_fik.n < (size_t)cap * 6 * 256) {
@@ -834,8 +837,7 @@ struct gq_ctx {
         a.work = p_work;
         a.nwork = (int)work.size();
         a.fnbr = d_fnbr.p;
-        a.fbase = d_fbase.p;
-        a.sfbase = p_sfbase;
+        a.frow = d_frow.p;
         a.fik = d_fik.p;
         a.gphi = d_gphi.p;
         a.fptr = d_face_ptr.p;

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
             const int f = side == 0 ? 4 : 5;
             const int nb = a.fnbr[slot * 6 + f];
             if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, tx, ty)];
-            else pv = step_face_ghost(a, a.sfbase[a.slot_sample[slot]] + a.fbase[slot * 6 + f] + t);
+            else pv = step_face_ghost(a, a.frow[slot * 6 + f] + t);
             kv = a.fik[((size_t)slot * 6 + f) * 256 + t];
         }
         s_phi[l] = pv;
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
         const int nb = a.fnbr[slot * 6 + f];
         float pv;
         if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, u, v)];
-        else pv = step_face_ghost(a, a.sfbase[a.slot_sample[slot]] + a.fbase[slot * 6 + f] + u + 16 * v);
+        else pv = step_face_ghost(a, a.frow[slot * 6 + f] + u + 16 * v);
         const float kv = a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
         int l;
         switch (f) {
@@ -282,6 +282,19 @@ __global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ p
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     if (r < nrows && lane == 0) out[r] = acc;
+}
+
+// Absolute first face-ghost row of every face of the listed slots (one table
+// read in the step instead of the sample's base through the slot's sample).
+__global__ __launch_bounds__(256) void k_face_rows_abs(const int *__restrict__ work, int nwork,
+                                                       const int *__restrict__ fbase, const int *__restrict__ sfbase,
+                                                       const int *__restrict__ slot_sample, int *__restrict__ frow)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nwork * 6) return;
+    const int slot = work[i / 6], f = i % 6;
+    const int fb = fbase[slot * 6 + f];
+    frow[slot * 6 + f] = fb >= 0 ? sfbase[slot_sample[slot]] + fb : -1;
 }
 
 // 1/K of every face-halo cell (static between topology / parameter changes).
@@ -368,20 +381,19 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         rho = q[2];
         dt = q[3];
     }
-    const int rbase = a.sfbase[a.slot_sample[slot]];
     // halo role of this thread for the x/y faces: t < 64 -> face t/16, coordinate t%16
     const int hf = t >> 4, hu = t & 15;
     int hnb = -1, hrow = 0;
     if (t < 64) {
         hnb = a.fnbr[slot * 6 + hf];
-        if (hnb < 0) hrow = rbase + a.fbase[slot * 6 + hf];
+        if (hnb < 0) hrow = a.frow[slot * 6 + hf];
     }
     // z-1 plane of this column
     float pB, kB;
     {
         const int nb = a.fnbr[slot * 6 + 4];
         if (nb >= 0) pB = a.phi_in[(size_t)nb * B3 + nbr_face_cell(4, tx, ty)];
-        else pB = step_face_ghost(a, rbase + a.fbase[slot * 6 + 4] + t);
+        else pB = step_face_ghost(a, a.frow[slot * 6 + 4] + t);
         kB = a.fik[((size_t)slot * 6 + 4) * 256 + t];
     }
     float pC = a.phi_in[base + t];
@@ -424,7 +436,7 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         else {
             const int nb = a.fnbr[slot * 6 + 5];
             if (nb >= 0) pF = a.phi_in[(size_t)nb * B3 + nbr_face_cell(5, tx, ty)];
-            else pF = step_face_ghost(a, rbase + a.fbase[slot * 6 + 5] + t);
+            else pF = step_face_ghost(a, a.frow[slot * 6 + 5] + t);
             kF = a.fik[((size_t)slot * 6 + 5) * 256 + t];
         }
         // prefetch plane k+2
@@ -1355,6 +1367,15 @@ hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *s
 {
     if (nrows <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_face_ghosts, dim3((4 * nrows + 255) / 256), dim3(256), 0, st, phi, ptr, src, w, out, nrows);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_rows_abs(const int *work, int nwork, const int *fbase, const int *sfbase,
+                                const int *slot_sample, int *frow, hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_face_rows_abs, dim3((unsigned)((nwork * 6 + 255) / 256)), dim3(256), 0, st, work, nwork,
+                       fbase, sfbase, slot_sample, frow);
     return hipGetLastError();
 }
 

@@ -16,8 +16,7 @@ struct StepArgs {
     const int *work;   // Morton-ordered slots
     int nwork;
     const int *fnbr;   // [slot*6+f]
-    const int *fbase;  // [slot*6+f] first face-ghost row, relative to the sample's (-1: same level)
-    const int *sfbase; // [sample] first face-ghost row of the sample
+    const int *frow;   // [slot*6+f] first face-ghost row of the face (absolute; unused for same-level faces)
     const float *fik;  // [slot*6+f][256]
     const float *gphi; // cross-level face ghosts of phi_in, or null: the step evaluates the rows itself
     const int64_t *fptr;  // face-ghost rows (ptr/src/w) for the in-kernel evaluation
@@ -89,6 +88,9 @@ hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st);
 hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *src, const float *w, float *out,
                               int nrows, hipStream_t st);
 // sprm: per-sample Dw, Dg (ensembles) or null for (Dw, Dg)
+// frow[slot*6+f] = sfbase[sample(slot)] + fbase[slot*6+f] for the listed slots
+hipError_t launch_face_rows_abs(const int *work, int nwork, const int *fbase, const int *sfbase,
+                                const int *slot_sample, int *frow, hipStream_t st);
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
                             const int *fbase, const int *sfbase, const int *slot_sample, const int64_t *ptr,
                             const int *src, const float *w, float *fik, float Dw, float Dg, const float *sprm,

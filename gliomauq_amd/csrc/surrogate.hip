@@ -40,63 +40,105 @@ __device__ __forceinline__ bf16x8 zero8()
 
 }  // namespace
 
-// One wave: 16 consecutive voxels x NT tiles of 16 output channels.  Lane l
-// supplies A rows (voxel l%16, k = k0 + 8*(l/16) .. +7) and B columns
-// (channel l%16 of tile t, same k); it receives D rows 4*(l/16)+r, column l%16.
-template <int NT, bool OUT_BF16>
-__global__ __launch_bounds__(256) void k_conv3(ConvArgs a)
+// 3^3 convolution by voxel bricks: one workgroup per (8x8x4-voxel output
+// brick, group of NTG 16-channel output tiles).  The brick's 10x10x6 input halo
+// is staged in LDS channels-last, 32 input channels at a time (one 16-byte
+// read per lane and tap: no transposition, the GEMM's k runs along the
+// channels of a tap), and each wave runs 4 voxel tiles x NTG channel tiles
+// over the 27 taps, the weights read once per K-step and reused by the 4
+// voxel tiles.
+namespace {
+constexpr int CB_LD = 40;  // LDS channel stride per halo voxel (32 + 8 pad: banks)
+}  // namespace
+
+template <int NTG, bool OUT_BF16>
+__global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
 {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int64_t D = a.in.D, D3 = D * D * D;
-    const int64_t nvox = D3 * a.batch;
-    const int64_t vbase = ((int64_t)blockIdx.x * 4 + wave) * 16;
-    if (vbase >= nvox) return;
-    const int i = lane & 15, kq = lane >> 4;
-    // voxel of this lane's A row
-    const int64_t v = vbase + i;
-    const bool vok = v < nvox;
-    const int64_t vb = vok ? v / D3 : 0, vr = vok ? v - vb * D3 : 0;
-    const int x = (int)(vr % D), y = (int)((vr / D) % D), z = (int)(vr / (D * D));
+    __shared__ __attribute__((aligned(16))) __bf16 s_in[600 * CB_LD];
+    const int D = a.in.D;
+    const int nbx = D >> 3, nbz = D >> 2;
+    const int b = blockIdx.x;
+    const int bx = b % nbx, by = (b / nbx) % nbx, bz = (b / (nbx * nbx)) % nbz, bb = b / (nbx * nbx * nbz);
+    const int x0 = 8 * bx, y0 = 8 * by, z0 = 4 * bz;
+    const int nt0 = blockIdx.y * NTG;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, kq = lane >> 4;
     const int Cin = a.cin, K = 27 * Cin;
-    f32x4 acc[NT];
+    const int64_t D3 = (int64_t)D * D * D;
+    f32x4 acc[4][NTG];
 #pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        const int k = k0 + 8 * kq;
-        bf16x8 va = zero8();
-        if (k < K && vok) {
-            const int tap = k / Cin, ci = k - tap * Cin;
-            const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-            const int xx = x + dx, yy = y + dy, zz = z + dz;
-            if (xx >= 0 && xx < D && yy >= 0 && yy < D && zz >= 0 && zz < D) {
-                const int64_t vn = vb * D3 + xx + D * (yy + D * (int64_t)zz);
-                va = load8(a.in.p + vn * a.in.ld + a.in.off + ci);
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int n = 0; n < NTG; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // this lane's A rows: voxel tile mt = 4 wave + m covers brick rows 2 mt, 2 mt + 1 (row = y + 8 z)
+    int hbase[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int r = 2 * (4 * wave + m) + (li >> 3);
+        hbase[m] = (li & 7) + 10 * ((r & 7) + 10 * (r >> 3));
+    }
+    for (int c0 = 0; c0 < Cin; c0 += 32) {
+        __syncthreads();
+        for (int e = t; e < 600 * 4; e += 256) {
+            const int h = e >> 2, g = e & 3;
+            const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            const int ci = c0 + 8 * g;
+            bf16x8 v = zero8();
+            if (ci < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
+                const int64_t vn = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
+                v = load8(a.in.p + vn * a.in.ld + a.in.off + ci);
+            }
+            *reinterpret_cast<bf16x8 *>(&s_in[h * CB_LD + 8 * g]) = v;
+        }
+        __syncthreads();
+        const bool kok = c0 + 8 * kq < Cin;
+        // weights of tap + PF are loaded while tap computes (register ring of PF + 1 taps)
+        constexpr int PF = 3;
+        bf16x8 vb[PF + 1][NTG];
+        auto load_w = [&](int tap, bf16x8 *dst) {
+#pragma unroll
+            for (int n = 0; n < NTG; n++) {
+                const int co = 16 * (nt0 + n) + li;
+                dst[n] = (kok && co < a.cout) ? load8(a.w + (int64_t)co * K + tap * Cin + c0 + 8 * kq) : zero8();
+            }
+        };
+#pragma unroll
+        for (int p = 0; p < PF; p++) load_w(p, vb[p]);
+#pragma unroll
+        for (int tap = 0; tap < 27; tap++) {
+            if (tap + PF < 27) load_w(tap + PF, vb[(tap + PF) % (PF + 1)]);
+            const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
+            const int hoff = dx + 10 * (dy + 10 * dz);
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_in[(hbase[m] + hoff) * CB_LD + 8 * kq]);
+#pragma unroll
+                for (int n = 0; n < NTG; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb[tap % (PF + 1)][n], acc[m][n], 0, 0, 0);
             }
         }
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-            const int co = 16 * t + i;
-            bf16x8 vw = zero8();
-            if (k < K && co < a.cout) vw = load8(a.w + (int64_t)co * K + k);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vw, acc[t], 0, 0, 0);
-        }
     }
-    // epilogue: D row 4*kq + r = voxel, column i = channel within the tile
+    // D row 4 kq + rr = voxel of the tile, column li = output channel of the tile
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int co = 16 * t + i;
+    for (int n = 0; n < NTG; n++) {
+        const int co = 16 * (nt0 + n) + li;
         if (co >= a.cout) continue;
-        const float b = a.bias ? a.bias[co] : 0.f;
+        const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int64_t vo = vbase + 4 * kq + r;
-            if (vo >= nvox) continue;
-            float val = acc[t][r] + b;
-            if (a.relu) val = val > 0.f ? val : kLeak * val;
-            const int64_t o = vo * a.out_ld + a.out_off + co;
-            if (OUT_BF16) a.out_bf[o] = (__bf16)val;
-            else a.out_f[o] = a.accumulate ? a.out_f[o] + val : val;
+        for (int m = 0; m < 4; m++) {
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int i = 4 * kq + rr;
+                const int r = 2 * (4 * wave + m) + (i >> 3);
+                const int gx = x0 + (i & 7), gy = y0 + (r & 7), gz = z0 + (r >> 3);
+                const int64_t vo = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
+                float val = acc[m][n][rr] + bias;
+                if (a.relu) val = val > 0.f ? val : kLeak * val;
+                const int64_t o = vo * a.out_ld + a.out_off + co;
+                if (OUT_BF16) a.out_bf[o] = (__bf16)val;
+                else a.out_f[o] = a.accumulate ? a.out_f[o] + val : val;
+            }
         }
     }
 }
@@ -470,22 +512,23 @@ hipError_t launch_sur_lik_flip(const float *pred, int64_t n, int batch, const ui
 // ---------------------------------------------------------------------------
 hipError_t launch_conv3(const ConvArgs &a, hipStream_t st)
 {
-    const int64_t nvox = (int64_t)a.in.D * a.in.D * a.in.D * a.batch;
-    const dim3 grid((unsigned)((nvox + 63) / 64));
+    const int D = a.in.D;
     const int nt = (a.cout + 15) / 16;
-    if (a.cin % 8 != 0 || a.in.ld % 8 != 0 || a.in.off % 8 != 0 || nt > 12) return hipErrorInvalidValue;
+    if (a.cin % 8 != 0 || a.in.ld % 8 != 0 || a.in.off % 8 != 0 || a.cout % 16 != 0 || nt > 12 || D < 8 ||
+        D % 8 != 0)
+        return hipErrorInvalidValue;
+    const int ntg = nt <= 4 ? nt : (nt % 4 == 0 ? 4 : 3);  // output tiles per workgroup
+    if (nt % ntg != 0) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(a.batch * (D / 8) * (D / 8) * (D / 4)), (unsigned)(nt / ntg));
     const bool bf = a.out_bf != nullptr;
-#define GQ_CONV(N)                                                                  \
-    if (bf) hipLaunchKernelGGL((k_conv3<N, true>), grid, dim3(256), 0, st, a);      \
-    else hipLaunchKernelGGL((k_conv3<N, false>), grid, dim3(256), 0, st, a);
-    switch (nt) {
+#define GQ_CONV(N)                                                                        \
+    if (bf) hipLaunchKernelGGL((k_conv3_brick<N, true>), grid, dim3(256), 0, st, a);      \
+    else hipLaunchKernelGGL((k_conv3_brick<N, false>), grid, dim3(256), 0, st, a);
+    switch (ntg) {
     case 1: GQ_CONV(1) break;
     case 2: GQ_CONV(2) break;
     case 3: GQ_CONV(3) break;
     case 4: GQ_CONV(4) break;
-    case 6: GQ_CONV(6) break;
-    case 8: GQ_CONV(8) break;
-    case 12: GQ_CONV(12) break;
     default: return hipErrorInvalidValue;
     }
 #undef GQ_CONV
