@@ -14,11 +14,14 @@
 // Terms whose adjoint factor is zero are skipped (their coefficients can be
 // NaN next to CSF, where the forward clamps NaN to 0).
 //
-// One backward step is one kernel: it stages u_n and 1/K with a one-cell halo
-// in LDS, builds mu from lambda_{n+1} and u_{n+1} (mask from the clamped
-// state, so the mask equals the forward's branch bit for bit), writes
-// lambda_n and adds the two parameter-gradient contributions of its slab to
-// per-slab FP64 accumulators (each owned by one workgroup: deterministic).
+// One backward step is one kernel: it stages u_n, 1/K (precomputed once per
+// solve) and mu_{n+1} with a one-cell halo in LDS, computes lambda_n and
+// writes mu_n = [0 < u_n < 1] lambda_n -- the mask of the next backward step,
+// taken from the clamped state u_n this step already reads, equal to the
+// forward's branch bit for bit -- or lambda_n itself at the last step; it adds
+// the two parameter-gradient contributions of its slab to per-slab FP64
+// accumulators (each owned by one workgroup: deterministic).  Per cell it
+// moves 16 bytes: u_n, 1/K, mu_{n+1} in, mu_n out.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -37,10 +40,12 @@ constexpr int PL = LW * LW;
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// the step kernels' 1/K, rounded the same way (contraction off, explicit FMA)
 __device__ __forceinline__ float inv_k(float w, float g, float Dw, float Dg)
 {
+#pragma clang fp contract(off)
     const float b = w + g;
-    const float D = w * Dw + g * Dg;
+    const float D = __builtin_fmaf(g, Dg, w * Dw);
     return b * frcp(D);
 }
 
@@ -56,7 +61,7 @@ __device__ __forceinline__ int nbr_face_cell(int f, int u, int v)
     }
 }
 
-__device__ __forceinline__ float mu_of(float lam, float unext) { return (unext > 0.f && unext < 1.f) ? lam : 0.f; }
+__device__ __forceinline__ float mu_of(float lam, float u) { return (u > 0.f && u < 1.f) ? lam : 0.f; }
 
 __device__ __forceinline__ double wave_sum(double x)
 {
@@ -86,18 +91,15 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
     const int tx = t & 15, ty = t >> 4;
     const size_t base = (size_t)slot * B3;
     const float *q = a.sprm + 4 * a.slot_sample[slot];
-    const float Dw = q[0], Dg = q[1], rho = q[2], dt = q[3];
+    const float rho = q[2], dt = q[3];
 
-    float breg[ZT];
 #pragma unroll
     for (int k = 0; k < ZT; k++) {
         const size_t i = base + (z0 + k) * 256 + t;
-        const float wm = a.pw[i], gm = a.pg[i];
-        breg[k] = wm + gm;
         const int l = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
         s_u[l] = a.u[i];
-        s_ik[l] = inv_k(wm, gm, Dw, Dg);
-        s_mu[l] = mu_of(a.lam_in[i], a.u_next[i]);
+        s_ik[l] = a.ik[i];
+        s_mu[l] = a.mu_in[i];
     }
 #pragma unroll
     for (int side = 0; side < 2; side++) {
@@ -110,8 +112,8 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
             i = (size_t)a.fnbr[slot * 6 + f] * B3 + nbr_face_cell(f, tx, ty);
         }
         s_u[l] = a.u[i];
-        s_ik[l] = inv_k(a.pw[i], a.pg[i], Dw, Dg);
-        s_mu[l] = mu_of(a.lam_in[i], a.u_next[i]);
+        s_ik[l] = a.ik[i];
+        s_mu[l] = a.mu_in[i];
     }
     for (int r4 = t; r4 < 4 * 16 * ZT; r4 += 256) {
         const int f = r4 / (16 * ZT);
@@ -126,8 +128,8 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
         default: l = (u + 1) + LW * (B + 1) + PL * (v - z0 + 1); break;
         }
         s_u[l] = a.u[i];
-        s_ik[l] = inv_k(a.pw[i], a.pg[i], Dw, Dg);
-        s_mu[l] = mu_of(a.lam_in[i], a.u_next[i]);
+        s_ik[l] = a.ik[i];
+        s_mu[l] = a.mu_in[i];
     }
     __syncthreads();
 
@@ -142,7 +144,9 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
         float lam = 0.f;
         if (muc != 0.f) {  // the cell's own update was inside (0,1): its coefficients are finite
             float d[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            if (breg[k] + uc > 0.f) {
+            // tissue_c = pw+pg+u_c > 0; with pw, pg >= 0 and Dw, Dg > 0, 1/K is finite and
+            // positive exactly where pw+pg > 0 (else 0/0 = NaN), as for the neighbours below
+            if ((kc > 0.f && kc < INFINITY) || uc > 0.f) {
 #pragma unroll
                 for (int f = 0; f < 6; f++) d[f] = 2.f * frcp(kc + s_ik[c + offs[f]]);
             }
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
             }
         }
         lam += dt * ih2 * tr;
-        a.lam_out[base + (z0 + k) * 256 + t] = lam;
+        a.out[base + (z0 + k) * 256 + t] = a.out_lambda ? lam : mu_of(lam, uc);
     }
     // block sums of the two parameter gradients -> this slab's accumulators
     gD = wave_sum(gD);
@@ -186,12 +190,26 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
     }
 }
 
-// dL/du of the flip-rate model at the final state (uniform grids): only the
-// Gaussian PET term depends smoothly on u; the Bernoulli terms are piecewise
-// constant in u (derivative zero almost everywhere).
+// 1/K of every cell of the listed slots (the adjoint's coefficients are fixed
+// during a solve: pw, pg and the sample's Dw, Dg do not change).
+__global__ __launch_bounds__(256) void k_adj_prep(const float *__restrict__ pw, const float *__restrict__ pg,
+                                                  const int *__restrict__ work, const int *__restrict__ slot_sample,
+                                                  const float *__restrict__ sprm, float *__restrict__ ik)
+{
+    const int slot = work[blockIdx.x];
+    const float *q = sprm + 4 * slot_sample[slot];
+    const float Dw = q[0], Dg = q[1];
+    const size_t base = (size_t)slot * B3;
+    for (int i = threadIdx.x; i < B3; i += 256) ik[base + i] = inv_k(pw[base + i], pg[base + i], Dw, Dg);
+}
+
+// mu_N = [0 < u_N < 1] dL/du of the flip-rate model at the final state
+// (uniform grids): only the Gaussian PET term depends smoothly on u; the
+// Bernoulli terms are piecewise constant in u (derivative zero almost
+// everywhere).
 __global__ __launch_bounds__(256) void k_lik_flip_grad(const float *__restrict__ u, const float *__restrict__ pet,
                                                        const int *__restrict__ work, const int *__restrict__ index,
-                                                       int gpd, int stride, double inv_s2, float *__restrict__ lam)
+                                                       int gpd, int stride, double inv_s2, float *__restrict__ mu)
 {
     // observations on the grid coarser by `stride`: only model voxels at multiples of it are observed
     const int slot = work[blockIdx.x];
@@ -206,7 +224,7 @@ __global__ __launch_bounds__(256) void k_lik_flip_grad(const float *__restrict__
             const float p = pet[gx / stride + go * (gy / stride + go * (gz / stride))];
             if (!__builtin_isnan(p)) g = (float)(((double)p - (double)u[base + i]) * inv_s2);
         }
-        lam[base + i] = g;
+        mu[base + i] = mu_of(g, u[base + i]);
     }
 }
 
@@ -282,6 +300,14 @@ hipError_t launch_adj_step(const AdjArgs &a, hipStream_t st)
 {
     if (a.nwork <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_adj_step<8>, dim3(a.nwork * 2), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_prep(const float *pw, const float *pg, const int *work, int nwork, const int *slot_sample,
+                           const float *sprm, float *ik, hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_prep, dim3(nwork), dim3(256), 0, st, pw, pg, work, slot_sample, sprm, ik);
     return hipGetLastError();
 }
 

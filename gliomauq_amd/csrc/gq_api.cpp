@@ -200,6 +200,7 @@ struct gq_ctx {
     // discrete adjoint: checkpoints, recomputed states, adjoint fields, accumulators
     DevBuf<float> d_ckpt, d_stash, d_lam;
     DevBuf<double> d_gacc, d_icp, d_icg;
+    DevBuf<float> d_adj_ik;  // per-cell 1/K of the adjoint
 
     // captured step chunks
     hipGraph_t graph = nullptr;
@@ -1490,13 +1491,15 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
         int lc = 0;
         HIPCHK(gq::launch_lik_flip_grad(ctx->phi[ctx->cur], ctx->d_petf.p, ctx->p_work, nw, ctx->d_index.p, gpd,
                                         ostride, 1.0 / (sigma * sigma), lam[lc], st));
+        ctx->d_adj_ik.reserve(pool);
+        HIPCHK(gq::launch_adj_prep(ctx->pw, ctx->pg, ctx->p_work, nw, ctx->d_slot_sample.p, ctx->d_sprm.p,
+                                   ctx->d_adj_ik.p, st));
         const int NS = gq::adj_slabs();
         ctx->d_gacc.reserve((size_t)ctx->cap * NS * 2);
         HIPCHK(hipMemsetAsync(ctx->d_gacc.p, 0, (size_t)ctx->cap * NS * 2 * sizeof(double), st));
         HIPCHK(hipEventRecord(ctx->events[2], st));
         gq::AdjArgs g;
-        g.pw = ctx->pw;
-        g.pg = ctx->pg;
+        g.ik = ctx->d_adj_ik.p;
         g.work = ctx->p_work;
         g.nwork = nw;
         g.fnbr = ctx->d_fnbr.p;
@@ -1521,9 +1524,9 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
             }
             for (int j = m - 1; j >= 0; j--) {
                 g.u = state(j);
-                g.u_next = state(j + 1);
-                g.lam_in = lam[lc];
-                g.lam_out = lam[lc ^ 1];
+                g.mu_in = lam[lc];
+                g.out = lam[lc ^ 1];
+                g.out_lambda = seg == 0 && j == 0;  // lambda_0 for the initial-condition gradient
                 HIPCHK(gq::launch_adj_step(g, st));
                 lc ^= 1;
             }

@@ -56,11 +56,10 @@ struct UniformArgs {
 // neighbour at the same level): lambda_n from lambda_{n+1}, u_n and u_{n+1}.
 struct AdjArgs {
     const float *u;       // u_n
-    const float *u_next;  // u_{n+1} (its clamp gives the forward's mask)
-    const float *pw;
-    const float *pg;
-    const float *lam_in;  // lambda_{n+1}
-    float *lam_out;       // lambda_n
+    const float *ik;      // 1/K per cell (launch_adj_prep)
+    const float *mu_in;   // mu_{n+1} = [0 < u_{n+1} < 1] lambda_{n+1}
+    float *out;           // mu_n, or lambda_n when out_lambda
+    int out_lambda;
     const int *work;
     int nwork;
     const int *fnbr;
@@ -72,9 +71,13 @@ struct AdjArgs {
 };
 int adj_slabs();
 hipError_t launch_adj_step(const AdjArgs &a, hipStream_t st);
-// lam = dL/du of the flip-rate model's PET term at u (uniform grids; observations on the grid coarser by stride)
+// ik = 1/K of every cell of the listed slots for the slot's sample's (Dw, Dg)
+hipError_t launch_adj_prep(const float *pw, const float *pg, const int *work, int nwork, const int *slot_sample,
+                           const float *sprm, float *ik, hipStream_t st);
+// mu = [0 < u < 1] dL/du of the flip-rate model's PET term at u (uniform grids; observations on the grid
+// coarser by stride)
 hipError_t launch_lik_flip_grad(const float *u, const float *pet, const int *work, int nwork, const int *index,
-                                int gpd, int stride, double inv_s2, float *lam, hipStream_t st);
+                                int gpd, int stride, double inv_s2, float *mu, hipStream_t st);
 // Gaussian initial condition per sample, icp[s*5] = x0, y0, z0 (voxels), A, sigma (voxels)
 hipError_t launch_ic_gaussian(float *phi, const int *work, int nwork, const int *index, const int *slot_sample,
                               const double *icp, int gpd, hipStream_t st);
