@@ -153,6 +153,8 @@ struct gq_ctx {
     bool face_rows_valid = false;  // device face-ghost rows (ptr/src/w) match the tables
     DevBuf<double> d_ghost;  // cross-level lab points (3 fields for a split)
     DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2, d_sprm;
+    DevBuf<float> d_ikc;          // per-cell 1/K codes of the step (pool layout)
+    bool step_coeffs = true;      // false: the step recomputes 1/K from pw, pg every step
     bool fik_valid = false;       // false: every slot's face 1/K must be recomputed
     std::vector<char> fik_stale;  // per sample: recompute that sample's slots
     std::vector<float> fik_key;   // Dw, Dg (per sample when per-sample parameters are set)
@@ -435,8 +437,9 @@ struct gq_ctx {
         HIPCHK(gq::launch_face_rows_abs(p_work, (int)work.size(), d_fbase.p, p_sfbase, d_slot_sample.p, d_frow.p, st));
         face_nrows = (int)faces.size() * 256;
         face_rows_valid = false;
-        if (d_fik.n < (size_t)cap * 6 * 256) {
+        if (d_fik.n < (size_t)cap * 6 * 256 || d_ikc.n < (size_t)cap * gq::BS3) {
             d_fik.reserve((size_t)cap * 6 * 256);  // contents not kept
+            d_ikc.reserve((size_t)cap * gq::BS3);
             fik_valid = false;
         }
         if (!d_ih2.p) {
@@ -525,8 +528,8 @@ struct gq_ctx {
             HIPCHK(hipStreamSynchronize(st));  // sl is released on return
         }
         HIPCHK(gq::launch_face_prep(pw, pg, list, n, d_fnbr.p, d_fbase.p, p_sfbase, d_slot_sample.p, d_face_ptr.p,
-                                    d_face_src.p, d_face_w.p, d_fik.p, key[0], key[1], per_sample ? d_sprm.p : nullptr,
-                                    st));
+                                    d_face_src.p, d_face_w.p, d_fik.p, d_ikc.p, key[0], key[1],
+                                    per_sample ? d_sprm.p : nullptr, st));
         fik_stale.assign(S, 0);
     }
 
@@ -840,6 +843,7 @@ struct gq_ctx {
         a.fnbr = d_fnbr.p;
         a.frow = d_frow.p;
         a.fik = d_fik.p;
+        a.ikc = step_coeffs ? d_ikc.p : nullptr;
         a.gphi = d_gphi.p;
         a.fptr = d_face_ptr.p;
         a.fsrc = d_face_src.p;
@@ -1832,6 +1836,14 @@ gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt)
     });
 }
 
+gq_status gq_set_step_coeffs(gq_ctx *ctx, int precomputed)
+{
+    return guard(ctx, [&] {
+        if (precomputed != 0 && precomputed != 1) throw gq_error(GQ_ERR_ARG, "precomputed must be 0 or 1");
+        ctx->step_coeffs = precomputed == 1;
+    });
+}
+
 gq_status gq_autotune_info(gq_ctx *ctx, int *zt, int *fused, double us[10])
 {
     return guard(ctx, [&] {
@@ -1867,7 +1879,8 @@ gq_status gq_memory_info(gq_ctx *ctx, int64_t *bytes, int *pool_blocks)
     return guard(ctx, [&] {
         int64_t b = (int64_t)ctx->cap * gq::BS3 * 4 * 4;
         b += (int64_t)(ctx->d_ghost.n * 8);
-        b += (int64_t)(ctx->d_face_w.n * 4 + ctx->d_face_src.n * 4 + ctx->d_face_ptr.n * 8 + ctx->d_fik.n * 4);
+        b += (int64_t)(ctx->d_face_w.n * 4 + ctx->d_face_src.n * 4 + ctx->d_face_ptr.n * 8 + ctx->d_fik.n * 4 +
+                       ctx->d_ikc.n * 4);
         if (bytes) *bytes = b;
         if (pool_blocks) *pool_blocks = ctx->cap;
     });

@@ -63,6 +63,50 @@ __device__ __forceinline__ float inv_k(float w, float g, float Dw, float Dg)
     return b * frcp(D);
 }
 
+// Per-cell step coefficient, precomputed once per anatomy and parameter set:
+// the cell's 1/K where that value also settles the tissue test, else the
+// marker IK_SLOW (the step then recomputes 1/K and p_w + p_g from pw, pg).
+// With p_w, p_g >= 0 and phi >= 0, tissue = p_w + p_g + phi > 0 holds exactly
+// when p_w + p_g > 0 (then 1/K is never NaN) or phi > 0 (p_w = p_g = 0, 1/K =
+// 0/0 = NaN); cells with a negative p_w or p_g (Interp4 prolongation can make
+// them) or with p_w = -p_g != 0 take the marker, and a cell whose phi is
+// negative (after a split) recomputes its own sum in the step.  The step's
+// results are bit-identical to recomputing everything every step.
+constexpr unsigned IK_SLOW = 0xFFFFFFFFu;
+
+__device__ __forceinline__ float ik_code(float w, float g, float Dw, float Dg)
+{
+#pragma clang fp contract(off)
+    const float ik = inv_k(w, g, Dw, Dg);
+    const float b = w + g;
+    const bool nan = ik != ik;
+    return ((b > 0.f && !nan) || (b == 0.f && nan)) ? ik : __uint_as_float(IK_SLOW);
+}
+
+// 1/K of a neighbour cell from its code.
+__device__ __forceinline__ float ik_of(float code, const float *__restrict__ pw, const float *__restrict__ pg,
+                                       size_t i, float Dw, float Dg)
+{
+    return __float_as_uint(code) != IK_SLOW ? code : inv_k(pw[i], pg[i], Dw, Dg);
+}
+
+// 1/K of the updated cell and a stand-in for p_w + p_g that gives the same
+// tissue test (bsum + phi > 0) as the true sum.
+__device__ __forceinline__ void ik_centre(float code, float pc, const float *__restrict__ pw,
+                                          const float *__restrict__ pg, size_t i, float Dw, float Dg, float &ik,
+                                          float &bsum)
+{
+#pragma clang fp contract(off)
+    if (__float_as_uint(code) != IK_SLOW && pc >= 0.f) {
+        ik = code;
+        bsum = code != code ? 0.f : 1.f;
+    } else {
+        const float w = pw[i], g = pg[i];
+        ik = inv_k(w, g, Dw, Dg);
+        bsum = w + g;
+    }
+}
+
 // One cell of the fused RD operator + UpdateTumor (Case2,
 // Glioma_ReactionDiffusionOperator.h:150-210, UpdateTumor :224-236): phi of
 // the cell and its six face neighbours (-x,+x,-y,+y,-z,+z), their 1/K, and
@@ -183,16 +227,35 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
     }
 
     float breg[ZT];
+    if (a.ikc) {
+        float p[ZT], c[ZT];
 #pragma unroll
-    for (int k = 0; k < ZT; k++) {
-        const size_t i = base + (z0 + k) * 256 + t;
-        const float p = a.phi_in[i];
-        const float wm = a.pw[i];
-        const float gm = a.pg[i];
-        breg[k] = wm + gm;
-        const int l = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
-        s_phi[l] = p;
-        s_ik[l] = inv_k(wm, gm, Dw, Dg);
+        for (int k = 0; k < ZT; k++) {
+            const size_t i = base + (z0 + k) * 256 + t;
+            p[k] = a.phi_in[i];
+            c[k] = a.ikc[i];
+        }
+#pragma unroll
+        for (int k = 0; k < ZT; k++) {
+            const size_t i = base + (z0 + k) * 256 + t;
+            const int l = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
+            float ik;
+            ik_centre(c[k], p[k], a.pw, a.pg, i, Dw, Dg, ik, breg[k]);
+            s_phi[l] = p[k];
+            s_ik[l] = ik;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < ZT; k++) {
+            const size_t i = base + (z0 + k) * 256 + t;
+            const float p = a.phi_in[i];
+            const float wm = a.pw[i];
+            const float gm = a.pg[i];
+            breg[k] = wm + gm;
+            const int l = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
+            s_phi[l] = p;
+            s_ik[l] = inv_k(wm, gm, Dw, Dg);
+        }
     }
     // z halo planes
 #pragma unroll
@@ -203,7 +266,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
         if (z >= 0 && z < B) {
             const size_t i = base + z * 256 + t;
             pv = a.phi_in[i];
-            kv = inv_k(a.pw[i], a.pg[i], Dw, Dg);
+            kv = a.ikc ? ik_of(a.ikc[i], a.pw, a.pg, i, Dw, Dg) : inv_k(a.pw[i], a.pg[i], Dw, Dg);
         } else {
             const int f = side == 0 ? 4 : 5;
             const int nb = a.fnbr[slot * 6 + f];
@@ -297,15 +360,17 @@ __global__ __launch_bounds__(256) void k_face_rows_abs(const int *__restrict__ w
     frow[slot * 6 + f] = fb >= 0 ? sfbase[slot_sample[slot]] + fb : -1;
 }
 
-// 1/K of every face-halo cell (static between topology / parameter changes).
+// 1/K of every face-halo cell and, when ikc is set, the per-cell step
+// coefficients of the block (static between topology / parameter changes).
 __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw, const float *__restrict__ pg,
                                                    const int *__restrict__ work, int nwork,
                                                    const int *__restrict__ fnbr, const int *__restrict__ fbase,
                                                    const int *__restrict__ sfbase,
                                                    const int *__restrict__ slot_sample,
                                                    const int64_t *__restrict__ ptr, const int *__restrict__ src,
-                                                   const float *__restrict__ w, float *__restrict__ fik, float Dw,
-                                                   float Dg, const float *__restrict__ sprm)
+                                                   const float *__restrict__ w, float *__restrict__ fik,
+                                                   float *__restrict__ ikc, float Dw, float Dg,
+                                                   const float *__restrict__ sprm)
 {
     const int wi = blockIdx.x;
     if (wi >= nwork) return;
@@ -354,6 +419,17 @@ __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw,
         }
         fik[((size_t)slot * 6 + f) * 256 + t] = inv_k(wm, gm, Dw, Dg);
     }
+    if (ikc) {
+        const size_t b0 = (size_t)slot * B3;
+        float wv[16], gv[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            wv[k] = pw[b0 + k * 256 + t];
+            gv[k] = pg[b0 + k * 256 + t];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) ikc[b0 + k * 256 + t] = ik_code(wv[k], gv[k], Dw, Dg);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -396,17 +472,25 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         else pB = step_face_ghost(a, a.frow[slot * 6 + 4] + t);
         kB = a.fik[((size_t)slot * 6 + 4) * 256 + t];
     }
+    // 1/K and p_w + p_g of plane z of this column, from the precomputed
+    // coefficients (code = the loaded value) or from pw, pg (w, g)
+    const bool pre = a.ikc != nullptr;
+    auto resolve = [&](int z, float p, float cw, float g, float &ik, float &bs) {
+        if (pre) ik_centre(cw, p, a.pw, a.pg, base + z * 256 + t, Dw, Dg, ik, bs);
+        else {
+            ik = inv_k(cw, g, Dw, Dg);
+            bs = cw + g;
+        }
+    };
     float pC = a.phi_in[base + t];
     float bC;
     float kC;
-    {
-        const float wm = a.pw[base + t], gm = a.pg[base + t];
-        bC = wm + gm;
-        kC = inv_k(wm, gm, Dw, Dg);
-    }
-    // z+1 plane (register prefetch)
+    if (pre) resolve(0, pC, a.ikc[base + t], 0.f, kC, bC);
+    else resolve(0, pC, a.pw[base + t], a.pg[base + t], kC, bC);
+    // z+1 plane (register prefetch): wF holds the code when the coefficients are precomputed
     float pF = a.phi_in[base + 256 + t];
-    float wF = a.pw[base + 256 + t], gF = a.pg[base + 256 + t];
+    float wF = pre ? a.ikc[base + 256 + t] : a.pw[base + 256 + t];
+    float gF = pre ? 0.f : a.pg[base + 256 + t];
     const float ih2 = a.ih2_lvl[a.level[slot]];
     for (int k = 0; k < B; k++) {
         const int buf = k & 1;
@@ -431,8 +515,8 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
             sk[l] = kv;
         }
         // plane k+1's 1/K for this column (pF, wF, gF were loaded one plane ahead)
-        float kF;
-        if (k + 1 < B) kF = inv_k(wF, gF, Dw, Dg);
+        float kF, bF = 0.f;
+        if (k + 1 < B) resolve(k + 1, pF, wF, gF, kF, bF);
         else {
             const int nb = a.fnbr[slot * 6 + 5];
             if (nb >= 0) pF = a.phi_in[(size_t)nb * B3 + nbr_face_cell(5, tx, ty)];
@@ -444,8 +528,11 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         if (k + 2 < B) {
             const size_t i = base + (k + 2) * 256 + t;
             pN = a.phi_in[i];
-            wN = a.pw[i];
-            gN = a.pg[i];
+            if (pre) wN = a.ikc[i];
+            else {
+                wN = a.pw[i];
+                gN = a.pg[i];
+            }
         }
         __syncthreads();
         const float pc = pC;
@@ -459,7 +546,7 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         kB = kC;
         pC = pF;
         kC = kF;
-        bC = wF + gF;
+        bC = bF;
         pF = pN;
         wF = wN;
         gF = gN;
@@ -1381,12 +1468,12 @@ hipError_t launch_face_rows_abs(const int *work, int nwork, const int *fbase, co
 
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
                             const int *fbase, const int *sfbase, const int *slot_sample, const int64_t *ptr,
-                            const int *src, const float *w, float *fik, float Dw, float Dg, const float *sprm,
-                            hipStream_t st)
+                            const int *src, const float *w, float *fik, float *ikc, float Dw, float Dg,
+                            const float *sprm, hipStream_t st)
 {
     if (nwork <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_face_prep, dim3(nwork), dim3(256), 0, st, pw, pg, work, nwork, fnbr, fbase, sfbase,
-                       slot_sample, ptr, src, w, fik, Dw, Dg, sprm);
+                       slot_sample, ptr, src, w, fik, ikc, Dw, Dg, sprm);
     return hipGetLastError();
 }
 

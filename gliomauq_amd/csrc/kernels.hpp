@@ -18,6 +18,7 @@ struct StepArgs {
     const int *fnbr;   // [slot*6+f]
     const int *frow;   // [slot*6+f] first face-ghost row of the face (absolute; unused for same-level faces)
     const float *fik;  // [slot*6+f][256]
+    const float *ikc;  // [slot*4096] per-cell 1/K codes (k_face_prep), or null: 1/K from pw, pg every step
     const float *gphi; // cross-level face ghosts of phi_in, or null: the step evaluates the rows itself
     const int64_t *fptr;  // face-ghost rows (ptr/src/w) for the in-kernel evaluation
     const int *fsrc;
@@ -96,8 +97,8 @@ hipError_t launch_face_rows_abs(const int *work, int nwork, const int *fbase, co
                                 const int *slot_sample, int *frow, hipStream_t st);
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
                             const int *fbase, const int *sfbase, const int *slot_sample, const int64_t *ptr,
-                            const int *src, const float *w, float *fik, float Dw, float Dg, const float *sprm,
-                            hipStream_t st);
+                            const int *src, const float *w, float *fik, float *ikc, float Dw, float Dg,
+                            const float *sprm, hipStream_t st);
 // Slot-indexed tables and the scatter of packed rows {slot, level, index[3],
 // sample, lutoff, fnbr[6], fbase[6], nbr26[26]} into them.
 struct SlotTables {

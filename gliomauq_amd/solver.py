@@ -232,6 +232,13 @@ def set_autotune(ctx, mode=1, fixed_zt=16):
     ctx.call("gq_set_autotune", int(mode), int(fixed_zt))
 
 
+def set_step_coeffs(ctx, precomputed=True):
+    """True (default): the step reads 1/K precomputed per cell (phi and 1/K,
+    8 B per cell); False: it recomputes 1/K from pw, pg every step.  Same
+    bits either way."""
+    ctx.call("gq_set_step_coeffs", 1 if precomputed else 0)
+
+
 def autotune_info(ctx):
     """(slab depth, face ghosts in the step?, {"<depth>": us, "<depth>f": us})
     of the last step call and the last autotune timings."""

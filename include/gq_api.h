@@ -257,6 +257,13 @@ gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel);
  * launch; mode 3: always fixed_zt with the in-step evaluation.  Every
  * configuration gives the same bits. */
 gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt);
+/* Step coefficients (no reference counterpart; the reference's functor
+ * recomputes D/b from p_w, p_g per face, Glioma_ReactionDiffusionOperator.h:
+ * 174-188).  precomputed = 1 (default): 1/K of every cell is computed once
+ * per anatomy, topology and parameter set, and a step reads phi and 1/K
+ * (8 B per cell) instead of phi, pw, pg (12 B); 0: the step recomputes 1/K
+ * from pw, pg every step.  Both give the same bits. */
+gq_status gq_set_step_coeffs(gq_ctx *ctx, int precomputed);
 /* Slab depth and face-ghost mode (1 = evaluated in the step) used by the last
  * step call, and the last autotune timings in us per step: [0..4] depths 16,
  * 8, 4, 2, 1 with the separate face-ghost launch (both kernels timed), [5..9]

@@ -144,10 +144,11 @@ def test_step_variants_bit_identical():
     # a tumour inside one 32^3 octant: the other seven compress to level 1
     u0 = A.initial_condition(n, (n // 4, n // 4, n // 4), 0.6, 3.0)
     outs = []
-    for mode in (2, 3):
-        for zt in (16, 8, 4, 2, 1):
+    for mode, zt, pre in [(m, z, True) for m in (2, 3) for z in (16, 8, 4, 2, 1)] + [(2, 4, False), (2, 1, False)]:
+        if True:
             ctx = gq.Context(0)
             gq.set_autotune(ctx, mode, zt)
+            gq.set_step_coeffs(ctx, pre)
             g = gq.Grid(ctx, 4, 3, 4)
             for name, v in (("phi", u0), ("pw", pwm), ("pg", pgm)):
                 g.write_volume(name, v)
@@ -161,3 +162,42 @@ def test_step_variants_bit_identical():
     assert 8 < outs[0][0] < 64  # the grid is adaptive
     for nb, a in outs[1:]:
         assert nb == outs[0][0] and np.array_equal(a, outs[0][1])
+
+
+@pytest.mark.parametrize("zt", [16, 4, 1])
+def test_precomputed_coefficients_bit_identical(ctx, zt):
+    """The per-cell 1/K codes give the same bits as recomputing 1/K from pw,
+    pg every step, also for the cells that take the slow path: negative pw or
+    pg (as Interp4 prolongation makes them), p_w = -p_g != 0, p_w + p_g < 0,
+    CSF cells with and without tumour, and negative phi (as after a split)."""
+    n = 32
+    rng = np.random.default_rng(11)
+    pw = rng.random((n, n, n)).astype(np.float32)
+    pg = (1 - pw).astype(np.float32)
+    phi = (0.5 * rng.random((n, n, n))).astype(np.float32)
+    flat = [a.reshape(-1) for a in (pw, pg, phi)]
+    idx = rng.permutation(n ** 3)
+    k = 0
+    for w_, g_, p_ in [(-0.01, 0.4, None), (0.3, -0.02, None), (0.2, -0.2, None), (-0.1, -0.05, None),
+                       (0.0, 0.0, 0.3), (0.0, 0.0, 0.0), (0.0, 0.0, -0.01), (None, None, -0.02),
+                       (-0.0, 0.0, 0.1), (1e-30, 0.0, None)]:
+        sel = idx[k:k + 400]
+        k += 400
+        if w_ is not None:
+            flat[0][sel] = w_
+            flat[1][sel] = g_
+        if p_ is not None:
+            flat[2][sel] = p_
+    res = []
+    for pre in (False, True):
+        c = gq.Context(0)
+        gq.set_autotune(c, 2, zt)
+        gq.set_step_coeffs(c, pre)
+        g = gq.Grid(c, 2, 1, 2)
+        for name, v in (("phi", phi), ("pw", pw), ("pg", pg)):
+            g.sample_volume(name, v)
+        gq.fk_steps(g, 2e-4, 2e-5, 0.3, 0.5, 3)
+        res.append(g.dump_uniform())
+        c.close()
+    assert np.isfinite(res[0]).all()
+    assert res[0].tobytes() == res[1].tobytes()
