@@ -136,11 +136,18 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
     const float ih2 = a.ih2_lvl[a.level[slot]];
     const int offs[6] = {-1, 1, -LW, LW, -PL, PL};
     double gD = 0.0, gR = 0.0;
+#pragma unroll
     for (int k = 0; k < ZT; k++) {
         const int c = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
         const float uc = s_u[c];
         const float kc = s_ik[c];
         const float muc = s_mu[c];
+        // face coefficients 2 / (1/K_c + 1/K_k): the same value serves the
+        // cell's own update and, transposed, the neighbour's (the sum is
+        // commutative); a face whose value is not finite is never used
+        float fc[6];
+#pragma unroll
+        for (int f = 0; f < 6; f++) fc[f] = 2.f * frcp(kc + s_ik[c + offs[f]]);
         float lam = 0.f;
         if (muc != 0.f) {  // the cell's own update was inside (0,1): its coefficients are finite
             float d[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -148,7 +155,7 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
             // positive exactly where pw+pg > 0 (else 0/0 = NaN), as for the neighbours below
             if ((kc > 0.f && kc < INFINITY) || uc > 0.f) {
 #pragma unroll
-                for (int f = 0; f < 6; f++) d[f] = 2.f * frcp(kc + s_ik[c + offs[f]]);
+                for (int f = 0; f < 6; f++) d[f] = fc[f];
             }
             const float sdf = d[0] + d[1] + d[2] + d[3] + d[4] + d[5];
             const float diff = d[0] * (s_u[c - 1] - uc) + d[1] * (s_u[c + 1] - uc) + d[2] * (s_u[c - LW] - uc) +
@@ -169,7 +176,7 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
                 // u >= 0, so tissue_n > 0 <=> 1/K_n finite || u_n > 0
                 const float kn = s_ik[n];
                 const bool tissue = (kn > 0.f && kn < INFINITY) || s_u[n] > 0.f;
-                if (tissue) tr += mun * (2.f * frcp(kn + kc));
+                if (tissue) tr += mun * fc[f];
             }
         }
         lam += dt * ih2 * tr;

@@ -124,14 +124,23 @@ __device__ __forceinline__ float fk_cell(float pc, float pW, float pE, float pS,
         d3 = 2.f * frcp(kc + kN);
         d4 = 2.f * frcp(kc + kB);
         d5 = 2.f * frcp(kc + kF);
+        // Neumann doubling, sequential as the reference (:198-203).  It can
+        // only act when some coefficient is zero (1/K = inf next to the
+        // face); outside the tissue every coefficient is zero and doubling
+        // them changes nothing.  A NaN coefficient makes the result 0
+        // whatever the doubling does, so min() may ignore NaNs.
+        const float mn = __builtin_fminf(__builtin_fminf(__builtin_fminf(__builtin_fabsf(d0), __builtin_fabsf(d1)),
+                                                         __builtin_fminf(__builtin_fabsf(d2), __builtin_fabsf(d3))),
+                                         __builtin_fminf(__builtin_fabsf(d4), __builtin_fabsf(d5)));
+        if (mn == 0.f) {
+            if (d0 == 0.f && pW > 0.f) d1 *= 2.f;
+            if (d1 == 0.f && pE > 0.f) d0 *= 2.f;
+            if (d2 == 0.f && pS > 0.f) d3 *= 2.f;
+            if (d3 == 0.f && pN > 0.f) d2 *= 2.f;
+            if (d4 == 0.f && pB > 0.f) d5 *= 2.f;
+            if (d5 == 0.f && pF > 0.f) d4 *= 2.f;
+        }
     }
-    // Neumann doubling, sequential as the reference (:198-203)
-    if (d0 == 0.f && pW > 0.f) d1 *= 2.f;
-    if (d1 == 0.f && pE > 0.f) d0 *= 2.f;
-    if (d2 == 0.f && pS > 0.f) d3 *= 2.f;
-    if (d3 == 0.f && pN > 0.f) d2 *= 2.f;
-    if (d4 == 0.f && pB > 0.f) d5 *= 2.f;
-    if (d5 == 0.f && pF > 0.f) d4 *= 2.f;
     const float sdf = d0 + d1 + d2 + d3 + d4 + d5;
     const float reac = rho * pc * (1.f - pc);
     float dphi;
@@ -300,7 +309,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
     __syncthreads();
 
     const float ih2 = a.ih2_lvl[a.level[slot]];
-#pragma unroll 2
+#pragma unroll
     for (int k = 0; k < ZT; k++) {
         const int c = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
         const float pc = s_phi[c];

@@ -294,6 +294,31 @@ def ess_batch_means(x, n_batches=None):
     return float(min(n, n * var / bvar)) if bvar > 0 else float(n)
 
 
+def ess_chains(x):
+    """ESS of C parallel chains x[iteration, chain] (one parameter) from the
+    spread of the chain means: C * var / var(chain means), at most the sample
+    count.  Each chain is one batch, so the estimate needs no batch length
+    shorter than the chains themselves; chains that have not forgotten
+    dispersed starts inflate var(chain means) and lower the estimate."""
+    x = np.asarray(x, np.float64)
+    n, c = x.shape
+    if c < 2 or n < 2:
+        return float(n * c)
+    var = x.var(ddof=1)
+    if var == 0:
+        return 1.0
+    vm = x.mean(0).var(ddof=1)
+    return float(min(n * c, c * var / vm)) if vm > 0 else float(n * c)
+
+
+def ess_parallel(x):
+    """Conservative ESS of parallel chains x[iteration, chain] (one
+    parameter): the smaller of the per-chain batch-means sum and the
+    chain-means estimate."""
+    x = np.asarray(x, np.float64)
+    return min(sum(ess_batch_means(x[:, k]) for k in range(x.shape[1])), ess_chains(x))
+
+
 def tmcmc(post, prior_sample, n_pop, rng, cov_scale=0.2, max_stages=50, moves=1, cov_target=1.0):
     """Transitional MCMC: tempered stages p_j ~ L^q_j prior, q_j chosen so the
     coefficient of variation of the incremental weights is cov_target;

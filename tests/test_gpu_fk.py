@@ -201,3 +201,24 @@ def test_precomputed_coefficients_bit_identical(ctx, zt):
         c.close()
     assert np.isfinite(res[0]).all()
     assert res[0].tobytes() == res[1].tobytes()
+
+
+def test_neumann_doubling_matches_oracle(ctx):
+    """Dw = 0 with white-matter-only cells (pw = 1, pg = 0: D = 0, 1/K = inf)
+    next to grey-matter cells: the faces between them get df = 0, and the
+    reference doubles the opposite face (Glioma_ReactionDiffusionOperator.h:
+    198-203).  FP32 field against the FP64 oracle."""
+    n = 32
+    rng = np.random.default_rng(17)
+    pw = (rng.random((n, n, n)) < 0.3).astype(np.float32)
+    pg = (1 - pw).astype(np.float32)
+    phi = (0.6 * rng.random((n, n, n))).astype(np.float32)
+    Dw, Dg, rho, dt, steps = 0.0, 2e-5, 0.05, 0.9 / (n * n * 6 * 2e-5), 4
+    g = gq.Grid(ctx, 2, 1, 2)
+    for name, v in (("phi", phi), ("pw", pw), ("pg", pg)):
+        g.sample_volume(name, v)
+    gq.fk_steps(g, Dw, Dg, rho, dt, steps)
+    u = g.dump_uniform()
+    ref = O.fk_uniform_steps(phi.astype(np.float64), pw.astype(np.float64), pg.astype(np.float64), 16, 1.0 / n,
+                             np.float32(Dw), np.float32(Dg), np.float32(rho), dt, steps, mode="f64")
+    assert rel_l2(u, ref) <= 1e-5 and np.max(np.abs(u - ref)) <= 1e-5

@@ -83,3 +83,37 @@ def test_tmcmc_reaches_posterior():
     assert stages[-1]["q"] == 1.0
     assert np.all(np.abs(th.mean(0) - MU) < 0.15 * SD)
     assert np.all(np.abs(th.std(0) / SD - 1) < 0.1)
+
+
+def _ar1_chains(rng, n, c, phi, x0=None):
+    x = np.zeros((n, c))
+    x[0] = rng.standard_normal(c) if x0 is None else x0
+    e = rng.standard_normal((n, c)) * np.sqrt(1 - phi * phi)
+    for t in range(1, n):
+        x[t] = phi * x[t - 1] + e[t]
+    return x
+
+
+def test_ess_chains_ar1():
+    """Parallel stationary AR(1) chains: ESS = n C (1 - phi) / (1 + phi)
+    (integrated autocorrelation time (1 + phi) / (1 - phi))."""
+    rng = np.random.default_rng(4)
+    n, c, phi = 1000, 200, 0.8
+    x = _ar1_chains(rng, n, c, phi)
+    exact = n * c * (1 - phi) / (1 + phi)
+    assert abs(SM.ess_chains(x) / exact - 1) < 0.25
+    assert abs(SM.ess_parallel(x) / exact - 1) < 0.25
+    # short chains (60 samples, correlation time 39): the information in a
+    # chain mean is n^2 / sum_ij phi^|i-j| samples; batch means over-count
+    # (batches shorter than the correlation), the chain-means estimate does
+    # not, and the parallel estimate takes the smaller
+    n, c, phi = 60, 400, 0.95
+    xs = _ar1_chains(rng, n, c, phi)
+    k = np.arange(1, n)
+    ex = c * n * n / (n + 2 * np.sum((n - k) * phi ** k))
+    bm = sum(SM.ess_batch_means(xs[:, j]) for j in range(c))
+    assert bm > 2 * ex
+    assert abs(SM.ess_parallel(xs) / ex - 1) < 0.25
+    # chains that still remember dispersed starts: lower estimate
+    xd = _ar1_chains(rng, 200, 100, 0.9, x0=np.linspace(-20, 20, 100))
+    assert SM.ess_chains(xd) < 0.9 * 200 * 100 * 0.1 / 1.9
