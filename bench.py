@@ -638,6 +638,8 @@ def main():
     ap.add_argument("--steps-adj-reps", type=int, default=2)
     ap.add_argument("--samples", type=int, default=256, help="ensemble samples")
     ap.add_argument("--kernel-timing", action="store_true", help="event-time every step kernel (no graphs)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="step autotune mode (1: slab depths; 4: also the column kernel and in-step face ghosts)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (samples would shard across GPUs; "
@@ -666,6 +668,7 @@ def main():
     tol_r, tol_c = 1e-3, 1e-4
 
     ctx = gq.Context(0)
+    gq.set_autotune(ctx, args.autotune)
 
     def fresh_grid():
         g = gq.Grid(ctx, bpd, max_level, 4)

@@ -63,6 +63,15 @@ __device__ __forceinline__ int nbr_face_cell(int f, int u, int v)
 
 __device__ __forceinline__ float mu_of(float lam, float u) { return (u > 0.f && u < 1.f) ? lam : 0.f; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous chunk of the Morton-ordered work list so the halo reads of
+// neighbouring blocks hit the same L2 (nwg is a multiple of 8).
+__device__ __forceinline__ int xcd_remap(int wg, int nwg)
+{
+    const int per = nwg >> 3;
+    return (wg & 7) * per + (wg >> 3);
+}
+
 __device__ __forceinline__ double wave_sum(double x)
 {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
@@ -81,8 +90,8 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
     __shared__ float s_ik[LZ * PL];
     __shared__ float s_mu[LZ * PL];
     __shared__ double s_red[2][4];
-    const int item = blockIdx.x;
-    if (item >= a.nwork * NS) return;
+    const int item = xcd_remap(blockIdx.x, gridDim.x);
+    if (item >= a.nwork * NS) return;  // padding workgroups (whole workgroup exits)
     const int w = item / NS;
     const int slab = item - w * NS;
     const int z0 = slab * ZT;
@@ -301,12 +310,15 @@ __global__ __launch_bounds__(256) void k_ic_grad(const float *__restrict__ lam0,
     }
 }
 
-int adj_slabs() { return 2; }
+constexpr int ADJ_ZT = 4;  // slab depth: 23 KB of LDS per workgroup, six workgroups per CU
+
+int adj_slabs() { return B / ADJ_ZT; }
 
 hipError_t launch_adj_step(const AdjArgs &a, hipStream_t st)
 {
     if (a.nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_step<8>, dim3(a.nwork * 2), dim3(256), 0, st, a);
+    const int grid = (a.nwork * (B / ADJ_ZT) + 7) & ~7;
+    hipLaunchKernelGGL(k_adj_step<ADJ_ZT>, dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

@@ -154,6 +154,7 @@ struct gq_ctx {
     DevBuf<double> d_ghost;  // cross-level lab points (3 fields for a split)
     DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2, d_sprm;
     DevBuf<float> d_ikc;          // per-cell 1/K codes of the step (pool layout)
+    DevBuf<int> d_wrec;           // per-work-item step records (gq::WREC ints each)
     bool step_coeffs = true;      // false: the step recomputes 1/K from pw, pg every step
     bool fik_valid = false;       // false: every slot's face 1/K must be recomputed
     std::vector<char> fik_stale;  // per sample: recompute that sample's slots
@@ -451,6 +452,9 @@ struct gq_ctx {
             }
             d_ih2.upload(ih2, st);
         }
+        d_wrec.reserve((size_t)std::max<size_t>(1, work.size()) * gq::WREC);
+        HIPCHK(gq::launch_work_records(p_work, (int)work.size(), d_fnbr.p, d_frow.p, d_level.p, d_ih2.p,
+                                       d_slot_sample.p, d_wrec.p, st));
         HIPCHK(hipStreamSynchronize(st));  // the host vectors above are released on return
         tables_valid = true;
     }
@@ -844,6 +848,7 @@ struct gq_ctx {
         a.frow = d_frow.p;
         a.fik = d_fik.p;
         a.ikc = step_coeffs ? d_ikc.p : nullptr;
+        a.wrec = d_wrec.p;
         a.gphi = d_gphi.p;
         a.fptr = d_face_ptr.p;
         a.fsrc = d_face_src.p;

@@ -184,15 +184,39 @@ __device__ __forceinline__ float face_row_sum(const float *__restrict__ phi, con
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
     int64_t k = ptr[row];
     const int64_t e = ptr[row + 1];
-    for (; k + 3 < e; k += 4) {
-        s0 = __builtin_fmaf(w[k], phi[src[k]], s0);
-        s1 = __builtin_fmaf(w[k + 1], phi[src[k + 1]], s1);
-        s2 = __builtin_fmaf(w[k + 2], phi[src[k + 2]], s2);
-        s3 = __builtin_fmaf(w[k + 3], phi[src[k + 3]], s3);
+    // eight terms per round trip: all index/weight loads, then all values,
+    // then the FMAs in term order (term j into partial sum j mod 4)
+    for (; k < e; k += 8) {
+        int si[8];
+        float wk[8], v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int64_t kk = k + j < e ? k + j : k;
+            si[j] = src[kk];
+            wk[j] = w[kk];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = phi[si[j]];
+        if (k + 8 <= e) {
+            s0 = __builtin_fmaf(wk[0], v[0], s0);
+            s1 = __builtin_fmaf(wk[1], v[1], s1);
+            s2 = __builtin_fmaf(wk[2], v[2], s2);
+            s3 = __builtin_fmaf(wk[3], v[3], s3);
+            s0 = __builtin_fmaf(wk[4], v[4], s0);
+            s1 = __builtin_fmaf(wk[5], v[5], s1);
+            s2 = __builtin_fmaf(wk[6], v[6], s2);
+            s3 = __builtin_fmaf(wk[7], v[7], s3);
+        } else {
+            const int m = (int)(e - k);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if (j < m) {
+                    float &acc = (j & 3) == 0 ? s0 : (j & 3) == 1 ? s1 : (j & 3) == 2 ? s2 : s3;
+                    acc = __builtin_fmaf(wk[j], v[j], acc);
+                }
+            }
+        }
     }
-    if (k < e) s0 = __builtin_fmaf(w[k], phi[src[k]], s0);
-    if (k + 1 < e) s1 = __builtin_fmaf(w[k + 1], phi[src[k + 1]], s1);
-    if (k + 2 < e) s2 = __builtin_fmaf(w[k + 2], phi[src[k + 2]], s2);
     return (s0 + s1) + (s2 + s3);
 }
 
@@ -222,13 +246,14 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
     if (item >= a.nwork * NS) return;  // padding workgroups (whole workgroup exits)
     const int w = item / NS;
     const int z0 = (item - w * NS) * ZT;
-    const int slot = a.work[w];
+    const int *rec = a.wrec + (size_t)w * WREC;
+    const int slot = rec[0];
     const int t = threadIdx.x;
     const int tx = t & 15, ty = t >> 4;
     const size_t base = (size_t)slot * B3;
     float Dw = a.Dw, Dg = a.Dg, rho = a.rho, dt = a.dt;
     if (a.sprm) {
-        const float *q = a.sprm + 4 * a.slot_sample[slot];
+        const float *q = a.sprm + 4 * rec[14];
         Dw = q[0];
         Dg = q[1];
         rho = q[2];
@@ -278,24 +303,37 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
             kv = a.ikc ? ik_of(a.ikc[i], a.pw, a.pg, i, Dw, Dg) : inv_k(a.pw[i], a.pg[i], Dw, Dg);
         } else {
             const int f = side == 0 ? 4 : 5;
-            const int nb = a.fnbr[slot * 6 + f];
-            if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, tx, ty)];
-            else pv = step_face_ghost(a, a.frow[slot * 6 + f] + t);
-            kv = a.fik[((size_t)slot * 6 + f) * 256 + t];
+            const int nb = rec[2 + f];
+            if (nb >= 0) {
+                const size_t j = (size_t)nb * B3 + nbr_face_cell(f, tx, ty);
+                pv = a.phi_in[j];
+                // same level: the neighbour's own 1/K code (its block is being read anyway)
+                kv = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + f) * 256 + t];
+            } else {
+                pv = step_face_ghost(a, rec[8 + f] + t);
+                kv = a.fik[((size_t)slot * 6 + f) * 256 + t];
+            }
         }
         s_phi[l] = pv;
         s_ik[l] = kv;
     }
     // x / y faces of the slab: 4 faces x 16 x ZT cells
+    const int fn0 = rec[2], fn1 = rec[3], fn2 = rec[4], fn3 = rec[5];
+    const int fr0 = rec[8], fr1 = rec[9], fr2 = rec[10], fr3 = rec[11];
     for (int q = t; q < 4 * 16 * ZT; q += 256) {
         const int f = q / (16 * ZT);
         const int r = q - f * 16 * ZT;
         const int u = r & 15, v = z0 + (r >> 4);
-        const int nb = a.fnbr[slot * 6 + f];
-        float pv;
-        if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, u, v)];
-        else pv = step_face_ghost(a, a.frow[slot * 6 + f] + u + 16 * v);
-        const float kv = a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
+        const int nb = f == 0 ? fn0 : f == 1 ? fn1 : f == 2 ? fn2 : fn3;
+        float pv, kv;
+        if (nb >= 0) {
+            const size_t j = (size_t)nb * B3 + nbr_face_cell(f, u, v);
+            pv = a.phi_in[j];
+            kv = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
+        } else {
+            pv = step_face_ghost(a, (f == 0 ? fr0 : f == 1 ? fr1 : f == 2 ? fr2 : fr3) + u + 16 * v);
+            kv = a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
+        }
         int l;
         switch (f) {
         case 0: l = 0 + LW * (u + 1) + PL * (v - z0 + 1); break;
@@ -308,7 +346,7 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
     }
     __syncthreads();
 
-    const float ih2 = a.ih2_lvl[a.level[slot]];
+    const float ih2 = __int_as_float(rec[1]);
 #pragma unroll
     for (int k = 0; k < ZT; k++) {
         const int c = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
@@ -367,6 +405,26 @@ __global__ __launch_bounds__(256) void k_face_rows_abs(const int *__restrict__ w
     const int slot = work[i / 6], f = i % 6;
     const int fb = fbase[slot * 6 + f];
     frow[slot * 6 + f] = fb >= 0 ? sfbase[slot_sample[slot]] + fb : -1;
+}
+
+// Per-work-item records of the step (built with the tables).
+__global__ __launch_bounds__(256) void k_work_records(const int *__restrict__ work, int nwork,
+                                                      const int *__restrict__ fnbr, const int *__restrict__ frow,
+                                                      const int *__restrict__ level, const float *__restrict__ ih2_lvl,
+                                                      const int *__restrict__ slot_sample, int *__restrict__ wrec)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nwork) return;
+    const int slot = work[i];
+    int *r = wrec + (size_t)i * WREC;
+    r[0] = slot;
+    r[1] = __float_as_int(ih2_lvl[level[slot]]);
+    for (int f = 0; f < 6; f++) {
+        r[2 + f] = fnbr[slot * 6 + f];
+        r[8 + f] = frow[slot * 6 + f];
+    }
+    r[14] = slot_sample[slot];
+    r[15] = 0;
 }
 
 // 1/K of every face-halo cell and, when ikc is set, the per-cell step
@@ -454,13 +512,14 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
     __shared__ float s_ik[2][PLW * PLW];
     const int item = xcd_remap(blockIdx.x, gridDim.x);
     if (item >= a.nwork) return;
-    const int slot = a.work[item];
+    const int *rec = a.wrec + (size_t)item * WREC;
+    const int slot = rec[0];
     const int t = threadIdx.x;
     const int tx = t & 15, ty = t >> 4;
     const size_t base = (size_t)slot * B3;
     float Dw = a.Dw, Dg = a.Dg, rho = a.rho, dt = a.dt;
     if (a.sprm) {
-        const float *q = a.sprm + 4 * a.slot_sample[slot];
+        const float *q = a.sprm + 4 * rec[14];
         Dw = q[0];
         Dg = q[1];
         rho = q[2];
@@ -470,16 +529,21 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
     const int hf = t >> 4, hu = t & 15;
     int hnb = -1, hrow = 0;
     if (t < 64) {
-        hnb = a.fnbr[slot * 6 + hf];
-        if (hnb < 0) hrow = a.frow[slot * 6 + hf];
+        hnb = hf == 0 ? rec[2] : hf == 1 ? rec[3] : hf == 2 ? rec[4] : rec[5];
+        if (hnb < 0) hrow = hf == 0 ? rec[8] : hf == 1 ? rec[9] : hf == 2 ? rec[10] : rec[11];
     }
     // z-1 plane of this column
     float pB, kB;
     {
-        const int nb = a.fnbr[slot * 6 + 4];
-        if (nb >= 0) pB = a.phi_in[(size_t)nb * B3 + nbr_face_cell(4, tx, ty)];
-        else pB = step_face_ghost(a, a.frow[slot * 6 + 4] + t);
-        kB = a.fik[((size_t)slot * 6 + 4) * 256 + t];
+        const int nb = rec[6];
+        if (nb >= 0) {
+            const size_t j = (size_t)nb * B3 + nbr_face_cell(4, tx, ty);
+            pB = a.phi_in[j];
+            kB = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + 4) * 256 + t];
+        } else {
+            pB = step_face_ghost(a, rec[12] + t);
+            kB = a.fik[((size_t)slot * 6 + 4) * 256 + t];
+        }
     }
     // 1/K and p_w + p_g of plane z of this column, from the precomputed
     // coefficients (code = the loaded value) or from pw, pg (w, g)
@@ -500,7 +564,7 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
     float pF = a.phi_in[base + 256 + t];
     float wF = pre ? a.ikc[base + 256 + t] : a.pw[base + 256 + t];
     float gF = pre ? 0.f : a.pg[base + 256 + t];
-    const float ih2 = a.ih2_lvl[a.level[slot]];
+    const float ih2 = __int_as_float(rec[1]);
     for (int k = 0; k < B; k++) {
         const int buf = k & 1;
         float *sp = s_phi[buf];
@@ -509,10 +573,15 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         sp[c] = pC;
         sk[c] = kC;
         if (t < 64) {
-            float pv;
-            if (hnb >= 0) pv = a.phi_in[(size_t)hnb * B3 + nbr_face_cell(hf, hu, k)];
-            else pv = step_face_ghost(a, hrow + hu + 16 * k);
-            const float kv = a.fik[((size_t)slot * 6 + hf) * 256 + hu + 16 * k];
+            float pv, kv;
+            if (hnb >= 0) {
+                const size_t j = (size_t)hnb * B3 + nbr_face_cell(hf, hu, k);
+                pv = a.phi_in[j];
+                kv = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + hf) * 256 + hu + 16 * k];
+            } else {
+                pv = step_face_ghost(a, hrow + hu + 16 * k);
+                kv = a.fik[((size_t)slot * 6 + hf) * 256 + hu + 16 * k];
+            }
             int l;
             switch (hf) {
             case 0: l = 0 + PLW * (hu + 1); break;
@@ -527,10 +596,15 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         float kF, bF = 0.f;
         if (k + 1 < B) resolve(k + 1, pF, wF, gF, kF, bF);
         else {
-            const int nb = a.fnbr[slot * 6 + 5];
-            if (nb >= 0) pF = a.phi_in[(size_t)nb * B3 + nbr_face_cell(5, tx, ty)];
-            else pF = step_face_ghost(a, a.frow[slot * 6 + 5] + t);
-            kF = a.fik[((size_t)slot * 6 + 5) * 256 + t];
+            const int nb = rec[7];
+            if (nb >= 0) {
+                const size_t j = (size_t)nb * B3 + nbr_face_cell(5, tx, ty);
+                pF = a.phi_in[j];
+                kF = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + 5) * 256 + t];
+            } else {
+                pF = step_face_ghost(a, rec[13] + t);
+                kF = a.fik[((size_t)slot * 6 + 5) * 256 + t];
+            }
         }
         // prefetch plane k+2
         float pN = 0.f, wN = 0.f, gN = 0.f;
@@ -1472,6 +1546,15 @@ hipError_t launch_face_rows_abs(const int *work, int nwork, const int *fbase, co
     if (nwork <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_face_rows_abs, dim3((unsigned)((nwork * 6 + 255) / 256)), dim3(256), 0, st, work, nwork,
                        fbase, sfbase, slot_sample, frow);
+    return hipGetLastError();
+}
+
+hipError_t launch_work_records(const int *work, int nwork, const int *fnbr, const int *frow, const int *level,
+                               const float *ih2_lvl, const int *slot_sample, int *wrec, hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_work_records, dim3((unsigned)((nwork + 255) / 256)), dim3(256), 0, st, work, nwork, fnbr,
+                       frow, level, ih2_lvl, slot_sample, wrec);
     return hipGetLastError();
 }
 

@@ -28,7 +28,11 @@ struct StepArgs {
     float Dw, Dg, rho, dt;     // used when sprm is null
     const int *slot_sample;    // [slot] sample of the block (ensembles)
     const float *sprm;         // [sample*4] Dw, Dg, rho, dt (ensembles) or null
+    // [w*16] per work item: slot, 1/h^2 (float bits), fnbr[6], frow[6], sample, 0 -- one scalar load
+    // gives a workgroup everything the slot tables would give it through dependent look-ups
+    const int *wrec;
 };
+constexpr int WREC = 16;
 
 // Location tables for ghost evaluation on the device: the leaf covering a
 // point is found through a look-up over the finest level.
@@ -93,6 +97,8 @@ hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *s
                               int nrows, hipStream_t st);
 // sprm: per-sample Dw, Dg (ensembles) or null for (Dw, Dg)
 // frow[slot*6+f] = sfbase[sample(slot)] + fbase[slot*6+f] for the listed slots
+hipError_t launch_work_records(const int *work, int nwork, const int *fnbr, const int *frow, const int *level,
+                               const float *ih2_lvl, const int *slot_sample, int *wrec, hipStream_t st);
 hipError_t launch_face_rows_abs(const int *work, int nwork, const int *fbase, const int *sfbase,
                                 const int *slot_sample, int *frow, hipStream_t st);
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
