@@ -223,9 +223,11 @@ def mcmc_workload(args):
     random-walk Metropolis with one forward solve per proposal, on the same
     posterior of theta = (log Dw, log rho, x0, y0, z0, A) (uniform grid,
     flip-rate likelihood of synthetic observations).  Both samplers get the
-    same diagonal preconditioner from a pilot run and the same wall-time
-    budget; ESS min over parameters, for the chains the smaller of the summed
-    per-chain batch means and the chain-means estimate."""
+    same full-covariance preconditioner (the covariance of a batched pilot
+    population) and the same wall-time budget; ESS: min over parameters of
+    the multi-chain autocorrelation estimate (per-chain batch means saturate
+    at the number of batches when chains are correlated over more iterations
+    than a batch holds, and are reported beside it)."""
     import gliomauq_amd as gq
     from gliomauq_amd import anatomy as A
     from gliomauq_amd import sampler as SM
@@ -258,8 +260,18 @@ def mcmc_workload(args):
     log("[bench] scales %s" % scale)
     th0 = theta_true + 0.5 * scale * rng.standard_normal((C, 6))
     log("[bench] mcmc pilot")
-    pil, _, logeps = SM.mala(post_b, th0, args.mcmc_pilot, scale, rng, adapt_until=args.mcmc_pilot)
-    start = pil[-1]
+    pil, _, _ = SM.mala(post_b, th0, args.mcmc_pilot, scale, rng, adapt_until=args.mcmc_pilot)
+    # population preconditioner: covariance of a batched-RWM pilot over all chains (the posterior
+    # is narrow and correlated: log Dw with log rho, the IC position with the amplitude); every
+    # sampler, the plain one included, uses the same Cholesky factor
+    op, _, _ = SM.rwm_batched(post_b, pil[-1], 100, scale, rng, adapt_until=100)
+    chol = SM.population_cholesky(op[50:])
+    start = op[-1]
+    log("[bench] mcmc step-size adaptation")
+    oa, _, logeps = SM.mala(post_b, start, 30, chol, rng, adapt_until=30)
+    start_m = oa[-1]
+    ob, _, lfb = SM.rwm_batched(post_b, start, 60, chol, rng, adapt_until=60)
+    start_b = ob[-1]
 
     def timed_loop(step, state):
         xs, acs = [], []
@@ -271,27 +283,26 @@ def mcmc_workload(args):
     log("[bench] mcmc MALA %d chains, %.0f s" % (C, args.mcmc_budget))
 
     def step_mala(st, xs, acs):
-        o, a, le = SM.mala(post_b, st[0], 10, scale, rng, logeps=st[1])  # adapted step sizes kept
+        o, a, le = SM.mala(post_b, st[0], 10, chol, rng, logeps=st[1])  # adapted step sizes kept
         xs.append(o)
         acs.append(a)
         return (o[-1], le)
-    xm, am, wall_m = timed_loop(step_mala, (start, logeps))
+    xm, am, wall_m = timed_loop(step_mala, (start_m, logeps))
 
     log("[bench] mcmc batched RWM %d chains, %.0f s" % (C, args.mcmc_budget))
-    ob, _, lfb = SM.rwm_batched(post_b, start, 100, scale, rng, adapt_until=100)
 
     def step_rwmb(st, xs, acs):
-        o, a, lf = SM.rwm_batched(post_b, st[0], 10, scale, rng, logf=st[1])
+        o, a, lf = SM.rwm_batched(post_b, st[0], 10, chol, rng, logf=st[1])
         xs.append(o)
         acs.append(a)
         return (o[-1], lf)
-    xb, ab, wall_b = timed_loop(step_rwmb, (ob[-1], lfb))
+    xb, ab, wall_b = timed_loop(step_rwmb, (start_b, lfb))
 
     log("[bench] mcmc RWM 1 chain, %.0f s" % args.mcmc_budget)
-    out_r, acc_r, logf = SM.rwm(post_1, start[0], 200, scale, rng, adapt_until=200)
+    out_r, acc_r, logf = SM.rwm(post_1, start[0], 200, chol, rng, adapt_until=200)
 
     def step_rwm(st, xs, acs):
-        o, a, lf = SM.rwm(post_1, st[0], 50, scale, rng, logf=st[1])
+        o, a, lf = SM.rwm(post_1, st[0], 50, chol, rng, logf=st[1])
         xs.append(o[:, None, :])
         acs.append(a[:, None])
         return (o[-1], lf)
@@ -299,10 +310,12 @@ def mcmc_workload(args):
 
     rates = {}
     for name, x, a, w in (("rwm_plain", xr, ar, wall_r), ("rwm_%d" % C, xb, ab, wall_b), ("mala_%d" % C, xm, am, wall_m)):
-        e = min(SM.ess_parallel(x[:, :, p]) for p in range(6))
+        e = min(SM.ess_autocorr(x[:, :, p]) for p in range(6))
         e_bm = min(sum(SM.ess_batch_means(x[:, k, p]) for k in range(x.shape[1])) for p in range(6))
+        e_cm = min(SM.ess_chains(x[:, :, p]) for p in range(6)) if x.shape[1] > 1 else None
         rates[name] = {"ess_per_hour": e / w * 3600, "ess": e, "ess_batch_means_sum": e_bm,
-                       "ess_per_hour_batch_means_sum": e_bm / w * 3600, "chains": int(x.shape[1]),
+                       "ess_per_hour_batch_means_sum": e_bm / w * 3600, "ess_chain_means": e_cm,
+                       "chains": int(x.shape[1]),
                        "iterations": int(x.shape[0]), "wall_s": w, "accept": float(a.mean()),
                        "posterior_mean": x.reshape(-1, 6).mean(0).tolist()}
     ref_mean = np.array(rates["rwm_plain"]["posterior_mean"])
@@ -324,10 +337,11 @@ def mcmc_workload(args):
                                "from the discrete adjoint) vs plain RWM (one forward solve per proposal), uniform "
                                "%d^3, %d steps, flip-rate likelihood, 6 parameters, %.0f s per sampler"
                                % (C, n, steps, args.mcmc_budget), "seed": args.seed, "best": best,
-                   "ess": "min over parameters; chains: the smaller of the summed per-chain batch means and "
-                          "C var / var(chain means)"},
+                   "ess": "min over parameters of the multi-chain autocorrelation ESS (Vehtari et al. 2021, "
+                          "Geyer initial monotone sequence); batch-means sums and chain-means estimates beside it"},
         "mcmc": rates,
-        "theta_true": theta_true.tolist(), "preconditioner": scale.tolist(),
+        "theta_true": theta_true.tolist(), "probe_scale": scale.tolist(),
+        "preconditioner_sd": np.sqrt(np.diag(chol @ chol.T)).tolist(),
     }
     print(json.dumps(res), flush=True)
 
@@ -505,7 +519,7 @@ def surrogate_workload(args):
     x_m, a_m, w_m, _ = run_loop(step_mala, (o[-1], le))
 
     def ess(x):
-        return min(SM.ess_parallel(x[:, :, p]) for p in range(6))
+        return min(SM.ess_autocorr(x[:, :, p]) for p in range(6))
 
     rates = {}
     for name, x, a, w in (("rwm_plain", x_r, a_r, w_r), ("rwm_16", x_b, a_b, w_b), ("da_rwm_16", x_d, a_d, w_d),

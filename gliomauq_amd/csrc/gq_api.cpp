@@ -65,13 +65,39 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Page-locked host staging buffer (hipHostMalloc): copies from and to it are
+// plain DMA transfers, with no staging through a driver buffer.  Growing it
+// synchronises the stream first, so a transfer in flight never loses its source.
+template <typename T>
+struct PinnedBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void reserve(size_t m, hipStream_t st)
+    {
+        if (m <= n) return;
+        if (p) {
+            HIPCHK(hipStreamSynchronize(st));
+            (void)hipHostFree(p);
+            p = nullptr;
+            n = 0;
+        }
+        const size_t cap = std::max<size_t>(m, 4096);
+        HIPCHK(hipHostMalloc((void **)&p, cap * sizeof(T), hipHostMallocDefault));
+        n = cap;
+    }
+    ~PinnedBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 // Host int arrays copied to the device in one transfer: the parts lie back to
-// back (each at a 64-int boundary) in one device buffer.  The host copy stays
-// alive in the object; callers synchronise the stream before the next upload
-// through the same object.
+// back (each at a 64-int boundary) in one device buffer, staged in pinned
+// memory.  Callers synchronise the stream before the next upload through the
+// same object (the staging buffer is rewritten then).
 struct PackedInts {
     DevBuf<int> dev;
-    std::vector<int> host;
+    PinnedBuf<int> host;
     std::vector<size_t> off;
     void upload(std::initializer_list<const std::vector<int> *> parts, hipStream_t st)
     {
@@ -81,11 +107,11 @@ struct PackedInts {
             off.push_back(total);
             total += (v->size() + 63) & ~(size_t)63;
         }
-        host.resize(std::max<size_t>(total, 1));
+        host.reserve(std::max<size_t>(total, 1), st);
         size_t k = 0;
-        for (const std::vector<int> *v : parts) std::copy(v->begin(), v->end(), host.begin() + off[k++]);
-        dev.reserve(host.size());
-        if (total) HIPCHK(hipMemcpyAsync(dev.p, host.data(), total * sizeof(int), hipMemcpyHostToDevice, st));
+        for (const std::vector<int> *v : parts) std::copy(v->begin(), v->end(), host.p + off[k++]);
+        dev.reserve(std::max<size_t>(total, 1));
+        if (total) HIPCHK(hipMemcpyAsync(dev.p, host.p, total * sizeof(int), hipMemcpyHostToDevice, st));
     }
     int *ptr(int i) const { return dev.p + off[i]; }
 };
@@ -147,6 +173,8 @@ struct gq_ctx {
     PackedInts tabs;
     int *p_work = nullptr, *p_sfbase = nullptr, *p_lut = nullptr, *p_lutf = nullptr, *p_faces = nullptr;
     PackedInts lists;  // slot lists of one adaptation kernel sequence
+    PinnedBuf<double> h_det;   // FWT details read back
+    PinnedBuf<int64_t> h_total;  // face-row term count read back
     DevBuf<int64_t> d_face_ptr;
     DevBuf<char> d_scan_tmp;
     int face_nrows = 0;
@@ -477,9 +505,10 @@ struct gq_ctx {
             HIPCHK(gq::scan_counts(d_counts.p, d_face_ptr.p, face_nrows, nullptr, tb, st));
             d_scan_tmp.reserve(tb + 1);
             HIPCHK(gq::scan_counts(d_counts.p, d_face_ptr.p, face_nrows, d_scan_tmp.p, tb, st));
-            int64_t total = 0;
-            HIPCHK(hipMemcpyAsync(&total, d_face_ptr.p + face_nrows, sizeof(total), hipMemcpyDeviceToHost, st));
+            h_total.reserve(1, st);
+            HIPCHK(hipMemcpyAsync(h_total.p, d_face_ptr.p + face_nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
+            const int64_t total = *h_total.p;
             if (total > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "face rows exceed 2^31 terms");
             d_face_src.reserve((size_t)std::max<int64_t>(1, total));
             d_face_w.reserve((size_t)std::max<int64_t>(1, total));
@@ -556,8 +585,10 @@ struct gq_ctx {
         HIPCHK(gq::launch_lab_ghosts(la, fl, 1, lists.ptr(0), lists.ptr(1), (int)items.size(), d_ghost.p, 0, false,
                                      st));
         HIPCHK(gq::launch_fwt_detail(order, la, d_ghost.p, phi[cur], lists.ptr(0), (int)slots.size(), d_dbl.p, st));
-        HIPCHK(hipMemcpyAsync(out.data(), d_dbl.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        h_det.reserve(out.size(), st);
+        HIPCHK(hipMemcpyAsync(h_det.p, d_dbl.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        std::copy(h_det.p, h_det.p + out.size(), out.begin());
         return out;
     }
 
@@ -901,9 +932,13 @@ struct gq_ctx {
                 a.phi_in = phi[c];
                 a.phi_out = phi[c ^ 1];
                 a.nwork = nwork[j];
-                if (pk) HIPCHK(hipEventRecord(events[2 * j], st));
-                if (nwork[j] > 0) HIPCHK(gq::launch_fk_step(a, zt, st));
-                if (pk) HIPCHK(hipEventRecord(events[2 * j + 1], st));
+                if (nwork[j] > 0) {
+                    if (pk) HIPCHK(gq::launch_fk_step(a, zt, st, events[2 * j], events[2 * j + 1]));
+                    else HIPCHK(gq::launch_fk_step(a, zt, st));
+                } else if (pk) {
+                    HIPCHK(hipEventRecord(events[2 * j], st));
+                    HIPCHK(hipEventRecord(events[2 * j + 1], st));
+                }
                 c ^= 1;
             }
         };

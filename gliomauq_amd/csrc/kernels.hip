@@ -6,6 +6,7 @@
 // same-level neighbour of each face/edge/corner region and, for regions
 // covered by a coarser or finer leaf, the first row of a CSR table of
 // (source cell, weight) terms.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -1512,21 +1513,22 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *partial, int
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st)
+hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     if (a.nwork <= 0) return hipSuccess;
+    // with events: timestamps taken by the dispatch itself (no separate event packets)
     if (zt == 1) {
         const int grid = (a.nwork + 7) & ~7;
-        hipLaunchKernelGGL(k_fk_step_col, dim3(grid), dim3(256), 0, st, a);
+        hipExtLaunchKernelGGL(k_fk_step_col, dim3(grid), dim3(256), 0, st, e0, e1, 0, a);
         return hipGetLastError();
     }
     const int items = a.nwork * (16 / zt);
     const int grid = (items + 7) & ~7;
     switch (zt) {
-    case 16: hipLaunchKernelGGL(k_fk_step<16>, dim3(grid), dim3(256), 0, st, a); break;
-    case 8: hipLaunchKernelGGL(k_fk_step<8>, dim3(grid), dim3(256), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_fk_step<4>, dim3(grid), dim3(256), 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_fk_step<2>, dim3(grid), dim3(256), 0, st, a); break;
+    case 16: hipExtLaunchKernelGGL(k_fk_step<16>, dim3(grid), dim3(256), 0, st, e0, e1, 0, a); break;
+    case 8: hipExtLaunchKernelGGL(k_fk_step<8>, dim3(grid), dim3(256), 0, st, e0, e1, 0, a); break;
+    case 4: hipExtLaunchKernelGGL(k_fk_step<4>, dim3(grid), dim3(256), 0, st, e0, e1, 0, a); break;
+    case 2: hipExtLaunchKernelGGL(k_fk_step<2>, dim3(grid), dim3(256), 0, st, e0, e1, 0, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -92,7 +92,9 @@ hipError_t launch_ic_grad(const float *lam0, const int *work, int nwork, const i
 
 // zt: slab depth (16, 8, 4 or 2 planes per workgroup), or 1: one workgroup per
 // block streaming its planes through registers (k_fk_step_col)
-hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st);
+// e0/e1 (optional): events the dispatch itself records at the kernel's start and end
+hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st, hipEvent_t e0 = nullptr,
+                          hipEvent_t e1 = nullptr);
 hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *src, const float *w, float *out,
                               int nrows, hipStream_t st);
 // sprm: per-sample Dw, Dg (ensembles) or null for (Dw, Dg)

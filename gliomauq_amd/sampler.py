@@ -15,7 +15,10 @@ evaluated for many parameter sets per GPU call:
                      adapted towards an acceptance of 0.57 during burn-in
   tmcmc              transitional MCMC (tempered stages, resampling and
                      per-stage Metropolis moves on the whole population)
+  ess_autocorr       effective sample size of one or many chains from their
+                     autocorrelations (the estimator the benchmarks report)
   ess_batch_means    effective sample size by batch means
+  ess_chains         effective sample size from the spread of chain means
 """
 import math
 import time
@@ -130,6 +133,29 @@ class SurrogatePosterior:
         return np.where(inside, ll[:, 0], -np.inf)
 
 
+def _precond(scale):
+    """Preconditioner helpers for a vector of standard deviations (diagonal)
+    or a lower-triangular Cholesky factor L of a covariance M = L L^T:
+    (noise z -> L z, gradient g -> M g, quadratic form v -> v^T M^-1 v),
+    each applied along the last axis."""
+    s = np.asarray(scale, np.float64)
+    if s.ndim == 1:
+        m = s * s
+        return (lambda z: z * s), (lambda g: g * m), (lambda v: np.sum(v * v / m, -1))
+    mm = s @ s.T
+    li = np.linalg.inv(s)
+    return (lambda z: z @ s.T), (lambda g: g @ mm), (lambda v: np.sum((v @ li.T) ** 2, -1))
+
+
+def population_cholesky(samples, ridge=1e-12):
+    """Cholesky factor of the covariance of pooled samples (n, 6) (a small
+    relative ridge keeps it positive definite)."""
+    x = np.asarray(samples, np.float64).reshape(-1, 6)
+    c = np.cov(x, rowvar=False)
+    c += ridge * np.diag(np.diag(c)) + 1e-300 * np.eye(6)
+    return np.linalg.cholesky(c)
+
+
 def da_rwm(post, spost, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, logf=None, callback=None,
            diag=None):
     """Delayed-acceptance random-walk Metropolis for a population of chains:
@@ -149,8 +175,9 @@ def da_rwm(post, spost, theta0, n_iter, scale, rng, adapt_until=0, target=0.234,
     out = np.zeros((n_iter, Cn, 6))
     acc = np.zeros((n_iter, Cn), bool)
     n_full = 0
+    noise = _precond(scale)[0]
     for it in range(n_iter):
-        prop = th + np.exp(logf)[:, None] * scale * rng.standard_normal((Cn, 6))
+        prop = th + np.exp(logf)[:, None] * noise(rng.standard_normal((Cn, 6)))
         lsp = spost.logpost(prop)
         a1 = np.log(rng.random(Cn)) < np.where(np.isfinite(lsp), lsp - ls, -np.inf)
         idx = np.nonzero(a1)[0]
@@ -185,8 +212,9 @@ def rwm_batched(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, l
     logf = np.zeros(Cn) if logf is None else np.array(logf, np.float64)
     out = np.zeros((n_iter, Cn, 6))
     acc = np.zeros((n_iter, Cn), bool)
+    noise = _precond(scale)[0]
     for it in range(n_iter):
-        prop = th + np.exp(logf)[:, None] * scale * rng.standard_normal((Cn, 6))
+        prop = th + np.exp(logf)[:, None] * noise(rng.standard_normal((Cn, 6)))
         lpp = _batched(post.logpost, prop, post.batch)
         a = np.log(rng.random(Cn)) < np.where(np.isfinite(lpp), lpp - lp, -np.inf)
         th[a] = prop[a]
@@ -223,15 +251,17 @@ def probe_scales(post, theta, drop=2.0, lo=1e-7, hi=1.0, n=16):
 
 def rwm(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, callback=None, logf=0.0):
     """Random-walk Metropolis, one chain, one forward solve per proposal.
-    Gaussian proposals N(0, diag((f s)^2)); log f starts at `logf` and adapts
+    Gaussian proposals N(0, f^2 M) with M = diag(s^2) for a vector `scale`
+    or M = L L^T for a Cholesky factor L; log f starts at `logf` and adapts
     towards `target` acceptance for the first `adapt_until` iterations.
     Returns (samples, accepted, final log f)."""
     th = np.asarray(theta0, np.float64).copy()
     lp = post.logpost(th[None])[0]
     out = np.zeros((n_iter, 6))
     acc = np.zeros(n_iter, bool)
+    noise = _precond(scale)[0]
     for it in range(n_iter):
-        prop = th + np.exp(logf) * scale * rng.standard_normal(6)
+        prop = th + np.exp(logf) * noise(rng.standard_normal(6))
         lpp = post.logpost(prop[None])[0]
         if np.log(rng.random()) < lpp - lp:
             th, lp = prop, lpp
@@ -246,24 +276,25 @@ def rwm(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, callback=
 
 def mala(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.57, callback=None, logeps=None):
     """Metropolis-adjusted Langevin for a population of chains (theta0: (C, 6)),
-    preconditioner diag(scale^2), per-chain step sizes exp(logeps) (default
+    preconditioner M = diag(scale^2) (vector) or L L^T (Cholesky factor L),
+    per-chain step sizes exp(logeps) (default
     0.5) adapted towards `target` acceptance for the first `adapt_until`
     iterations.  Returns (samples, accepted, final logeps)."""
     th = np.asarray(theta0, np.float64).copy()
     C = th.shape[0]
-    M = np.asarray(scale, np.float64) ** 2
+    noise, mgrad, quad = _precond(scale)
     lp, g = post.logpost_grad(th)
     logeps = np.full(C, math.log(0.5)) if logeps is None else np.array(logeps, np.float64)
     out = np.zeros((n_iter, C, 6))
     acc = np.zeros((n_iter, C), bool)
     for it in range(n_iter):
         eps = np.exp(logeps)[:, None]
-        mean_f = th + 0.5 * eps ** 2 * M * g
-        prop = mean_f + eps * np.sqrt(M) * rng.standard_normal((C, 6))
+        mean_f = th + 0.5 * eps ** 2 * mgrad(g)
+        prop = mean_f + eps * noise(rng.standard_normal((C, 6)))
         lpp, gp = post.logpost_grad(prop)
-        mean_b = prop + 0.5 * eps ** 2 * M * gp
-        lq_f = -np.sum((prop - mean_f) ** 2 / M, 1) / (2 * eps[:, 0] ** 2)
-        lq_b = -np.sum((th - mean_b) ** 2 / M, 1) / (2 * eps[:, 0] ** 2)
+        mean_b = prop + 0.5 * eps ** 2 * mgrad(gp)
+        lq_f = -quad(prop - mean_f) / (2 * eps[:, 0] ** 2)
+        lq_b = -quad(th - mean_b) / (2 * eps[:, 0] ** 2)
         la = lpp - lp + lq_b - lq_f
         a = np.log(rng.random(C)) < np.where(np.isfinite(la), la, -np.inf)
         th = np.where(a[:, None], prop, th)
@@ -309,6 +340,45 @@ def ess_chains(x):
         return 1.0
     vm = x.mean(0).var(ddof=1)
     return float(min(n * c, c * var / vm)) if vm > 0 else float(n * c)
+
+
+def ess_autocorr(x):
+    """ESS of one or more chains x[iteration, chain] (one parameter) from the
+    chains' autocorrelations: the multi-chain estimator of Vehtari, Gelman,
+    Simpson, Carpenter and Buerkner (2021, without rank normalisation) --
+    rho_t = 1 - (W - mean_c acov_c(t)) / var+, var+ = (n-1)/n W + B/n --
+    summed with Geyer's initial positive and monotone sequence; n C / tau.
+    Unlike batch means it does not saturate at the number of batches when
+    the chains are correlated over more iterations than a batch holds."""
+    x = np.asarray(x, np.float64)
+    if x.ndim == 1:
+        x = x[:, None]
+    n, c = x.shape
+    if n < 4:
+        return float(n * c)
+    xc = x - x.mean(0)
+    m = 1
+    while m < 2 * n:
+        m *= 2
+    f = np.fft.rfft(xc, m, axis=0)
+    acov = np.fft.irfft(f * np.conj(f), m, axis=0)[:n] / n  # biased autocovariance per chain
+    w = acov[0].mean() * n / (n - 1.0)  # mean within-chain variance
+    b_n = x.mean(0).var(ddof=1) if c > 1 else 0.0
+    var_plus = (n - 1.0) / n * w + b_n
+    if var_plus <= 0:
+        return 1.0
+    rho = 1.0 - (w - acov.mean(1)) / var_plus
+    rho[0] = 1.0
+    tau = -1.0
+    prev = np.inf
+    for k in range(0, n - 1, 2):
+        pk = rho[k] + rho[k + 1]
+        if pk <= 0:
+            break
+        pk = min(pk, prev)  # monotone
+        tau += 2.0 * pk
+        prev = pk
+    return float(min(n * c * math.log10(n * c), n * c / max(tau, 1e-12)))
 
 
 def ess_parallel(x):

@@ -117,3 +117,71 @@ def test_ess_chains_ar1():
     # chains that still remember dispersed starts: lower estimate
     xd = _ar1_chains(rng, 200, 100, 0.9, x0=np.linspace(-20, 20, 100))
     assert SM.ess_chains(xd) < 0.9 * 200 * 100 * 0.1 / 1.9
+
+
+class CorrGauss:
+    """Correlated Gaussian: log p = -0.5 (x - mu)^T P (x - mu)."""
+
+    def __init__(self, mu, cov, batch=64):
+        self.mu = np.asarray(mu, float)
+        self.cov = np.asarray(cov, float)
+        self.P = np.linalg.inv(self.cov)
+        self.batch = batch
+        self.lo = np.full(6, -50.0)
+        self.hi = np.full(6, 50.0)
+
+    def logpost(self, th):
+        d = np.atleast_2d(th) - self.mu
+        return -0.5 * np.einsum("ij,jk,ik->i", d, self.P, d)
+
+    def logpost_grad(self, th):
+        d = np.atleast_2d(th) - self.mu
+        return self.logpost(th), -d @ self.P
+
+
+def _corr_cov():
+    rng = np.random.default_rng(9)
+    a = rng.standard_normal((6, 6))
+    q, _ = np.linalg.qr(a)
+    return q @ np.diag([1e-4, 1e-2, 0.3, 1.0, 4.0, 9.0]) @ q.T  # condition number 9e4
+
+
+def test_full_covariance_preconditioner():
+    """With L = chol(cov of a pilot population), MALA and batched RWM recover
+    the mean and covariance of a badly conditioned correlated Gaussian; the
+    proposal density of MALA uses M^-1 = (L L^T)^-1 (a diagonal stand-in
+    would bias the covariance)."""
+    cov = _corr_cov()
+    mu = MU
+    post = CorrGauss(mu, cov)
+    rng = np.random.default_rng(3)
+    pilot = mu + rng.multivariate_normal(np.zeros(6), cov, 400)
+    L = SM.population_cholesky(pilot)
+    th0 = mu + rng.multivariate_normal(np.zeros(6), cov, 64)
+    for name in ("mala", "rwm"):
+        if name == "mala":
+            out, acc, _ = SM.mala(post, th0, 1500, L, rng, adapt_until=300)
+        else:
+            out, acc, _ = SM.rwm_batched(post, th0, 3000, L, rng, adapt_until=600)
+        burn = len(out) // 5
+        x = out[burn:].reshape(-1, 6)
+        emp = np.cov(x, rowvar=False)
+        # relative error of the covariance in the metric of the target
+        w = np.linalg.cholesky(np.linalg.inv(cov))
+        err = np.abs(w.T @ emp @ w - np.eye(6)).max()
+        assert err < 0.12, (name, err)
+        assert np.abs(np.linalg.solve(np.linalg.cholesky(cov), x.mean(0) - mu)).max() < 0.1
+
+
+def test_ess_autocorr_ar1():
+    """Autocorrelation ESS against the exact AR(1) value n C (1 - phi)/(1 + phi):
+    a long single chain, many chains, and a strongly correlated chain where
+    batch means saturate near the number of batches (sqrt(n))."""
+    rng = np.random.default_rng(8)
+    for n, c, phi, tol in [(100000, 1, 0.9, 0.1), (1000, 200, 0.8, 0.1), (20000, 1, 0.99, 0.3)]:
+        x = _ar1_chains(rng, n, c, phi)
+        exact = n * c * (1 - phi) / (1 + phi)
+        assert abs(SM.ess_autocorr(x) / exact - 1) < tol, (n, c, phi)
+    x = _ar1_chains(rng, 20000, 1, 0.995)
+    assert SM.ess_batch_means(x[:, 0]) > 2.5 * 20000 * 0.005 / 1.995
+    assert SM.ess_autocorr(x) < 2.0 * 20000 * 0.005 / 1.995
