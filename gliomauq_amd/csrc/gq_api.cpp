@@ -9,6 +9,7 @@
 #include <exception>
 #include <initializer_list>
 #include <chrono>
+#include <climits>
 #include <map>
 #include <cmath>
 #include <cstring>
@@ -179,6 +180,7 @@ struct gq_ctx {
     DevBuf<char> d_scan_tmp;
     int face_nrows = 0;
     bool face_rows_valid = false;  // device face-ghost rows (ptr/src/w) match the tables
+    int face_rows_npos = 0;        // ... for the samples at the first face_rows_npos positions
     DevBuf<double> d_ghost;  // cross-level lab points (3 fields for a split)
     DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2, d_sprm;
     DevBuf<float> d_ikc;          // per-cell 1/K codes of the step (pool layout)
@@ -490,23 +492,30 @@ struct gq_ctx {
     // Face-ghost rows of the step (ptr/src/w of every cross-level face-halo
     // cell), enumerated on the device.  Only stepping needs them, so a table
     // rebuild between a refinement and the compression that follows does not.
-    void ensure_face_rows()
+    //
+    // npos: only the samples at the first npos positions of the work order
+    // (the ones still stepping: a prefix) need rows; the faces of the others
+    // get none until a later call asks for them.
+    void ensure_face_rows(int npos = INT_MAX)
     {
         ensure_tables();
-        if (face_rows_valid) return;
+        const int S = nsamples();
+        npos = std::max(0, std::min(npos, S));
+        if (face_rows_valid && face_rows_npos >= npos) return;
         ScopeTimer tm(t_tables);
-        const int nfaces = face_nrows / 256;
-        d_face_ptr.reserve((size_t)face_nrows + 1);
+        const int nfaces = face_begin[npos];
+        const int nrows = nfaces * 256;
+        d_face_ptr.reserve((size_t)nrows + 1);
         if (nfaces > 0) {
             const gq::LabArgs la = lab_args();
-            d_counts.reserve((size_t)face_nrows);
+            d_counts.reserve((size_t)nrows);
             HIPCHK(gq::launch_face_rows_count(la, p_faces, nfaces, d_counts.p, st));
             size_t tb = 0;
-            HIPCHK(gq::scan_counts(d_counts.p, d_face_ptr.p, face_nrows, nullptr, tb, st));
+            HIPCHK(gq::scan_counts(d_counts.p, d_face_ptr.p, nrows, nullptr, tb, st));
             d_scan_tmp.reserve(tb + 1);
-            HIPCHK(gq::scan_counts(d_counts.p, d_face_ptr.p, face_nrows, d_scan_tmp.p, tb, st));
+            HIPCHK(gq::scan_counts(d_counts.p, d_face_ptr.p, nrows, d_scan_tmp.p, tb, st));
             h_total.reserve(1, st);
-            HIPCHK(hipMemcpyAsync(h_total.p, d_face_ptr.p + face_nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(h_total.p, d_face_ptr.p + nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             const int64_t total = *h_total.p;
             if (total > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "face rows exceed 2^31 terms");
@@ -516,6 +525,7 @@ struct gq_ctx {
         }
         d_gphi.reserve((size_t)std::max(1, face_nrows));
         face_rows_valid = true;
+        face_rows_npos = npos;
     }
 
     gq::LabArgs lab_args() const
@@ -533,26 +543,29 @@ struct gq_ctx {
 
     // 1/K of the face halos for (Dw, Dg), or for the per-sample parameters in d_sprm (per_sample).  Only the
     // slots of samples whose topology changed are recomputed unless the parameters or the anatomy changed.
-    void ensure_fik(const std::vector<float> &key, bool per_sample)
+    // npos: as for ensure_face_rows -- the samples at later positions keep
+    // their stale mark until a call covers them.
+    void ensure_fik(const std::vector<float> &key, bool per_sample, int npos = INT_MAX)
     {
-        ensure_face_rows();
+        ensure_face_rows(npos);
         const int S = nsamples();
+        const int P = face_rows_npos;  // samples whose face rows exist
         if (!fik_valid || key != fik_key) {
             fik_stale.assign(S, 1);
             fik_valid = true;
             fik_key = key;
         }
-        bool all = true, any = false;
-        for (int s = 0; s < S; s++) {
-            all = all && fik_stale[s];
-            any = any || fik_stale[s];
+        bool all = P == S, any = false;
+        for (int pos = 0; pos < P; pos++) {
+            all = all && fik_stale[sample_order[pos]];
+            any = any || fik_stale[sample_order[pos]];
         }
         if (!any) return;
         const int *list = p_work;
         int n = (int)work.size();
         if (!all) {
             std::vector<int> sl;
-            for (int pos = 0; pos < S; pos++)
+            for (int pos = 0; pos < P; pos++)
                 if (fik_stale[sample_order[pos]])
                     sl.insert(sl.end(), work.begin() + work_begin[pos], work.begin() + work_begin[pos + 1]);
             d_list.upload(sl, st);
@@ -563,7 +576,7 @@ struct gq_ctx {
         HIPCHK(gq::launch_face_prep(pw, pg, list, n, d_fnbr.p, d_fbase.p, p_sfbase, d_slot_sample.p, d_face_ptr.p,
                                     d_face_src.p, d_face_w.p, d_fik.p, d_ikc.p, key[0], key[1],
                                     per_sample ? d_sprm.p : nullptr, st));
-        fik_stale.assign(S, 0);
+        for (int pos = 0; pos < P; pos++) fik_stale[sample_order[pos]] = 0;
     }
 
     // FWT detail of each listed slot (any samples)
@@ -1344,7 +1357,9 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
             const int mx = *std::max_element(rem.begin(), rem.end());
             if (mx <= 0) break;
             const int m = std::min(K, mx);
-            ctx->ensure_fik(key, true);
+            int nrun = 0;  // samples still stepping: a prefix of the work order
+            while (nrun < S && rem[ctx->sample_order[nrun]] > 0) nrun++;
+            ctx->ensure_fik(key, true, nrun);
             // per-step prefixes
             std::vector<int> nw(m), nr(m);
             for (int j = 0; j < m; j++) {

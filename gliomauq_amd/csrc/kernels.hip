@@ -305,15 +305,11 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
         } else {
             const int f = side == 0 ? 4 : 5;
             const int nb = rec[2 + f];
-            if (nb >= 0) {
-                const size_t j = (size_t)nb * B3 + nbr_face_cell(f, tx, ty);
-                pv = a.phi_in[j];
-                // same level: the neighbour's own 1/K code (its block is being read anyway)
-                kv = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + f) * 256 + t];
-            } else {
-                pv = step_face_ghost(a, rec[8 + f] + t);
-                kv = a.fik[((size_t)slot * 6 + f) * 256 + t];
-            }
+            if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, tx, ty)];
+            else pv = step_face_ghost(a, rec[8 + f] + t);
+            // the face table, contiguous per face (measured faster than the
+            // neighbour's own 1/K codes, which are strided across its rows)
+            kv = a.fik[((size_t)slot * 6 + f) * 256 + t];
         }
         s_phi[l] = pv;
         s_ik[l] = kv;
@@ -326,15 +322,10 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
         const int r = q - f * 16 * ZT;
         const int u = r & 15, v = z0 + (r >> 4);
         const int nb = f == 0 ? fn0 : f == 1 ? fn1 : f == 2 ? fn2 : fn3;
-        float pv, kv;
-        if (nb >= 0) {
-            const size_t j = (size_t)nb * B3 + nbr_face_cell(f, u, v);
-            pv = a.phi_in[j];
-            kv = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
-        } else {
-            pv = step_face_ghost(a, (f == 0 ? fr0 : f == 1 ? fr1 : f == 2 ? fr2 : fr3) + u + 16 * v);
-            kv = a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
-        }
+        float pv;
+        if (nb >= 0) pv = a.phi_in[(size_t)nb * B3 + nbr_face_cell(f, u, v)];
+        else pv = step_face_ghost(a, (f == 0 ? fr0 : f == 1 ? fr1 : f == 2 ? fr2 : fr3) + u + 16 * v);
+        const float kv = a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
         int l;
         switch (f) {
         case 0: l = 0 + LW * (u + 1) + PL * (v - z0 + 1); break;
@@ -537,14 +528,9 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
     float pB, kB;
     {
         const int nb = rec[6];
-        if (nb >= 0) {
-            const size_t j = (size_t)nb * B3 + nbr_face_cell(4, tx, ty);
-            pB = a.phi_in[j];
-            kB = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + 4) * 256 + t];
-        } else {
-            pB = step_face_ghost(a, rec[12] + t);
-            kB = a.fik[((size_t)slot * 6 + 4) * 256 + t];
-        }
+        if (nb >= 0) pB = a.phi_in[(size_t)nb * B3 + nbr_face_cell(4, tx, ty)];
+        else pB = step_face_ghost(a, rec[12] + t);
+        kB = a.fik[((size_t)slot * 6 + 4) * 256 + t];
     }
     // 1/K and p_w + p_g of plane z of this column, from the precomputed
     // coefficients (code = the loaded value) or from pw, pg (w, g)
@@ -574,15 +560,10 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         sp[c] = pC;
         sk[c] = kC;
         if (t < 64) {
-            float pv, kv;
-            if (hnb >= 0) {
-                const size_t j = (size_t)hnb * B3 + nbr_face_cell(hf, hu, k);
-                pv = a.phi_in[j];
-                kv = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + hf) * 256 + hu + 16 * k];
-            } else {
-                pv = step_face_ghost(a, hrow + hu + 16 * k);
-                kv = a.fik[((size_t)slot * 6 + hf) * 256 + hu + 16 * k];
-            }
+            float pv;
+            if (hnb >= 0) pv = a.phi_in[(size_t)hnb * B3 + nbr_face_cell(hf, hu, k)];
+            else pv = step_face_ghost(a, hrow + hu + 16 * k);
+            const float kv = a.fik[((size_t)slot * 6 + hf) * 256 + hu + 16 * k];
             int l;
             switch (hf) {
             case 0: l = 0 + PLW * (hu + 1); break;
@@ -598,14 +579,9 @@ __global__ __launch_bounds__(256) void k_fk_step_col(StepArgs a)
         if (k + 1 < B) resolve(k + 1, pF, wF, gF, kF, bF);
         else {
             const int nb = rec[7];
-            if (nb >= 0) {
-                const size_t j = (size_t)nb * B3 + nbr_face_cell(5, tx, ty);
-                pF = a.phi_in[j];
-                kF = a.ikc ? ik_of(a.ikc[j], a.pw, a.pg, j, Dw, Dg) : a.fik[((size_t)slot * 6 + 5) * 256 + t];
-            } else {
-                pF = step_face_ghost(a, rec[13] + t);
-                kF = a.fik[((size_t)slot * 6 + 5) * 256 + t];
-            }
+            if (nb >= 0) pF = a.phi_in[(size_t)nb * B3 + nbr_face_cell(5, tx, ty)];
+            else pF = step_face_ghost(a, rec[13] + t);
+            kF = a.fik[((size_t)slot * 6 + 5) * 256 + t];
         }
         // prefetch plane k+2
         float pN = 0.f, wN = 0.f, gN = 0.f;

@@ -243,9 +243,10 @@ gq_status gq_loglik_flip(gq_ctx *ctx, const double params[4], double out[4]);
  * Device time of the last gq_fk_steps call, summed over its step-kernel
  * launches (ms), with the number of launches and leaf cells updated. */
 gq_status gq_last_step_timing(gq_ctx *ctx, double *step_kernel_ms, int *launches, int64_t *cell_updates);
-/* per_kernel != 0: bracket every step-kernel launch of gq_fk_steps with a
- * pair of HIP events on the context stream (default 0: one pair around the
- * whole call, which then includes the cross-level ghost kernels). */
+/* per_kernel != 0: time every step-kernel launch of gq_fk_steps with a pair
+ * of HIP events that the kernel's dispatch records at its start and end
+ * (hipExtLaunchKernel; default 0: one pair around the whole call, which then
+ * includes the cross-level ghost kernels). */
 gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel);
 /* Step autotuner.  mode 1 (default): time the slab depths 16/8/4/2 (planes
  * of a 16^3 block per workgroup) with the cross-level face ghosts from a
