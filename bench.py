@@ -276,8 +276,12 @@ def mcmc_workload(args):
     def timed_loop(step, state):
         xs, acs = [], []
         t0 = time.perf_counter()
+        t_log = t0
         while time.perf_counter() - t0 < args.mcmc_budget:
             state = step(state, xs, acs)
+            if time.perf_counter() - t_log > 30.0:  # progress from the sampler itself
+                t_log = time.perf_counter()
+                log("[bench]   %.0f s, %d iterations" % (t_log - t0, sum(len(x) for x in xs)))
         return np.concatenate(xs, 0), np.concatenate(acs, 0), time.perf_counter() - t0
 
     log("[bench] mcmc MALA %d chains, %.0f s" % (C, args.mcmc_budget))
@@ -478,8 +482,12 @@ def surrogate_workload(args):
     def run_loop(fn, state):
         xs, acs = [], []
         t0 = time.perf_counter()
+        t_log = t0
         while time.perf_counter() - t0 < budget:
             state = fn(state, xs, acs)
+            if time.perf_counter() - t_log > 30.0:  # progress from the sampler itself
+                t_log = time.perf_counter()
+                log("[bench]   %.0f s, %d iterations" % (t_log - t0, sum(len(x) for x in xs)))
         return np.concatenate(xs, 0), np.concatenate(acs, 0), time.perf_counter() - t0, state
 
     # plain RWM

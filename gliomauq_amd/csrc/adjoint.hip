@@ -151,6 +151,15 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
         const float uc = s_u[c];
         const float kc = s_ik[c];
         const float muc = s_mu[c];
+        float mun6[6];
+#pragma unroll
+        for (int f = 0; f < 6; f++) mun6[f] = s_mu[c + offs[f]];
+        if ((muc == 0.f) & (mun6[0] == 0.f) & (mun6[1] == 0.f) & (mun6[2] == 0.f) & (mun6[3] == 0.f) &
+            (mun6[4] == 0.f) & (mun6[5] == 0.f)) {
+            // no adjoint flows through this cell: lambda = 0 and no gradient terms
+            a.out[base + (z0 + k) * 256 + t] = 0.f;
+            continue;
+        }
         // face coefficients 2 / (1/K_c + 1/K_k): the same value serves the
         // cell's own update and, transposed, the neighbour's (the sum is
         // commutative); a face whose value is not finite is never used
@@ -178,7 +187,7 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
 #pragma unroll
         for (int f = 0; f < 6; f++) {
             const int n = c + offs[f];
-            const float mun = s_mu[n];
+            const float mun = mun6[f];
             if (mun != 0.f) {
                 // the neighbour's tissue switch pw+pg+u > 0: with Dw, Dg > 0, 1/K is
                 // finite and positive exactly where pw+pg > 0 (else 0/0 = NaN), and

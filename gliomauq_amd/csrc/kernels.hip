@@ -117,6 +117,10 @@ __device__ __forceinline__ float fk_cell(float pc, float pW, float pE, float pS,
                                          float bsum, float ih2, float rho, float dt)
 {
 #pragma clang fp contract(off)
+    // phi = 0 on the cell and its six neighbours: every term below is zero
+    // (or NaN next to CSF, which the clamp turns into 0) and the update is +0
+    // exactly, so waves far from the tumour skip the arithmetic.
+    if ((pc == 0.f) & (pW == 0.f) & (pE == 0.f) & (pS == 0.f) & (pN == 0.f) & (pB == 0.f) & (pF == 0.f)) return 0.f;
     float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
     if (bsum + pc > 0.f) {  // tissue = p_w + p_g + phi > 0 (:155-157)
         d0 = 2.f * frcp(kc + kW);
