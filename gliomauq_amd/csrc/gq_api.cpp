@@ -185,6 +185,9 @@ struct gq_ctx {
     DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2, d_sprm;
     DevBuf<float> d_ikc;          // per-cell 1/K codes of the step (pool layout)
     DevBuf<int> d_wrec;           // per-work-item step records (gq::WREC ints each)
+    bool fuse2 = false;           // fuse step pairs without cross-level faces into one launch (opt-in)
+    std::vector<int> buf_after;   // advance(): ping-pong buffer holding the state after each step
+    int last_nlaunch = 0;         // advance(): kernel launches of the last call
     bool step_coeffs = true;      // false: the step recomputes 1/K from pw, pg every step
     bool fik_valid = false;       // false: every slot's face 1/K must be recomputed
     std::vector<char> fik_stale;  // per sample: recompute that sample's slots
@@ -893,6 +896,7 @@ struct gq_ctx {
         a.fik = d_fik.p;
         a.ikc = step_coeffs ? d_ikc.p : nullptr;
         a.wrec = d_wrec.p;
+        a.nbr26 = d_nbr26.p;
         a.gphi = d_gphi.p;
         a.fptr = d_face_ptr.p;
         a.fsrc = d_face_src.p;
@@ -936,24 +940,36 @@ struct gq_ctx {
         if (m == 0) return 0.0;
         const bool pk = per_kernel_timing && !use_graph;
         need_events(pk ? 2 * m : 2);
+        // steps j, j+1 over the same work prefix without cross-level faces run as one fused two-step launch
+        const bool can_fuse = fuse2 && a.ikc != nullptr;
+        buf_after.assign(m, 0);
+        int nlaunch = 0, final_c = cur;
         auto enqueue = [&](int c0) {
             int c = c0;
-            for (int j = 0; j < m; j++) {
-                if (nrows[j] > 0 && !fused)
+            nlaunch = 0;
+            for (int j = 0; j < m;) {
+                const bool pair = can_fuse && j + 1 < m && nwork[j] > 0 && nwork[j + 1] == nwork[j] &&
+                                  nrows[j] == 0 && nrows[j + 1] == 0;
+                if (!pair && nrows[j] > 0 && !fused)
                     HIPCHK(gq::launch_face_ghosts(phi[c], d_face_ptr.p, d_face_src.p, d_face_w.p, d_gphi.p, nrows[j],
                                                   st));
                 a.phi_in = phi[c];
                 a.phi_out = phi[c ^ 1];
                 a.nwork = nwork[j];
-                if (nwork[j] > 0) {
-                    if (pk) HIPCHK(gq::launch_fk_step(a, zt, st, events[2 * j], events[2 * j + 1]));
-                    else HIPCHK(gq::launch_fk_step(a, zt, st));
-                } else if (pk) {
-                    HIPCHK(hipEventRecord(events[2 * j], st));
-                    HIPCHK(hipEventRecord(events[2 * j + 1], st));
+                hipEvent_t e0 = pk ? events[2 * nlaunch] : nullptr, e1 = pk ? events[2 * nlaunch + 1] : nullptr;
+                if (pair) HIPCHK(gq::launch_fk_step2(a, st, e0, e1));
+                else if (nwork[j] > 0) HIPCHK(gq::launch_fk_step(a, zt, st, e0, e1));
+                else if (pk) {
+                    HIPCHK(hipEventRecord(e0, st));
+                    HIPCHK(hipEventRecord(e1, st));
                 }
                 c ^= 1;
+                nlaunch++;
+                buf_after[j] = c;  // a pair leaves no state after its first step (no sample stops there)
+                if (pair) buf_after[j + 1] = c;
+                j += pair ? 2 : 1;
             }
+            final_c = c;
         };
         double total = 0.0;
         if (use_graph) {
@@ -995,7 +1011,7 @@ struct gq_ctx {
             if (!pk) HIPCHK(hipEventRecord(events[1], st));
             HIPCHK(hipStreamSynchronize(st));
             if (pk) {
-                for (int j = 0; j < m; j++) {
+                for (int j = 0; j < nlaunch; j++) {
                     float ms = 0;
                     HIPCHK(hipEventElapsedTime(&ms, events[2 * j], events[2 * j + 1]));
                     total += ms;
@@ -1006,7 +1022,8 @@ struct gq_ctx {
                 total = ms;
             }
         }
-        if (m & 1) cur ^= 1;
+        cur = final_c;
+        last_nlaunch = nlaunch;
         return total;
     }
 
@@ -1306,14 +1323,15 @@ gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
         ctx->last_cfg = cfg;
         const int chunk = 1024;
         double total = 0.0;
-        int done = 0;
+        int done = 0, launches = 0;
         while (done < n_steps) {
             const int m = std::min(chunk, n_steps - done);
             total += ctx->advance(a, cfg, std::vector<int>(m, nwork), std::vector<int>(m, nrows), false);
+            launches += ctx->last_nlaunch;
             done += m;
         }
         ctx->last_ms = total;
-        ctx->last_launches = n_steps;
+        ctx->last_launches = launches;
         ctx->last_cells = (int64_t)n_steps * nwork * gq::BS3;
     });
 }
@@ -1376,14 +1394,15 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
             const int c0 = ctx->cur;
             ms += ctx->advance(a, cfg, nw, nr, use_graph);
             chunks++;
-            // a sample that took a number of steps of the other parity holds its state in the other buffer
+            // a sample that stopped early may hold its state in the other ping-pong buffer
             for (int pos = 0; pos < S; pos++) {
                 const int s = ctx->sample_order[pos];
                 const int ms_s = std::min(rem[s], m);
-                if ((ms_s & 1) != (m & 1)) {
+                const int src = ms_s > 0 ? ctx->buf_after[ms_s - 1] : c0;
+                if (src != ctx->cur) {
                     const int b0 = ctx->work_begin[pos], b1 = ctx->work_begin[pos + 1];
-                    HIPCHK(gq::launch_copy_blocks(ctx->phi[ctx->cur], ctx->phi[c0 ^ (ms_s & 1)], ctx->p_work + b0,
-                                                  b1 - b0, ctx->st));
+                    HIPCHK(gq::launch_copy_blocks(ctx->phi[ctx->cur], ctx->phi[src], ctx->p_work + b0, b1 - b0,
+                                                  ctx->st));
                 }
             }
             std::vector<int> adapt;
@@ -1514,10 +1533,17 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
         for (int seg = 0; seg < nseg; seg++) {
             HIPCHK(hipMemcpyAsync(ctx->d_ckpt.p + seg * pool, ctx->phi[ctx->cur], pbytes, hipMemcpyDeviceToDevice, st));
             const int m = std::min(E, n_steps - seg * E);
-            for (int j = 0; j < m; j++) {
+            const bool pairs = ctx->fuse2 && a.ikc != nullptr;  // uniform grid: fused two-step launches
+            for (int j = 0; j < m;) {
                 a.phi_in = ctx->phi[ctx->cur];
                 a.phi_out = ctx->phi[ctx->cur ^ 1];
-                HIPCHK(gq::launch_fk_step(a, zt, st));
+                if (pairs && j + 1 < m) {
+                    HIPCHK(gq::launch_fk_step2(a, st));
+                    j += 2;
+                } else {
+                    HIPCHK(gq::launch_fk_step(a, zt, st));
+                    j++;
+                }
                 ctx->cur ^= 1;
             }
         }
@@ -1888,6 +1914,14 @@ gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt)
         ctx->autotune_all = mode == 4;
         if (mode == 2 || mode == 3)
             for (int b = 0; b < 32; b++) ctx->zt_cache[b] = gq_ctx::make_cfg(fixed_zt, mode == 3);
+    });
+}
+
+gq_status gq_set_step_fusion(gq_ctx *ctx, int on)
+{
+    return guard(ctx, [&] {
+        if (on != 0 && on != 1) throw gq_error(GQ_ERR_ARG, "on must be 0 or 1");
+        ctx->fuse2 = on == 1;
     });
 }
 

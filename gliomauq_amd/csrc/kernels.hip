@@ -356,6 +356,106 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two fused steps for a block whose 26 neighbours are all leaves of its own
+// level (every block of a work list without cross-level faces: a uniform
+// grid).  One workgroup of 1024 threads per block stages phi_n and 1/K over
+// the block and a two-cell halo ([-2,18)^3, from the 26 neighbours), computes
+// phi_{n+1} over [-1,17)^3 (the block and the one-cell halo the second step
+// reads, i.e. the neighbours' own first-step values, recomputed here), then
+// phi_{n+2} of the block.  Both steps use fk_cell on the same inputs as two
+// single-step launches, so the result is the same to the bit; HBM sees one
+// read of phi and the 1/K codes and one write per two steps.  phi_in and
+// phi_out are distinct buffers (neighbours read phi_in concurrently).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_fk_step2(StepArgs a)
+{
+    constexpr int R = 20, R2 = R * R;  // phi_n, 1/K over [-2,18)^3
+    constexpr int Q = 18, Q2 = Q * Q;  // tissue sums, phi_{n+1} over [-1,17)^3
+    __shared__ float s_p0[R * R2];
+    __shared__ float s_ik[R * R2];
+    __shared__ float s_bs[Q * Q2];
+    __shared__ float s_p1[Q * Q2];
+    __shared__ int s_nb[27];
+    const int item = xcd_remap(blockIdx.x, gridDim.x);
+    if (item >= a.nwork) return;  // padding workgroups (whole workgroup exits)
+    const int *rec = a.wrec + (size_t)item * WREC;
+    const int slot = rec[0];
+    const int t = threadIdx.x;
+    float Dw = a.Dw, Dg = a.Dg, rho = a.rho, dt = a.dt;
+    if (a.sprm) {
+        const float *q = a.sprm + 4 * rec[14];
+        Dw = q[0];
+        Dg = q[1];
+        rho = q[2];
+        dt = q[3];
+    }
+    const float ih2 = __int_as_float(rec[1]);
+    if (t < 27) {
+        // region (dx,dy,dz) in {-1,0,1}^3 -> t = (dx+1) + 3(dy+1) + 9(dz+1); the
+        // table leaves the centre out.  A missing neighbour (never launched so)
+        // falls back to the block itself rather than an invalid slot.
+        int nb = slot;
+        if (t != 13) nb = a.nbr26[(size_t)slot * 26 + (t < 13 ? t : t - 1)];
+        s_nb[t] = nb >= 0 ? nb : slot;
+    }
+    __syncthreads();
+    // all loads of the staging issued before any is used
+    constexpr int NST = (R * R2 + 1023) / 1024;
+    float pv[NST], cv[NST];
+    size_t jv[NST];
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int i = t + 1024 * r;
+        jv[r] = 0;
+        pv[r] = cv[r] = 0.f;
+        if (i < R * R2) {
+            const int x = i % R - 2, y = (i / R) % R - 2, z = i / R2 - 2;
+            const int rx = x < 0 ? 0 : (x < B ? 1 : 2);
+            const int ry = y < 0 ? 0 : (y < B ? 1 : 2);
+            const int rz = z < 0 ? 0 : (z < B ? 1 : 2);
+            jv[r] = (size_t)s_nb[rx + 3 * ry + 9 * rz] * B3 + ((x + B) & 15) + B * ((y + B) & 15) +
+                    256 * ((z + B) & 15);
+            pv[r] = a.phi_in[jv[r]];
+            cv[r] = a.ikc[jv[r]];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int i = t + 1024 * r;
+        if (i < R * R2) {
+            const int lx = i % R, ly = (i / R) % R, lz = i / R2;
+            float ik, bs;
+            ik_centre(cv[r], pv[r], a.pw, a.pg, jv[r], Dw, Dg, ik, bs);
+            s_p0[i] = pv[r];
+            s_ik[i] = ik;
+            if (lx >= 1 && lx <= Q && ly >= 1 && ly <= Q && lz >= 1 && lz <= Q)
+                s_bs[(lx - 1) + Q * (ly - 1) + Q2 * (lz - 1)] = bs;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < Q * Q2; i += 1024) {
+        const int qx = i % Q, qy = (i / Q) % Q, qz = i / Q2;
+        const int c = (qx + 1) + R * (qy + 1) + R2 * (qz + 1);
+        s_p1[i] = fk_cell(s_p0[c], s_p0[c - 1], s_p0[c + 1], s_p0[c - R], s_p0[c + R], s_p0[c - R2], s_p0[c + R2],
+                          s_ik[c], s_ik[c - 1], s_ik[c + 1], s_ik[c - R], s_ik[c + R], s_ik[c - R2], s_ik[c + R2],
+                          s_bs[i], ih2, rho, dt);
+    }
+    __syncthreads();
+    const size_t base = (size_t)slot * B3;
+#pragma unroll
+    for (int r = 0; r < B3 / 1024; r++) {
+        const int i = t + 1024 * r;
+        const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
+        const int c1 = (x + 1) + Q * (y + 1) + Q2 * (z + 1);
+        const int c0 = (x + 2) + R * (y + 2) + R2 * (z + 2);
+        a.phi_out[base + i] =
+            fk_cell(s_p1[c1], s_p1[c1 - 1], s_p1[c1 + 1], s_p1[c1 - Q], s_p1[c1 + Q], s_p1[c1 - Q2], s_p1[c1 + Q2],
+                    s_ik[c0], s_ik[c0 - 1], s_ik[c0 + 1], s_ik[c0 - R], s_ik[c0 + R], s_ik[c0 - R2], s_ik[c0 + R2],
+                    s_bs[c1], ih2, rho, dt);
+    }
+}
+
 // Cross-level face ghosts of phi for the step: four lanes per row, partial
 // sums combined with lane shuffles.
 __global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ phi, const int64_t *__restrict__ ptr,
@@ -1511,6 +1611,15 @@ hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st, hipEvent_t 
     case 2: hipExtLaunchKernelGGL(k_fk_step<2>, dim3(grid), dim3(256), 0, st, e0, e1, 0, a); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_fk_step2(const StepArgs &a, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+{
+    if (a.nwork <= 0) return hipSuccess;
+    if (!a.ikc || !a.wrec || !a.nbr26 || a.phi_in == a.phi_out) return hipErrorInvalidValue;
+    const int grid = (a.nwork + 7) & ~7;
+    hipExtLaunchKernelGGL(k_fk_step2, dim3(grid), dim3(1024), 0, st, e0, e1, 0, a);
     return hipGetLastError();
 }
 

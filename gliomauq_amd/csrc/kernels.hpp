@@ -31,6 +31,7 @@ struct StepArgs {
     // [w*16] per work item: slot, 1/h^2 (float bits), fnbr[6], frow[6], sample, 0 -- one scalar load
     // gives a workgroup everything the slot tables would give it through dependent look-ups
     const int *wrec;
+    const int *nbr26;  // [slot*26] same-level neighbour per region (the fused two-step kernel)
 };
 constexpr int WREC = 16;
 
@@ -92,6 +93,10 @@ hipError_t launch_ic_grad(const float *lam0, const int *work, int nwork, const i
 
 // zt: slab depth (16, 8, 4 or 2 planes per workgroup), or 1: one workgroup per
 // block streaming its planes through registers (k_fk_step_col)
+// Two fused steps phi_in -> phi_out (distinct buffers) for a work list
+// whose blocks have all 26 neighbours at their own level (no cross-level
+// faces); needs the per-cell 1/K codes.
+hipError_t launch_fk_step2(const StepArgs &a, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // e0/e1 (optional): events the dispatch itself records at the kernel's start and end
 hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st, hipEvent_t e0 = nullptr,
                           hipEvent_t e1 = nullptr);

@@ -239,6 +239,13 @@ def set_step_coeffs(ctx, precomputed=True):
     ctx.call("gq_set_step_coeffs", 1 if precomputed else 0)
 
 
+def set_step_fusion(ctx, on=True):
+    """True: pairs of steps over a work list without cross-level faces
+    (uniform grids) run as one fused kernel launch; False (the context's
+    default, measured faster): one launch per step.  Same bits either way."""
+    ctx.call("gq_set_step_fusion", 1 if on else 0)
+
+
 def autotune_info(ctx):
     """(slab depth, face ghosts in the step?, {"<depth>": us, "<depth>f": us})
     of the last step call and the last autotune timings."""

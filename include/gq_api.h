@@ -265,6 +265,16 @@ gq_status gq_set_autotune(gq_ctx *ctx, int mode, int fixed_zt);
  * (8 B per cell) instead of phi, pw, pg (12 B); 0: the step recomputes 1/K
  * from pw, pg every step.  Both give the same bits. */
 gq_status gq_set_step_coeffs(gq_ctx *ctx, int precomputed);
+/* Step fusion (no reference counterpart; the reference applies one step per
+ * pipeline_process/process pair, Glioma_HG_UQ.cpp:266-277).  on = 1: two
+ * consecutive steps over a work list without cross-level faces (a uniform
+ * grid) run as one kernel that stages phi and 1/K with a two-cell halo and
+ * recomputes the neighbours' first-step halo values, so HBM sees one read and
+ * one write per two steps; 0 (default): one launch per step, which measured
+ * faster on MI355X (the fused kernel is bound by its redundant first-step
+ * arithmetic).  Both give the same bits.  Needs the precomputed step
+ * coefficients. */
+gq_status gq_set_step_fusion(gq_ctx *ctx, int on);
 /* Slab depth and face-ghost mode (1 = evaluated in the step) used by the last
  * step call, and the last autotune timings in us per step: [0..4] depths 16,
  * 8, 4, 2, 1 with the separate face-ghost launch (both kernels timed), [5..9]

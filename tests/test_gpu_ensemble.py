@@ -64,6 +64,20 @@ def run_single(ctx, pwm, pgm, u0, prm, n_steps, adaptive=True):
     return g
 
 
+def advance_single(g, prm, n_steps):
+    """The ensemble's per-sample schedule on one grid: adaptation after every
+    full chunk of K steps, final refinement."""
+    done = 0
+    while done < n_steps:
+        m = min(K, n_steps - done)
+        gq.fk_steps(g, *prm, m)
+        done += m
+        if m == K:
+            gq.Science.AutomaticRefinement(g, TOL_R, 3, 1)
+            gq.Science.AutomaticCompression(g, TOL_C, -1)
+    gq.Science.AutomaticRefinement(g, TOL_R, 3, 1)
+
+
 def make_ensemble(ctx, pwm, pgm, ics):
     ens = gq.Ensemble(ctx, len(ics), N // 16, 3, 4)
     ens.write_volume("pw", pwm)
@@ -102,6 +116,57 @@ def test_ensemble_equals_single_runs(inputs):
     # the samples did evolve differently
     assert len({c for c in counts.tolist()}) > 1
     ctx.close()
+
+
+def test_ensemble_second_run_with_more_running_samples(inputs):
+    """Face-ghost rows and face 1/K are built only for the samples still
+    stepping; a second run in which a sample that stopped early steps again
+    (same work order, longer running prefix) must rebuild them for it."""
+    pwm, pgm, ics, _ = inputs
+    phases = [[30, 7], [30, 20]]
+    singles = []
+    for s in range(2):
+        ctx = gq.Context(0)
+        g = gq.Grid(ctx, N // 16, 3, 4)
+        g.write_volume("phi", ics[s])
+        g.write_volume("pw", pwm)
+        g.write_volume("pg", pgm)
+        for ph in phases:
+            advance_single(g, sample_params(s), ph[s])
+        singles.append((g.dump_uniform(), g.n_blocks()))
+        ctx.close()
+    ctx = gq.Context(0)
+    ens = make_ensemble(ctx, pwm, pgm, ics[:2])
+    prm = np.array([sample_params(s) for s in range(2)])
+    for ph in phases:
+        ens.run(prm, ph, adapt_every=K, tol_refine=TOL_R, tol_compress=TOL_C, graph=True)
+    counts = ens.block_counts()
+    for s in range(2):
+        assert counts[s] == singles[s][1]
+        assert np.array_equal(ens.sample(s).dump_uniform(), singles[s][0]), "sample %d differs" % s
+    ctx.close()
+
+
+def test_ensemble_fused_steps_uneven_counts(inputs):
+    """Uniform grids with per-sample step counts 5, 8, 3: step pairs over an
+    unchanged running prefix run fused; each sample ends in the ping-pong
+    buffer its last launch wrote and is copied to the current one.  Same bits
+    as one launch per step, with and without graph capture."""
+    pwm, pgm, ics, _ = inputs
+    steps = [5, 8, 3]
+    out = []
+    for fuse in (False, True):
+        for graph in (False, True):
+            ctx = gq.Context(0)
+            gq.set_step_fusion(ctx, fuse)
+            ens = make_ensemble(ctx, pwm, pgm, ics[:3])
+            prm = np.array([sample_params(s) for s in range(3)])
+            ens.run(prm, steps, adapt_every=0, graph=graph)
+            out.append([ens.sample(s).dump_uniform() for s in range(3)])
+            ctx.close()
+    for o in out[1:]:
+        for s in range(3):
+            assert o[s].tobytes() == out[0][s].tobytes(), "sample %d differs" % s
 
 
 def test_ensemble_uniform_vs_oracle(inputs):

@@ -164,14 +164,9 @@ def test_step_variants_bit_identical():
         assert nb == outs[0][0] and np.array_equal(a, outs[0][1])
 
 
-@pytest.mark.parametrize("zt", [16, 4, 1])
-def test_precomputed_coefficients_bit_identical(ctx, zt):
-    """The per-cell 1/K codes give the same bits as recomputing 1/K from pw,
-    pg every step, also for the cells that take the slow path: negative pw or
-    pg (as Interp4 prolongation makes them), p_w = -p_g != 0, p_w + p_g < 0,
-    CSF cells with and without tumour, and negative phi (as after a split)."""
-    n = 32
-    rng = np.random.default_rng(11)
+def awkward_case(n, seed=11):
+    """Anatomy and field with every case the per-cell 1/K codes treat apart."""
+    rng = np.random.default_rng(seed)
     pw = rng.random((n, n, n)).astype(np.float32)
     pg = (1 - pw).astype(np.float32)
     phi = (0.5 * rng.random((n, n, n))).astype(np.float32)
@@ -188,6 +183,38 @@ def test_precomputed_coefficients_bit_identical(ctx, zt):
             flat[1][sel] = g_
         if p_ is not None:
             flat[2][sel] = p_
+    return pw, pg, phi
+
+
+@pytest.mark.parametrize("n,steps", [(32, 7), (64, 6)])
+def test_fused_steps_bit_identical(n, steps):
+    """Two steps fused into one launch (uniform grids) give the same bits as
+    one launch per step, on anatomy with every special case of the 1/K codes
+    and a field with zero regions, for odd and even step counts."""
+    pw, pg, phi = awkward_case(n)
+    phi[:, :, : n // 2] = 0.0  # a zero half: the closed-form zero update at its edge
+    res = []
+    for fuse in (False, True):
+        c = gq.Context(0)
+        gq.set_step_fusion(c, fuse)
+        g = gq.Grid(c, n // 16, int(np.log2(n // 16)), 2)
+        for name, v in (("phi", phi), ("pw", pw), ("pg", pg)):
+            g.sample_volume(name, v)
+        gq.fk_steps(g, 2e-4, 2e-5, 0.3, 0.5, steps)
+        res.append(g.dump_uniform())
+        c.close()
+    assert np.isfinite(res[0]).all() and np.count_nonzero(res[0]) > 0
+    assert res[0].tobytes() == res[1].tobytes()
+
+
+@pytest.mark.parametrize("zt", [16, 4, 1])
+def test_precomputed_coefficients_bit_identical(ctx, zt):
+    """The per-cell 1/K codes give the same bits as recomputing 1/K from pw,
+    pg every step, also for the cells that take the slow path: negative pw or
+    pg (as Interp4 prolongation makes them), p_w = -p_g != 0, p_w + p_g < 0,
+    CSF cells with and without tumour, and negative phi (as after a split)."""
+    n = 32
+    pw, pg, phi = awkward_case(n)
     res = []
     for pre in (False, True):
         c = gq.Context(0)
