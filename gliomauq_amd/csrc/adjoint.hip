@@ -179,8 +179,9 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
             const float diff = d[0] * (s_u[c - 1] - uc) + d[1] * (s_u[c + 1] - uc) + d[2] * (s_u[c - LW] - uc) +
                                d[3] * (s_u[c + LW] - uc) + d[4] * (s_u[c - PL] - uc) + d[5] * (s_u[c + PL] - uc);
             lam = muc * (1.f + dt * (rho * (1.f - 2.f * uc) - ih2 * sdf));
-            gD += (double)muc * (double)dt * (double)ih2 * (double)diff;
-            gR += (double)muc * (double)dt * (double)rho * (double)uc * (1.0 - (double)uc);
+            // the per-block factors dt ih2 and dt rho are applied once to the sums
+            gD += (double)muc * (double)diff;
+            gR += (double)muc * ((double)uc * (1.0 - (double)uc));
         }
         // transpose: neighbour k's update reads u_c with k's coefficient toward c
         float tr = 0.f;
@@ -201,8 +202,8 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
         a.out[base + (z0 + k) * 256 + t] = a.out_lambda ? lam : mu_of(lam, uc);
     }
     // block sums of the two parameter gradients -> this slab's accumulators
-    gD = wave_sum(gD);
-    gR = wave_sum(gR);
+    gD = wave_sum(gD) * ((double)dt * (double)ih2);
+    gR = wave_sum(gR) * ((double)dt * (double)rho);
     const int lane = t & 63, wv = t >> 6;
     if (lane == 0) {
         s_red[0][wv] = gD;
