@@ -46,8 +46,11 @@ struct DevBuf {
     void reserve(size_t m)
     {
         if (m <= n) return;
+        // regrowth takes 1.5x the old capacity: a grid that grows step by step
+        // reallocates (hipFree synchronises the device) a logarithmic number of times
+        const size_t grow = n ? n + n / 2 : 0;
         release();
-        size_t cap = std::max<size_t>(m, 16);
+        size_t cap = std::max<size_t>(std::max(m, grow), 16);
         HIPCHK(hipMalloc((void **)&p, cap * sizeof(T)));
         n = cap;
     }
@@ -59,7 +62,11 @@ struct DevBuf {
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            // kernels or copies still queued may read the old buffer
+            (void)hipDeviceSynchronize();
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -76,13 +83,14 @@ struct PinnedBuf {
     void reserve(size_t m, hipStream_t st)
     {
         if (m <= n) return;
+        const size_t grow = n + n / 2;
         if (p) {
             HIPCHK(hipStreamSynchronize(st));
             (void)hipHostFree(p);
             p = nullptr;
             n = 0;
         }
-        const size_t cap = std::max<size_t>(m, 4096);
+        const size_t cap = std::max<size_t>(std::max(m, grow), 4096);
         HIPCHK(hipHostMalloc((void **)&p, cap * sizeof(T), hipHostMallocDefault));
         n = cap;
     }
@@ -94,12 +102,17 @@ struct PinnedBuf {
 
 // Host int arrays copied to the device in one transfer: the parts lie back to
 // back (each at a 64-int boundary) in one device buffer, staged in pinned
-// memory.  Callers synchronise the stream before the next upload through the
-// same object (the staging buffer is rewritten then).
+// memory.  The caller's vectors may be released as soon as upload() returns:
+// the transfer reads the staging buffer, and the next upload through the same
+// object waits for the previous transfer (an event) before rewriting it.  The
+// device buffer is rewritten in stream order, after the kernels queued on the
+// same stream that read its previous contents.
 struct PackedInts {
     DevBuf<int> dev;
     PinnedBuf<int> host;
     std::vector<size_t> off;
+    hipEvent_t done = nullptr;
+    bool pending = false;
     void upload(std::initializer_list<const std::vector<int> *> parts, hipStream_t st)
     {
         size_t total = 0;
@@ -108,13 +121,24 @@ struct PackedInts {
             off.push_back(total);
             total += (v->size() + 63) & ~(size_t)63;
         }
+        if (pending) HIPCHK(hipEventSynchronize(done));
+        pending = false;
         host.reserve(std::max<size_t>(total, 1), st);
         size_t k = 0;
         for (const std::vector<int> *v : parts) std::copy(v->begin(), v->end(), host.p + off[k++]);
         dev.reserve(std::max<size_t>(total, 1));
-        if (total) HIPCHK(hipMemcpyAsync(dev.p, host.p, total * sizeof(int), hipMemcpyHostToDevice, st));
+        if (total) {
+            HIPCHK(hipMemcpyAsync(dev.p, host.p, total * sizeof(int), hipMemcpyHostToDevice, st));
+            if (!done) HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(done, st));
+            pending = true;
+        }
     }
     int *ptr(int i) const { return dev.p + off[i]; }
+    ~PackedInts()
+    {
+        if (done) (void)hipEventDestroy(done);
+    }
 };
 
 double now_s()
@@ -174,6 +198,7 @@ struct gq_ctx {
     PackedInts tabs;
     int *p_work = nullptr, *p_sfbase = nullptr, *p_lut = nullptr, *p_lutf = nullptr, *p_faces = nullptr;
     PackedInts lists;  // slot lists of one adaptation kernel sequence
+    PackedInts flist;  // slots whose face 1/K is recomputed
     PinnedBuf<double> h_det;   // FWT details read back
     PinnedBuf<int64_t> h_total;  // face-row term count read back
     DevBuf<int64_t> d_face_ptr;
@@ -374,6 +399,7 @@ struct gq_ctx {
             for (int s = 0; s < S; s++) sample_order[s] = s;
         }
         const bool full = uploaded_cap != cap;
+        bool pageable = full;  // copies straight from pageable host vectors are queued below
         h_fnbr.resize((size_t)cap * 6, -1);
         h_nbr26.resize((size_t)cap * 26, -1);
         h_level.resize((size_t)cap, 0);
@@ -484,11 +510,13 @@ struct gq_ctx {
                 ih2[l] = (float)(1. / (hd * hd));
             }
             d_ih2.upload(ih2, st);
+            pageable = true;
         }
         d_wrec.reserve((size_t)std::max<size_t>(1, work.size()) * gq::WREC);
         HIPCHK(gq::launch_work_records(p_work, (int)work.size(), d_fnbr.p, d_frow.p, d_level.p, d_ih2.p,
                                        d_slot_sample.p, d_wrec.p, st));
-        HIPCHK(hipStreamSynchronize(st));  // the host vectors above are released on return
+        // the packed lists went through pinned staging; copies from pageable vectors finish first
+        if (pageable) HIPCHK(hipStreamSynchronize(st));
         tables_valid = true;
     }
 
@@ -571,10 +599,9 @@ struct gq_ctx {
             for (int pos = 0; pos < P; pos++)
                 if (fik_stale[sample_order[pos]])
                     sl.insert(sl.end(), work.begin() + work_begin[pos], work.begin() + work_begin[pos + 1]);
-            d_list.upload(sl, st);
-            list = d_list.p;
+            flist.upload({&sl}, st);
+            list = flist.ptr(0);
             n = (int)sl.size();
-            HIPCHK(hipStreamSynchronize(st));  // sl is released on return
         }
         HIPCHK(gq::launch_face_prep(pw, pg, list, n, d_fnbr.p, d_fbase.p, p_sfbase, d_slot_sample.p, d_face_ptr.p,
                                     d_face_src.p, d_face_w.p, d_fik.p, d_ikc.p, key[0], key[1],
@@ -660,7 +687,8 @@ struct gq_ctx {
                                      st));
         HIPCHK(gq::launch_split(order, la, d_ghost.p, phi[cur], pw, pg, lists.ptr(0), lists.ptr(1), (int)pslots.size(),
                                 st));
-        HIPCHK(hipStreamSynchronize(st));
+        // no wait: the host updates the topology while the split runs; later work on
+        // the parents' slots (reused by later allocations) is queued behind it
         for (int s : pslots) free_slots.push_back(s);
         parallel_for(n, [&](int k) {
             if (!parents[k].empty()) G(which[k]).replace_leaves(parents[k], add[k]);
@@ -716,7 +744,6 @@ struct gq_ctx {
         }
         lists.upload({&pslots, &cslots}, st);
         HIPCHK(gq::launch_collapse(phi[cur], pw, pg, lists.ptr(0), lists.ptr(1), (int)pslots.size(), st));
-        HIPCHK(hipStreamSynchronize(st));
         for (int s : cslots) free_slots.push_back(s);
         parallel_for(n, [&](int k) {
             if (!plan[k].empty()) G(which[k]).replace_leaves(remove[k], add[k]);
