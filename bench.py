@@ -99,6 +99,7 @@ def ensemble_workload(args):
         steps[s] = gq.n_steps_until(T[s], dt)
     ic = np.array([[c[2], c[1], c[0], 0.1, 2.0] for c in centers], np.float64)
     ctx = gq.Context(0)
+    gq.set_step_persistent(ctx, args.persistent)
     # observations of a ground-truth solve at perturbed parameters
     gt = gq.Ensemble(ctx, 1, bpd, 3, 4)
     gt.write_volume("pw", pwm)
@@ -147,6 +148,7 @@ def ensemble_workload(args):
                    "loglik_mean": float(np.mean(ll[:, 0]))},
         "breakdown": {"device_chunk_ms": st["device_ms"], "adapt_s": st["adapt_s"], "chunks": st["chunks"],
                       "adaptations": st["adaptations"], "graph": bool(st["graph"]), "slab_depth": st["zt"],
+                      "persistent": gq.persistent_info(ctx),
                       "host": {k: v - hs0[k] for k, v in gq.host_stats(ctx).items()}},
     }
     if args.kernel_timing:
@@ -671,6 +673,8 @@ def main():
     ap.add_argument("--steps-adj-reps", type=int, default=2)
     ap.add_argument("--samples", type=int, default=256, help="ensemble samples")
     ap.add_argument("--kernel-timing", action="store_true", help="event-time every step kernel (no graphs)")
+    ap.add_argument("--persistent", type=int, default=0, choices=[0, 1, 2],
+                    help="persistent step chunks (cooperative launch per chunk): 0 never, 1 autotuner candidate, 2 always")
     ap.add_argument("--autotune", type=int, default=1,
                     help="step autotune mode (1: slab depths; 4: also the column kernel and in-step face ghosts)")
     args = ap.parse_args()
@@ -702,6 +706,7 @@ def main():
 
     ctx = gq.Context(0)
     gq.set_autotune(ctx, args.autotune)
+    gq.set_step_persistent(ctx, args.persistent)
 
     def fresh_grid():
         g = gq.Grid(ctx, bpd, max_level, 4)
@@ -803,7 +808,8 @@ def main():
     }
     res["breakdown"] = {"fk_steps_s": t_steps, "adapt_s": t_adapt, "step_kernel_s": kernel_s,
                         "host": {k: hs1[k] - hs0[k] for k in hs1}, "slab_depth": zt,
-                        "face_ghosts_in_step": fused, "autotune_us": tune}
+                        "face_ghosts_in_step": fused, "autotune_us": tune,
+                        "persistent": gq.persistent_info(ctx)}
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         res["cpu_baseline"] = cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt)

@@ -237,9 +237,19 @@ struct gq_ctx {
     // step configuration: slab depth zt (16/8/4/2, 1 = column streaming) and
     // whether the step evaluates the cross-level face-ghost rows itself
     // (fused) or reads them from a separate k_face_ghosts launch
+    // and whether a chunk of steps runs as one persistent cooperative launch
     static int cfg_zt(int cfg) { return cfg & 31; }
     static bool cfg_fused(int cfg) { return (cfg >> 5) & 1; }
-    static int make_cfg(int zt, bool fused) { return zt | (fused ? 32 : 0); }
+    static bool cfg_persist(int cfg) { return (cfg >> 6) & 1; }
+    static int make_cfg(int zt, bool fused, bool persist = false)
+    {
+        return zt | (fused ? 32 : 0) | (persist ? 64 : 0);
+    }
+    // persistent chunks: 0 never (default: measured slower, the grid-wide
+    // barriers cost more than a launch), 1 an autotuner candidate, 2 always
+    int persist_mode = 0;
+    double tune_persist_us = 0.0;
+    DevBuf<float> d_tune;  // second state buffer of the persistent candidate's timing
     std::map<int, int> zt_cache;  // work-list size bucket -> cfg
     int last_cfg = 16;
     double tune_us[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -868,7 +878,10 @@ struct gq_ctx {
         while ((1 << bucket) < nwork) bucket++;
         auto it = zt_cache.find(bucket);
         if (it != zt_cache.end()) return it->second;
-        if (!autotune) return make_cfg(zt_heuristic(nwork), false);
+        if (!autotune) {
+            const int zt = zt_heuristic(nwork);
+            return make_cfg(zt, false, persist_mode == 2 && zt >= 2);
+        }
         ScopeTimer tm(t_tune);
         static const int cand[5] = {16, 8, 4, 2, 1};
         hipEvent_t e0, e1;
@@ -905,8 +918,44 @@ struct gq_ctx {
                 }
             }
         }
+        // the persistent chunk at the fastest depth, timed over 4-step launches
+        // on a copy of the state (phi[cur^1] and a scratch buffer) so the
+        // live state is never overwritten
+        tune_persist_us = 0.0;
+        const int bzt = cfg_zt(best);
+        if (persist_mode == 1 && bzt >= 2 && nwork > 0) {
+            gq::PersistArgs p;
+            p.a = a;
+            if (cfg_fused(best)) p.a.gphi = nullptr;
+            d_tune.reserve((size_t)cap * gq::BS3);
+            HIPCHK(hipMemcpyAsync(phi[cur ^ 1], phi[cur], (size_t)cap * gq::BS3 * sizeof(float),
+                                  hipMemcpyDeviceToDevice, st));
+            p.phi[0] = phi[cur ^ 1];
+            p.phi[1] = d_tune.p;
+            p.gout = d_gphi.p;
+            p.c0 = 0;
+            p.m = 4;
+            for (int j = 0; j < p.m; j++) {
+                p.nwork[j] = nwork;
+                p.nrows[j] = nrows;
+            }
+            const hipError_t e = gq::launch_fk_persistent(p, bzt, st);
+            if (e == hipSuccess) {
+                HIPCHK(hipEventRecord(e0, st));
+                for (int r = 0; r < 2; r++) HIPCHK(gq::launch_fk_persistent(p, bzt, st));
+                HIPCHK(hipEventRecord(e1, st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                tune_persist_us = 1e3 * ms / (2 * p.m);
+                if (ms < best_ms * (2 * p.m) / 5) best = make_cfg(bzt, cfg_fused(best), true);
+            } else {
+                (void)hipGetLastError();  // not launchable cooperatively here: not a candidate
+            }
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        if (persist_mode == 2 && bzt >= 2) best = make_cfg(bzt, cfg_fused(best), true);
         zt_cache[bucket] = best;
         return best;
     }
@@ -971,9 +1020,33 @@ struct gq_ctx {
         const bool can_fuse = fuse2 && a.ikc != nullptr;
         buf_after.assign(m, 0);
         int nlaunch = 0, final_c = cur;
+        const bool persist = cfg_persist(cfg) && zt >= 2;
         auto enqueue = [&](int c0) {
             int c = c0;
             nlaunch = 0;
+            if (persist) {  // launches of up to PMAX steps
+                for (int j0 = 0; j0 < m; j0 += gq::PMAX) {
+                    gq::PersistArgs p;
+                    p.a = a;
+                    p.phi[0] = phi[c];
+                    p.phi[1] = phi[c ^ 1];
+                    p.gout = d_gphi.p;
+                    p.c0 = 0;
+                    p.m = std::min(gq::PMAX, m - j0);
+                    for (int j = 0; j < p.m; j++) {
+                        p.nwork[j] = nwork[j0 + j];
+                        p.nrows[j] = fused ? 0 : nrows[j0 + j];
+                        buf_after[j0 + j] = c ^ ((j + 1) & 1);
+                    }
+                    if (pk) HIPCHK(hipEventRecord(events[2 * nlaunch], st));
+                    HIPCHK(gq::launch_fk_persistent(p, zt, st));
+                    if (pk) HIPCHK(hipEventRecord(events[2 * nlaunch + 1], st));
+                    nlaunch++;
+                    c ^= p.m & 1;
+                }
+                final_c = c;
+                return;
+            }
             for (int j = 0; j < m;) {
                 const bool pair = can_fuse && j + 1 < m && nwork[j] > 0 && nwork[j + 1] == nwork[j] &&
                                   nrows[j] == 0 && nrows[j + 1] == 0;
@@ -999,7 +1072,7 @@ struct gq_ctx {
             final_c = c;
         };
         double total = 0.0;
-        if (use_graph) {
+        if (use_graph && !persist) {  // a cooperative launch is not captured: one launch per chunk anyway
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
@@ -1967,6 +2040,23 @@ gq_status gq_autotune_info(gq_ctx *ctx, int *zt, int *fused, double us[10])
         if (fused) *fused = gq_ctx::cfg_fused(ctx->last_cfg) ? 1 : 0;
         if (us)
             for (int k = 0; k < 10; k++) us[k] = ctx->tune_us[k];
+    });
+}
+
+gq_status gq_set_step_persistent(gq_ctx *ctx, int mode)
+{
+    return guard(ctx, [&] {
+        if (mode < 0 || mode > 2) throw gq_error(GQ_ERR_ARG, "persistent mode must be 0, 1 or 2");
+        ctx->persist_mode = mode;
+        ctx->zt_cache.clear();
+    });
+}
+
+gq_status gq_persistent_info(gq_ctx *ctx, int *used, double *tuned_us)
+{
+    return guard(ctx, [&] {
+        if (used) *used = gq_ctx::cfg_persist(ctx->last_cfg) ? 1 : 0;
+        if (tuned_us) *tuned_us = ctx->tune_persist_us;
     });
 }
 

@@ -13,6 +13,8 @@
 
 #include "kernels.hpp"
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <hipcub/device/device_scan.hpp>
 
 namespace gq {
@@ -240,15 +242,10 @@ __device__ __forceinline__ float step_face_ghost(const StepArgs &a, int row)
 // level) or the cross-level ghost rows; x/y halos from the face neighbours.
 // ---------------------------------------------------------------------------
 template <int ZT>
-__global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
+__device__ __forceinline__ void fk_step_item(const StepArgs &a, int item, float *s_phi, float *s_ik)
 {
     constexpr int NS = B / ZT;       // slabs per block
-    constexpr int LZ = ZT + 2;       // LDS planes
     constexpr int PL = LW * LW;      // LDS plane size
-    __shared__ float s_phi[LZ * PL];
-    __shared__ float s_ik[LZ * PL];
-    const int item = xcd_remap(blockIdx.x, gridDim.x);
-    if (item >= a.nwork * NS) return;  // padding workgroups (whole workgroup exits)
     const int w = item / NS;
     const int z0 = (item - w * NS) * ZT;
     const int *rec = a.wrec + (size_t)w * WREC;
@@ -356,6 +353,17 @@ __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
     }
 }
 
+template <int ZT>
+__global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
+{
+    constexpr int NS = B / ZT;
+    __shared__ float s_phi[(ZT + 2) * LW * LW];
+    __shared__ float s_ik[(ZT + 2) * LW * LW];
+    const int item = xcd_remap(blockIdx.x, gridDim.x);
+    if (item >= a.nwork * NS) return;  // padding workgroups (whole workgroup exits)
+    fk_step_item<ZT>(a, item, s_phi, s_ik);
+}
+
 // ---------------------------------------------------------------------------
 // Two fused steps for a block whose 26 neighbours are all leaves of its own
 // level (every block of a work list without cross-level faces: a uniform
@@ -458,12 +466,12 @@ __global__ __launch_bounds__(1024) void k_fk_step2(StepArgs a)
 
 // Cross-level face ghosts of phi for the step: four lanes per row, partial
 // sums combined with lane shuffles.
-__global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ phi, const int64_t *__restrict__ ptr,
-                                                     const int *__restrict__ src, const float *__restrict__ w,
-                                                     float *__restrict__ out, int nrows)
+// g: global lane (four per row); every lane of a wave calls it (shuffles).
+__device__ __forceinline__ void face_ghost_lanes(const float *__restrict__ phi, const int64_t *__restrict__ ptr,
+                                                 const int *__restrict__ src, const float *__restrict__ w,
+                                                 float *__restrict__ out, int nrows, int g)
 {
 #pragma clang fp contract(off)
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = g >> 2, lane = g & 3;
     float acc = 0.f;
     if (r < nrows) {
@@ -488,6 +496,57 @@ __global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ p
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     if (r < nrows && lane == 0) out[r] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_face_ghosts(const float *__restrict__ phi, const int64_t *__restrict__ ptr,
+                                                     const int *__restrict__ src, const float *__restrict__ w,
+                                                     float *__restrict__ out, int nrows)
+{
+    face_ghost_lanes(phi, ptr, src, w, out, nrows, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent chunk of steps: one cooperative launch advances m steps.  Each
+// step is two phases separated by grid-wide barriers (cooperative groups):
+// the cross-level face ghosts of phi_in (four lanes per row, as
+// k_face_ghosts), then the step of every work item of that step's prefix
+// (fk_step_item, as k_fk_step<ZT>), so every value equals the per-step
+// launches' to the bit.  The workgroups stay resident for the whole chunk
+// (the launch fails rather than oversubscribe the device); each XCD walks a
+// contiguous range of the work items, the same range every step.
+// ---------------------------------------------------------------------------
+template <int ZT>
+__global__ __launch_bounds__(256) void k_fk_persistent(PersistArgs p)
+{
+    constexpr int NS = B / ZT;
+    __shared__ float s_phi[(ZT + 2) * LW * LW];
+    __shared__ float s_ik[(ZT + 2) * LW * LW];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int per = gridDim.x >> 3;  // workgroups per XCD (the launcher uses a multiple of 8)
+    const int xcd = blockIdx.x & 7, rank = blockIdx.x >> 3;
+    int c = p.c0;
+    for (int j = 0; j < p.m; j++) {
+        const int nrows = p.nrows[j];
+        if (nrows > 0 && p.a.gphi) {  // uniform over the grid
+            for (int g0 = blockIdx.x * 256; g0 < 4 * nrows; g0 += gridDim.x * 256)
+                face_ghost_lanes(p.phi[c], p.a.fptr, p.a.fsrc, p.a.fw, p.gout, nrows, g0 + (int)threadIdx.x);
+            __threadfence();
+            grid.sync();
+        }
+        StepArgs a = p.a;
+        a.phi_in = p.phi[c];
+        a.phi_out = p.phi[c ^ 1];
+        a.nwork = p.nwork[j];
+        const int items = a.nwork * NS;
+        const int lo = (int)((int64_t)items * xcd / 8), hi = (int)((int64_t)items * (xcd + 1) / 8);
+        for (int it = lo + rank; it < hi; it += per) {
+            fk_step_item<ZT>(a, it, s_phi, s_ik);
+            __syncthreads();  // the LDS tiles are reused by the next item
+        }
+        __threadfence();
+        grid.sync();
+        c ^= 1;
+    }
 }
 
 // Absolute first face-ghost row of every face of the listed slots (one table
@@ -1612,6 +1671,53 @@ hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st, hipEvent_t 
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int ZT>
+static hipError_t persistent_zt(const PersistArgs &p, hipStream_t st, int max_items)
+{
+    static int slots[17] = {0};  // resident workgroups per CU, by slab depth
+    static int ncu = 0;
+    if (!slots[ZT]) {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        int coop = 0;
+        e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+        if (e != hipSuccess) return e;
+        if (!coop) return hipErrorNotSupported;
+        e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        int n = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(k_fk_persistent<ZT>), 256,
+                                                         0);
+        if (e != hipSuccess) return e;
+        if (n <= 0) return hipErrorNotSupported;
+        slots[ZT] = n;
+    }
+    int grid = slots[ZT] * ncu;
+    grid = std::min(grid, std::max(8, (max_items + 7) & ~7));
+    grid &= ~7;
+    if (grid < 8) return hipErrorNotSupported;
+    PersistArgs q = p;
+    void *args[] = {&q};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_fk_persistent<ZT>), dim3(grid), dim3(256),
+                                      args, 0, st);
+}
+
+hipError_t launch_fk_persistent(const PersistArgs &p, int zt, hipStream_t st)
+{
+    if (p.m <= 0) return hipSuccess;
+    if (p.m > PMAX || !p.a.wrec) return hipErrorInvalidValue;
+    int max_items = 0;
+    for (int j = 0; j < p.m; j++) max_items = std::max(max_items, p.nwork[j] * (16 / std::max(zt, 1)));
+    switch (zt) {
+    case 16: return persistent_zt<16>(p, st, max_items);
+    case 8: return persistent_zt<8>(p, st, max_items);
+    case 4: return persistent_zt<4>(p, st, max_items);
+    case 2: return persistent_zt<2>(p, st, max_items);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_fk_step2(const StepArgs &a, hipStream_t st, hipEvent_t e0, hipEvent_t e1)

@@ -35,6 +35,20 @@ struct StepArgs {
 };
 constexpr int WREC = 16;
 
+// One cooperative launch over m <= PMAX steps (k_fk_persistent): step j
+// updates the first nwork[j] work items and evaluates the first nrows[j]
+// cross-level face-ghost rows of phi[c] into gout (a.gphi == gout; with
+// a.gphi null the step evaluates the rows itself); c = c0 ^ (j & 1).
+constexpr int PMAX = 16;
+struct PersistArgs {
+    StepArgs a;
+    float *phi[2];
+    float *gout;
+    int c0, m;
+    int nwork[PMAX];
+    int nrows[PMAX];
+};
+
 // Location tables for ghost evaluation on the device: the leaf covering a
 // point is found through a look-up over the finest level.
 struct LabArgs {
@@ -102,6 +116,9 @@ hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st, hipEvent_t 
                           hipEvent_t e1 = nullptr);
 hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *src, const float *w, float *out,
                               int nrows, hipStream_t st);
+// m steps in one cooperative launch at slab depth zt (16/8/4/2);
+// hipErrorNotSupported where the device cannot launch it cooperatively.
+hipError_t launch_fk_persistent(const PersistArgs &p, int zt, hipStream_t st);
 // sprm: per-sample Dw, Dg (ensembles) or null for (Dw, Dg)
 // frow[slot*6+f] = sfbase[sample(slot)] + fbase[slot*6+f] for the listed slots
 hipError_t launch_work_records(const int *work, int nwork, const int *fnbr, const int *frow, const int *level,

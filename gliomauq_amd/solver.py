@@ -246,6 +246,23 @@ def set_step_fusion(ctx, on=True):
     ctx.call("gq_set_step_fusion", 1 if on else 0)
 
 
+def set_step_persistent(ctx, mode=2):
+    """Persistent chunks of steps (one cooperative launch per chunk between
+    adaptations, grid-wide barriers between the face-ghost and step phases):
+    0 never (the context's default, measured faster), 1 an autotuner
+    candidate, 2 always.  Same bits in every mode."""
+    ctx.call("gq_set_step_persistent", int(mode))
+
+
+def persistent_info(ctx):
+    """(last step call ran persistent chunks?, autotuner's persistent timing
+    in us per step or 0)."""
+    used = C.c_int()
+    us = C.c_double()
+    ctx.call("gq_persistent_info", C.byref(used), C.byref(us))
+    return bool(used.value), us.value
+
+
 def autotune_info(ctx):
     """(slab depth, face ghosts in the step?, {"<depth>": us, "<depth>f": us})
     of the last step call and the last autotune timings."""

@@ -275,6 +275,20 @@ gq_status gq_set_step_coeffs(gq_ctx *ctx, int precomputed);
  * arithmetic).  Both give the same bits.  Needs the precomputed step
  * coefficients. */
 gq_status gq_set_step_fusion(gq_ctx *ctx, int on);
+/* Persistent step chunks (no reference counterpart; the reference runs one
+ * pipeline_process/process pair per step, Glioma_HG_UQ.cpp:266-277).  A chunk
+ * of up to 16 steps between adaptations runs as one cooperative launch whose
+ * workgroups stay resident, with grid-wide barriers between the face-ghost
+ * and step phases of every step.  mode 0 (default): never -- measured
+ * slower on MI355X, where each grid-wide barrier costs more than a kernel
+ * launch; 1: an autotuner candidate, timed at the fastest slab depth beside
+ * one launch per step and kept where faster; 2: always (error where the
+ * device cannot launch it cooperatively; slab depths 16-2).  Every mode
+ * gives the same bits. */
+gq_status gq_set_step_persistent(gq_ctx *ctx, int mode);
+/* Whether the last step call ran persistent chunks, and the autotuner's last
+ * timing of the persistent candidate in us per step (0 if not timed). */
+gq_status gq_persistent_info(gq_ctx *ctx, int *used, double *tuned_us);
 /* Slab depth and face-ghost mode (1 = evaluated in the step) used by the last
  * step call, and the last autotune timings in us per step: [0..4] depths 16,
  * 8, 4, 2, 1 with the separate face-ghost launch (both kernels timed), [5..9]

@@ -169,6 +169,31 @@ def test_ensemble_fused_steps_uneven_counts(inputs):
             assert o[s].tobytes() == out[0][s].tobytes(), "sample %d differs" % s
 
 
+def test_ensemble_persistent_bit_identical(inputs):
+    """Adaptive ensemble with uneven step counts: chunks as persistent
+    cooperative launches (the running prefix shrinks inside a launch) give
+    the same fields, topologies and log-likelihoods as graph-captured
+    per-step launches."""
+    pwm, pgm, ics, obs = inputs
+    steps = [23, 37, 12]
+    out = []
+    for persist in (0, 2):
+        ctx = gq.Context(0)
+        gq.set_step_persistent(ctx, persist)
+        ens = make_ensemble(ctx, pwm, pgm, ics[:3])
+        prm = np.array([sample_params(s) for s in range(3)])
+        ens.run(prm, steps, adapt_every=K, tol_refine=TOL_R, tol_compress=TOL_C, graph=True)
+        assert gq.persistent_info(ctx)[0] == (persist == 2)
+        out.append(([ens.sample(s).dump_uniform() for s in range(3)], ens.block_counts().tolist(),
+                    ens.loglik_flip(*obs)))
+        ctx.close()
+    (ua, ca, la), (ub, cb, lb) = out
+    assert ca == cb
+    for s in range(3):
+        assert ua[s].tobytes() == ub[s].tobytes(), "sample %d differs" % s
+    assert np.array_equal(la, lb)
+
+
 def test_ensemble_uniform_vs_oracle(inputs):
     pwm, pgm, ics, _ = inputs
     ctx = gq.Context(0)
