@@ -1373,7 +1373,9 @@ __global__ __launch_bounds__(512) void k_fwt_detail(LabArgs a, const double *__r
 // lab(s) * ((wZ*wY)*wX), w = Hs(2s - d), for phi, pw and pg.  One workgroup
 // per (parent, child, field): the child's 16^3 points read parent points
 // m-1..m+2 (Interp4) or m..m+1 (Interp2) around m = d/2, an 11^3 (9^3)
-// window of the parent lab staged in LDS.
+// window of the parent lab staged in LDS.  Four workgroups per child (one
+// per quarter of its z range); each wave takes the points of one (x, y)
+// parity class, so the tap loops of its lanes have the same trip counts.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restrict__ ghost3, float *phi, float *pw,
                                                float *pg, const int *__restrict__ parents,
@@ -1387,7 +1389,7 @@ __global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restri
     __shared__ double win[WL * WL * WL];
     const int pidx = blockIdx.x;
     const int child = blockIdx.y;
-    const int f = blockIdx.z;
+    const int f = blockIdx.z >> 2, zq = blockIdx.z & 3;
     const int pslot = parents[pidx];
     const int cx = child & 1, cy = (child >> 1) & 1, cz = child >> 2;
     float *fld = f == 0 ? phi : (f == 1 ? pw : pg);
@@ -1433,8 +1435,11 @@ __global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restri
     }
     __syncthreads();
     const int cs = children[pidx * 8 + child];
-    for (int q = threadIdx.x; q < B3; q += 256) {
-        const int lx = q & 15, ly = (q >> 4) & 15, lz = q >> 8;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lx = 2 * (lane & 7) + (wave & 1), ly = 2 * (lane >> 3) + (wave >> 1);
+    for (int it = 0; it < 4; it++) {
+        const int lz = 4 * zq + it;
+        const int q = lx + B * (ly + B * lz);
         // child point in parent-lab coordinates: m = (16*c + l) / 2, parity of l
         const int mx = 8 * cx + (lx >> 1), my = 8 * cy + (ly >> 1), mz = 8 * cz + (lz >> 1);
         const int px = lx & 1, py = ly & 1, pz = lz & 1;
@@ -1831,7 +1836,7 @@ hipError_t launch_split(int order, const LabArgs &a, const double *ghost3, float
                         const int *parents, const int *children, int nparents, hipStream_t st)
 {
     if (nparents <= 0) return hipSuccess;
-    const dim3 grid(nparents, 8, 3);
+    const dim3 grid(nparents, 8, 3 * 4);  // (parent, child, field x z quarter)
     if (order == 4)
         hipLaunchKernelGGL(k_split<4>, grid, dim3(256), 0, st, a, ghost3, phi, pw, pg, parents, children, nparents);
     else
