@@ -676,7 +676,9 @@ def main():
     ap.add_argument("--persistent", type=int, default=0, choices=[0, 1, 2],
                     help="persistent step chunks (cooperative launch per chunk): 0 never, 1 autotuner candidate, 2 always")
     ap.add_argument("--autotune", type=int, default=1,
-                    help="step autotune mode (1: slab depths; 4: also the column kernel and in-step face ghosts)")
+                    help="step autotune mode (1: slab depths; 4: also the column kernel and in-step face ghosts; "
+                         "2: the slab depth --zt)")
+    ap.add_argument("--zt", type=int, default=16, help="slab depth of --autotune 2/3 (16, 8, 4, 2, 1)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (samples would shard across GPUs; "
@@ -705,7 +707,7 @@ def main():
     tol_r, tol_c = 1e-3, 1e-4
 
     ctx = gq.Context(0)
-    gq.set_autotune(ctx, args.autotune)
+    gq.set_autotune(ctx, args.autotune, args.zt)
     gq.set_step_persistent(ctx, args.persistent)
 
     def fresh_grid():

@@ -877,11 +877,8 @@ struct gq_ctx {
         int bucket = 0;
         while ((1 << bucket) < nwork) bucket++;
         auto it = zt_cache.find(bucket);
-        if (it != zt_cache.end()) return it->second;
-        if (!autotune) {
-            const int zt = zt_heuristic(nwork);
-            return make_cfg(zt, false, persist_mode == 2 && zt >= 2);
-        }
+        if (it != zt_cache.end()) return with_persist(it->second);
+        if (!autotune) return with_persist(make_cfg(zt_heuristic(nwork), false));
         ScopeTimer tm(t_tune);
         static const int cand[5] = {16, 8, 4, 2, 1};
         hipEvent_t e0, e1;
@@ -955,9 +952,17 @@ struct gq_ctx {
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        if (persist_mode == 2 && bzt >= 2) best = make_cfg(bzt, cfg_fused(best), true);
         zt_cache[bucket] = best;
-        return best;
+        return with_persist(best);
+    }
+
+    // the persistent-chunk mode applied to a cached or chosen configuration
+    // (mode 1 keeps what the autotuner measured faster)
+    int with_persist(int cfg) const
+    {
+        if (persist_mode == 0 || cfg_zt(cfg) < 2) return cfg & ~64;
+        if (persist_mode == 2) return cfg | 64;
+        return cfg;
     }
 
     gq::StepArgs step_args(const gq_fk_params *p)
@@ -2048,7 +2053,7 @@ gq_status gq_set_step_persistent(gq_ctx *ctx, int mode)
     return guard(ctx, [&] {
         if (mode < 0 || mode > 2) throw gq_error(GQ_ERR_ARG, "persistent mode must be 0, 1 or 2");
         ctx->persist_mode = mode;
-        ctx->zt_cache.clear();
+        if (mode == 1 && ctx->autotune) ctx->zt_cache.clear();  // time the candidate per bucket
     });
 }
 

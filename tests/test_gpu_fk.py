@@ -190,6 +190,8 @@ def test_persistent_chunks_bit_identical():
               % (persist, mode, zt, used, us, gq.autotune_info(ctx)[2]))
         if persist != 1:  # mode 1 keeps whichever the autotuner measured faster
             assert used == (persist == 2)
+        if mode in (2, 3):  # the fixed slab depth holds whatever the persistent mode
+            assert gq.autotune_info(ctx)[:2] == (zt, mode == 3)
         outs.append((g.n_blocks(), g.read("phi")))
         ctx.close()
     assert 8 < outs[0][0] < 64
