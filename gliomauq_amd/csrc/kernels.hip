@@ -1681,12 +1681,15 @@ hipError_t launch_fk_step(const StepArgs &a, int zt, hipStream_t st, hipEvent_t 
 template <int ZT>
 static hipError_t persistent_zt(const PersistArgs &p, hipStream_t st, int max_items)
 {
-    static int slots[17] = {0};  // resident workgroups per CU, by slab depth
-    static int ncu = 0;
-    if (!slots[ZT]) {
-        int dev = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e != hipSuccess) return e;
+    constexpr int MAXDEV = 64;
+    static int slots[MAXDEV] = {0};  // resident workgroups per CU of this slab depth, by device
+    static int ncus[MAXDEV] = {0};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAXDEV) return hipErrorNotSupported;
+    int &ncu = ncus[dev];
+    if (!slots[dev]) {
         int coop = 0;
         e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
         if (e != hipSuccess) return e;
@@ -1698,9 +1701,9 @@ static hipError_t persistent_zt(const PersistArgs &p, hipStream_t st, int max_it
                                                          0);
         if (e != hipSuccess) return e;
         if (n <= 0) return hipErrorNotSupported;
-        slots[ZT] = n;
+        slots[dev] = n;
     }
-    int grid = slots[ZT] * ncu;
+    int grid = slots[dev] * ncu;
     grid = std::min(grid, std::max(8, (max_items + 7) & ~7));
     grid &= ~7;
     if (grid < 8) return hipErrorNotSupported;
