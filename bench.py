@@ -749,7 +749,8 @@ def main():
         stats["kernel_ms"] += ms
         stats["cells"] += cells
         stats["launches"] += k
-        log("[bench] step %d/%d blocks %d" % (done, args.steps, g.n_blocks()))
+        if done % 100 == 0 or done == args.steps:
+            log("[bench] step %d/%d blocks %d" % (done, args.steps, g.n_blocks()))
 
     log("[bench] timed: %d steps" % args.steps)
     hs0 = gq.host_stats(ctx)
