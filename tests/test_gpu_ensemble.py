@@ -215,6 +215,28 @@ def test_ensemble_uniform_vs_oracle(inputs):
     ctx.close()
 
 
+def test_ensemble_zero_step_samples(inputs):
+    """Samples with zero steps beside a running one, and an ensemble where no
+    sample steps: the idle samples keep their initial state to the bit, the
+    running one equals its single run."""
+    pwm, pgm, ics, _ = inputs
+    ctx = gq.Context(0)
+    prm = np.array([sample_params(s) for s in range(3)])
+    ens = make_ensemble(ctx, pwm, pgm, ics[:3])
+    u_init = [ens.sample(s).dump_uniform() for s in range(3)]
+    ens.run(prm, [0, 0, 0], adapt_every=0)
+    for s in range(3):
+        assert ens.sample(s).dump_uniform().tobytes() == u_init[s].tobytes()
+    ens.run(prm, [0, 5, 0], adapt_every=0)
+    assert ens.sample(0).dump_uniform().tobytes() == u_init[0].tobytes()
+    assert ens.sample(2).dump_uniform().tobytes() == u_init[2].tobytes()
+    u1 = ens.sample(1).dump_uniform()
+    g = run_single(ctx, pwm, pgm, ics[1], sample_params(1), 5, adaptive=False)
+    assert np.array_equal(g.dump_uniform(), u1)
+    assert not np.array_equal(u1, u_init[1])
+    ctx.close()
+
+
 def test_ensemble_rejects_bad_parameters(inputs):
     pwm, pgm, ics, _ = inputs
     ctx = gq.Context(0)

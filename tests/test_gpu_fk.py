@@ -284,3 +284,42 @@ def test_neumann_doubling_matches_oracle(ctx):
     ref = O.fk_uniform_steps(phi.astype(np.float64), pw.astype(np.float64), pg.astype(np.float64), 16, 1.0 / n,
                              np.float32(Dw), np.float32(Dg), np.float32(rho), dt, steps, mode="f64")
     assert rel_l2(u, ref) <= 1e-5 and np.max(np.abs(u - ref)) <= 1e-5
+
+
+@pytest.mark.parametrize("bpd", [1, 2])
+def test_tiny_periodic_grids_and_zero_steps(bpd):
+    """Degenerate sizes: one block (its own neighbour across every face of the
+    periodic domain) and two blocks per dimension, at every slab depth, against
+    the FP64 oracle; zero steps leave the field unchanged to the bit."""
+    n = 16 * bpd
+    rng = np.random.default_rng(bpd)
+    z, y, x = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    pw = (0.5 + 0.4 * np.sin(2 * np.pi * x / n) * np.cos(2 * np.pi * z / n)).astype(np.float32)
+    pg = (1 - pw).astype(np.float32)
+    pw[2:4, 5:7, 1:3] = 0  # a CSF pocket
+    pg[2:4, 5:7, 1:3] = 0
+    phi = (0.4 * np.exp(-((x - n / 2) ** 2 + (y - 3) ** 2 + (z - n / 2) ** 2) / 10.0)).astype(np.float32)
+    phi += (0.01 * rng.random(phi.shape)).astype(np.float32)
+    h = 1.0 / n
+    Dw = 2e-4
+    Dg, rho = 0.1 * Dw, 0.4
+    dt = 0.9 * h * h / (6 * Dw)
+    steps = 9
+    ref = O.fk_uniform_steps(phi.astype(np.float64), pw.astype(np.float64), pg.astype(np.float64), 16, h, Dw, Dg,
+                             rho, dt, steps)
+    for zt in (16, 8, 4, 2, 1):
+        ctx = gq.Context(0)
+        gq.set_autotune(ctx, 2, zt)
+        g = gq.Grid(ctx, bpd, int(np.log2(bpd)), 4)
+        for name, v in (("phi", phi), ("pw", pw), ("pg", pg)):
+            g.sample_volume(name, v)
+        u_before = g.dump_uniform()
+        gq.fk_steps(g, Dw, Dg, rho, dt, 0)
+        assert g.dump_uniform().tobytes() == u_before.tobytes()
+        gq.fk_steps(g, Dw, Dg, rho, dt, steps)
+        u = g.dump_uniform()
+        e2 = rel_l2(u, ref)
+        print("bpd %d zt %d: rel L2 %.3e" % (bpd, zt, e2))
+        assert e2 <= 1e-5 and float(np.max(np.abs(u - ref))) <= 1e-5
+        assert zero_mismatch(u, ref) == 0
+        ctx.close()
