@@ -276,6 +276,12 @@ struct gq_ctx {
     // captured step chunks
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // everything the instantiated graph captured (launch arguments, buffers,
+    // per-step prefixes, configuration, start buffer): an identical key
+    // relaunches it without a new capture
+    std::vector<uint64_t> graph_key;
+    std::vector<int> graph_buf_after;
+    int graph_final_c = 0, graph_nlaunch = 0;
 
     ~gq_ctx()
     {
@@ -1077,7 +1083,36 @@ struct gq_ctx {
             final_c = c;
         };
         double total = 0.0;
-        if (use_graph && !persist) {  // a cooperative launch is not captured: one launch per chunk anyway
+        std::vector<uint64_t> key;
+        if (use_graph && !persist) {
+            auto u = [&](uint64_t v) { key.push_back(v); };
+            auto pt = [&](const void *q) { key.push_back((uint64_t)(uintptr_t)q); };
+            auto fb = [&](float f) {
+                uint32_t b;
+                std::memcpy(&b, &f, 4);
+                key.push_back(b);
+            };
+            pt(a.pw), pt(a.pg), pt(a.work), pt(a.fnbr), pt(a.frow), pt(a.fik), pt(a.ikc), pt(a.gphi), pt(a.fptr);
+            pt(a.fsrc), pt(a.fw), pt(a.level), pt(a.ih2_lvl), pt(a.slot_sample), pt(a.sprm), pt(a.wrec), pt(a.nbr26);
+            fb(a.Dw), fb(a.Dg), fb(a.rho), fb(a.dt);
+            pt(phi[0]), pt(phi[1]), pt(d_face_ptr.p), pt(d_face_src.p), pt(d_face_w.p), pt(d_gphi.p);
+            u((uint64_t)cfg), u((uint64_t)cur), u((uint64_t)can_fuse), u((uint64_t)m);
+            for (int j = 0; j < m; j++) u(((uint64_t)(uint32_t)nwork[j] << 32) | (uint32_t)nrows[j]);
+        }
+        if (use_graph && !persist && graph_exec && key == graph_key) {
+            // same launches as the instantiated graph: relaunch it
+            buf_after = graph_buf_after;
+            final_c = graph_final_c;
+            nlaunch = graph_nlaunch;
+            HIPCHK(hipEventRecord(events[0], st));
+            HIPCHK(hipGraphLaunch(graph_exec, st));
+            HIPCHK(hipEventRecord(events[1], st));
+            HIPCHK(hipStreamSynchronize(st));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, events[0], events[1]));
+            total = ms;
+        } else if (use_graph && !persist) {  // a cooperative launch is not captured: one launch per chunk anyway
+            graph_key.clear();
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
@@ -1103,6 +1138,10 @@ struct gq_ctx {
             if (!updated) HIPCHK(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
             if (graph) (void)hipGraphDestroy(graph);
             graph = g;
+            graph_key = key;
+            graph_buf_after = buf_after;
+            graph_final_c = final_c;
+            graph_nlaunch = nlaunch;
             HIPCHK(hipEventRecord(events[0], st));
             HIPCHK(hipGraphLaunch(graph_exec, st));
             HIPCHK(hipEventRecord(events[1], st));
@@ -1241,6 +1280,7 @@ static void create_samples(gq_ctx *ctx, int nsamples, int bpd, int max_level, in
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     ctx->graph_exec = nullptr;
     ctx->graph = nullptr;
+    ctx->graph_key.clear();
     ctx->tables_valid = false;
     ctx->fik_valid = false;
     ctx->h_fnbr.clear();

@@ -215,6 +215,31 @@ def test_ensemble_uniform_vs_oracle(inputs):
     ctx.close()
 
 
+def test_ensemble_graph_relaunch_bit_identical(inputs):
+    """Repeated runs whose launches are unchanged (same step counts, same
+    topology) relaunch the instantiated graph without a new capture, with
+    new per-sample parameters and fields each time (as a sampler does): the
+    results equal runs without graphs to the bit."""
+    pwm, pgm, ics, _ = inputs
+    steps = [5, 8, 3]
+    runs = [np.array([sample_params((s + k) % 4) for s in range(3)]) for k in range(3)]
+    out = []
+    for graph in (False, True):
+        ctx = gq.Context(0)
+        ens = make_ensemble(ctx, pwm, pgm, ics[:3])
+        res = []
+        for k, prm in enumerate(runs):
+            for s in range(3):
+                ens.write_volume("phi", ics[(s + k) % 3], sample=s)
+            ens.run(prm, steps, adapt_every=0, graph=graph)
+            res.append([ens.sample(s).dump_uniform() for s in range(3)])
+        out.append(res)
+        ctx.close()
+    for k in range(3):
+        for s in range(3):
+            assert out[0][k][s].tobytes() == out[1][k][s].tobytes(), "run %d sample %d differs" % (k, s)
+
+
 def test_ensemble_zero_step_samples(inputs):
     """Samples with zero steps beside a running one, and an ensemble where no
     sample steps: the idle samples keep their initial state to the bit, the
