@@ -161,7 +161,10 @@ int main(int argc, const char **argv)
         float Dg = (float)(0.1 * Dw);
         const double h = 1. / (16 * bpd);
         const double dt = cfl * h * h / (2. * 3 * std::max(Dw, Dg));
-        if (verbose) std::printf("Dg=%e, Dw=%e, dt= %f, rho=%f , h=%f\n", Dg, Dw, dt, rho_in, h);
+        if (verbose) {
+            std::printf("Dg=%e, Dw=%e, dt= %f, rho=%f , h=%f\n", Dg, Dw, dt, rho_in, h);
+            std::printf("params Dw=%.9g Dg=%.9g rho=%.9g dt=%.17g\n", Dw, Dg, rho_in, dt);
+        }
         gq::mrag::Glioma_ReactionDiffusionOperator rhs(Dw, Dg, rho_in);
         gq::mrag::UpdateTumor up(dt);
         double t = 0.0, when = dumpfreq;

@@ -337,11 +337,16 @@ class Glioma_HG_UQ:
         self.history = []
 
     def params(self, Dw_cm2_day, rho_day):
-        Dw = Dw_cm2_day / (self.L * self.L)
-        Dg = 0.1 * Dw
+        """(Dw, Dg, rho, dt) as Glioma_HG_UQ::run forms them (:519-539): the
+        inputs are read into Real = float, Dw/L^2 and 0.1*Dw are evaluated in
+        double and stored as float, dt is formed in double from those floats."""
+        f32 = np.float32
+        Dw = float(f32(float(f32(Dw_cm2_day)) / (self.L * self.L)))
+        Dg = float(f32(0.1 * Dw))
+        rho = float(f32(rho_day))
         h = 1.0 / (BS * self.grid.bpd)
         dt = self.cfl * h * h / (2.0 * 3 * max(Dw, Dg))
-        return Dw, Dg, rho_day, dt
+        return Dw, Dg, rho, dt
 
     def _adapt(self):
         g = self.grid

@@ -18,7 +18,12 @@
  *   - og_* (ghost expansion) against the reference's MRAG_BBInfoCreator
  *     compiled from source in oracle/_ref on toy hierarchies, plus
  *     polynomial exactness of the interpolating wavelets.
- *   - fwt_* against polynomial annihilation and single-spike responses.
+ *   - fwt_* against the mathematics (tests/test_oracle_math_pins.py):
+ *     details vanish on polynomials of degree < order per variable, equal
+ *     the Lagrange interpolation remainder on x^order and x^(order+1), and
+ *     single spikes give the Ga taps; code 0 is excluded from the maximum.
+ *   - dump_uniform_f64 against an independent covering-cell formulation
+ *     (replication factors 1/2/4 and the holes of coarser leaves).
  *   - lik_ref_* against the reference's likelihood program compiled from
  *     source in oracle/_ref, plus closed forms.
  *   - lik_flip_* against closed forms (constant fields, brute force).

@@ -2,7 +2,8 @@
 
 oracle/_ref holds the reference's own code compiled from /root/reference by
 oracle/Makefile (`make -C oracle ref`): the FK operator (Real=float and
-Real=double), the MRAG ghost-table builder and the likelihood program.
+Real=double), the MRAG ghost-table builder, the refine/compress planners and
+the likelihood program.
 Tests skip when these are absent.
 """
 import ctypes as C
@@ -75,6 +76,41 @@ def ghost_terms(levels, indices, target, order, B, s0, s1, cap=1 << 22):
     assert n == ng
     m = off[ng]
     return off, leaf[:m].copy(), cell[:m].copy(), w[:m].copy()
+
+
+def _plan_lib():
+    L = _load("libgqref_plan.so")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    u8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+    L.gqref_plan_refine.argtypes = [C.c_int, ip, ip, u8, C.c_int, C.c_int, ip]
+    L.gqref_plan_compress.argtypes = [C.c_int, ip, ip, u8, C.c_int, ip]
+    return L
+
+
+def plan_refine(keys, flagged, max_level):
+    """Leaves (level, x, y, z) that the reference's Refiner_SpaceExtension(1,
+    max_level) plan splits, for the leaf list `keys` with the set `flagged`."""
+    lv = np.ascontiguousarray([k[0] for k in keys], np.int32)
+    ix = np.ascontiguousarray([k[1:] for k in keys], np.int32).reshape(-1)
+    fl = np.ascontiguousarray([k in flagged for k in keys], np.uint8)
+    out = np.zeros(len(keys) + 1, np.int32)
+    n = _plan_lib().gqref_plan_refine(len(keys), lv, ix, fl, max_level, len(out), out)
+    if n < 0:
+        raise RuntimeError("reference refine planner harness failed: %d" % n)
+    return {keys[i] for i in out[:n]}
+
+
+def plan_compress(keys, flagged):
+    """Parents (level, x, y, z) that the reference's Compressor(1) plan
+    collapses, for the leaf list `keys` with the set `flagged`."""
+    lv = np.ascontiguousarray([k[0] for k in keys], np.int32)
+    ix = np.ascontiguousarray([k[1:] for k in keys], np.int32).reshape(-1)
+    fl = np.ascontiguousarray([k in flagged for k in keys], np.uint8)
+    out = np.zeros(4 * (len(keys) + 1), np.int32)
+    n = _plan_lib().gqref_plan_compress(len(keys), lv, ix, fl, len(keys) + 1, out)
+    if n < 0:
+        raise RuntimeError("reference compress planner harness failed: %d" % n)
+    return {tuple(int(v) for v in out[4 * i:4 * i + 4]) for i in range(n)}
 
 
 def write_matrix3d(path, arr):

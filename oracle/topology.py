@@ -14,6 +14,12 @@ product's C++ host code, of:
   * BlockCollapser::_collapseBlocks      MRAGcore/MRAGBlockCollapser.h:24-85
   * Science::AutomaticRefinement/Compression  MRAGscience/MRAGScienceCore.h:42-177
 
+Pins (tests/test_oracle_planner.py, tests/test_oracle_math_pins.py): the
+planners plan for plan against the reference's Refiner, Refiner_SpaceExtension
+and Compressor compiled from source (oracle/ref_harness_plan.cpp); the split
+filter by polynomial reproduction; the AutomaticRefinement/Compression loops
+by scripted-detail known answers.
+
 Block data are numpy arrays (B,B,B) per leaf keyed by blockID, indexed [z,y,x].
 Block IDs are handed out by a monotonically increasing counter, as
 BlockCollection::_createIDs (MRAGBlockCollection.inl:93-101).
@@ -56,9 +62,9 @@ class Topology:
         return max(k[0] for k in self.leaves)
 
     def _lut(self):
-        L = max(self.max_level, self.finest())
-        sig = (L, len(self.leaves), self._next)
+        sig = (id(self.leaves), len(self.leaves), self._next)
         if getattr(self, "_lut_sig", None) != sig:
+            L = max(self.max_level, self.finest())
             nF = 1 << L
             lut = np.empty((nF, nF, nF), object)
             for k in self.leaves:

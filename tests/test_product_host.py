@@ -106,3 +106,23 @@ def test_product_ghost_expansion_equals_oracle(order):
             assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), (t, p)
             n += 1
     assert n > 1000
+
+
+def test_solver_params_round_as_reference():
+    """Glioma_HG_UQ::run reads Dw, rho into Real = float and stores Dw/L^2 and
+    0.1*Dw as float (Glioma_HG_UQ.cpp:519-533); dt is formed in double from
+    those floats (:538-539).  The Python mirror hands the same values to the
+    step as gq_brain does."""
+    from types import SimpleNamespace
+    s = gq.Glioma_HG_UQ(SimpleNamespace(bpd=16), 17.0, cfl=0.9)
+    for Dw_cm, rho in ((0.0013, 0.025), (0.002385557629465, 0.0175151508213), (1e-4 / 3, 0.1)):
+        Dw, Dg, r, dt = s.params(Dw_cm, rho)
+        for v in (Dw, Dg, r):
+            assert float(np.float32(v)) == v
+        assert Dw == float(np.float32(np.float64(np.float32(Dw_cm)) / 289.0))
+        assert Dg == float(np.float32(0.1 * Dw))
+        assert r == float(np.float32(rho))
+        h = 1.0 / 256
+        assert dt == 0.9 * h * h / (6.0 * max(Dw, Dg))
+    # the rounding matters: the double quotient differs from the float one
+    assert s.params(0.0013, 0.025)[0] != 0.0013 / 289.0
