@@ -9,14 +9,16 @@ likelihood (FP64) against synthetic observations of a ground-truth run at
 perturbed parameters (Dw x1.25, rho x0.8).
 
 A bench "step" is one FK time step of that solve (with its share of the
-adaptation work); --steps K times K steps from t = 0 (590 = T of 300 days).
-The W warmup steps run on a separate copy of the grid.  `value` is leaf
+adaptation work).  The timed job is always the complete solve to T = 300
+days (590 steps): --steps is recorded but a partial solve would time only the
+uniform start of the grid, so it is rounded up to the whole solve.  The W
+warmup steps run on a separate copy of the grid.  `value` is leaf
 cell-updates per second of wall time over the timed region (inputs resident
 in HBM).  The roofline object is for the step kernel (k_fk_step): 16
 algorithmic bytes per cell update (read u, Pwm, Pgm; write u) over its
-event-timed device duration.  The CPU baseline is the FP64 CPU oracle
-(block-wise labs, OpenMP) on the first, still uniform, steps of the same
-solve.
+event-timed device duration.  The CPU baseline is the reference's own RD
+functor and UpdateTumor compiled from source (oracle/_ref, block-wise labs,
+OpenMP) on the first, still uniform, steps of the same solve.
 """
 import argparse
 import json
@@ -37,11 +39,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def pmc_traffic_ratio():
+def pmc_traffic_ratio(workload="adaptive256"):
     """HBM traffic / algorithmic bytes of k_fk_step from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs of this bench,
     calibrated on a copy kernel of known size); None if absent."""
-    path = os.path.join(ROOT, "profiles", "k_fk_step_adaptive256_traffic.md")
+    path = os.path.join(ROOT, "profiles", "k_fk_step_%s_traffic.md" % workload)
     try:
         for line in open(path):
             if line.startswith("{") and "traffic_over_algorithmic" in line:
@@ -51,24 +53,45 @@ def pmc_traffic_ratio():
     return None
 
 
-def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, n_steps=200):
+def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, n_steps=20, what="the initial uniform grid of this solve"):
+    """CPU baseline on a bounded sample: the reference's own RD functor and
+    UpdateTumor (compiled from its sources into oracle/_ref, Real = float,
+    cells with the CCell_extended footprint, OpenMP over blocks) on uniform
+    steps of the same grid; the FP64 oracle port where oracle/_ref is absent."""
     from oracle import oracle as O
+    from oracle import ref as R
     threads = min(16, os.cpu_count() or 1)
     n = u0.shape[0]
+    f32 = np.float32
+    if R.available("libgqref_fk_f32.so"):
+        a = [np.ascontiguousarray(v, f32) for v in (u0, pwm, pgm)]
+        R.uniform_steps(a[0], a[1], a[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, 1, threads=threads)  # warm
+        t0 = time.perf_counter()
+        R.uniform_steps(a[0], a[1], a[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, n_steps + 1, threads=threads)
+        t1 = time.perf_counter()
+        R.uniform_steps(a[0], a[1], a[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, 1, threads=threads)
+        t2 = time.perf_counter()
+        wall = (t1 - t0) - (t2 - t1)  # the set-up and one step subtracted
+        m = max(16, n // 2)
+        s1 = [np.ascontiguousarray(v[:m, :m, :m], f32) for v in (u0, pwm, pgm)]
+        t3 = time.perf_counter()
+        R.uniform_steps(s1[0], s1[1], s1[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, 3, threads=1)
+        t4 = time.perf_counter()
+        R.uniform_steps(s1[0], s1[1], s1[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, 1, threads=1)
+        t5 = time.perf_counter()
+        wall1 = (t4 - t3) - (t5 - t4)
+        return {"value": n ** 3 * n_steps / wall, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+                "sample": "the reference's Glioma_ReactionDiffusionOperator + UpdateTumor compiled from source "
+                          "(Real=float, CCell_extended-sized cells, per-block labs, OpenMP over blocks), %d uniform "
+                          "steps of %s (%d^3, %.1f s)" % (n_steps, what, n, wall),
+                "single_core": {"value": m ** 3 * 2 / wall1, "cores": 1,
+                                "sample": "2 steps of a %d^3 grid (%.1f s)" % (m, wall1)}}
     a = [np.ascontiguousarray(v, np.float64) for v in (u0, pwm, pgm)]
     t0 = time.perf_counter()
     O.fk_uniform_steps(a[0], a[1], a[2], 16, 1.0 / n, Dw, Dg, rho, dt, n_steps, threads=threads)
     wall = time.perf_counter() - t0
-    # the same oracle on one core, a shorter sample
-    n1 = max(1, n_steps // 20)
-    t1 = time.perf_counter()
-    O.fk_uniform_steps(a[0], a[1], a[2], 16, 1.0 / n, Dw, Dg, rho, dt, n1, threads=1)
-    wall1 = time.perf_counter() - t1
     return {"value": n ** 3 * n_steps / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": "FP64 CPU oracle, %d uniform steps of the %d^3 initial grid of this solve (%.1f s)"
-                      % (n_steps, n, wall),
-            "single_core": {"value": n ** 3 * n1 / wall1, "cores": 1,
-                            "sample": "%d steps of the same grid (%.1f s)" % (n1, wall1)}}
+            "sample": "FP64 CPU oracle, %d uniform steps of %s (%d^3, %.1f s)" % (n_steps, what, n, wall)}
 
 
 def ensemble_workload(args):
@@ -151,11 +174,34 @@ def ensemble_workload(args):
                       "persistent": gq.persistent_info(ctx),
                       "host": {k: v - hs0[k] for k, v in gq.host_stats(ctx).items()}},
     }
+    # the step kernel's own duration: the same run again with every step
+    # launch timed by its dispatch events (no graphs), outside the timed region
     if args.kernel_timing:
-        achieved = 16.0 * cells / (st["device_ms"] * 1e-3) / 1e9
-        res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_fk_step",
-                           "algorithmic_bytes_per_cell": 16}
+        kst = st
+    else:
+        log("[bench] ensemble: kernel-timed pass")
+        ek = make()
+        ctx.call("gq_set_step_timing", 1)
+        kst = ek.run(prm, steps, adapt_every=10, tol_refine=1e-3, tol_compress=1e-4, graph=False)
+        ctx.call("gq_set_step_timing", 0)
+        del ek
+    launches = int(steps.max())  # one step launch per step over the samples still running
+    achieved = 16.0 * kst["cell_updates"] / (kst["device_ms"] * 1e-3) / 1e9
+    ratio = pmc_traffic_ratio("ensemble256")
+    bpl = 16.0 * kst["cell_updates"] / launches
+    res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS,
+                       "traffic": None if ratio is None else ratio * bpl, "traffic_over_algorithmic": ratio,
+                       "kernel": "k_fk_step", "launches": launches,
+                       "kernel_avg_us": kst["device_ms"] * 1e3 / launches,
+                       "algorithmic_bytes_per_launch": bpl, "cells_per_launch": kst["cell_updates"] / launches,
+                       "algorithmic_bytes_per_cell": 16,
+                       "timing": "every step launch timed by its dispatch events in a second, untimed run"}
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline")
+        Dw0 = 0.0013 / L_CM ** 2
+        res["cpu_baseline"] = cpu_baseline(pwm, pgm, u0_gt, Dw0, 0.1 * Dw0, 0.025, 0.9 * h * h / (6 * Dw0),
+                                           n_steps=20, what="the 128^3 anatomy of the ensemble")
     print(json.dumps(res), flush=True)
 
 
@@ -722,12 +768,16 @@ def main():
                                adapt_every=args.adapt_every)
 
     s0 = solver(fresh_grid())
-    Dw, Dg, _, dt = s0.params(Dw_cm, rho)
+    Dw, Dg, rho_f, dt = s0.params(Dw_cm, rho)
+    T_days = 300.0
+    # the timed job is the complete solve of T = 300 days: a smaller --steps
+    # is rounded up to it (a partial solve would time the uniform start only)
+    n_total = gq.n_steps_until(T_days, dt)
 
     # observations from a ground-truth solve at perturbed parameters
     log("[bench] ground truth solve")
     gt = solver(fresh_grid())
-    gt.run(Dw_cm * 1.25, rho * 0.8, args.steps * dt * (0.999999))
+    gt.run(Dw_cm * 1.25, rho * 0.8, T_days)
     u_true = gt.grid.dump_uniform()
     t1, t2, pet = A.observations(u_true, seed=args.seed)
     del gt
@@ -735,34 +785,41 @@ def main():
     # warmup on a separate grid
     log("[bench] warmup %d steps" % args.warmup)
     gw = fresh_grid()
-    gq.fk_steps(gw, Dw, Dg, rho, dt, max(args.warmup, 1))
+    gq.fk_steps(gw, Dw, Dg, rho_f, dt, max(args.warmup, 1))
     del gw
 
     g = fresh_grid()
     lik = gq.FlipLikelihood(g, t1, t2, pet)
     s = solver(g)
     gq.set_step_timing(g, True)
-    stats = {"kernel_ms": 0.0, "cells": 0, "launches": 0, "last": 0}
+    stats = {"kernel_ms": 0.0, "cells": 0, "launches": 0, "adaptive_ms": 0.0, "adaptive_cells": 0,
+             "adaptive_launches": 0}
+    uniform_blocks = bpd ** 3
 
     def after_chunk(done):
         ms, k, cells = gq.step_timing(g)
         stats["kernel_ms"] += ms
         stats["cells"] += cells
         stats["launches"] += k
-        if done % 100 == 0 or done == args.steps:
-            log("[bench] step %d/%d blocks %d" % (done, args.steps, g.n_blocks()))
+        if k and cells < uniform_blocks * 4096 * k:  # launches after the first compression
+            stats["adaptive_ms"] += ms
+            stats["adaptive_cells"] += cells
+            stats["adaptive_launches"] += k
+        if done % 100 == 0 or done == n_total:
+            log("[bench] step %d/%d blocks %d" % (done, n_total, g.n_blocks()))
 
-    log("[bench] timed: %d steps" % args.steps)
+    log("[bench] timed: %d steps (T = %g days)" % (n_total, T_days))
     hs0 = gq.host_stats(ctx)
     t_steps = 0.0
     t_adapt = 0.0
     t0 = time.perf_counter()
-    # the same schedule as Glioma_HG_UQ.run, stopped after args.steps steps
+    # Glioma_HG_UQ.run: adaptation after every full chunk of adapt_every steps,
+    # final refinement, then the likelihood
     done = 0
-    while done < args.steps:
-        m = min(args.adapt_every, args.steps - done)
+    while done < n_total:
+        m = min(args.adapt_every, n_total - done)
         ta = time.perf_counter()
-        gq.fk_steps(g, Dw, Dg, rho, dt, m)
+        gq.fk_steps(g, Dw, Dg, rho_f, dt, m)
         tb = time.perf_counter()
         t_steps += tb - ta
         done += m
@@ -771,9 +828,11 @@ def main():
             tc = time.perf_counter()
             s._adapt()
             t_adapt += time.perf_counter() - tc
+    tc = time.perf_counter()
     gq.Science.AutomaticRefinement(g, tol_r, max_level, 1)
     ll = lik.loglik()
     wall = time.perf_counter() - t0
+    t_adapt += time.perf_counter() - tc
 
     hs1 = gq.host_stats(ctx)
     zt, fused, tune = gq.autotune_info(ctx)
@@ -783,31 +842,42 @@ def main():
     achieved = 16.0 * cells / kernel_s / 1e9
     bytes_per_launch = 16.0 * cells / max(stats["launches"], 1)
     ratio = pmc_traffic_ratio()
+    ad_launch = max(stats["adaptive_launches"], 1)
     res = {
         "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": n_total,
+        "steps_requested": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": wall * 1e3 / n_total,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "wavelet-adaptive single solve, 256^3-equivalent (levels <= 4, 16^3 blocks), "
-                               "Interp4 tol 1e-3/1e-4 every 10 steps, FP32, %d steps of dt=%.4f d" % (args.steps, dt),
+        "config": {"workload": "wavelet-adaptive single solve to T = 300 days, 256^3-equivalent (levels <= 4, "
+                               "16^3 blocks), Interp4 tol 1e-3/1e-4 every 10 steps, final refinement and FP64 "
+                               "flip log-likelihood, FP32 field, %d steps of dt=%.4f d; the whole solve is timed "
+                               "whatever --steps asks" % (n_total, dt),
                    "n": n, "seed": args.seed, "final_blocks": g.n_blocks(), "loglik": float(ll[0])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (None if ratio is None else ratio * bytes_per_launch),
                      "traffic_over_algorithmic": ratio,
                      "traffic_source": "profiles/k_fk_step_adaptive256_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
-                                       "WRITE_SIZE, calibrated on a copy kernel)",
-                     "kernel": "k_fk_step", "kernel_avg_us": stats["kernel_ms"] * 1e3 / max(stats["launches"], 1),
+                                       "WRITE_SIZE on the launches after the first compression, calibrated on a "
+                                       "copy kernel)",
+                     "kernel": "k_fk_step", "launches": stats["launches"],
+                     "kernel_avg_us": stats["kernel_ms"] * 1e3 / max(stats["launches"], 1),
                      "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "algorithmic_bytes_per_cell": 16},
+                     "algorithmic_bytes_per_cell": 16,
+                     "adaptive_launches": {"launches": stats["adaptive_launches"],
+                                           "avg_us": stats["adaptive_ms"] * 1e3 / ad_launch,
+                                           "cells_per_launch": stats["adaptive_cells"] / ad_launch,
+                                           "frac": (16.0 * stats["adaptive_cells"] / max(stats["adaptive_ms"], 1e-9)
+                                                    / 1e6 / HBM_PEAK_GBS)}},
     }
     res["breakdown"] = {"fk_steps_s": t_steps, "adapt_s": t_adapt, "step_kernel_s": kernel_s,
                         "host": {k: hs1[k] - hs0[k] for k in hs1}, "slab_depth": zt,
@@ -815,7 +885,7 @@ def main():
                         "persistent": gq.persistent_info(ctx)}
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
-        res["cpu_baseline"] = cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt)
+        res["cpu_baseline"] = cpu_baseline(pwm, pgm, u0, Dw, Dg, rho_f, dt)
     print(json.dumps(res), flush=True)
 
 

@@ -273,20 +273,36 @@ struct gq_ctx {
     DevBuf<double> d_gacc, d_icp, d_icg;
     DevBuf<float> d_adj_ik;  // per-cell 1/K of the adjoint
 
-    // captured step chunks
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    // everything the instantiated graph captured (launch arguments, buffers,
-    // per-step prefixes, configuration, start buffer): an identical key
-    // relaunches it without a new capture
-    std::vector<uint64_t> graph_key;
-    std::vector<int> graph_buf_after;
-    int graph_final_c = 0, graph_nlaunch = 0;
+    // Captured step chunks, a few launch shapes at a time.  The key holds
+    // everything a capture records (launch arguments, buffers, per-step
+    // prefixes, configuration, start buffer): an identical key relaunches the
+    // instantiated graph without a new capture.  A new shape is captured and
+    // instantiated once; the least recently used entry is destroyed when the
+    // cache is full (callers alternating between shapes -- odd and even step
+    // counts, stage-2 batches of different sizes -- no longer capture on
+    // every call).  Instantiated graphs are never updated in place.
+    struct CachedGraph {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+        std::vector<int> buf_after;
+        int final_c = 0, nlaunch = 0;
+        uint64_t used = 0;
+    };
+    static constexpr int GRAPH_CACHE = 8;
+    std::vector<CachedGraph> graphs;
+    uint64_t graph_clock = 0;
+    int64_t n_captures = 0, n_relaunches = 0;
+
+    void clear_graphs()
+    {
+        for (auto &c : graphs)
+            if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        graphs.clear();
+    }
 
     ~gq_ctx()
     {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        clear_graphs();
         for (auto e : events) (void)hipEventDestroy(e);
         release_pool();
         if (st) (void)hipStreamDestroy(st);
@@ -1099,20 +1115,25 @@ struct gq_ctx {
             u((uint64_t)cfg), u((uint64_t)cur), u((uint64_t)can_fuse), u((uint64_t)m);
             for (int j = 0; j < m; j++) u(((uint64_t)(uint32_t)nwork[j] << 32) | (uint32_t)nrows[j]);
         }
-        if (use_graph && !persist && graph_exec && key == graph_key) {
-            // same launches as the instantiated graph: relaunch it
-            buf_after = graph_buf_after;
-            final_c = graph_final_c;
-            nlaunch = graph_nlaunch;
+        CachedGraph *hit = nullptr;
+        if (use_graph && !persist)
+            for (auto &c : graphs)
+                if (c.key == key) hit = &c;
+        if (hit) {
+            // same launches as an instantiated graph: relaunch it
+            hit->used = ++graph_clock;
+            n_relaunches++;
+            buf_after = hit->buf_after;
+            final_c = hit->final_c;
+            nlaunch = hit->nlaunch;
             HIPCHK(hipEventRecord(events[0], st));
-            HIPCHK(hipGraphLaunch(graph_exec, st));
+            HIPCHK(hipGraphLaunch(hit->exec, st));
             HIPCHK(hipEventRecord(events[1], st));
             HIPCHK(hipStreamSynchronize(st));
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, events[0], events[1]));
             total = ms;
         } else if (use_graph && !persist) {  // a cooperative launch is not captured: one launch per chunk anyway
-            graph_key.clear();
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
@@ -1123,27 +1144,27 @@ struct gq_ctx {
                 throw;
             }
             HIPCHK(hipStreamEndCapture(st, &g));
-            bool updated = false;
-            if (graph_exec) {
-                hipGraphExecUpdateResult res;
-                hipGraphNode_t bad = nullptr;
-                updated = hipGraphExecUpdate(graph_exec, g, &bad, &res) == hipSuccess &&
-                          res == hipGraphExecUpdateSuccess;
-                if (!updated) {
-                    (void)hipGetLastError();
-                    (void)hipGraphExecDestroy(graph_exec);
-                    graph_exec = nullptr;
-                }
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);  // the instantiated graph does not refer to its source
+            HIPCHK(ie);
+            n_captures++;
+            if ((int)graphs.size() >= GRAPH_CACHE) {  // every earlier launch has completed (synchronised)
+                auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                            [](const CachedGraph &x, const CachedGraph &y) { return x.used < y.used; });
+                (void)hipGraphExecDestroy(lru->exec);
+                graphs.erase(lru);
             }
-            if (!updated) HIPCHK(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
-            if (graph) (void)hipGraphDestroy(graph);
-            graph = g;
-            graph_key = key;
-            graph_buf_after = buf_after;
-            graph_final_c = final_c;
-            graph_nlaunch = nlaunch;
+            CachedGraph c;
+            c.key = key;
+            c.exec = ex;
+            c.buf_after = buf_after;
+            c.final_c = final_c;
+            c.nlaunch = nlaunch;
+            c.used = ++graph_clock;
+            graphs.push_back(std::move(c));
             HIPCHK(hipEventRecord(events[0], st));
-            HIPCHK(hipGraphLaunch(graph_exec, st));
+            HIPCHK(hipGraphLaunch(ex, st));
             HIPCHK(hipEventRecord(events[1], st));
             HIPCHK(hipStreamSynchronize(st));
             float ms = 0;
@@ -1276,11 +1297,7 @@ static void create_samples(gq_ctx *ctx, int nsamples, int bpd, int max_level, in
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<Sample> smp(nsamples);
     for (auto &sm : smp) sm.grid = std::make_unique<gq::Grid>(bpd, max_level, order);
-    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    ctx->graph_exec = nullptr;
-    ctx->graph = nullptr;
-    ctx->graph_key.clear();
+    ctx->clear_graphs();
     ctx->tables_valid = false;
     ctx->fik_valid = false;
     ctx->h_fnbr.clear();
@@ -2117,6 +2134,15 @@ gq_status gq_host_stats(gq_ctx *ctx, double out[8])
         out[5] = ctx->t_tune;
         out[6] = ctx->n_tables;
         out[7] = ctx->n_fwt;
+    });
+}
+
+gq_status gq_graph_info(gq_ctx *ctx, int64_t *captures, int64_t *relaunches, int *cached)
+{
+    return guard(ctx, [&] {
+        if (captures) *captures = ctx->n_captures;
+        if (relaunches) *relaunches = ctx->n_relaunches;
+        if (cached) *cached = (int)ctx->graphs.size();
     });
 }
 

@@ -281,6 +281,15 @@ def host_stats(ctx):
     return dict(zip(keys, out.tolist()))
 
 
+def graph_info(ctx):
+    """(captures, relaunches, cached graphs) of the context's step chunks."""
+    c = C.c_int64()
+    r = C.c_int64()
+    k = C.c_int()
+    ctx.call("gq_graph_info", C.byref(c), C.byref(r), C.byref(k))
+    return c.value, r.value, k.value
+
+
 class BlockFWT:
     @staticmethod
     def details(grid):

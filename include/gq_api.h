@@ -298,6 +298,12 @@ gq_status gq_autotune_info(gq_ctx *ctx, int *zt, int *fused, double us[10]);
  * FWT calls, splits, collapses, autotuning} and the counts of table builds
  * and FWT calls, accumulated over the context's life. */
 gq_status gq_host_stats(gq_ctx *ctx, double out[8]);
+/* Graph-captured step chunks of gq_ensemble_run (no reference counterpart):
+ * chunks captured and instantiated, chunks relaunched from an instantiated
+ * graph with identical launches, and graphs currently kept (at most 8, the
+ * least recently used one is destroyed first; graphs are never updated in
+ * place). */
+gq_status gq_graph_info(gq_ctx *ctx, int64_t *captures, int64_t *relaunches, int *cached);
 /* Device-memory footprint of the context (bytes) and pool capacity (blocks). */
 gq_status gq_memory_info(gq_ctx *ctx, int64_t *bytes, int *pool_blocks);
 

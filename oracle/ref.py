@@ -52,6 +52,24 @@ def update(phi, dphidt, dt_step, real="f64"):
     return p
 
 
+def uniform_steps(phi, pw, pg, h, Dw, Dg, rho, dt, n_steps, threads=0, real="f32"):
+    """n_steps of the reference's RD functor + UpdateTumor (compiled from
+    source) on a uniform periodic N^3 grid of 16^3 blocks, cells with the
+    CCell_extended footprint, OpenMP over blocks; volumes [z, y, x]."""
+    L = _load("libgqref_fk_%s.so" % real)
+    dt_ = np.float64 if real == "f64" else np.float32
+    ct = C.c_double if real == "f64" else C.c_float
+    ptr = np.ctypeslib.ndpointer(dt_, flags="C_CONTIGUOUS")
+    f = getattr(L, "gqref_uniform_steps_%s" % real)
+    f.argtypes = [ptr, ptr, ptr, C.c_int, C.c_double, ct, ct, ct, C.c_double, C.c_int, C.c_int]
+    p = np.ascontiguousarray(phi, dt_).copy()
+    rc = f(p, np.ascontiguousarray(pw, dt_), np.ascontiguousarray(pg, dt_), p.shape[0], h, Dw, Dg, rho, dt, n_steps,
+           threads)
+    if rc != 0:
+        raise RuntimeError("reference uniform driver failed: %d" % rc)
+    return p
+
+
 def ghost_terms(levels, indices, target, order, B, s0, s1, cap=1 << 22):
     """Reference MRAG_BBInfoCreator instruction lists for one block's lab.
 

@@ -240,6 +240,35 @@ def test_ensemble_graph_relaunch_bit_identical(inputs):
             assert out[0][k][s].tobytes() == out[1][k][s].tobytes(), "run %d sample %d differs" % (k, s)
 
 
+def test_ensemble_graph_cache_alternating_shapes(inputs):
+    """Runs alternating between two launch shapes (odd and even step counts,
+    so the start buffer alternates too) capture each shape once and relaunch
+    it afterwards; every result equals the run without graphs to the bit."""
+    pwm, pgm, ics, _ = inputs
+    shapes = [[5, 8, 3], [4, 7, 3]]
+    out = []
+    for graph in (False, True):
+        ctx = gq.Context(0)
+        ens = make_ensemble(ctx, pwm, pgm, ics[:3])
+        res = []
+        for k in range(10):
+            prm = np.array([sample_params((s + k) % 4) for s in range(3)])
+            for s in range(3):
+                ens.write_volume("phi", ics[(s + k) % 3], sample=s)
+            ens.run(prm, shapes[k % 2], adapt_every=0, graph=graph)
+            res.append([ens.sample(s).dump_uniform() for s in range(3)])
+        if graph:
+            cap, rel, kept = gq.graph_info(ctx)
+            print("captures %d relaunches %d cached %d" % (cap, rel, kept))
+            # two shapes x two start buffers at most
+            assert cap <= 4 and cap + rel == 10 and kept == cap
+        out.append(res)
+        ctx.close()
+    for k in range(10):
+        for s in range(3):
+            assert out[0][k][s].tobytes() == out[1][k][s].tobytes(), "run %d sample %d differs" % (k, s)
+
+
 def test_ensemble_zero_step_samples(inputs):
     """Samples with zero steps beside a running one, and an ensemble where no
     sample steps: the idle samples keep their initial state to the bit, the

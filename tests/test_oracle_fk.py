@@ -165,3 +165,29 @@ def test_harmonic_mean_symmetry():
     d = O.fk_block(phi, pw, pg, H, DW, DG, RHO)
     dm = O.fk_block(phi[:, :, ::-1].copy(), pw[:, :, ::-1].copy(), pg[:, :, ::-1].copy(), H, DW, DG, RHO)
     np.testing.assert_allclose(dm[:, :, ::-1], d, rtol=1e-13, atol=1e-18)
+
+
+@need_ref
+@pytest.mark.parametrize("real", ["f32", "f64"])
+def test_uniform_steps_bit_exact_vs_reference_driver(real):
+    """Several steps of the oracle's uniform driver against the reference's
+    functors driven block by block with CCell_extended-sized cells (the CPU
+    baseline of the benchmark), on anatomy with CSF and a tumour."""
+    rng = np.random.default_rng(5)
+    N = 32
+    phi = np.zeros((N, N, N))
+    phi[12:20, 10:18, 14:22] = rng.random((8, 8, 8)) * 0.8
+    pw = rng.random((N, N, N))
+    pg = 1 - pw
+    csf = rng.random((N, N, N)) < 0.05
+    pw[csf] = 0
+    pg[csf] = 0
+    h = 1.0 / N
+    dt = 0.9 * h * h / (6 * DW)
+    mode = "f64" if real == "f64" else "f32ref"
+    dtp = np.float64 if real == "f64" else np.float32
+    a = [x.astype(dtp) for x in (phi, pw, pg)]
+    want = R.uniform_steps(a[0], a[1], a[2], h, DW, DG, RHO, dt, 6, threads=4, real=real)
+    got = O.fk_uniform_steps(a[0], a[1], a[2], 16, h, DW, DG, RHO, dt, 6, mode=mode)
+    assert (want > 0).sum() > 1000
+    assert same(np.ascontiguousarray(got, dtp), want)
