@@ -410,6 +410,10 @@ class HGG_Likelihood:
         self.step_ti = step_ti
         pts = np.zeros((0, 3), np.int32) if points_xyz is None else np.ascontiguousarray(points_xyz, np.int32)
         self._keep = [np.ascontiguousarray(a, np.float64) for a in (pet, t1, t2)] + [pts]
+        if any(a.shape != (gpd, gpd, gpd) for a in self._keep[:3]):
+            raise ValueError("PET, T1, T2 must be volumes of the model's %d^3 grid" % gpd)
+        if pts.ndim != 2 or pts.shape[1] != 3:
+            raise ValueError("points must be an (n, 3) array")
         grid.ctx.call("gq_obs_ref_upload", gpd, ptr(self._keep[0], C.c_double), ptr(self._keep[1], C.c_double),
                       ptr(self._keep[2], C.c_double), ptr(pts, C.c_int32), len(pts))
 
@@ -486,9 +490,11 @@ class Ensemble:
     def set_observations(self, t1, t2, pet):
         """Flip-rate model observations on the gpd^3 grid or on a grid coarser
         by an integer factor (gq_obs_flip_upload)."""
-        gpd = np.asarray(t1).shape[0]
         keep = (np.ascontiguousarray(t1, np.uint8), np.ascontiguousarray(t2, np.uint8),
                 np.ascontiguousarray(pet, np.float32))
+        gpd = keep[0].shape[0]
+        if any(a.shape != (gpd, gpd, gpd) for a in keep):
+            raise ValueError("observations must be three cubic volumes of one size")
         self.ctx.call("gq_obs_flip_upload", gpd, ptr(keep[0], C.c_uint8), ptr(keep[1], C.c_uint8),
                       ptr(keep[2], C.c_float))
 
@@ -526,10 +532,14 @@ class FlipLikelihood:
     """Flip-rate Bernoulli (T1Gd, FLAIR) + Gaussian PET likelihood on the device."""
 
     def __init__(self, grid, t1, t2, pet, th1=0.7, th2=0.25, flip=0.05, sigma=0.05):
+        """t1, t2, pet: observation volumes on the model's gpd^3 grid or on a
+        grid coarser by an integer factor (gq_obs_flip_upload)."""
         self.grid = grid
-        gpd = grid.bpd * BS
         self._keep = (np.ascontiguousarray(t1, np.uint8), np.ascontiguousarray(t2, np.uint8),
                       np.ascontiguousarray(pet, np.float32))
+        gpd = self._keep[0].shape[0]
+        if any(a.shape != (gpd, gpd, gpd) for a in self._keep):
+            raise ValueError("observations must be three cubic volumes of one size")
         self.params = np.array([th1, th2, flip, sigma], np.float64)
         grid.ctx.call("gq_obs_flip_upload", gpd, ptr(self._keep[0], C.c_uint8), ptr(self._keep[1], C.c_uint8),
                       ptr(self._keep[2], C.c_float))

@@ -146,14 +146,27 @@ class LeafGrid:
             raise RuntimeError("ghost expansion overflow")
         return p
 
-    def fwt_details(self, phi):
-        """Per-leaf detail magnitude (FWTReport::getOverAll_DetailMaxMag)."""
+    def fwt_details(self, phi, which=None, threads=8):
+        """Per-leaf detail magnitude (FWTReport::getOverAll_DetailMaxMag) of
+        the leaves `which` (default all; others are left 0).  The C calls
+        release the GIL, so leaves run on `threads` host threads."""
         s0, s1 = (-2, 4) if self.order == 4 else (-1, 2)
         out = np.zeros(self.n)
         L = self.B + s1 - s0 - 1
-        for b in range(self.n):
-            lab = self.fill_lab(phi, b, s0, s1)
+        data = np.ascontiguousarray(phi, np.float64)
+        todo = range(self.n) if which is None else list(which)
+
+        def one(b):
+            lab = self.fill_lab(data, b, s0, s1)
             out[b] = lib().fwt_block_detail_f64(lab, self.B, s0, L, self.order)
+
+        if threads > 1 and len(todo) > 8:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(threads) as ex:
+                list(ex.map(one, todo))
+        else:
+            for b in todo:
+                one(b)
         return out
 
     def dump_uniform(self, phi, gpd):

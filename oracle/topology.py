@@ -190,11 +190,12 @@ class AdaptiveField:
         """blockID -> detail (FP64 oracle FWT on the current topology)."""
         keys, g = self.t.leaf_grid()
         phi = self.stacked("phi", keys)
-        det = g.fwt_details(phi)
-        out = {self.t.leaves[k]: det[i] for i, k in enumerate(keys)}
-        if ids is not None:
-            out = {i: out[i] for i in ids}
-        return out
+        pos = {self.t.leaves[k]: i for i, k in enumerate(keys)}
+        which = None if ids is None else [pos[i] for i in ids]
+        det = g.fwt_details(phi, which)
+        if ids is None:
+            return {self.t.leaves[k]: det[i] for i, k in enumerate(keys)}
+        return {i: det[pos[i]] for i in ids}
 
     def _split(self, parents):
         """BlockSplitter::_splitBlocks as a gather: every child point d gets
@@ -305,3 +306,31 @@ class AdaptiveField:
             if max_loops > 0 and loops >= max_loops:
                 break
         return total
+
+
+def run_adaptive(field, Dw, Dg, rho, dt, n_steps, adapt_every, tol_refine, tol_compress, max_level, threads=8,
+                 log=None):
+    """Glioma_HG_UQ::run (Glioma/Glioma_HG_UQ.cpp:544-575) in FP64 on the
+    oracle's adaptive field, with the adaptation after every full chunk of
+    `adapt_every` steps (AutomaticRefinement with one loop, then
+    AutomaticCompression without a loop limit) and the final refinement.
+    Returns the leaf count after every adaptation and after the final one."""
+    history = []
+    done = 0
+    while done < n_steps:
+        m = min(adapt_every, n_steps - done)
+        keys, g = field.t.leaf_grid()
+        st = {f: field.stacked(f, keys) for f in field.FIELDS}
+        phi = g.fk_steps(st["phi"], st["pw"], st["pg"], Dw, Dg, rho, dt, m, threads)
+        for i, k in enumerate(keys):
+            field.data[field.t.leaves[k]]["phi"] = phi[i]
+        done += m
+        if m == adapt_every:
+            field.automatic_refinement(tol_refine, max_level, 1)
+            field.automatic_compression(tol_compress, -1)
+            history.append(len(field.t.leaves))
+            if log:
+                log("oracle: step %d leaves %d" % (done, len(field.t.leaves)))
+    field.automatic_refinement(tol_refine, max_level, 1)
+    history.append(len(field.t.leaves))
+    return history
