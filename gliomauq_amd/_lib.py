@@ -27,7 +27,7 @@ EXPORTS = [
     "gq_topo_leaves", "gq_topo_refine", "gq_topo_compress", "gq_topo_point_terms",
     "gq_grid_write_volume", "gq_ensemble_create", "gq_ensemble_size", "gq_sample_select",
     "gq_ensemble_block_counts", "gq_ensemble_run", "gq_ensemble_stats", "gq_ensemble_loglik_flip",
-    "gq_ensemble_ic_gaussian", "gq_adjoint_flip",
+    "gq_ensemble_ic_gaussian", "gq_adjoint_flip", "gq_loglik_flip_volume_noalloc",
     "gq_sur_create", "gq_sur_destroy", "gq_sur_last_error", "gq_sur_set_anatomy", "gq_sur_param_count",
     "gq_sur_get_params", "gq_sur_set_params", "gq_sur_predict", "gq_sur_loss_grad", "gq_sur_train_step",
     "gq_sur_timing", "gq_sur_obs_flip_upload", "gq_sur_loglik_flip",
@@ -121,6 +121,7 @@ def load():
         "gq_ensemble_loglik_flip": [vp, f64p, f64p],
         "gq_ensemble_ic_gaussian": [vp, f64p],
         "gq_adjoint_flip": [vp, C.POINTER(FKParams), C.c_int, C.c_int, f64p, f64p, f64p, f64p],
+        "gq_loglik_flip_volume_noalloc": [vp, C.c_int, vp, vp, vp, C.c_int, f64p, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -239,6 +239,24 @@ gq_status gq_loglik_ref_volume(gq_ctx *ctx, const double *model, int gpd, const 
 gq_status gq_obs_flip_upload(gq_ctx *ctx, int gpd, const uint8_t *t1, const uint8_t *t2, const float *pet);
 gq_status gq_loglik_flip(gq_ctx *ctx, const double params[4], double out[4]);
 
+/* ---- no allocation ---------------------------------------------------------
+ * No reference counterpart: for callers that cannot allocate (code under HIP
+ * graph capture, tight sampler loops, harnesses that forbid hipMalloc).  The
+ * flip-rate log-likelihood of gq_loglik_flip on a model volume already in
+ * device memory, with the same kernels and the same bits: model is gpd^3
+ * doubles on the device (x fastest); t1, t2, pet are device arrays on an
+ * obs_gpd^3 grid (obs_gpd divides gpd; observation voxel o is compared with
+ * model voxel (gpd/obs_gpd)*o); out is 4 device doubles {logL, L_T1,
+ * L_FLAIR, L_PET}; stream is a hipStream_t (null: the default stream).  The
+ * call only enqueues kernels on the stream and returns: it allocates nothing
+ * and does not synchronise.  Its partial sums live in a fixed device array
+ * of the library (48 KiB per GPU, kept until the process ends): calls that
+ * overlap on one GPU race on it.  gpd is at most GQ_NOALLOC_MAX_GPD. */
+#define GQ_NOALLOC_MAX_GPD 1024
+gq_status gq_loglik_flip_volume_noalloc(const double *model, int gpd, const uint8_t *t1, const uint8_t *t2,
+                                        const float *pet, int obs_gpd, const double params[4], double *out,
+                                        void *stream);
+
 /* ---- measurement ----------------------------------------------------------
  * Device time of the last gq_fk_steps call, summed over its step-kernel
  * launches (ms), with the number of launches and leaf cells updated. */
