@@ -104,23 +104,10 @@ def ensemble_workload(args):
     n = 128
     bpd = n // 16
     S = args.samples
-    rng = np.random.default_rng(args.seed + 77)
     log("[bench] ensemble: anatomy %d^3, %d samples" % (n, S))
     pwm, pgm, u0_gt, cvox = A.config_inputs(n, L_CM, args.seed)
-    wm = np.argwhere(pwm > 0.5)
-    Dw_cm = np.exp(rng.uniform(np.log(2e-4), np.log(1.5e-2), S))
-    rho = np.exp(rng.uniform(np.log(2e-3), np.log(0.2), S))
-    T = rng.uniform(30.0, 300.0, S)
-    centers = wm[rng.integers(len(wm), size=S)]  # (z, y, x)
+    prm, steps, ic = A.ensemble_prior(S, n, L_CM, args.seed, pwm=pwm)
     h = 1.0 / n
-    prm = np.zeros((S, 4))
-    steps = np.zeros(S, np.int32)
-    for s in range(S):
-        Dw = Dw_cm[s] / L_CM ** 2
-        dt = 0.9 * h * h / (6 * Dw)
-        prm[s] = (Dw, 0.1 * Dw, rho[s], dt)
-        steps[s] = gq.n_steps_until(T[s], dt)
-    ic = np.array([[c[2], c[1], c[0], 0.1, 2.0] for c in centers], np.float64)
     ctx = gq.Context(0)
     gq.set_step_persistent(ctx, args.persistent)
     # observations of a ground-truth solve at perturbed parameters

@@ -95,3 +95,34 @@ def config_inputs(n, L=17.0, seed=0):
     c = pick_seed_voxel(pwm, n, L, seed)
     u0 = initial_condition(n, c)
     return pwm, pgm, u0, c
+
+
+def ensemble_prior(S, n, L=17.0, seed=0, pwm=None, cfl=0.9):
+    """The batched-ensemble workload's samples: S parameter sets from a seeded
+    prior -- log-uniform Dw in [2e-4, 1.5e-2] cm^2/day, log-uniform rho in
+    [2e-3, 0.2] /day, initial point uniform over the voxels with Pwm > 0.5, T
+    uniform in [30, 300] days -- as (params (S, 4) = Dw, Dg, rho, dt in
+    unit-cube units with dt at `cfl` of the finest level of n^3, step counts of
+    `while (t <= T)`, Gaussian ICs (S, 5) = x0, y0, z0, 0.1, 2 voxels)."""
+    if pwm is None:
+        pwm = anatomy(n, L, seed)[0]
+    rng = np.random.default_rng(seed + 77)
+    wm = np.argwhere(pwm > 0.5)
+    Dw_cm = np.exp(rng.uniform(np.log(2e-4), np.log(1.5e-2), S))
+    rho = np.exp(rng.uniform(np.log(2e-3), np.log(0.2), S))
+    T = rng.uniform(30.0, 300.0, S)
+    centers = wm[rng.integers(len(wm), size=S)]  # (z, y, x)
+    h = 1.0 / n
+    prm = np.zeros((S, 4))
+    steps = np.zeros(S, np.int32)
+    for s in range(S):
+        Dw = Dw_cm[s] / L ** 2
+        dt = cfl * h * h / (6 * Dw)
+        prm[s] = (Dw, 0.1 * Dw, rho[s], dt)
+        t, k = 0.0, 0
+        while t <= T[s]:
+            k += 1
+            t += dt
+        steps[s] = k
+    ic = np.array([[c[2], c[1], c[0], 0.1, 2.0] for c in centers], np.float64)
+    return prm, steps, ic

@@ -47,6 +47,8 @@ def lib():
         L.lik_flip_f64.restype = C.c_double
         L.adj_fk_f64.argtypes = [dp, dp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                  C.c_int, dp, dp, dp, dp, dp]
+        L.adj_fk_f64_ckpt.argtypes = [dp, dp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_int, C.c_int, C.c_int, dp, C.c_void_p, C.c_void_p, C.c_double, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -216,6 +218,33 @@ def adjoint(u0, pw, pg, h, Dw, Dg, rho, dt, n_steps, dLduN_fn):
         raise MemoryError("adjoint oracle allocation failed")
     ct = np.ascontiguousarray(dLduN_fn(uN), np.float64)
     rc = lib().adj_fk_f64(a[0], a[1], N, h, Dw, Dg, rho, dt, n_steps, a[2], ct, uN, d0, g)
+    if rc != 0:
+        raise MemoryError("adjoint oracle allocation failed")
+    return uN, d0, g[0], g[1]
+
+
+def adjoint_ckpt(u0, pw, pg, h, Dw, Dg, rho, dt, n_steps, dLduN_fn=None, ckpt_every=50, threads=0, pet=None,
+                 sigma=0.05):
+    """adjoint() with checkpoints every ckpt_every steps and recomputation
+    (memory of about n_steps/ckpt_every + ckpt_every states), OpenMP.  The
+    terminal cotangent is dLduN_fn(uN), or, with pet given (NaN where not
+    observed), that of the Gaussian PET term in one forward pass."""
+    N = u0.shape[0]
+    a = [np.ascontiguousarray(x, np.float64) for x in (pw, pg, u0)]
+    uN = np.zeros_like(a[2])
+    d0 = np.zeros_like(a[2])
+    g = np.zeros(2)
+    if pet is not None:
+        pf = np.ascontiguousarray(pet, np.float32)
+        rc = lib().adj_fk_f64_ckpt(a[0], a[1], N, h, Dw, Dg, rho, dt, n_steps, ckpt_every, threads, a[2], None,
+                                   pf.ctypes.data_as(C.c_void_p), sigma, uN, d0, g)
+    else:
+        rc = lib().adj_fk_f64_ckpt(a[0], a[1], N, h, Dw, Dg, rho, dt, n_steps, ckpt_every, threads, a[2], None,
+                                   None, sigma, uN, d0, g)
+        if rc == 0:
+            ct = np.ascontiguousarray(dLduN_fn(uN), np.float64)
+            rc = lib().adj_fk_f64_ckpt(a[0], a[1], N, h, Dw, Dg, rho, dt, n_steps, ckpt_every, threads, a[2],
+                                       ct.ctypes.data_as(C.c_void_p), None, sigma, uN, d0, g)
     if rc != 0:
         raise MemoryError("adjoint oracle allocation failed")
     return uN, d0, g[0], g[1]

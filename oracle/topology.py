@@ -334,3 +334,38 @@ def run_adaptive(field, Dw, Dg, rho, dt, n_steps, adapt_every, tol_refine, tol_c
     field.automatic_refinement(tol_refine, max_level, 1)
     history.append(len(field.t.leaves))
     return history
+
+
+def run_adaptive_days(field, Dw, Dg, rho, dt, T_days, dumpfreq, tol_refine, tol_compress, max_level, threads=8):
+    """Glioma_HG_UQ::run with the reference's own time schedule
+    (Glioma/Glioma_HG_UQ.cpp:541-575): `while (t <= tend)` one step, t += dt,
+    and whenever t >= whenToWrite the adaptation (refine with one loop,
+    compress without a loop limit) and whenToWrite += dumpfreq; the final
+    refinement.  Returns (steps, leaf count after every adaptation)."""
+    t, when, steps = 0.0, float(dumpfreq), 0
+    history = []
+    pending = 0
+
+    def flush(m):
+        keys, g = field.t.leaf_grid()
+        st = {f: field.stacked(f, keys) for f in field.FIELDS}
+        phi = g.fk_steps(st["phi"], st["pw"], st["pg"], Dw, Dg, rho, dt, m, threads)
+        for i, k in enumerate(keys):
+            field.data[field.t.leaves[k]]["phi"] = phi[i]
+
+    while t <= T_days:
+        pending += 1
+        t += dt
+        steps += 1
+        if t >= when:
+            flush(pending)
+            pending = 0
+            field.automatic_refinement(tol_refine, max_level, 1)
+            field.automatic_compression(tol_compress, -1)
+            history.append(len(field.t.leaves))
+            when += dumpfreq
+    if pending:
+        flush(pending)
+    field.automatic_refinement(tol_refine, max_level, 1)
+    history.append(len(field.t.leaves))
+    return steps, history

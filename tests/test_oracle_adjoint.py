@@ -106,3 +106,38 @@ def test_adjoint_of_pure_reaction_is_closed_form():
     assert np.allclose(d0, lam, rtol=1e-12, atol=1e-14)
     assert abs(gR - gr.sum()) <= 1e-10 * abs(gr.sum())
     assert abs(gD) < 1e-20
+
+
+def test_checkpointed_adjoint_equals_stored_states():
+    """adjoint_ckpt (checkpoints + recomputation, OpenMP) against adjoint
+    (every state stored): the same uN and dL/du0 to the bit, the gradient
+    sums within FP64 reordering."""
+    n, steps = 32, 37
+    rng = np.random.default_rng(9)
+    pw = rng.random((n, n, n))
+    pg = 1 - pw
+    pw[10:13, 4:7, 20:23] = 0
+    pg[10:13, 4:7, 20:23] = 0
+    u0 = O.ic_gaussian(n, 12.3, 5.2, 20.4, 0.7, 2.5)
+    pet = np.where(rng.random((n, n, n)) < 0.5, rng.random((n, n, n)), np.nan)
+
+    def ct(u):
+        c = np.zeros_like(u)
+        m = ~np.isnan(pet)
+        c[m] = (pet[m] - u[m]) / 0.05 ** 2
+        return c
+
+    Dw = 0.02 / 17.0 ** 2
+    h = 1.0 / n
+    dt = 0.5 * h * h / (6 * Dw)
+    a = O.adjoint(u0, pw, pg, h, Dw, 0.1 * Dw, 0.1, dt, steps, ct)
+    for ck in (1, 5, 50):
+        b = O.adjoint_ckpt(u0, pw, pg, h, Dw, 0.1 * Dw, 0.1, dt, steps, ct, ckpt_every=ck, threads=4)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert abs(a[2] - b[2]) <= 1e-12 * abs(a[2]) and abs(a[3] - b[3]) <= 1e-12 * abs(a[3])
+    # the PET cotangent formed inside, in one forward pass
+    c = O.adjoint_ckpt(u0, pw, pg, h, Dw, 0.1 * Dw, 0.1, dt, steps, ckpt_every=5, threads=4,
+                       pet=pet.astype(np.float32), sigma=0.05)
+    a32 = O.adjoint(u0, pw, pg, h, Dw, 0.1 * Dw, 0.1, dt, steps,
+                    lambda u: np.where(np.isnan(pet), 0.0, (pet.astype(np.float32).astype(np.float64) - u) / 0.05 ** 2))
+    assert np.array_equal(a32[1], c[1]) and abs(a32[2] - c[2]) <= 1e-12 * abs(a32[2])

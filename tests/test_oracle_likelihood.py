@@ -83,3 +83,36 @@ def test_flip_model_closed_form_and_brute_force():
     t1b.reshape(-1)[:5] ^= 1
     v2, p2 = O.lik_flip(u, t1b, t2, pet)
     assert abs((p2[0] - parts[0]) - 5 * (np.log(0.05) - np.log(0.95))) < 1e-9
+
+
+def reference_inputs_128(seed=3):
+    """Seeded 128^3 volumes matching the reference's Sphere.dat coordinates
+    (the points lie in [23, 114]^3): a tumour-like model field, PET on 40 % of
+    the voxels, T1/T2 masks."""
+    rng = np.random.default_rng(seed)
+    n = 128
+    z, y, x = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    r = np.sqrt((x - 66.0) ** 2 + (y - 70.0) ** 2 + (z - 60.0) ** 2)
+    model = 1.0 / (1.0 + np.exp((r - 25.0) / 4.0))
+    pet = np.where(rng.random((n, n, n)) < 0.4, np.clip(model + 0.05 * rng.standard_normal((n, n, n)), 0, None), 0.)
+    t1 = ((model > 0.7) ^ (rng.random((n, n, n)) < 0.05)).astype(np.float64)
+    t2 = ((model > 0.25) ^ (rng.random((n, n, n)) < 0.05)).astype(np.float64)
+    return model, pet, t1, t2
+
+
+@pytest.mark.skipif(not R.available("likelihood"), reason="oracle/_ref not built")
+def test_ref_model_with_reference_sphere_and_input():
+    """The reference's own Sphere.dat (80,160 points, 10,036 on the stride-2
+    lattice) and LikelihoodInput.txt: the oracle against the reference
+    likelihood program compiled from source, selective and all-voxel modes."""
+    fx = np.load(os.path.join(GOLD, "reference_fixtures.npz"))
+    pts = fx["sphere"].astype(np.float64)  # (3, 80160)
+    params = fx["likelihood_input"]
+    assert pts.shape == (3, 80160) and list(params) == PARAMS
+    assert int(np.sum(np.all(pts % 2 == 0, axis=0))) == 10036
+    model, pet, t1, t2 = reference_inputs_128()
+    for sel in (1, 0):
+        want, _ = R.run_likelihood(model, pet, t1, t2, params, points_xyz=np.ascontiguousarray(pts), selective=sel)
+        got, _ = O.lik_ref(model, pet, t1, t2, params, selective=bool(sel), points=np.ascontiguousarray(pts))
+        print("selective=%d: reference program %.6f, oracle %.6f" % (sel, want, -got))
+        assert abs(-got - want) < 1.5e-6 * max(1.0, abs(want) * 1e-6)
