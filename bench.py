@@ -712,6 +712,10 @@ def main():
                     help="step autotune mode (1: slab depths; 4: also the column kernel and in-step face ghosts; "
                          "2: the slab depth --zt)")
     ap.add_argument("--zt", type=int, default=16, help="slab depth of --autotune 2/3 (16, 8, 4, 2, 1)")
+    ap.add_argument("--timing", default="inline", choices=["separate", "inline"],
+                    help="step-kernel durations from a second solve (separate) or from the timed solve (inline: "
+                         "per-launch events, no graphs)")
+    ap.add_argument("--step-graphs", type=int, default=1, help="gq_fk_steps replays graphs (1) or launches (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (samples would shard across GPUs; "
@@ -775,59 +779,70 @@ def main():
     gq.fk_steps(gw, Dw, Dg, rho_f, dt, max(args.warmup, 1))
     del gw
 
-    g = fresh_grid()
-    lik = gq.FlipLikelihood(g, t1, t2, pet)
-    s = solver(g)
-    gq.set_step_timing(g, True)
-    stats = {"kernel_ms": 0.0, "cells": 0, "launches": 0, "adaptive_ms": 0.0, "adaptive_cells": 0,
-             "adaptive_launches": 0}
     uniform_blocks = bpd ** 3
 
-    def after_chunk(done):
-        ms, k, cells = gq.step_timing(g)
-        stats["kernel_ms"] += ms
-        stats["cells"] += cells
-        stats["launches"] += k
-        if k and cells < uniform_blocks * 4096 * k:  # launches after the first compression
-            stats["adaptive_ms"] += ms
-            stats["adaptive_cells"] += cells
-            stats["adaptive_launches"] += k
-        if done % 100 == 0 or done == n_total:
-            log("[bench] step %d/%d blocks %d" % (done, n_total, g.n_blocks()))
+    def solve(per_kernel, progress):
+        """The complete solve (Glioma_HG_UQ.run: adaptation after every full
+        chunk of adapt_every steps, final refinement) and the likelihood;
+        per_kernel: every step launch timed by its own dispatch events (no
+        graphs).  Returns (wall s, stats, grid, loglik)."""
+        g = fresh_grid()
+        lik = gq.FlipLikelihood(g, t1, t2, pet)
+        s = solver(g)
+        gq.set_step_timing(g, per_kernel)
+        st = {"kernel_ms": 0.0, "cells": 0, "launches": 0, "adaptive_ms": 0.0, "adaptive_cells": 0,
+              "adaptive_launches": 0, "steps_s": 0.0, "adapt_s": 0.0}
+        done = 0
+        t0 = time.perf_counter()
+        while done < n_total:
+            m = min(args.adapt_every, n_total - done)
+            ta = time.perf_counter()
+            gq.fk_steps(g, Dw, Dg, rho_f, dt, m)
+            st["steps_s"] += time.perf_counter() - ta
+            done += m
+            ms, k, cells = gq.step_timing(g)
+            st["kernel_ms"] += ms
+            st["cells"] += cells
+            st["launches"] += k
+            if k and cells < uniform_blocks * 4096 * k:  # launches after the first compression
+                st["adaptive_ms"] += ms
+                st["adaptive_cells"] += cells
+                st["adaptive_launches"] += k
+            if progress and (done % 100 == 0 or done == n_total):
+                log("[bench] step %d/%d blocks %d" % (done, n_total, g.n_blocks()))
+            if m == args.adapt_every:
+                tc = time.perf_counter()
+                s._adapt()
+                st["adapt_s"] += time.perf_counter() - tc
+        tc = time.perf_counter()
+        gq.Science.AutomaticRefinement(g, tol_r, max_level, 1)
+        ll = lik.loglik()
+        wall = time.perf_counter() - t0
+        st["adapt_s"] += time.perf_counter() - tc
+        return wall, st, g, ll
 
     log("[bench] timed: %d steps (T = %g days)" % (n_total, T_days))
+    gq.set_step_graphs(ctx, args.step_graphs)
     hs0 = gq.host_stats(ctx)
-    t_steps = 0.0
-    t_adapt = 0.0
-    t0 = time.perf_counter()
-    # Glioma_HG_UQ.run: adaptation after every full chunk of adapt_every steps,
-    # final refinement, then the likelihood
-    done = 0
-    while done < n_total:
-        m = min(args.adapt_every, n_total - done)
-        ta = time.perf_counter()
-        gq.fk_steps(g, Dw, Dg, rho_f, dt, m)
-        tb = time.perf_counter()
-        t_steps += tb - ta
-        done += m
-        after_chunk(done)
-        if m == args.adapt_every:
-            tc = time.perf_counter()
-            s._adapt()
-            t_adapt += time.perf_counter() - tc
-    tc = time.perf_counter()
-    gq.Science.AutomaticRefinement(g, tol_r, max_level, 1)
-    ll = lik.loglik()
-    wall = time.perf_counter() - t0
-    t_adapt += time.perf_counter() - tc
-
+    wall, main, g, ll = solve(args.timing == "inline", True)
     hs1 = gq.host_stats(ctx)
+    final_blocks = g.n_blocks()
+    del g
+    if args.timing == "inline":
+        stats = main
+    else:
+        # the step kernel's own launch durations: the same solve again with every step launch
+        # timed by its dispatch events (the timed solve replays graphs, which cannot time launches)
+        log("[bench] kernel-timed solve")
+        _, stats, gk, _ = solve(True, False)
+        del gk
+    cells = main["cells"]
+    t_steps, t_adapt = main["steps_s"], main["adapt_s"]
     zt, fused, tune = gq.autotune_info(ctx)
-    cells = stats["cells"]
     value = cells / wall
     kernel_s = stats["kernel_ms"] * 1e-3
-    achieved = 16.0 * cells / kernel_s / 1e9
-    bytes_per_launch = 16.0 * cells / max(stats["launches"], 1)
+    achieved = 16.0 * stats["cells"] / kernel_s / 1e9
+    bytes_per_launch = 16.0 * stats["cells"] / max(stats["launches"], 1)
     ratio = pmc_traffic_ratio()
     ad_launch = max(stats["adaptive_launches"], 1)
     res = {
@@ -848,7 +863,7 @@ def main():
                                "16^3 blocks), Interp4 tol 1e-3/1e-4 every 10 steps, final refinement and FP64 "
                                "flip log-likelihood, FP32 field, %d steps of dt=%.4f d; the whole solve is timed "
                                "whatever --steps asks" % (n_total, dt),
-                   "n": n, "seed": args.seed, "final_blocks": g.n_blocks(), "loglik": float(ll[0])},
+                   "n": n, "seed": args.seed, "final_blocks": final_blocks, "loglik": float(ll[0])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (None if ratio is None else ratio * bytes_per_launch),
@@ -857,6 +872,10 @@ def main():
                                        "WRITE_SIZE on the launches after the first compression, calibrated on a "
                                        "copy kernel)",
                      "kernel": "k_fk_step", "launches": stats["launches"],
+                     "timing": ("every step launch timed by its dispatch events in the timed solve"
+                                if args.timing == "inline" else
+                                "every step launch timed by its dispatch events in a second, identical solve "
+                                "(the timed solve replays graphs)"),
                      "kernel_avg_us": stats["kernel_ms"] * 1e3 / max(stats["launches"], 1),
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "algorithmic_bytes_per_cell": 16,
@@ -869,7 +888,8 @@ def main():
     res["breakdown"] = {"fk_steps_s": t_steps, "adapt_s": t_adapt, "step_kernel_s": kernel_s,
                         "host": {k: hs1[k] - hs0[k] for k in hs1}, "slab_depth": zt,
                         "face_ghosts_in_step": fused, "autotune_us": tune,
-                        "persistent": gq.persistent_info(ctx)}
+                        "persistent": gq.persistent_info(ctx), "graphs": list(gq.graph_info(ctx)),
+                        "timed_solve_kernel_ms": main["kernel_ms"]}
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         res["cpu_baseline"] = cpu_baseline(pwm, pgm, u0, Dw, Dg, rho_f, dt)

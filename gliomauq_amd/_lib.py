@@ -23,7 +23,7 @@ EXPORTS = [
     "gq_dump_uniform", "gq_obs_ref_upload", "gq_loglik_ref", "gq_loglik_ref_volume", "gq_obs_flip_upload", "gq_loglik_flip",
     "gq_last_step_timing", "gq_set_step_timing", "gq_memory_info", "gq_set_autotune", "gq_autotune_info",
     "gq_set_step_coeffs", "gq_set_step_fusion", "gq_set_step_persistent", "gq_persistent_info",
-    "gq_host_stats", "gq_graph_info", "gq_topo_create", "gq_topo_destroy",
+    "gq_host_stats", "gq_graph_info", "gq_set_step_graphs", "gq_topo_create", "gq_topo_destroy",
     "gq_topo_leaves", "gq_topo_refine", "gq_topo_compress", "gq_topo_point_terms",
     "gq_grid_write_volume", "gq_ensemble_create", "gq_ensemble_size", "gq_sample_select",
     "gq_ensemble_block_counts", "gq_ensemble_run", "gq_ensemble_stats", "gq_ensemble_loglik_flip",
@@ -105,6 +105,7 @@ def load():
         "gq_persistent_info": [vp, ip, f64p],
         "gq_host_stats": [vp, f64p],
         "gq_graph_info": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), ip],
+        "gq_set_step_graphs": [vp, C.c_int],
         "gq_topo_create": [C.POINTER(vp), C.c_int, C.c_int, C.c_int],
         "gq_topo_destroy": [vp],
         "gq_topo_leaves": [vp, i32p, i32p, i32p, C.c_int, ip],

@@ -148,15 +148,21 @@ int main(int argc, const char **argv)
         const int smax = std::max(GM.n[0], std::max(GM.n[1], GM.n[2]));
         const double L = smax * 0.1;  // voxel spacing 1 mm -> cm (:104)
 
-        gq::mrag::Context ctx((int)a.num("-device", 0));
-        gq::mrag::Grid grid(ctx, bpd, maxlevel, order);
+        using namespace gq::mrag;
+        Context ctx((int)a.num("-device", 0));
+        Grid<> grid(ctx, bpd, maxlevel, order);
         std::vector<float> phi, pw, pg;
         patient_anatomy(GM, WM, CSF, T1, T2, tic, grid.getBlocksInfo(), phi, pw, pg);
         grid.write(GQ_FIELD_PHI, phi);
         grid.write(GQ_FIELD_PWM, pw);
         grid.write(GQ_FIELD_PGM, pg);
 
-        // Glioma_HG_UQ::run (:511-588)
+        // Glioma_HG_UQ::run (:511-588), in the reference's own calls
+        BlockProcessing blockProcessing;
+        BlockFWT<Wavelets_Interp4thOrder, Block16> blockfwt;
+        Profiler profiler;
+        BoundaryInfo *boundaryInfo = &grid.getBoundaryInfo();
+        const int nParallelGranularity = grid.getBlocksInfo().size() <= 8 ? 1 : 4;
         float Dw = (float)(Dw_in / (L * L));
         float Dg = (float)(0.1 * Dw);
         const double h = 1. / (16 * bpd);
@@ -165,24 +171,28 @@ int main(int argc, const char **argv)
             std::printf("Dg=%e, Dw=%e, dt= %f, rho=%f , h=%f\n", Dg, Dw, dt, rho_in, h);
             std::printf("params Dw=%.9g Dg=%.9g rho=%.9g dt=%.17g\n", Dw, Dg, rho_in, dt);
         }
-        gq::mrag::Glioma_ReactionDiffusionOperator rhs(Dw, Dg, rho_in);
-        gq::mrag::UpdateTumor up(dt);
         double t = 0.0, when = dumpfreq;
         int iters = 0;
         while (t <= tend) {
-            gq::mrag::BlockProcessing::steps(grid, rhs, up, 1);
+            // _reactionDiffusionStep (:266-277)
+            std::vector<BlockInfo> vInfo = grid.getBlocksInfo();
+            const BlockCollection<Block16> &collection = grid.getBlockCollection();
+            Glioma_ReactionDiffusionOperator<3> rhs(Dw, Dg, rho_in);
+            UpdateTumor<3> updateTumor(dt);
+            blockProcessing.pipeline_process(vInfo, collection, *boundaryInfo, rhs);
+            BlockProcessing::process(vInfo, collection, updateTumor, nParallelGranularity);
             t += dt;
             iters++;
             if (t >= when) {
                 if (adaptive) {
-                    gq::mrag::Science::AutomaticRefinement(grid, tolr, maxlevel, 1);
-                    gq::mrag::Science::AutomaticCompression(grid, tolc, -1);
+                    Science::AutomaticRefinement<0, 0>(grid, blockfwt, tolr, maxlevel, 1, &profiler);
+                    Science::AutomaticCompression<0, 0>(grid, blockfwt, tolc, -1, &profiler);
                 }
                 when += dumpfreq;
                 if (verbose) std::printf("Dumping data at time t=%f (blocks %d)\n", t, grid.size());
             }
         }
-        if (adaptive) gq::mrag::Science::AutomaticRefinement(grid, tolr, maxlevel, 1);
+        if (adaptive) Science::AutomaticRefinement<0, 0>(grid, blockfwt, tolr, maxlevel, 1, &profiler);
         std::vector<double> u = grid.dumpUniform();
         const int gpd = bpd * 16;
         gq::io::write_matrix3d("HGG_data.dat", u.data(), gpd, gpd, gpd);

@@ -289,6 +289,7 @@ struct gq_ctx {
         uint64_t used = 0;
     };
     static constexpr int GRAPH_CACHE = 8;
+    bool fk_graphs = true;  // gq_fk_steps replays its chunks as graphs (unless every launch is timed)
     std::vector<CachedGraph> graphs;
     uint64_t graph_clock = 0;
     int64_t n_captures = 0, n_relaunches = 0;
@@ -1486,9 +1487,10 @@ gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
         const int chunk = 1024;
         double total = 0.0;
         int done = 0, launches = 0;
+        const bool graph = ctx->fk_graphs && !ctx->per_kernel_timing;
         while (done < n_steps) {
             const int m = std::min(chunk, n_steps - done);
-            total += ctx->advance(a, cfg, std::vector<int>(m, nwork), std::vector<int>(m, nrows), false);
+            total += ctx->advance(a, cfg, std::vector<int>(m, nwork), std::vector<int>(m, nrows), graph);
             launches += ctx->last_nlaunch;
             done += m;
         }
@@ -2144,6 +2146,11 @@ gq_status gq_graph_info(gq_ctx *ctx, int64_t *captures, int64_t *relaunches, int
         if (relaunches) *relaunches = ctx->n_relaunches;
         if (cached) *cached = (int)ctx->graphs.size();
     });
+}
+
+gq_status gq_set_step_graphs(gq_ctx *ctx, int on)
+{
+    return guard(ctx, [&] { ctx->fk_graphs = on != 0; });
 }
 
 gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel)

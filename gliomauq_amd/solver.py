@@ -223,6 +223,11 @@ def set_step_timing(grid, per_kernel):
     grid.ctx.call("gq_set_step_timing", 1 if per_kernel else 0)
 
 
+def set_step_graphs(ctx, on):
+    """gq_fk_steps as graph replays (default) or direct launches (same bits)."""
+    ctx.call("gq_set_step_graphs", 1 if on else 0)
+
+
 def set_autotune(ctx, mode=1, fixed_zt=16):
     """mode 1: autotune the step slab depth (16, 8, 4, 2 planes) with the
     cross-level face ghosts from a separate launch; 4: also the column kernel

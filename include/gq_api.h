@@ -316,7 +316,12 @@ gq_status gq_autotune_info(gq_ctx *ctx, int *zt, int *fused, double us[10]);
  * FWT calls, splits, collapses, autotuning} and the counts of table builds
  * and FWT calls, accumulated over the context's life. */
 gq_status gq_host_stats(gq_ctx *ctx, double out[8]);
-/* Graph-captured step chunks of gq_ensemble_run (no reference counterpart):
+/* on = 1 (default): gq_fk_steps captures its launches as a HIP graph and
+ * replays it (relaunching a cached graph when the launches repeat), unless
+ * every launch is timed (gq_set_step_timing 1); 0: direct launches.  Both give
+ * the same bits (no reference counterpart). */
+gq_status gq_set_step_graphs(gq_ctx *ctx, int on);
+/* Graph-captured step chunks of gq_fk_steps and gq_ensemble_run (no reference counterpart):
  * chunks captured and instantiated, chunks relaunched from an instantiated
  * graph with identical launches, and graphs currently kept (at most 8, the
  * least recently used one is destroyed first; graphs are never updated in
