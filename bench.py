@@ -53,40 +53,49 @@ def pmc_traffic_ratio(workload="adaptive256"):
     return None
 
 
-def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, n_steps=20, what="the initial uniform grid of this solve"):
+def cpu_baseline(pwm, pgm, u0, Dw, Dg, rho, dt, target_s=12.0, what="the initial uniform grid of this solve"):
     """CPU baseline on a bounded sample: the reference's own RD functor and
     UpdateTumor (compiled from its sources into oracle/_ref, Real = float,
     cells with the CCell_extended footprint, OpenMP over blocks) on uniform
-    steps of the same grid; the FP64 oracle port where oracle/_ref is absent."""
+    steps of the same grid, as many as fit about `target_s` seconds (plus a
+    single-core sample of a quarter of the grid, about a quarter of that);
+    the FP64 oracle port where oracle/_ref is absent."""
     from oracle import oracle as O
     from oracle import ref as R
     threads = min(16, os.cpu_count() or 1)
     n = u0.shape[0]
     f32 = np.float32
     if R.available("libgqref_fk_f32.so"):
+        def timed(a, nn, steps, thr):
+            # wall of `steps` steps with the set-up (one step's call minus its step) subtracted
+            t0 = time.perf_counter()
+            R.uniform_steps(a[0], a[1], a[2], 1.0 / nn, f32(Dw), f32(Dg), f32(rho), dt, steps + 1, threads=thr)
+            t1 = time.perf_counter()
+            R.uniform_steps(a[0], a[1], a[2], 1.0 / nn, f32(Dw), f32(Dg), f32(rho), dt, 1, threads=thr)
+            t2 = time.perf_counter()
+            return (t1 - t0) - (t2 - t1)
+
         a = [np.ascontiguousarray(v, f32) for v in (u0, pwm, pgm)]
-        R.uniform_steps(a[0], a[1], a[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, 1, threads=threads)  # warm
-        t0 = time.perf_counter()
-        R.uniform_steps(a[0], a[1], a[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, n_steps + 1, threads=threads)
-        t1 = time.perf_counter()
-        R.uniform_steps(a[0], a[1], a[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, 1, threads=threads)
-        t2 = time.perf_counter()
-        wall = (t1 - t0) - (t2 - t1)  # the set-up and one step subtracted
+        timed(a, n, 1, threads)  # warm
+        per = timed(a, n, 4, threads) / 4
+        n_steps = int(max(4, min(2000, target_s / max(per, 1e-6))))
+        wall = timed(a, n, n_steps, threads)
         m = max(16, n // 2)
         s1 = [np.ascontiguousarray(v[:m, :m, :m], f32) for v in (u0, pwm, pgm)]
-        t3 = time.perf_counter()
-        R.uniform_steps(s1[0], s1[1], s1[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, 3, threads=1)
-        t4 = time.perf_counter()
-        R.uniform_steps(s1[0], s1[1], s1[2], 1.0 / n, f32(Dw), f32(Dg), f32(rho), dt, 1, threads=1)
-        t5 = time.perf_counter()
-        wall1 = (t4 - t3) - (t5 - t4)
+        per1 = timed(s1, n, 2, 1) / 2
+        n1 = int(max(2, min(2000, 0.25 * target_s / max(per1, 1e-6))))
+        wall1 = timed(s1, n, n1, 1)
         return {"value": n ** 3 * n_steps / wall, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
                 "sample": "the reference's Glioma_ReactionDiffusionOperator + UpdateTumor compiled from source "
                           "(Real=float, CCell_extended-sized cells, per-block labs, OpenMP over blocks), %d uniform "
                           "steps of %s (%d^3, %.1f s)" % (n_steps, what, n, wall),
-                "single_core": {"value": m ** 3 * 2 / wall1, "cores": 1,
-                                "sample": "2 steps of a %d^3 grid (%.1f s)" % (m, wall1)}}
+                "single_core": {"value": m ** 3 * n1 / wall1, "cores": 1,
+                                "sample": "%d steps of a %d^3 grid (%.1f s)" % (n1, m, wall1)}}
     a = [np.ascontiguousarray(v, np.float64) for v in (u0, pwm, pgm)]
+    t0 = time.perf_counter()
+    O.fk_uniform_steps(a[0], a[1], a[2], 16, 1.0 / n, Dw, Dg, rho, dt, 2, threads=threads)
+    per = (time.perf_counter() - t0) / 2
+    n_steps = int(max(2, min(2000, target_s / max(per, 1e-6))))
     t0 = time.perf_counter()
     O.fk_uniform_steps(a[0], a[1], a[2], 16, 1.0 / n, Dw, Dg, rho, dt, n_steps, threads=threads)
     wall = time.perf_counter() - t0
@@ -109,6 +118,7 @@ def ensemble_workload(args):
     prm, steps, ic = A.ensemble_prior(S, n, L_CM, args.seed, pwm=pwm)
     h = 1.0 / n
     ctx = gq.Context(0)
+    gq.set_autotune(ctx, args.autotune, args.zt)
     gq.set_step_persistent(ctx, args.persistent)
     # observations of a ground-truth solve at perturbed parameters
     gt = gq.Ensemble(ctx, 1, bpd, 3, 4)
@@ -179,6 +189,8 @@ def ensemble_workload(args):
     res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS,
                        "traffic": None if ratio is None else ratio * bpl, "traffic_over_algorithmic": ratio,
+                       "traffic_source": "profiles/k_fk_step_ensemble256_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
+                                         "WRITE_SIZE over every step launch, calibrated on a copy kernel)",
                        "kernel": "k_fk_step", "launches": launches,
                        "kernel_avg_us": kst["device_ms"] * 1e3 / launches,
                        "algorithmic_bytes_per_launch": bpl, "cells_per_launch": kst["cell_updates"] / launches,
@@ -188,7 +200,7 @@ def ensemble_workload(args):
         log("[bench] cpu baseline")
         Dw0 = 0.0013 / L_CM ** 2
         res["cpu_baseline"] = cpu_baseline(pwm, pgm, u0_gt, Dw0, 0.1 * Dw0, 0.025, 0.9 * h * h / (6 * Dw0),
-                                           n_steps=20, what="the 128^3 anatomy of the ensemble")
+                                           what="the 128^3 anatomy of the ensemble")
     print(json.dumps(res), flush=True)
 
 
@@ -308,11 +320,12 @@ def mcmc_workload(args):
     ob, _, lfb = SM.rwm_batched(post_b, start, 60, chol, rng, adapt_until=60)
     start_b = ob[-1]
 
-    def timed_loop(step, state):
+    def timed_loop(step, state, budget=None):
+        budget = budget or args.mcmc_budget
         xs, acs = [], []
         t0 = time.perf_counter()
         t_log = t0
-        while time.perf_counter() - t0 < args.mcmc_budget:
+        while time.perf_counter() - t0 < budget:
             state = step(state, xs, acs)
             if time.perf_counter() - t_log > 30.0:  # progress from the sampler itself
                 t_log = time.perf_counter()
@@ -337,7 +350,8 @@ def mcmc_workload(args):
         return (o[-1], lf)
     xb, ab, wall_b = timed_loop(step_rwmb, (start_b, lfb))
 
-    log("[bench] mcmc RWM 1 chain, %.0f s" % args.mcmc_budget)
+    plain_budget = args.mcmc_plain_budget or args.mcmc_budget
+    log("[bench] mcmc RWM 1 chain, %.0f s" % plain_budget)
     out_r, acc_r, logf = SM.rwm(post_1, start[0], 200, chol, rng, adapt_until=200)
 
     def step_rwm(st, xs, acs):
@@ -345,14 +359,17 @@ def mcmc_workload(args):
         xs.append(o[:, None, :])
         acs.append(a[:, None])
         return (o[-1], lf)
-    xr, ar, wall_r = timed_loop(step_rwm, (out_r[-1], logf))
+    xr, ar, wall_r = timed_loop(step_rwm, (out_r[-1], logf), plain_budget)
 
     rates = {}
     for name, x, a, w in (("rwm_plain", xr, ar, wall_r), ("rwm_%d" % C, xb, ab, wall_b), ("mala_%d" % C, xm, am, wall_m)):
         e = min(SM.ess_autocorr(x[:, :, p]) for p in range(6))
         e_bm = min(sum(SM.ess_batch_means(x[:, k, p]) for k in range(x.shape[1])) for p in range(6))
         e_cm = min(SM.ess_chains(x[:, :, p]) for p in range(6)) if x.shape[1] > 1 else None
-        rates[name] = {"ess_per_hour": e / w * 3600, "ess": e, "ess_batch_means_sum": e_bm,
+        _, halves = split_ess_rate(x, w, SM.ess_autocorr)
+        boot = bootstrap_ess_rate(x, w, SM.ess_autocorr, rng) if x.shape[1] > 1 else None
+        rates[name] = {"ess_per_hour": e / w * 3600, "ess": e, "ess_per_hour_halves": halves,
+                       "ess_per_hour_chain_bootstrap_90": boot, "ess_batch_means_sum": e_bm,
                        "ess_per_hour_batch_means_sum": e_bm / w * 3600, "ess_chain_means": e_cm,
                        "chains": int(x.shape[1]),
                        "iterations": int(x.shape[0]), "wall_s": w, "accept": float(a.mean()),
@@ -362,6 +379,12 @@ def mcmc_workload(args):
     for k in rates:
         pm = np.array(rates[k]["posterior_mean"])
         rates[k]["speedup_vs_plain"] = rates[k]["ess_per_hour"] / max(rates["rwm_plain"]["ess_per_hour"], 1e-30)
+        lo_k = min(rates[k]["ess_per_hour_halves"] + ([rates[k]["ess_per_hour_chain_bootstrap_90"][0]]
+                                                       if rates[k]["ess_per_hour_chain_bootstrap_90"] else []))
+        hi_k = max(rates[k]["ess_per_hour_halves"] + ([rates[k]["ess_per_hour_chain_bootstrap_90"][1]]
+                                                       if rates[k]["ess_per_hour_chain_bootstrap_90"] else []))
+        pl = rates["rwm_plain"]["ess_per_hour_halves"]
+        rates[k]["speedup_vs_plain_interval"] = [lo_k / max(max(pl), 1e-30), hi_k / max(min(pl), 1e-30)]
         rates[k]["speedup_vs_plain_batch_means_sum"] = (rates[k]["ess_per_hour_batch_means_sum"] /
                                                         max(rates["rwm_plain"]["ess_per_hour_batch_means_sum"], 1e-30))
         rates[k]["mean_dev_over_plain_sd"] = (np.abs(pm - ref_mean) / np.maximum(sd_r, 1e-30)).tolist()
@@ -385,15 +408,182 @@ def mcmc_workload(args):
     print(json.dumps(res), flush=True)
 
 
+def langevin_workload(args):
+    """16 Langevin (MALA) chains over the posterior of theta = (log Dw,
+    log rho, x0, y0, z0, A) on the uniform 128^3 grid, 2000 steps, gradients
+    of the flip-rate log-likelihood from the checkpointed discrete adjoint
+    (checkpoints every 50 steps), per-chain step sizes tuned to 0.57
+    acceptance during the warmup, then --lg-iters iterations per chain.  The
+    run is resumable: after every block of iterations the chain states, step
+    sizes, preconditioner, samples, sampling wall time and RNG state go to
+    --lg-state (npz), and a later process continues from there.  The last
+    process optionally runs plain RWM (one forward solve per proposal) on the
+    same posterior for --lg-plain-budget seconds and compares ESS per hour."""
+    import gliomauq_amd as gq
+    from gliomauq_amd import anatomy as A
+    from gliomauq_amd import sampler as SM
+
+    n, steps, C, ck = args.lg_n, args.lg_steps, args.lg_chains, 50
+    pwm, pgm, u0, c = A.config_inputs(n, L_CM, args.seed)
+    h = 1.0 / n
+    dt_unit = 0.9 * h * h / (6 * 1.5e-2 / L_CM ** 2)  # stable for every Dw of the prior
+    theta_true = np.array([np.log(0.0013 * 1.25), np.log(0.025 * 0.8) + 1.0, c[2] + 0.3, c[1] - 0.4, c[0] + 0.2, 0.1])
+    lo = np.array([np.log(2e-4), np.log(2e-3), c[2] - 6, c[1] - 6, c[0] - 6, 0.02])
+    hi = np.array([np.log(1.5e-2), np.log(0.2), c[2] + 6, c[1] + 6, c[0] + 6, 0.3])
+    ctx = gq.Context(0)
+    truth = gq.Ensemble(ctx, 1, n // 16, int(np.log2(n // 16)), 4)
+    truth.write_volume("pw", pwm)
+    truth.write_volume("pg", pgm)
+    truth.ic_gaussian(np.array([[*theta_true[2:5], theta_true[5], 2.0]]))
+    Dt = np.exp(theta_true[0]) / L_CM ** 2
+    truth.run(np.array([[Dt, 0.1 * Dt, np.exp(theta_true[1]), dt_unit]]), [steps], adapt_every=0)
+    obs = A.observations(truth.sample(0).dump_uniform(), seed=args.seed)
+    del truth
+    post = SM.Posterior(gq.Context(0), pwm, pgm, obs, steps, dt_unit, lo, hi, batch=C, ckpt_every=ck)
+    state = args.lg_state
+    rng = np.random.default_rng(args.seed + 7)
+    if state and os.path.exists(state):
+        z = np.load(state)
+        th, logeps, chol = z["theta"], z["logeps"], z["chol"]
+        xs, acs = z["samples"], z["accepted"]
+        wall, warm_s, n_grad = float(z["wall_s"]), float(z["warmup_s"]), int(z["grad_evals"])
+        rng.bit_generator.state = json.loads(str(z["rng"]))
+        log("[bench] langevin: resumed from %s at %d iterations (%.0f s sampled)" % (state, len(xs), wall))
+    else:
+        t0 = time.perf_counter()
+        log("[bench] langevin: scale probe")
+        scale = SM.probe_scales(post, theta_true, drop=2.0)
+        th0 = theta_true + 0.5 * scale * rng.standard_normal((C, 6))
+        log("[bench] langevin: pilot (%d MALA iterations, diagonal preconditioner)" % args.lg_pilot)
+        pil, _, le = SM.mala(post, th0, args.lg_pilot, scale, rng, adapt_until=args.lg_pilot)
+        log("[bench] langevin: population preconditioner (100 batched RWM iterations)")
+        op, _, _ = SM.rwm_batched(post, pil[-1], 100, scale, rng, adapt_until=100)
+        chol = SM.population_cholesky(op[50:])
+        log("[bench] langevin: step-size tuning towards 0.57 (%d iterations)" % args.lg_tune)
+        ot, at, logeps = SM.mala(post, op[-1], args.lg_tune, chol, rng, adapt_until=args.lg_tune)
+        log("[bench] langevin: tuned acceptance over the last half %.3f, step sizes %s"
+            % (at[args.lg_tune // 2:].mean(), np.round(np.exp(logeps), 3).tolist()))
+        th = ot[-1]
+        xs, acs = np.zeros((0, C, 6)), np.zeros((0, C), bool)
+        wall, warm_s, n_grad = 0.0, time.perf_counter() - t0, 0
+
+    def save():
+        if state:
+            np.savez(state + ".tmp.npz", theta=th, logeps=logeps, chol=chol, samples=xs, accepted=acs, wall_s=wall,
+                     warmup_s=warm_s, grad_evals=n_grad, rng=json.dumps(rng.bit_generator.state))
+            os.replace(state + ".tmp.npz", state)
+
+    save()
+    t_call = time.perf_counter()
+    blk = 50
+    while len(xs) < args.lg_iters and time.perf_counter() - t_call < args.lg_budget:
+        k = min(blk, args.lg_iters - len(xs))
+        t0 = time.perf_counter()
+        o, a, logeps = SM.mala(post, th, k, chol, rng, logeps=logeps)  # step sizes fixed after tuning
+        wall += time.perf_counter() - t0
+        n_grad += C * (k + 1)
+        th = o[-1]
+        xs = np.concatenate([xs, o], 0)
+        acs = np.concatenate([acs, a], 0)
+        save()
+        log("[bench] langevin: %d / %d iterations, %.0f s sampled, acceptance %.3f"
+            % (len(xs), args.lg_iters, wall, acs.mean()))
+    done = len(xs) >= args.lg_iters
+    res_m = None
+    if len(xs) >= 8:
+        e = min(SM.ess_autocorr(xs[:, :, p]) for p in range(6))
+        full, halves = split_ess_rate(xs, wall, SM.ess_autocorr)
+        boot = bootstrap_ess_rate(xs, wall, SM.ess_autocorr, rng)
+        res_m = {"chains": C, "iterations": int(len(xs)), "wall_s": wall, "warmup_s": warm_s,
+                 "accept": float(acs.mean()), "accept_per_chain": acs.mean(0).tolist(),
+                 "step_sizes": np.exp(logeps).tolist(), "gradients": n_grad, "gradients_per_s": n_grad / wall,
+                 "ess": e, "ess_per_hour": full, "ess_per_hour_halves": halves,
+                 "ess_per_hour_chain_bootstrap_90": boot, "posterior_mean": xs.reshape(-1, 6).mean(0).tolist(),
+                 "posterior_sd": xs.reshape(-1, 6).std(0).tolist()}
+    res_p = None
+    if done and args.lg_plain_budget > 0:
+        log("[bench] langevin: plain RWM, 1 chain, %.0f s" % args.lg_plain_budget)
+        post_1 = SM.Posterior(gq.Context(0), pwm, pgm, obs, steps, dt_unit, lo, hi, batch=1)
+        out_r, _, logf = SM.rwm(post_1, th[0], 200, chol, rng, adapt_until=200)
+        xr, ar = [], []
+        st = out_r[-1]
+        t0 = time.perf_counter()
+        t_log = t0
+        while time.perf_counter() - t0 < args.lg_plain_budget:
+            o, a, logf = SM.rwm(post_1, st, 50, chol, rng, logf=logf)
+            st = o[-1]
+            xr.append(o[:, None, :])
+            ar.append(a[:, None])
+            if time.perf_counter() - t_log > 30.0:
+                t_log = time.perf_counter()
+                log("[bench] langevin: plain RWM %.0f s, %d iterations" % (t_log - t0, 50 * len(xr)))
+        wr = time.perf_counter() - t0
+        xr = np.concatenate(xr, 0)
+        ar = np.concatenate(ar, 0)
+        full_r, halves_r = split_ess_rate(xr, wr, SM.ess_autocorr)
+        pm_r = xr.reshape(-1, 6).mean(0)
+        sd_r = xr.reshape(-1, 6).std(0)
+        res_p = {"iterations": int(len(xr)), "wall_s": wr, "accept": float(ar.mean()),
+                 "ess": min(SM.ess_autocorr(xr[:, :, p]) for p in range(6)), "ess_per_hour": full_r,
+                 "ess_per_hour_halves": halves_r, "posterior_mean": pm_r.tolist(), "posterior_sd": sd_r.tolist()}
+        if res_m:
+            pm = np.array(res_m["posterior_mean"])
+            lo_k = min(res_m["ess_per_hour_halves"] + [res_m["ess_per_hour_chain_bootstrap_90"][0]])
+            hi_k = max(res_m["ess_per_hour_halves"] + [res_m["ess_per_hour_chain_bootstrap_90"][1]])
+            res_m["speedup_vs_plain"] = res_m["ess_per_hour"] / max(full_r, 1e-30)
+            res_m["speedup_vs_plain_interval"] = [lo_k / max(max(halves_r), 1e-30), hi_k / max(min(halves_r), 1e-30)]
+            res_m["mean_dev_over_plain_sd"] = (np.abs(pm - pm_r) / np.maximum(sd_r, 1e-30)).tolist()
+    res = {
+        "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
+        "value": res_m["ess_per_hour"] if res_m else None, "unit": "effective posterior samples/hour",
+        "n_gpus": 1, "steps": steps, "warmup": args.lg_pilot + 100 + args.lg_tune, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "Langevin (MALA) sampling: %d chains, uniform %d^3, %d steps, gradients from the "
+                               "discrete adjoint (checkpoints every %d steps), step sizes tuned to 0.57 acceptance, "
+                               "%d iterations per chain%s" % (C, n, steps, ck, args.lg_iters,
+                                                               "" if done else " (partial: resumable)"),
+                   "seed": args.seed, "complete": done,
+                   "ess": "min over parameters of the multi-chain autocorrelation ESS (Vehtari et al. 2021)"},
+        "mala": res_m, "rwm_plain": res_p, "theta_true": theta_true.tolist(),
+    }
+    print(json.dumps(res), flush=True)
+
+
+def bootstrap_ess_rate(x, wall, ess_fn, rng, reps=100):
+    """5 and 95 % points of the ESS per hour over resamplings of the chains
+    (with replacement; x: [iteration, chain, parameter])."""
+    c = x.shape[1]
+    vals = []
+    for _ in range(reps):
+        y = x[:, rng.integers(c, size=c), :]
+        vals.append(min(ess_fn(y[:, :, p]) for p in range(y.shape[2])) / wall * 3600)
+    return [float(np.percentile(vals, 5)), float(np.percentile(vals, 95))]
+
+
+def split_ess_rate(x, wall, ess_fn):
+    """ESS per hour of a run and the interval its two halves give (x:
+    [iteration, chain, parameter]; the smallest ESS over parameters)."""
+    def e(y):
+        return min(ess_fn(y[:, :, p]) for p in range(y.shape[2]))
+    n = x.shape[0]
+    full = e(x) / wall * 3600
+    halves = [e(x[: n // 2]) / (wall / 2) * 3600, e(x[n // 2:]) / (wall / 2) * 3600] if n >= 8 else [full, full]
+    return full, [min(halves), max(halves)]
+
+
 def surrogate_workload(args):
-    """Learned surrogate + delayed acceptance.  Full model: uniform 128^3
-    grid, --sur-steps steps of a fixed dt, Gaussian IC, flip-rate likelihood
-    against observations on the 64^3 grid (every other voxel).  Training
-    fields: the batched solver's final states at parameters drawn around the
-    truth, downsampled to 64^3.  The surrogate is trained by the bf16 MFMA
-    kernels; then 16-chain delayed-acceptance RWM (surrogate first stage,
-    full solver second) is compared with 16-chain batched RWM, 16-chain MALA
-    and plain one-chain RWM at the same wall-time budget."""
+    """Learned surrogate + delayed acceptance with the adaptive solver as the
+    judge.  Full model: the FP32 wavelet-adaptive solver on 128^3 (levels 1-3,
+    Interp4 tol 1e-3/1e-4 every 10 steps, final refinement), --sur-steps
+    steps of a fixed dt, Gaussian IC, flip-rate likelihood against
+    observations on the 64^3 grid (every other voxel).  Training fields: the
+    batched adaptive ensemble's final states (256 samples per call) at
+    parameters drawn around the truth, downsampled to 64^3.  The surrogate is
+    trained by the bf16 MFMA kernels; then 16-chain delayed acceptance with
+    Langevin proposals on the surrogate's gradient (first stage) and the
+    adaptive solver (second stage) is compared with 16-chain batched RWM,
+    16-chain delayed-acceptance RWM and plain one-chain RWM (one adaptive
+    solve per proposal) at the same wall-time budget."""
     import gliomauq_amd as gq
     from gliomauq_amd import anatomy as A
     from gliomauq_amd import sampler as SM
@@ -408,111 +598,102 @@ def surrogate_workload(args):
     lo = np.array([np.log(2e-4), np.log(2e-3), c[2] - 8, c[1] - 8, c[0] - 8, 0.02])
     hi = np.array([np.log(1.5e-2), np.log(0.2), c[2] + 8, c[1] + 8, c[0] + 8, 0.3])
     half = (slice(None, None, 2),) * 3
+    nogrid = tuple(np.zeros((n // 2,) * 3, t) for t in (np.uint8, np.uint8, np.float32))
 
-    def solve_fields(ths, ens):
-        """final states at 64^3 for parameter rows ths (batched)"""
-        out = []
-        for i in range(0, len(ths), ens.n):
-            blk = ths[i:i + ens.n]
-            m = len(blk)
-            full = np.tile(0.5 * (lo + hi), (ens.n, 1))
-            full[:m] = blk
-            Dw = np.exp(full[:, 0]) / L_CM ** 2
-            prm = np.stack([Dw, 0.1 * Dw, np.exp(full[:, 1]), np.full(ens.n, dt)], 1)
-            ens.ic_gaussian(np.stack([full[:, 2], full[:, 3], full[:, 4], full[:, 5], np.full(ens.n, 2.0)], 1))
-            st = np.zeros(ens.n, np.int32)
-            st[:m] = steps
-            ens.run(prm, st, adapt_every=0)
-            for s in range(m):
-                out.append(ens.sample(s).dump_uniform()[half].astype(np.float32))
-        return np.array(out)
+    # the batched adaptive solver generates the truth and the training fields
+    gen = SM.AdaptivePosterior(gq.Context(0), pwm, pgm, nogrid, steps, dt, lo, hi, batch=args.sur_gen_batch)
 
-    ctx = gq.Context(0)
-    gen = gq.Ensemble(ctx, args.sur_gen_batch, n // 16, 3, 4)
-    gen.write_volume("pw", pwm)
-    gen.write_volume("pg", pgm)
-    log("[bench] surrogate: truth + %d training fields (%d steps at 128^3)" % (args.sur_fields, steps))
-    u_true = solve_fields(tt[None], gen)[0]
+    def solve_fields(ths):
+        out = np.zeros((len(ths), n // 2, n // 2, n // 2), np.float32)
+        t_log = time.perf_counter()
+        for i in range(0, len(ths), gen.batch):
+            blk = ths[i:i + gen.batch]
+            gen.logpost(blk)
+            for s in range(len(blk)):
+                out[i + s] = gen.ens.sample(s).dump_uniform()[half]
+            if time.perf_counter() - t_log > 30.0:
+                t_log = time.perf_counter()
+                log("[bench]   %d of %d fields" % (i + len(blk), len(ths)))
+        return out
+
+    log("[bench] surrogate: truth + %d training fields (adaptive 128^3, %d steps)" % (args.sur_fields, steps))
+    u_true = solve_fields(tt[None])[0]
     obs64 = A.observations(u_true, seed=args.seed)
-    # samplers on the full posterior (observations at 64^3)
-    post_b = SM.Posterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=C)
-    post_1 = SM.Posterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=1)
-    log("[bench] scale probe")
-    scale = SM.probe_scales(post_b, tt, drop=2.0)
-    # training parameters: a fixed box around the mode, or (--sur-box probe) the
-    # mode +- K probe scales, the region the chains explore
-    if args.sur_box == "probe":
-        span = args.sur_box_k * scale
-    else:
-        span = np.array([0.15, 0.15, 1.5, 1.5, 1.5, 0.02])
+    span = np.array([0.15, 0.15, 1.5, 1.5, 1.5, 0.02])
     t0 = time.perf_counter()
-    ths = tt + span * rng.uniform(-1, 1, (args.sur_fields + 64, 6))
-    fields = solve_fields(ths, gen)
+    ths = tt + span * rng.uniform(-1, 1, (args.sur_fields + 256, 6))
+    fields = solve_fields(ths)
     t_gen = time.perf_counter() - t0
-    log("[bench] generated %d fields in %.1f s" % (len(fields), t_gen))
+    leaves = np.concatenate(gen.leaves)
+    log("[bench] generated %d fields in %.1f s (final leaves %d..%d)" % (len(fields), t_gen, leaves.min(),
+                                                                         leaves.max()))
     del gen
-    sur = SU.Surrogate(batch=args.sur_batch, seed=args.seed)
+    # the samplers' full model: the same adaptive solver, observations at 64^3
+    post_b = SM.AdaptivePosterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=C)
+    post_1 = SM.AdaptivePosterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=1)
+    sur = SU.Surrogate(batch=64, seed=args.seed)
     sur.set_anatomy(pwm[half], pgm[half])
     mu = tt[[0, 1, 5]]
     sd = span[[0, 1, 5]]
     spost = SM.SurrogatePosterior(sur, obs64, lo, hi, ratio=2, norm_mu=mu, norm_sd=sd)
-    P, U0 = spost.inputs(ths)
     ntr = args.sur_fields
-    log("[bench] surrogate: training %d epochs" % args.sur_epochs)
+    P_all, U0_all = spost.inputs(ths)
+    log("[bench] surrogate: training %d epochs on %d fields" % (args.sur_epochs, ntr))
     t1 = time.perf_counter()
     losses = []
     fwd_ms = bwd_ms = 0.0
     nsteps_tr = 0
+    B = args.sur_batch
     for ep in range(args.sur_epochs):
         perm = rng.permutation(ntr)
         lr = args.sur_lr * (0.5 ** (ep // max(1, args.sur_epochs // 3)))
-        for i in range(0, ntr - args.sur_batch + 1, args.sur_batch):
-            j = perm[i:i + args.sur_batch]
-            losses.append(sur.train_step(P[j], U0[j], fields[j], lr))
+        for i in range(0, ntr - B + 1, B):
+            j = perm[i:i + B]
+            losses.append(sur.train_step(P_all[j], U0_all[j], fields[j], lr))
             tm = sur.timing()
             fwd_ms += tm["forward_ms"]
             bwd_ms += tm["backward_ms"]
             nsteps_tr += 1
         log("[bench] epoch %d loss %.4e (%.1f s, device fwd+bwd %.2f ms/step)"
-            % (ep, np.mean(losses[-(ntr // args.sur_batch):]), time.perf_counter() - t1,
-               (fwd_ms + bwd_ms) / max(nsteps_tr, 1)))
+            % (ep, np.mean(losses[-(ntr // B):]), time.perf_counter() - t1, (fwd_ms + bwd_ms) / max(nsteps_tr, 1)))
     t_train = time.perf_counter() - t1
-    f1, f2 = SU.conv_flops(args.sur_batch)
-    # held-out field error and likelihood agreement
-    pv = sur.predict(P[ntr:], U0[ntr:])
-    err = np.sqrt(np.mean((pv - fields[ntr:]) ** 2, axis=(1, 2, 3))) / np.sqrt(np.mean(fields[ntr:] ** 2, axis=(1, 2, 3)))
-    # serving throughput
+    f1, f2 = SU.conv_flops(B)
+    Ph, U0h = P_all[ntr:], U0_all[ntr:]
+    pv = sur.predict(Ph, U0h)
+    held = fields[ntr:]
+    err = np.sqrt(np.mean((pv - held) ** 2, axis=(1, 2, 3))) / np.sqrt(np.mean(held ** 2, axis=(1, 2, 3)))
     t2 = time.perf_counter()
     reps = 20
     for _ in range(reps):
-        sur.predict(P[:args.sur_batch], U0[:args.sur_batch])
+        sur.predict(Ph[:64], U0h[:64])
     t_pred = (time.perf_counter() - t2) / reps
     pred_ms = sur.timing()["forward_ms"]
-    sur_res = {"train_fields": ntr, "gen_s": t_gen, "train_s": t_train, "train_steps": nsteps_tr,
-               "loss_first": losses[0], "loss_last": float(np.mean(losses[-(ntr // args.sur_batch):])),
-               "heldout_rel_l2_mean": float(err.mean()), "heldout_rel_l2_max": float(err.max()),
+    f64_, _ = SU.conv_flops(64)
+    sur_res = {"train_fields": ntr, "field_source": "batched adaptive solver (levels 1-3 of 128^3), 64^3",
+               "gen_s": t_gen, "train_s": t_train, "train_steps": nsteps_tr,
+               "loss_first": losses[0], "loss_last": float(np.mean(losses[-(ntr // B):])),
+               "heldout_fields": len(held), "heldout_rel_l2_mean": float(err.mean()),
+               "heldout_rel_l2_max": float(err.max()),
                "train_tflops": (f2 * nsteps_tr) / ((fwd_ms + bwd_ms) * 1e-3) / 1e12,
-               "forward_tflops": f1 / (pred_ms * 1e-3) / 1e12,
-               "mfma_util_forward": f1 / (pred_ms * 1e-3) / 2.5e15,
+               "forward_tflops_batch64": f64_ / (pred_ms * 1e-3) / 1e12,
+               "mfma_util_forward": f64_ / (pred_ms * 1e-3) / 2.5e15,
                "mfma_util_train": (f2 * nsteps_tr) / ((fwd_ms + bwd_ms) * 1e-3) / 2.5e15,
-               "predict_ms_per_batch": pred_ms, "predict_wall_s_per_batch": t_pred,
-               "train_fwd_ms_per_step": fwd_ms / max(nsteps_tr, 1), "train_bwd_ms_per_step": bwd_ms / max(nsteps_tr, 1)}
+               "predict_ms_per_batch64": pred_ms, "predict_wall_s_per_batch64": t_pred}
     log("[bench] surrogate %s" % json.dumps(sur_res))
+    del fields, P_all, U0_all
     if args.sur_train_only:
         print(json.dumps({"metric": "surrogate training only", "surrogate": sur_res}), flush=True)
         return
 
-    start = tt + 0.3 * scale * rng.standard_normal((C, 6))
     budget = args.mcmc_budget
-    res_s = {}
-    # population preconditioner (as in the mcmc workload): covariance of a batched-RWM pilot over
-    # the chains; every sampler, the plain one included, proposes with the same Cholesky factor
+    log("[bench] scale probe")
+    scale = SM.probe_scales(post_b, tt, drop=2.0)
+    start = tt + 0.3 * scale * rng.standard_normal((C, 6))
     log("[bench] preconditioner pilot")
-    op, _, _ = SM.rwm_batched(post_b, start, 60, scale, rng, adapt_until=60)
-    chol = SM.population_cholesky(op[30:])
+    op, _, _ = SM.rwm_batched(post_b, start, 40, scale, rng, adapt_until=40)
+    chol = SM.population_cholesky(op[20:])
     start = op[-1]
-    probe = scale
-    scale = chol
+    fd = 1e-2 * np.sqrt(np.diag(chol @ chol.T))
 
     def run_loop(fn, state):
         xs, acs = [], []
@@ -520,95 +701,92 @@ def surrogate_workload(args):
         t_log = t0
         while time.perf_counter() - t0 < budget:
             state = fn(state, xs, acs)
-            if time.perf_counter() - t_log > 30.0:  # progress from the sampler itself
+            if time.perf_counter() - t_log > 30.0:
                 t_log = time.perf_counter()
                 log("[bench]   %.0f s, %d iterations" % (t_log - t0, sum(len(x) for x in xs)))
         return np.concatenate(xs, 0), np.concatenate(acs, 0), time.perf_counter() - t0, state
 
-    # plain RWM
-    log("[bench] plain RWM")
-    o, a, lf = SM.rwm(post_1, start[0], 40, scale, rng, adapt_until=40)
+    log("[bench] plain RWM (one adaptive solve per proposal)")
+    o, a, lf = SM.rwm(post_1, start[0], 20, chol, rng, adapt_until=20)
 
     def step_rwm(st, xs, acs):
-        o, a, lf = SM.rwm(post_1, st[0], 20, scale, rng, logf=st[1])
+        o, a, lf = SM.rwm(post_1, st[0], 10, chol, rng, logf=st[1])
         xs.append(o[:, None, :])
         acs.append(a[:, None])
         return (o[-1], lf)
     x_r, a_r, w_r, _ = run_loop(step_rwm, (o[-1], lf))
-    # batched RWM
     log("[bench] batched RWM")
-    o, a, lf = SM.rwm_batched(post_b, start, 20, scale, rng, adapt_until=20)
+    o, a, lf = SM.rwm_batched(post_b, start, 10, chol, rng, adapt_until=10)
 
     def step_rwmb(st, xs, acs):
-        o, a, lf = SM.rwm_batched(post_b, st[0], 5, scale, rng, logf=st[1])
+        o, a, lf = SM.rwm_batched(post_b, st[0], 3, chol, rng, logf=st[1])
         xs.append(o)
         acs.append(a)
         return (o[-1], lf)
     x_b, a_b, w_b, _ = run_loop(step_rwmb, (o[-1], lf))
-    # DA RWM
-    log("[bench] delayed-acceptance RWM")
-    o, a, lf, nf0 = SM.da_rwm(post_b, spost, start, 20, scale, rng, adapt_until=20)
-    nfull = [0]
-    diag = []
+    log("[bench] delayed acceptance, Langevin proposals on the surrogate gradient")
+    nfull = {"da_mala_16": 0, "da_rwm_16": 0}
+    diag = {"da_mala_16": [], "da_rwm_16": []}
+    s1 = []
+    o, a, le, nf, a1 = SM.da_mala(post_b, spost, start, 10, chol, rng, fd, adapt_until=10)
 
-    def step_da(st, xs, acs):
-        o, a, lf, nf = SM.da_rwm(post_b, spost, st[0], 5, scale, rng, logf=st[1], diag=diag)
-        nfull[0] += nf
-        xs.append(o)
-        acs.append(a)
-        return (o[-1], lf)
-    x_d, a_d, w_d, _ = run_loop(step_da, (o[-1], lf))
-    # MALA
-    log("[bench] MALA")
-    o, a, le = SM.mala(post_b, start, 10, scale, rng, adapt_until=10)
-
-    def step_mala(st, xs, acs):
-        o, a, le = SM.mala(post_b, st[0], 3, scale, rng, logeps=st[1])
+    def step_dam(st, xs, acs):
+        o, a, le, nf, a1 = SM.da_mala(post_b, spost, st[0], 3, chol, rng, fd, logeps=st[1], diag=diag["da_mala_16"])
+        nfull["da_mala_16"] += nf
+        s1.append(a1)
         xs.append(o)
         acs.append(a)
         return (o[-1], le)
-    x_m, a_m, w_m, _ = run_loop(step_mala, (o[-1], le))
+    x_dm, a_dm, w_dm, _ = run_loop(step_dam, (o[-1], le))
+    log("[bench] delayed acceptance, random-walk proposals")
+    o, a, lf, nf = SM.da_rwm(post_b, spost, start, 10, chol, rng, adapt_until=10)
 
-    def ess(x):
-        return min(SM.ess_autocorr(x[:, :, p]) for p in range(6))
+    def step_dar(st, xs, acs):
+        o, a, lf, nf = SM.da_rwm(post_b, spost, st[0], 3, chol, rng, logf=st[1], diag=diag["da_rwm_16"])
+        nfull["da_rwm_16"] += nf
+        xs.append(o)
+        acs.append(a)
+        return (o[-1], lf)
+    x_dr, a_dr, w_dr, _ = run_loop(step_dar, (o[-1], lf))
 
     rates = {}
-    for name, x, a, w in (("rwm_plain", x_r, a_r, w_r), ("rwm_16", x_b, a_b, w_b), ("da_rwm_16", x_d, a_d, w_d),
-                          ("mala_16", x_m, a_m, w_m)):
-        e = ess(x)
-        e_bm = min(sum(SM.ess_batch_means(x[:, k, p]) for k in range(x.shape[1])) for p in range(6))
-        rates[name] = {"ess": e, "ess_per_hour": e / w * 3600, "ess_batch_means_sum": e_bm,
-                       "iterations": int(x.shape[0]),
-                       "accept": float(a.mean()), "wall_s": w,
-                       "posterior_mean": x.reshape(-1, 6).mean(0).tolist()}
-    rates["da_rwm_16"]["full_solves"] = nfull[0]
-    rates["da_rwm_16"]["proposals"] = int(x_d.shape[0] * C)
-    rates["da_rwm_16"]["stage2_rejection_rate"] = 1.0 - float(a_d.sum()) / max(nfull[0], 1)
-    if diag:
-        ds = np.concatenate([d[0] for d in diag])
-        df = np.concatenate([d[1] for d in diag])
-        ok = np.isfinite(ds) & np.isfinite(df)
-        ds, df = ds[ok], df[ok]
-        if len(ds) > 2:
-            # does the surrogate order the proposals it passes on the way the full posterior does?
-            rates["da_rwm_16"]["stage2_corr_sur_full"] = float(np.corrcoef(ds, df)[0, 1])
-            rates["da_rwm_16"]["stage2_slope_full_on_sur"] = float(np.polyfit(ds, df, 1)[0])
-            rates["da_rwm_16"]["stage2_mean_dlogpost_sur_full"] = [float(ds.mean()), float(df.mean())]
-    ref_mean = np.array(rates["rwm_16"]["posterior_mean"])
+    runs = (("rwm_plain", x_r, a_r, w_r), ("rwm_16", x_b, a_b, w_b), ("da_mala_16", x_dm, a_dm, w_dm),
+            ("da_rwm_16", x_dr, a_dr, w_dr))
+    for name, x, a, w in runs:
+        r, iv = split_ess_rate(x, w, SM.ess_autocorr)
+        rates[name] = {"ess": r * w / 3600, "ess_per_hour": r, "ess_per_hour_halves": iv,
+                       "iterations": int(x.shape[0]), "chains": int(x.shape[1]), "accept": float(a.mean()),
+                       "wall_s": w, "posterior_mean": x[x.shape[0] // 4:].reshape(-1, 6).mean(0).tolist()}
+    rates["da_mala_16"]["stage1_accept"] = float(np.concatenate(s1).mean()) if s1 else None
+    for k in ("da_mala_16", "da_rwm_16"):
+        x = dict((r[0], r[1]) for r in runs)[k]
+        acc_n = int(dict((r[0], r[2]) for r in runs)[k].sum())
+        rates[k]["full_solves"] = nfull[k]
+        rates[k]["proposals"] = int(x.shape[0] * C)
+        rates[k]["stage2_rejection_rate"] = 1.0 - acc_n / max(nfull[k], 1)
+        if diag[k]:
+            ds = np.concatenate([d[0] for d in diag[k]])
+            df = np.concatenate([d[1] for d in diag[k]])
+            ok = np.isfinite(ds) & np.isfinite(df)
+            if ok.sum() > 2:
+                rates[k]["stage2_corr_sur_full"] = float(np.corrcoef(ds[ok], df[ok])[0, 1])
+    ref_mean = np.array(rates["rwm_plain"]["posterior_mean"])
+    sd_ref = np.sqrt(np.diag(chol @ chol.T))
     for k in rates:
-        rates[k]["mean_rel_dev_vs_rwm16"] = (np.abs(np.array(rates[k]["posterior_mean"]) - ref_mean) /
-                                            np.abs(ref_mean)).max()
-    best = max(rates, key=lambda k: rates[k]["ess_per_hour"])
+        m = np.array(rates[k]["posterior_mean"])
+        rates[k]["mean_rel_dev_vs_plain"] = float((np.abs(m - ref_mean) / np.abs(ref_mean)).max())
+        rates[k]["mean_dev_vs_plain_in_sd"] = (np.abs(m - ref_mean) / sd_ref).tolist()
     res = {
         "metric": "cell-updates/s + %HBM roofline; posterior samples/hour vs plain MCMC at fixed error",
-        "value": rates[best]["ess_per_hour"], "unit": "effective posterior samples/hour", "n_gpus": 1,
-        "steps": steps, "warmup": 20, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "value": rates["da_mala_16"]["ess_per_hour"], "unit": "effective posterior samples/hour", "n_gpus": 1,
+        "steps": steps, "warmup": 10, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 (surrogate) / f32 (solver)", "data": "synthetic",
-        "config": {"workload": "surrogate + delayed acceptance: 128^3 uniform solver, %d steps, observations 64^3; "
-                               "surrogate 64^3 encoder-decoder (16-128 channels, 3^3), bf16 MFMA, %d fields, "
-                               "%d epochs" % (steps, ntr, args.sur_epochs), "seed": args.seed, "best": best,
-                   "training_box": args.sur_box, "training_span": span.tolist(), "probe_scale": probe.tolist(),
-                   "preconditioner_sd": np.sqrt(np.diag(chol @ chol.T)).tolist()},
+        "config": {"workload": "surrogate + delayed acceptance (Langevin first stage on the surrogate), full model "
+                               "the adaptive 128^3 solver (levels 1-3, %d steps), observations 64^3; surrogate "
+                               "64^3 encoder-decoder (16-128 channels, 3^3), bf16 MFMA, %d adaptive-solver fields, "
+                               "%d epochs; %d chains, %.0f s per sampler" % (steps, ntr, args.sur_epochs, C, budget),
+                   "seed": args.seed, "training_span": span.tolist(),
+                   "preconditioner_sd": sd_ref.tolist(), "fd_step": fd.tolist()},
         "surrogate": sur_res,
         "mcmc": rates,
         "speedup_vs_plain": {k: rates[k]["ess_per_hour"] / max(rates["rwm_plain"]["ess_per_hour"], 1e-30)
@@ -680,18 +858,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="adaptive",
-                    choices=["adaptive", "ensemble", "adjoint", "mcmc", "surrogate", "tmcmc"])
+                    choices=["adaptive", "ensemble", "adjoint", "mcmc", "surrogate", "tmcmc", "langevin"])
     ap.add_argument("--tm-pop", type=int, default=512)
     ap.add_argument("--tm-batch", type=int, default=64)
     ap.add_argument("--tm-moves", type=int, default=3)
     ap.add_argument("--tm-stages", type=int, default=40)
     ap.add_argument("--tm-stride", type=int, default=4)
     ap.add_argument("--sur-steps", type=int, default=2000)
-    ap.add_argument("--sur-fields", type=int, default=1024)
+    ap.add_argument("--sur-fields", type=int, default=8192)
     ap.add_argument("--sur-epochs", type=int, default=6)
     ap.add_argument("--sur-batch", type=int, default=8)
     ap.add_argument("--sur-lr", type=float, default=3e-4)
-    ap.add_argument("--sur-gen-batch", type=int, default=64)
+    ap.add_argument("--sur-gen-batch", type=int, default=256)
     ap.add_argument("--sur-train-only", action="store_true")
     ap.add_argument("--sur-box", default="fixed", choices=["fixed", "probe"],
                     help="training parameters: fixed box around the mode, or mode +- K probe scales")
@@ -700,7 +878,18 @@ def main():
     ap.add_argument("--mcmc-steps", type=int, default=500)
     ap.add_argument("--mcmc-pilot", type=int, default=40)
     ap.add_argument("--mcmc-budget", type=float, default=60.0, help="wall seconds per sampler")
+    ap.add_argument("--mcmc-plain-budget", type=float, default=0.0,
+                    help="wall seconds of the plain one-chain sampler (default: --mcmc-budget)")
     ap.add_argument("--mcmc-chains", type=int, default=256, help="MALA chains of the mcmc workload")
+    ap.add_argument("--lg-n", type=int, default=128)
+    ap.add_argument("--lg-steps", type=int, default=2000)
+    ap.add_argument("--lg-chains", type=int, default=16)
+    ap.add_argument("--lg-iters", type=int, default=5000, help="Langevin iterations per chain after tuning")
+    ap.add_argument("--lg-pilot", type=int, default=40)
+    ap.add_argument("--lg-tune", type=int, default=200)
+    ap.add_argument("--lg-budget", type=float, default=120.0, help="wall seconds of sampling in this process")
+    ap.add_argument("--lg-plain-budget", type=float, default=0.0)
+    ap.add_argument("--lg-state", default="", help="npz file to resume from and save to")
     ap.add_argument("--chains", type=int, default=16, help="adjoint chains")
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--steps-adj-reps", type=int, default=2)
@@ -731,6 +920,8 @@ def main():
         return surrogate_workload(args)
     if args.workload == "tmcmc":
         return tmcmc_workload(args)
+    if args.workload == "langevin":
+        return langevin_workload(args)
 
     import gliomauq_amd as gq
     from gliomauq_amd import anatomy as A
@@ -869,8 +1060,8 @@ def main():
                      "traffic": (None if ratio is None else ratio * bytes_per_launch),
                      "traffic_over_algorithmic": ratio,
                      "traffic_source": "profiles/k_fk_step_adaptive256_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
-                                       "WRITE_SIZE on the launches after the first compression, calibrated on a "
-                                       "copy kernel)",
+                                       "WRITE_SIZE over the 590 step launches of the timed solve, calibrated on "
+                                       "a copy kernel)",
                      "kernel": "k_fk_step", "launches": stats["launches"],
                      "timing": ("every step launch timed by its dispatch events in the timed solve"
                                 if args.timing == "inline" else

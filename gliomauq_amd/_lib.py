@@ -25,7 +25,7 @@ EXPORTS = [
     "gq_set_step_coeffs", "gq_set_step_fusion", "gq_set_step_persistent", "gq_persistent_info",
     "gq_host_stats", "gq_graph_info", "gq_set_step_graphs", "gq_topo_create", "gq_topo_destroy",
     "gq_topo_leaves", "gq_topo_refine", "gq_topo_compress", "gq_topo_point_terms",
-    "gq_grid_write_volume", "gq_ensemble_create", "gq_ensemble_size", "gq_sample_select",
+    "gq_grid_write_volume", "gq_ensemble_create", "gq_ensemble_reset", "gq_ensemble_size", "gq_sample_select",
     "gq_ensemble_block_counts", "gq_ensemble_run", "gq_ensemble_stats", "gq_ensemble_loglik_flip",
     "gq_ensemble_ic_gaussian", "gq_adjoint_flip", "gq_loglik_flip_volume_noalloc",
     "gq_sur_create", "gq_sur_destroy", "gq_sur_last_error", "gq_sur_set_anatomy", "gq_sur_param_count",
@@ -115,6 +115,7 @@ def load():
         "gq_grid_write_volume": [vp, C.c_int, f32p, C.c_int, C.c_int],
         "gq_ensemble_create": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
         "gq_ensemble_size": [vp, ip],
+        "gq_ensemble_reset": [vp],
         "gq_sample_select": [vp, C.c_int],
         "gq_ensemble_block_counts": [vp, i32p],
         "gq_ensemble_run": [vp, C.POINTER(FKParams), i32p, C.POINTER(RunOpts)],

@@ -1292,7 +1292,7 @@ const char *gq_status_string(gq_status s)
 
 const char *gq_last_error(const gq_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
 
-static void create_samples(gq_ctx *ctx, int nsamples, int bpd, int max_level, int order)
+static void create_samples(gq_ctx *ctx, int nsamples, int bpd, int max_level, int order, bool keep_pool = false)
 {
     if (nsamples <= 0) throw gq_error(GQ_ERR_ARG, "nsamples must be positive");
     HIPCHK(hipSetDevice(ctx->device));
@@ -1309,14 +1309,14 @@ static void create_samples(gq_ctx *ctx, int nsamples, int bpd, int max_level, in
     ctx->h_lutoff.clear();
     ctx->h_fbase.clear();
     ctx->uploaded_cap = 0;
-    ctx->release_pool();
+    const int n = (int)smp[0].grid->leaves().size();
+    const int64_t want = (int64_t)n * nsamples;
+    if (want * gq::BS3 > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "block pool beyond 2^31 cells");
+    if (!keep_pool || ctx->cap < want) ctx->release_pool();
     ctx->free_slots.clear();
     ctx->samples = std::move(smp);
     ctx->cur_sample = 0;
     ctx->sample_order.clear();
-    const int n = (int)ctx->samples[0].grid->leaves().size();
-    const int64_t want = (int64_t)n * nsamples;
-    if (want * gq::BS3 > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "block pool beyond 2^31 cells");
     ctx->grow((int)std::max<int64_t>(64, want + want / 2));
     // slots in Morton order of the initial leaves, sample after sample
     std::vector<int> mo = gq::morton_order(*ctx->samples[0].grid);
@@ -1343,6 +1343,15 @@ gq_status gq_grid_create(gq_ctx *ctx, int bpd, int max_level, int order)
 gq_status gq_ensemble_create(gq_ctx *ctx, int nsamples, int bpd, int max_level, int order)
 {
     return guard(ctx, [&] { create_samples(ctx, nsamples, bpd, max_level, order); });
+}
+
+gq_status gq_ensemble_reset(gq_ctx *ctx)
+{
+    return guard(ctx, [&] {
+        ctx->need_grid();
+        const gq::Grid &g = ctx->G(0);
+        create_samples(ctx, ctx->nsamples(), g.blocks_per_dim(), g.max_level(), g.order(), true);
+    });
 }
 
 gq_status gq_ensemble_size(gq_ctx *ctx, int *nsamples)

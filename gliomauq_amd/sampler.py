@@ -10,9 +10,13 @@ evaluated for many parameter sets per GPU call:
                      a batch of parameter sets on a uniform grid, box prior
   rwm                plain random-walk Metropolis, one forward solve per
                      proposal (the reference's pattern, on the GPU)
+  AdaptivePosterior  the same with the FP32 wavelet-adaptive solver
   mala               Metropolis-adjusted Langevin, many chains per call,
                      gradient from gq_adjoint_flip, per-chain step sizes
                      adapted towards an acceptance of 0.57 during burn-in
+  da_rwm, da_mala    delayed acceptance with the learned surrogate as the
+                     first stage (random-walk or Langevin proposals built on
+                     the surrogate's gradient), the full solver second
   tmcmc              transitional MCMC (tempered stages, resampling and
                      per-stage Metropolis moves on the whole population)
   ess_autocorr       effective sample size of one or many chains from their
@@ -89,6 +93,42 @@ class Posterior:
         return np.where(inside, ll[:m, 0], -np.inf)
 
 
+class AdaptivePosterior(Posterior):
+    """Log-posterior of the same model with the FP32 wavelet-adaptive solver
+    (Glioma_HG_UQ::run per parameter set, batched): every call starts each
+    sample from the uniform base-level grid (gq_ensemble_reset), runs
+    n_steps with adaptation (refine one loop, compress) after every
+    `adapt_every` steps and the final refinement, and evaluates the flip-rate
+    likelihood on the uniform dump of the adapted grid."""
+
+    def __init__(self, ctx, pwm, pgm, obs, n_steps, dt_days, lo, hi, batch, L_cm=17.0, sigma_ic=2.0,
+                 lik=(0.7, 0.25, 0.05, 0.05), adapt_every=10, tol_refine=1e-3, tol_compress=1e-4):
+        super().__init__(ctx, pwm, pgm, obs, n_steps, dt_days, lo, hi, batch, L_cm=L_cm, sigma_ic=sigma_ic, lik=lik)
+        self.pwm = np.ascontiguousarray(pwm, np.float32)
+        self.pgm = np.ascontiguousarray(pgm, np.float32)
+        self.adapt_every = adapt_every
+        self.tol = (tol_refine, tol_compress)
+        self.leaves = []
+
+    def logpost_grad(self, th):
+        raise NotImplementedError("the adjoint covers uniform grids only")
+
+    def logpost(self, th):
+        m, inside, prm, ic = self._inputs(th)
+        ens = self.ens
+        ens.reset()
+        ens.write_volume("pw", self.pwm)
+        ens.write_volume("pg", self.pgm)
+        ens.ic_gaussian(ic)
+        steps = np.zeros(self.batch, np.int32)
+        steps[:m] = self.n_steps
+        ens.run(prm, steps, adapt_every=self.adapt_every, tol_refine=self.tol[0], tol_compress=self.tol[1])
+        ll = ens.loglik_flip(th1=self.lik[0], th2=self.lik[1], flip=self.lik[2], sigma=self.lik[3])
+        self.leaves.append(ens.block_counts()[:m].copy())
+        self.n_evals += m
+        return np.where(inside, ll[:m, 0], -np.inf)
+
+
 class SurrogatePosterior:
     """Log-posterior with the surrogate's prediction in place of the solver:
     the same parameters, prior box and flip-rate likelihood, observations on
@@ -131,6 +171,34 @@ class SurrogatePosterior:
         ll = self.sur.loglik_flip(p, u0, self.lik)
         self.n_evals += len(th)
         return np.where(inside, ll[:, 0], -np.inf)
+
+    def logpost_pet_grad(self, th, step):
+        """Log-posterior and the gradient of its Gaussian PET term (the
+        Bernoulli terms are piecewise constant in the field, as for the
+        adjoint) by central differences of the surrogate's own predictions,
+        all 13 parameter sets per chain in batched calls; `step` = difference
+        step per parameter.  The gradient is a deterministic function of
+        theta, so Langevin proposals built on it keep exact acceptance ratios."""
+        th = np.asarray(th, np.float64).reshape(-1, 6)
+        C = len(th)
+        step = np.asarray(step, np.float64)
+        rows = [th]
+        for k in range(6):
+            for sgn in (1.0, -1.0):
+                t = th.copy()
+                t[:, k] += sgn * step[k]
+                rows.append(t)
+        allp = np.concatenate(rows)
+        p, u0 = self.inputs(allp)
+        ll = self.sur.loglik_flip(p, u0, self.lik)
+        self.n_evals += len(allp)
+        inside = np.all((th >= self.lo) & (th <= self.hi), axis=1)
+        lp = np.where(inside, ll[:C, 0], -np.inf)
+        pet = ll[:, 3].reshape(13, C)
+        g = np.zeros((C, 6))
+        for k in range(6):
+            g[:, k] = (pet[1 + 2 * k] - pet[2 + 2 * k]) / (2.0 * step[k])
+        return lp, g
 
 
 def _precond(scale):
@@ -201,6 +269,61 @@ def da_rwm(post, spost, theta0, n_iter, scale, rng, adapt_until=0, target=0.234,
         if callback:
             callback(it, th, lp)
     return out, acc, logf, n_full
+
+
+def da_mala(post, spost, theta0, n_iter, scale, rng, fd_step, adapt_until=0, target=0.57, logeps=None, diag=None):
+    """Delayed acceptance with Langevin proposals for a population of chains
+    (the gradient-based first stage): theta' = theta + eps^2/2 M g~(theta) +
+    eps L z with g~ the surrogate's log-posterior gradient
+    (SurrogatePosterior.logpost_pet_grad) and M = L L^T (or diag(scale^2)).
+    Stage 1 accepts with the surrogate's Metropolis-Hastings ratio
+    pi~(t') q(t|t') / (pi~(t) q(t'|t)); the survivors are solved with the
+    full model `post` in one batched call and accepted with
+    pi(t') pi~(t) / (pi(t) pi~(t')) (Christen & Fox 2005), so the chains
+    target pi exactly.  Step sizes adapt per chain towards `target` overall
+    (two-stage) acceptance for the first `adapt_until` iterations.  Returns
+    (samples, accepted, final log eps, full solves, stage-1 acceptances)."""
+    th = np.asarray(theta0, np.float64).copy()
+    Cn = th.shape[0]
+    noise, mgrad, quad = _precond(scale)
+    lp = _batched(post.logpost, th, post.batch)
+    ls, gs = spost.logpost_pet_grad(th, fd_step)
+    logeps = np.full(Cn, math.log(0.5)) if logeps is None else np.array(logeps, np.float64)
+    out = np.zeros((n_iter, Cn, 6))
+    acc = np.zeros((n_iter, Cn), bool)
+    acc1 = np.zeros((n_iter, Cn), bool)
+    n_full = 0
+    for it in range(n_iter):
+        eps = np.exp(logeps)[:, None]
+        mean_f = th + 0.5 * eps ** 2 * mgrad(gs)
+        prop = mean_f + eps * noise(rng.standard_normal((Cn, 6)))
+        lsp, gsp = spost.logpost_pet_grad(prop, fd_step)
+        mean_b = prop + 0.5 * eps ** 2 * mgrad(gsp)
+        lq_f = -quad(prop - mean_f) / (2 * eps[:, 0] ** 2)
+        lq_b = -quad(th - mean_b) / (2 * eps[:, 0] ** 2)
+        la1 = np.where(np.isfinite(lsp), lsp - ls + lq_b - lq_f, -np.inf)
+        a1 = np.log(rng.random(Cn)) < la1
+        idx = np.nonzero(a1)[0]
+        a = np.zeros(Cn, bool)
+        if len(idx):
+            lpp = _batched(post.logpost, prop[idx], post.batch)
+            n_full += len(idx)
+            if diag is not None:
+                diag.append((lsp[idx] - ls[idx], lpp - lp[idx]))
+            r2 = (lpp - lp[idx]) - (lsp[idx] - ls[idx])
+            ok = np.log(rng.random(len(idx))) < np.where(np.isfinite(r2), r2, -np.inf)
+            j = idx[ok]
+            a[j] = True
+            th[j] = prop[j]
+            lp[j] = lpp[ok]
+            ls[j] = lsp[j]
+            gs[j] = gsp[j]
+        acc[it] = a
+        acc1[it] = a1
+        if it < adapt_until:
+            logeps += (a.astype(np.float64) - target) / math.sqrt(it + 1.0)
+        out[it] = th
+    return out, acc, logeps, n_full, acc1
 
 
 def rwm_batched(post, theta0, n_iter, scale, rng, adapt_until=0, target=0.234, logf=None):

@@ -457,6 +457,11 @@ class Ensemble:
         ctx.call("gq_ensemble_create", nsamples, blocks_per_dim, max_level, wavelet_order)
         ctx.owner = self
 
+    def reset(self):
+        """Every sample back to its uniform base-level grid, keeping the device
+        pool (gq_ensemble_reset); fields must be written again."""
+        self.ctx.call("gq_ensemble_reset")
+
     def sample(self, s):
         """Grid view of sample s (Grid methods address that sample)."""
         if not 0 <= s < self.n:

@@ -93,6 +93,12 @@ gq_status gq_grid_write_volume(gq_ctx *ctx, int field, const float *vol, int gpd
  * sample (default 0); gq_fk_steps advances every sample with one parameter
  * set. */
 gq_status gq_ensemble_create(gq_ctx *ctx, int nsamples, int blocks_per_dim, int max_level, int wavelet_order);
+/* Every sample back to the uniform base-level grid of the configuration it
+ * was created with, keeping the device block pool (no allocation where it
+ * fits, as gq_ensemble_create would do): the way to start a new batch of
+ * adaptive solves.  Field contents are undefined until written (anatomy by
+ * gq_grid_write_volume, phi by it or gq_ensemble_ic_gaussian). */
+gq_status gq_ensemble_reset(gq_ctx *ctx);
 gq_status gq_ensemble_size(gq_ctx *ctx, int *nsamples);
 gq_status gq_sample_select(gq_ctx *ctx, int sample);
 /* Leaf-block count of every sample (nsamples int32). */

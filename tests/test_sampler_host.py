@@ -185,3 +185,42 @@ def test_ess_autocorr_ar1():
     x = _ar1_chains(rng, 20000, 1, 0.995)
     assert SM.ess_batch_means(x[:, 0]) > 2.5 * 20000 * 0.005 / 1.995
     assert SM.ess_autocorr(x) < 2.0 * 20000 * 0.005 / 1.995
+
+
+class SurrogateGauss:
+    """A deliberately wrong surrogate of Gauss(MU, SD): shifted mean and
+    wider spread, with the central-difference interface of
+    SurrogatePosterior.logpost_pet_grad."""
+
+    def __init__(self, mu, sd):
+        self.mu = np.asarray(mu, float) + 0.6 * SD
+        self.sd = np.asarray(sd, float) * 1.4
+        self.n_evals = 0
+
+    def _lp(self, th):
+        return -0.5 * np.sum(((th - self.mu) / self.sd) ** 2, 1)
+
+    def logpost_pet_grad(self, th, step):
+        th = np.atleast_2d(th)
+        self.n_evals += 13 * len(th)
+        g = np.zeros_like(th)
+        for k in range(6):
+            e = np.zeros(6)
+            e[k] = step[k]
+            g[:, k] = (self._lp(th + e) - self._lp(th - e)) / (2 * step[k])
+        return self._lp(th), g
+
+
+def test_da_mala_targets_the_full_posterior():
+    """Delayed acceptance with Langevin proposals on a wrong surrogate still
+    samples the full posterior (the second stage corrects it); dropping the
+    second stage would sample the surrogate's shifted, wider Gaussian."""
+    rng = np.random.default_rng(4)
+    post = Gauss(MU, SD)
+    spost = SurrogateGauss(MU, SD)
+    th0 = MU + SD * rng.standard_normal((32, 6))
+    out, acc, _, n_full, acc1 = SM.da_mala(post, spost, th0, 3000, SD, rng, fd_step=1e-3 * SD, adapt_until=500)
+    x = out[500:].reshape(-1, 6)
+    assert n_full < 3000 * 32 and 0.4 < acc[500:].mean() < 0.75 and acc1[500:].mean() > acc[500:].mean()
+    assert np.all(np.abs(x.mean(0) - MU) < 0.1 * SD)
+    assert np.all(np.abs(x.std(0) / SD - 1) < 0.08)
