@@ -11,8 +11,10 @@
 //   lambda_c = mu_c (1 + dt (rho (1 - 2 u_c) - ih2 sum_k df_ck)) + dt ih2 sum_k df_kc mu_k
 //   dL/dlog Dw += mu_c dt ih2 sum_k df_ck (u_k - u_c)     (df ~ Dw when Dg = Dw/10)
 //   dL/dlog rho += mu_c dt rho u_c (1 - u_c)
-// Terms whose adjoint factor is zero are skipped (their coefficients can be
-// NaN next to CSF, where the forward clamps NaN to 0).
+// Coefficients can be NaN next to CSF (pw = pg = 0), where the forward clamps
+// NaN to 0; the adjoint factors of those terms are zero, and the kernel's flux
+// form (k_adj_step) never evaluates them.  As for the step kernels, pw and pg
+// are non-negative (tissue probabilities).
 //
 // One backward step is one kernel: it stages u_n, 1/K (precomputed once per
 // solve) and mu_{n+1} with a one-cell halo in LDS, computes lambda_n and
@@ -81,9 +83,20 @@ __device__ __forceinline__ double wave_sum(double x)
 }  // namespace
 
 // ZT planes of one block per workgroup of 256 threads (thread = one (x,y) column).
+//
+// The update in flux form.  With fc_ck = 2 r_ck, r_ck = 1 / (1/K_c + 1/K_k):
+//   lambda_c = mu_c (1 + dt rho (1 - 2 u_c)) + 2 dt ih2 sum_k r_ck (mu_k - mu_c)
+// which equals the per-cell form above because mu_c != 0 implies tissue_c and
+// finite coefficients of c (else the forward's v_c is NaN or 0 and the mask
+// is 0), and likewise for every neighbour k with mu_k != 0.  The 1/K array
+// holds +inf where the forward's 1/K is NaN (pw + pg = 0, k_adj_prep), so a
+// face touching such a cell has r = 0 -- and there mu_k = mu_c = 0 anyway:
+// no term needs a guard.  The z faces of a column are shared between the
+// two cells they separate (a + b == b + a: the same bits either way).
 template <int ZT>
 __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
 {
+#pragma clang fp contract(off)
     constexpr int NS = B / ZT;
     constexpr int LZ = ZT + 2;
     __shared__ float s_u[LZ * PL];
@@ -102,33 +115,33 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
     const float *q = a.sprm + 4 * a.slot_sample[slot];
     const float rho = q[2], dt = q[3];
 
+    // every cell this thread stages -- its ZT interior cells, one cell of each
+    // z-halo plane and ZT/4 cells of the four side faces -- with all loads in
+    // flight before the first LDS store
+    static_assert(ZT % 4 == 0, "side faces: ZT/4 cells per thread");
+    constexpr int NSIDE = ZT / 4;
+    constexpr int NL = ZT + 2 + NSIDE;
+    const int *fnb = a.fnbr + slot * 6;
+    const int fn0 = fnb[0], fn1 = fnb[1], fn2 = fnb[2], fn3 = fnb[3], fn4 = fnb[4], fn5 = fnb[5];
+    size_t gi[NL];
+    int li[NL];
 #pragma unroll
     for (int k = 0; k < ZT; k++) {
-        const size_t i = base + (z0 + k) * 256 + t;
-        const int l = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
-        s_u[l] = a.u[i];
-        s_ik[l] = a.ik[i];
-        s_mu[l] = a.mu_in[i];
+        gi[k] = base + (z0 + k) * 256 + t;
+        li[k] = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
     }
+    li[ZT] = (tx + 1) + LW * (ty + 1);
+    gi[ZT] = z0 > 0 ? base + (z0 - 1) * 256 + t : (size_t)fn4 * B3 + nbr_face_cell(4, tx, ty);
+    li[ZT + 1] = (tx + 1) + LW * (ty + 1) + PL * (ZT + 1);
+    gi[ZT + 1] = z0 + ZT < B ? base + (z0 + ZT) * 256 + t : (size_t)fn5 * B3 + nbr_face_cell(5, tx, ty);
 #pragma unroll
-    for (int side = 0; side < 2; side++) {
-        const int z = side == 0 ? z0 - 1 : z0 + ZT;
-        const int l = (tx + 1) + LW * (ty + 1) + PL * (side == 0 ? 0 : ZT + 1);
-        size_t i;
-        if (z >= 0 && z < B) i = base + z * 256 + t;
-        else {
-            const int f = side == 0 ? 4 : 5;
-            i = (size_t)a.fnbr[slot * 6 + f] * B3 + nbr_face_cell(f, tx, ty);
-        }
-        s_u[l] = a.u[i];
-        s_ik[l] = a.ik[i];
-        s_mu[l] = a.mu_in[i];
-    }
-    for (int r4 = t; r4 < 4 * 16 * ZT; r4 += 256) {
+    for (int j = 0; j < NSIDE; j++) {
+        const int r4 = t + 256 * j;
         const int f = r4 / (16 * ZT);
         const int r = r4 - f * 16 * ZT;
         const int u = r & 15, v = z0 + (r >> 4);
-        const size_t i = (size_t)a.fnbr[slot * 6 + f] * B3 + nbr_face_cell(f, u, v);
+        const int nb = f == 0 ? fn0 : f == 1 ? fn1 : f == 2 ? fn2 : fn3;
+        gi[ZT + 2 + j] = (size_t)nb * B3 + nbr_face_cell(f, u, v);
         int l;
         switch (f) {
         case 0: l = 0 + LW * (u + 1) + PL * (v - z0 + 1); break;
@@ -136,73 +149,70 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
         case 2: l = (u + 1) + 0 + PL * (v - z0 + 1); break;
         default: l = (u + 1) + LW * (B + 1) + PL * (v - z0 + 1); break;
         }
-        s_u[l] = a.u[i];
-        s_ik[l] = a.ik[i];
-        s_mu[l] = a.mu_in[i];
+        li[ZT + 2 + j] = l;
+    }
+    float vu[NL], vk[NL], vm[NL];
+#pragma unroll
+    for (int j = 0; j < NL; j++) {
+        vu[j] = a.u[gi[j]];
+        vk[j] = a.ik[gi[j]];
+        vm[j] = a.mu_in[gi[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < NL; j++) {
+        s_u[li[j]] = vu[j];
+        s_ik[li[j]] = vk[j];
+        s_mu[li[j]] = vm[j];
     }
     __syncthreads();
 
     const float ih2 = a.ih2_lvl[a.level[slot]];
-    const int offs[6] = {-1, 1, -LW, LW, -PL, PL};
+    const float dti = dt * ih2;
     double gD = 0.0, gR = 0.0;
+    float r_up = 0.f;  // r of the +z face of the previous cell of this column, when computed
+    bool have_up = false;
 #pragma unroll
     for (int k = 0; k < ZT; k++) {
         const int c = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
-        const float uc = s_u[c];
-        const float kc = s_ik[c];
         const float muc = s_mu[c];
-        float mun6[6];
-#pragma unroll
-        for (int f = 0; f < 6; f++) mun6[f] = s_mu[c + offs[f]];
-        if ((muc == 0.f) & (mun6[0] == 0.f) & (mun6[1] == 0.f) & (mun6[2] == 0.f) & (mun6[3] == 0.f) &
-            (mun6[4] == 0.f) & (mun6[5] == 0.f)) {
+        const float m0 = s_mu[c - 1], m1 = s_mu[c + 1], m2 = s_mu[c - LW], m3 = s_mu[c + LW], m4 = s_mu[c - PL],
+                    m5 = s_mu[c + PL];
+        if ((muc == 0.f) & (m0 == 0.f) & (m1 == 0.f) & (m2 == 0.f) & (m3 == 0.f) & (m4 == 0.f) & (m5 == 0.f)) {
             // no adjoint flows through this cell: lambda = 0 and no gradient terms
             a.out[base + (z0 + k) * 256 + t] = 0.f;
+            have_up = false;
             continue;
         }
-        // face coefficients 2 / (1/K_c + 1/K_k): the same value serves the
-        // cell's own update and, transposed, the neighbour's (the sum is
-        // commutative); a face whose value is not finite is never used
-        float fc[6];
-#pragma unroll
-        for (int f = 0; f < 6; f++) fc[f] = 2.f * frcp(kc + s_ik[c + offs[f]]);
-        float lam = 0.f;
-        if (muc != 0.f) {  // the cell's own update was inside (0,1): its coefficients are finite
-            float d[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            // tissue_c = pw+pg+u_c > 0; with pw, pg >= 0 and Dw, Dg > 0, 1/K is finite and
-            // positive exactly where pw+pg > 0 (else 0/0 = NaN), as for the neighbours below
-            if ((kc > 0.f && kc < INFINITY) || uc > 0.f) {
-#pragma unroll
-                for (int f = 0; f < 6; f++) d[f] = fc[f];
-            }
-            const float sdf = d[0] + d[1] + d[2] + d[3] + d[4] + d[5];
-            const float diff = d[0] * (s_u[c - 1] - uc) + d[1] * (s_u[c + 1] - uc) + d[2] * (s_u[c - LW] - uc) +
-                               d[3] * (s_u[c + LW] - uc) + d[4] * (s_u[c - PL] - uc) + d[5] * (s_u[c + PL] - uc);
-            lam = muc * (1.f + dt * (rho * (1.f - 2.f * uc) - ih2 * sdf));
-            // the per-block factors dt ih2 and dt rho are applied once to the sums
-            gD += (double)muc * (double)diff;
-            gR += (double)muc * ((double)uc * (1.0 - (double)uc));
+        const float uc = s_u[c];
+        const float kc = s_ik[c];
+        const float r0 = frcp(kc + s_ik[c - 1]), r1 = frcp(kc + s_ik[c + 1]);
+        const float r2 = frcp(kc + s_ik[c - LW]), r3 = frcp(kc + s_ik[c + LW]);
+        const float r4 = have_up ? r_up : frcp(kc + s_ik[c - PL]);
+        const float r5 = frcp(kc + s_ik[c + PL]);
+        r_up = r5;
+        have_up = true;
+        float fl = r0 * (m0 - muc);
+        fl = __builtin_fmaf(r1, m1 - muc, fl);
+        fl = __builtin_fmaf(r2, m2 - muc, fl);
+        fl = __builtin_fmaf(r3, m3 - muc, fl);
+        fl = __builtin_fmaf(r4, m4 - muc, fl);
+        fl = __builtin_fmaf(r5, m5 - muc, fl);
+        const float lam = __builtin_fmaf(2.f * dti, fl, muc * (1.f + dt * (rho * (1.f - 2.f * uc))));
+        if (muc != 0.f) {
+            float df = r0 * (s_u[c - 1] - uc);
+            df = __builtin_fmaf(r1, s_u[c + 1] - uc, df);
+            df = __builtin_fmaf(r2, s_u[c - LW] - uc, df);
+            df = __builtin_fmaf(r3, s_u[c + LW] - uc, df);
+            df = __builtin_fmaf(r4, s_u[c - PL] - uc, df);
+            df = __builtin_fmaf(r5, s_u[c + PL] - uc, df);
+            // the per-block factors 2 dt ih2 and dt rho are applied once to the sums
+            gD = __builtin_fma((double)muc, (double)df, gD);
+            gR = __builtin_fma((double)muc, (double)uc * (1.0 - (double)uc), gR);
         }
-        // transpose: neighbour k's update reads u_c with k's coefficient toward c
-        float tr = 0.f;
-#pragma unroll
-        for (int f = 0; f < 6; f++) {
-            const int n = c + offs[f];
-            const float mun = mun6[f];
-            if (mun != 0.f) {
-                // the neighbour's tissue switch pw+pg+u > 0: with Dw, Dg > 0, 1/K is
-                // finite and positive exactly where pw+pg > 0 (else 0/0 = NaN), and
-                // u >= 0, so tissue_n > 0 <=> 1/K_n finite || u_n > 0
-                const float kn = s_ik[n];
-                const bool tissue = (kn > 0.f && kn < INFINITY) || s_u[n] > 0.f;
-                if (tissue) tr += mun * fc[f];
-            }
-        }
-        lam += dt * ih2 * tr;
         a.out[base + (z0 + k) * 256 + t] = a.out_lambda ? lam : mu_of(lam, uc);
     }
     // block sums of the two parameter gradients -> this slab's accumulators
-    gD = wave_sum(gD) * ((double)dt * (double)ih2);
+    gD = wave_sum(gD) * (2.0 * (double)dt * (double)ih2);
     gR = wave_sum(gR) * ((double)dt * (double)rho);
     const int lane = t & 63, wv = t >> 6;
     if (lane == 0) {
@@ -217,7 +227,9 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
 }
 
 // 1/K of every cell of the listed slots (the adjoint's coefficients are fixed
-// during a solve: pw, pg and the sample's Dw, Dg do not change).
+// during a solve: pw, pg and the sample's Dw, Dg do not change), +inf where
+// the step's 1/K is NaN (pw + pg = 0: no tissue), so that every face
+// coefficient touching such a cell is 0 in the adjoint step.
 __global__ __launch_bounds__(256) void k_adj_prep(const float *__restrict__ pw, const float *__restrict__ pg,
                                                   const int *__restrict__ work, const int *__restrict__ slot_sample,
                                                   const float *__restrict__ sprm, float *__restrict__ ik)
@@ -226,7 +238,10 @@ __global__ __launch_bounds__(256) void k_adj_prep(const float *__restrict__ pw, 
     const float *q = sprm + 4 * slot_sample[slot];
     const float Dw = q[0], Dg = q[1];
     const size_t base = (size_t)slot * B3;
-    for (int i = threadIdx.x; i < B3; i += 256) ik[base + i] = inv_k(pw[base + i], pg[base + i], Dw, Dg);
+    for (int i = threadIdx.x; i < B3; i += 256) {
+        const float v = inv_k(pw[base + i], pg[base + i], Dw, Dg);
+        ik[base + i] = v == v ? v : INFINITY;
+    }
 }
 
 // mu_N = [0 < u_N < 1] dL/du of the flip-rate model at the final state

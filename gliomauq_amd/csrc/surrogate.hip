@@ -143,6 +143,105 @@ __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
     }
 }
 
+// The same convolution for layers with few input channels (CIN = 8, 16, 48):
+// the GEMM's k = tap * CIN + ci runs in steps of 32 across taps, so a step
+// with CIN < 32 covers several taps and only the last step of 27 * CIN is
+// padded (7 steps instead of 27 for CIN = 8, 14 instead of 27 for 16, 41
+// instead of 54 for 48).  The brick's 10x10x6 halo is staged with all CIN
+// channels at once; lane group kq reads its 8 k of a step from the halo voxel
+// of its own tap.  Same output mapping as k_conv3_brick.
+template <int CIN, int NTG, bool OUT_BF16>
+__global__ __launch_bounds__(256) void k_conv3_flat(ConvArgs a)
+{
+    constexpr int LD = CIN + 8;  // LDS channel stride per halo voxel (pad: banks)
+    constexpr int KT = 27 * CIN;
+    constexpr int NST = (KT + 31) / 32;
+    constexpr int G8 = CIN / 8;
+    __shared__ __attribute__((aligned(16))) __bf16 s_in[600 * LD];
+    const int D = a.in.D;
+    const int nbx = D >> 3, nbz = D >> 2;
+    const int b = blockIdx.x;
+    const int bx = b % nbx, by = (b / nbx) % nbx, bz = (b / (nbx * nbx)) % nbz, bb = b / (nbx * nbx * nbz);
+    const int x0 = 8 * bx, y0 = 8 * by, z0 = 4 * bz;
+    const int nt0 = blockIdx.y * NTG;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, kq = lane >> 4;
+    const int64_t D3 = (int64_t)D * D * D;
+    f32x4 acc[4][NTG];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int n = 0; n < NTG; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    int hbase[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int r = 2 * (4 * wave + m) + (li >> 3);
+        hbase[m] = (li & 7) + 10 * ((r & 7) + 10 * (r >> 3));
+    }
+    for (int e = t; e < 600 * G8; e += 256) {
+        const int h = e / G8, g = e - h * G8;
+        const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+        const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+        bf16x8 v = zero8();
+        if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
+            const int64_t vn = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
+            v = load8(a.in.p + vn * a.in.ld + a.in.off + 8 * g);
+        }
+        *reinterpret_cast<bf16x8 *>(&s_in[h * LD + 8 * g]) = v;
+    }
+    __syncthreads();
+    constexpr int PF = 3;
+    bf16x8 vb[PF + 1][NTG];
+    auto load_w = [&](int st, bf16x8 *dst) {
+        const int k = 32 * st + 8 * kq;
+#pragma unroll
+        for (int n = 0; n < NTG; n++) {
+            const int co = 16 * (nt0 + n) + li;
+            dst[n] = (k < KT && co < a.cout) ? load8(a.w + (int64_t)co * KT + k) : zero8();
+        }
+    };
+#pragma unroll
+    for (int p = 0; p < PF; p++)
+        if (p < NST) load_w(p, vb[p]);
+#pragma unroll
+    for (int st = 0; st < NST; st++) {
+        if (st + PF < NST) load_w(st + PF, vb[(st + PF) % (PF + 1)]);
+        const int k = 32 * st + 8 * kq;
+        // past the last tap the weights are zero; read a valid voxel so the product is 0
+        const int tap = k < KT ? k / CIN : 26;
+        const int ci = k < KT ? k - tap * CIN : 0;
+        const int hoff = tap % 3 + 10 * ((tap / 3) % 3 + 10 * (tap / 9));
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_in[(hbase[m] + hoff) * LD + ci]);
+#pragma unroll
+            for (int n = 0; n < NTG; n++)
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb[st % (PF + 1)][n], acc[m][n], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NTG; n++) {
+        const int co = 16 * (nt0 + n) + li;
+        if (co >= a.cout) continue;
+        const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int i = 4 * kq + rr;
+                const int r = 2 * (4 * wave + m) + (i >> 3);
+                const int gx = x0 + (i & 7), gy = y0 + (r & 7), gz = z0 + (r >> 3);
+                const int64_t vo = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
+                float val = acc[m][n][rr] + bias;
+                if (a.relu) val = val > 0.f ? val : kLeak * val;
+                const int64_t o = vo * a.out_ld + a.out_off + co;
+                if (OUT_BF16) a.out_bf[o] = (__bf16)val;
+                else a.out_f[o] = a.accumulate ? a.out_f[o] + val : val;
+            }
+        }
+    }
+}
+
 // Weight gradient by voxel bricks: dW[co][tap*Cin + ci] = sum_v dZ[v][co] X[v + tap][ci].
 // Workgroup (16-channel input tile, output group of <= 32 channels, brick
 // group): for each 8x8x4-voxel brick of its group it stages dZ^T [co][voxel]
@@ -282,79 +381,144 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const float *partial, int
     out[i] = s;
 }
 
-// 2x2x2 average pooling (forward) and its adjoint (accumulated into dx).
-__global__ __launch_bounds__(256) void k_avgpool(View in, View out, int C, int batch)
+// Pooling, upsampling and the leaky-ReLU backward move 8 channels per thread
+// (one 16-byte bf16 transfer, or two for FP32): edges are powers of two (lg =
+// log2 D) and channel counts, strides and offsets multiples of 8, which the
+// launchers check.  Per channel the arithmetic and its order are those of a
+// one-channel-per-thread loop.
+namespace {
+
+__device__ __forceinline__ void f32x8_load(const float *p, float *v)
 {
-    const int64_t Do = out.D, n = Do * Do * Do * batch * C;
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    const int c = (int)(e % C);
-    const int64_t v = e / C, D3o = Do * Do * Do, b = v / D3o, r = v - b * D3o;
-    const int x = (int)(r % Do), y = (int)((r / Do) % Do), z = (int)(r / (Do * Do));
-    const int64_t Di = in.D, D3i = Di * Di * Di;
-    float s = 0.f;
-    for (int k = 0; k < 8; k++) {
-        const int64_t vi = b * D3i + (2 * x + (k & 1)) + Di * ((2 * y + ((k >> 1) & 1)) + Di * (int64_t)(2 * z + (k >> 2)));
-        s += bf2f(in.p[vi * in.ld + in.off + c]);
-    }
-    out.p[v * out.ld + out.off + c] = (__bf16)(0.125f * s);
+    const float4 a = reinterpret_cast<const float4 *>(p)[0], b = reinterpret_cast<const float4 *>(p)[1];
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
 }
 
-__global__ __launch_bounds__(256) void k_avgpool_bwd(const float *dout, int64_t dout_ld, FView din, int C, int batch)
+__device__ __forceinline__ void f32x8_store(float *p, const float *v)
+{
+    reinterpret_cast<float4 *>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+    reinterpret_cast<float4 *>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
+// voxel v of a batch of (2^lg)^3 volumes -> item b and coordinates
+struct Vox {
+    int64_t b;
+    int x, y, z;
+};
+__device__ __forceinline__ Vox vox_of(int64_t v, int lg)
+{
+    const int m = (1 << lg) - 1;
+    const int r = (int)(v & (((int64_t)1 << (3 * lg)) - 1));
+    return Vox{v >> (3 * lg), r & m, (r >> lg) & m, r >> (2 * lg)};
+}
+__device__ __forceinline__ int64_t vox_index(int64_t b, int x, int y, int z, int lg)
+{
+    return (b << (3 * lg)) + x + ((int64_t)y << lg) + ((int64_t)z << (2 * lg));
+}
+
+}  // namespace
+
+// 2x2x2 average pooling (forward) and its adjoint (accumulated into dx).
+__global__ __launch_bounds__(256) void k_avgpool(View in, View out, int C, int batch, int lgo)
+{
+    const int G = C >> 3;
+    const int64_t n = ((int64_t)batch << (3 * lgo)) * G;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int64_t v = e / G;
+    const int c = 8 * (int)(e - v * G);
+    const Vox o = vox_of(v, lgo);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int64_t vi = vox_index(o.b, 2 * o.x + (k & 1), 2 * o.y + ((k >> 1) & 1), 2 * o.z + (k >> 2), lgo + 1);
+        const bf16x8 q = load8(in.p + vi * in.ld + in.off + c);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] += bf2f(q[j]);
+    }
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = (__bf16)(0.125f * s[j]);
+    *reinterpret_cast<bf16x8 *>(out.p + v * out.ld + out.off + c) = r;
+}
+
+__global__ __launch_bounds__(256) void k_avgpool_bwd(const float *dout, int64_t dout_ld, FView din, int C, int batch,
+                                                     int lgi)
 {
     // din at the fine level; each fine voxel receives dout(parent) / 8
-    const int64_t Di = din.D, n = Di * Di * Di * batch * C;
+    const int G = C >> 3;
+    const int64_t n = ((int64_t)batch << (3 * lgi)) * G;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
-    const int c = (int)(e % C);
-    const int64_t v = e / C, D3i = Di * Di * Di, b = v / D3i, r = v - b * D3i;
-    const int x = (int)(r % Di), y = (int)((r / Di) % Di), z = (int)(r / (Di * Di));
-    const int64_t Do = Di / 2;
-    const int64_t vo = b * Do * Do * Do + (x >> 1) + Do * ((y >> 1) + Do * (int64_t)(z >> 1));
-    din.p[v * din.ld + din.off + c] += 0.125f * dout[vo * dout_ld + c];
+    const int64_t v = e / G;
+    const int c = 8 * (int)(e - v * G);
+    const Vox f = vox_of(v, lgi);
+    const int64_t vo = vox_index(f.b, f.x >> 1, f.y >> 1, f.z >> 1, lgi - 1);
+    float g[8], d[8];
+    f32x8_load(dout + vo * dout_ld + c, g);
+    float *p = din.p + v * din.ld + din.off + c;
+    f32x8_load(p, d);
+#pragma unroll
+    for (int j = 0; j < 8; j++) d[j] += 0.125f * g[j];
+    f32x8_store(p, d);
 }
 
-// Nearest-neighbour 2x upsampling (forward) and its adjoint (sum of 8 children).
-__global__ __launch_bounds__(256) void k_upsample(View in, View out, int C, int batch)
+// Nearest-neighbour 2x upsampling (forward) and its adjoint (sum of 8 children):
+// one thread per coarse voxel and 8 channels.
+__global__ __launch_bounds__(256) void k_upsample(View in, View out, int C, int batch, int lgi)
 {
-    const int64_t Do = out.D, n = Do * Do * Do * batch * C;
+    const int G = C >> 3;
+    const int64_t n = ((int64_t)batch << (3 * lgi)) * G;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
-    const int c = (int)(e % C);
-    const int64_t v = e / C, D3o = Do * Do * Do, b = v / D3o, r = v - b * D3o;
-    const int x = (int)(r % Do), y = (int)((r / Do) % Do), z = (int)(r / (Do * Do));
-    const int64_t Di = in.D;
-    const int64_t vi = b * Di * Di * Di + (x >> 1) + Di * ((y >> 1) + Di * (int64_t)(z >> 1));
-    out.p[v * out.ld + out.off + c] = in.p[vi * in.ld + in.off + c];
-}
-
-__global__ __launch_bounds__(256) void k_upsample_bwd(FView dout, float *din, int64_t din_ld, int C, int batch)
-{
-    const int64_t Di = dout.D / 2, n = Di * Di * Di * batch * C;
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    const int c = (int)(e % C);
-    const int64_t v = e / C, D3i = Di * Di * Di, b = v / D3i, r = v - b * D3i;
-    const int x = (int)(r % Di), y = (int)((r / Di) % Di), z = (int)(r / (Di * Di));
-    const int64_t Do = dout.D, D3o = Do * Do * Do;
-    float s = 0.f;
+    const int64_t v = e / G;
+    const int c = 8 * (int)(e - v * G);
+    const Vox q = vox_of(v, lgi);
+    const bf16x8 val = load8(in.p + v * in.ld + in.off + c);
+#pragma unroll
     for (int k = 0; k < 8; k++) {
-        const int64_t vo = b * D3o + (2 * x + (k & 1)) + Do * ((2 * y + ((k >> 1) & 1)) + Do * (int64_t)(2 * z + (k >> 2)));
-        s += dout.p[vo * dout.ld + dout.off + c];
+        const int64_t vo = vox_index(q.b, 2 * q.x + (k & 1), 2 * q.y + ((k >> 1) & 1), 2 * q.z + (k >> 2), lgi + 1);
+        *reinterpret_cast<bf16x8 *>(out.p + vo * out.ld + out.off + c) = val;
     }
-    din[v * din_ld + c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_upsample_bwd(FView dout, float *din, int64_t din_ld, int C, int batch,
+                                                      int lgi)
+{
+    const int G = C >> 3;
+    const int64_t n = ((int64_t)batch << (3 * lgi)) * G;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int64_t v = e / G;
+    const int c = 8 * (int)(e - v * G);
+    const Vox q = vox_of(v, lgi);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int64_t vo = vox_index(q.b, 2 * q.x + (k & 1), 2 * q.y + ((k >> 1) & 1), 2 * q.z + (k >> 2), lgi + 1);
+        float g[8];
+        f32x8_load(dout.p + vo * dout.ld + dout.off + c, g);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] += g[j];
+    }
+    f32x8_store(din + v * din_ld + c, s);
 }
 
 // dZ = dY * (Y > 0 ? 1 : leak) -> bf16 (leaky-ReLU backward through the post-activation value)
 __global__ __launch_bounds__(256) void k_relu_bwd(FView dy, View y, __bf16 *dz, int C, int64_t nvox)
 {
+    const int G = C >> 3;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= nvox * C) return;
-    const int c = (int)(e % C);
-    const int64_t v = e / C;
-    const float g = dy.p[v * dy.ld + dy.off + c];
-    const float yv = bf2f(y.p[v * y.ld + y.off + c]);
-    dz[v * C + c] = (__bf16)(yv > 0.f ? g : kLeak * g);
+    if (e >= nvox * G) return;
+    const int64_t v = e / G;
+    const int c = 8 * (int)(e - v * G);
+    float g[8];
+    f32x8_load(dy.p + v * dy.ld + dy.off + c, g);
+    const bf16x8 yv = load8(y.p + v * y.ld + y.off + c);
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = (__bf16)(bf2f(yv[j]) > 0.f ? g[j] : kLeak * g[j]);
+    *reinterpret_cast<bf16x8 *>(dz + v * C + c) = r;
 }
 
 // Output head: p = sigmoid(sum_c w[c] h[c] + b); loss = mean (p - t)^2 +
@@ -367,11 +531,17 @@ __global__ __launch_bounds__(256) void k_head(HeadArgs a)
     const bool ok = v < a.nvox;
     float h[16];
     float zl = a.b;
+    {
+        const bf16x8 h0 = ok ? load8(a.h.p + v * a.h.ld + a.h.off) : zero8();
+        const bf16x8 h1 = ok ? load8(a.h.p + v * a.h.ld + a.h.off + 8) : zero8();
 #pragma unroll
-    for (int c = 0; c < 16; c++) {
-        h[c] = ok ? bf2f(a.h.p[v * a.h.ld + a.h.off + c]) : 0.f;
-        zl += a.w[c] * h[c];
+        for (int c = 0; c < 8; c++) {
+            h[c] = bf2f(h0[c]);
+            h[c + 8] = bf2f(h1[c]);
+        }
     }
+#pragma unroll
+    for (int c = 0; c < 16; c++) zl += a.w[c] * h[c];
     const float p = 1.f / (1.f + __expf(-zl));
     float part[18];
 #pragma unroll
@@ -521,6 +691,25 @@ hipError_t launch_conv3(const ConvArgs &a, hipStream_t st)
     if (nt % ntg != 0) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(a.batch * (D / 8) * (D / 8) * (D / 4)), (unsigned)(nt / ntg));
     const bool bf = a.out_bf != nullptr;
+    if (a.cin == 8 || a.cin == 16 || a.cin == 48) {
+#define GQ_FLAT(C, N)                                                                     \
+    if (bf) hipLaunchKernelGGL((k_conv3_flat<C, N, true>), grid, dim3(256), 0, st, a);    \
+    else hipLaunchKernelGGL((k_conv3_flat<C, N, false>), grid, dim3(256), 0, st, a);
+#define GQ_FLAT_N(C)                    \
+    switch (ntg) {                      \
+    case 1: GQ_FLAT(C, 1) break;        \
+    case 2: GQ_FLAT(C, 2) break;        \
+    case 3: GQ_FLAT(C, 3) break;        \
+    case 4: GQ_FLAT(C, 4) break;        \
+    default: return hipErrorInvalidValue; \
+    }
+        if (a.cin == 8) { GQ_FLAT_N(8) }
+        else if (a.cin == 16) { GQ_FLAT_N(16) }
+        else { GQ_FLAT_N(48) }
+#undef GQ_FLAT_N
+#undef GQ_FLAT
+        return hipGetLastError();
+    }
 #define GQ_CONV(N)                                                                        \
     if (bf) hipLaunchKernelGGL((k_conv3_brick<N, true>), grid, dim3(256), 0, st, a);      \
     else hipLaunchKernelGGL((k_conv3_brick<N, false>), grid, dim3(256), 0, st, a);
@@ -564,42 +753,65 @@ hipError_t launch_reduce_chunks(const float *partial, int nchunks, int64_t n, fl
 
 static dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+namespace {
+// log2 of a power-of-two edge, or -1
+int lg_edge(int D)
+{
+    if (D < 1 || (D & (D - 1)) != 0) return -1;
+    int l = 0;
+    while ((1 << l) < D) l++;
+    return l;
+}
+bool ch8(int C, int64_t ld, int64_t off) { return C > 0 && C % 8 == 0 && ld % 8 == 0 && off % 8 == 0 && off + C <= ld; }
+}  // namespace
+
 hipError_t launch_avgpool(const View &in, const View &out, int C, int batch, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_avgpool, g1((int64_t)out.D * out.D * out.D * batch * C), dim3(256), 0, st, in, out, C,
-                       batch);
+    const int lgo = lg_edge(out.D);
+    if (lgo < 0 || in.D != 2 * out.D || !ch8(C, in.ld, in.off) || !ch8(C, out.ld, out.off)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_avgpool, g1((int64_t)out.D * out.D * out.D * batch * (C / 8)), dim3(256), 0, st, in, out, C,
+                       batch, lgo);
     return hipGetLastError();
 }
 
 hipError_t launch_avgpool_bwd(const float *dout, int64_t dout_ld, const FView &din, int C, int batch, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_avgpool_bwd, g1((int64_t)din.D * din.D * din.D * batch * C), dim3(256), 0, st, dout,
-                       dout_ld, din, C, batch);
+    const int lgi = lg_edge(din.D);
+    if (lgi < 1 || !ch8(C, dout_ld, 0) || !ch8(C, din.ld, din.off)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_avgpool_bwd, g1((int64_t)din.D * din.D * din.D * batch * (C / 8)), dim3(256), 0, st, dout,
+                       dout_ld, din, C, batch, lgi);
     return hipGetLastError();
 }
 
 hipError_t launch_upsample(const View &in, const View &out, int C, int batch, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_upsample, g1((int64_t)out.D * out.D * out.D * batch * C), dim3(256), 0, st, in, out, C,
-                       batch);
+    const int lgi = lg_edge(in.D);
+    if (lgi < 0 || out.D != 2 * in.D || !ch8(C, in.ld, in.off) || !ch8(C, out.ld, out.off)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_upsample, g1((int64_t)in.D * in.D * in.D * batch * (C / 8)), dim3(256), 0, st, in, out, C,
+                       batch, lgi);
     return hipGetLastError();
 }
 
 hipError_t launch_upsample_bwd(const FView &dout, float *din, int64_t din_ld, int C, int batch, hipStream_t st)
 {
     const int64_t Di = dout.D / 2;
-    hipLaunchKernelGGL(k_upsample_bwd, g1(Di * Di * Di * batch * C), dim3(256), 0, st, dout, din, din_ld, C, batch);
+    const int lgi = lg_edge((int)Di);
+    if (lgi < 0 || !ch8(C, dout.ld, dout.off) || !ch8(C, din_ld, 0)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_upsample_bwd, g1(Di * Di * Di * batch * (C / 8)), dim3(256), 0, st, dout, din, din_ld, C,
+                       batch, lgi);
     return hipGetLastError();
 }
 
 hipError_t launch_relu_bwd(const FView &dy, const View &y, __bf16 *dz, int C, int64_t nvox, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_relu_bwd, g1(nvox * C), dim3(256), 0, st, dy, y, dz, C, nvox);
+    if (!ch8(C, dy.ld, dy.off) || !ch8(C, y.ld, y.off)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_relu_bwd, g1(nvox * (C / 8)), dim3(256), 0, st, dy, y, dz, C, nvox);
     return hipGetLastError();
 }
 
 hipError_t launch_head(const HeadArgs &a, hipStream_t st)
 {
+    if (!ch8(16, a.h.ld, a.h.off)) return hipErrorInvalidValue;  // two 8-channel loads per voxel
     hipLaunchKernelGGL(k_head, g1(a.nvox), dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -269,6 +269,42 @@ def test_ensemble_graph_cache_alternating_shapes(inputs):
             assert out[0][k][s].tobytes() == out[1][k][s].tobytes(), "run %d sample %d differs" % (k, s)
 
 
+def test_ensemble_reset_equals_fresh_ensemble(inputs):
+    """gq_ensemble_reset returns every sample to its uniform base-level grid
+    in the pool it already holds: after an adaptive run, a reset, new fields
+    and a second adaptive run, every sample equals the same second run on a
+    freshly created ensemble to the bit (leaf sets, dumps), and the
+    observations set before the reset still serve the likelihood."""
+    pwm, pgm, ics, obs = inputs
+    prm_a = np.array([sample_params(s) for s in range(3)])
+    prm_b = np.array([sample_params((s + 1) % 4) for s in range(3)])
+    steps_b = [STEPS[(s + 1) % 4] for s in range(3)]
+    ctx = gq.Context(0)
+    ens = make_ensemble(ctx, pwm, pgm, ics[:3])
+    ens.set_observations(*obs)
+    ens.run(prm_a, STEPS[:3], adapt_every=K, tol_refine=TOL_R, tol_compress=TOL_C)
+    assert np.any(ens.block_counts() != (N // 16) ** 3)  # the first run adapted
+    ens.reset()
+    assert np.all(ens.block_counts() == (N // 16) ** 3)
+    ens.write_volume("pw", pwm)
+    ens.write_volume("pg", pgm)
+    for s in range(3):
+        ens.write_volume("phi", ics[(s + 1) % 4], sample=s)
+    ens.run(prm_b, steps_b, adapt_every=K, tol_refine=TOL_R, tol_compress=TOL_C)
+    got = [(ens.sample(s).levels_indices()[:2], ens.sample(s).dump_uniform()) for s in range(3)]
+    ll = ens.loglik_flip()
+    del ens
+    fresh = make_ensemble(ctx, pwm, pgm, [ics[(s + 1) % 4] for s in range(3)])
+    fresh.set_observations(*obs)
+    fresh.run(prm_b, steps_b, adapt_every=K, tol_refine=TOL_R, tol_compress=TOL_C)
+    for s in range(3):
+        lv, ix = fresh.sample(s).levels_indices()[:2]
+        assert np.array_equal(lv, got[s][0][0]) and np.array_equal(ix, got[s][0][1]), s
+        assert fresh.sample(s).dump_uniform().tobytes() == got[s][1].tobytes(), "sample %d differs" % s
+    assert np.array_equal(fresh.loglik_flip(), ll)
+    ctx.close()
+
+
 def test_ensemble_zero_step_samples(inputs):
     """Samples with zero steps beside a running one, and an ensemble where no
     sample steps: the idle samples keep their initial state to the bit, the
