@@ -37,8 +37,18 @@ namespace {
 
 constexpr int B = 16;
 constexpr int B3 = 4096;
-constexpr int LW = B + 2;
-constexpr int PL = LW * LW;
+// LDS tile of a slab: ZT + 2 planes of 18 rows (y = -1 .. 16) x 16 columns,
+// then the x halo columns (x = -1, x = 16) of the ZT interior planes; rows of
+// 16 floats keep the 64 lanes of a wave (4 rows of 16) on distinct banks.
+constexpr int RW = 16;
+constexpr int PL = 18 * RW;
+template <int ZT>
+struct Tile {
+    static constexpr int XW = (ZT + 2) * PL, XE = XW + 16 * ZT, SIZE = XE + 16 * ZT;
+    __device__ static int cell(int x, int y, int zl) { return x + RW * (y + 1) + PL * zl; }
+    __device__ static int west(int y, int k) { return XW + y + 16 * k; }
+    __device__ static int east(int y, int k) { return XE + y + 16 * k; }
+};
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -98,10 +108,10 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
 {
 #pragma clang fp contract(off)
     constexpr int NS = B / ZT;
-    constexpr int LZ = ZT + 2;
-    __shared__ float s_u[LZ * PL];
-    __shared__ float s_ik[LZ * PL];
-    __shared__ float s_mu[LZ * PL];
+    typedef Tile<ZT> T;
+    __shared__ float s_u[T::SIZE];
+    __shared__ float s_ik[T::SIZE];
+    __shared__ float s_mu[T::SIZE];
     __shared__ double s_red[2][4];
     const int item = xcd_remap(blockIdx.x, gridDim.x);
     if (item >= a.nwork * NS) return;  // padding workgroups (whole workgroup exits)
@@ -128,11 +138,11 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
 #pragma unroll
     for (int k = 0; k < ZT; k++) {
         gi[k] = base + (z0 + k) * 256 + t;
-        li[k] = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
+        li[k] = T::cell(tx, ty, k + 1);
     }
-    li[ZT] = (tx + 1) + LW * (ty + 1);
+    li[ZT] = T::cell(tx, ty, 0);
     gi[ZT] = z0 > 0 ? base + (z0 - 1) * 256 + t : (size_t)fn4 * B3 + nbr_face_cell(4, tx, ty);
-    li[ZT + 1] = (tx + 1) + LW * (ty + 1) + PL * (ZT + 1);
+    li[ZT + 1] = T::cell(tx, ty, ZT + 1);
     gi[ZT + 1] = z0 + ZT < B ? base + (z0 + ZT) * 256 + t : (size_t)fn5 * B3 + nbr_face_cell(5, tx, ty);
 #pragma unroll
     for (int j = 0; j < NSIDE; j++) {
@@ -144,10 +154,10 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
         gi[ZT + 2 + j] = (size_t)nb * B3 + nbr_face_cell(f, u, v);
         int l;
         switch (f) {
-        case 0: l = 0 + LW * (u + 1) + PL * (v - z0 + 1); break;
-        case 1: l = (B + 1) + LW * (u + 1) + PL * (v - z0 + 1); break;
-        case 2: l = (u + 1) + 0 + PL * (v - z0 + 1); break;
-        default: l = (u + 1) + LW * (B + 1) + PL * (v - z0 + 1); break;
+        case 0: l = T::west(u, v - z0); break;
+        case 1: l = T::east(u, v - z0); break;
+        case 2: l = T::cell(u, -1, v - z0 + 1); break;
+        default: l = T::cell(u, B, v - z0 + 1); break;
         }
         li[ZT + 2 + j] = l;
     }
@@ -173,9 +183,11 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
     bool have_up = false;
 #pragma unroll
     for (int k = 0; k < ZT; k++) {
-        const int c = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
+        const int c = T::cell(tx, ty, k + 1);
+        const int cw = tx > 0 ? c - 1 : T::west(ty, k);
+        const int ce = tx < B - 1 ? c + 1 : T::east(ty, k);
         const float muc = s_mu[c];
-        const float m0 = s_mu[c - 1], m1 = s_mu[c + 1], m2 = s_mu[c - LW], m3 = s_mu[c + LW], m4 = s_mu[c - PL],
+        const float m0 = s_mu[cw], m1 = s_mu[ce], m2 = s_mu[c - RW], m3 = s_mu[c + RW], m4 = s_mu[c - PL],
                     m5 = s_mu[c + PL];
         if ((muc == 0.f) & (m0 == 0.f) & (m1 == 0.f) & (m2 == 0.f) & (m3 == 0.f) & (m4 == 0.f) & (m5 == 0.f)) {
             // no adjoint flows through this cell: lambda = 0 and no gradient terms
@@ -185,8 +197,8 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
         }
         const float uc = s_u[c];
         const float kc = s_ik[c];
-        const float r0 = frcp(kc + s_ik[c - 1]), r1 = frcp(kc + s_ik[c + 1]);
-        const float r2 = frcp(kc + s_ik[c - LW]), r3 = frcp(kc + s_ik[c + LW]);
+        const float r0 = frcp(kc + s_ik[cw]), r1 = frcp(kc + s_ik[ce]);
+        const float r2 = frcp(kc + s_ik[c - RW]), r3 = frcp(kc + s_ik[c + RW]);
         const float r4 = have_up ? r_up : frcp(kc + s_ik[c - PL]);
         const float r5 = frcp(kc + s_ik[c + PL]);
         r_up = r5;
@@ -199,10 +211,10 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjArgs a)
         fl = __builtin_fmaf(r5, m5 - muc, fl);
         const float lam = __builtin_fmaf(2.f * dti, fl, muc * (1.f + dt * (rho * (1.f - 2.f * uc))));
         if (muc != 0.f) {
-            float df = r0 * (s_u[c - 1] - uc);
-            df = __builtin_fmaf(r1, s_u[c + 1] - uc, df);
-            df = __builtin_fmaf(r2, s_u[c - LW] - uc, df);
-            df = __builtin_fmaf(r3, s_u[c + LW] - uc, df);
+            float df = r0 * (s_u[cw] - uc);
+            df = __builtin_fmaf(r1, s_u[ce] - uc, df);
+            df = __builtin_fmaf(r2, s_u[c - RW] - uc, df);
+            df = __builtin_fmaf(r3, s_u[c + RW] - uc, df);
             df = __builtin_fmaf(r4, s_u[c - PL] - uc, df);
             df = __builtin_fmaf(r5, s_u[c + PL] - uc, df);
             // the per-block factors 2 dt ih2 and dt rho are applied once to the sums

@@ -23,8 +23,24 @@ namespace {
 
 constexpr int B = 16;
 constexpr int B3 = 4096;
-constexpr int LW = B + 2;  // step lab width (halo 1)
 constexpr int PLW = B + 2; // plane width of the column-streaming step
+
+// LDS tile of a step slab: ZT + 2 planes (z0-1 .. z0+ZT) of 18 rows (y = -1
+// .. 16) x 16 columns (x = 0 .. 15), then the x halo columns (x = -1, x = 16)
+// of the ZT interior planes.  A row stride of 16 floats puts the 64 lanes of
+// a wave (4 rows of 16) on distinct banks for every read but the two x halo
+// columns (with rows of 18, rows 0 and 1 of a wave shared two banks).
+template <int ZT>
+struct SlabTile {
+    static constexpr int RW = 16, PL = 18 * 16;
+    static constexpr int XW = (ZT + 2) * PL, XE = XW + 16 * ZT;
+    static constexpr int SIZE = XE + 16 * ZT;
+    // x 0..15, y -1..16, plane zl 0..ZT+1 (zl = z - z0 + 1)
+    __device__ static int cell(int x, int y, int zl) { return x + RW * (y + 1) + PL * zl; }
+    // x = -1 / x = 16 of row y 0..15 of interior plane k 0..ZT-1
+    __device__ static int west(int y, int k) { return XW + y + 16 * k; }
+    __device__ static int east(int y, int k) { return XE + y + 16 * k; }
+};
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -237,15 +253,15 @@ __device__ __forceinline__ float step_face_ghost(const StepArgs &a, int row)
 // Fused RD operator + UpdateTumor.  A workgroup of 256 threads updates a slab
 // of 16 x 16 x ZT cells of one block (ZT = 16: the whole block); each thread
 // owns one (x,y) column of ZT cells.  phi and 1/K of the slab and its 1-cell
-// halo are staged in LDS ((ZT+2) x 18 x 18 x 2 floats).  Halo planes in z come
+// halo are staged in LDS (SlabTile: 16-float rows, x halos apart).  Halo planes in z come
 // from the same block when inside it, else from the z-neighbour block (same
 // level) or the cross-level ghost rows; x/y halos from the face neighbours.
 // ---------------------------------------------------------------------------
 template <int ZT>
 __device__ __forceinline__ void fk_step_item(const StepArgs &a, int item, float *s_phi, float *s_ik)
 {
-    constexpr int NS = B / ZT;       // slabs per block
-    constexpr int PL = LW * LW;      // LDS plane size
+    constexpr int NS = B / ZT;  // slabs per block
+    typedef SlabTile<ZT> T;
     const int w = item / NS;
     const int z0 = (item - w * NS) * ZT;
     const int *rec = a.wrec + (size_t)w * WREC;
@@ -274,7 +290,7 @@ __device__ __forceinline__ void fk_step_item(const StepArgs &a, int item, float 
 #pragma unroll
         for (int k = 0; k < ZT; k++) {
             const size_t i = base + (z0 + k) * 256 + t;
-            const int l = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
+            const int l = T::cell(tx, ty, k + 1);
             float ik;
             ik_centre(c[k], p[k], a.pw, a.pg, i, Dw, Dg, ik, breg[k]);
             s_phi[l] = p[k];
@@ -288,7 +304,7 @@ __device__ __forceinline__ void fk_step_item(const StepArgs &a, int item, float 
             const float wm = a.pw[i];
             const float gm = a.pg[i];
             breg[k] = wm + gm;
-            const int l = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
+            const int l = T::cell(tx, ty, k + 1);
             s_phi[l] = p;
             s_ik[l] = inv_k(wm, gm, Dw, Dg);
         }
@@ -297,7 +313,7 @@ __device__ __forceinline__ void fk_step_item(const StepArgs &a, int item, float 
 #pragma unroll
     for (int side = 0; side < 2; side++) {
         const int z = side == 0 ? z0 - 1 : z0 + ZT;
-        const int l = (tx + 1) + LW * (ty + 1) + PL * (side == 0 ? 0 : ZT + 1);
+        const int l = T::cell(tx, ty, side == 0 ? 0 : ZT + 1);
         float pv, kv;
         if (z >= 0 && z < B) {
             const size_t i = base + z * 256 + t;
@@ -329,10 +345,10 @@ __device__ __forceinline__ void fk_step_item(const StepArgs &a, int item, float 
         const float kv = a.fik[((size_t)slot * 6 + f) * 256 + u + 16 * v];
         int l;
         switch (f) {
-        case 0: l = 0 + LW * (u + 1) + PL * (v - z0 + 1); break;
-        case 1: l = (B + 1) + LW * (u + 1) + PL * (v - z0 + 1); break;
-        case 2: l = (u + 1) + 0 + PL * (v - z0 + 1); break;
-        default: l = (u + 1) + LW * (B + 1) + PL * (v - z0 + 1); break;
+        case 0: l = T::west(u, v - z0); break;
+        case 1: l = T::east(u, v - z0); break;
+        case 2: l = T::cell(u, -1, v - z0 + 1); break;
+        default: l = T::cell(u, B, v - z0 + 1); break;
         }
         s_phi[l] = pv;
         s_ik[l] = kv;
@@ -342,13 +358,15 @@ __device__ __forceinline__ void fk_step_item(const StepArgs &a, int item, float 
     const float ih2 = __int_as_float(rec[1]);
 #pragma unroll
     for (int k = 0; k < ZT; k++) {
-        const int c = (tx + 1) + LW * (ty + 1) + PL * (k + 1);
+        const int c = T::cell(tx, ty, k + 1);
+        const int cw = tx > 0 ? c - 1 : T::west(ty, k);
+        const int ce = tx < B - 1 ? c + 1 : T::east(ty, k);
         const float pc = s_phi[c];
-        const float pW = s_phi[c - 1], pE = s_phi[c + 1];
-        const float pS = s_phi[c - LW], pN = s_phi[c + LW];
-        const float pB = s_phi[c - PL], pF = s_phi[c + PL];
-        const float pn = fk_cell(pc, pW, pE, pS, pN, pB, pF, s_ik[c], s_ik[c - 1], s_ik[c + 1], s_ik[c - LW],
-                                 s_ik[c + LW], s_ik[c - PL], s_ik[c + PL], breg[k], ih2, rho, dt);
+        const float pW = s_phi[cw], pE = s_phi[ce];
+        const float pS = s_phi[c - T::RW], pN = s_phi[c + T::RW];
+        const float pB = s_phi[c - T::PL], pF = s_phi[c + T::PL];
+        const float pn = fk_cell(pc, pW, pE, pS, pN, pB, pF, s_ik[c], s_ik[cw], s_ik[ce], s_ik[c - T::RW],
+                                 s_ik[c + T::RW], s_ik[c - T::PL], s_ik[c + T::PL], breg[k], ih2, rho, dt);
         a.phi_out[base + (z0 + k) * 256 + t] = pn;
     }
 }
@@ -357,8 +375,8 @@ template <int ZT>
 __global__ __launch_bounds__(256) void k_fk_step(StepArgs a)
 {
     constexpr int NS = B / ZT;
-    __shared__ float s_phi[(ZT + 2) * LW * LW];
-    __shared__ float s_ik[(ZT + 2) * LW * LW];
+    __shared__ float s_phi[SlabTile<ZT>::SIZE];
+    __shared__ float s_ik[SlabTile<ZT>::SIZE];
     const int item = xcd_remap(blockIdx.x, gridDim.x);
     if (item >= a.nwork * NS) return;  // padding workgroups (whole workgroup exits)
     fk_step_item<ZT>(a, item, s_phi, s_ik);
@@ -519,8 +537,8 @@ template <int ZT>
 __global__ __launch_bounds__(256) void k_fk_persistent(PersistArgs p)
 {
     constexpr int NS = B / ZT;
-    __shared__ float s_phi[(ZT + 2) * LW * LW];
-    __shared__ float s_ik[(ZT + 2) * LW * LW];
+    __shared__ float s_phi[SlabTile<ZT>::SIZE];
+    __shared__ float s_ik[SlabTile<ZT>::SIZE];
     cooperative_groups::grid_group grid = cooperative_groups::this_grid();
     const int per = gridDim.x >> 3;  // workgroups per XCD (the launcher uses a multiple of 8)
     const int xcd = blockIdx.x & 7, rank = blockIdx.x >> 3;

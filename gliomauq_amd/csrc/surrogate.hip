@@ -38,6 +38,22 @@ __device__ __forceinline__ bf16x8 zero8()
     return z;
 }
 
+// 8 input channels c .. c+7 (c a multiple of 8) of voxel (gx, gy, gz) of item
+// bb: below cup from the half-resolution tensor `up` at (gx/2, gy/2, gz/2)
+// (nearest 2x upsampling), else channel c - cup of `in` (edge in.D).
+__device__ __forceinline__ bf16x8 load_input8(const View &in, const View &up, int cup, int64_t bb, int gx, int gy,
+                                              int gz, int c)
+{
+    if (c < cup) {
+        const int64_t Du = up.D;
+        const int64_t vu = bb * Du * Du * Du + (gx >> 1) + Du * ((gy >> 1) + Du * (int64_t)(gz >> 1));
+        return load8(up.p + vu * up.ld + up.off + c);
+    }
+    const int64_t D = in.D;
+    const int64_t vn = bb * D * D * D + gx + D * (gy + D * (int64_t)gz);
+    return load8(in.p + vn * in.ld + in.off + (c - cup));
+}
+
 }  // namespace
 
 // 3^3 convolution by voxel bricks: one workgroup per (8x8x4-voxel output
@@ -85,10 +101,8 @@ __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
             const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
             const int ci = c0 + 8 * g;
             bf16x8 v = zero8();
-            if (ci < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
-                const int64_t vn = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
-                v = load8(a.in.p + vn * a.in.ld + a.in.off + ci);
-            }
+            if (ci < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D)
+                v = load_input8(a.in, a.up, a.cup, bb, gx, gy, gz, ci);
             *reinterpret_cast<bf16x8 *>(&s_in[h * CB_LD + 8 * g]) = v;
         }
         __syncthreads();
@@ -183,10 +197,8 @@ __global__ __launch_bounds__(256) void k_conv3_flat(ConvArgs a)
         const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
         const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
         bf16x8 v = zero8();
-        if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
-            const int64_t vn = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
-            v = load8(a.in.p + vn * a.in.ld + a.in.off + 8 * g);
-        }
+        if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D)
+            v = load_input8(a.in, a.up, a.cup, bb, gx, gy, gz, 8 * g);
         *reinterpret_cast<bf16x8 *>(&s_in[h * LD + 8 * g]) = v;
     }
     __syncthreads();
@@ -290,10 +302,8 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_brick(WgradArgs a, float *b
             const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
             const int c0 = ci0 + 8 * half;
             bf16x8 v = zero8();
-            if (c0 < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D) {
-                const int64_t vn = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
-                v = load8(a.x.p + vn * a.x.ld + a.x.off + c0);
-            }
+            if (c0 < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D)
+                v = load_input8(a.x, a.up, a.cup, bb, gx, gy, gz, c0);
             const int row = hz * 10 + hy;
 #pragma unroll
             for (int sh = 0; sh < 3; sh++) {
@@ -360,6 +370,21 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_brick(WgradArgs a, float *b
             bias_partial[(int64_t)bg * Cout + 32 * cog + t] = sb;
         }
     }
+}
+
+// out[i] = sum over chunks of partial[c*n + i] for few outputs and many
+// chunks: one workgroup per output, thread t sums chunks t, t+256, ... in
+// order, then a fixed tree over the workgroup (deterministic).
+__global__ __launch_bounds__(256) void k_reduce_chunks_few(const float *partial, int nchunks, int64_t n, float *out)
+{
+    __shared__ float red[4];
+    const int64_t i = blockIdx.x;
+    float s = 0.f;
+    for (int c = threadIdx.x; c < nchunks; c += 256) s += partial[(int64_t)c * n + i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[i] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // out[i] = sum over chunks c (in order) of partial[c*n + i]
@@ -463,25 +488,9 @@ __global__ __launch_bounds__(256) void k_avgpool_bwd(const float *dout, int64_t 
     f32x8_store(p, d);
 }
 
-// Nearest-neighbour 2x upsampling (forward) and its adjoint (sum of 8 children):
-// one thread per coarse voxel and 8 channels.
-__global__ __launch_bounds__(256) void k_upsample(View in, View out, int C, int batch, int lgi)
-{
-    const int G = C >> 3;
-    const int64_t n = ((int64_t)batch << (3 * lgi)) * G;
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    const int64_t v = e / G;
-    const int c = 8 * (int)(e - v * G);
-    const Vox q = vox_of(v, lgi);
-    const bf16x8 val = load8(in.p + v * in.ld + in.off + c);
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int64_t vo = vox_index(q.b, 2 * q.x + (k & 1), 2 * q.y + ((k >> 1) & 1), 2 * q.z + (k >> 2), lgi + 1);
-        *reinterpret_cast<bf16x8 *>(out.p + vo * out.ld + out.off + c) = val;
-    }
-}
-
+// Adjoint of nearest-neighbour 2x upsampling (sum of the 8 children): one
+// thread per coarse voxel and 8 channels.  (The forward upsampling is folded
+// into the convolutions' input reads, load_input8.)
 __global__ __launch_bounds__(256) void k_upsample_bwd(FView dout, float *din, int64_t din_ld, int C, int batch,
                                                       int lgi)
 {
@@ -559,11 +568,12 @@ __global__ __launch_bounds__(256) void k_head(HeadArgs a)
             const float dldp = (2.f * (p - tv) + a.lam * (s - m) * a.kappa) * invn;
             const float dz = dldp * p * (1.f - p);
             if (a.dh) {
+                float4 *dh4 = reinterpret_cast<float4 *>(a.dh + v * 16);
 #pragma unroll
-                for (int c = 0; c < 16; c++) {
-                    a.dh[v * 16 + c] = dz * a.w[c];
-                    part[c] = dz * h[c];
-                }
+                for (int q4 = 0; q4 < 4; q4++)
+                    dh4[q4] = make_float4(dz * a.w[4 * q4], dz * a.w[4 * q4 + 1], dz * a.w[4 * q4 + 2], dz * a.w[4 * q4 + 3]);
+#pragma unroll
+                for (int c = 0; c < 16; c++) part[c] = dz * h[c];
             }
             part[17] = dz;
         }
@@ -687,6 +697,9 @@ hipError_t launch_conv3(const ConvArgs &a, hipStream_t st)
     if (a.cin % 8 != 0 || a.in.ld % 8 != 0 || a.in.off % 8 != 0 || a.cout % 16 != 0 || nt > 12 || D < 8 ||
         D % 8 != 0)
         return hipErrorInvalidValue;
+    if (a.cup < 0 || a.cup % 8 != 0 || a.cup >= a.cin ||
+        (a.cup > 0 && (2 * a.up.D != D || a.up.ld % 8 != 0 || a.up.off % 8 != 0)))
+        return hipErrorInvalidValue;
     const int ntg = nt <= 4 ? nt : (nt % 4 == 0 ? 4 : 3);  // output tiles per workgroup
     if (nt % ntg != 0) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(a.batch * (D / 8) * (D / 8) * (D / 4)), (unsigned)(nt / ntg));
@@ -737,6 +750,9 @@ hipError_t launch_conv3_wgrad_brick(const WgradArgs &a, int ngroups, float *bias
     const int D = a.x.D;
     if (a.cout % 16 != 0 || a.cin % 8 != 0 || D < 8 || D % 8 != 0 || a.dz.D != D || ngroups < 1)
         return hipErrorInvalidValue;
+    if (a.cup < 0 || a.cup % 8 != 0 || a.cup >= a.cin ||
+        (a.cup > 0 && (2 * a.up.D != D || a.up.ld % 8 != 0 || a.up.off % 8 != 0)))
+        return hipErrorInvalidValue;
     const int nbricks = a.batch * (D / 8) * (D / 8) * (D / 4);
     const int per = (nbricks + ngroups - 1) / ngroups;
     const dim3 grid((unsigned)((a.cin + 15) / 16), (unsigned)((a.cout + 31) / 32), (unsigned)ngroups);
@@ -746,6 +762,10 @@ hipError_t launch_conv3_wgrad_brick(const WgradArgs &a, int ngroups, float *bias
 
 hipError_t launch_reduce_chunks(const float *partial, int nchunks, int64_t n, float *out, hipStream_t st)
 {
+    if (n <= 256 && nchunks >= 64) {  // a single workgroup would add every chunk serially
+        hipLaunchKernelGGL(k_reduce_chunks_few, dim3((unsigned)n), dim3(256), 0, st, partial, nchunks, n, out);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial, nchunks, n,
                        out);
     return hipGetLastError();
@@ -780,15 +800,6 @@ hipError_t launch_avgpool_bwd(const float *dout, int64_t dout_ld, const FView &d
     if (lgi < 1 || !ch8(C, dout_ld, 0) || !ch8(C, din.ld, din.off)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_avgpool_bwd, g1((int64_t)din.D * din.D * din.D * batch * (C / 8)), dim3(256), 0, st, dout,
                        dout_ld, din, C, batch, lgi);
-    return hipGetLastError();
-}
-
-hipError_t launch_upsample(const View &in, const View &out, int C, int batch, hipStream_t st)
-{
-    const int lgi = lg_edge(in.D);
-    if (lgi < 0 || out.D != 2 * in.D || !ch8(C, in.ld, in.off) || !ch8(C, out.ld, out.off)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_upsample, g1((int64_t)in.D * in.D * in.D * batch * (C / 8)), dim3(256), 0, st, in, out, C,
-                       batch, lgi);
     return hipGetLastError();
 }
 

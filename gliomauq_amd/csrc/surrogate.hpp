@@ -22,8 +22,14 @@ struct FView {
     int64_t off;
 };
 
+// A layer input of cin channels: channels [0, cup) are the half-resolution
+// tensor `up` upsampled 2x (nearest; the upsampling is folded into the
+// read), channels [cup, cin) come from `in` at the layer's resolution.  cup =
+// 0: every channel from `in`.
 struct ConvArgs {
-    View in;          // input (D^3 per batch item), cin channels at in.off
+    View in;          // input (D^3 per batch item), channels cup .. cin-1 at in.off
+    View up;          // channels 0 .. cup-1 at half resolution (cup > 0)
+    int cup;
     int cin, cout, batch;
     const __bf16 *w;  // [cout][27*cin]
     const float *bias;
@@ -35,7 +41,9 @@ struct ConvArgs {
 };
 
 struct WgradArgs {
-    View x;   // layer input, cin channels
+    View x;   // layer input, channels cup .. cin-1 (as ConvArgs::in)
+    View up;  // channels 0 .. cup-1 at half resolution (cup > 0)
+    int cup;
     View dz;  // gradient at the pre-activation, cout channels
     int cin, cout, batch;
     float *partial;  // [groups][cout][27*cin]
@@ -61,7 +69,6 @@ int wgrad_groups(int cin, int cout, int D, int batch);
 hipError_t launch_reduce_chunks(const float *partial, int nchunks, int64_t n, float *out, hipStream_t st);
 hipError_t launch_avgpool(const View &in, const View &out, int C, int batch, hipStream_t st);
 hipError_t launch_avgpool_bwd(const float *dout, int64_t dout_ld, const FView &din, int C, int batch, hipStream_t st);
-hipError_t launch_upsample(const View &in, const View &out, int C, int batch, hipStream_t st);
 hipError_t launch_upsample_bwd(const FView &dout, float *din, int64_t din_ld, int C, int batch, hipStream_t st);
 hipError_t launch_relu_bwd(const FView &dy, const View &y, __bf16 *dz, int C, int64_t nvox, hipStream_t st);
 hipError_t launch_head(const HeadArgs &a, hipStream_t st);

@@ -66,7 +66,7 @@ struct gq_sur {
     float *d_hw = nullptr;
     int64_t step = 0;
     // activations (bf16)
-    __bf16 *x0 = nullptr, *cat1 = nullptr, *p1 = nullptr, *cat2 = nullptr, *p2 = nullptr, *cat3 = nullptr,
+    __bf16 *x0 = nullptr, *e1 = nullptr, *p1 = nullptr, *e2 = nullptr, *p2 = nullptr, *e3 = nullptr,
            *p3 = nullptr, *e4 = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr, *dz = nullptr;
     // gradients (FP32)
     float *dh = nullptr, *dcat1 = nullptr, *dcat2 = nullptr, *dcat3 = nullptr, *dd2 = nullptr, *dd3 = nullptr,
@@ -148,11 +148,11 @@ struct gq_sur {
         d_hw = A<float>(16);
         const int64_t V64 = (int64_t)64 * 64 * 64 * batch, V32 = V64 / 8, V16 = V32 / 8, V8 = V16 / 8;
         x0 = A<__bf16>(V64 * 8);
-        cat1 = A<__bf16>(V64 * 48);
+        e1 = A<__bf16>(V64 * 16);
         p1 = A<__bf16>(V32 * 16);
-        cat2 = A<__bf16>(V32 * 96);
+        e2 = A<__bf16>(V32 * 32);
         p2 = A<__bf16>(V16 * 32);
-        cat3 = A<__bf16>(V16 * 192);
+        e3 = A<__bf16>(V16 * 64);
         p3 = A<__bf16>(V8 * 64);
         e4 = A<__bf16>(V8 * 128);
         d3 = A<__bf16>(V16 * 64);
@@ -200,11 +200,15 @@ struct gq_sur {
     static gq::sur::View V(__bf16 *p, int D, int64_t ld, int64_t off) { return gq::sur::View{p, D, ld, off}; }
     static gq::sur::FView F(float *p, int D, int64_t ld, int64_t off) { return gq::sur::FView{p, D, ld, off}; }
 
-    void conv_fwd(int l, gq::sur::View in, __bf16 *out, int64_t ld, int64_t off, int B)
+    // conv l on input channels [up (upsampled, cup channels), in]; bf16 out at (ld, off)
+    void conv_fwd(int l, gq::sur::View in, __bf16 *out, int64_t ld, int64_t off, int B,
+                  gq::sur::View up = gq::sur::View{}, int cup = 0)
     {
         const Conv &c = cv[l];
         gq::sur::ConvArgs a{};
         a.in = in;
+        a.up = up;
+        a.cup = cup;
         a.cin = c.cin;
         a.cout = c.cout;
         a.batch = B;
@@ -239,12 +243,14 @@ struct gq_sur {
         SCHK(gq::sur::launch_conv3(a, st));
     }
 
-    void conv_wgrad(int l, gq::sur::View x, int B)
+    void conv_wgrad(int l, gq::sur::View x, int B, gq::sur::View up = gq::sur::View{}, int cup = 0)
     {
         const Conv &c = cv[l];
         const int ng = gq::sur::wgrad_groups(c.cin, c.cout, c.D, B);
         gq::sur::WgradArgs a{};
         a.x = x;
+        a.up = up;
+        a.cup = cup;
         a.dz = V(dz, c.D, c.cout, 0);
         a.cin = c.cin;
         a.cout = c.cout;
@@ -269,19 +275,18 @@ struct gq_sur {
     {
         const int64_t D3 = 64 * 64 * 64;
         SCHK(gq::sur::launch_assemble_input(prm, u0, pw, pg, D3, B, x0, st));
-        conv_fwd(0, V(x0, 64, 8, 0), cat1, 48, 32, B);                      // E1 -> cat1[32..47]
-        SCHK(gq::sur::launch_avgpool(V(cat1, 64, 48, 32), V(p1, 32, 16, 0), 16, B, st));
-        conv_fwd(1, V(p1, 32, 16, 0), cat2, 96, 64, B);                     // E2 -> cat2[64..95]
-        SCHK(gq::sur::launch_avgpool(V(cat2, 32, 96, 64), V(p2, 16, 32, 0), 32, B, st));
-        conv_fwd(2, V(p2, 16, 32, 0), cat3, 192, 128, B);                   // E3 -> cat3[128..191]
-        SCHK(gq::sur::launch_avgpool(V(cat3, 16, 192, 128), V(p3, 8, 64, 0), 64, B, st));
-        conv_fwd(3, V(p3, 8, 64, 0), e4, 128, 0, B);                        // E4
-        SCHK(gq::sur::launch_upsample(V(e4, 8, 128, 0), V(cat3, 16, 192, 0), 128, B, st));
-        conv_fwd(4, V(cat3, 16, 192, 0), d3, 64, 0, B);                     // D3
-        SCHK(gq::sur::launch_upsample(V(d3, 16, 64, 0), V(cat2, 32, 96, 0), 64, B, st));
-        conv_fwd(5, V(cat2, 32, 96, 0), d2, 32, 0, B);                      // D2
-        SCHK(gq::sur::launch_upsample(V(d2, 32, 32, 0), V(cat1, 64, 48, 0), 32, B, st));
-        conv_fwd(6, V(cat1, 64, 48, 0), d1, 16, 0, B);                      // D1
+        // decoder inputs are [upsampled coarser output, skip]: the upsampling and the
+        // concatenation happen in the convolutions' reads (ConvArgs::up / cup)
+        conv_fwd(0, V(x0, 64, 8, 0), e1, 16, 0, B);                                   // E1
+        SCHK(gq::sur::launch_avgpool(V(e1, 64, 16, 0), V(p1, 32, 16, 0), 16, B, st));
+        conv_fwd(1, V(p1, 32, 16, 0), e2, 32, 0, B);                                  // E2
+        SCHK(gq::sur::launch_avgpool(V(e2, 32, 32, 0), V(p2, 16, 32, 0), 32, B, st));
+        conv_fwd(2, V(p2, 16, 32, 0), e3, 64, 0, B);                                  // E3
+        SCHK(gq::sur::launch_avgpool(V(e3, 16, 64, 0), V(p3, 8, 64, 0), 64, B, st));
+        conv_fwd(3, V(p3, 8, 64, 0), e4, 128, 0, B);                                  // E4
+        conv_fwd(4, V(e3, 16, 64, 0), d3, 64, 0, B, V(e4, 8, 128, 0), 128);          // D3 on [up(E4), E3]
+        conv_fwd(5, V(e2, 32, 32, 0), d2, 32, 0, B, V(d3, 16, 64, 0), 64);           // D2 on [up(D3), E2]
+        conv_fwd(6, V(e1, 64, 16, 0), d1, 16, 0, B, V(d2, 32, 32, 0), 32);           // D1 on [up(D2), E1]
         gq::sur::HeadArgs h{};
         h.h = V(d1, 64, 16, 0);
         h.w = d_hw;
@@ -303,17 +308,17 @@ struct gq_sur {
         const int64_t V64 = (int64_t)64 * 64 * 64 * B, V32 = V64 / 8, V16 = V32 / 8, V8 = V16 / 8;
         // D1
         SCHK(gq::sur::launch_relu_bwd(F(dh, 64, 16, 0), V(d1, 64, 16, 0), dz, 16, V64, st));
-        conv_wgrad(6, V(cat1, 64, 48, 0), B);
+        conv_wgrad(6, V(e1, 64, 16, 0), B, V(d2, 32, 32, 0), 32);
         conv_bwd_data(6, dcat1, 48, 0, B);
         // D2
         SCHK(gq::sur::launch_upsample_bwd(F(dcat1, 64, 48, 0), dd2, 32, 32, B, st));
         SCHK(gq::sur::launch_relu_bwd(F(dd2, 32, 32, 0), V(d2, 32, 32, 0), dz, 32, V32, st));
-        conv_wgrad(5, V(cat2, 32, 96, 0), B);
+        conv_wgrad(5, V(e2, 32, 32, 0), B, V(d3, 16, 64, 0), 64);
         conv_bwd_data(5, dcat2, 96, 0, B);
         // D3
         SCHK(gq::sur::launch_upsample_bwd(F(dcat2, 32, 96, 0), dd3, 64, 64, B, st));
         SCHK(gq::sur::launch_relu_bwd(F(dd3, 16, 64, 0), V(d3, 16, 64, 0), dz, 64, V16, st));
-        conv_wgrad(4, V(cat3, 16, 192, 0), B);
+        conv_wgrad(4, V(e3, 16, 64, 0), B, V(e4, 8, 128, 0), 128);
         conv_bwd_data(4, dcat3, 192, 0, B);
         // E4
         SCHK(gq::sur::launch_upsample_bwd(F(dcat3, 16, 192, 0), de4, 128, 128, B, st));
@@ -322,17 +327,17 @@ struct gq_sur {
         conv_bwd_data(3, dp3, 64, 0, B);
         // E3: skip gradient in dcat3[128..191] plus the pooled path
         SCHK(gq::sur::launch_avgpool_bwd(dp3, 64, F(dcat3, 16, 192, 128), 64, B, st));
-        SCHK(gq::sur::launch_relu_bwd(F(dcat3, 16, 192, 128), V(cat3, 16, 192, 128), dz, 64, V16, st));
+        SCHK(gq::sur::launch_relu_bwd(F(dcat3, 16, 192, 128), V(e3, 16, 64, 0), dz, 64, V16, st));
         conv_wgrad(2, V(p2, 16, 32, 0), B);
         conv_bwd_data(2, dp2, 32, 0, B);
         // E2
         SCHK(gq::sur::launch_avgpool_bwd(dp2, 32, F(dcat2, 32, 96, 64), 32, B, st));
-        SCHK(gq::sur::launch_relu_bwd(F(dcat2, 32, 96, 64), V(cat2, 32, 96, 64), dz, 32, V32, st));
+        SCHK(gq::sur::launch_relu_bwd(F(dcat2, 32, 96, 64), V(e2, 32, 32, 0), dz, 32, V32, st));
         conv_wgrad(1, V(p1, 32, 16, 0), B);
         conv_bwd_data(1, dp1, 16, 0, B);
         // E1
         SCHK(gq::sur::launch_avgpool_bwd(dp1, 16, F(dcat1, 64, 48, 32), 16, B, st));
-        SCHK(gq::sur::launch_relu_bwd(F(dcat1, 64, 48, 32), V(cat1, 64, 48, 32), dz, 16, V64, st));
+        SCHK(gq::sur::launch_relu_bwd(F(dcat1, 64, 48, 32), V(e1, 64, 16, 0), dz, 16, V64, st));
         conv_wgrad(0, V(x0, 64, 8, 0), B);
     }
 
