@@ -257,7 +257,9 @@ gq_status gq_loglik_flip(gq_ctx *ctx, const double params[4], double out[4]);
  * call only enqueues kernels on the stream and returns: it allocates nothing
  * and does not synchronise.  Its partial sums live in a fixed device array
  * of the library (48 KiB per GPU, kept until the process ends): calls that
- * overlap on one GPU race on it.  gpd is at most GQ_NOALLOC_MAX_GPD. */
+ * overlap on one GPU race on it.  The current HIP device must be the one that
+ * holds the arrays and the stream (the array's per-GPU copy is looked up on
+ * the current device).  gpd is at most GQ_NOALLOC_MAX_GPD. */
 #define GQ_NOALLOC_MAX_GPD 1024
 gq_status gq_loglik_flip_volume_noalloc(const double *model, int gpd, const uint8_t *t1, const uint8_t *t2,
                                         const float *pet, int obs_gpd, const double params[4], double *out,
