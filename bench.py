@@ -616,10 +616,27 @@ def surrogate_workload(args):
                 log("[bench]   %d of %d fields" % (i + len(blk), len(ths)))
         return out
 
-    log("[bench] surrogate: truth + %d training fields (adaptive 128^3, %d steps)" % (args.sur_fields, steps))
+    log("[bench] surrogate: truth + %d training fields (adaptive 128^3, %d steps)"
+        % (0 if args.sur_load else args.sur_fields, steps))
     u_true = solve_fields(tt[None])[0]
     obs64 = A.observations(u_true, seed=args.seed)
     span = np.array([0.15, 0.15, 1.5, 1.5, 1.5, 0.02])
+    mu = tt[[0, 1, 5]]
+    sd = span[[0, 1, 5]]
+    if args.sur_load:
+        # a surrogate trained by an earlier process of this workload (--sur-save)
+        z = np.load(args.sur_load)
+        sur_res = json.loads(str(z["result"]))
+        del gen
+        post_b = SM.AdaptivePosterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=C)
+        post_1 = SM.AdaptivePosterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=1)
+        sur = SU.Surrogate(batch=64, seed=args.seed)
+        sur.set_anatomy(pwm[half], pgm[half])
+        sur.set_params(z["params"])
+        spost = SM.SurrogatePosterior(sur, obs64, lo, hi, ratio=2, norm_mu=mu, norm_sd=sd)
+        log("[bench] surrogate: loaded %s (%d training fields, held-out error %.4f)"
+            % (args.sur_load, sur_res["train_fields"], sur_res["heldout_rel_l2_mean"]))
+        return surrogate_samplers(args, post_b, post_1, spost, sur_res, tt, lo, hi, span, rng)
     t0 = time.perf_counter()
     ths = tt + span * rng.uniform(-1, 1, (args.sur_fields + 256, 6))
     fields = solve_fields(ths)
@@ -633,8 +650,6 @@ def surrogate_workload(args):
     post_1 = SM.AdaptivePosterior(gq.Context(0), pwm, pgm, obs64, steps, dt, lo, hi, batch=1)
     sur = SU.Surrogate(batch=64, seed=args.seed)
     sur.set_anatomy(pwm[half], pgm[half])
-    mu = tt[[0, 1, 5]]
-    sd = span[[0, 1, 5]]
     spost = SM.SurrogatePosterior(sur, obs64, lo, hi, ratio=2, norm_mu=mu, norm_sd=sd)
     ntr = args.sur_fields
     P_all, U0_all = spost.inputs(ths)
@@ -669,7 +684,8 @@ def surrogate_workload(args):
     t_pred = (time.perf_counter() - t2) / reps
     pred_ms = sur.timing()["forward_ms"]
     f64_, _ = SU.conv_flops(64)
-    sur_res = {"train_fields": ntr, "field_source": "batched adaptive solver (levels 1-3 of 128^3), 64^3",
+    sur_res = {"train_fields": ntr, "epochs": args.sur_epochs,
+               "field_source": "batched adaptive solver (levels 1-3 of 128^3), 64^3",
                "gen_s": t_gen, "train_s": t_train, "train_steps": nsteps_tr,
                "loss_first": losses[0], "loss_last": float(np.mean(losses[-(ntr // B):])),
                "heldout_fields": len(held), "heldout_rel_l2_mean": float(err.mean()),
@@ -681,10 +697,20 @@ def surrogate_workload(args):
                "predict_ms_per_batch64": pred_ms, "predict_wall_s_per_batch64": t_pred}
     log("[bench] surrogate %s" % json.dumps(sur_res))
     del fields, P_all, U0_all
+    if args.sur_save:
+        np.savez(args.sur_save, params=sur.get_params(), result=json.dumps(sur_res))
+        log("[bench] surrogate: saved %s" % args.sur_save)
     if args.sur_train_only:
         print(json.dumps({"metric": "surrogate training only", "surrogate": sur_res}), flush=True)
         return
+    return surrogate_samplers(args, post_b, post_1, spost, sur_res, tt, lo, hi, span, rng)
 
+
+def surrogate_samplers(args, post_b, post_1, spost, sur_res, tt, lo, hi, span, rng):
+    """The sampler comparison of the surrogate workload (see surrogate_workload)."""
+    from gliomauq_amd import sampler as SM
+
+    C, steps = args.chains, args.sur_steps
     budget = args.mcmc_budget
     log("[bench] scale probe")
     scale = SM.probe_scales(post_b, tt, drop=2.0)
@@ -784,7 +810,8 @@ def surrogate_workload(args):
         "config": {"workload": "surrogate + delayed acceptance (Langevin first stage on the surrogate), full model "
                                "the adaptive 128^3 solver (levels 1-3, %d steps), observations 64^3; surrogate "
                                "64^3 encoder-decoder (16-128 channels, 3^3), bf16 MFMA, %d adaptive-solver fields, "
-                               "%d epochs; %d chains, %.0f s per sampler" % (steps, ntr, args.sur_epochs, C, budget),
+                               "%d epochs; %d chains, %.0f s per sampler"
+                               % (steps, sur_res["train_fields"], sur_res["epochs"], C, budget),
                    "seed": args.seed, "training_span": span.tolist(),
                    "preconditioner_sd": sd_ref.tolist(), "fd_step": fd.tolist()},
         "surrogate": sur_res,
@@ -871,6 +898,8 @@ def main():
     ap.add_argument("--sur-lr", type=float, default=3e-4)
     ap.add_argument("--sur-gen-batch", type=int, default=256)
     ap.add_argument("--sur-train-only", action="store_true")
+    ap.add_argument("--sur-save", default="", help="npz: the trained surrogate's parameters and training figures")
+    ap.add_argument("--sur-load", default="", help="npz written by --sur-save: skip generation and training")
     ap.add_argument("--sur-box", default="fixed", choices=["fixed", "probe"],
                     help="training parameters: fixed box around the mode, or mode +- K probe scales")
     ap.add_argument("--sur-box-k", type=float, default=4.0)

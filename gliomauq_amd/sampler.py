@@ -150,24 +150,37 @@ class SurrogatePosterior:
         self.sd = np.ones(3) if norm_sd is None else np.asarray(norm_sd, np.float64)
         self.n_evals = 0
 
-    def inputs(self, th):
+    def inputs(self, th, out=None):
+        """Surrogate inputs of parameter sets th: (normalised parameters,
+        position maps); the maps go to `out` ([>= len(th)][64][64][64]
+        float32) when given."""
         th = np.asarray(th, np.float64).reshape(-1, 6)
         p = np.zeros((len(th), 4), np.float32)
         p[:, :3] = (th[:, [0, 1, 5]] - self.mu) / self.sd
         n = 64
         i = np.arange(n, dtype=np.float64)
-        u0 = np.empty((len(th), n, n, n), np.float32)
-        for b, t in enumerate(th):
-            d = [i - t[2 + a] / self.ratio for a in range(3)]
-            d = [x - n * np.floor(x / n + 0.5) for x in d]
-            dz, dy, dx = np.meshgrid(d[2], d[1], d[0], indexing="ij")
-            u0[b] = np.exp(-(dx * dx + dy * dy + dz * dz) / (2 * self.sigma ** 2))  # position map, unit peak
+        # position map with unit peak, periodic distances: the Gaussian is the
+        # product of one 1-D factor per axis
+        d = i[None, None, :] - th[:, 2:5, None] / self.ratio  # [item][axis x, y, z][i]
+        d -= n * np.floor(d / n + 0.5)
+        g = np.exp(-(d * d) / (2 * self.sigma ** 2)).astype(np.float32)
+        u0 = np.empty((len(th), n, n, n), np.float32) if out is None else out[:len(th)]
+        np.multiply(g[:, 2, :, None, None], g[:, 1, None, :, None], out=u0)
+        u0 *= g[:, 0, None, None, :]
         return p, u0
+
+    def _inputs_scratch(self, th):
+        # inputs of a call consumed at once, in a buffer kept between calls
+        # (large fresh arrays cost more in page faults than in arithmetic)
+        m = len(np.asarray(th).reshape(-1, 6))
+        if getattr(self, "_buf", None) is None or len(self._buf) < m:
+            self._buf = np.empty((m, 64, 64, 64), np.float32)
+        return self.inputs(th, out=self._buf)
 
     def logpost(self, th):
         th = np.asarray(th, np.float64).reshape(-1, 6)
         inside = np.all((th >= self.lo) & (th <= self.hi), axis=1)
-        p, u0 = self.inputs(th)
+        p, u0 = self._inputs_scratch(th)
         ll = self.sur.loglik_flip(p, u0, self.lik)
         self.n_evals += len(th)
         return np.where(inside, ll[:, 0], -np.inf)
@@ -189,7 +202,7 @@ class SurrogatePosterior:
                 t[:, k] += sgn * step[k]
                 rows.append(t)
         allp = np.concatenate(rows)
-        p, u0 = self.inputs(allp)
+        p, u0 = self._inputs_scratch(allp)
         ll = self.sur.loglik_flip(p, u0, self.lik)
         self.n_evals += len(allp)
         inside = np.all((th >= self.lo) & (th <= self.hi), axis=1)

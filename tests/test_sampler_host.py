@@ -224,3 +224,35 @@ def test_da_mala_targets_the_full_posterior():
     assert n_full < 3000 * 32 and 0.4 < acc[500:].mean() < 0.75 and acc1[500:].mean() > acc[500:].mean()
     assert np.all(np.abs(x.mean(0) - MU) < 0.1 * SD)
     assert np.all(np.abs(x.std(0) / SD - 1) < 0.08)
+
+
+def test_surrogate_inputs_position_maps():
+    """The surrogate's position map (a separable product of three periodic
+    1-D Gaussian factors) against the 3-D Gaussian of the squared periodic
+    distance, evaluated directly; the scratch-buffer path gives the same."""
+
+    class NoSurrogate:
+        batch = 4
+
+        def set_observations(self, *a):
+            pass
+
+    lo, hi = np.zeros(6), np.full(6, 200.0)
+    sp = SM.SurrogatePosterior(NoSurrogate(), (None, None, None), lo, hi, ratio=2, norm_mu=np.zeros(3),
+                               norm_sd=np.ones(3))
+    th = np.array([[0.1, 0.2, 10.0, 64.0, 127.0, 0.1], [0.3, 0.4, 1.0, 126.5, 70.25, 0.2],
+                   [0.5, 0.6, 63.0, 0.0, 33.3, 0.3]])
+    p, u = sp.inputs(th)
+    n = 64
+    i = np.arange(n, dtype=np.float64)
+    for b, t in enumerate(th):
+        d = [i - t[2 + a] / 2 for a in range(3)]
+        d = [x - n * np.floor(x / n + 0.5) for x in d]
+        dz, dy, dx = np.meshgrid(d[2], d[1], d[0], indexing="ij")
+        ref = np.exp(-(dx * dx + dy * dy + dz * dz) / (2 * sp.sigma ** 2))
+        assert np.allclose(u[b], ref, rtol=1e-6, atol=1e-7), b
+        # unit peak at the nearest voxel of the centre (periodic)
+        assert u[b].max() <= 1.0 and u[b].max() > 0.5
+    assert np.allclose(p[:, :3], th[:, [0, 1, 5]])
+    p2, u2 = sp._inputs_scratch(th)
+    assert np.array_equal(u2, u) and np.array_equal(p2, p)
