@@ -192,11 +192,12 @@ struct gq_ctx {
     std::vector<int> work;             // live slots: samples in sample_order, leaves in Morton order
     std::vector<int> work_begin;       // [S+1] per position in sample_order
     std::vector<int> face_begin;       // [S+1] cross-level faces per position
-    DevBuf<int> d_level, d_index, d_fnbr, d_fbase, d_frow, d_face_src, d_nbr26, d_lutoff, d_slot_sample, d_counts;
+    DevBuf<int> d_level, d_index, d_fnbr, d_fbase, d_face_src, d_nbr26, d_lutoff, d_slot_sample, d_counts;
     // per-topology lists in one transfer: work list, sample face-row bases, base-level and finest look-ups,
     // cross-level faces, changed slot rows
     PackedInts tabs;
     int *p_work = nullptr, *p_sfbase = nullptr, *p_lut = nullptr, *p_lutf = nullptr, *p_faces = nullptr;
+    int *p_wrec = nullptr;  // per-work-item step records (gq::WREC ints each), in the same transfer
     PackedInts lists;  // slot lists of one adaptation kernel sequence
     PackedInts flist;  // slots whose face 1/K is recomputed
     PinnedBuf<double> h_det;   // FWT details read back
@@ -209,7 +210,6 @@ struct gq_ctx {
     DevBuf<double> d_ghost;  // cross-level lab points (3 fields for a split)
     DevBuf<float> d_face_w, d_fik, d_gphi, d_ih2, d_sprm;
     DevBuf<float> d_ikc;          // per-cell 1/K codes of the step (pool layout)
-    DevBuf<int> d_wrec;           // per-work-item step records (gq::WREC ints each)
     bool fuse2 = false;           // fuse step pairs without cross-level faces into one launch (opt-in)
     std::vector<int> buf_after;   // advance(): ping-pong buffer holding the state after each step
     int last_nlaunch = 0;         // advance(): kernel launches of the last call
@@ -515,19 +515,41 @@ struct gq_ctx {
                     r++;
                 }
         }
-        tabs.upload({&work, &sfbase, &lut_slot, &lutf, &faces, &packed}, st);
+        // per-work-item step records: slot, 1/h^2 of its level (float bits),
+        // face neighbours, absolute first face-ghost rows, sample
+        static const std::vector<float> ih2 = [] {
+            std::vector<float> v(16);
+            for (int l = 0; l < 16; l++) {
+                const float h = (float)(std::ldexp(1.0, -l) / gq::BS);  // BlockInfo::h (Real)
+                const double hd = h;
+                v[l] = (float)(1. / (hd * hd));
+            }
+            return v;
+        }();
+        std::vector<int> wrec(std::max<size_t>(1, work.size()) * gq::WREC, 0);
+        for (size_t i = 0; i < work.size(); i++) {
+            const int sl = work[i];
+            int *r = wrec.data() + i * gq::WREC;
+            r[0] = sl;
+            std::memcpy(&r[1], &ih2[h_level[sl]], sizeof(float));
+            for (int f = 0; f < 6; f++) {
+                r[2 + f] = h_fnbr[(size_t)sl * 6 + f];
+                const int fb = h_fbase[(size_t)sl * 6 + f];
+                r[8 + f] = fb >= 0 ? sfbase[h_ssample[sl]] + fb : -1;
+            }
+            r[14] = h_ssample[sl];
+        }
+        tabs.upload({&work, &sfbase, &lut_slot, &lutf, &faces, &packed, &wrec}, st);
         p_work = tabs.ptr(0);
         p_sfbase = tabs.ptr(1);
         p_lut = tabs.ptr(2);
         p_lutf = tabs.ptr(3);
         p_faces = tabs.ptr(4);
+        p_wrec = tabs.ptr(6);
         if (!packed.empty()) {
             gq::SlotTables t{d_level.p, d_index.p, d_slot_sample.p, d_lutoff.p, d_fnbr.p, d_fbase.p, d_nbr26.p};
             HIPCHK(gq::launch_scatter_slot_rows(t, tabs.ptr(5), (int)(packed.size() / ROW), st));
         }
-        // absolute face-row bases for the step (after the slot rows and the sample bases are in place)
-        d_frow.reserve((size_t)cap * 6);
-        HIPCHK(gq::launch_face_rows_abs(p_work, (int)work.size(), d_fbase.p, p_sfbase, d_slot_sample.p, d_frow.p, st));
         face_nrows = (int)faces.size() * 256;
         face_rows_valid = false;
         if (d_fik.n < (size_t)cap * 6 * 256 || d_ikc.n < (size_t)cap * gq::BS3) {
@@ -536,18 +558,9 @@ struct gq_ctx {
             fik_valid = false;
         }
         if (!d_ih2.p) {
-            std::vector<float> ih2(16);
-            for (int l = 0; l < 16; l++) {
-                const float h = (float)(std::ldexp(1.0, -l) / gq::BS);  // BlockInfo::h (Real)
-                const double hd = h;
-                ih2[l] = (float)(1. / (hd * hd));
-            }
             d_ih2.upload(ih2, st);
             pageable = true;
         }
-        d_wrec.reserve((size_t)std::max<size_t>(1, work.size()) * gq::WREC);
-        HIPCHK(gq::launch_work_records(p_work, (int)work.size(), d_fnbr.p, d_frow.p, d_level.p, d_ih2.p,
-                                       d_slot_sample.p, d_wrec.p, st));
         // the packed lists went through pinned staging; copies from pageable vectors finish first
         if (pageable) HIPCHK(hipStreamSynchronize(st));
         tables_valid = true;
@@ -650,7 +663,6 @@ struct gq_ctx {
         if (slots.empty()) return out;
         ScopeTimer tm(t_fwt);
         n_fwt++;
-        d_dbl.reserve(slots.size());
         const gq::LabArgs la = lab_args();
         const int order = samples[0].grid->order();
         d_ghost.reserve(slots.size() * (size_t)gq::lab_points(order));
@@ -660,9 +672,10 @@ struct gq_ctx {
         lists.upload({&slots, &items}, st);
         HIPCHK(gq::launch_lab_ghosts(la, fl, 1, lists.ptr(0), lists.ptr(1), (int)items.size(), d_ghost.p, 0, false,
                                      st));
-        HIPCHK(gq::launch_fwt_detail(order, la, d_ghost.p, phi[cur], lists.ptr(0), (int)slots.size(), d_dbl.p, st));
+        // the details go straight to page-locked host memory (no copy launch);
+        // they are visible to the host once the stream has synchronised
         h_det.reserve(out.size(), st);
-        HIPCHK(hipMemcpyAsync(h_det.p, d_dbl.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(gq::launch_fwt_detail(order, la, d_ghost.p, phi[cur], lists.ptr(0), (int)slots.size(), h_det.p, st));
         HIPCHK(hipStreamSynchronize(st));
         std::copy(h_det.p, h_det.p + out.size(), out.begin());
         return out;
@@ -850,6 +863,23 @@ struct gq_ctx {
                 if (!live[k]) continue;
                 const Sample &sm = samples[which[k]];
                 const auto &L = sm.grid->leaves();
+                if (loops > 0) {
+                    // Only candidates (leaves not yet found nice) can be
+                    // flagged, so the parents this pass could collapse are
+                    // among those the plan collapses with every candidate
+                    // flagged.  None: the pass collapses nothing whatever the
+                    // details are and the loop ends here, as it would after
+                    // the transform (MRAGScienceCore.h:161-162); the transform
+                    // is skipped.
+                    std::unordered_set<int> cand_ids;
+                    for (const auto &lf : L)
+                        if (!nice[k].count(lf.block_id)) cand_ids.insert(lf.block_id);
+                    ScopeTimer tm(t_plan);
+                    if (sm.grid->plan_compress(cand_ids).empty()) {
+                        live[k] = 0;
+                        continue;
+                    }
+                }
                 for (int li : sm.t.order) {
                     if (nice[k].count(L[li].block_id)) continue;
                     cand_slots.push_back(L[li].slot);
@@ -996,10 +1026,9 @@ struct gq_ctx {
         a.work = p_work;
         a.nwork = (int)work.size();
         a.fnbr = d_fnbr.p;
-        a.frow = d_frow.p;
         a.fik = d_fik.p;
         a.ikc = step_coeffs ? d_ikc.p : nullptr;
-        a.wrec = d_wrec.p;
+        a.wrec = p_wrec;
         a.nbr26 = d_nbr26.p;
         a.gphi = d_gphi.p;
         a.fptr = d_face_ptr.p;
@@ -1109,7 +1138,7 @@ struct gq_ctx {
                 std::memcpy(&b, &f, 4);
                 key.push_back(b);
             };
-            pt(a.pw), pt(a.pg), pt(a.work), pt(a.fnbr), pt(a.frow), pt(a.fik), pt(a.ikc), pt(a.gphi), pt(a.fptr);
+            pt(a.pw), pt(a.pg), pt(a.work), pt(a.fnbr), pt(a.fik), pt(a.ikc), pt(a.gphi), pt(a.fptr);
             pt(a.fsrc), pt(a.fw), pt(a.level), pt(a.ih2_lvl), pt(a.slot_sample), pt(a.sprm), pt(a.wrec), pt(a.nbr26);
             fb(a.Dw), fb(a.Dg), fb(a.rho), fb(a.dt);
             pt(phi[0]), pt(phi[1]), pt(d_face_ptr.p), pt(d_face_src.p), pt(d_face_w.p), pt(d_gphi.p);

@@ -210,22 +210,31 @@ std::vector<std::array<int, 4>> Grid::plan_compress(const std::unordered_set<int
 {
     // Compressor::createPlan (MRAGCompressor.h:68-118): collapse a parent
     // whose 8 children are all leaves, all flagged, and whose children's
-    // neighbours are within one level of the parent.
-    std::set<std::array<int, 4>> parents;
+    // neighbours are within one level of the parent.  Flagged leaves grouped
+    // by parent: eight distinct leaves with one parent are its eight
+    // children.  Parents come out in (level, x, y, z) order.
+    std::vector<uint64_t> keys;
+    keys.reserve(flagged_ids.size());
     for (const Leaf &lf : leaves_)
-        if (lf.level > 0) parents.insert({lf.level - 1, lf.idx[0] >> 1, lf.idx[1] >> 1, lf.idx[2] >> 1});
+        if (lf.level > 0 && flagged_ids.count(lf.block_id))
+            keys.push_back(leaf_key(lf.level - 1, lf.idx[0] >> 1, lf.idx[1] >> 1, lf.idx[2] >> 1));
+    std::sort(keys.begin(), keys.end());
     std::vector<std::array<int, 4>> out;
-    for (const auto &p : parents) {
-        int ch[8];
+    for (size_t a = 0; a + 8 <= keys.size();) {
+        if (keys[a] != keys[a + 7]) {
+            a++;
+            continue;
+        }
+        const uint64_t k = keys[a];
+        const std::array<int, 4> p = {(int)(k >> 60), (int)((k >> 40) & 0xfffff), (int)((k >> 20) & 0xfffff),
+                                      (int)(k & 0xfffff)};
         bool ok = true;
         for (int c = 0; c < 8 && ok; c++) {
-            ch[c] = find(p[0] + 1, 2 * p[1] + (c & 1), 2 * p[2] + ((c >> 1) & 1), 2 * p[3] + ((c >> 2) & 1));
-            ok = ch[c] >= 0 && flagged_ids.count(leaves_[ch[c]].block_id);
+            const int ch = find(p[0] + 1, 2 * p[1] + (c & 1), 2 * p[2] + ((c >> 1) & 1), 2 * p[3] + ((c >> 2) & 1));
+            ok = ch >= 0 && !any_ring(ch, [&](int q, bool) { return std::abs(leaves_[q].level - p[0]) > 1; });
         }
-        if (!ok) continue;
-        for (int c = 0; c < 8 && ok; c++)
-            ok = !any_ring(ch[c], [&](int k, bool) { return std::abs(leaves_[k].level - p[0]) > 1; });
         if (ok) out.push_back(p);
+        a += 8;
     }
     return out;
 }
@@ -355,7 +364,7 @@ void build_sample_tables(const Grid &g, int *fnbr, int *nbr26, SampleTables &t)
         }
     }
     t.lut.resize(gl.size());
-    for (size_t k = 0; k < gl.size(); k++) t.lut[k] = L[gl[k]].slot;
+    for (size_t k = 0; k < gl.size(); k++) t.lut[k] = L[gl[k]].slot | L[gl[k]].level << LUT_LEVEL_SHIFT;
     // base-level look-up for the uniform dump / likelihood
     const int bpd = g.blocks_per_dim();
     const int bl = g.base_level();

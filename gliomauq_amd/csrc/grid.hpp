@@ -27,6 +27,10 @@ namespace gq {
 
 constexpr int BS = 16;          // cells per block edge
 constexpr int BS3 = BS * BS * BS;
+// device look-up entries: slot | level << LUT_LEVEL_SHIFT (slots stay below
+// 2^19: the pool holds at most 2^31 cells; levels are at most 8)
+constexpr int LUT_LEVEL_SHIFT = 24;
+constexpr int LUT_SLOT_MASK = (1 << LUT_LEVEL_SHIFT) - 1;
 
 struct Leaf {
     int level;
@@ -113,7 +117,7 @@ inline uint64_t leaf_key(int l, int x, int y, int z)
 struct SampleTables {
     std::vector<int> order;     // leaf indices in Morton order
     std::vector<int> faces;     // slot*6+f of every cross-level face, leaves in Morton order
-    std::vector<int> lut;       // (2^lut_level)^3 finest block cells -> slot
+    std::vector<int> lut;       // (2^lut_level)^3 finest block cells -> slot | level << LUT_LEVEL_SHIFT
     std::vector<int> lut_slot;  // bpd^3 base-level blocks -> slot (uniform view)
 };
 

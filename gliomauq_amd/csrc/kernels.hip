@@ -567,39 +567,6 @@ __global__ __launch_bounds__(256) void k_fk_persistent(PersistArgs p)
     }
 }
 
-// Absolute first face-ghost row of every face of the listed slots (one table
-// read in the step instead of the sample's base through the slot's sample).
-__global__ __launch_bounds__(256) void k_face_rows_abs(const int *__restrict__ work, int nwork,
-                                                       const int *__restrict__ fbase, const int *__restrict__ sfbase,
-                                                       const int *__restrict__ slot_sample, int *__restrict__ frow)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nwork * 6) return;
-    const int slot = work[i / 6], f = i % 6;
-    const int fb = fbase[slot * 6 + f];
-    frow[slot * 6 + f] = fb >= 0 ? sfbase[slot_sample[slot]] + fb : -1;
-}
-
-// Per-work-item records of the step (built with the tables).
-__global__ __launch_bounds__(256) void k_work_records(const int *__restrict__ work, int nwork,
-                                                      const int *__restrict__ fnbr, const int *__restrict__ frow,
-                                                      const int *__restrict__ level, const float *__restrict__ ih2_lvl,
-                                                      const int *__restrict__ slot_sample, int *__restrict__ wrec)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nwork) return;
-    const int slot = work[i];
-    int *r = wrec + (size_t)i * WREC;
-    r[0] = slot;
-    r[1] = __float_as_int(ih2_lvl[level[slot]]);
-    for (int f = 0; f < 6; f++) {
-        r[2 + f] = fnbr[slot * 6 + f];
-        r[8 + f] = frow[slot * 6 + f];
-    }
-    r[14] = slot_sample[slot];
-    r[15] = 0;
-}
-
 // 1/K of every face-halo cell and, when ikc is set, the per-cell step
 // coefficients of the block (static between topology / parameter changes).
 __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw, const float *__restrict__ pg,
@@ -612,17 +579,18 @@ __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw,
                                                    float *__restrict__ ikc, float Dw, float Dg,
                                                    const float *__restrict__ sprm)
 {
+    // blockIdx.y: 0-5 the face halos, 6 the block's codes (independent
+    // chains of loads in separate workgroups)
     const int wi = blockIdx.x;
     if (wi >= nwork) return;
     const int slot = work[wi];
-    const int rb = sfbase[slot_sample[slot]];
     if (sprm) {
         Dw = sprm[4 * slot_sample[slot]];
         Dg = sprm[4 * slot_sample[slot] + 1];
     }
     const int t = threadIdx.x;
-#pragma unroll
-    for (int f = 0; f < 6; f++) {
+    const int f = blockIdx.y;
+    if (f < 6) {
         const int nb = fnbr[slot * 6 + f];
         float wm, gm;
         if (nb >= 0) {
@@ -630,7 +598,7 @@ __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw,
             wm = pw[i];
             gm = pg[i];
         } else {
-            const int r = rb + fbase[slot * 6 + f] + t;
+            const int r = sfbase[slot_sample[slot]] + fbase[slot * 6 + f] + t;
             wm = 0.f;
             gm = 0.f;
             // terms in order, eight loaded per round trip
@@ -658,8 +626,7 @@ __global__ __launch_bounds__(256) void k_face_prep(const float *__restrict__ pw,
             }
         }
         fik[((size_t)slot * 6 + f) * 256 + t] = inv_k(wm, gm, Dw, Dg);
-    }
-    if (ikc) {
+    } else if (ikc) {
         const size_t b0 = (size_t)slot * B3;
         float wv[16], gv[16];
 #pragma unroll
@@ -821,8 +788,10 @@ __device__ __forceinline__ double hs_w(int order, int i)
     }
 }
 
-__device__ __forceinline__ int lying_slot(const LabArgs &g, const int *__restrict__ lut, int lam, int64_t px,
-                                          int64_t py, int64_t pz)
+// Look-up entry of the leaf holding the level-lam point p: its slot and its
+// level in one word (LUT_LEVEL_SHIFT), so resolving a point costs one load.
+__device__ __forceinline__ int lying_entry(const LabArgs &g, const int *__restrict__ lut, int lam, int64_t px,
+                                           int64_t py, int64_t pz)
 {
     // finest block cell: floor(p * 2^(L-lam) / 16), periodic (power-of-two wrap)
     const int sh = g.lut_level - lam;
@@ -830,6 +799,8 @@ __device__ __forceinline__ int lying_slot(const LabArgs &g, const int *__restric
     const int64_t fx = ((px << sh) >> 4) & m, fy = ((py << sh) >> 4) & m, fz = ((pz << sh) >> 4) & m;
     return lut[fx + (m + 1) * (fy + (m + 1) * fz)];
 }
+__device__ __forceinline__ int entry_slot(int e) { return e & LUT_SLOT_MASK; }
+__device__ __forceinline__ int entry_level(int e) { return e >> LUT_LEVEL_SHIFT; }
 
 struct DfsFrame {
     int lam;
@@ -855,8 +826,8 @@ __device__ void ghost_dfs(const LabArgs &g, const int *__restrict__ lut, const f
     while (true) {
         // resolve (lam, p, w) to a value or a synthesis frame
         while (true) {
-            const int slot = lying_slot(g, lut, lam, p[0], p[1], p[2]);
-            const int l = g.level[slot];
+            const int e = lying_entry(g, lut, lam, p[0], p[1], p[2]);
+            const int slot = entry_slot(e), l = entry_level(e);
             if (l == lam) {
                 const int64_t msk = ((int64_t)16 << lam) - 1;
                 const int cx = (int)((p[0] & msk) - (int64_t)g.index[slot * 3 + 0] * 16);
@@ -953,7 +924,10 @@ __device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, co
         }
     }
     const int nc = n[0] * n[1] * n[2];
-    constexpr int NB = MODE == 1 ? 8 : 4;  // children per batch (register pressure of the loads and stores)
+    // children per batch (register pressure of the loads and stores): a
+    // batch resolves with two rounds of loads, look-up entries (slot and
+    // level in one word) and then values
+    constexpr int NB = MODE == 1 ? 8 : 4;
     for (int c0 = 0; c0 < nc; c0 += NB) {
         // a child i at level cl lies in a leaf of level cl (value at i) or of a
         // finer level (the walk descends once: the point 2i at level cl+1,
@@ -968,8 +942,9 @@ __device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, co
                 __dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), hs_w(g.order, (int)(2 * iy - p[1]))),
                 hs_w(g.order, (int)(2 * iz - p[2])));
             cw[k] = __dmul_rn(w, ww);
-            sa[k] = lying_slot(g, lut, cl, ix, iy, iz);
-            sb[k] = lying_slot(g, lut, cl + 1, 2 * ix, 2 * iy, 2 * iz);
+            const bool in = c0 + k < nc;
+            sa[k] = in ? lying_entry(g, lut, cl, ix, iy, iz) : 0;
+            sb[k] = in ? lying_entry(g, lut, cl + 1, 2 * ix, 2 * iy, 2 * iz) : 0;
             ca[k] = (int)((ix & 15) + 16 * ((iy & 15) + 16 * (iz & 15)));
             cb[k] = (int)(((2 * ix) & 15) + 16 * (((2 * iy) & 15) + 16 * ((2 * iz) & 15)));
         }
@@ -977,9 +952,9 @@ __device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, co
         bool plain = true;  // every child resolved by at most one descent
 #pragma unroll
         for (int k = 0; k < NB; k++) {
-            const int la = g.level[sa[k]], lb = g.level[sb[k]];
+            const int la = entry_level(sa[k]), lb = entry_level(sb[k]);
             const bool fine = la > cl;
-            sl[k] = fine ? sb[k] : sa[k];
+            sl[k] = entry_slot(fine ? sb[k] : sa[k]);
             cell[k] = fine ? cb[k] : ca[k];
             plain = plain && (c0 + k >= nc || la == cl || (fine && lb == cl + 1));
         }
@@ -987,7 +962,8 @@ __device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, co
 #pragma unroll
         for (int k = 0; k < NB; k++)
 #pragma unroll
-            for (int f = 0; f < NF; f++) val[k][f] = MODE == 0 ? flds[f][(size_t)sl[k] * 4096 + cell[k]] : 0.f;
+            for (int f = 0; f < NF; f++)
+                val[k][f] = MODE == 0 && c0 + k < nc ? flds[f][(size_t)sl[k] * 4096 + cell[k]] : 0.f;
         if (!plain) {
             for (int c = c0; c < nc; c++) {
                 const int64_t ix = i0[0] + c % n[0], iy = i0[1] + (c / n[0]) % n[1], iz = i0[2] + c / (n[0] * n[1]);
@@ -995,8 +971,9 @@ __device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, co
                     __dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), hs_w(g.order, (int)(2 * iy - p[1]))),
                     hs_w(g.order, (int)(2 * iz - p[2])));
                 const double cwc = __dmul_rn(w, ww);
-                const int slot = lying_slot(g, lut, cl, ix, iy, iz);
-                if (g.level[slot] == cl) {
+                const int e = lying_entry(g, lut, cl, ix, iy, iz);
+                const int slot = entry_slot(e);
+                if (entry_level(e) == cl) {
                     const int ce = (int)((ix & 15) + 16 * ((iy & 15) + 16 * (iz & 15)));
                     if (MODE == 0) {
 #pragma unroll
@@ -1042,8 +1019,8 @@ __device__ void ghost_value(const LabArgs &g, const int *__restrict__ lut, const
                             int64_t px, int64_t py, int64_t pz, double *acc)
 {
     for (;;) {
-        const int slot = lying_slot(g, lut, lam, px, py, pz);
-        const int l = g.level[slot];
+        const int e = lying_entry(g, lut, lam, px, py, pz);
+        const int slot = entry_slot(e), l = entry_level(e);
         if (l == lam) {
             const size_t i = (size_t)slot * 4096 + (px & 15) + 16 * ((py & 15) + 16 * (pz & 15));
 #pragma unroll
@@ -1215,8 +1192,8 @@ __device__ void ghost_terms(const LabArgs &g, const int *__restrict__ lut, int l
                             int64_t pz, int *count, int *src, float *wout)
 {
     for (;;) {
-        const int slot = lying_slot(g, lut, lam, px, py, pz);
-        const int l = g.level[slot];
+        const int e = lying_entry(g, lut, lam, px, py, pz);
+        const int slot = entry_slot(e), l = entry_level(e);
         if (l == lam) {
             if (MODE == 2) {
                 src[*count] = slot * 4096 + (int)((px & 15) + 16 * ((py & 15) + 16 * (pz & 15)));
@@ -1489,16 +1466,29 @@ __global__ __launch_bounds__(256) void k_split(LabArgs a, const double *__restri
 __global__ __launch_bounds__(256) void k_collapse(float *phi, float *pw, float *pg, const int *__restrict__ parents,
                                                   const int *__restrict__ children)
 {
+    // blockIdx.y = child c: its 8^3 even points fill octant c of the parent,
+    // two cells per thread, every load issued before the first store (the
+    // parent's slot is a fresh slot: no child reads it)
+    const int c = blockIdx.y;
     const int pslot = parents[blockIdx.x];
-    for (int q = threadIdx.x; q < B3; q += blockDim.x) {
-        const int x = q & 15, y = (q >> 4) & 15, z = q >> 8;
-        const int c = (x >> 3) + 2 * ((y >> 3) + 2 * (z >> 3));
-        const int cs = children[blockIdx.x * 8 + c];
-        const size_t src = (size_t)cs * B3 + 2 * (x & 7) + B * (2 * (y & 7) + B * 2 * (z & 7));
-        const size_t dst = (size_t)pslot * B3 + q;
-        phi[dst] = phi[src];
-        pw[dst] = pw[src];
-        pg[dst] = pg[src];
+    const int cs = children[blockIdx.x * 8 + c];
+    float v[2][3];
+    size_t dst[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int q = threadIdx.x + 256 * k;  // 0..511 over the 8^3 octant
+        const int x = q & 7, y = (q >> 3) & 7, z = q >> 6;
+        const size_t src = (size_t)cs * B3 + 2 * x + B * (2 * y + B * 2 * z);
+        dst[k] = (size_t)pslot * B3 + (x + 8 * (c & 1)) + B * ((y + 8 * ((c >> 1) & 1)) + B * (z + 8 * (c >> 2)));
+        v[k][0] = phi[src];
+        v[k][1] = pw[src];
+        v[k][2] = pg[src];
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        phi[dst[k]] = v[k][0];
+        pw[dst[k]] = v[k][1];
+        pg[dst[k]] = v[k][2];
     }
 }
 
@@ -1862,31 +1852,13 @@ hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *s
     return hipGetLastError();
 }
 
-hipError_t launch_face_rows_abs(const int *work, int nwork, const int *fbase, const int *sfbase,
-                                const int *slot_sample, int *frow, hipStream_t st)
-{
-    if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_face_rows_abs, dim3((unsigned)((nwork * 6 + 255) / 256)), dim3(256), 0, st, work, nwork,
-                       fbase, sfbase, slot_sample, frow);
-    return hipGetLastError();
-}
-
-hipError_t launch_work_records(const int *work, int nwork, const int *fnbr, const int *frow, const int *level,
-                               const float *ih2_lvl, const int *slot_sample, int *wrec, hipStream_t st)
-{
-    if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_work_records, dim3((unsigned)((nwork + 255) / 256)), dim3(256), 0, st, work, nwork, fnbr,
-                       frow, level, ih2_lvl, slot_sample, wrec);
-    return hipGetLastError();
-}
-
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
                             const int *fbase, const int *sfbase, const int *slot_sample, const int64_t *ptr,
                             const int *src, const float *w, float *fik, float *ikc, float Dw, float Dg,
                             const float *sprm, hipStream_t st)
 {
     if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_face_prep, dim3(nwork), dim3(256), 0, st, pw, pg, work, nwork, fnbr, fbase, sfbase,
+    hipLaunchKernelGGL(k_face_prep, dim3(nwork, ikc ? 7 : 6), dim3(256), 0, st, pw, pg, work, nwork, fnbr, fbase, sfbase,
                        slot_sample, ptr, src, w, fik, ikc, Dw, Dg, sprm);
     return hipGetLastError();
 }
@@ -1983,7 +1955,7 @@ hipError_t launch_collapse(float *phi, float *pw, float *pg, const int *parents,
                            hipStream_t st)
 {
     if (nparents <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_collapse, dim3(nparents), dim3(256), 0, st, phi, pw, pg, parents, children);
+    hipLaunchKernelGGL(k_collapse, dim3(nparents, 8), dim3(256), 0, st, phi, pw, pg, parents, children);
     return hipGetLastError();
 }
 

@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "grid.hpp"  // BS, look-up entry layout
+
 namespace gq {
 
 struct StepArgs {
@@ -16,7 +18,6 @@ struct StepArgs {
     const int *work;   // Morton-ordered slots
     int nwork;
     const int *fnbr;   // [slot*6+f]
-    const int *frow;   // [slot*6+f] first face-ghost row of the face (absolute; unused for same-level faces)
     const float *fik;  // [slot*6+f][256]
     const float *ikc;  // [slot*4096] per-cell 1/K codes (k_face_prep), or null: 1/K from pw, pg every step
     const float *gphi; // cross-level face ghosts of phi_in, or null: the step evaluates the rows itself
@@ -28,7 +29,8 @@ struct StepArgs {
     float Dw, Dg, rho, dt;     // used when sprm is null
     const int *slot_sample;    // [slot] sample of the block (ensembles)
     const float *sprm;         // [sample*4] Dw, Dg, rho, dt (ensembles) or null
-    // [w*16] per work item: slot, 1/h^2 (float bits), fnbr[6], frow[6], sample, 0 -- one scalar load
+    // [w*16] per work item: slot, 1/h^2 (float bits), fnbr[6], absolute first face-ghost row
+    // of each face (-1 for same-level faces), sample, 0 -- one scalar load
     // gives a workgroup everything the slot tables would give it through dependent look-ups
     const int *wrec;
     const int *nbr26;  // [slot*26] same-level neighbour per region (the fused two-step kernel)
@@ -53,7 +55,8 @@ struct PersistArgs {
 // point is found through a look-up over the finest level.
 struct LabArgs {
     const int *nbr26;  // [slot*26] same-level neighbour slot per region or -1
-    const int *lut;    // per sample (2^lut_level)^3 finest block cells -> slot, concatenated
+    // per sample (2^lut_level)^3 finest block cells -> slot | level << LUT_LEVEL_SHIFT, concatenated
+    const int *lut;
     const int *lutoff; // [slot] offset of the slot's sample look-up in lut
     const int *level;  // [slot]
     const int *index;  // [slot*3]
@@ -120,11 +123,6 @@ hipError_t launch_face_ghosts(const float *phi, const int64_t *ptr, const int *s
 // hipErrorNotSupported where the device cannot launch it cooperatively.
 hipError_t launch_fk_persistent(const PersistArgs &p, int zt, hipStream_t st);
 // sprm: per-sample Dw, Dg (ensembles) or null for (Dw, Dg)
-// frow[slot*6+f] = sfbase[sample(slot)] + fbase[slot*6+f] for the listed slots
-hipError_t launch_work_records(const int *work, int nwork, const int *fnbr, const int *frow, const int *level,
-                               const float *ih2_lvl, const int *slot_sample, int *wrec, hipStream_t st);
-hipError_t launch_face_rows_abs(const int *work, int nwork, const int *fbase, const int *sfbase,
-                                const int *slot_sample, int *frow, hipStream_t st);
 hipError_t launch_face_prep(const float *pw, const float *pg, const int *work, int nwork, const int *fnbr,
                             const int *fbase, const int *sfbase, const int *slot_sample, const int64_t *ptr,
                             const int *src, const float *w, float *fik, float *ikc, float Dw, float Dg,

@@ -102,8 +102,12 @@ static int run_grid(int bpd, int max_level, int order, int iters, unsigned seed)
                 if (nb < -1 || nb >= cap || nb == -7) return fail("fnbr entry", it);
             }
         }
-        for (int s : t.lut)
-            if (s < 0 || s >= cap) return fail("lut entry", it);
+        std::vector<int> slot_level(cap, -1);
+        for (const auto &lf : g.leaves()) slot_level[lf.slot] = lf.level;
+        for (int e : t.lut) {
+            const int s = e & gq::LUT_SLOT_MASK;
+            if (e < 0 || s >= cap || slot_level[s] != (e >> gq::LUT_LEVEL_SHIFT)) return fail("lut entry", it);
+        }
         // ghost expansion of a few halo points of random leaves
         for (int k = 0; k < 8; k++) {
             const int leaf = (int)(rng() % g.leaves().size());
