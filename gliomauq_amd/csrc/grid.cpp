@@ -6,6 +6,8 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <mutex>
+#include <utility>
 #include <unordered_set>
 
 namespace gq {
@@ -153,6 +155,50 @@ bool Grid::any_ring(int i, F &&f) const
     return false;
 }
 
+template <class F>
+bool Grid::any_ring_balanced(int i, F &&f) const
+{
+    // The ring walk of any_ring on a grid whose neighbouring leaves differ by
+    // at most one level (the invariant every plan keeps): a neighbour's box
+    // is at least half the leaf's box s and aligned to its own size, so per
+    // axis the look-up cells bs - 1, bs, bs + s/2 and be meet every
+    // neighbour (the full ring has (s+2)^3 - s^3 cells; this walk at most 56).
+    const Leaf &lf = leaves_[i];
+    const int d = lut_level_ - lf.level;
+    if (d <= 1) return any_ring(i, std::forward<F>(f));  // the full ring has at most 56 cells
+    const int64_t nF = 1LL << lut_level_, mask = nF - 1;
+    const int64_t sz = 1LL << d;
+    int64_t c[3][4];
+    bool in[3][4];
+    int n = 0;
+    for (int a = 0; a < 3; a++) {
+        const int64_t bs = (int64_t)lf.idx[a] << d;
+        int k = 0;
+        c[a][k] = bs - 1, in[a][k++] = false;
+        c[a][k] = bs, in[a][k++] = true;
+        c[a][k] = bs + sz / 2, in[a][k++] = true;
+        c[a][k] = bs + sz, in[a][k++] = false;
+        n = k;
+    }
+    for (int kz = 0; kz < n; kz++) {
+        const int64_t z = c[2][kz];
+        for (int ky = 0; ky < n; ky++) {
+            const int64_t y = c[1][ky];
+            const int64_t row = nF * ((y & mask) + nF * (z & mask));
+            const bool yzimg = y < 0 || y >= nF || z < 0 || z >= nF;
+            for (int kx = 0; kx < n; kx++) {
+                if (in[0][kx] && in[1][ky] && in[2][kz]) continue;
+                const int64_t x = c[0][kx];
+                const bool img = yzimg || x < 0 || x >= nF;
+                const int k = lut_[(size_t)((x & mask) + row)];
+                if (k == i && !img) continue;
+                if (f(k, img)) return true;
+            }
+        }
+    }
+    return false;
+}
+
 void Grid::neighbors(int i, std::vector<std::pair<int, bool>> &out) const
 {
     out.clear();
@@ -175,7 +221,7 @@ std::vector<int> Grid::plan_refine(const std::unordered_set<int> &flagged_ids) c
     for (int i = 0; i < n; i++) {
         if (flag[i]) continue;
         const int li = leaves_[i].level;
-        if (any_ring(i, [&](int k, bool img) { return !img && leaves_[k].level > li; })) flag[i] = 1;
+        if (any_ring_balanced(i, [&](int k, bool img) { return !img && leaves_[k].level > li; })) flag[i] = 1;
     }
     // step 4: never refine on the finest allowed level (:54-63)
     for (int i = 0; i < n; i++)
@@ -190,7 +236,7 @@ std::vector<int> Grid::plan_refine(const std::unordered_set<int> &flagged_ids) c
         new_v.clear();
         for (int v : old_v) {
             const int nl = leaves_[v].level + 1;
-            any_ring(v, [&](int k, bool) {
+            any_ring_balanced(v, [&](int k, bool) {
                 if (std::abs(leaves_[k].level - nl) > 1 && !victim[k]) {
                     victim[k] = 1;
                     new_v.push_back(k);
@@ -231,7 +277,7 @@ std::vector<std::array<int, 4>> Grid::plan_compress(const std::unordered_set<int
         bool ok = true;
         for (int c = 0; c < 8 && ok; c++) {
             const int ch = find(p[0] + 1, 2 * p[1] + (c & 1), 2 * p[2] + ((c >> 1) & 1), 2 * p[3] + ((c >> 2) & 1));
-            ok = ch >= 0 && !any_ring(ch, [&](int q, bool) { return std::abs(leaves_[q].level - p[0]) > 1; });
+            ok = ch >= 0 && !any_ring_balanced(ch, [&](int q, bool) { return std::abs(leaves_[q].level - p[0]) > 1; });
         }
         if (ok) out.push_back(p);
         a += 8;
@@ -337,44 +383,52 @@ void build_sample_tables(const Grid &g, int *fnbr, int *nbr26, SampleTables &t)
     const std::vector<int> &gl = g.lut();
     const int lutl = g.lut_level();
     const size_t nF = (size_t)1 << lutl;
+    // device look-up entries first: the neighbour search below reads slot and
+    // level of a look-up cell in one load
+    t.lut.resize(gl.size());
+    for (size_t k = 0; k < gl.size(); k++) t.lut[k] = L[gl[k]].slot | L[gl[k]].level << LUT_LEVEL_SHIFT;
+    const int *lut = t.lut.data();
     for (int i : t.order) {
         const Leaf &lf = L[i];
         const int m = 1 << lf.level, sh = lutl - lf.level;
         int *nb26 = nbr26 + (size_t)lf.slot * 26;
         // a same-level neighbour box is a leaf iff the leaf holding its corner
         // look-up cell has that level (boxes of one level are aligned)
+        size_t xs[3], ys[3], zs[3];
+        for (int d = 0; d < 3; d++) {
+            xs[d] = (size_t)((lf.idx[0] + d - 1) & (m - 1)) << sh;
+            ys[d] = nF * ((size_t)((lf.idx[1] + d - 1) & (m - 1)) << sh);
+            zs[d] = nF * nF * ((size_t)((lf.idx[2] + d - 1) & (m - 1)) << sh);
+        }
+        const int want = lf.level << LUT_LEVEL_SHIFT;
         int c = 0;
-        for (int dz = -1; dz <= 1; dz++) {
-            const size_t z = (size_t)((lf.idx[2] + dz) & (m - 1)) << sh;
-            for (int dy = -1; dy <= 1; dy++) {
-                const size_t y = (size_t)((lf.idx[1] + dy) & (m - 1)) << sh;
-                for (int dx = -1; dx <= 1; dx++, c++) {
+        for (int dz = 0; dz < 3; dz++)
+            for (int dy = 0; dy < 3; dy++) {
+                const int *row = lut + ys[dy] + zs[dz];
+                for (int dx = 0; dx < 3; dx++, c++) {
                     if (c == 13) continue;
-                    const size_t x = (size_t)((lf.idx[0] + dx) & (m - 1)) << sh;
-                    const Leaf &q = L[gl[x + nF * (y + nF * z)]];
-                    nb26[c < 13 ? c : c - 1] = q.level == lf.level ? q.slot : -1;
+                    const int e = row[xs[dx]];
+                    nb26[c < 13 ? c : c - 1] = (e & ~LUT_SLOT_MASK) == want ? (e & LUT_SLOT_MASK) : -1;
                 }
             }
-        }
         for (int f = 0; f < 6; f++) {
-            const int c = face_codes[f];
-            const int nb = nb26[c < 13 ? c : c - 1];
+            const int cc = face_codes[f];
+            const int nb = nb26[cc < 13 ? cc : cc - 1];
             fnbr[(size_t)lf.slot * 6 + f] = nb;
             if (nb < 0) t.faces.push_back(lf.slot * 6 + f);
         }
     }
-    t.lut.resize(gl.size());
-    for (size_t k = 0; k < gl.size(); k++) t.lut[k] = L[gl[k]].slot | L[gl[k]].level << LUT_LEVEL_SHIFT;
-    // base-level look-up for the uniform dump / likelihood
+    // base-level look-up for the uniform dump / likelihood: the leaf holding
+    // base block (x, y, z) holds finest look-up cell (x, y, z) << (lut_level - base_level)
     const int bpd = g.blocks_per_dim();
-    const int bl = g.base_level();
-    t.lut_slot.assign((size_t)bpd * bpd * bpd, -1);
+    const int d = lutl - g.base_level();
+    t.lut_slot.resize((size_t)bpd * bpd * bpd);
     for (int z = 0; z < bpd; z++)
-        for (int y = 0; y < bpd; y++)
-            for (int x = 0; x < bpd; x++) {
-                const int64_t p[3] = {(int64_t)x * BS, (int64_t)y * BS, (int64_t)z * BS};
-                t.lut_slot[(size_t)x + bpd * ((size_t)y + bpd * z)] = L[g.lying_leaf(bl, p)].slot;
-            }
+        for (int y = 0; y < bpd; y++) {
+            const int *row = t.lut.data() + nF * (((size_t)y << d) + nF * ((size_t)z << d));
+            int *out = t.lut_slot.data() + (size_t)bpd * ((size_t)y + (size_t)bpd * z);
+            for (int x = 0; x < bpd; x++) out[x] = row[(size_t)x << d] & LUT_SLOT_MASK;
+        }
 }
 
 namespace {
@@ -393,9 +447,40 @@ inline uint64_t spread3(uint64_t x)
 
 std::vector<int> morton_order(const Grid &g)
 {
-    // codes use 3 * lut_level <= 24 bits: code << 32 | leaf index sorts as one integer
     const auto &L = g.leaves();
     const int lv = g.lut_level();
+    if (lv <= 6) {
+        // Leaves are aligned boxes of look-up cells, each a contiguous run of
+        // the cells' Morton order starting at its corner: walking the look-up
+        // in Morton order and taking each leaf at its first cell gives the
+        // leaves in Morton order of their corners.
+        static std::vector<int> walk[7];
+        static std::once_flag once[7];
+        std::call_once(once[lv], [&] {
+            const int64_t n = 1LL << lv;
+            std::vector<int> &w = walk[lv];
+            w.resize((size_t)(n * n * n));
+            for (int64_t m = 0; m < n * n * n; m++) {
+                int64_t c[3] = {0, 0, 0};
+                for (int b = 0; b < lv; b++)
+                    for (int a = 0; a < 3; a++) c[a] |= ((m >> (3 * b + a)) & 1) << b;
+                w[(size_t)m] = (int)(c[0] + n * (c[1] + n * c[2]));
+            }
+        });
+        const std::vector<int> &lut = g.lut();
+        std::vector<char> seen(L.size(), 0);
+        std::vector<int> out;
+        out.reserve(L.size());
+        for (int cell : walk[lv]) {
+            const int i = lut[(size_t)cell];
+            if (!seen[i]) {
+                seen[i] = 1;
+                out.push_back(i);
+            }
+        }
+        return out;
+    }
+    // codes use 3 * lut_level <= 24 bits: code << 32 | leaf index sorts as one integer
     std::vector<uint64_t> keys(L.size());
     for (size_t i = 0; i < L.size(); i++) {
         const int d = lv - L[i].level;

@@ -97,6 +97,8 @@ public:
 private:
     template <class F>
     bool any_ring(int i, F &&f) const;
+    template <class F>
+    bool any_ring_balanced(int i, F &&f) const;  // any_ring on a 2:1-balanced grid
     void rebuild_index();
     void expand_rec(int lam, const int64_t p[3], double w, std::vector<Term> &T, int depth) const;
 

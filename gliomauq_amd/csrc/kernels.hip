@@ -909,21 +909,33 @@ __device__ void ghost_dfs(const LabArgs &g, const int *__restrict__ lut, const f
 // cell (p mod 16) per axis.
 template <int MODE, int NF>
 __device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, const float *const *flds, int lam,
-                               const int64_t p[3], double w, double *acc, int *count, int *src, float *wout)
+                               const int64_t p[3], int ep, double w, double *acc, int *count, int *src, float *wout)
 {
     const int cl = lam - 1;
+    // ep: look-up entry of the coarser leaf C holding p.  When C has level
+    // cl, a child inside C's box is a point of C itself (cell = i mod 16 per
+    // axis) and needs no look-up; C's block index at level cl is p >> 5 per
+    // axis (p / 2 / 16), compared modulo the 2^cl blocks of that level.
+    const bool c_own = entry_level(ep) == cl;
+    const int64_t mcl = ((int64_t)1 << cl) - 1;
+    const int64_t cbx = (p[0] >> 5) & mcl, cby = (p[1] >> 5) & mcl, cbz = (p[2] >> 5) & mcl;
+    // per axis: p odd -> 4 (Interp4) or 2 (Interp2) children from i0, p even
+    // -> the child p/2 alone; child k of an axis has weight Hs(2(i0 + k) - p)
+    // (Interp4: Hs(-3), Hs(-1), Hs(1), Hs(3); Interp2: Hs(-1), Hs(1); even: Hs(0))
     int64_t i0[3];
-    int n[3];
+    int ln[3];  // log2 of the children per axis
+    bool odd[3];
     for (int d = 0; d < 3; d++) {
-        if (p[d] & 1) {
-            n[d] = g.order == 4 ? 4 : 2;
-            i0[d] = (p[d] - (g.order == 4 ? 3 : 1)) >> 1;
-        } else {
-            n[d] = 1;
-            i0[d] = p[d] >> 1;
-        }
+        odd[d] = (p[d] & 1) != 0;
+        ln[d] = odd[d] ? (g.order == 4 ? 2 : 1) : 0;
+        i0[d] = odd[d] ? (p[d] - (g.order == 4 ? 3 : 1)) >> 1 : p[d] >> 1;
     }
-    const int nc = n[0] * n[1] * n[2];
+    const int nc = 1 << (ln[0] + ln[1] + ln[2]);
+    auto hw = [&](int d, int k) -> double {
+        if (!odd[d]) return 1.0;
+        if (g.order != 4) return 0.5;
+        return (k == 0 || k == 3) ? -1.0 / 16.0 : 9.0 / 16.0;
+    };
     // children per batch (register pressure of the loads and stores): a
     // batch resolves with two rounds of loads, look-up entries (slot and
     // level in one word) and then values
@@ -937,14 +949,14 @@ __device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, co
 #pragma unroll
         for (int k = 0; k < NB; k++) {
             const int c = c0 + k < nc ? c0 + k : c0;
-            const int64_t ix = i0[0] + c % n[0], iy = i0[1] + (c / n[0]) % n[1], iz = i0[2] + c / (n[0] * n[1]);
-            const double ww = __dmul_rn(
-                __dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), hs_w(g.order, (int)(2 * iy - p[1]))),
-                hs_w(g.order, (int)(2 * iz - p[2])));
+            const int kx = c & ((1 << ln[0]) - 1), ky = (c >> ln[0]) & ((1 << ln[1]) - 1), kz = c >> (ln[0] + ln[1]);
+            const int64_t ix = i0[0] + kx, iy = i0[1] + ky, iz = i0[2] + kz;
+            const double ww = __dmul_rn(__dmul_rn(hw(0, kx), hw(1, ky)), hw(2, kz));
             cw[k] = __dmul_rn(w, ww);
             const bool in = c0 + k < nc;
-            sa[k] = in ? lying_entry(g, lut, cl, ix, iy, iz) : 0;
-            sb[k] = in ? lying_entry(g, lut, cl + 1, 2 * ix, 2 * iy, 2 * iz) : 0;
+            const bool in_c = c_own && ((ix >> 4) & mcl) == cbx && ((iy >> 4) & mcl) == cby && ((iz >> 4) & mcl) == cbz;
+            sa[k] = in ? (in_c ? ep : lying_entry(g, lut, cl, ix, iy, iz)) : 0;
+            sb[k] = in && !in_c ? lying_entry(g, lut, cl + 1, 2 * ix, 2 * iy, 2 * iz) : 0;
             ca[k] = (int)((ix & 15) + 16 * ((iy & 15) + 16 * (iz & 15)));
             cb[k] = (int)(((2 * ix) & 15) + 16 * (((2 * iy) & 15) + 16 * ((2 * iz) & 15)));
         }
@@ -966,10 +978,10 @@ __device__ void synth_children(const LabArgs &g, const int *__restrict__ lut, co
                 val[k][f] = MODE == 0 && c0 + k < nc ? flds[f][(size_t)sl[k] * 4096 + cell[k]] : 0.f;
         if (!plain) {
             for (int c = c0; c < nc; c++) {
-                const int64_t ix = i0[0] + c % n[0], iy = i0[1] + (c / n[0]) % n[1], iz = i0[2] + c / (n[0] * n[1]);
-                const double ww = __dmul_rn(
-                    __dmul_rn(hs_w(g.order, (int)(2 * ix - p[0])), hs_w(g.order, (int)(2 * iy - p[1]))),
-                    hs_w(g.order, (int)(2 * iz - p[2])));
+                const int kx = c & ((1 << ln[0]) - 1), ky = (c >> ln[0]) & ((1 << ln[1]) - 1),
+                          kz = c >> (ln[0] + ln[1]);
+                const int64_t ix = i0[0] + kx, iy = i0[1] + ky, iz = i0[2] + kz;
+                const double ww = __dmul_rn(__dmul_rn(hw(0, kx), hw(1, ky)), hw(2, kz));
                 const double cwc = __dmul_rn(w, ww);
                 const int e = lying_entry(g, lut, cl, ix, iy, iz);
                 const int slot = entry_slot(e);
@@ -1018,8 +1030,9 @@ template <int NF>
 __device__ void ghost_value(const LabArgs &g, const int *__restrict__ lut, const float *const *flds, int lam,
                             int64_t px, int64_t py, int64_t pz, double *acc)
 {
+    int e;
     for (;;) {
-        const int e = lying_entry(g, lut, lam, px, py, pz);
+        e = lying_entry(g, lut, lam, px, py, pz);
         const int slot = entry_slot(e), l = entry_level(e);
         if (l == lam) {
             const size_t i = (size_t)slot * 4096 + (px & 15) + 16 * ((py & 15) + 16 * (pz & 15));
@@ -1039,7 +1052,7 @@ __device__ void ghost_value(const LabArgs &g, const int *__restrict__ lut, const
 #pragma unroll
     for (int f = 0; f < NF; f++) acc[f] = 0.0;
     const int64_t p[3] = {px, py, pz};
-    synth_children<0, NF>(g, lut, flds, lam, p, 1.0, acc, nullptr, nullptr, nullptr);
+    synth_children<0, NF>(g, lut, flds, lam, p, e, 1.0, acc, nullptr, nullptr, nullptr);
 }
 
 template <int S0, int S1>
@@ -1172,9 +1185,11 @@ __global__ __launch_bounds__(64) void k_lab_ghosts(LabArgs a, FieldPtrs flds, co
     const int lvl = a.level[slot];
     const int q = (it & 7) * 64 + (int)threadIdx.x;
     if (q < npts) {
-        const int x = UsedCoords<ORDER, FOR_SPLIT>::at(sx + q % nx),
-                  y = UsedCoords<ORDER, FOR_SPLIT>::at(sy + (q / nx) % ny),
-                  z = UsedCoords<ORDER, FOR_SPLIT>::at(sz + q / (nx * ny));
+        // the counts per axis are 1, 2 or 16: shifts instead of divisions
+        const int lx = __builtin_ctz(nx), ly = __builtin_ctz(ny);
+        const int x = UsedCoords<ORDER, FOR_SPLIT>::at(sx + (q & (nx - 1))),
+                  y = UsedCoords<ORDER, FOR_SPLIT>::at(sy + ((q >> lx) & (ny - 1))),
+                  z = UsedCoords<ORDER, FOR_SPLIT>::at(sz + (q >> (lx + ly)));
         double v[NF];
         ghost_value<NF>(a, lut, flds.p, lvl, ox + x, oy + y, oz + z, v);
         const size_t o = (size_t)j * LabGeom<S0, S1>::N + (x - S0) + L * ((y - S0) + L * (z - S0));
@@ -1191,8 +1206,9 @@ template <int MODE>
 __device__ void ghost_terms(const LabArgs &g, const int *__restrict__ lut, int lam, int64_t px, int64_t py,
                             int64_t pz, int *count, int *src, float *wout)
 {
+    int e;
     for (;;) {
-        const int e = lying_entry(g, lut, lam, px, py, pz);
+        e = lying_entry(g, lut, lam, px, py, pz);
         const int slot = entry_slot(e), l = entry_level(e);
         if (l == lam) {
             if (MODE == 2) {
@@ -1213,7 +1229,7 @@ __device__ void ghost_terms(const LabArgs &g, const int *__restrict__ lut, int l
     }
     const int64_t p[3] = {px, py, pz};
     double unused = 0.0;
-    synth_children<MODE, 1>(g, lut, nullptr, lam, p, 1.0, &unused, count, src, wout);
+    synth_children<MODE, 1>(g, lut, nullptr, lam, p, e, 1.0, &unused, count, src, wout);
 }
 }  // namespace
 
