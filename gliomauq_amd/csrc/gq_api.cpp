@@ -113,20 +113,35 @@ struct PackedInts {
     std::vector<size_t> off;
     hipEvent_t done = nullptr;
     bool pending = false;
+    size_t total = 0;
     void upload(std::initializer_list<const std::vector<int> *> parts, hipStream_t st)
     {
-        size_t total = 0;
+        std::vector<size_t> sizes;
+        for (const std::vector<int> *v : parts) sizes.push_back(v->size());
+        stage(sizes, st);
+        size_t k = 0;
+        for (const std::vector<int> *v : parts) std::copy(v->begin(), v->end(), host.p + off[k++]);
+        send(st);
+    }
+    // Parts of the given sizes laid out in the staging buffer (once the
+    // previous transfer through this object has completed); the caller fills
+    // them through host_ptr() and then calls send().
+    void stage(const std::vector<size_t> &sizes, hipStream_t st)
+    {
+        total = 0;
         off.clear();
-        for (const std::vector<int> *v : parts) {
+        for (size_t n : sizes) {
             off.push_back(total);
-            total += (v->size() + 63) & ~(size_t)63;
+            total += (n + 63) & ~(size_t)63;
         }
         if (pending) HIPCHK(hipEventSynchronize(done));
         pending = false;
         host.reserve(std::max<size_t>(total, 1), st);
-        size_t k = 0;
-        for (const std::vector<int> *v : parts) std::copy(v->begin(), v->end(), host.p + off[k++]);
         dev.reserve(std::max<size_t>(total, 1));
+    }
+    int *host_ptr(int i) const { return host.p + off[i]; }
+    void send(hipStream_t st)
+    {
         if (total) {
             HIPCHK(hipMemcpyAsync(dev.p, host.p, total * sizeof(int), hipMemcpyHostToDevice, st));
             if (!done) HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
@@ -464,7 +479,7 @@ struct gq_ctx {
             sm.dirty = false;
         });
         // merged lists in work order
-        std::vector<int> lutf((size_t)S * lut_n), lut_slot, faces, sfbase(S, 0);
+        std::vector<int> faces, sfbase(S, 0);
         work.clear();
         work_begin.assign(1, 0);
         face_begin.assign(1, 0);
@@ -478,13 +493,8 @@ struct gq_ctx {
             work_begin.push_back((int)work.size());
             face_begin.push_back((int)faces.size());
         }
-        for (int s = 0; s < S; s++) {
-            const Sample &sm = samples[s];
-            std::copy(sm.t.lut.begin(), sm.t.lut.end(), lutf.begin() + (size_t)s * lut_n);
-            lut_slot.insert(lut_slot.end(), sm.t.lut_slot.begin(), sm.t.lut_slot.end());
-        }
         if ((int64_t)faces.size() * 256 > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "too many cross-level faces");
-        std::vector<int> packed;
+        size_t nrow = 0;  // slot rows scattered on the device (every row after a pool growth)
         if (full) {
             d_level.upload(h_level, st);
             d_index.upload(h_index, st);
@@ -495,24 +505,39 @@ struct gq_ctx {
             d_nbr26.upload(h_nbr26, st);
             uploaded_cap = cap;
         } else {
-            // only the rows of the rebuilt samples' slots, scattered on the device
-            size_t nrow = 0;
             for (int s : dirty) nrow += samples[s].grid->leaves().size();
-            packed.resize(nrow * ROW);
-            size_t r = 0;
+        }
+        const size_t lslot_n = samples[0].t.lut_slot.size();
+        // one transfer, filled in place in the page-locked staging buffer: work
+        // list, sample face-row bases, base-level look-ups, finest look-ups,
+        // cross-level faces, changed slot rows, step records
+        tabs.stage({work.size(), (size_t)S, (size_t)S * lslot_n, (size_t)S * lut_n, faces.size(), nrow * ROW,
+                    std::max<size_t>(1, work.size()) * gq::WREC},
+                   st);
+        std::copy(work.begin(), work.end(), tabs.host_ptr(0));
+        std::copy(sfbase.begin(), sfbase.end(), tabs.host_ptr(1));
+        for (int s = 0; s < S; s++) {
+            const Sample &sm = samples[s];
+            std::copy(sm.t.lut_slot.begin(), sm.t.lut_slot.end(), tabs.host_ptr(2) + (size_t)s * lslot_n);
+            std::copy(sm.t.lut.begin(), sm.t.lut.end(), tabs.host_ptr(3) + (size_t)s * lut_n);
+        }
+        std::copy(faces.begin(), faces.end(), tabs.host_ptr(4));
+        if (nrow) {
+            int *q = tabs.host_ptr(5);
             for (int s : dirty)
                 for (const auto &lf : samples[s].grid->leaves()) {
-                    int *q = packed.data() + r * ROW;
-                    const int sl = lf.slot;
-                    q[0] = sl;
-                    q[1] = h_level[sl];
-                    for (int a = 0; a < 3; a++) q[2 + a] = h_index[(size_t)sl * 3 + a];
-                    q[5] = h_ssample[sl];
+                    const size_t sl = (size_t)lf.slot;
+                    q[0] = lf.slot;
+                    q[1] = lf.level;
+                    q[2] = lf.idx[0];
+                    q[3] = lf.idx[1];
+                    q[4] = lf.idx[2];
+                    q[5] = s;
                     q[6] = h_lutoff[sl];
-                    for (int f = 0; f < 6; f++) q[7 + f] = h_fnbr[(size_t)sl * 6 + f];
-                    for (int f = 0; f < 6; f++) q[13 + f] = h_fbase[(size_t)sl * 6 + f];
-                    for (int c = 0; c < 26; c++) q[19 + c] = h_nbr26[(size_t)sl * 26 + c];
-                    r++;
+                    std::memcpy(q + 7, &h_fnbr[sl * 6], 6 * sizeof(int));
+                    std::memcpy(q + 13, &h_fbase[sl * 6], 6 * sizeof(int));
+                    std::memcpy(q + 19, &h_nbr26[sl * 26], 26 * sizeof(int));
+                    q += ROW;
                 }
         }
         // per-work-item step records: slot, 1/h^2 of its level (float bits),
@@ -526,10 +551,9 @@ struct gq_ctx {
             }
             return v;
         }();
-        std::vector<int> wrec(std::max<size_t>(1, work.size()) * gq::WREC, 0);
         for (size_t i = 0; i < work.size(); i++) {
             const int sl = work[i];
-            int *r = wrec.data() + i * gq::WREC;
+            int *r = tabs.host_ptr(6) + i * gq::WREC;
             r[0] = sl;
             std::memcpy(&r[1], &ih2[h_level[sl]], sizeof(float));
             for (int f = 0; f < 6; f++) {
@@ -538,17 +562,18 @@ struct gq_ctx {
                 r[8 + f] = fb >= 0 ? sfbase[h_ssample[sl]] + fb : -1;
             }
             r[14] = h_ssample[sl];
+            r[15] = 0;
         }
-        tabs.upload({&work, &sfbase, &lut_slot, &lutf, &faces, &packed, &wrec}, st);
+        tabs.send(st);
         p_work = tabs.ptr(0);
         p_sfbase = tabs.ptr(1);
         p_lut = tabs.ptr(2);
         p_lutf = tabs.ptr(3);
         p_faces = tabs.ptr(4);
         p_wrec = tabs.ptr(6);
-        if (!packed.empty()) {
+        if (nrow) {
             gq::SlotTables t{d_level.p, d_index.p, d_slot_sample.p, d_lutoff.p, d_fnbr.p, d_fbase.p, d_nbr26.p};
-            HIPCHK(gq::launch_scatter_slot_rows(t, tabs.ptr(5), (int)(packed.size() / ROW), st));
+            HIPCHK(gq::launch_scatter_slot_rows(t, tabs.ptr(5), (int)nrow, st));
         }
         face_nrows = (int)faces.size() * 256;
         face_rows_valid = false;
@@ -682,7 +707,9 @@ struct gq_ctx {
     }
 
     // Refine the flagged block ids of each listed sample (Refiner_SpaceExtension plan + split).
-    std::vector<int> refine_batch(const std::vector<int> &which, const std::vector<std::unordered_set<int>> &ids)
+    // flags[k]: per block id of sample which[k], nflag[k] of them set.
+    std::vector<int> refine_batch(const std::vector<int> &which, const std::vector<std::vector<char>> &flags,
+                                  const std::vector<int> &nflag)
     {
         need_grid();
         const int n = (int)which.size();
@@ -690,7 +717,7 @@ struct gq_ctx {
         {
             ScopeTimer tm(t_plan);
             parallel_for(n, [&](int k) {
-                if (!ids[k].empty()) parents[k] = G(which[k]).plan_refine(ids[k]);
+                if (nflag[k] > 0) parents[k] = G(which[k]).plan_refine(flags[k]);
             });
         }
         size_t np = 0;
@@ -747,7 +774,8 @@ struct gq_ctx {
         return counts;
     }
 
-    std::vector<int> compress_batch(const std::vector<int> &which, const std::vector<std::unordered_set<int>> &ids)
+    std::vector<int> compress_batch(const std::vector<int> &which, const std::vector<std::vector<char>> &flags,
+                                    const std::vector<int> &nflag)
     {
         need_grid();
         const int n = (int)which.size();
@@ -755,7 +783,7 @@ struct gq_ctx {
         {
             ScopeTimer tm(t_plan);
             parallel_for(n, [&](int k) {
-                if (!ids[k].empty()) plan[k] = G(which[k]).plan_compress(ids[k]);
+                if (nflag[k] > 0) plan[k] = G(which[k]).plan_compress(flags[k]);
             });
         }
         size_t np = 0;
@@ -806,8 +834,8 @@ struct gq_ctx {
     std::vector<int> auto_refine(const std::vector<int> &which, double tol, int max_level, int max_loops)
     {
         const int n = (int)which.size();
-        std::vector<std::unordered_set<int>> nice(n);
-        std::vector<int> total(n, 0);
+        std::vector<std::vector<char>> nice(n), flagged(n);  // per block id
+        std::vector<int> nflag(n), total(n, 0);
         std::vector<char> live(n, 1);
         int loops = 0;
         while (true) {
@@ -817,9 +845,10 @@ struct gq_ctx {
                 if (!live[k]) continue;
                 const Sample &sm = samples[which[k]];
                 const auto &L = sm.grid->leaves();
+                nice[k].resize(sm.grid->id_bound(), 0);
                 for (int li : sm.t.order) {
                     if (max_level >= 0 && max_level <= L[li].level) continue;
-                    if (nice[k].count(L[li].block_id)) continue;
+                    if (nice[k][L[li].block_id]) continue;
                     cand_slots.push_back(L[li].slot);
                     cand_k.push_back(k);
                     cand_li.push_back(li);
@@ -827,20 +856,23 @@ struct gq_ctx {
             }
             if (cand_slots.empty()) break;
             std::vector<double> det = fwt_slots(cand_slots);
-            std::vector<std::unordered_set<int>> flagged(n);
+            for (int k = 0; k < n; k++) {
+                flagged[k].assign(G(which[k]).id_bound(), 0);
+                nflag[k] = 0;
+            }
             for (size_t c = 0; c < cand_slots.size(); c++) {
                 const int k = cand_k[c];
                 const int id = G(which[k]).leaves()[cand_li[c]].block_id;
-                if (det[c] >= tol) flagged[k].insert(id);
-                else nice[k].insert(id);
+                if (det[c] >= tol) flagged[k][id] = 1, nflag[k]++;
+                else nice[k][id] = 1;
             }
             bool any = false;
             for (int k = 0; k < n; k++) {
-                if (flagged[k].empty()) live[k] = 0;
+                if (nflag[k] == 0) live[k] = 0;
                 else any = true;
             }
             if (!any) break;
-            std::vector<int> r = refine_batch(which, flagged);
+            std::vector<int> r = refine_batch(which, flagged, nflag);
             for (int k = 0; k < n; k++) total[k] += r[k];
             loops++;
             if (max_loops > 0 && loops >= max_loops) break;
@@ -852,9 +884,9 @@ struct gq_ctx {
     std::vector<int> auto_compress(const std::vector<int> &which, double tol, int max_loops)
     {
         const int n = (int)which.size();
-        std::vector<std::unordered_set<int>> nice(n);
-        std::vector<int> total(n, 0);
-        std::vector<char> live(n, 1);
+        std::vector<std::vector<char>> nice(n), flagged(n);  // per block id
+        std::vector<int> nflag(n), total(n, 0);
+        std::vector<char> live(n, 1), cand;
         int loops = 0;
         while (true) {
             ensure_tables();
@@ -863,6 +895,7 @@ struct gq_ctx {
                 if (!live[k]) continue;
                 const Sample &sm = samples[which[k]];
                 const auto &L = sm.grid->leaves();
+                nice[k].resize(sm.grid->id_bound(), 0);
                 if (loops > 0) {
                     // Only candidates (leaves not yet found nice) can be
                     // flagged, so the parents this pass could collapse are
@@ -871,17 +904,16 @@ struct gq_ctx {
                     // details are and the loop ends here, as it would after
                     // the transform (MRAGScienceCore.h:161-162); the transform
                     // is skipped.
-                    std::unordered_set<int> cand_ids;
-                    for (const auto &lf : L)
-                        if (!nice[k].count(lf.block_id)) cand_ids.insert(lf.block_id);
+                    cand.assign(nice[k].size(), 0);
+                    for (const auto &lf : L) cand[lf.block_id] = !nice[k][lf.block_id];
                     ScopeTimer tm(t_plan);
-                    if (sm.grid->plan_compress(cand_ids).empty()) {
+                    if (sm.grid->plan_compress(cand).empty()) {
                         live[k] = 0;
                         continue;
                     }
                 }
                 for (int li : sm.t.order) {
-                    if (nice[k].count(L[li].block_id)) continue;
+                    if (nice[k][L[li].block_id]) continue;
                     cand_slots.push_back(L[li].slot);
                     cand_k.push_back(k);
                     cand_li.push_back(li);
@@ -889,18 +921,21 @@ struct gq_ctx {
             }
             if (cand_slots.empty()) break;
             std::vector<double> det = fwt_slots(cand_slots);
-            std::vector<std::unordered_set<int>> flagged(n);
+            for (int k = 0; k < n; k++) {
+                flagged[k].assign(G(which[k]).id_bound(), 0);
+                nflag[k] = 0;
+            }
             for (size_t c = 0; c < cand_slots.size(); c++) {
                 const int k = cand_k[c];
                 const int id = G(which[k]).leaves()[cand_li[c]].block_id;
-                if (det[c] < tol) flagged[k].insert(id);
-                else nice[k].insert(id);
+                if (det[c] < tol) flagged[k][id] = 1, nflag[k]++;
+                else nice[k][id] = 1;
             }
-            std::vector<int> r = compress_batch(which, flagged);
+            std::vector<int> r = compress_batch(which, flagged, nflag);
             bool any = false;
             for (int k = 0; k < n; k++) {
                 total[k] += r[k];
-                if (flagged[k].empty() || r[k] == 0) live[k] = 0;
+                if (nflag[k] == 0 || r[k] == 0) live[k] = 0;
                 else any = true;
             }
             if (!any) break;
@@ -1884,8 +1919,11 @@ gq_status gq_refine_blocks(gq_ctx *ctx, const int32_t *ids, int n, int *n_out)
 {
     return guard(ctx, [&] {
         if (n < 0 || (n > 0 && !ids)) throw gq_error(GQ_ERR_ARG, "bad id list");
-        std::vector<std::unordered_set<int>> s(1, std::unordered_set<int>(ids, ids + n));
-        const std::vector<int> r = ctx->refine_batch({ctx->cur_sample}, s);
+        const gq::Grid &g = ctx->G();
+        std::vector<std::vector<char>> f(1, g.id_flags(std::unordered_set<int>(ids, ids + n)));
+        int nf = 0;
+        for (char c : f[0]) nf += c;
+        const std::vector<int> r = ctx->refine_batch({ctx->cur_sample}, f, {nf});
         if (n_out) *n_out = r[0];
     });
 }
@@ -1894,8 +1932,11 @@ gq_status gq_compress_blocks(gq_ctx *ctx, const int32_t *ids, int n, int *n_out)
 {
     return guard(ctx, [&] {
         if (n < 0 || (n > 0 && !ids)) throw gq_error(GQ_ERR_ARG, "bad id list");
-        std::vector<std::unordered_set<int>> s(1, std::unordered_set<int>(ids, ids + n));
-        const std::vector<int> r = ctx->compress_batch({ctx->cur_sample}, s);
+        const gq::Grid &g = ctx->G();
+        std::vector<std::vector<char>> f(1, g.id_flags(std::unordered_set<int>(ids, ids + n)));
+        int nf = 0;
+        for (char c : f[0]) nf += c;
+        const std::vector<int> r = ctx->compress_batch({ctx->cur_sample}, f, {nf});
         if (n_out) *n_out = r[0];
     });
 }

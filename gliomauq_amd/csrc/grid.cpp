@@ -211,11 +211,33 @@ void Grid::neighbors(int i, std::vector<std::pair<int, bool>> &out) const
     out.erase(std::unique(out.begin(), out.end()), out.end());
 }
 
+std::vector<char> Grid::id_flags(const std::unordered_set<int> &ids) const
+{
+    std::vector<char> f(next_id_, 0);
+    for (int id : ids)
+        if (id >= 0 && id < next_id_) f[id] = 1;
+    return f;
+}
+
 std::vector<int> Grid::plan_refine(const std::unordered_set<int> &flagged_ids) const
 {
+    return plan_refine(id_flags(flagged_ids));
+}
+
+std::vector<std::array<int, 4>> Grid::plan_compress(const std::unordered_set<int> &flagged_ids) const
+{
+    return plan_compress(id_flags(flagged_ids));
+}
+
+std::vector<int> Grid::plan_refine(const std::vector<char> &flag_by_id) const
+{
     const int n = (int)leaves_.size();
+    const int nf = (int)flag_by_id.size();
     std::vector<char> flag(n, 0);
-    for (int i = 0; i < n; i++) flag[i] = flagged_ids.count(leaves_[i].block_id) ? 1 : 0;
+    for (int i = 0; i < n; i++) {
+        const int id = leaves_[i].block_id;
+        flag[i] = id < nf && flag_by_id[id] ? 1 : 0;
+    }
     // Refiner_SpaceExtension step 3: a leaf with a finer, non-image neighbour
     // is refined as well (MRAGRefiner_SpaceExtension.h:35-51)
     for (int i = 0; i < n; i++) {
@@ -252,17 +274,17 @@ std::vector<int> Grid::plan_refine(const std::unordered_set<int> &flagged_ids) c
     return out;
 }
 
-std::vector<std::array<int, 4>> Grid::plan_compress(const std::unordered_set<int> &flagged_ids) const
+std::vector<std::array<int, 4>> Grid::plan_compress(const std::vector<char> &flag_by_id) const
 {
     // Compressor::createPlan (MRAGCompressor.h:68-118): collapse a parent
     // whose 8 children are all leaves, all flagged, and whose children's
     // neighbours are within one level of the parent.  Flagged leaves grouped
     // by parent: eight distinct leaves with one parent are its eight
     // children.  Parents come out in (level, x, y, z) order.
+    const int nf = (int)flag_by_id.size();
     std::vector<uint64_t> keys;
-    keys.reserve(flagged_ids.size());
     for (const Leaf &lf : leaves_)
-        if (lf.level > 0 && flagged_ids.count(lf.block_id))
+        if (lf.level > 0 && lf.block_id < nf && flag_by_id[lf.block_id])
             keys.push_back(leaf_key(lf.level - 1, lf.idx[0] >> 1, lf.idx[1] >> 1, lf.idx[2] >> 1));
     std::sort(keys.begin(), keys.end());
     std::vector<std::array<int, 4>> out;

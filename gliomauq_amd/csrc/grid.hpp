@@ -83,8 +83,13 @@ public:
     void neighbors(int i, std::vector<std::pair<int, bool>> &out) const;
 
     // Planning.  Return leaf indices (refine) or parent keys (compress).
+    // Flags by block id: flag[id] != 0 (ids at or above flag.size() unflagged).
+    std::vector<int> plan_refine(const std::vector<char> &flag_by_id) const;
+    std::vector<std::array<int, 4>> plan_compress(const std::vector<char> &flag_by_id) const;
     std::vector<int> plan_refine(const std::unordered_set<int> &flagged_ids) const;
     std::vector<std::array<int, 4>> plan_compress(const std::unordered_set<int> &flagged_ids) const;
+    std::vector<char> id_flags(const std::unordered_set<int> &ids) const;
+    int id_bound() const { return next_id_; }  // every block id is below this
 
     // Topology edits (slots are assigned by the caller).
     int new_block_id() { return next_id_++; }

@@ -1305,29 +1305,43 @@ struct Taps<2, true> {  // Ga on [-2,1): {-1/2, 1, -1/2}
     __device__ static constexpr double w(int k) { return k == 1 ? 1.0 : -0.5; }
 };
 
-template <int ORDER, int L, int S0, bool GX, bool GY, bool GZ>
-__device__ __forceinline__ double fwt_filter(const double *lab, int dx, int dy, int dz)
+
+// All seven detail codes of coarse point d in one pass over the lab: the G
+// taps of each axis contain the H tap (i = 0), so every code's taps are a
+// subset of the G x G x G taps.  Each lab value is read once and added to
+// every code whose filter has that tap, with that code's weight
+// ((wZ*wY)*wX); each code still sees its taps in its own z, y, x loop order
+// (the reference's: r += lab(2d - i) * ((wZ*wY)*wX), i over the filter's
+// support, skipping the zero taps adds exact zeros only).  Codes c = 1..7 in r[c-1]
+// (bit 0: G in x, bit 1: G in y, bit 2: G in z).
+template <int ORDER, int L, int S0>
+__device__ __forceinline__ void fwt_codes(const double *lab, int dx, int dy, int dz, double (&r)[7])
 {
-    typedef Taps<ORDER, GX> TX;
-    typedef Taps<ORDER, GY> TY;
-    typedef Taps<ORDER, GZ> TZ;
-    double r = 0.0;
+    typedef Taps<ORDER, true> G;
+    constexpr int KH = ORDER == 4 ? 3 : 2;  // the G tap with i = 0
 #pragma unroll
-    for (int kz = 0; kz < TZ::n; kz++) {
-        const double wZ = TZ::w(kz);
-        const int z = 2 * dz - TZ::i(kz) - S0;
+    for (int c = 0; c < 7; c++) r[c] = 0.0;
 #pragma unroll
-        for (int ky = 0; ky < TY::n; ky++) {
-            const double wZwY = __dmul_rn(wZ, TY::w(ky));
-            const int y = 2 * dy - TY::i(ky) - S0;
+    for (int kz = 0; kz < G::n; kz++) {
+        const int z = 2 * dz - G::i(kz) - S0;
 #pragma unroll
-            for (int kx = 0; kx < TX::n; kx++) {
-                const int x = 2 * dx - TX::i(kx) - S0;
-                r = __dadd_rn(r, __dmul_rn(lab[x + L * (y + L * z)], __dmul_rn(wZwY, TX::w(kx))));
+        for (int ky = 0; ky < G::n; ky++) {
+            const int y = 2 * dy - G::i(ky) - S0;
+#pragma unroll
+            for (int kx = 0; kx < G::n; kx++) {
+                const int x = 2 * dx - G::i(kx) - S0;
+                const double val = lab[x + L * (y + L * z)];
+#pragma unroll
+                for (int c = 1; c <= 7; c++) {
+                    const bool gx = c & 1, gy = c & 2, gz = c & 4;
+                    if ((gx || kx == KH) && (gy || ky == KH) && (gz || kz == KH)) {
+                        const double wz = gz ? G::w(kz) : 1.0, wy = gy ? G::w(ky) : 1.0, wx = gx ? G::w(kx) : 1.0;
+                        r[c - 1] = __dadd_rn(r[c - 1], __dmul_rn(val, __dmul_rn(__dmul_rn(wz, wy), wx)));
+                    }
+                }
             }
         }
     }
-    return r;
 }
 }  // namespace
 
@@ -1353,13 +1367,7 @@ __global__ __launch_bounds__(512) void k_fwt_detail(LabArgs a, const double *__r
         const int q = threadIdx.x;  // one coarse point per thread
         const int dx = q & 7, dy = (q >> 3) & 7, dz = q >> 6;
         double v[7];
-        v[0] = fwt_filter<ORDER, L, S0, true, false, false>(lab, dx, dy, dz);
-        v[1] = fwt_filter<ORDER, L, S0, false, true, false>(lab, dx, dy, dz);
-        v[2] = fwt_filter<ORDER, L, S0, true, true, false>(lab, dx, dy, dz);
-        v[3] = fwt_filter<ORDER, L, S0, false, false, true>(lab, dx, dy, dz);
-        v[4] = fwt_filter<ORDER, L, S0, true, false, true>(lab, dx, dy, dz);
-        v[5] = fwt_filter<ORDER, L, S0, false, true, true>(lab, dx, dy, dz);
-        v[6] = fwt_filter<ORDER, L, S0, true, true, true>(lab, dx, dy, dz);
+        fwt_codes<ORDER, L, S0>(lab, dx, dy, dz, v);
 #pragma unroll
         for (int c = 0; c < 7; c++) {
             const double av = fabs(v[c]);
