@@ -291,11 +291,12 @@ struct gq_ctx {
     // Captured step chunks, a few launch shapes at a time.  The key holds
     // everything a capture records (launch arguments, buffers, per-step
     // prefixes, configuration, start buffer): an identical key relaunches the
-    // instantiated graph without a new capture.  A new shape is captured and
-    // instantiated once; the least recently used entry is destroyed when the
-    // cache is full (callers alternating between shapes -- odd and even step
-    // counts, stage-2 batches of different sizes -- no longer capture on
-    // every call).  Instantiated graphs are never updated in place.
+    // instantiated graph without a new capture.  A shape is captured and
+    // instantiated on its second sighting (the first runs as plain launches);
+    // the least recently used entry is destroyed when the cache is full
+    // (callers alternating between shapes -- odd and even step counts,
+    // stage-2 batches of different sizes -- no longer capture on every call).
+    // Instantiated graphs are never updated in place.
     struct CachedGraph {
         std::vector<uint64_t> key;
         hipGraphExec_t exec = nullptr;
@@ -308,6 +309,26 @@ struct gq_ctx {
     std::vector<CachedGraph> graphs;
     uint64_t graph_clock = 0;
     int64_t n_captures = 0, n_relaunches = 0;
+    // Hashes of recently launched keys that have no graph: a key is captured
+    // on its second sighting only, so a launch shape that occurs once (an
+    // adaptive chunk's prefixes, a sampler batch of an unusual size) runs as
+    // plain launches and never costs a capture and an instantiation.
+    static constexpr int SEEN_KEYS = 64;
+    // Instantiations per context are bounded: past this many captures every
+    // new shape runs as plain launches (cached graphs are still relaunched),
+    // so a long-running caller never instantiates and destroys graphs without
+    // end.
+    static constexpr int64_t MAX_CAPTURES = 4096;
+    uint64_t seen_keys[SEEN_KEYS] = {};
+    int seen_next = 0;
+    bool key_seen(uint64_t h)
+    {
+        for (uint64_t k : seen_keys)
+            if (k == h) return true;
+        seen_keys[seen_next] = h;
+        seen_next = (seen_next + 1) % SEEN_KEYS;
+        return false;
+    }
 
     void clear_graphs()
     {
@@ -1181,9 +1202,16 @@ struct gq_ctx {
             for (int j = 0; j < m; j++) u(((uint64_t)(uint32_t)nwork[j] << 32) | (uint32_t)nrows[j]);
         }
         CachedGraph *hit = nullptr;
-        if (use_graph && !persist)
+        bool capture = false;
+        if (use_graph && !persist) {
             for (auto &c : graphs)
                 if (c.key == key) hit = &c;
+            if (!hit) {
+                uint64_t h = 1469598103934665603ULL;  // FNV-1a over the key words
+                for (uint64_t v : key) h = (h ^ v) * 1099511628211ULL;
+                capture = key_seen(h | 1) && n_captures < MAX_CAPTURES;  // | 1: never the empty slot value 0
+            }
+        }
         if (hit) {
             // same launches as an instantiated graph: relaunch it
             hit->used = ++graph_clock;
@@ -1198,7 +1226,7 @@ struct gq_ctx {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, events[0], events[1]));
             total = ms;
-        } else if (use_graph && !persist) {  // a cooperative launch is not captured: one launch per chunk anyway
+        } else if (capture) {  // a cooperative launch is not captured: one launch per chunk anyway
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {

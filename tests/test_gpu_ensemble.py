@@ -242,8 +242,9 @@ def test_ensemble_graph_relaunch_bit_identical(inputs):
 
 def test_ensemble_graph_cache_alternating_shapes(inputs):
     """Runs alternating between two launch shapes (odd and even step counts,
-    so the start buffer alternates too) capture each shape once and relaunch
-    it afterwards; every result equals the run without graphs to the bit."""
+    so the start buffer alternates too) launch each shape directly the first
+    time, capture it the second time and relaunch it afterwards; every result
+    equals the run without graphs to the bit."""
     pwm, pgm, ics, _ = inputs
     shapes = [[5, 8, 3], [4, 7, 3]]
     out = []
@@ -260,8 +261,10 @@ def test_ensemble_graph_cache_alternating_shapes(inputs):
         if graph:
             cap, rel, kept = gq.graph_info(ctx)
             print("captures %d relaunches %d cached %d" % (cap, rel, kept))
-            # two shapes x two start buffers at most
-            assert cap <= 4 and cap + rel == 10 and kept == cap
+            # two shapes x two start buffers at most; each key's first run is
+            # direct, its second a capture, later ones relaunches
+            direct = 10 - cap - rel
+            assert cap <= 4 and kept == cap and 1 <= direct <= 4 and rel >= 10 - 2 * 4
         out.append(res)
         ctx.close()
     for k in range(10):
