@@ -1607,99 +1607,102 @@ __global__ __launch_bounds__(256) void k_lik_flip(UniformArgs u, const uint8_t *
     block_sum_store<3>(acc, partial + (size_t)blockIdx.y * gridDim.x * 3);
 }
 
-// The same likelihood for power-of-two model and observation grids (every
-// grid of this path): a thread takes 4 consecutive observation voxels (one
-// 4-byte load of each mask, one 16-byte load of PET), and the model value of
-// each comes from shifts and masks -- the base-level block, its leaf slot
-// (lut_slot), the leaf's level and index, the cell and the replication test
-// of voxel_value -- with no 64-bit division.  Reads 10 B per voxel (fewer
-// model bytes where leaves are coarser than the model grid).
-__device__ __forceinline__ bool voxel_value_pow2(const UniformArgs &u, int lgpd, int64_t mx, int64_t my, int64_t mz,
-                                                 double &val)
-{
-    if (u.vol) {
-        val = u.vol[mx + (my << lgpd) + (mz << (2 * lgpd))];
-        return true;
-    }
-    const int lb = lgpd - 4;  // log2 of the blocks per dimension
-    const int bx = (int)(mx >> 4), by = (int)(my >> 4), bz = (int)(mz >> 4);
-    const int slot = u.lut_slot[((size_t)blockIdx.y << (3 * lb)) + bx + (by << lb) + (bz << (2 * lb))];
-    const int lr = lb - u.level[slot];  // log2 of the voxels per leaf cell per axis
-    const int lx = (int)mx - (u.index[slot * 3 + 0] << (4 + lr));
-    const int ly = (int)my - (u.index[slot * 3 + 1] << (4 + lr));
-    const int lz = (int)mz - (u.index[slot * 3 + 2] << (4 + lr));
-    const int ix = lx >> lr, iy = ly >> lr, iz = lz >> lr;
-    const int rep = lr < 2 ? (1 << lr) : 4;
-    if (lx - (ix << lr) >= rep || ly - (iy << lr) >= rep || lz - (iz << lr) >= rep) return false;
-    val = (double)u.phi[(size_t)slot * B3 + ix + B * (iy + B * iz)];
-    return true;
-}
-
-// Model values of observation voxels ox..ox+3 of one row (ox a multiple of
-// 4) at stride 2^ls <= 4: the four model voxels lie in one base-level block,
-// so the leaf look-up (slot, level, index) and the y/z replication tests are
-// done once for the four.
-__device__ __forceinline__ void row4_values(const UniformArgs &u, int lgpd, int ls, int64_t ox, int64_t oy, int64_t oz,
-                                            double (&m)[4])
-{
-    const int64_t mx = ox << ls, my = oy << ls, mz = oz << ls;
-    if (u.vol) {
-        const double *p = u.vol + mx + (my << lgpd) + (mz << (2 * lgpd));
-#pragma unroll
-        for (int j = 0; j < 4; j++) m[j] = p[j << ls];
-        return;
-    }
-    const int lb = lgpd - 4;
-    const int slot = u.lut_slot[((size_t)blockIdx.y << (3 * lb)) + (int)(mx >> 4) + ((int)(my >> 4) << lb) +
-                                ((int)(mz >> 4) << (2 * lb))];
-    const int lr = lb - u.level[slot];
-    const int rep = lr < 2 ? (1 << lr) : 4;
-    const int lx = (int)mx - (u.index[slot * 3 + 0] << (4 + lr));
-    const int ly = (int)my - (u.index[slot * 3 + 1] << (4 + lr));
-    const int lz = (int)mz - (u.index[slot * 3 + 2] << (4 + lr));
-    const int iy = ly >> lr, iz = lz >> lr;
-    const bool yz = ly - (iy << lr) < rep && lz - (iz << lr) < rep;
-    const float *row = u.phi + (size_t)slot * B3 + B * (iy + B * iz);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int x = lx + (j << ls);
-        const int ix = x >> lr;
-        m[j] = (yz && x - (ix << lr) < rep) ? (double)row[ix] : u.hole;
-    }
-}
-
+// Flip-rate likelihood over quads of 4 consecutive observation voxels of a
+// row, for observation strides 2^ls <= 4: a quad's four model voxels lie in
+// one base-level block, so the leaf look-up (slot, level, index) and the y/z
+// replication tests are done once for the four.  U grid-stride iterations run
+// at once in three rounds of loads (observations and look-up slots, leaf
+// level and index, model rows), and the terms are added in the order of the
+// one-at-a-time loop.  VOL: the model is a uniform FP64 volume (u.vol).
+template <bool VOL>
 __global__ __launch_bounds__(256) void k_lik_flip4(UniformArgs u, int lgpd, int ls,
                                                    const uint8_t *__restrict__ t1, const uint8_t *__restrict__ t2,
                                                    const float *__restrict__ pet, double th1, double th2, double lp1,
                                                    double lp0, double c0, double is2, double *partial)
 {
     const int lg = lgpd - ls;  // log2 of the observation grid per dimension
+    const int lb = lgpd - 4;   // log2 of the base blocks per dimension
     const int64_t mask = ((int64_t)1 << lg) - 1;
     const int64_t nq = ((int64_t)1 << (3 * lg)) >> 2;
+    const int64_t S = (int64_t)gridDim.x * blockDim.x;
+    const int *lut = u.lut_slot + ((size_t)blockIdx.y << (3 * lb));
+    constexpr int U = 4;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = q << 2;
-        const uint32_t m1 = *reinterpret_cast<const uint32_t *>(t1 + v);
-        const uint32_t m2 = *reinterpret_cast<const uint32_t *>(t2 + v);
-        const float4 pv = *reinterpret_cast<const float4 *>(pet + v);
-        const float pp[4] = {pv.x, pv.y, pv.z, pv.w};
-        const int64_t ox = v & mask, oy = (v >> lg) & mask, oz = v >> (2 * lg);
-        double m[4];
-        if (ls <= 2) {
-            row4_values(u, lgpd, ls, ox, oy, oz, m);
-        } else {
+    for (int64_t q0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nq; q0 += U * S) {
+        uint32_t m1[U], m2[U];
+        float4 pv[U];
+        int mx[U], my[U], mz[U], slot[U];
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (!voxel_value_pow2(u, lgpd, (ox + j) << ls, oy << ls, oz << ls, m[j])) m[j] = u.hole;
+        for (int k = 0; k < U; k++) {
+            const int64_t q = q0 + k * S < nq ? q0 + k * S : q0;
+            const int64_t v = q << 2;
+            m1[k] = *reinterpret_cast<const uint32_t *>(t1 + v);
+            m2[k] = *reinterpret_cast<const uint32_t *>(t2 + v);
+            pv[k] = *reinterpret_cast<const float4 *>(pet + v);
+            mx[k] = (int)((v & mask) << ls);
+            my[k] = (int)(((v >> lg) & mask) << ls);
+            mz[k] = (int)((v >> (2 * lg)) << ls);
+            if (!VOL) slot[k] = lut[(mx[k] >> 4) + ((my[k] >> 4) << lb) + ((mz[k] >> 4) << (2 * lb))];
+        }
+        double m[U][4];
+        if (VOL) {
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const double *p = u.vol + mx[k] + ((int64_t)my[k] << lgpd) + ((int64_t)mz[k] << (2 * lgpd));
+#pragma unroll
+                for (int j = 0; j < 4; j++) m[k][j] = p[j << ls];
+            }
+        } else {
+            int lv[U], i0[U], i1[U], i2[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                lv[k] = u.level[slot[k]];
+                i0[k] = u.index[slot[k] * 3 + 0];
+                i1[k] = u.index[slot[k] * 3 + 1];
+                i2[k] = u.index[slot[k] * 3 + 2];
+            }
+            float f[U][4];
+            int hx[U][4];
+            bool yz[U];
+            int rep[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int lr = lb - lv[k];  // log2 of the voxels per leaf cell per axis
+                rep[k] = lr < 2 ? (1 << lr) : 4;
+                const int lx = mx[k] - (i0[k] << (4 + lr));
+                const int ly = my[k] - (i1[k] << (4 + lr));
+                const int lz = mz[k] - (i2[k] << (4 + lr));
+                const int iy = ly >> lr, iz = lz >> lr;
+                yz[k] = ly - (iy << lr) < rep[k] && lz - (iz << lr) < rep[k];
+                const float *row = u.phi + (size_t)slot[k] * B3 + B * (iy + B * iz);
+                // every voxel's cell lies in the block (a hole reads its cell
+                // too): loads without branches, the hole chosen afterwards
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int x = lx + (j << ls);
+                    const int ix = x >> lr;
+                    f[k][j] = row[ix];
+                    hx[k][j] = x - (ix << lr);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < U; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) m[k][j] = (yz[k] && hx[k][j] < rep[k]) ? (double)f[k][j] : u.hole;
         }
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int o1 = (int)((m1 >> (8 * j)) & 0xff), o2 = (int)((m2 >> (8 * j)) & 0xff);
-            acc[0] += ((int)(m[j] > th1) == o1) ? lp1 : lp0;
-            acc[1] += ((int)(m[j] > th2) == o2) ? lp1 : lp0;
-            if (!__builtin_isnan(pp[j])) {
-                const double r = (double)pp[j] - m[j];
-                acc[2] += c0 - r * r * is2;
+        for (int k = 0; k < U; k++) {
+            if (q0 + k * S >= nq) break;
+            const float pp[4] = {pv[k].x, pv[k].y, pv[k].z, pv[k].w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int o1 = (int)((m1[k] >> (8 * j)) & 0xff), o2 = (int)((m2[k] >> (8 * j)) & 0xff);
+                acc[0] += ((int)(m[k][j] > th1) == o1) ? lp1 : lp0;
+                acc[1] += ((int)(m[k][j] > th2) == o2) ? lp1 : lp0;
+                if (!__builtin_isnan(pp[j])) {
+                    const double r = (double)pp[j] - m[k][j];
+                    acc[2] += c0 - r * r * is2;
+                }
             }
         }
     }
@@ -2010,11 +2013,14 @@ hipError_t launch_lik_flip(const UniformArgs &u, int nsamples, const uint8_t *t1
         return (1 << l) == x ? l : -1;
     };
     const int lgpd = lg2(u.gpd), ls = lg2(u.obs_stride);
-    const bool pow2 = lgpd >= 4 && ls >= 0 && lgpd - ls >= 2 &&  // >= 4 observation voxels per row
+    const bool pow2 = lgpd >= 4 && ls >= 0 && ls <= 2 &&  // a quad's model voxels in one base block
                       ((uintptr_t)t1 & 3) == 0 && ((uintptr_t)t2 & 3) == 0 && ((uintptr_t)pet & 15) == 0;
-    if (pow2)
-        hipLaunchKernelGGL(k_lik_flip4, dim3(nblocks, nsamples), dim3(256), 0, st, u, lgpd, ls, t1, t2, pet, th1, th2,
-                           lp1, lp0, c0, is2, partial);
+    if (pow2 && u.vol)
+        hipLaunchKernelGGL(k_lik_flip4<true>, dim3(nblocks, nsamples), dim3(256), 0, st, u, lgpd, ls, t1, t2, pet, th1,
+                           th2, lp1, lp0, c0, is2, partial);
+    else if (pow2)
+        hipLaunchKernelGGL(k_lik_flip4<false>, dim3(nblocks, nsamples), dim3(256), 0, st, u, lgpd, ls, t1, t2, pet,
+                           th1, th2, lp1, lp0, c0, is2, partial);
     else
         hipLaunchKernelGGL(k_lik_flip, dim3(nblocks, nsamples), dim3(256), 0, st, u, t1, t2, pet, th1, th2, lp1, lp0,
                            c0, is2, partial);
