@@ -1821,8 +1821,8 @@ gq_status gq_adjoint_flip(gq_ctx *ctx, const gq_fk_params *p, int n_steps, int c
             const double lp1 = std::log(1. - eps), lp0 = std::log(eps);
             const double c0 = -0.5 * std::log(2. * M_PI * sigma * sigma);
             const double is2 = 1. / (2. * sigma * sigma);
-            const int64_t n = (int64_t)gpd * gpd * gpd;
-            const int nb = gq::reduce_blocks(n);
+            const int64_t og = gpd / ostride;
+            const int nb = gq::flip_blocks(og * og * og);
             ctx->d_partial.reserve((size_t)S * nb * 3 + (size_t)S * 3);
             double *res = ctx->d_partial.p + (size_t)S * nb * 3;
             HIPCHK(gq::launch_lik_flip(u, S, ctx->d_t1u8.p, ctx->d_t2u8.p, ctx->d_petf.p, th1, th2, lp1, lp0, c0, is2,
@@ -2028,8 +2028,8 @@ gq_status gq_loglik_flip(gq_ctx *ctx, const double params[4], double out[4])
         const double lp1 = std::log(1. - eps), lp0 = std::log(eps);
         const double c0 = -0.5 * std::log(2. * M_PI * sigma * sigma);
         const double is2 = 1. / (2. * sigma * sigma);
-        const int64_t n = (int64_t)u.gpd * u.gpd * u.gpd;
-        const int nb = gq::reduce_blocks(n);
+        const int64_t og = u.gpd / u.obs_stride;
+        const int nb = gq::flip_blocks(og * og * og);
         ctx->d_partial.reserve((size_t)nb * 3 + 8);
         double *res = ctx->d_partial.p + (size_t)nb * 3;
         HIPCHK(gq::launch_lik_flip(u, 1, ctx->d_t1u8.p, ctx->d_t2u8.p, ctx->d_petf.p, th1, th2, lp1, lp0, c0, is2,
@@ -2057,8 +2057,8 @@ gq_status gq_ensemble_loglik_flip(gq_ctx *ctx, const double params[4], double *o
         const double lp1 = std::log(1. - eps), lp0 = std::log(eps);
         const double c0 = -0.5 * std::log(2. * M_PI * sigma * sigma);
         const double is2 = 1. / (2. * sigma * sigma);
-        const int64_t n = (int64_t)u.gpd * u.gpd * u.gpd;
-        const int nb = gq::reduce_blocks(n);
+        const int64_t og = u.gpd / u.obs_stride;
+        const int nb = gq::flip_blocks(og * og * og);
         ctx->d_partial.reserve((size_t)S * nb * 3 + (size_t)S * 3);
         double *res = ctx->d_partial.p + (size_t)S * nb * 3;
         HIPCHK(gq::launch_lik_flip(u, S, ctx->d_t1u8.p, ctx->d_t2u8.p, ctx->d_petf.p, th1, th2, lp1, lp0, c0, is2,

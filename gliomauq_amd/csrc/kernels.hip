@@ -1995,6 +1995,16 @@ hipError_t launch_dump_uniform(const UniformArgs &u, double *out, hipStream_t st
     return hipGetLastError();
 }
 
+int flip_blocks(int64_t n_obs)
+{
+    // at least 16 observation voxels per thread: one pass of k_lik_flip4's
+    // four quads, so no iteration loads a clamped duplicate
+    int64_t nb = n_obs / 4096;
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    return (int)nb;
+}
+
 int reduce_blocks(int64_t n)
 {
     int64_t nb = (n + 255) / 256;

@@ -168,6 +168,8 @@ hipError_t launch_collapse(float *phi, float *pw, float *pg, const int *parents,
                            hipStream_t st);
 hipError_t launch_dump_uniform(const UniformArgs &u, double *out, hipStream_t st);
 int reduce_blocks(int64_t n);
+// workgroups of the flip-rate likelihood for n_obs observation voxels (at most 2048)
+int flip_blocks(int64_t n_obs);
 // nsamples views (lut_slot per sample); partial: nsamples*nblocks*3, out: nsamples*3
 hipError_t launch_lik_flip(const UniformArgs &u, int nsamples, const uint8_t *t1, const uint8_t *t2, const float *pet,
                            double th1, double th2, double lp1, double lp0, double c0, double is2, double *partial,

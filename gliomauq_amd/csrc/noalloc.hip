@@ -23,7 +23,7 @@
 namespace {
 
 // 3 partial sums per workgroup of the likelihood kernel, at most 2048
-// workgroups (gq::reduce_blocks)
+// workgroups (gq::flip_blocks)
 constexpr int SCRATCH_DOUBLES = 3 * 2048;
 constexpr int MAX_DEVICES = 64;
 
@@ -67,8 +67,8 @@ extern "C" gq_status gq_loglik_flip_volume_noalloc(const double *model, int gpd,
     if (gpd <= 0 || gpd > GQ_NOALLOC_MAX_GPD || obs_gpd <= 0 || gpd % obs_gpd != 0) return GQ_ERR_ARG;
     const double th1 = params[0], th2 = params[1], eps = params[2], sigma = params[3];
     if (!(eps > 0 && eps < 1) || !(sigma > 0)) return GQ_ERR_NUMERIC;
-    const int64_t n = (int64_t)gpd * gpd * gpd;
-    const int nb = gq::reduce_blocks(n);
+    const int64_t og = obs_gpd;
+    const int nb = gq::flip_blocks(og * og * og);
     if (3 * nb > SCRATCH_DOUBLES) return GQ_ERR_UNSUPPORTED;
     double *partial = nullptr;
     if (scratch_ptr(&partial) != hipSuccess) return GQ_ERR_HIP;
