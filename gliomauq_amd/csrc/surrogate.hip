@@ -158,19 +158,24 @@ __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
 }
 
 // The same convolution for layers with few input channels (CIN = 8, 16, 48):
-// the GEMM's k = tap * CIN + ci runs in steps of 32 across taps, so a step
-// with CIN < 32 covers several taps and only the last step of 27 * CIN is
-// padded (7 steps instead of 27 for CIN = 8, 14 instead of 27 for 16, 41
-// instead of 54 for 48).  The brick's 10x10x6 halo is staged with all CIN
-// channels at once; lane group kq reads its 8 k of a step from the halo voxel
-// of its own tap.  Same output mapping as k_conv3_brick.
-template <int CIN, int NTG, bool OUT_BF16>
+// the GEMM's k = tap * C + ci runs in steps of 32 across taps, so a step with
+// C < 32 covers several taps and only the last step of 27 * C is padded (7
+// steps instead of 27 for CIN = 8, 14 instead of 27 for 16).  The input
+// channels are staged CP at a time (passes over channel blocks of CP, the
+// same accumulators): the brick's 10x10x6 halo with CP channels per voxel in
+// LDS (CIN = 48 in two passes of 24: 38 KB of LDS instead of 67, four
+// workgroups per CU instead of two; 42 steps instead of 41).  Lane group kq
+// reads its 8 k of a step from the halo voxel of its own tap.  Same output
+// mapping as k_conv3_brick.
+template <int CIN, int CP, int NTG, bool OUT_BF16>
 __global__ __launch_bounds__(256) void k_conv3_flat(ConvArgs a)
 {
-    constexpr int LD = CIN + 8;  // LDS channel stride per halo voxel (pad: banks)
-    constexpr int KT = 27 * CIN;
-    constexpr int NST = (KT + 31) / 32;
-    constexpr int G8 = CIN / 8;
+    static_assert(CIN % CP == 0 && CP % 8 == 0, "channel passes");
+    constexpr int LD = CP + 8;  // LDS channel stride per halo voxel (pad: banks)
+    constexpr int KT = 27 * CIN;   // the weights' k extent
+    constexpr int KP = 27 * CP;    // k of one pass
+    constexpr int NST = (KP + 31) / 32;
+    constexpr int G8 = CP / 8;
     __shared__ __attribute__((aligned(16))) __bf16 s_in[600 * LD];
     const int D = a.in.D;
     const int nbx = D >> 3, nbz = D >> 2;
@@ -192,43 +197,53 @@ __global__ __launch_bounds__(256) void k_conv3_flat(ConvArgs a)
         const int r = 2 * (4 * wave + m) + (li >> 3);
         hbase[m] = (li & 7) + 10 * ((r & 7) + 10 * (r >> 3));
     }
-    for (int e = t; e < 600 * G8; e += 256) {
-        const int h = e / G8, g = e - h * G8;
-        const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
-        const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-        bf16x8 v = zero8();
-        if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D)
-            v = load_input8(a.in, a.up, a.cup, bb, gx, gy, gz, 8 * g);
-        *reinterpret_cast<bf16x8 *>(&s_in[h * LD + 8 * g]) = v;
-    }
-    __syncthreads();
-    constexpr int PF = 3;
-    bf16x8 vb[PF + 1][NTG];
-    auto load_w = [&](int st, bf16x8 *dst) {
-        const int k = 32 * st + 8 * kq;
-#pragma unroll
-        for (int n = 0; n < NTG; n++) {
-            const int co = 16 * (nt0 + n) + li;
-            dst[n] = (k < KT && co < a.cout) ? load8(a.w + (int64_t)co * KT + k) : zero8();
+#pragma unroll 1
+    for (int cb = 0; cb < CIN; cb += CP) {
+        if (cb > 0) __syncthreads();  // the previous pass has read the halo
+        for (int e = t; e < 600 * G8; e += 256) {
+            const int h = e / G8, g = e - h * G8;
+            const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            bf16x8 v = zero8();
+            if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D)
+                v = load_input8(a.in, a.up, a.cup, bb, gx, gy, gz, cb + 8 * g);
+            *reinterpret_cast<bf16x8 *>(&s_in[h * LD + 8 * g]) = v;
         }
-    };
+        __syncthreads();
+        // the lane's k offset, opaque to the compiler in each pass: the
+        // per-step tap and channel offsets are recomputed per pass instead of
+        // being kept in registers across passes
+        int kq8 = 8 * kq;
+        asm volatile("" : "+v"(kq8));
+        constexpr int PF = 3;
+        bf16x8 vb[PF + 1][NTG];
+        auto load_w = [&](int st, bf16x8 *dst) {
+            const int k = 32 * st + kq8;  // k of this pass
+            const int tap = k / CP, ci = k - tap * CP;
 #pragma unroll
-    for (int p = 0; p < PF; p++)
-        if (p < NST) load_w(p, vb[p]);
+            for (int n = 0; n < NTG; n++) {
+                const int co = 16 * (nt0 + n) + li;
+                dst[n] = (k < KP && co < a.cout) ? load8(a.w + (int64_t)co * KT + tap * CIN + cb + ci) : zero8();
+            }
+        };
 #pragma unroll
-    for (int st = 0; st < NST; st++) {
-        if (st + PF < NST) load_w(st + PF, vb[(st + PF) % (PF + 1)]);
-        const int k = 32 * st + 8 * kq;
-        // past the last tap the weights are zero; read a valid voxel so the product is 0
-        const int tap = k < KT ? k / CIN : 26;
-        const int ci = k < KT ? k - tap * CIN : 0;
-        const int hoff = tap % 3 + 10 * ((tap / 3) % 3 + 10 * (tap / 9));
+        for (int p = 0; p < PF; p++)
+            if (p < NST) load_w(p, vb[p]);
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_in[(hbase[m] + hoff) * LD + ci]);
+        for (int st = 0; st < NST; st++) {
+            if (st + PF < NST) load_w(st + PF, vb[(st + PF) % (PF + 1)]);
+            const int k = 32 * st + kq8;
+            // past the last tap the weights are zero; read a valid voxel so the product is 0
+            const int tap = k < KP ? k / CP : 26;
+            const int ci = k < KP ? k - tap * CP : 0;
+            const int hoff = tap % 3 + 10 * ((tap / 3) % 3 + 10 * (tap / 9));
 #pragma unroll
-            for (int n = 0; n < NTG; n++)
-                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb[st % (PF + 1)][n], acc[m][n], 0, 0, 0);
+            for (int m = 0; m < 4; m++) {
+                const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_in[(hbase[m] + hoff) * LD + ci]);
+#pragma unroll
+                for (int n = 0; n < NTG; n++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb[st % (PF + 1)][n], acc[m][n], 0, 0, 0);
+            }
         }
     }
 #pragma unroll
@@ -705,9 +720,9 @@ hipError_t launch_conv3(const ConvArgs &a, hipStream_t st)
     const dim3 grid((unsigned)(a.batch * (D / 8) * (D / 8) * (D / 4)), (unsigned)(nt / ntg));
     const bool bf = a.out_bf != nullptr;
     if (a.cin == 8 || a.cin == 16 || a.cin == 48) {
-#define GQ_FLAT(C, N)                                                                     \
-    if (bf) hipLaunchKernelGGL((k_conv3_flat<C, N, true>), grid, dim3(256), 0, st, a);    \
-    else hipLaunchKernelGGL((k_conv3_flat<C, N, false>), grid, dim3(256), 0, st, a);
+#define GQ_FLAT(C, N)                                                                                        \
+    if (bf) hipLaunchKernelGGL((k_conv3_flat<C, (C > 32 ? C / 2 : C), N, true>), grid, dim3(256), 0, st, a);  \
+    else hipLaunchKernelGGL((k_conv3_flat<C, (C > 32 ? C / 2 : C), N, false>), grid, dim3(256), 0, st, a);
 #define GQ_FLAT_N(C)                    \
     switch (ntg) {                      \
     case 1: GQ_FLAT(C, 1) break;        \
