@@ -41,8 +41,9 @@ public:
     int workers() const { return (int)th_.size(); }
 
     // f(begin, end) for consecutive chunks of [0, n) of `grain` items; the
-    // calling thread takes chunks too.  Returns when every chunk is done (no
-    // chunk runs after that); the first exception thrown by f is rethrown.
+    // calling thread takes chunks too (and runs f(0, n) itself when the pool
+    // has no workers or n fits one chunk).  Returns when every chunk is done
+    // (no chunk runs after that); the first exception thrown by f is rethrown.
     void run(int n, int grain, const std::function<void(int, int)> &f)
     {
         if (n <= 0) return;
