@@ -593,7 +593,7 @@ struct gq_ctx {
             }
             return v;
         }();
-        pool().run((int)work.size(), PAR_ITEMS, [&](int b, int e) {
+        auto records = [&](int b, int e) {
             for (int i = b; i < e; i++) {
                 const int sl = work[i];
                 int *r = tabs.host_ptr(6) + (size_t)i * gq::WREC;
@@ -607,7 +607,9 @@ struct gq_ctx {
                 r[14] = h_ssample[sl];
                 r[15] = 0;
             }
-        });
+        };
+        if ((int)work.size() >= PAR_ITEMS) pool().run((int)work.size(), PAR_ITEMS, records);
+        else records(0, (int)work.size());
         tabs.send(st);
         p_work = tabs.ptr(0);
         p_sfbase = tabs.ptr(1);
