@@ -752,6 +752,11 @@ struct gq_ctx {
         return out;
     }
 
+    // The same-level neighbour rows of sample s (slot-indexed) when they
+    // describe its current leaves: every topology edit marks the sample
+    // dirty, and only its own new leaves take its freed slots.
+    const int *nbr26_of(int s) const { return samples[s].dirty ? nullptr : h_nbr26.data(); }
+
     // Refine the flagged block ids of each listed sample (Refiner_SpaceExtension plan + split).
     // flags[k]: per block id of sample which[k], nflag[k] of them set.
     std::vector<int> refine_batch(const std::vector<int> &which, const std::vector<std::vector<char>> &flags,
@@ -763,7 +768,7 @@ struct gq_ctx {
         {
             ScopeTimer tm(t_plan);
             parallel_for(n, [&](int k) {
-                if (nflag[k] > 0) parents[k] = G(which[k]).plan_refine(flags[k]);
+                if (nflag[k] > 0) parents[k] = G(which[k]).plan_refine(flags[k], nbr26_of(which[k]));
             });
         }
         size_t np = 0;
@@ -829,7 +834,7 @@ struct gq_ctx {
         {
             ScopeTimer tm(t_plan);
             parallel_for(n, [&](int k) {
-                if (nflag[k] > 0) plan[k] = G(which[k]).plan_compress(flags[k]);
+                if (nflag[k] > 0) plan[k] = G(which[k]).plan_compress(flags[k], nbr26_of(which[k]));
             });
         }
         size_t np = 0;
@@ -953,7 +958,7 @@ struct gq_ctx {
                     cand.assign(nice[k].size(), 0);
                     for (const auto &lf : L) cand[lf.block_id] = !nice[k][lf.block_id];
                     ScopeTimer tm(t_plan);
-                    if (sm.grid->plan_compress(cand).empty()) {
+                    if (sm.grid->plan_compress(cand, nbr26_of(which[k])).empty()) {
                         live[k] = 0;
                         continue;
                     }

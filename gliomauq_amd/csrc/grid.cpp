@@ -199,6 +199,36 @@ bool Grid::any_ring_balanced(int i, F &&f) const
     return false;
 }
 
+template <class F>
+bool Grid::any_other_level(int i, const int *nbr26, F &&f) const
+{
+    // The 26 same-level boxes around leaf i (periodic); where the table has
+    // no same-level leaf, the box lies inside one coarser leaf or is split
+    // into finer leaves, and the leaf holding its corner look-up cell has that
+    // coarser or finer level.  Every neighbour of another level touching the
+    // leaf is found this way: a coarser one contains the box on its side, a
+    // finer one splits it.  f(k, image) per such box, image = the box is
+    // reached through the periodic boundary; true stops the walk.
+    const Leaf &lf = leaves_[i];
+    const int *nb = nbr26 + (size_t)lf.slot * 26;
+    const int m = 1 << lf.level, sh = lut_level_ - lf.level;
+    const int64_t nF = 1LL << lut_level_;
+    for (int c = 0; c < 26; c++) {
+        if (nb[c] >= 0) continue;
+        const int cc = c < 13 ? c : c + 1;
+        const int d[3] = {cc % 3 - 1, cc / 3 % 3 - 1, cc / 9 - 1};
+        bool img = false;
+        int64_t q[3];
+        for (int a = 0; a < 3; a++) {
+            const int v = lf.idx[a] + d[a];
+            img = img || v < 0 || v >= m;
+            q[a] = (int64_t)(v & (m - 1)) << sh;
+        }
+        if (f(lut_[(size_t)(q[0] + nF * (q[1] + nF * q[2]))], img)) return true;
+    }
+    return false;
+}
+
 void Grid::neighbors(int i, std::vector<std::pair<int, bool>> &out) const
 {
     out.clear();
@@ -229,21 +259,26 @@ std::vector<std::array<int, 4>> Grid::plan_compress(const std::unordered_set<int
     return plan_compress(id_flags(flagged_ids));
 }
 
-std::vector<int> Grid::plan_refine(const std::vector<char> &flag_by_id) const
+std::vector<int> Grid::plan_refine(const std::vector<char> &flag_by_id, const int *nbr26) const
 {
     const int n = (int)leaves_.size();
     const int nf = (int)flag_by_id.size();
     std::vector<char> flag(n, 0);
+    int top = 0;  // finest leaf level
     for (int i = 0; i < n; i++) {
         const int id = leaves_[i].block_id;
         flag[i] = id < nf && flag_by_id[id] ? 1 : 0;
+        top = std::max(top, leaves_[i].level);
     }
     // Refiner_SpaceExtension step 3: a leaf with a finer, non-image neighbour
-    // is refined as well (MRAGRefiner_SpaceExtension.h:35-51)
+    // is refined as well (MRAGRefiner_SpaceExtension.h:35-51); leaves on the
+    // finest level present have none, and step 4 unflags the finest allowed
+    // level anyway
     for (int i = 0; i < n; i++) {
-        if (flag[i]) continue;
         const int li = leaves_[i].level;
-        if (any_ring_balanced(i, [&](int k, bool img) { return !img && leaves_[k].level > li; })) flag[i] = 1;
+        if (flag[i] || li >= top || li >= max_level_) continue;
+        auto finer = [&](int k, bool img) { return !img && leaves_[k].level > li; };
+        if (nbr26 ? any_other_level(i, nbr26, finer) : any_ring_balanced(i, finer)) flag[i] = 1;
     }
     // step 4: never refine on the finest allowed level (:54-63)
     for (int i = 0; i < n; i++)
@@ -258,13 +293,15 @@ std::vector<int> Grid::plan_refine(const std::vector<char> &flag_by_id) const
         new_v.clear();
         for (int v : old_v) {
             const int nl = leaves_[v].level + 1;
-            any_ring_balanced(v, [&](int k, bool) {
+            auto jump = [&](int k, bool) {
                 if (std::abs(leaves_[k].level - nl) > 1 && !victim[k]) {
                     victim[k] = 1;
                     new_v.push_back(k);
                 }
                 return false;
-            });
+            };
+            if (nbr26) any_other_level(v, nbr26, jump);
+            else any_ring_balanced(v, jump);
         }
         std::swap(old_v, new_v);
     }
@@ -274,7 +311,7 @@ std::vector<int> Grid::plan_refine(const std::vector<char> &flag_by_id) const
     return out;
 }
 
-std::vector<std::array<int, 4>> Grid::plan_compress(const std::vector<char> &flag_by_id) const
+std::vector<std::array<int, 4>> Grid::plan_compress(const std::vector<char> &flag_by_id, const int *nbr26) const
 {
     // Compressor::createPlan (MRAGCompressor.h:68-118): collapse a parent
     // whose 8 children are all leaves, all flagged, and whose children's
@@ -299,7 +336,8 @@ std::vector<std::array<int, 4>> Grid::plan_compress(const std::vector<char> &fla
         bool ok = true;
         for (int c = 0; c < 8 && ok; c++) {
             const int ch = find(p[0] + 1, 2 * p[1] + (c & 1), 2 * p[2] + ((c >> 1) & 1), 2 * p[3] + ((c >> 2) & 1));
-            ok = ch >= 0 && !any_ring_balanced(ch, [&](int q, bool) { return std::abs(leaves_[q].level - p[0]) > 1; });
+            auto jump = [&](int q, bool) { return std::abs(leaves_[q].level - p[0]) > 1; };
+            ok = ch >= 0 && !(nbr26 ? any_other_level(ch, nbr26, jump) : any_ring_balanced(ch, jump));
         }
         if (ok) out.push_back(p);
         a += 8;

@@ -84,8 +84,12 @@ public:
 
     // Planning.  Return leaf indices (refine) or parent keys (compress).
     // Flags by block id: flag[id] != 0 (ids at or above flag.size() unflagged).
-    std::vector<int> plan_refine(const std::vector<char> &flag_by_id) const;
-    std::vector<std::array<int, 4>> plan_compress(const std::vector<char> &flag_by_id) const;
+    // nbr26: optional slot-indexed same-level neighbour table of exactly these
+    // leaves (as build_sample_tables writes it); with it the planners look
+    // only at the directions where a leaf meets another level (same plans).
+    std::vector<int> plan_refine(const std::vector<char> &flag_by_id, const int *nbr26 = nullptr) const;
+    std::vector<std::array<int, 4>> plan_compress(const std::vector<char> &flag_by_id,
+                                                  const int *nbr26 = nullptr) const;
     std::vector<int> plan_refine(const std::unordered_set<int> &flagged_ids) const;
     std::vector<std::array<int, 4>> plan_compress(const std::unordered_set<int> &flagged_ids) const;
     std::vector<char> id_flags(const std::unordered_set<int> &ids) const;
@@ -104,6 +108,8 @@ private:
     bool any_ring(int i, F &&f) const;
     template <class F>
     bool any_ring_balanced(int i, F &&f) const;  // any_ring on a 2:1-balanced grid
+    template <class F>
+    bool any_other_level(int i, const int *nbr26, F &&f) const;  // via the same-level table
     void rebuild_index();
     void expand_rec(int lam, const int64_t p[3], double w, std::vector<Term> &T, int depth) const;
 

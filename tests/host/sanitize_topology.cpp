@@ -6,7 +6,9 @@
 // every adaptation -- refine/compress planning and topology edits, slot
 // assignment as gq_api.cpp does it (free-list reuse), per-sample table builds
 // (build_sample_tables), Morton ordering and ghost-point expansion -- and
-// checks the level-jump invariant and the slot tables after every edit.
+// checks the level-jump invariant and the slot tables after every edit, and
+// that the planners give the same plans through the same-level table as
+// through their ring walks.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -107,6 +109,15 @@ static int run_grid(int bpd, int max_level, int order, int iters, unsigned seed)
         for (int e : t.lut) {
             const int s = e & gq::LUT_SLOT_MASK;
             if (e < 0 || s >= cap || slot_level[s] != (e >> gq::LUT_LEVEL_SHIFT)) return fail("lut entry", it);
+        }
+        // the planners through the same-level table plan exactly as the ring walks
+        for (int k = 0; k < 4; k++) {
+            std::vector<char> fl(g.id_bound(), 0);
+            const unsigned pct = k == 0 ? 2 : k == 1 ? 20 : k == 2 ? 60 : 95;
+            for (const auto &lf : g.leaves())
+                if (rng() % 100 < pct) fl[lf.block_id] = 1;
+            if (g.plan_refine(fl) != g.plan_refine(fl, nbr26.data())) return fail("refine plan via table", it);
+            if (g.plan_compress(fl) != g.plan_compress(fl, nbr26.data())) return fail("compress plan via table", it);
         }
         // ghost expansion of a few halo points of random leaves
         for (int k = 0; k < 8; k++) {
