@@ -35,6 +35,11 @@ HBM_PEAK_GBS = 8000.0
 L_CM = 17.0
 
 
+def ordinal(k):
+    """'' for 1, else '2nd ', '3rd ', '4th ', ... (every k-th launch)."""
+    return "" if k == 1 else {2: "2nd ", 3: "3rd "}.get(k, "%dth " % k)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -934,6 +939,8 @@ def main():
                     help="step-kernel durations from a second solve (separate) or from the timed solve (inline: "
                          "per-launch events, no graphs)")
     ap.add_argument("--step-graphs", type=int, default=1, help="gq_fk_steps replays graphs (1) or launches (0)")
+    ap.add_argument("--timing-stride", type=int, default=3,
+                    help="inline timing: events on every k-th step launch (counted across the solve; 1: all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (samples would shard across GPUs; "
@@ -1009,18 +1016,19 @@ def main():
         g = fresh_grid()
         lik = gq.FlipLikelihood(g, t1, t2, pet)
         s = solver(g)
-        gq.set_step_timing(g, per_kernel)
-        st = {"kernel_ms": 0.0, "cells": 0, "launches": 0, "adaptive_ms": 0.0, "adaptive_cells": 0,
+        gq.set_step_timing(g, per_kernel, args.timing_stride)
+        st = {"kernel_ms": 0.0, "cells": 0, "all_cells": 0, "launches": 0, "adaptive_ms": 0.0, "adaptive_cells": 0,
               "adaptive_launches": 0, "steps_s": 0.0, "adapt_s": 0.0}
         done = 0
         t0 = time.perf_counter()
         while done < n_total:
             m = min(args.adapt_every, n_total - done)
+            st["all_cells"] += m * g.n_blocks() * 4096
             ta = time.perf_counter()
             gq.fk_steps(g, Dw, Dg, rho_f, dt, m)
             st["steps_s"] += time.perf_counter() - ta
             done += m
-            ms, k, cells = gq.step_timing(g)
+            ms, k, cells = gq.step_timing(g)  # the timed launches of the call and their cell updates
             st["kernel_ms"] += ms
             st["cells"] += cells
             st["launches"] += k
@@ -1056,7 +1064,7 @@ def main():
         log("[bench] kernel-timed solve")
         _, stats, gk, _ = solve(True, False)
         del gk
-    cells = main["cells"]
+    cells = main["all_cells"]
     t_steps, t_adapt = main["steps_s"], main["adapt_s"]
     zt, fused, tune = gq.autotune_info(ctx)
     value = cells / wall
@@ -1092,10 +1100,12 @@ def main():
                                        "WRITE_SIZE over the 590 step launches of the timed solve, calibrated on "
                                        "a copy kernel)",
                      "kernel": "k_fk_step", "launches": stats["launches"],
-                     "timing": ("every step launch timed by its dispatch events in the timed solve"
+                     "timing": ("every %sstep launch of the timed solve timed by its dispatch events "
+                                "(hipExtLaunchKernel)" % ordinal(args.timing_stride)
                                 if args.timing == "inline" else
                                 "every step launch timed by its dispatch events in a second, identical solve "
                                 "(the timed solve replays graphs)"),
+                     "timed_launches_of": n_total,
                      "kernel_avg_us": stats["kernel_ms"] * 1e3 / max(stats["launches"], 1),
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "algorithmic_bytes_per_cell": 16,

@@ -277,6 +277,10 @@ struct gq_ctx {
 
     // timing
     bool per_kernel_timing = false;
+    int timing_stride = 1;       // time every timing_stride-th step launch
+    uint64_t timing_count = 0;   // step launches seen by the timing, across calls
+    int last_timed = 0;          // timed launches of the last advance
+    int64_t last_timed_cells = 0;  // cell updates of those launches
     std::vector<hipEvent_t> events;
     double last_ms = 0;
     int last_launches = 0;
@@ -1162,7 +1166,8 @@ struct gq_ctx {
         // steps j, j+1 over the same work prefix without cross-level faces run as one fused two-step launch
         const bool can_fuse = fuse2 && a.ikc != nullptr;
         buf_after.assign(m, 0);
-        int nlaunch = 0, final_c = cur;
+        int nlaunch = 0, final_c = cur, ntimed = 0;
+        int64_t timed_cells = 0;
         const bool persist = cfg_persist(cfg) && zt >= 2;
         auto enqueue = [&](int c0) {
             int c = c0;
@@ -1181,9 +1186,13 @@ struct gq_ctx {
                         p.nrows[j] = fused ? 0 : nrows[j0 + j];
                         buf_after[j0 + j] = c ^ ((j + 1) & 1);
                     }
-                    if (pk) HIPCHK(hipEventRecord(events[2 * nlaunch], st));
+                    if (pk) HIPCHK(hipEventRecord(events[2 * ntimed], st));
                     HIPCHK(gq::launch_fk_persistent(p, zt, st));
-                    if (pk) HIPCHK(hipEventRecord(events[2 * nlaunch + 1], st));
+                    if (pk) HIPCHK(hipEventRecord(events[2 * ntimed + 1], st));
+                    if (pk) {
+                        ntimed++;
+                        for (int j = 0; j < p.m; j++) timed_cells += (int64_t)p.nwork[j] * gq::BS3;
+                    }
                     nlaunch++;
                     c ^= p.m & 1;
                 }
@@ -1199,10 +1208,16 @@ struct gq_ctx {
                 a.phi_in = phi[c];
                 a.phi_out = phi[c ^ 1];
                 a.nwork = nwork[j];
-                hipEvent_t e0 = pk ? events[2 * nlaunch] : nullptr, e1 = pk ? events[2 * nlaunch + 1] : nullptr;
+                // per-kernel timing: every timing_stride-th launch carries its events
+                const bool tj = pk && timing_count++ % (uint64_t)timing_stride == 0;
+                hipEvent_t e0 = tj ? events[2 * ntimed] : nullptr, e1 = tj ? events[2 * ntimed + 1] : nullptr;
+                if (tj) {
+                    ntimed++;
+                    timed_cells += (int64_t)nwork[j] * (pair ? 2 : 1) * gq::BS3;
+                }
                 if (pair) HIPCHK(gq::launch_fk_step2(a, st, e0, e1));
                 else if (nwork[j] > 0) HIPCHK(gq::launch_fk_step(a, zt, st, e0, e1));
-                else if (pk) {
+                else if (tj) {
                     HIPCHK(hipEventRecord(e0, st));
                     HIPCHK(hipEventRecord(e1, st));
                 }
@@ -1299,7 +1314,7 @@ struct gq_ctx {
             if (!pk) HIPCHK(hipEventRecord(events[1], st));
             HIPCHK(hipStreamSynchronize(st));
             if (pk) {
-                for (int j = 0; j < nlaunch; j++) {
+                for (int j = 0; j < ntimed; j++) {
                     float ms = 0;
                     HIPCHK(hipEventElapsedTime(&ms, events[2 * j], events[2 * j + 1]));
                     total += ms;
@@ -1312,6 +1327,8 @@ struct gq_ctx {
         }
         cur = final_c;
         last_nlaunch = nlaunch;
+        last_timed = ntimed;
+        last_timed_cells = timed_cells;
         return total;
     }
 
@@ -1618,16 +1635,18 @@ gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
         const int chunk = 1024;
         double total = 0.0;
         int done = 0, launches = 0;
+        int64_t timed_cells = 0;
         const bool graph = ctx->fk_graphs && !ctx->per_kernel_timing;
         while (done < n_steps) {
             const int m = std::min(chunk, n_steps - done);
             total += ctx->advance(a, cfg, std::vector<int>(m, nwork), std::vector<int>(m, nrows), graph);
-            launches += ctx->last_nlaunch;
+            launches += ctx->per_kernel_timing ? ctx->last_timed : ctx->last_nlaunch;
+            timed_cells += ctx->last_timed_cells;
             done += m;
         }
         ctx->last_ms = total;
         ctx->last_launches = launches;
-        ctx->last_cells = (int64_t)n_steps * nwork * gq::BS3;
+        ctx->last_cells = ctx->per_kernel_timing ? timed_cells : (int64_t)n_steps * nwork * gq::BS3;
     });
 }
 
@@ -2292,7 +2311,12 @@ gq_status gq_set_step_graphs(gq_ctx *ctx, int on)
 
 gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel)
 {
-    return guard(ctx, [&] { ctx->per_kernel_timing = per_kernel != 0; });
+    return guard(ctx, [&] {
+        if (per_kernel < 0) throw gq_error(GQ_ERR_ARG, "per_kernel must be >= 0");
+        ctx->per_kernel_timing = per_kernel != 0;
+        ctx->timing_stride = std::max(1, per_kernel);
+        ctx->timing_count = 0;
+    });
 }
 
 gq_status gq_memory_info(gq_ctx *ctx, int64_t *bytes, int *pool_blocks)

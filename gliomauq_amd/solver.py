@@ -219,8 +219,11 @@ def step_timing(grid):
     return ms.value, n.value, cells.value
 
 
-def set_step_timing(grid, per_kernel):
-    grid.ctx.call("gq_set_step_timing", 1 if per_kernel else 0)
+def set_step_timing(grid, per_kernel, stride=1):
+    """per_kernel: time step-kernel launches with dispatch events, every
+    `stride`-th launch (counted across calls); step_timing then reports the
+    timed launches and their cell updates."""
+    grid.ctx.call("gq_set_step_timing", int(stride) if per_kernel else 0)
 
 
 def set_step_graphs(ctx, on):

@@ -266,13 +266,16 @@ gq_status gq_loglik_flip_volume_noalloc(const double *model, int gpd, const uint
                                         void *stream);
 
 /* ---- measurement ----------------------------------------------------------
- * Device time of the last gq_fk_steps call, summed over its step-kernel
- * launches (ms), with the number of launches and leaf cells updated. */
+ * Device time of the last gq_fk_steps call, summed over its timed step-kernel
+ * launches (ms), with the number of those launches and the leaf cells they
+ * updated (every launch and cell of the call without per-kernel timing). */
 gq_status gq_last_step_timing(gq_ctx *ctx, double *step_kernel_ms, int *launches, int64_t *cell_updates);
-/* per_kernel != 0: time every step-kernel launch of gq_fk_steps with a pair
- * of HIP events that the kernel's dispatch records at its start and end
- * (hipExtLaunchKernel; default 0: one pair around the whole call, which then
- * includes the cross-level ghost kernels). */
+/* per_kernel = k >= 1: time every k-th step-kernel launch of gq_fk_steps
+ * (counted across calls from this call on; 1: every launch) with a pair of
+ * HIP events that the kernel's dispatch records at its start and end
+ * (hipExtLaunchKernel: the events cost host time per launch, so a stride
+ * keeps the timed solve close to an untimed one); 0 (default): one pair
+ * around the whole call, which then includes the cross-level ghost kernels. */
 gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel);
 /* Step autotuner.  mode 1 (default): time the slab depths 16/8/4/2 (planes
  * of a 16^3 block per workgroup) with the cross-level face ghosts from a

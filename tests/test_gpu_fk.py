@@ -110,6 +110,39 @@ def test_fk_steps_rejects_bad_parameters(ctx):
     assert e.value.status == 1
 
 
+def test_step_timing_every_kth_launch(ctx):
+    """gq_set_step_timing(k): events on every k-th step launch, counted
+    across calls; the field is the untimed run's to the bit."""
+    n = 32
+    rng = np.random.default_rng(5)
+    phi = (0.4 * rng.random((n, n, n))).astype(np.float32)
+    pw = rng.random((n, n, n)).astype(np.float32)
+    pg = (1 - pw).astype(np.float32)
+    outs = []
+    for stride in (0, 1, 3):
+        g = gq.Grid(ctx, 2, 1, 2)
+        for name, v in (("phi", phi), ("pw", pw), ("pg", pg)):
+            g.sample_volume(name, v)
+        gq.set_step_timing(g, stride > 0, max(stride, 1))
+        got = []
+        for m in (10, 10, 4):
+            gq.fk_steps(g, 1e-5, 1e-6, 0.03, 0.5, m)
+            got.append(gq.step_timing(g))
+        outs.append(g.dump_uniform())
+        cells = 8 * 4096
+        if stride == 1:
+            assert [k for _, k, _ in got] == [10, 10, 4]
+        elif stride == 3:
+            # launches 0..23: timed at 0, 3, ..., 21 -> 4 in the first call (0, 3, 6, 9),
+            # 3 in the second (12, 15, 18), 1 in the third (21)
+            assert [k for _, k, _ in got] == [4, 3, 1]
+        if stride > 0:
+            for ms, k, c in got:
+                assert c == k * cells and ms > 0
+    gq.set_step_timing(g, False)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
 def test_block_processing_mirror(ctx):
     """pipeline_process(RD) + process(UpdateTumor) == one fused step."""
     n = 32
