@@ -7,12 +7,12 @@ block-operator API (see solver.py) plus the synthetic-input generator.
 from . import _lib  # noqa: F401
 from .solver import (BlockFWT, BlockProcessing, Context, Ensemble, FlipLikelihood, Glioma_HG_UQ,  # noqa: F401
                      Glioma_ReactionDiffusionOperator, Grid, HGG_Likelihood, Science, Topology, UpdateTumor,
-                     fk_steps, n_steps_until, set_step_timing, step_timing, set_autotune, autotune_info, host_stats,
-                     set_step_coeffs, set_step_fusion, set_step_persistent, persistent_info, graph_info,
-                     set_step_graphs)
+                     fk_steps, n_steps_until, set_step_timing, step_timing, step_timing_drain, set_autotune,
+                     autotune_info, host_stats, set_step_coeffs, set_step_fusion, set_step_persistent,
+                     persistent_info, graph_info, set_step_graphs)
 
 __all__ = ["BlockFWT", "BlockProcessing", "Context", "Ensemble", "FlipLikelihood", "Glioma_HG_UQ",
            "Glioma_ReactionDiffusionOperator", "Grid", "HGG_Likelihood", "Science", "Topology", "UpdateTumor",
-           "fk_steps", "n_steps_until", "set_step_timing", "step_timing", "set_autotune", "autotune_info",
-           "host_stats", "set_step_coeffs", "set_step_fusion", "set_step_persistent", "persistent_info",
+           "fk_steps", "n_steps_until", "set_step_timing", "step_timing", "step_timing_drain", "set_autotune",
+           "autotune_info", "host_stats", "set_step_coeffs", "set_step_fusion", "set_step_persistent", "persistent_info",
            "graph_info", "set_step_graphs"]

@@ -21,7 +21,7 @@ EXPORTS = [
     "gq_grid_block_count", "gq_grid_blocks_info", "gq_grid_write_field", "gq_grid_read_field", "gq_fk_steps",
     "gq_fwt_details", "gq_refine_blocks", "gq_compress_blocks", "gq_auto_refine", "gq_auto_compress",
     "gq_dump_uniform", "gq_obs_ref_upload", "gq_loglik_ref", "gq_loglik_ref_volume", "gq_obs_flip_upload", "gq_loglik_flip",
-    "gq_last_step_timing", "gq_set_step_timing", "gq_memory_info", "gq_set_autotune", "gq_autotune_info",
+    "gq_last_step_timing", "gq_set_step_timing", "gq_step_timing_drain", "gq_memory_info", "gq_set_autotune", "gq_autotune_info",
     "gq_set_step_coeffs", "gq_set_step_fusion", "gq_set_step_persistent", "gq_persistent_info",
     "gq_host_stats", "gq_graph_info", "gq_set_step_graphs", "gq_topo_create", "gq_topo_destroy",
     "gq_topo_leaves", "gq_topo_refine", "gq_topo_compress", "gq_topo_point_terms",
@@ -96,6 +96,7 @@ def load():
         "gq_loglik_flip": [vp, f64p, f64p],
         "gq_last_step_timing": [vp, f64p, ip, C.POINTER(C.c_int64)],
         "gq_set_step_timing": [vp, C.c_int],
+        "gq_step_timing_drain": [vp, f32p, C.POINTER(C.c_int64), C.c_int, ip],
         "gq_memory_info": [vp, C.POINTER(C.c_int64), ip],
         "gq_set_autotune": [vp, C.c_int, C.c_int],
         "gq_autotune_info": [vp, ip, ip, f64p],

@@ -281,8 +281,70 @@ struct gq_ctx {
     uint64_t timing_count = 0;   // step launches seen by the timing, across calls
     int last_timed = 0;          // timed launches of the last advance
     int64_t last_timed_cells = 0;  // cell updates of those launches
+    // Step timing without waiting: advance records an event pair per timed
+    // launch (or one around a chunk) and queues it; the pairs are read when a
+    // caller asks for timings (or when many are queued), so a chunk of steps
+    // never ends in a stream synchronisation of its own.
+    struct TimedPair {
+        hipEvent_t e0, e1;
+        int64_t cells;  // cell updates the pair covers
+        uint64_t call;  // gq_fk_steps / gq_ensemble_run call that recorded it
+    };
+    struct TimedSpan {
+        float ms;
+        int64_t cells;
+        uint64_t call;
+    };
+    std::vector<hipEvent_t> event_free;
+    std::vector<TimedPair> timing_pending;
+    std::vector<TimedSpan> timing_done;
+    uint64_t timing_call = 0;
+    static constexpr size_t TIMING_PENDING_MAX = 4096, TIMING_DONE_MAX = 1 << 20;
+
+    hipEvent_t take_event()
+    {
+        if (event_free.empty()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = event_free.back();
+        event_free.pop_back();
+        return e;
+    }
+    void queue_pair(hipEvent_t e0, hipEvent_t e1, int64_t cells)
+    {
+        if (timing_pending.size() >= TIMING_PENDING_MAX) drain_timing();
+        timing_pending.push_back({e0, e1, cells, timing_call});
+    }
+    // Wait for the queued pairs (stream order: the last one ends last) and
+    // move their durations to timing_done; the events go back to the pool.
+    void drain_timing()
+    {
+        if (timing_pending.empty()) return;
+        HIPCHK(hipEventSynchronize(timing_pending.back().e1));
+        if (timing_done.size() + timing_pending.size() > TIMING_DONE_MAX)
+            timing_done.erase(timing_done.begin(), timing_done.begin() + timing_done.size() / 2);
+        for (const TimedPair &q : timing_pending) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, q.e0, q.e1));
+            timing_done.push_back({ms, q.cells, q.call});
+            event_free.push_back(q.e0);
+            event_free.push_back(q.e1);
+        }
+        timing_pending.clear();
+    }
+    // device ms of call `call`'s timed spans (drains first)
+    double call_ms(uint64_t call)
+    {
+        drain_timing();
+        double t = 0.0;
+        for (auto it = timing_done.rbegin(); it != timing_done.rend() && it->call >= call; ++it)
+            if (it->call == call) t += it->ms;
+        return t;
+    }
     std::vector<hipEvent_t> events;
-    double last_ms = 0;
+    uint64_t last_call = 0;  // timing call id of the last gq_fk_steps
     int last_launches = 0;
     int64_t last_cells = 0;
 
@@ -338,6 +400,8 @@ struct gq_ctx {
 
     void clear_graphs()
     {
+        if (graphs.empty()) return;
+        if (st) (void)hipStreamSynchronize(st);  // a graph may still be running
         for (auto &c : graphs)
             if (c.exec) (void)hipGraphExecDestroy(c.exec);
         graphs.clear();
@@ -345,8 +409,14 @@ struct gq_ctx {
 
     ~gq_ctx()
     {
+        if (st) (void)hipStreamSynchronize(st);
         clear_graphs();
         for (auto e : events) (void)hipEventDestroy(e);
+        for (const auto &q : timing_pending) {
+            (void)hipEventDestroy(q.e0);
+            (void)hipEventDestroy(q.e1);
+        }
+        for (auto e : event_free) (void)hipEventDestroy(e);
         release_pool();
         if (st) (void)hipStreamDestroy(st);
     }
@@ -1151,18 +1221,22 @@ struct gq_ctx {
 
     // Advance m steps; step j updates the first nwork[j] work slots and the
     // first nrows[j] face rows (a prefix of the samples in work order).
-    // Returns the device time (ms): step kernels only with per-kernel timing,
-    // else the whole chunk.
-    double advance(gq::StepArgs a, int cfg, const std::vector<int> &nwork, const std::vector<int> &nrows,
-                   bool use_graph)
+    // Queues the device timing (call_ms): the step kernels only with
+    // per-kernel timing, else the whole chunk.  Does not wait for the steps.
+    void advance(gq::StepArgs a, int cfg, const std::vector<int> &nwork, const std::vector<int> &nrows,
+                 bool use_graph)
     {
         const int zt = cfg_zt(cfg);
         const bool fused = cfg_fused(cfg);
         if (fused) a.gphi = nullptr;
         const int m = (int)nwork.size();
-        if (m == 0) return 0.0;
+        last_nlaunch = 0;
+        last_timed = 0;
+        last_timed_cells = 0;
+        if (m == 0) return;
         const bool pk = per_kernel_timing && !use_graph;
-        need_events(pk ? 2 * m : 2);
+        int64_t chunk_cells = 0;
+        for (int j = 0; j < m; j++) chunk_cells += (int64_t)nwork[j] * gq::BS3;
         // steps j, j+1 over the same work prefix without cross-level faces run as one fused two-step launch
         const bool can_fuse = fuse2 && a.ikc != nullptr;
         buf_after.assign(m, 0);
@@ -1186,12 +1260,16 @@ struct gq_ctx {
                         p.nrows[j] = fused ? 0 : nrows[j0 + j];
                         buf_after[j0 + j] = c ^ ((j + 1) & 1);
                     }
-                    if (pk) HIPCHK(hipEventRecord(events[2 * ntimed], st));
+                    hipEvent_t e0 = pk ? take_event() : nullptr, e1 = pk ? take_event() : nullptr;
+                    if (pk) HIPCHK(hipEventRecord(e0, st));
                     HIPCHK(gq::launch_fk_persistent(p, zt, st));
-                    if (pk) HIPCHK(hipEventRecord(events[2 * ntimed + 1], st));
                     if (pk) {
+                        HIPCHK(hipEventRecord(e1, st));
+                        int64_t cl = 0;
+                        for (int j = 0; j < p.m; j++) cl += (int64_t)p.nwork[j] * gq::BS3;
+                        queue_pair(e0, e1, cl);
                         ntimed++;
-                        for (int j = 0; j < p.m; j++) timed_cells += (int64_t)p.nwork[j] * gq::BS3;
+                        timed_cells += cl;
                     }
                     nlaunch++;
                     c ^= p.m & 1;
@@ -1210,16 +1288,18 @@ struct gq_ctx {
                 a.nwork = nwork[j];
                 // per-kernel timing: every timing_stride-th launch carries its events
                 const bool tj = pk && timing_count++ % (uint64_t)timing_stride == 0;
-                hipEvent_t e0 = tj ? events[2 * ntimed] : nullptr, e1 = tj ? events[2 * ntimed + 1] : nullptr;
-                if (tj) {
-                    ntimed++;
-                    timed_cells += (int64_t)nwork[j] * (pair ? 2 : 1) * gq::BS3;
-                }
+                hipEvent_t e0 = tj ? take_event() : nullptr, e1 = tj ? take_event() : nullptr;
                 if (pair) HIPCHK(gq::launch_fk_step2(a, st, e0, e1));
                 else if (nwork[j] > 0) HIPCHK(gq::launch_fk_step(a, zt, st, e0, e1));
                 else if (tj) {
                     HIPCHK(hipEventRecord(e0, st));
                     HIPCHK(hipEventRecord(e1, st));
+                }
+                if (tj) {
+                    const int64_t cl = (int64_t)nwork[j] * (pair ? 2 : 1) * gq::BS3;
+                    queue_pair(e0, e1, cl);
+                    ntimed++;
+                    timed_cells += cl;
                 }
                 c ^= 1;
                 nlaunch++;
@@ -1229,7 +1309,6 @@ struct gq_ctx {
             }
             final_c = c;
         };
-        double total = 0.0;
         std::vector<uint64_t> key;
         if (use_graph && !persist) {
             auto u = [&](uint64_t v) { key.push_back(v); };
@@ -1264,13 +1343,11 @@ struct gq_ctx {
             buf_after = hit->buf_after;
             final_c = hit->final_c;
             nlaunch = hit->nlaunch;
-            HIPCHK(hipEventRecord(events[0], st));
+            hipEvent_t e0 = take_event(), e1 = take_event();
+            HIPCHK(hipEventRecord(e0, st));
             HIPCHK(hipGraphLaunch(hit->exec, st));
-            HIPCHK(hipEventRecord(events[1], st));
-            HIPCHK(hipStreamSynchronize(st));
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, events[0], events[1]));
-            total = ms;
+            HIPCHK(hipEventRecord(e1, st));
+            queue_pair(e0, e1, chunk_cells);
         } else if (capture) {  // a cooperative launch is not captured: one launch per chunk anyway
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1287,9 +1364,10 @@ struct gq_ctx {
             (void)hipGraphDestroy(g);  // the instantiated graph does not refer to its source
             HIPCHK(ie);
             n_captures++;
-            if ((int)graphs.size() >= GRAPH_CACHE) {  // every earlier launch has completed (synchronised)
+            if ((int)graphs.size() >= GRAPH_CACHE) {
                 auto lru = std::min_element(graphs.begin(), graphs.end(),
                                             [](const CachedGraph &x, const CachedGraph &y) { return x.used < y.used; });
+                HIPCHK(hipStreamSynchronize(st));  // the evicted graph may still be running
                 (void)hipGraphExecDestroy(lru->exec);
                 graphs.erase(lru);
             }
@@ -1301,35 +1379,24 @@ struct gq_ctx {
             c.nlaunch = nlaunch;
             c.used = ++graph_clock;
             graphs.push_back(std::move(c));
-            HIPCHK(hipEventRecord(events[0], st));
+            hipEvent_t e0 = take_event(), e1 = take_event();
+            HIPCHK(hipEventRecord(e0, st));
             HIPCHK(hipGraphLaunch(ex, st));
-            HIPCHK(hipEventRecord(events[1], st));
-            HIPCHK(hipStreamSynchronize(st));
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, events[0], events[1]));
-            total = ms;
+            HIPCHK(hipEventRecord(e1, st));
+            queue_pair(e0, e1, chunk_cells);
         } else {
-            if (!pk) HIPCHK(hipEventRecord(events[0], st));
+            hipEvent_t e0 = pk ? nullptr : take_event(), e1 = pk ? nullptr : take_event();
+            if (!pk) HIPCHK(hipEventRecord(e0, st));
             enqueue(cur);
-            if (!pk) HIPCHK(hipEventRecord(events[1], st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (pk) {
-                for (int j = 0; j < ntimed; j++) {
-                    float ms = 0;
-                    HIPCHK(hipEventElapsedTime(&ms, events[2 * j], events[2 * j + 1]));
-                    total += ms;
-                }
-            } else {
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, events[0], events[1]));
-                total = ms;
+            if (!pk) {
+                HIPCHK(hipEventRecord(e1, st));
+                queue_pair(e0, e1, chunk_cells);
             }
         }
         cur = final_c;
         last_nlaunch = nlaunch;
         last_timed = ntimed;
         last_timed_cells = timed_cells;
-        return total;
     }
 
     gq::UniformArgs uniform_args(int sample)
@@ -1633,18 +1700,18 @@ gq_status gq_fk_steps(gq_ctx *ctx, const gq_fk_params *p, int n_steps)
         const int cfg = ctx->choose_step(a, nrows);
         ctx->last_cfg = cfg;
         const int chunk = 1024;
-        double total = 0.0;
         int done = 0, launches = 0;
         int64_t timed_cells = 0;
         const bool graph = ctx->fk_graphs && !ctx->per_kernel_timing;
+        ctx->timing_call++;
         while (done < n_steps) {
             const int m = std::min(chunk, n_steps - done);
-            total += ctx->advance(a, cfg, std::vector<int>(m, nwork), std::vector<int>(m, nrows), graph);
+            ctx->advance(a, cfg, std::vector<int>(m, nwork), std::vector<int>(m, nrows), graph);
             launches += ctx->per_kernel_timing ? ctx->last_timed : ctx->last_nlaunch;
             timed_cells += ctx->last_timed_cells;
             done += m;
         }
-        ctx->last_ms = total;
+        ctx->last_call = ctx->timing_call;  // its device time is read on request (gq_last_step_timing)
         ctx->last_launches = launches;
         ctx->last_cells = ctx->per_kernel_timing ? timed_cells : (int64_t)n_steps * nwork * gq::BS3;
     });
@@ -1683,8 +1750,9 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
         ctx->d_sprm.upload(prm, ctx->st);
         std::vector<int> rem(n_steps, n_steps + S);  // steps left per sample
         const int K = adaptive ? o->adapt_every : std::max(1, std::min(1024, *std::max_element(rem.begin(), rem.end())));
-        double ms = 0.0, t_adapt = 0.0, cells = 0.0;
+        double t_adapt = 0.0, cells = 0.0;
         int chunks = 0, adapts = 0;
+        const uint64_t call = ++ctx->timing_call;
         while (true) {
             const int mx = *std::max_element(rem.begin(), rem.end());
             if (mx <= 0) break;
@@ -1706,7 +1774,7 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
             const int cfg = ctx->choose_step(a, nr[0]);
             ctx->last_cfg = cfg;
             const int c0 = ctx->cur;
-            ms += ctx->advance(a, cfg, nw, nr, use_graph);
+            ctx->advance(a, cfg, nw, nr, use_graph);
             chunks++;
             // a sample that stopped early may hold its state in the other ping-pong buffer
             for (int pos = 0; pos < S; pos++) {
@@ -1742,7 +1810,7 @@ gq_status gq_ensemble_run(gq_ctx *ctx, const gq_fk_params *p, const int32_t *n_s
         }
         HIPCHK(hipStreamSynchronize(ctx->st));
         ctx->ens_stats[0] = cells;
-        ctx->ens_stats[1] = ms;
+        ctx->ens_stats[1] = ctx->call_ms(call);
         ctx->ens_stats[2] = chunks;
         ctx->ens_stats[3] = adapts;
         ctx->ens_stats[4] = t_adapt;
@@ -2214,10 +2282,30 @@ gq_status gq_loglik_ref_volume(gq_ctx *ctx, const double *model, int gpd, const 
     });
 }
 
+gq_status gq_step_timing_drain(gq_ctx *ctx, float *ms, int64_t *cells, int cap, int *n)
+{
+    return guard(ctx, [&] {
+        if (!n) throw gq_error(GQ_ERR_ARG, "null count");
+        ctx->drain_timing();
+        const int cnt = (int)ctx->timing_done.size();
+        if (!ms) {
+            *n = cnt;
+            return;
+        }
+        if (cap < cnt) throw gq_error(GQ_ERR_ARG, "capacity below the number of timed entries");
+        for (int i = 0; i < cnt; i++) {
+            ms[i] = ctx->timing_done[i].ms;
+            if (cells) cells[i] = ctx->timing_done[i].cells;
+        }
+        ctx->timing_done.clear();
+        *n = cnt;
+    });
+}
+
 gq_status gq_last_step_timing(gq_ctx *ctx, double *ms, int *launches, int64_t *cells)
 {
     return guard(ctx, [&] {
-        if (ms) *ms = ctx->last_ms;
+        if (ms) *ms = ctx->last_call ? ctx->call_ms(ctx->last_call) : 0.0;
         if (launches) *launches = ctx->last_launches;
         if (cells) *cells = ctx->last_cells;
     });

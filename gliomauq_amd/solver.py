@@ -219,6 +219,19 @@ def step_timing(grid):
     return ms.value, n.value, cells.value
 
 
+def step_timing_drain(ctx):
+    """(ms, cells) arrays of every step launch timed since the last drain
+    (per-kernel timing) or of every chunk of steps (otherwise), in stream
+    order (gq_step_timing_drain); waits for the timed work."""
+    c = ctx.ctx if isinstance(ctx, Grid) else ctx
+    n = C.c_int()
+    c.call("gq_step_timing_drain", None, None, 0, C.byref(n))
+    ms = np.zeros(max(n.value, 1), np.float32)
+    cells = np.zeros(max(n.value, 1), np.int64)
+    c.call("gq_step_timing_drain", ptr(ms, C.c_float), ptr(cells, C.c_int64), ms.size, C.byref(n))
+    return ms[:n.value].astype(np.float64), cells[:n.value]
+
+
 def set_step_timing(grid, per_kernel, stride=1):
     """per_kernel: time step-kernel launches with dispatch events, every
     `stride`-th launch (counted across calls); step_timing then reports the

@@ -277,6 +277,14 @@ gq_status gq_last_step_timing(gq_ctx *ctx, double *step_kernel_ms, int *launches
  * keeps the timed solve close to an untimed one); 0 (default): one pair
  * around the whole call, which then includes the cross-level ghost kernels. */
 gq_status gq_set_step_timing(gq_ctx *ctx, int per_kernel);
+/* Device time of every step-kernel launch timed since the last drain (with
+ * per-kernel timing) or of every chunk of steps (without), in stream order:
+ * ms[i] and the cell updates cells[i] it covers.  Stepping does not wait for
+ * its timing events; this call does.  ms == NULL: *n = the number of
+ * entries, which are kept; else cap must be at least that number
+ * (GQ_ERR_ARG otherwise, entries kept), and the entries are copied and
+ * dropped. */
+gq_status gq_step_timing_drain(gq_ctx *ctx, float *ms, int64_t *cells, int cap, int *n);
 /* Step autotuner.  mode 1 (default): time the slab depths 16/8/4/2 (planes
  * of a 16^3 block per workgroup) with the cross-level face ghosts from a
  * separate launch, on the grid the first time a block count of a new

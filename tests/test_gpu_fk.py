@@ -124,6 +124,7 @@ def test_step_timing_every_kth_launch(ctx):
         for name, v in (("phi", phi), ("pw", pw), ("pg", pg)):
             g.sample_volume(name, v)
         gq.set_step_timing(g, stride > 0, max(stride, 1))
+        gq.step_timing_drain(g)  # entries of earlier work on this context
         got = []
         for m in (10, 10, 4):
             gq.fk_steps(g, 1e-5, 1e-6, 0.03, 0.5, m)
@@ -139,7 +140,18 @@ def test_step_timing_every_kth_launch(ctx):
         if stride > 0:
             for ms, k, c in got:
                 assert c == k * cells and ms > 0
-    gq.set_step_timing(g, False)
+            # every timed launch of the three calls, in order, then nothing
+            n = gq._lib.C.c_int()
+            ms1 = np.zeros(1, np.float32)
+            if sum(k for _, k, _ in got) > 1:  # a capacity below the count is an error, entries kept
+                with pytest.raises(gq._lib.GQError) as e:
+                    g.ctx.call("gq_step_timing_drain", gq._lib.ptr(ms1, gq._lib.C.c_float), None, 1,
+                               gq._lib.C.byref(n))
+                assert e.value.status == 1
+            ms, cl = gq.step_timing_drain(g)
+            assert ms.size == sum(k for _, k, _ in got) and np.all(ms > 0) and np.all(cl == cells)
+            assert abs(ms.sum() - sum(t for t, _, _ in got)) <= 1e-3 * ms.sum()
+            assert gq.step_timing_drain(g)[0].size == 0
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
