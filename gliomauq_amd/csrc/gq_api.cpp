@@ -537,8 +537,8 @@ struct gq_ctx {
     }
 
     // Slot-indexed tables on the host (persistent, sized by the pool) and the
-    // pool size they were last uploaded for.
-    std::vector<int> h_level, h_index, h_ssample, h_lutoff, h_fbase;
+    // pool size the device copies were last sized for.
+    std::vector<int> h_level, h_ssample, h_lutoff, h_fbase;
     int uploaded_cap = 0;
     static constexpr int ROW = 45;  // packed row: slot, level, index[3], sample, lutoff, fnbr[6], fbase[6], nbr26[26]
 
@@ -554,11 +554,10 @@ struct gq_ctx {
             for (int s = 0; s < S; s++) sample_order[s] = s;
         }
         const bool full = uploaded_cap != cap;
-        bool pageable = full;  // copies straight from pageable host vectors are queued below
+        bool pageable = false;  // copies straight from pageable host vectors are queued below
         h_fnbr.resize((size_t)cap * 6, -1);
         h_nbr26.resize((size_t)cap * 26, -1);
         h_level.resize((size_t)cap, 0);
-        h_index.resize((size_t)cap * 3, 0);
         h_ssample.resize((size_t)cap, 0);
         h_lutoff.resize((size_t)cap, 0);
         h_fbase.resize((size_t)cap * 6, -1);
@@ -576,7 +575,6 @@ struct gq_ctx {
             const auto &L = sm.grid->leaves();
             for (const auto &lf : L) {
                 h_level[lf.slot] = lf.level;
-                for (int a = 0; a < 3; a++) h_index[(size_t)lf.slot * 3 + a] = lf.idx[a];
                 h_ssample[lf.slot] = s;
                 h_lutoff[lf.slot] = s * lut_n;
                 for (int f = 0; f < 6; f++) h_fbase[(size_t)lf.slot * 6 + f] = -1;
@@ -601,19 +599,25 @@ struct gq_ctx {
             face_begin.push_back((int)faces.size());
         }
         if ((int64_t)faces.size() * 256 > INT32_MAX) throw gq_error(GQ_ERR_NOMEM, "too many cross-level faces");
-        size_t nrow = 0;  // slot rows scattered on the device (every row after a pool growth)
+        // slot rows scattered on the device: those of the changed samples, or
+        // after a pool growth (device arrays regrown, contents dropped) every
+        // live slot's row -- no kernel reads the row of a free slot
+        std::vector<int> row_samples;
         if (full) {
-            d_level.upload(h_level, st);
-            d_index.upload(h_index, st);
-            d_slot_sample.upload(h_ssample, st);
-            d_lutoff.upload(h_lutoff, st);
-            d_fnbr.upload(h_fnbr, st);
-            d_fbase.upload(h_fbase, st);
-            d_nbr26.upload(h_nbr26, st);
+            d_level.reserve((size_t)cap);
+            d_index.reserve((size_t)cap * 3);
+            d_slot_sample.reserve((size_t)cap);
+            d_lutoff.reserve((size_t)cap);
+            d_fnbr.reserve((size_t)cap * 6);
+            d_fbase.reserve((size_t)cap * 6);
+            d_nbr26.reserve((size_t)cap * 26);
             uploaded_cap = cap;
+            for (int s = 0; s < S; s++) row_samples.push_back(s);
         } else {
-            for (int s : dirty) nrow += samples[s].grid->leaves().size();
+            row_samples = dirty;
         }
+        size_t nrow = 0;
+        for (int s : row_samples) nrow += samples[s].grid->leaves().size();
         const size_t lslot_n = samples[0].t.lut_slot.size();
         // one transfer, filled in place in the page-locked staging buffer: work
         // list, sample face-row bases, base-level look-ups, finest look-ups,
@@ -630,12 +634,13 @@ struct gq_ctx {
         }
         std::copy(faces.begin(), faces.end(), tabs.host_ptr(4));
         if (nrow) {
-            // rows of the dirty samples in order; samples on the worker
-            // threads when there are enough rows to pay for them
-            std::vector<size_t> roff(dirty.size() + 1, 0);
-            for (size_t k = 0; k < dirty.size(); k++) roff[k + 1] = roff[k] + samples[dirty[k]].grid->leaves().size();
+            // rows of those samples in order; samples on the worker threads
+            // when there are enough rows to pay for them
+            std::vector<size_t> roff(row_samples.size() + 1, 0);
+            for (size_t k = 0; k < row_samples.size(); k++)
+                roff[k + 1] = roff[k] + samples[row_samples[k]].grid->leaves().size();
             auto pack = [&](int k) {
-                const int s = dirty[k];
+                const int s = row_samples[k];
                 int *q = tabs.host_ptr(5) + roff[k] * ROW;
                 for (const gq::Leaf &lf : samples[s].grid->leaves()) {
                     const size_t sl = (size_t)lf.slot;
@@ -652,9 +657,9 @@ struct gq_ctx {
                     q += ROW;
                 }
             };
-            if (nrow >= PAR_ITEMS) parallel_for((int)dirty.size(), pack);
+            if (nrow >= PAR_ITEMS) parallel_for((int)row_samples.size(), pack);
             else
-                for (int k = 0; k < (int)dirty.size(); k++) pack(k);
+                for (int k = 0; k < (int)row_samples.size(); k++) pack(k);
         }
         // per-work-item step records: slot, 1/h^2 of its level (float bits),
         // face neighbours, absolute first face-ghost rows, sample
@@ -1510,7 +1515,6 @@ static void create_samples(gq_ctx *ctx, int nsamples, int bpd, int max_level, in
     ctx->h_fnbr.clear();
     ctx->h_nbr26.clear();
     ctx->h_level.clear();
-    ctx->h_index.clear();
     ctx->h_ssample.clear();
     ctx->h_lutoff.clear();
     ctx->h_fbase.clear();
