@@ -158,25 +158,48 @@ __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
 }
 
 // The same convolution for layers with few input channels (CIN = 8, 16, 48):
-// the GEMM's k = tap * C + ci runs in steps of 32 across taps, so a step with
-// C < 32 covers several taps and only the last step of 27 * C is padded (7
-// steps instead of 27 for CIN = 8, 14 instead of 27 for 16).  The input
-// channels are staged CP at a time (passes over channel blocks of CP, the
-// same accumulators): the brick's 10x10x6 halo with CP channels per voxel in
-// LDS (CIN = 48 in two passes of 24: 38 KB of LDS instead of 67, four
-// workgroups per CU instead of two; 42 steps instead of 41).  Lane group kq
-// reads its 8 k of a step from the halo voxel of its own tap.  Same output
-// mapping as k_conv3_brick.
+// the GEMM's k = tap * C + ci runs in steps of 32 across taps.  The input
+// channels are staged CP at a time (CP = 8, 16 or 32: passes over channel
+// blocks of CP, the same accumulators), the brick's 10x10x6 halo with CP
+// channels per voxel in LDS, so a 32-k step covers 32 / CP whole taps: lane
+// group kq reads channels 8 * (kq % (CP / 8)) .. +7 of tap
+// TPS * st + kq / (CP / 8) (TPS = 32 / CP taps per step).  Every per-step
+// offset is then a compile-time constant or a select among TPS of them: the
+// weights are read from one per-lane pointer at immediate offsets, and the four
+// voxel tiles of a wave lie two halo rows apart, so their LDS reads are one
+// address plus immediate offsets (no division or 64-bit address arithmetic in
+// the step loop).  CIN = 8: 7 steps; 16: 14; 48 in three passes of 16: 42
+// (38 KB of LDS with the row padding, four workgroups per CU).  Only the last step of a pass can
+// run past tap 26: its weights are zero there and its reads stay in the halo.
+// Same output mapping as k_conv3_brick.
+namespace {
+// LDS layout of the halo for CP channels per voxel: voxel (hx, hy, hz) at
+// hx * LD + hy * RST + hz * PST (elements).  A wave's 16-lane read covers 8
+// voxels of each of two rows; with 16 lanes x 16 bytes per LDS cycle the
+// rows must fall on complementary halves of the 64 banks: the row stride
+// (in 16-byte units) is odd for 32-byte voxels (CP = 8) and 8 mod 16 for
+// 48-byte voxels (CP = 16), hence the row padding.
+template <int CP>
+struct FlatHalo {
+    static constexpr int LD = CP + 8;                                        // voxel stride (pad: banks)
+    static constexpr int RST = 10 * LD + (CP == 8 ? 8 : CP == 16 ? 80 : 48);  // row stride
+    static constexpr int PST = 10 * RST;                                     // plane stride
+    static constexpr int SIZE = 6 * PST;
+    // offset of tap (dz+1)*9 + (dy+1)*3 + (dx+1) relative to the output voxel's (0,0,0) tap
+    __host__ __device__ static constexpr int tap(int t) { return (t % 3) * LD + ((t / 3) % 3) * RST + (t / 9) * PST; }
+};
+}  // namespace
+
 template <int CIN, int CP, int NTG, bool OUT_BF16>
 __global__ __launch_bounds__(256) void k_conv3_flat(ConvArgs a)
 {
-    static_assert(CIN % CP == 0 && CP % 8 == 0, "channel passes");
-    constexpr int LD = CP + 8;  // LDS channel stride per halo voxel (pad: banks)
+    static_assert(CIN % CP == 0 && (CP == 8 || CP == 16 || CP == 32), "channel passes of 8, 16 or 32");
+    typedef FlatHalo<CP> H;
     constexpr int KT = 27 * CIN;   // the weights' k extent
-    constexpr int KP = 27 * CP;    // k of one pass
-    constexpr int NST = (KP + 31) / 32;
-    constexpr int G8 = CP / 8;
-    __shared__ __attribute__((aligned(16))) __bf16 s_in[600 * LD];
+    constexpr int GPT = CP / 8;    // lane groups per tap
+    constexpr int TPS = 4 / GPT;   // taps per step
+    constexpr int NST = (27 + TPS - 1) / TPS;
+    __shared__ __attribute__((aligned(16))) __bf16 s_in[H::SIZE];
     const int D = a.in.D;
     const int nbx = D >> 3, nbz = D >> 2;
     const int b = blockIdx.x;
@@ -185,61 +208,87 @@ __global__ __launch_bounds__(256) void k_conv3_flat(ConvArgs a)
     const int nt0 = blockIdx.y * NTG;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 15, kq = lane >> 4;
+    const int jt = kq / GPT, ci = 8 * (kq % GPT);  // the lane's tap within a step, its channels within a tap
     const int64_t D3 = (int64_t)D * D * D;
     f32x4 acc[4][NTG];
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
         for (int n = 0; n < NTG; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-    int hbase[4];
+    // voxel tile m of this wave: rows r = 8 * wave + 2 * m + (li >> 3) of the brick's 8 (y) x 4 (z)
+    // rows, i.e. halo voxel (li & 7, 2 m + (li >> 3), wave) at tap 0: tiles 2 rows apart
+    const int hbase0 = (li & 7) * H::LD + (li >> 3) * H::RST + wave * H::PST;
+    // The thread's halo elements (voxel h, 8-channel group g) are the same in
+    // every pass: their LDS offsets and voxel indices in `in` and in the
+    // half-resolution `up` are computed once (vin < 0: outside the volume).
+    constexpr int NE = (600 * GPT + 255) / 256;
+    int e_lds[NE], e_g[NE], e_in[NE], e_up[NE];
+    const int Du = a.cup > 0 ? a.up.D : 0;
 #pragma unroll
-    for (int m = 0; m < 4; m++) {
-        const int r = 2 * (4 * wave + m) + (li >> 3);
-        hbase[m] = (li & 7) + 10 * ((r & 7) + 10 * (r >> 3));
+    for (int q = 0; q < NE; q++) {
+        const int e = t + 256 * q;
+        const int h = e / GPT, g = e - h * GPT;
+        const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+        const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+        const bool inside = gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D;
+        e_lds[q] = e < 600 * GPT ? hx * H::LD + hy * H::RST + hz * H::PST + 8 * g : -1;
+        e_g[q] = 8 * g;
+        e_in[q] = inside ? gx + D * (gy + D * gz) : -1;
+        e_up[q] = inside ? (gx >> 1) + Du * ((gy >> 1) + Du * (gz >> 1)) : 0;
     }
+    const __bf16 *in_item = a.in.p + bb * D3 * a.in.ld + a.in.off;
+    const __bf16 *up_item = a.cup > 0 ? a.up.p + bb * (int64_t)Du * Du * Du * a.up.ld + a.up.off : nullptr;
+    const __bf16 *sa = s_in + hbase0 + ci;
+    // weights PF steps ahead (an L2 hit takes several hundred cycles; a step's
+    // MFMAs about 64 per wave), the first PF issued before the halo staging
+    constexpr int PF = NTG == 1 ? 6 : 3;
 #pragma unroll 1
     for (int cb = 0; cb < CIN; cb += CP) {
-        if (cb > 0) __syncthreads();  // the previous pass has read the halo
-        for (int e = t; e < 600 * G8; e += 256) {
-            const int h = e / G8, g = e - h * G8;
-            const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
-            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-            bf16x8 v = zero8();
-            if (gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D)
-                v = load_input8(a.in, a.up, a.cup, bb, gx, gy, gz, cb + 8 * g);
-            *reinterpret_cast<bf16x8 *>(&s_in[h * LD + 8 * g]) = v;
-        }
-        __syncthreads();
-        // the lane's k offset, opaque to the compiler in each pass: the
-        // per-step tap and channel offsets are recomputed per pass instead of
-        // being kept in registers across passes
-        int kq8 = 8 * kq;
-        asm volatile("" : "+v"(kq8));
-        constexpr int PF = 3;
+        // per lane: weights of channel ci of tap jt of step 0 (tap TPS * st + jt at + st * TPS * CIN)
+        const __bf16 *wl[NTG];
+#pragma unroll
+        for (int n = 0; n < NTG; n++) wl[n] = a.w + (int64_t)(16 * (nt0 + n) + li) * KT + jt * CIN + cb + ci;
         bf16x8 vb[PF + 1][NTG];
         auto load_w = [&](int st, bf16x8 *dst) {
-            const int k = 32 * st + kq8;  // k of this pass
-            const int tap = k / CP, ci = k - tap * CP;
+            const bool ok = TPS * st + jt < 27;  // false only in a pass's last step
 #pragma unroll
-            for (int n = 0; n < NTG; n++) {
-                const int co = 16 * (nt0 + n) + li;
-                dst[n] = (k < KP && co < a.cout) ? load8(a.w + (int64_t)co * KT + tap * CIN + cb + ci) : zero8();
-            }
+            for (int n = 0; n < NTG; n++) dst[n] = ok ? load8(wl[n] + st * TPS * CIN) : zero8();
         };
 #pragma unroll
         for (int p = 0; p < PF; p++)
             if (p < NST) load_w(p, vb[p]);
+        if (cb > 0) __syncthreads();  // the previous pass has read the halo
+#pragma unroll
+        for (int q = 0; q < NE; q++) {
+            if (e_lds[q] < 0) continue;
+            bf16x8 v = zero8();
+            if (e_in[q] >= 0) {
+                // channels c .. c+7: below cup from `up` (nearest 2x upsampling), else channel c - cup of `in`
+                const int c = cb + e_g[q];
+                v = c < a.cup ? load8(up_item + (int64_t)e_up[q] * a.up.ld + c)
+                              : load8(in_item + (int64_t)e_in[q] * a.in.ld + (c - a.cup));
+            }
+            *reinterpret_cast<bf16x8 *>(&s_in[e_lds[q]]) = v;
+        }
+        __syncthreads();
 #pragma unroll
         for (int st = 0; st < NST; st++) {
             if (st + PF < NST) load_w(st + PF, vb[(st + PF) % (PF + 1)]);
-            const int k = 32 * st + kq8;
-            // past the last tap the weights are zero; read a valid voxel so the product is 0
-            const int tap = k < KP ? k / CP : 26;
-            const int ci = k < KP ? k - tap * CP : 0;
-            const int hoff = tap % 3 + 10 * ((tap / 3) % 3 + 10 * (tap / 9));
+            // halo offset of the lane's tap: one of TPS constants (past tap 26: tap 26's, weights zero)
+            int ho;
+            if constexpr (TPS == 1) {
+                ho = H::tap(st);
+            } else {
+                const int t0 = TPS * st;
+                ho = H::tap(t0 < 27 ? t0 : 26);
+#pragma unroll
+                for (int q = 1; q < TPS; q++)
+                    if (jt == q) ho = H::tap(t0 + q < 27 ? t0 + q : 26);
+            }
+            const __bf16 *sp = sa + ho;
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_in[(hbase[m] + hoff) * LD + ci]);
+                const bf16x8 va = *reinterpret_cast<const bf16x8 *>(sp + 2 * m * H::RST);
 #pragma unroll
                 for (int n = 0; n < NTG; n++)
                     acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb[st % (PF + 1)][n], acc[m][n], 0, 0, 0);
@@ -249,7 +298,6 @@ __global__ __launch_bounds__(256) void k_conv3_flat(ConvArgs a)
 #pragma unroll
     for (int n = 0; n < NTG; n++) {
         const int co = 16 * (nt0 + n) + li;
-        if (co >= a.cout) continue;
         const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
         for (int m = 0; m < 4; m++) {
@@ -721,8 +769,8 @@ hipError_t launch_conv3(const ConvArgs &a, hipStream_t st)
     const bool bf = a.out_bf != nullptr;
     if (a.cin == 8 || a.cin == 16 || a.cin == 48) {
 #define GQ_FLAT(C, N)                                                                                        \
-    if (bf) hipLaunchKernelGGL((k_conv3_flat<C, (C > 32 ? C / 2 : C), N, true>), grid, dim3(256), 0, st, a);  \
-    else hipLaunchKernelGGL((k_conv3_flat<C, (C > 32 ? C / 2 : C), N, false>), grid, dim3(256), 0, st, a);
+    if (bf) hipLaunchKernelGGL((k_conv3_flat<C, (C > 32 ? 16 : C), N, true>), grid, dim3(256), 0, st, a);  \
+    else hipLaunchKernelGGL((k_conv3_flat<C, (C > 32 ? 16 : C), N, false>), grid, dim3(256), 0, st, a);
 #define GQ_FLAT_N(C)                    \
     switch (ntg) {                      \
     case 1: GQ_FLAT(C, 1) break;        \
