@@ -827,6 +827,44 @@ def surrogate_samplers(args, post_b, post_1, spost, sur_res, tt, lo, hi, span, r
     print(json.dumps(res), flush=True)
 
 
+def surrogate_kernels_workload(args):
+    """The surrogate's kernels alone, on random weights and inputs (no field
+    generation): --sk-mode predict runs batch-64 predictions, train runs
+    batch-8 training steps; one untimed call, then --sk-reps timed ones.  A
+    short, fixed workload for kernel traces and counters."""
+    from gliomauq_amd import surrogate as SU
+    rng = np.random.default_rng(args.seed)
+    n = SU.N
+    batch = 64 if args.sk_mode == "predict" else 8
+    sur = SU.Surrogate(batch=batch, seed=1)
+    pw = rng.random((n, n, n)).astype(np.float32)
+    sur.set_anatomy(pw, (1 - pw).astype(np.float32))
+    prm = rng.random((batch, 4)).astype(np.float32)
+    u0 = (0.5 * rng.random((batch, n, n, n))).astype(np.float32)
+    tgt = rng.random((batch, n, n, n)).astype(np.float32)
+
+    def call():
+        if args.sk_mode == "predict":
+            sur.predict(prm, u0)
+        else:
+            sur.train_step(prm, u0, tgt, 1e-4)
+
+    call()
+    t0 = time.perf_counter()
+    for _ in range(args.sk_reps):
+        call()
+    wall = (time.perf_counter() - t0) / max(args.sk_reps, 1)
+    tm = sur.timing()
+    fwd, both = SU.conv_flops(batch)
+    dev_ms = tm["forward_ms"] + tm["backward_ms"]
+    flops = fwd if args.sk_mode == "predict" else both
+    print(json.dumps({"metric": "surrogate convolution throughput", "value": flops / (dev_ms * 1e-3) / 1e12,
+                      "unit": "TFLOP/s", "n_gpus": 1, "steps": args.sk_reps, "warmup": 1,
+                      "ms_per_step": dev_ms, "host_ms_per_call": wall * 1e3, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic (random weights)",
+                      "config": {"workload": "surrogate %s, batch %d, 64^3" % (args.sk_mode, batch)}}), flush=True)
+
+
 def tmcmc_workload(args):
     """Transitional MCMC (the reference's sampler family) over the batched
     likelihood: a population of parameter sets evaluated --tm-batch at a time
@@ -890,7 +928,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="adaptive",
-                    choices=["adaptive", "ensemble", "adjoint", "mcmc", "surrogate", "tmcmc", "langevin"])
+                    choices=["adaptive", "ensemble", "adjoint", "mcmc", "surrogate", "tmcmc", "langevin",
+                             "surrogate_kernels"])
+    ap.add_argument("--sk-mode", default="predict", choices=["predict", "train"],
+                    help="surrogate_kernels workload: batch-64 predictions or batch-8 training steps")
+    ap.add_argument("--sk-reps", type=int, default=5)
     ap.add_argument("--tm-pop", type=int, default=512)
     ap.add_argument("--tm-batch", type=int, default=64)
     ap.add_argument("--tm-moves", type=int, default=3)
@@ -956,6 +998,8 @@ def main():
         return surrogate_workload(args)
     if args.workload == "tmcmc":
         return tmcmc_workload(args)
+    if args.workload == "surrogate_kernels":
+        return surrogate_kernels_workload(args)
     if args.workload == "langevin":
         return langevin_workload(args)
 
