@@ -330,16 +330,20 @@ constexpr int WB_ROWS = 61;   // 10 x 6 halo rows + 1 pad row per input channel 
 constexpr int WB_DZLD = 264;  // 256 voxels + 8 pad per output channel
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_conv3_wgrad_brick(WgradArgs a, float *bias_partial, int bricks_per_group)
+// NCOT: 16-channel output tiles per group (1 or 2); the groups of one launch
+// start at output group cog0.  A wave's tiles are tap * NCOT + cot for tap =
+// wave / NCOT + (4 / NCOT) j, all with cot = wave % NCOT: one dZ^T read per
+// K-step serves all of them, and their X^T offsets are set once.
+template <int NCOT>
+__global__ __launch_bounds__(256) void k_conv3_wgrad_brick(WgradArgs a, float *bias_partial, int bricks_per_group,
+                                                           int cog0)
 {
     __shared__ __attribute__((aligned(16))) __bf16 s_x[3 * 16 * WB_ROWS * 8];
     __shared__ __attribute__((aligned(16))) __bf16 s_dz[32 * WB_DZLD];
     __shared__ float s_b[256];
-    const int cit = blockIdx.x, cog = blockIdx.y, bg = blockIdx.z;
+    const int cit = blockIdx.x, cog = cog0 + blockIdx.y, bg = blockIdx.z;
     const int Cin = a.cin, Cout = a.cout;
-    const int nco = Cout - 32 * cog < 32 ? Cout - 32 * cog : 32;  // 16 or 32
-    const int ncot = nco >> 4;
-    const int ntiles = 27 * ncot;
+    constexpr int nco = 16 * NCOT;
     const int D = a.x.D;
     const int nbx = D >> 3, nbz = D >> 2;
     const int64_t D3 = (int64_t)D * D * D;
@@ -349,10 +353,19 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_brick(WgradArgs a, float *b
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 15, kq = lane >> 4;
     const int ci0 = 16 * cit;
-    constexpr int MT = 14;  // tiles per wave: ceil(54 / 4)
+    constexpr int MT = (27 * NCOT + 3) / 4;  // tiles per wave
+    constexpr int TSTEP = 4 / NCOT;          // tap stride between a wave's tiles
+    const int cot = wave % NCOT, tap0 = wave / NCOT;
     f32x4 acc[MT];
+    int boff[MT];  // X^T element offset of tile j at brick row 0
 #pragma unroll
-    for (int j = 0; j < MT; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < MT; j++) {
+        acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int tap = tap0 + TSTEP * j < 27 ? tap0 + TSTEP * j : 26;
+        const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
+        boff[j] = ((dx * 16 + li) * WB_ROWS + dz * 10 + dy) * 8;
+    }
+    const int aoff = (16 * cot + li) * WB_DZLD;
     float bsum = 0.f;
     for (int b = b0; b < b1; b++) {
         const int bx = b % nbx, by = (b / nbx) % nbx, bz = (b / (nbx * nbx)) % nbz, bb = b / (nbx * nbx * nbz);
@@ -394,16 +407,12 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_brick(WgradArgs a, float *b
 #pragma unroll 1
         for (int ks = 0; ks < 8; ks++) {
             const int r = 4 * ks + kq;  // brick row: y = r % 8, z = r / 8
-            const int ry = r & 7, rz = r >> 3;
+            const int rowoff = ((r >> 3) * 10 + (r & 7)) * 8;
+            const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_dz[aoff + 8 * r]);
 #pragma unroll
             for (int j = 0; j < MT; j++) {
-                const int tile = wave + 4 * j;
-                if (tile < ntiles) {
-                    const int tap = tile / ncot, cot = tile - tap * ncot;
-                    const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
-                    const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_dz[(16 * cot + li) * WB_DZLD + 8 * r]);
-                    const bf16x8 vb = *reinterpret_cast<const bf16x8 *>(
-                        &s_x[((dx * 16 + li) * WB_ROWS + (rz + dz) * 10 + (ry + dy)) * 8]);
+                if (tap0 + TSTEP * j < 27) {  // uniform over the wave
+                    const bf16x8 vb = *reinterpret_cast<const bf16x8 *>(&s_x[boff[j] + rowoff]);
                     acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc[j], 0, 0, 0);
                 }
             }
@@ -413,9 +422,8 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_brick(WgradArgs a, float *b
     const int Kw = 27 * Cin;
 #pragma unroll
     for (int j = 0; j < MT; j++) {
-        const int tile = wave + 4 * j;
-        if (tile >= ntiles) continue;
-        const int tap = tile / ncot, cot = tile - tap * ncot;
+        const int tap = tap0 + TSTEP * j;
+        if (tap >= 27) continue;
         const int ci = ci0 + li;
         if (ci >= Cin) continue;
 #pragma unroll
@@ -818,8 +826,14 @@ hipError_t launch_conv3_wgrad_brick(const WgradArgs &a, int ngroups, float *bias
         return hipErrorInvalidValue;
     const int nbricks = a.batch * (D / 8) * (D / 8) * (D / 4);
     const int per = (nbricks + ngroups - 1) / ngroups;
-    const dim3 grid((unsigned)((a.cin + 15) / 16), (unsigned)((a.cout + 31) / 32), (unsigned)ngroups);
-    hipLaunchKernelGGL(k_conv3_wgrad_brick, grid, dim3(256), 0, st, a, bias_partial, per);
+    // output groups of 32 channels, then a last group of 16 when cout is an odd multiple of 16
+    const int full = a.cout / 32;
+    if (full > 0)
+        hipLaunchKernelGGL(k_conv3_wgrad_brick<2>, dim3((unsigned)((a.cin + 15) / 16), (unsigned)full,
+                           (unsigned)ngroups), dim3(256), 0, st, a, bias_partial, per, 0);
+    if (a.cout % 32 != 0)
+        hipLaunchKernelGGL(k_conv3_wgrad_brick<1>, dim3((unsigned)((a.cin + 15) / 16), 1u, (unsigned)ngroups),
+                           dim3(256), 0, st, a, bias_partial, per, full);
     return hipGetLastError();
 }
 
