@@ -371,33 +371,63 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad_brick(WgradArgs a, float *b
         const int bx = b % nbx, by = (b / nbx) % nbx, bz = (b / (nbx * nbx)) % nbz, bb = b / (nbx * nbx * nbz);
         const int x0 = 8 * bx, y0 = 8 * by, z0 = 4 * bz;
         __syncthreads();  // the previous brick's LDS reads are done
-        // X^T halo, three x-shifted copies: copy s, channel c, row (hz*10 + hy), xp <- halo x = xp + s
-        for (int e = t; e < 600 * 2; e += 256) {
-            const int h = e >> 1, half = e & 1;
-            const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
-            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+        // X^T halo, three x-shifted copies: copy s, channel c, row (hz*10 + hy), xp <- halo x = xp + s.
+        // One thread per (halo row, 8-channel half): the row's 10 voxels are loaded and transposed
+        // in registers (bf16 pairs packed per dword), and each copy's channel row of 8 x values is
+        // one 16-byte store.
+        if (t < 120) {
+            const int row = t >> 1, half = t & 1;
+            const int hy = row % 10, hz = row / 10;
+            const int gy = y0 + hy - 1, gz = z0 + hz - 1;
             const int c0 = ci0 + 8 * half;
-            bf16x8 v = zero8();
-            if (c0 < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D)
-                v = load_input8(a.x, a.up, a.cup, bb, gx, gy, gz, c0);
-            const int row = hz * 10 + hy;
+            const bool rowin = c0 < Cin && gy >= 0 && gy < D && gz >= 0 && gz < D;
+            uint32_t w[10][4];
 #pragma unroll
-            for (int sh = 0; sh < 3; sh++) {
-                const int xp = hx - sh;
-                if (xp < 0 || xp >= 8) continue;
-#pragma unroll
-                for (int j = 0; j < 8; j++) s_x[((sh * 16 + 8 * half + j) * WB_ROWS + row) * 8 + xp] = v[j];
+            for (int x = 0; x < 10; x++) {
+                const int gx = x0 + x - 1;
+                bf16x8 v = zero8();
+                if (rowin && gx >= 0 && gx < D) v = load_input8(a.x, a.up, a.cup, bb, gx, gy, gz, c0);
+                const uint4 u = __builtin_bit_cast(uint4, v);
+                w[x][0] = u.x, w[x][1] = u.y, w[x][2] = u.z, w[x][3] = u.w;
             }
-        }
-        // dZ^T [co][voxel], voxel = x + 8 (y + 8 z) inside the brick
-        const int ng = nco >> 3;
-        for (int e = t; e < 256 * ng; e += 256) {
-            const int vox = e / ng, g = e - vox * ng;
-            const int lx = vox & 7, ly = (vox >> 3) & 7, lz = vox >> 6;
-            const int64_t vn = bb * D3 + (x0 + lx) + (int64_t)D * ((y0 + ly) + (int64_t)D * (z0 + lz));
-            const bf16x8 v = load8(a.dz.p + vn * a.dz.ld + a.dz.off + 32 * cog + 8 * g);
 #pragma unroll
-            for (int j = 0; j < 8; j++) s_dz[(8 * g + j) * WB_DZLD + vox] = v[j];
+            for (int c = 0; c < 8; c++)
+#pragma unroll
+                for (int sh = 0; sh < 3; sh++) {
+                    uint32_t d[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const uint32_t lo = w[sh + 2 * k][c >> 1], hi = w[sh + 2 * k + 1][c >> 1];
+                        d[k] = (c & 1) ? (lo >> 16) | (hi & 0xffff0000u) : (lo & 0xffffu) | (hi << 16);
+                    }
+                    *reinterpret_cast<uint4 *>(&s_x[((sh * 16 + 8 * half + c) * WB_ROWS + row) * 8]) =
+                        uint4{d[0], d[1], d[2], d[3]};
+                }
+        }
+        // dZ^T [co][voxel], voxel = x + 8 (y + 8 z) inside the brick: one thread of waves 2-3 (the
+        // halo rows use waves 0-1) per (8-voxel x row, 8-channel group), transposed in registers as
+        // above, one 16-byte store per channel
+        constexpr int ng = nco >> 3;
+        if (t >= 128 && t - 128 < 32 * ng) {
+            const int g = (t - 128) % ng, ryz = (t - 128) / ng;
+            const int ly = ryz & 7, lz = ryz >> 3;
+            uint32_t w[8][4];
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                const int64_t vn = bb * D3 + (x0 + x) + (int64_t)D * ((y0 + ly) + (int64_t)D * (z0 + lz));
+                const uint4 u = __builtin_bit_cast(uint4, load8(a.dz.p + vn * a.dz.ld + a.dz.off + 32 * cog + 8 * g));
+                w[x][0] = u.x, w[x][1] = u.y, w[x][2] = u.z, w[x][3] = u.w;
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                uint32_t d[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t lo = w[2 * k][c >> 1], hi = w[2 * k + 1][c >> 1];
+                    d[k] = (c & 1) ? (lo >> 16) | (hi & 0xffff0000u) : (lo & 0xffffu) | (hi << 16);
+                }
+                *reinterpret_cast<uint4 *>(&s_dz[(8 * g + c) * WB_DZLD + 8 * ryz]) = uint4{d[0], d[1], d[2], d[3]};
+            }
         }
         __syncthreads();
         if (cit == 0 && (t & 31) < nco) {
