@@ -141,15 +141,16 @@ __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
         const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
         for (int m = 0; m < 4; m++) {
+            // rows 4 kq .. 4 kq + 3 of tile m: voxels x0 + 4 (kq & 1) + rr of one (y, z) row
+            const int r = 2 * (4 * wave + m) + (kq >> 1);
+            const int64_t v0 =
+                bb * D3 + (x0 + 4 * (kq & 1)) + (int64_t)D * ((y0 + (r & 7)) + (int64_t)D * (z0 + (r >> 3)));
+            const int64_t o0 = v0 * a.out_ld + a.out_off + co;
 #pragma unroll
             for (int rr = 0; rr < 4; rr++) {
-                const int i = 4 * kq + rr;
-                const int r = 2 * (4 * wave + m) + (i >> 3);
-                const int gx = x0 + (i & 7), gy = y0 + (r & 7), gz = z0 + (r >> 3);
-                const int64_t vo = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
                 float val = acc[m][n][rr] + bias;
                 if (a.relu) val = val > 0.f ? val : kLeak * val;
-                const int64_t o = vo * a.out_ld + a.out_off + co;
+                const int64_t o = o0 + (int64_t)rr * a.out_ld;
                 if (OUT_BF16) a.out_bf[o] = (__bf16)val;
                 else a.out_f[o] = a.accumulate ? a.out_f[o] + val : val;
             }
@@ -301,15 +302,16 @@ __global__ __launch_bounds__(256) void k_conv3_flat(ConvArgs a)
         const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
         for (int m = 0; m < 4; m++) {
+            // rows 4 kq .. 4 kq + 3 of tile m: voxels x0 + 4 (kq & 1) + rr of one (y, z) row
+            const int r = 2 * (4 * wave + m) + (kq >> 1);
+            const int64_t v0 =
+                bb * D3 + (x0 + 4 * (kq & 1)) + (int64_t)D * ((y0 + (r & 7)) + (int64_t)D * (z0 + (r >> 3)));
+            const int64_t o0 = v0 * a.out_ld + a.out_off + co;
 #pragma unroll
             for (int rr = 0; rr < 4; rr++) {
-                const int i = 4 * kq + rr;
-                const int r = 2 * (4 * wave + m) + (i >> 3);
-                const int gx = x0 + (i & 7), gy = y0 + (r & 7), gz = z0 + (r >> 3);
-                const int64_t vo = bb * D3 + gx + (int64_t)D * (gy + (int64_t)D * gz);
                 float val = acc[m][n][rr] + bias;
                 if (a.relu) val = val > 0.f ? val : kLeak * val;
-                const int64_t o = vo * a.out_ld + a.out_off + co;
+                const int64_t o = o0 + (int64_t)rr * a.out_ld;
                 if (OUT_BF16) a.out_bf[o] = (__bf16)val;
                 else a.out_f[o] = a.accumulate ? a.out_f[o] + val : val;
             }
