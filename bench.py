@@ -985,7 +985,7 @@ def main():
                     help="inline timing: events on every k-th step launch (counted across the solve; 1: all)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (samples would shard across GPUs; "
+        print("bench.py measures one GPU: there is no multi-GPU path (samples would shard across GPUs; "
               "run --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":

@@ -186,7 +186,7 @@ void fk_update_f32ref(float *phi, const float *dphidt, long n, double dt)
 }
 
 /* ------------------------------------------------------------------------ */
-/* Uniform periodic block grid (configs 0 and 3): N^3 cells as (N/B)^3 blocks */
+/* Uniform periodic block grid (the 128^3 plain and adjoint workloads): N^3 cells as (N/B)^3 blocks */
 /* of B^3.  Restates Glioma_HG_UQ::_reactionDiffusionStep (Glioma_HG_UQ.cpp:  */
 /* 266-277): per block, BlockLab::load of a ghosted copy (same-level periodic */
 /* neighbours, MRAGBlockLab.h:154-301), the RD functor, then a separate       */
