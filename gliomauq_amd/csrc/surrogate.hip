@@ -64,13 +64,22 @@ __device__ __forceinline__ bf16x8 load_input8(const View &in, const View &up, in
 // over the 27 taps, the weights read once per K-step and reused by the 4
 // voxel tiles.
 namespace {
-constexpr int CB_LD = 40;  // LDS channel stride per halo voxel (32 + 8 pad: banks)
+// LDS layout of the halo: voxel (hx, hy, hz) at hx * CB_LD + hy * CB_RST +
+// hz * CB_PST (elements).  A 16-lane A read covers 8 voxels of each of two
+// adjacent rows; at 80-byte voxels (5 x 16 bytes: 8 voxels on 8 distinct
+// 16-byte bank groups) a row stride of 8 mod 16 groups puts the second row on
+// the other 8, so the read takes one LDS cycle (unpadded rows, 50 groups,
+// share two groups and take two).
+constexpr int CB_LD = 40;                  // voxel stride (32 channels + 8 pad)
+constexpr int CB_RST = 10 * CB_LD + 48;    // row stride: 56 groups of 16 bytes
+constexpr int CB_PST = 10 * CB_RST;        // plane stride
+constexpr int CB_SIZE = 6 * CB_PST;        // 53,760 bytes: three workgroups per CU
 }  // namespace
 
 template <int NTG, bool OUT_BF16>
 __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 s_in[600 * CB_LD];
+    __shared__ __attribute__((aligned(16))) __bf16 s_in[CB_SIZE];
     const int D = a.in.D;
     const int nbx = D >> 3, nbz = D >> 2;
     const int b = blockIdx.x;
@@ -91,7 +100,7 @@ __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
 #pragma unroll
     for (int m = 0; m < 4; m++) {
         const int r = 2 * (4 * wave + m) + (li >> 3);
-        hbase[m] = (li & 7) + 10 * ((r & 7) + 10 * (r >> 3));
+        hbase[m] = (li & 7) * CB_LD + (r & 7) * CB_RST + (r >> 3) * CB_PST;
     }
     for (int c0 = 0; c0 < Cin; c0 += 32) {
         __syncthreads();
@@ -103,7 +112,7 @@ __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
             bf16x8 v = zero8();
             if (ci < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D)
                 v = load_input8(a.in, a.up, a.cup, bb, gx, gy, gz, ci);
-            *reinterpret_cast<bf16x8 *>(&s_in[h * CB_LD + 8 * g]) = v;
+            *reinterpret_cast<bf16x8 *>(&s_in[hx * CB_LD + hy * CB_RST + hz * CB_PST + 8 * g]) = v;
         }
         __syncthreads();
         const bool kok = c0 + 8 * kq < Cin;
@@ -123,10 +132,10 @@ __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
         for (int tap = 0; tap < 27; tap++) {
             if (tap + PF < 27) load_w(tap + PF, vb[(tap + PF) % (PF + 1)]);
             const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
-            const int hoff = dx + 10 * (dy + 10 * dz);
+            const int hoff = dx * CB_LD + dy * CB_RST + dz * CB_PST;
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_in[(hbase[m] + hoff) * CB_LD + 8 * kq]);
+                const bf16x8 va = *reinterpret_cast<const bf16x8 *>(&s_in[hbase[m] + hoff + 8 * kq]);
 #pragma unroll
                 for (int n = 0; n < NTG; n++)
                     acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb[tap % (PF + 1)][n], acc[m][n], 0, 0, 0);
