@@ -102,17 +102,40 @@ __global__ __launch_bounds__(256) void k_conv3_brick(ConvArgs a)
         const int r = 2 * (4 * wave + m) + (li >> 3);
         hbase[m] = (li & 7) * CB_LD + (r & 7) * CB_RST + (r >> 3) * CB_PST;
     }
+    // staging: 600 halo voxels x 4 groups of 8 channels, 10 per thread in
+    // batches of NB loads in flight (more at once cost occupancy)
+    // Loads are unconditional (a lane with nothing to load reads its item's
+    // first voxel and stores zeros), so no branch joins wait on them.
+    constexpr int NB = NTG <= 2 ? 5 : NTG == 3 ? 3 : 2;
+    const int64_t Du = a.cup > 0 ? a.up.D : 0;
+    const __bf16 *in_item = a.in.p + bb * D3 * a.in.ld + a.in.off;
+    const __bf16 *up_item = a.cup > 0 ? a.up.p + bb * Du * Du * Du * a.up.ld + a.up.off : in_item;
     for (int c0 = 0; c0 < Cin; c0 += 32) {
         __syncthreads();
-        for (int e = t; e < 600 * 4; e += 256) {
-            const int h = e >> 2, g = e & 3;
-            const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
-            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-            const int ci = c0 + 8 * g;
-            bf16x8 v = zero8();
-            if (ci < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D)
-                v = load_input8(a.in, a.up, a.cup, bb, gx, gy, gz, ci);
-            *reinterpret_cast<bf16x8 *>(&s_in[hx * CB_LD + hy * CB_RST + hz * CB_PST + 8 * g]) = v;
+#pragma unroll 1
+        for (int e0 = t; e0 < 600 * 4; e0 += 256 * NB) {
+            bf16x8 sv[NB];
+            int so[NB];
+            bool ok[NB];
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const int e = e0 + 256 * q;
+                const int h = e >> 2, g = e & 3;
+                const int hx = h % 10, hy = (h / 10) % 10, hz = h / 100;
+                const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+                const int ci = c0 + 8 * g;
+                so[q] = e < 600 * 4 ? hx * CB_LD + hy * CB_RST + hz * CB_PST + 8 * g : -1;
+                ok[q] = e < 600 * 4 && ci < Cin && gx >= 0 && gx < D && gy >= 0 && gy < D && gz >= 0 && gz < D;
+                const int64_t vu = (gx >> 1) + Du * ((gy >> 1) + Du * (int64_t)(gz >> 1));
+                const int64_t vn = gx + (int64_t)D * (gy + (int64_t)D * gz);
+                const __bf16 *src = !ok[q] ? in_item
+                                    : ci < a.cup ? up_item + vu * a.up.ld + ci
+                                                 : in_item + vn * a.in.ld + (ci - a.cup);
+                sv[q] = load8(src);
+            }
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+                if (so[q] >= 0) *reinterpret_cast<bf16x8 *>(&s_in[so[q]]) = ok[q] ? sv[q] : zero8();
         }
         __syncthreads();
         const bool kok = c0 + 8 * kq < Cin;
